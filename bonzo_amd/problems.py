@@ -1,0 +1,302 @@
+"""Initial conditions for the configurations the MHD step is run on.
+
+Each generator returns ``(prim, bfld)`` over the full grid of a
+``GridShape`` (active cells set, ghosts left for the boundary fill), with the
+reference's layout: ``prim`` is [9][Nz][Ny][Nx] with rows rho, vx, vy, vz, p,
+passive scalar, Bx, By, Bz (cell-centred B is filled later by ``interp_bc``),
+``bfld`` is [3][Nz][Ny][Nx] indexed by the lower face
+(src/coordinates/grid.pyx:28-52, src/integration/ct.pyx:110-111).
+
+Face-centred fields are built as discrete curls of an edge-centred vector
+potential so that the discrete divergence is zero to round-off, as the
+reference's Orszag-Tang setup does (src/problem/otvortex.pyx:38-66).
+Random fields use numpy's PCG64 with an explicit seed.
+"""
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+RHO, VX, VY, VZ, PR, PSC, BXC, BYC, BZC = range(9)
+
+
+@dataclass(frozen=True)
+class GridShape:
+    """Active size, ghost count and box of a uniform Cartesian grid."""
+    nx: int
+    ny: int = 1
+    nz: int = 1
+    ng: int = 3
+    lmin: tuple = (0.0, 0.0, 0.0)
+    lmax: tuple = (1.0, 1.0, 1.0)
+
+    @property
+    def dims(self):
+        return 3 if self.nz > 1 else (2 if self.ny > 1 else 1)
+
+    @property
+    def ntot(self):
+        d = self.dims
+        return (self.nx + 2 * self.ng,
+                self.ny + 2 * self.ng if d >= 2 else 1,
+                self.nz + 2 * self.ng if d >= 3 else 1)
+
+    @property
+    def nact(self):
+        return (self.nx, self.ny, self.nz)
+
+    def active(self):
+        """Slices (k, j, i) of the active cells."""
+        g = self.ng
+        d = self.dims
+        si = slice(g, g + self.nx)
+        sj = slice(g, g + self.ny) if d >= 2 else slice(0, 1)
+        sk = slice(g, g + self.nz) if d >= 3 else slice(0, 1)
+        return sk, sj, si
+
+    def dx(self, axis):
+        return (self.lmax[axis] - self.lmin[axis]) / self.nact[axis]
+
+    def centres(self, axis):
+        """Cell-centre coordinates of the active cells along an axis.
+
+        Uniform faces start at 0 regardless of lmin (src/coordinates/
+        coord.pyx:205-209), so centres are (i+1/2)*dx."""
+        n = self.nact[axis]
+        return (np.arange(n) + 0.5) * self.dx(axis)
+
+    def empty(self):
+        nt = self.ntot
+        prim = np.zeros((9, nt[2], nt[1], nt[0]))
+        bfld = np.zeros((3, nt[2], nt[1], nt[0]))
+        return prim, bfld
+
+
+def _put(arr, shape, values):
+    sk, sj, si = shape.active()
+    arr[sk, sj, si] = values
+
+
+def orszag_tang(n, ng=3, b_eps=1e-20):
+    """2D Orszag-Tang vortex on [0,1]^2 in Heaviside-Lorentz-free units:
+    rho=25/(36 pi), p=5/(12 pi), v=(-sin 2pi y, sin 2pi x), B from
+    A_z = B0 (cos 4pi x/(4pi) + cos 2pi y/(2pi)) at cell corners with
+    B0 = 1/sqrt(4 pi) (src/problem/otvortex.pyx:38-66 scaled by 1/(4 pi)).
+    ``b_eps`` is the reference's 1e-20 offset that keeps face B non-zero
+    (otvortex.pyx:65-66)."""
+    s = GridShape(n, n, 1, ng)
+    prim, bfld = s.empty()
+    x = s.centres(0)
+    y = s.centres(1)
+    X, Y = np.meshgrid(x, y, indexing="xy")
+    _put(prim[RHO], s, 25.0 / (36.0 * math.pi))
+    _put(prim[PR], s, 5.0 / (12.0 * math.pi))
+    _put(prim[VX], s, -np.sin(2 * math.pi * Y))
+    _put(prim[VY], s, np.sin(2 * math.pi * X))
+    b0 = 1.0 / math.sqrt(4.0 * math.pi)
+    dx, dy = s.dx(0), s.dx(1)
+    xe = np.arange(n + 1) * dx          # corner coordinates
+    ye = np.arange(n + 1) * dy
+    XE, YE = np.meshgrid(xe, ye, indexing="xy")
+    az = b0 * (np.cos(4 * math.pi * XE) / (4 * math.pi) + np.cos(2 * math.pi * YE) / (2 * math.pi))
+    bx = (az[1:, :-1] - az[:-1, :-1]) / dy + b_eps   # face (i-1/2, j)
+    by = -(az[:-1, 1:] - az[:-1, :-1]) / dx + b_eps  # face (i, j-1/2)
+    _put(bfld[0], s, bx)
+    _put(bfld[1], s, by)
+    return s, prim, bfld
+
+
+def _kgrid(n):
+    k = np.fft.fftfreq(n, d=1.0 / n)
+    return k
+
+
+def _band_field(rng, n, slope, kmin, kmax, ncomp):
+    """Real Gaussian fields with |f_k| proportional to k**slope on the shell
+    band kmin <= |k| <= kmax (integer wavenumbers of a unit box), from white
+    noise filtered in Fourier space.  Returns complex spectra (ncomp, n, n, n)."""
+    k = _kgrid(n)
+    KZ, KY, KX = np.meshgrid(k, k, k, indexing="ij")
+    kk = np.sqrt(KX ** 2 + KY ** 2 + KZ ** 2)
+    band = (kk >= kmin) & (kk <= kmax)
+    amp = np.zeros_like(kk)
+    amp[band] = kk[band] ** slope
+    out = []
+    for _ in range(ncomp):
+        noise = rng.standard_normal((n, n, n))
+        out.append(np.fft.fftn(noise) * amp)
+    return np.array(out), (KX, KY, KZ)
+
+
+def turbulence(n, ng=3, seed_b=1001, seed_v=1002, b0=0.2, db_rms=0.05, mach=0.5,
+               rho0=1.0, p0=0.6, kmin=2.0, kmax=8.0):
+    """3D decaying MHD turbulence on the periodic unit cube.
+
+    rho=1, p=0.6 (c_s=1 at gamma=5/3), uniform Bx=b0.  The velocity is a
+    solenoidal Gaussian field with a k^-2 shell spectrum on kmin<=|k|<=kmax
+    and rms |v| = mach.  The field perturbation is the discrete curl of an
+    edge-centred Gaussian vector potential with |A_k| ~ k^-3 (k^-2 shell
+    spectrum of B) scaled to rms |dB| = db_rms; it is solenoidal to round-off
+    on the staggered grid."""
+    s = GridShape(n, n, n, ng)
+    prim, bfld = s.empty()
+    d = 1.0 / n
+    # velocity: |v_k| ~ k^-2 (shell spectrum 4 pi k^2 |v_k|^2 ~ k^-2)
+    rng = np.random.Generator(np.random.PCG64(seed_v))
+    vk, (KX, KY, KZ) = _band_field(rng, n, -2.0, kmin, kmax, 3)
+    k2 = KX ** 2 + KY ** 2 + KZ ** 2
+    k2[0, 0, 0] = 1.0
+    kdotv = KX * vk[0] + KY * vk[1] + KZ * vk[2]
+    vk[0] -= KX * kdotv / k2
+    vk[1] -= KY * kdotv / k2
+    vk[2] -= KZ * kdotv / k2
+    v = np.real(np.fft.ifftn(vk, axes=(1, 2, 3)))
+    v *= mach / math.sqrt(np.mean(np.sum(v ** 2, axis=0)))
+    del vk, kdotv
+    # vector potential on cell edges: A_x at (i, j-1/2, k-1/2),
+    # A_y at (i-1/2, j, k-1/2), A_z at (i-1/2, j-1/2, k)
+    rng = np.random.Generator(np.random.PCG64(seed_b))
+    ak, _ = _band_field(rng, n, -3.0, kmin, kmax, 3)
+    ph = lambda kv: np.exp(-1j * math.pi * kv / n)  # shift by -dx/2 of a unit box grid
+    ax = np.real(np.fft.ifftn(ak[0] * ph(KY) * ph(KZ)))
+    ay = np.real(np.fft.ifftn(ak[1] * ph(KX) * ph(KZ)))
+    az = np.real(np.fft.ifftn(ak[2] * ph(KX) * ph(KY)))
+    del ak, KX, KY, KZ, k2
+
+    def curl(ax, ay, az):
+        # arrays are [k][j][i]; roll(-1) along an axis reads index+1
+        bx = (np.roll(az, -1, axis=1) - az) / d - (np.roll(ay, -1, axis=0) - ay) / d
+        by = (np.roll(ax, -1, axis=0) - ax) / d - (np.roll(az, -1, axis=2) - az) / d
+        bz = (np.roll(ay, -1, axis=2) - ay) / d - (np.roll(ax, -1, axis=1) - ax) / d
+        return bx, by, bz
+
+    bx, by, bz = curl(ax, ay, az)
+    # face-to-centre average for the rms of the perturbation
+    cx = 0.5 * (bx + np.roll(bx, -1, axis=2))
+    cy = 0.5 * (by + np.roll(by, -1, axis=1))
+    cz = 0.5 * (bz + np.roll(bz, -1, axis=0))
+    scale = db_rms / math.sqrt(np.mean(cx ** 2 + cy ** 2 + cz ** 2))
+    del cx, cy, cz
+    bx, by, bz = curl(ax * scale, ay * scale, az * scale)
+    _put(prim[RHO], s, rho0)
+    _put(prim[PR], s, p0)
+    _put(prim[VX], s, v[0])
+    _put(prim[VY], s, v[1])
+    _put(prim[VZ], s, v[2])
+    _put(bfld[0], s, bx + b0)
+    _put(bfld[1], s, by)
+    _put(bfld[2], s, bz)
+    return s, prim, bfld
+
+
+def brio_wu(n, axis=0, ng=3, transverse=4):
+    """Brio-Wu shock tube (src/problem/st1d.pyx:66-83) laid along ``axis``
+    of a grid with ``transverse`` cells in the other active axes (1D if
+    transverse is 0).  Left: rho=1, p=1, B=(0.75, 1, 0); right: rho=0.125,
+    p=0.1, B=(0.75, -1, 0), rotated so that the normal component lies along
+    ``axis``; the interface is at the box centre."""
+    sizes = [transverse] * 3 if transverse else [1, 1, 1]
+    sizes[axis] = n
+    if not transverse:
+        if axis != 0:
+            raise ValueError("1D tubes lie along x")
+    s = GridShape(sizes[0], sizes[1], sizes[2], ng)
+    prim, bfld = s.empty()
+    xc = s.centres(axis)
+    left = xc <= 0.5
+    shape = [1, 1, 1]
+    shape[2 - axis] = n
+    L = left.reshape(shape)
+    sk, sj, si = s.active()
+    act = (slice(None), sk, sj, si)
+    full = np.ones((s.nz if s.dims >= 3 else 1, s.ny if s.dims >= 2 else 1, s.nx))
+    prim[RHO][act[1:]] = np.where(L, 1.0, 0.125) * full
+    prim[PR][act[1:]] = np.where(L, 1.0, 0.1) * full
+    bn, bt = 0.75, np.where(L, 1.0, -1.0) * full
+    # components (normal, first transverse, second transverse) in cyclic order
+    tr1 = (axis + 1) % 3
+    bfld[axis][act[1:]] = bn * full
+    bfld[tr1][act[1:]] = bt
+    return s, prim, bfld
+
+
+def linear_wave(n, wave="fast", amp=1e-6, ng=3):
+    """1D linear MHD eigenmode on [0,1] (state of src/problem/linwaves1d.pyx:
+    48-104): rho=1, p=0.6, B=(1, sqrt 2, 1/2), gamma=5/3 (c_s=1, c_A,x=1,
+    c_fast=2, c_slow=1/2), perturbation amp*cos(2 pi x) times the eigenvector
+    of the fast, slow or Alfven wave.  The pressure perturbation is the
+    adiabatic one, dp = c_s^2 drho; the reference's commented formulas put the
+    conserved energy perturbation of the eigenvector into p (linwaves1d.pyx:
+    91,103), which is not an eigenmode."""
+    s = GridShape(n, 1, 1, ng)
+    prim, bfld = s.empty()
+    x = s.centres(0)
+    a = amp * np.cos(2 * math.pi * x)
+    r5 = math.sqrt(5.0)
+    if wave == "fast":
+        ev = dict(rho=1 / r5, vx=-2 / r5, vy=2 * math.sqrt(2) / 3 / r5, vz=1.0 / 3 / r5,
+                  by=4 * math.sqrt(2) / 3 / r5, bz=2.0 / 3 / r5, p=1.0 / r5)
+    elif wave == "slow":
+        ev = dict(rho=2 / r5, vx=-1 / r5, vy=-4 * math.sqrt(2) / 3 / r5, vz=-2.0 / 3 / r5,
+                  by=-2 * math.sqrt(2) / 3 / r5, bz=-1.0 / 3 / r5, p=2.0 / r5)
+    elif wave == "alfven":
+        ev = dict(rho=0.0, vx=0.0, vy=-1.0 / 3, vz=2 * math.sqrt(2) / 3, by=-1.0 / 3,
+                  bz=2 * math.sqrt(2) / 3, p=0.0)
+    else:
+        raise ValueError(wave)
+    _put(prim[RHO], s, 1.0 + ev["rho"] * a)
+    _put(prim[VX], s, ev["vx"] * a)
+    _put(prim[VY], s, ev["vy"] * a)
+    _put(prim[VZ], s, ev["vz"] * a)
+    _put(prim[PR], s, 0.6 + ev["p"] * a)
+    _put(bfld[0], s, 1.0)
+    _put(bfld[1], s, math.sqrt(2.0) + ev["by"] * a)
+    _put(bfld[2], s, 0.5 + ev["bz"] * a)
+    return s, prim, bfld
+
+
+def blast2d(n, ng=3, seed=5005, p_in=10.0, p_out=0.1, radius=0.1):
+    """2D MHD blast on the periodic unit square: rho=1, p=p_out, uniform B at
+    a random angle with |B| in [0.5, 2], p=p_in inside r<radius of the centre
+    (cf. src/problem/blast_MHD.pyx:24-59)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    theta = rng.uniform(0.0, 2 * math.pi)
+    bmag = rng.uniform(0.5, 2.0)
+    s = GridShape(n, n, 1, ng)
+    prim, bfld = s.empty()
+    x = s.centres(0)
+    y = s.centres(1)
+    X, Y = np.meshgrid(x, y, indexing="xy")
+    r = np.sqrt((X - 0.5) ** 2 + (Y - 0.5) ** 2)
+    _put(prim[RHO], s, 1.0)
+    _put(prim[PR], s, np.where(r < radius, p_in, p_out))
+    _put(bfld[0], s, bmag * math.cos(theta))
+    _put(bfld[1], s, bmag * math.sin(theta))
+    return s, prim, bfld
+
+
+def riemann_tube(n, rng, ng=3):
+    """One seeded 1D MHD Riemann problem on [0,1]: rho in [0.1,1], p in
+    [0.1,1], |v| <= 1 along a random direction, Bx in [0,1] shared by both
+    sides, By, Bz in [-1,1]; interface at x=0.5."""
+    s = GridShape(n, 1, 1, ng)
+    prim, bfld = s.empty()
+    x = s.centres(0)
+    left = x <= 0.5
+    bx = rng.uniform(0.0, 1.0)
+    states = []
+    for _ in range(2):
+        d = rng.standard_normal(3)
+        d /= np.linalg.norm(d)
+        v = d * rng.uniform(0.0, 1.0)
+        states.append(dict(rho=rng.uniform(0.1, 1.0), p=rng.uniform(0.1, 1.0), v=v,
+                           by=rng.uniform(-1.0, 1.0), bz=rng.uniform(-1.0, 1.0)))
+    L, R = states
+    _put(prim[RHO], s, np.where(left, L["rho"], R["rho"]))
+    _put(prim[PR], s, np.where(left, L["p"], R["p"]))
+    for c, var in enumerate((VX, VY, VZ)):
+        _put(prim[var], s, np.where(left, L["v"][c], R["v"][c]))
+    _put(bfld[0], s, bx)
+    _put(bfld[1], s, np.where(left, L["by"], R["by"]))
+    _put(bfld[2], s, np.where(left, L["bz"], R["bz"]))
+    return s, prim, bfld

@@ -1,0 +1,1111 @@
+/*
+ * bnz_oracle.c -- CPU oracle for the Van-Leer MHD step (TEST INFRASTRUCTURE).
+ *
+ * The checker for the HIP path, never the product.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.
+ *
+ * Restates sergekomarov/bonzo's algorithm with the reference's operation
+ * order, so that an IEEE build (no contraction, no fast-math) is reproducible
+ * bit for bit by any other implementation that keeps the same order.
+ * Decisions where the shipped reference cannot be followed literally:
+ *   D1  swap_array_ptrs is a no-op (src/util.pyx:204-208); the y/z sweeps
+ *       here use the double-buffered left state the code evidently intends
+ *       (src/integration/godunov.pyx:141-160,185-204).
+ *   D2  fms() ignores its bx argument (src/mhd/eos_c.h:474-476), so new_dt
+ *       uses the x fast speed for all axes (new_dt.pyx:76-91).  Kept
+ *       (dt_fms_ref=1, default); dt_fms_ref=0 gives per-axis fast speeds.
+ *   D11 advance_b's By loop lacks the +1 face (src/integration/ct.pyx:47-57).
+ *       The never-written face feeds interp_bc and, through cons2prim's
+ *       magnetic pressure, the PLM state of the last active y face, so the
+ *       face is updated here like Bx and Bz (ct.pyx:26,75).
+ *   D14 z1 outflow passes k1 instead of k1-1 (grid_bc_funcs.pyx:179 vs
+ *       :121,:150) and overwrites the first active plane; the intent (k1-1)
+ *       is followed.
+ *   FMAX/FMIN are one-argument macros (src/defs_c.h:142-143) called with two
+ *       arguments (src/mhd/fluxes_c.c:38-40,208-209); fmax/fmin are used.
+ *   2D: advance_b's Bz loop runs k in [0,1] with Ntot_z=1 (ct.pyx:75), an
+ *       out-of-bounds write; only k=0 is updated here.
+ */
+#include "bnz_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* Variable slots, src/defs_c.h:68-82 with MFIELD, NMODE=9. */
+enum { RHO = 0, MX = 1, MY = 2, MZ = 3, EN = 4, PSC = 5, BX = 6, BY = 7, BZ = 8 };
+enum { VX = 1, VY = 2, VZ = 3, PR = 4 };
+#define NMODE OR_NMODE
+
+static inline double sqr(double x) { return x * x; }
+/* SIGN(0)=0, src/defs_c.h:150 */
+static inline double sign0(double a) { return (double)((0.0 < a) - (a < 0.0)); }
+
+/* ------------------------------------------------------------------------ */
+/* EOS point functions                                                       */
+/* ------------------------------------------------------------------------ */
+
+/* src/mhd/eos_c.h:20-55 */
+void or_cons2prim(double *w, const double *u, double gam)
+{
+  double rhoi = 1. / u[RHO];
+  w[RHO] = u[RHO];
+  w[VX] = u[MX] * rhoi;
+  w[VY] = u[MY] * rhoi;
+  w[VZ] = u[MZ] * rhoi;
+  double v2 = sqr(w[VX]) + sqr(w[VY]) + sqr(w[VZ]);
+  double b2 = sqr(u[BX]) + sqr(u[BY]) + sqr(u[BZ]);
+  w[PR] = (gam - 1.) * (u[EN] - 0.5 * (b2 + v2 * u[RHO]));
+  w[PSC] = u[PSC];
+  w[BX] = u[BX];
+  w[BY] = u[BY];
+  w[BZ] = u[BZ];
+}
+
+/* src/mhd/eos_c.h:60-95 */
+void or_prim2cons(double *u, const double *w, double gam)
+{
+  double gamm1i = 1. / (gam - 1.);
+  u[RHO] = w[RHO];
+  u[MX] = w[RHO] * w[VX];
+  u[MY] = w[RHO] * w[VY];
+  u[MZ] = w[RHO] * w[VZ];
+  double v2 = sqr(w[VX]) + sqr(w[VY]) + sqr(w[VZ]);
+  double b2 = sqr(w[BX]) + sqr(w[BY]) + sqr(w[BZ]);
+  u[EN] = 0.5 * (w[RHO] * v2 + b2) + w[PR] * gamm1i;
+  u[PSC] = w[PSC];
+  u[BX] = w[BX];
+  u[BY] = w[BY];
+  u[BZ] = w[BZ];
+}
+
+/* src/mhd/eos_c.h:416-461 (MFIELD, no CGL/TWOTEMP: a=1) */
+void or_primcons2flux(double *f, const double *w, const double *u, double gam)
+{
+  (void)gam;
+  double bx2 = sqr(w[BX]);
+  double b2 = bx2 + sqr(w[BY]) + sqr(w[BZ]);
+  double bxby = w[BX] * w[BY];
+  double bxbz = w[BX] * w[BZ];
+  double ptot = 0.;
+  ptot += 0.5 * b2;
+  double a = 1.;
+  ptot += w[PR];
+  for (int n = 0; n < BX; ++n) f[n] = u[n] * w[VX];
+  f[MX] += ptot;
+  f[EN] += ptot * w[VX];
+  f[MX] -= a * bx2;
+  f[MY] -= a * bxby;
+  f[MZ] -= a * bxbz;
+  f[EN] -= a * (w[VX] * bx2 + w[VY] * bxby + w[VZ] * bxbz);
+  f[BX] = 0.;
+  f[BY] = w[VX] * w[BY] - w[VY] * w[BX];
+  f[BZ] = w[VX] * w[BZ] - w[VZ] * w[BX];
+}
+
+/* src/mhd/eos_c.h:468-530 (MFIELD branch).  bx is ignored as in the
+   reference (D2): the x component w[BX] is always used. */
+double or_fms(const double *w, double bx, double gam)
+{
+  (void)bx;
+  double bx2 = sqr(w[BX]);
+  double b2 = bx2 + sqr(w[BY]) + sqr(w[BZ]);
+  double p = w[PR];
+  double gamp = gam * p;
+  double c1 = gamp + b2;
+  double c2 = sqrt(fabs(c1 * c1 - 4. * gamp * bx2));
+  return sqrt(0.5 / w[RHO] * (c1 + c2));
+}
+
+/* src/coordinates/reconstr_c.h:39-44 */
+double or_vl_lim(double a, double b)
+{
+  double ab = a * b;
+  return (ab > 0.) ? (2. * ab / (a + b)) : 0.;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Riemann solvers, pencil ABI                                               */
+/* ------------------------------------------------------------------------ */
+
+/* src/mhd/fluxes_c.c:25-42 (FMAX/FMIN read as fmax/fmin) */
+static inline void sLsR_simple(double *sL, double *sR, const double *wL, const double *wR,
+                               double gam)
+{
+  double bx = wL[BX];
+  double cfL = or_fms(wL, bx, gam);
+  double cfR = or_fms(wR, bx, gam);
+  double cf_max = fmax(cfL, cfR);
+  *sL = fmin(wL[VX], wR[VX]) - cf_max;
+  *sR = fmax(wL[VX], wR[VX]) + cf_max;
+}
+
+static inline void load_face(double *wL, double *wR, double **_wL, double **_wR,
+                             const double *_bx, int i)
+{
+  for (int n = 0; n < NMODE; ++n) {
+    wL[n] = _wL[n][i];
+    wR[n] = _wR[n][i];
+  }
+  wL[BX] = _bx[i];
+  wR[BX] = _bx[i];
+}
+
+/* src/mhd/fluxes_c.c:175-223 */
+void or_hll_flux(double **_flux, double **_wL, double **_wR, double *_bx, int i1, int i2,
+                 double gam)
+{
+  double uL[NMODE], uR[NMODE], wL[NMODE], wR[NMODE], fL[NMODE], fR[NMODE];
+  for (int i = i1; i <= i2; ++i) {
+    load_face(wL, wR, _wL, _wR, _bx, i);
+    or_prim2cons(uL, wL, gam);
+    or_prim2cons(uR, wR, gam);
+    or_primcons2flux(fL, wL, uL, gam);
+    or_primcons2flux(fR, wR, uR, gam);
+    double sL = 0., sR = 0.;
+    sLsR_simple(&sL, &sR, wL, wR, gam);
+    double aL = fmin(sL, 0.);
+    double aR = fmax(sR, 0.);
+    double aLaR = aL * aR;
+    double aRmaLi = 1. / (aR - aL);
+    for (int n = 0; n < NMODE; ++n)
+      _flux[n][i] = (aR * fL[n] - aL * fR[n] + aLaR * (uR[n] - uL[n])) * aRmaLi;
+    _flux[BX][i] = 0.;
+  }
+}
+
+/* src/mhd/fluxes_c.c:447-607 (MFIELD, no TWOTEMP) */
+void or_hllt_flux(double **_flux, double **_wL, double **_wR, double *_bx, int i1, int i2,
+                  double gam)
+{
+  double uL[NMODE], uR[NMODE], uLs[NMODE], uRs[NMODE], uCs[NMODE];
+  double wL[NMODE], wR[NMODE], fL[NMODE], fR[NMODE];
+  for (int i = i1; i <= i2; ++i) {
+    load_face(wL, wR, _wL, _wR, _bx, i);
+    double bx = _bx[i];
+    or_prim2cons(uL, wL, gam);
+    or_prim2cons(uR, wR, gam);
+    or_primcons2flux(fL, wL, uL, gam);
+    or_primcons2flux(fR, wR, uR, gam);
+    double sL = 0., sR = 0.;
+    sLsR_simple(&sL, &sR, wL, wR, gam);
+    double sRsLi = 1. / (sR - sL);
+    double rhos = (sR * uR[RHO] - sL * uL[RHO] - (fR[RHO] - fL[RHO])) * sRsLi;
+    double mxs = (sR * uR[MX] - sL * uL[MX] - (fR[MX] - fL[MX])) * sRsLi;
+    double frhos = (sR * fL[RHO] - sL * fR[RHO] + sR * sL * (uR[RHO] - uL[RHO])) * sRsLi;
+    double us = frhos / rhos;
+    double pscs = (sR * uR[PSC] - sL * uL[PSC] - (fR[PSC] - fL[PSC])) * sRsLi;
+    uLs[RHO] = rhos;  uRs[RHO] = rhos;
+    uLs[MX] = mxs;    uRs[MX] = mxs;
+    uLs[PSC] = pscs;  uRs[PSC] = pscs;
+    uLs[MY] = rhos * wL[VY];
+    uRs[MY] = rhos * wR[VY];
+    uLs[MZ] = rhos * wL[VZ];
+    uRs[MZ] = rhos * wR[VZ];
+    double sLs = us, sRs = us;
+    double rhosi = 1. / rhos;
+    double sqrt_rhosi = sqrt(rhosi);
+    sLs -= fabs(bx) * sqrt_rhosi;
+    sRs += fabs(bx) * sqrt_rhosi;
+    double xL = 1. / ((sL - sLs) * (sL - sRs));
+    double xR = 1. / ((sR - sLs) * (sR - sRs));
+    double yL = bx * xL * (us - wL[VX]);
+    double yR = bx * xR * (us - wR[VX]);
+    uLs[MY] -= yL * wL[BY];
+    uRs[MY] -= yR * wR[BY];
+    uLs[MZ] -= yL * wL[BZ];
+    uRs[MZ] -= yR * wR[BZ];
+    yL = (wL[RHO] * sqr(sL - wL[VX]) - sqr(bx)) * rhosi * xL;
+    yR = (wR[RHO] * sqr(sR - wR[VX]) - sqr(bx)) * rhosi * xR;
+    uLs[BY] = yL * wL[BY];
+    uRs[BY] = yR * wR[BY];
+    uLs[BZ] = yL * wL[BZ];
+    uRs[BZ] = yR * wR[BZ];
+    uCs[RHO] = rhos;
+    uCs[MX] = mxs;
+    uCs[PSC] = pscs;
+    double sgnb = sign0(bx);
+    double x = 0.5 * sgnb / sqrt_rhosi;
+    uCs[MY] = 0.5 * (uLs[MY] + uRs[MY]) + x * (uRs[BY] - uLs[BY]);
+    uCs[MZ] = 0.5 * (uLs[MZ] + uRs[MZ]) + x * (uRs[BZ] - uLs[BZ]);
+    uCs[BY] = 0.5 * (uLs[BY] + uRs[BY]) + x * rhosi * (uRs[MY] - uLs[MY]);
+    uCs[BZ] = 0.5 * (uLs[BZ] + uRs[BZ]) + x * rhosi * (uRs[MZ] - uLs[MZ]);
+    /* The reference never sets the energy and Bx slots of the star states
+       (fluxes_c.c:504-573), so its EN flux in regions 2-4 reads
+       uninitialised stack memory.  Zero is used for those slots here; the
+       energy flux of HLLT is therefore only defined in regions 1 and 5. */
+    uLs[EN] = uRs[EN] = uCs[EN] = 0.;
+    uLs[BX] = uRs[BX] = uCs[BX] = 0.;
+    int reg = 0;
+    if (sL > 0.) reg = 1;
+    if (sL <= 0. && 0. < sLs) reg = 2;
+    if (sLs <= 0. && 0. < sRs) reg = 3;
+    if (sRs <= 0. && 0. < sR) reg = 4;
+    if (sR <= 0.) reg = 5;
+    for (int n = 0; n < NMODE; ++n) {
+      if (reg == 1) _flux[n][i] = fL[n];
+      else if (reg == 2) _flux[n][i] = fL[n] + sL * (uLs[n] - uL[n]);
+      else if (reg == 3)
+        _flux[n][i] = fL[n] + sL * (uLs[n] - uL[n]) + sLs * (uCs[n] - uLs[n]);
+      else if (reg == 4) _flux[n][i] = fR[n] + sR * (uRs[n] - uR[n]);
+      else if (reg == 5) _flux[n][i] = fR[n];
+    }
+    _flux[BX][i] = 0.;
+  }
+}
+
+/* src/mhd/fluxes_c.c:616-834 (no TWOTEMP).  The region chain is a
+   sequence of independent ifs, so the last true condition wins and, if none
+   holds (only with NaN speeds), the flux row keeps its previous content. */
+void or_hlld_flux(double **_flux, double **_wL, double **_wR, double *_bx, int i1, int i2,
+                  double gam)
+{
+  double uL[NMODE], uR[NMODE], uLs[NMODE], uRs[NMODE], uLss[NMODE], uRss[NMODE];
+  double wL[NMODE], wR[NMODE], fL[NMODE], fR[NMODE];
+  for (int i = i1; i <= i2; ++i) {
+    load_face(wL, wR, _wL, _wR, _bx, i);
+    double bx = _bx[i];
+    or_prim2cons(uL, wL, gam);
+    or_prim2cons(uR, wR, gam);
+    or_primcons2flux(fL, wL, uL, gam);
+    or_primcons2flux(fR, wR, uR, gam);
+    double sL = 0., sR = 0.;
+    sLsR_simple(&sL, &sR, wL, wR, gam);
+
+    double b2hL = 0.5 * (sqr(bx) + sqr(wL[BY]) + sqr(wL[BZ]));
+    double b2hR = 0.5 * (sqr(bx) + sqr(wR[BY]) + sqr(wR[BZ]));
+    double ptL = wL[PR] + b2hL;
+    double ptR = wR[PR] + b2hR;
+
+    double deni = 1. / (sR * uR[RHO] - sL * uL[RHO] - (fR[RHO] - fL[RHO]));
+    double sM = (sR * uR[MX] - sL * uL[MX] - (fR[MX] - fL[MX])) * deni;
+
+    double sLvxL = sL - wL[VX];
+    double sRvxR = sR - wR[VX];
+    double sLsMi = 1. / (sL - sM);
+    double sRsMi = 1. / (sR - sM);
+    double xL = sLvxL * sLsMi;
+    double xR = sRvxR * sRsMi;
+
+    uLs[RHO] = uL[RHO] * xL;
+    uRs[RHO] = uR[RHO] * xR;
+    uLs[PSC] = uL[PSC] * xL;
+    uRs[PSC] = uR[PSC] * xR;
+
+    double pts = wL[RHO] * sLvxL * (sM - wL[VX]) + ptL;
+
+    double rhosLvxL = wL[RHO] * sLvxL;
+    double rhosRvxR = wR[RHO] * sRvxR;
+    double bx2 = sqr(bx);
+    double wvbL = 1. / (rhosLvxL * (sL - sM) - bx2);
+    double wvbR = 1. / (rhosRvxR * (sR - sM) - bx2);
+    double bxsMvxL = bx * (sM - wL[VX]);
+    double bxsMvxR = bx * (sM - wR[VX]);
+    double wvL = bxsMvxL * wvbL;
+    double wvR = bxsMvxR * wvbR;
+    double vyLs = wL[VY] - wL[BY] * wvL;
+    double vyRs = wR[VY] - wR[BY] * wvR;
+    double vzLs = wL[VZ] - wL[BZ] * wvL;
+    double vzRs = wR[VZ] - wR[BZ] * wvR;
+
+    uLs[MX] = uLs[RHO] * sM;
+    uRs[MX] = uRs[RHO] * sM;
+    uLs[MY] = uLs[RHO] * vyLs;
+    uRs[MY] = uRs[RHO] * vyRs;
+    uLs[MZ] = uLs[RHO] * vzLs;
+    uRs[MZ] = uRs[RHO] * vzRs;
+
+    double rhosLvxL2_bx2 = rhosLvxL * sLvxL - bx2;
+    double rhosRvxR2_bx2 = rhosRvxR * sRvxR - bx2;
+    double wbL = rhosLvxL2_bx2 * wvbL;
+    double wbR = rhosRvxR2_bx2 * wvbR;
+    uLs[BY] = wL[BY] * wbL;
+    uRs[BY] = wR[BY] * wbR;
+    uLs[BZ] = wL[BZ] * wbL;
+    uRs[BZ] = wR[BZ] * wbR;
+
+    double vsbsL = sM * bx + vyLs * uLs[BY] + vzLs * uLs[BZ];
+    double vsbsR = sM * bx + vyRs * uRs[BY] + vzRs * uRs[BZ];
+    uLs[EN] = (sLvxL * uL[EN] - ptL * wL[VX] + pts * sM +
+               bx * (wL[VX] * bx + wL[VY] * wL[BY] + wL[VZ] * wL[BZ] - vsbsL)) * sLsMi;
+    uRs[EN] = (sRvxR * uR[EN] - ptR * wR[VX] + pts * sM +
+               bx * (wR[VX] * bx + wR[VY] * wR[BY] + wR[VZ] * wR[BZ] - vsbsR)) * sRsMi;
+
+    uLss[RHO] = uLs[RHO];
+    uRss[RHO] = uRs[RHO];
+    uLss[MX] = uLs[MX];
+    uRss[MX] = uRs[MX];
+    uLss[PSC] = uLs[PSC];
+    uRss[PSC] = uRs[PSC];
+
+    double sqrt_rhoLs = sqrt(uLs[RHO]);
+    double sqrt_rhoRs = sqrt(uRs[RHO]);
+    double sLs = sM - fabs(bx) / sqrt_rhoLs;
+    double sRs = sM + fabs(bx) / sqrt_rhoRs;
+    double rhoLRi = 1. / (sqrt_rhoLs + sqrt_rhoRs);
+    double sgnb = sign0(bx);
+
+    double vyss = (sqrt_rhoLs * vyLs + sqrt_rhoRs * vyRs + (uRs[BY] - uLs[BY]) * sgnb) * rhoLRi;
+    double vzss = (sqrt_rhoLs * vzLs + sqrt_rhoRs * vzRs + (uRs[BZ] - uLs[BZ]) * sgnb) * rhoLRi;
+    uLss[MY] = uLs[RHO] * vyss;
+    uRss[MY] = uRs[RHO] * vyss;
+    uLss[MZ] = uLs[RHO] * vzss;
+    uRss[MZ] = uRs[RHO] * vzss;
+
+    double byss = (sqrt_rhoLs * uRs[BY] + sqrt_rhoRs * uLs[BY] +
+                   sqrt_rhoLs * sqrt_rhoRs * (vyRs - vyLs) * sgnb) * rhoLRi;
+    double bzss = (sqrt_rhoLs * uRs[BZ] + sqrt_rhoRs * uLs[BZ] +
+                   sqrt_rhoLs * sqrt_rhoRs * (vzRs - vzLs) * sgnb) * rhoLRi;
+    uLss[BY] = byss;
+    uRss[BY] = byss;
+    uLss[BZ] = bzss;
+    uRss[BZ] = bzss;
+
+    double vssbss = sM * bx + vyss * byss + vzss * bzss;
+    uLss[EN] = uLs[EN] - sqrt_rhoLs * (vsbsL - vssbss) * sgnb;
+    uRss[EN] = uRs[EN] + sqrt_rhoRs * (vsbsR - vssbss) * sgnb;
+
+    /* Bx slot of the star states is never set in the reference; its flux is
+       overwritten with 0 below, so any value will do. */
+    uLs[BX] = uRs[BX] = uLss[BX] = uRss[BX] = bx;
+
+    int reg = 0;
+    if (sL > 0.) reg = 1;
+    if (sL <= 0. && 0. < sLs) reg = 2;
+    if (sLs <= 0. && 0. < sM) reg = 3;
+    if (sM <= 0. && 0. < sRs) reg = 4;
+    if (sRs <= 0. && 0. < sR) reg = 5;
+    if (sR <= 0.) reg = 6;
+    for (int n = 0; n < NMODE; ++n) {
+      if (reg == 1) _flux[n][i] = fL[n];
+      else if (reg == 2) _flux[n][i] = fL[n] + sL * (uLs[n] - uL[n]);
+      else if (reg == 3)
+        _flux[n][i] = fL[n] + sL * (uLs[n] - uL[n]) + sLs * (uLss[n] - uLs[n]);
+      else if (reg == 4)
+        _flux[n][i] = fR[n] + sR * (uRs[n] - uR[n]) + sRs * (uRss[n] - uRs[n]);
+      else if (reg == 5) _flux[n][i] = fR[n] + sR * (uRs[n] - uR[n]);
+      else if (reg == 6) _flux[n][i] = fR[n];
+    }
+    _flux[BX][i] = 0.;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Reconstruction                                                            */
+/* ------------------------------------------------------------------------ */
+
+/* src/coordinates/reconstr_c.c:8-22 */
+void or_reconstr_const(double **wl, double **wr, double ***w, double ***scr, int ax, int i1,
+                       int i2)
+{
+  (void)scr;
+  int ip = (ax == 0) ? 1 : 0;
+  for (int n = 0; n < NMODE; ++n)
+    for (int i = i1; i <= i2; ++i) {
+      wr[n][i] = w[0][n][i];
+      wl[n][i + ip] = w[0][n][i];
+    }
+}
+
+/* src/coordinates/reconstr_c.c:29-91, uniform flat branch, char_proj=0 */
+void or_reconstr_linear(double **wl, double **wr, double ***w, double ***scr, int ax, int i1,
+                        int i2)
+{
+  double **dm = scr[0], **dp = scr[1], **d = scr[0], **wc = scr[2];
+  int ip = (ax == 0) ? 1 : 0;
+  for (int n = 0; n < NMODE; n++)
+    for (int i = i1; i <= i2; i++) {
+      wc[n][i] = w[1][n][i];
+      dm[n][i] = w[1][n][i] - w[0][n][i];
+      dp[n][i] = w[2][n][i] - w[1][n][i];
+    }
+  for (int n = 0; n < NMODE; n++)
+    for (int i = i1; i <= i2; i++) d[n][i] = 0.5 * or_vl_lim(dm[n][i], dp[n][i]);
+  for (int n = 0; n < NMODE; n++)
+    for (int i = i1; i <= i2; i++) {
+      wr[n][i] = wc[n][i] - d[n][i];
+      wl[n][i + ip] = wc[n][i] + d[n][i];
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Grid, coordinates and simulation state                                    */
+/* ------------------------------------------------------------------------ */
+
+typedef void (*rsolver_fn)(double **, double **, double **, double *, int, int, double);
+typedef void (*reconstr_fn)(double **, double **, double ***, double ***, int, int, int);
+
+struct or_sim {
+  int dims, ng, reconstr_order, nthreads, dt_fms_ref;
+  int Nact[3], Ntot[3];
+  int i1, i2, j1, j2, k1, k2;
+  double lmin[3], lmax[3];
+  double *lf[3], *lv[3], *dlf[3], *dlf_inv[3], *dlv[3];
+  double gam, cour;
+  int rsolver;
+  int bc[3][2];
+  long ncell;
+  double *prim, *cons, *bfld, *cons_s, *bfld_s, *flx[3], *ec, *ee;
+  double time, dt;
+  int step;
+};
+
+#define IDX(s, n, k, j, i) \
+  ((((long)(n) * (s)->Ntot[2] + (k)) * (s)->Ntot[1] + (j)) * (s)->Ntot[0] + (i))
+
+int or_ntot(const or_sim *s, int axis) { return s->Ntot[axis]; }
+int or_ng(const or_sim *s) { return s->ng; }
+double *or_prim(or_sim *s) { return s->prim; }
+double *or_cons(or_sim *s) { return s->cons; }
+double *or_bfld(or_sim *s) { return s->bfld; }
+double or_time(const or_sim *s) { return s->time; }
+int or_step(const or_sim *s) { return s->step; }
+double or_last_dt(const or_sim *s) { return s->dt; }
+void or_set_dt_fms_ref(or_sim *s, int on) { s->dt_fms_ref = on; }
+
+double *or_coord(or_sim *s, int which, int axis)
+{
+  switch (which) {
+  case 0: return s->lf[axis];
+  case 1: return s->lv[axis];
+  case 2: return s->dlf[axis];
+  case 3: return s->dlf_inv[axis];
+  default: return s->dlv[axis];
+  }
+}
+
+/* Uniform Cartesian spacings: src/coordinates/coord.pyx:201-257 (uniform
+   branch, one block, ghost faces re-spaced for periodic axes) and
+   src/coordinates/cartesian.pyx:18-43. */
+static void set_coordinates(or_sim *s)
+{
+  for (int n = 0; n < 3; ++n) {
+    int N = s->Ntot[n];
+    s->lf[n] = calloc(N + 2, sizeof(double));
+    s->lv[n] = calloc(N + 2, sizeof(double));
+    s->dlf[n] = calloc(N + 2, sizeof(double));
+    s->dlf_inv[n] = calloc(N + 2, sizeof(double));
+    s->dlv[n] = calloc(N + 2, sizeof(double));
+    double *lf = s->lf[n];
+    if (N == 1) {
+      lf[0] = s->lmin[n];
+      lf[1] = s->lmax[n];
+    } else {
+      int ind1 = (n == 0) ? s->i1 : (n == 1) ? s->j1 : s->k1;
+      int ind2 = (n == 0) ? s->i2 : (n == 1) ? s->j2 : s->k2;
+      double dx = (s->lmax[n] - s->lmin[n]) / s->Nact[n];
+      double l1 = 0 * s->Nact[n] * dx; /* pos[n]=0 for a single block */
+      for (int i = 0; i < N + 1; ++i) lf[i] = l1 + (i - ind1) * dx;
+      if (s->bc[n][0] == 0)
+        for (int i = ind1 - 1; i > -1; --i) {
+          int i_ = ind2 - i + (ind1 - 1);
+          double dlf_ = lf[i_ + 1] - lf[i_];
+          lf[i] = lf[i + 1] - dlf_;
+        }
+      if (s->bc[n][1] == 0)
+        for (int i = ind2 + 1; i < N; ++i) {
+          int i_ = ind1 + i - (ind2 + 1);
+          double dlf_ = lf[i_ + 1] - lf[i_];
+          lf[i + 1] = lf[i] + dlf_;
+        }
+    }
+    for (int i = 0; i < N; ++i) {
+      s->dlf[n][i] = lf[i + 1] - lf[i];
+      s->dlf_inv[n][i] = 1. / s->dlf[n][i];
+    }
+    for (int i = 0; i < N; ++i) s->lv[n][i] = 0.5 * (lf[i] + lf[i + 1]);
+    for (int i = 1; i < N; ++i) s->dlv[n][i] = s->lv[n][i] - s->lv[n][i - 1];
+  }
+}
+
+or_sim *or_create(int nx, int ny, int nz, int reconstr_order, const double *lmin,
+                  const double *lmax, double gam, double cour, int rsolver, const int *bc_flags,
+                  int nthreads)
+{
+  or_sim *s = calloc(1, sizeof(or_sim));
+  s->dims = (nz > 1) ? 3 : (ny > 1) ? 2 : 1;
+  /* ng: src/coordinates/grid.pyx:129-134 */
+  s->reconstr_order = reconstr_order;
+  s->ng = (reconstr_order == 0) ? 2 : 3;
+  s->Nact[0] = nx;
+  s->Nact[1] = ny;
+  s->Nact[2] = nz;
+  s->Ntot[0] = nx + 2 * s->ng;
+  s->Ntot[1] = (s->dims >= 2) ? ny + 2 * s->ng : 1;
+  s->Ntot[2] = (s->dims >= 3) ? nz + 2 * s->ng : 1;
+  /* index ranges: src/coordinates/grid.pyx:155-161 */
+  s->i1 = s->ng;
+  s->i2 = nx + s->ng - 1;
+  s->j1 = (s->dims >= 2) ? s->ng : 0;
+  s->j2 = (s->dims >= 2) ? ny + s->ng - 1 : 0;
+  s->k1 = (s->dims >= 3) ? s->ng : 0;
+  s->k2 = (s->dims >= 3) ? nz + s->ng - 1 : 0;
+  for (int n = 0; n < 3; ++n) {
+    s->lmin[n] = lmin[n];
+    s->lmax[n] = lmax[n];
+    s->bc[n][0] = bc_flags[2 * n];
+    s->bc[n][1] = bc_flags[2 * n + 1];
+  }
+  s->gam = gam;
+  s->cour = cour;
+  s->rsolver = rsolver;
+  s->nthreads = nthreads > 0 ? nthreads : 1;
+  s->dt_fms_ref = 1;
+  set_coordinates(s);
+  s->ncell = (long)s->Ntot[0] * s->Ntot[1] * s->Ntot[2];
+  s->prim = calloc(NMODE * s->ncell, sizeof(double));
+  s->cons = calloc(NMODE * s->ncell, sizeof(double));
+  s->cons_s = calloc(NMODE * s->ncell, sizeof(double));
+  s->bfld = calloc(3 * s->ncell, sizeof(double));
+  s->bfld_s = calloc(3 * s->ncell, sizeof(double));
+  for (int d = 0; d < 3; ++d) s->flx[d] = calloc(NMODE * s->ncell, sizeof(double));
+  s->ec = calloc(3 * s->ncell, sizeof(double));
+  s->ee = calloc(3 * s->ncell, sizeof(double));
+  return s;
+}
+
+void or_destroy(or_sim *s)
+{
+  if (!s) return;
+  for (int n = 0; n < 3; ++n) {
+    free(s->lf[n]);
+    free(s->lv[n]);
+    free(s->dlf[n]);
+    free(s->dlf_inv[n]);
+    free(s->dlv[n]);
+    free(s->flx[n]);
+  }
+  free(s->prim);
+  free(s->cons);
+  free(s->cons_s);
+  free(s->bfld);
+  free(s->bfld_s);
+  free(s->ec);
+  free(s->ee);
+  free(s);
+}
+
+/* get_integr_lims: src/integration/integrator.pyx:375-383 */
+static void integr_lims(const or_sim *s, int order, int *l)
+{
+  l[0] = order + 1;
+  l[1] = s->Ntot[0] - 2 - order;
+  l[2] = order + 1;
+  l[3] = s->Ntot[1] - 2 - order;
+  l[4] = order + 1;
+  l[5] = s->Ntot[2] - 2 - order;
+  if (s->dims < 2) l[2] = l[3] = 0;
+  if (s->dims < 3) l[4] = l[5] = 0;
+}
+
+/* swap_mhd_vec: src/integration/godunov.pyx:297-324 */
+static void swap_mhd_vec(int ax, int *vars)
+{
+  vars[RHO] = RHO;
+  for (int n = 4; n < NMODE - 3; ++n) vars[n] = n;
+  if (ax == 0) {
+    vars[VX] = VX; vars[VY] = VY; vars[VZ] = VZ;
+    vars[BX] = BX; vars[BY] = BY; vars[BZ] = BZ;
+  } else if (ax == 1) {
+    vars[VX] = VY; vars[VY] = VZ; vars[VZ] = VX;
+    vars[BX] = BY; vars[BY] = BZ; vars[BZ] = BX;
+  } else {
+    vars[VX] = VZ; vars[VY] = VX; vars[VZ] = VY;
+    vars[BX] = BZ; vars[BY] = BX; vars[BZ] = BY;
+  }
+}
+
+static rsolver_fn pick_rsolver(int id)
+{
+  switch (id) {
+  case OR_RS_HLLD: return or_hlld_flux;
+  case OR_RS_HLLT: return or_hllt_flux;
+  default: return or_hll_flux;
+  }
+}
+
+static double **alloc_rows(int nrow, int len)
+{
+  double **r = malloc(nrow * sizeof(double *));
+  double *buf = calloc((size_t)nrow * len, sizeof(double));
+  for (int n = 0; n < nrow; ++n) r[n] = buf + (size_t)n * len;
+  return r;
+}
+static void free_rows(double **r)
+{
+  free(r[0]);
+  free(r);
+}
+
+/* godunov_fluxes: src/integration/godunov.pyx:20-210 (D1 applied) */
+static void godunov_fluxes(or_sim *s, const double *prim, const double *bfld, const int *lims,
+                           int order)
+{
+  int il = lims[0], iu = lims[1], jl = lims[2], ju = lims[3], kl = lims[4], ku = lims[5];
+  int varsx[NMODE], varsy[NMODE], varsz[NMODE];
+  swap_mhd_vec(0, varsx);
+  swap_mhd_vec(1, varsy);
+  swap_mhd_vec(2, varsz);
+  rsolver_fn rs = pick_rsolver(s->rsolver);
+  reconstr_fn rc = (order == 0) ? or_reconstr_const : or_reconstr_linear;
+  int nx = s->Ntot[0];
+  int len = nx + 4;
+  int nst = 2 * order + 1;
+  double *pbase = (double *)prim;
+
+#pragma omp parallel num_threads(s->nthreads)
+  {
+    double **wl = alloc_rows(NMODE, len), **wl_ = alloc_rows(NMODE, len);
+    double **wr = alloc_rows(NMODE, len);
+    double **scrrows[3] = {alloc_rows(NMODE, len), alloc_rows(NMODE, len),
+                           alloc_rows(NMODE, len)};
+    double ***scr = scrrows;
+    double *prim1_rows[3][NMODE];
+    double **prim1[3] = {prim1_rows[0], prim1_rows[1], prim1_rows[2]};
+    double *flx1[NMODE];
+
+    /* x-direction, godunov.pyx:84-112 */
+    int kb = kl - 1, ke = ku + 1, jb = jl - 1, je = ju + 1;
+    if (s->dims < 2) jb = je = 0;
+    if (s->dims < 3) kb = ke = 0;
+#pragma omp for schedule(dynamic)
+    for (int k = kb; k <= ke; ++k)
+      for (int j = jb; j <= je; ++j) {
+        for (int d = 0; d < nst; ++d)
+          for (int n = 0; n < NMODE; ++n)
+            prim1[d][n] = pbase + IDX(s, varsx[n], k, j, 0) + d - order;
+        rc(wl, wr, prim1, scr, 0, il - 1, iu + 1);
+        for (int n = 0; n < NMODE; ++n) flx1[n] = s->flx[0] + IDX(s, varsx[n], k, j, 0);
+        rs(flx1, wl, wr, (double *)bfld + IDX(s, 0, k, j, 0), il, iu + 1, s->gam);
+      }
+
+    if (s->dims >= 2) {
+      /* y-direction, godunov.pyx:118-160 */
+      int ib = il - 1, ie = iu + 1;
+      kb = kl - 1;
+      ke = ku + 1;
+      if (s->dims < 3) kb = ke = 0;
+#pragma omp for schedule(dynamic)
+      for (int k = kb; k <= ke; ++k) {
+        double **wlc = wl, **wln = wl_;
+        for (int d = 0; d < nst; ++d)
+          for (int n = 0; n < NMODE; ++n)
+            prim1[d][n] = pbase + IDX(s, varsy[n], k, jl - 1 + d - order, 0);
+        rc(wlc, wr, prim1, scr, 1, ib, ie);
+        for (int j = jl; j <= ju + 1; ++j) {
+          for (int d = 0; d < nst; ++d)
+            for (int n = 0; n < NMODE; ++n)
+              prim1[d][n] = pbase + IDX(s, varsy[n], k, j + d - order, 0);
+          rc(wln, wr, prim1, scr, 1, ib, ie);
+          for (int n = 0; n < NMODE; ++n) flx1[n] = s->flx[1] + IDX(s, varsy[n], k, j, 0);
+          rs(flx1, wlc, wr, (double *)bfld + IDX(s, 1, k, j, 0), ib, ie, s->gam);
+          double **t = wlc; /* D1: the intended swap */
+          wlc = wln;
+          wln = t;
+        }
+      }
+    }
+
+    if (s->dims >= 3) {
+      /* z-direction, godunov.pyx:166-204 */
+      int ib = il - 1, ie = iu + 1;
+      jb = jl - 1;
+      je = ju + 1;
+#pragma omp for schedule(dynamic)
+      for (int j = jb; j <= je; ++j) {
+        double **wlc = wl, **wln = wl_;
+        for (int d = 0; d < nst; ++d)
+          for (int n = 0; n < NMODE; ++n)
+            prim1[d][n] = pbase + IDX(s, varsz[n], kl - 1 + d - order, j, 0);
+        rc(wlc, wr, prim1, scr, 2, ib, ie);
+        for (int k = kl; k <= ku + 1; ++k) {
+          for (int d = 0; d < nst; ++d)
+            for (int n = 0; n < NMODE; ++n)
+              prim1[d][n] = pbase + IDX(s, varsz[n], k + d - order, j, 0);
+          rc(wln, wr, prim1, scr, 2, ib, ie);
+          for (int n = 0; n < NMODE; ++n) flx1[n] = s->flx[2] + IDX(s, varsz[n], k, j, 0);
+          rs(flx1, wlc, wr, (double *)bfld + IDX(s, 2, k, j, 0), ib, ie, s->gam);
+          double **t = wlc;
+          wlc = wln;
+          wln = t;
+        }
+      }
+    }
+    free_rows(wl);
+    free_rows(wl_);
+    free_rows(wr);
+    for (int d = 0; d < 3; ++d) free_rows(scrrows[d]);
+  }
+}
+
+/* advance_hydro: src/integration/godunov.pyx:215-252 (Cartesian metric,
+   src/coordinates/cartesian.pyx:99-115) */
+static void advance_hydro(or_sim *s, double *u1, const double *u0, const int *lims, double dt)
+{
+  double *fx = s->flx[0], *fy = s->flx[1], *fz = s->flx[2];
+  double **dlf = s->dlf;
+  for (int n = 0; n < NMODE - 3; ++n)
+    for (int k = lims[4]; k <= lims[5]; ++k)
+      for (int j = lims[2]; j <= lims[3]; ++j)
+        for (int i = lims[0]; i <= lims[1]; ++i) {
+          double dtdv = dt / (dlf[0][i] * dlf[1][j] * dlf[2][k]);
+          double dam = dlf[1][j] * dlf[2][k];
+          double dap = dlf[1][j] * dlf[2][k];
+          long c = IDX(s, n, k, j, i);
+          u1[c] = u0[c] - dtdv * (dap * fx[IDX(s, n, k, j, i + 1)] - dam * fx[c]);
+          if (s->dims >= 2) {
+            dam = dlf[0][i] * dlf[2][k];
+            dap = dlf[0][i] * dlf[2][k];
+            u1[c] -= dtdv * (dap * fy[IDX(s, n, k, j + 1, i)] - dam * fy[c]);
+          }
+          if (s->dims >= 3) {
+            dam = dlf[0][i] * dlf[1][j];
+            dap = dlf[0][i] * dlf[1][j];
+            u1[c] -= dtdv * (dap * fz[IDX(s, n, k + 1, j, i)] - dam * fz[c]);
+          }
+        }
+}
+
+/* ec_from_prim: src/integration/ct.pyx:128-148 */
+static void ec_from_prim(or_sim *s, const double *w, const int *lims)
+{
+  int il = lims[0] - 1, iu = lims[1] + 1, jl = lims[2] - 1, ju = lims[3] + 1;
+  int kl = lims[4] - 1, ku = lims[5] + 1;
+  if (s->dims < 2) jl = ju = 0;
+  if (s->dims < 3) kl = ku = 0;
+  double *ec = s->ec;
+  for (int k = kl; k <= ku; ++k)
+    for (int j = jl; j <= ju; ++j)
+      for (int i = il; i <= iu; ++i) {
+        double vx = w[IDX(s, VX, k, j, i)], vy = w[IDX(s, VY, k, j, i)];
+        double vz = w[IDX(s, VZ, k, j, i)], bx = w[IDX(s, BX, k, j, i)];
+        double by = w[IDX(s, BY, k, j, i)], bz = w[IDX(s, BZ, k, j, i)];
+        ec[IDX(s, 0, k, j, i)] = -(vy * bz - vz * by);
+        ec[IDX(s, 1, k, j, i)] = (vx * bz - vz * bx);
+        ec[IDX(s, 2, k, j, i)] = -(vx * by - vy * bx);
+      }
+}
+
+/* interp_ee1: src/integration/ct.pyx:153-215 */
+static void interp_ee1(or_sim *s, const int *lims)
+{
+  int il = lims[0], iu = lims[1] + 1, jl = lims[2], ju = lims[3] + 1;
+  int kl = lims[4], ku = lims[5] + 1;
+  if (s->dims < 2) jl = ju = 0;
+  if (s->dims < 3) kl = ku = 0;
+  double *ee = s->ee, *ec = s->ec, *fx = s->flx[0], *fy = s->flx[1], *fz = s->flx[2];
+#define A(arr, n, k, j, i) arr[IDX(s, n, k, j, i)]
+  if (s->dims == 3) {
+    for (int k = kl; k <= ku; ++k)
+      for (int j = jl; j <= ju; ++j)
+        for (int i = il; i <= iu; ++i) {
+          A(ee, 0, k, j, i) = (0.5 * (-A(fy, BZ, k - 1, j, i) - A(fy, BZ, k, j, i) +
+                                      A(fz, BY, k, j - 1, i) + A(fz, BY, k, j, i)) -
+                               0.25 * (A(ec, 0, k, j - 1, i) + A(ec, 0, k, j, i) +
+                                       A(ec, 0, k - 1, j - 1, i) + A(ec, 0, k - 1, j, i)));
+          A(ee, 1, k, j, i) = (0.5 * (-A(fz, BX, k, j, i - 1) - A(fz, BX, k, j, i) +
+                                      A(fx, BZ, k - 1, j, i) + A(fx, BZ, k, j, i)) -
+                               0.25 * (A(ec, 1, k, j, i - 1) + A(ec, 1, k, j, i) +
+                                       A(ec, 1, k - 1, j, i - 1) + A(ec, 1, k - 1, j, i)));
+          A(ee, 2, k, j, i) = (0.5 * (-A(fx, BY, k, j - 1, i) - A(fx, BY, k, j, i) +
+                                      A(fy, BX, k, j, i - 1) + A(fy, BX, k, j, i)) -
+                               0.25 * (A(ec, 2, k, j, i - 1) + A(ec, 2, k, j, i) +
+                                       A(ec, 2, k, j - 1, i - 1) + A(ec, 2, k, j - 1, i)));
+        }
+  } else if (s->dims == 2) {
+    for (int j = jl; j <= ju; ++j)
+      for (int i = il; i <= iu; ++i) {
+        A(ee, 0, 0, j, i) = -A(fy, BZ, 0, j, i);
+        A(ee, 1, 0, j, i) = A(fx, BZ, 0, j, i);
+        A(ee, 2, 0, j, i) = (0.5 * (-A(fx, BY, 0, j - 1, i) - A(fx, BY, 0, j, i) +
+                                    A(fy, BX, 0, j, i - 1) + A(fy, BX, 0, j, i)) -
+                             0.25 * (A(ec, 2, 0, j, i - 1) + A(ec, 2, 0, j, i) +
+                                     A(ec, 2, 0, j - 1, i - 1) + A(ec, 2, 0, j - 1, i)));
+      }
+  } else {
+    for (int i = il; i <= iu; ++i) {
+      A(ee, 0, 0, 0, i) = -A(ec, 0, 0, 0, i);
+      A(ee, 1, 0, 0, i) = A(fx, BZ, 0, 0, i);
+      A(ee, 2, 0, 0, i) = -A(fx, BY, 0, 0, i);
+    }
+  }
+}
+
+/* advance_b: src/integration/ct.pyx:15-91 (D11: By over j in [jl,ju+1];
+   2D Bz only at k=0).  Edge lengths and face areas per
+   src/coordinates/cartesian.pyx:57-109. */
+static void advance_b(or_sim *s, double *b1, const double *b0, const int *lims, double dt)
+{
+  double **dlf = s->dlf;
+  double *ee = s->ee;
+  int kz_hi = (s->dims >= 3) ? lims[5] + 1 : lims[5];
+  int jy_hi = (s->dims >= 2) ? lims[3] + 1 : lims[3];
+  for (int k = lims[4]; k <= lims[5]; ++k)
+    for (int j = lims[2]; j <= lims[3]; ++j)
+      for (int i = lims[0]; i <= lims[1] + 1; ++i) {
+        double dtda = dt / (dlf[1][j] * dlf[2][k]);
+        long c = IDX(s, 0, k, j, i);
+        b1[c] = b0[c];
+        if (s->dims >= 2) {
+          double dsm = dlf[2][k], dsp = dlf[2][k];
+          b1[c] -= dtda * (dsp * A(ee, 2, k, j + 1, i) - dsm * A(ee, 2, k, j, i));
+        }
+        if (s->dims >= 3) {
+          double dsm = dlf[1][j], dsp = dlf[1][j];
+          b1[c] += dtda * (dsp * A(ee, 1, k + 1, j, i) - dsm * A(ee, 1, k, j, i));
+        }
+      }
+  for (int k = lims[4]; k <= lims[5]; ++k)
+    for (int j = lims[2]; j <= jy_hi; ++j)
+      for (int i = lims[0]; i <= lims[1]; ++i) {
+        double dtda = dt / (dlf[0][i] * dlf[2][k]);
+        double dsm = dlf[2][k], dsp = dlf[2][k];
+        long c = IDX(s, 1, k, j, i);
+        b1[c] = b0[c] + dtda * (dsp * A(ee, 2, k, j, i + 1) - dsm * A(ee, 2, k, j, i));
+        if (s->dims >= 3) {
+          dsm = dlf[0][i];
+          dsp = dlf[0][i];
+          b1[c] -= dtda * (dsp * A(ee, 0, k + 1, j, i) - dsm * A(ee, 0, k, j, i));
+        }
+      }
+  for (int k = lims[4]; k <= kz_hi; ++k)
+    for (int j = lims[2]; j <= lims[3]; ++j)
+      for (int i = lims[0]; i <= lims[1]; ++i) {
+        double dtda = dt / (dlf[0][i] * dlf[1][j]);
+        double dsm = dlf[1][j], dsp = dlf[1][j];
+        long c = IDX(s, 2, k, j, i);
+        b1[c] = b0[c] - dtda * (dsp * A(ee, 1, k, j, i + 1) - dsm * A(ee, 1, k, j, i));
+        if (s->dims >= 2) {
+          dsm = dlf[0][i];
+          dsp = dlf[0][i];
+          b1[c] += dtda * (dsp * A(ee, 0, k, j + 1, i) - dsm * A(ee, 0, k, j, i));
+        }
+      }
+}
+
+/* interp_bc: src/integration/ct.pyx:96-123 */
+static void interp_bc(or_sim *s, double *u, const double *b, const int *lims)
+{
+  double **lf = s->lf, **lv = s->lv, **dlfi = s->dlf_inv;
+  for (int k = lims[4]; k <= lims[5]; ++k)
+    for (int j = lims[2]; j <= lims[3]; ++j)
+      for (int i = lims[0]; i <= lims[1]; ++i) {
+        A(u, BX, k, j, i) = ((lv[0][i] - lf[0][i]) * A(b, 0, k, j, i) +
+                             (lf[0][i + 1] - lv[0][i]) * A(b, 0, k, j, i + 1)) * dlfi[0][i];
+        if (s->dims >= 2)
+          A(u, BY, k, j, i) = ((lv[1][j] - lf[1][j]) * A(b, 1, k, j, i) +
+                               (lf[1][j + 1] - lv[1][j]) * A(b, 1, k, j + 1, i)) * dlfi[1][j];
+        else
+          A(u, BY, k, j, i) = A(b, 1, k, j, i);
+        if (s->dims >= 3)
+          A(u, BZ, k, j, i) = ((lv[2][k] - lf[2][k]) * A(b, 2, k, j, i) +
+                               (lf[2][k + 1] - lv[2][k]) * A(b, 2, k + 1, j, i)) * dlfi[2][k];
+        else
+          A(u, BZ, k, j, i) = A(b, 2, k, j, i);
+      }
+}
+
+/* src/mhd/eos.pyx:13-78 with src/mhd/eos_c.c:9-48 */
+static void cons2prim_lims(or_sim *s, double *w, const double *u, const int *lims)
+{
+  double uu[NMODE], ww[NMODE];
+  for (int k = lims[4]; k <= lims[5]; ++k)
+    for (int j = lims[2]; j <= lims[3]; ++j)
+      for (int i = lims[0]; i <= lims[1]; ++i) {
+        for (int n = 0; n < NMODE; ++n) uu[n] = A(u, n, k, j, i);
+        or_cons2prim(ww, uu, s->gam);
+        for (int n = 0; n < NMODE; ++n) A(w, n, k, j, i) = ww[n];
+      }
+}
+
+static void prim2cons_lims(or_sim *s, double *u, const double *w, const int *lims)
+{
+  double uu[NMODE], ww[NMODE];
+  for (int k = lims[4]; k <= lims[5]; ++k)
+    for (int j = lims[2]; j <= lims[3]; ++j)
+      for (int i = lims[0]; i <= lims[1]; ++i) {
+        for (int n = 0; n < NMODE; ++n) ww[n] = A(w, n, k, j, i);
+        or_prim2cons(uu, ww, s->gam);
+        for (int n = 0; n < NMODE; ++n) A(u, n, k, j, i) = uu[n];
+      }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Ghost fill: src/bc/grid_bc.pyx:612-623 (z, y, x; left then right),        */
+/* periodic copy_layer_* and outflow prolong_* from src/bc/util_bc.pyx.      */
+/* ------------------------------------------------------------------------ */
+
+static void bc_axis(or_sim *s, double *A3, int ax, int side)
+{
+  int ng = s->ng;
+  int nx = s->Ntot[0], ny = s->Ntot[1], nz = s->Ntot[2];
+  int a1 = (ax == 0) ? s->i1 : (ax == 1) ? s->j1 : s->k1;
+  int a2 = (ax == 0) ? s->i2 : (ax == 1) ? s->j2 : s->k2;
+  int flag = s->bc[ax][side];
+  int dst0, src0, width = ng, single_src = 0;
+  if (flag == 0) { /* grid_bc_funcs.pyx:36-105 */
+    if (side == 0) { dst0 = a1 - ng; src0 = a2 - ng + 1; }
+    else { dst0 = a2 + 1; src0 = a1; }
+  } else { /* outflow, grid_bc_funcs.pyx:112-196, util_bc.pyx:208-249 (D14) */
+    single_src = 1;
+    if (side == 0) { dst0 = a1 - ng; src0 = a1; }
+    else { dst0 = a2 + 1; src0 = a2; }
+  }
+#define AT(k, j, i) A3[((long)(k) * ny + (j)) * nx + (i)]
+  for (int g = 0; g < width; ++g) {
+    int d = dst0 + g, sidx = single_src ? src0 : src0 + g;
+    if (ax == 0) {
+      for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j) AT(k, j, d) = AT(k, j, sidx);
+    } else if (ax == 1) {
+      for (int k = 0; k < nz; ++k)
+        for (int i = 0; i < nx; ++i) AT(k, d, i) = AT(k, sidx, i);
+    } else {
+      for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) AT(d, j, i) = AT(sidx, j, i);
+    }
+  }
+#undef AT
+}
+
+static void apply_bc_arrays(or_sim *s, double *base, int narr)
+{
+  for (int n = 0; n < narr; ++n) {
+    double *A3 = base + (long)n * s->ncell;
+    if (s->dims >= 3) { bc_axis(s, A3, 2, 0); bc_axis(s, A3, 2, 1); }
+    if (s->dims >= 2) { bc_axis(s, A3, 1, 0); bc_axis(s, A3, 1, 1); }
+    bc_axis(s, A3, 0, 0);
+    bc_axis(s, A3, 0, 1);
+  }
+}
+
+/* apply_grid_bc over all NVAR: prim[0..8] and bfld[0..2]
+   (src/bc/grid_bc_funcs.pyx:10-29).  The per-variable ordering of the
+   reference (all variables per side) gives the same result because the
+   copies of different variables are independent. */
+void or_apply_grid_bc(or_sim *s)
+{
+  apply_bc_arrays(s, s->prim, NMODE);
+  apply_bc_arrays(s, s->bfld, 3);
+}
+
+/* init_sim after set_problem: src/integration/vanleer.pyx:38-48 */
+void or_init(or_sim *s)
+{
+  int lims_act[6] = {s->i1, s->i2, s->j1, s->j2, s->k1, s->k2};
+  int lims_tot[6] = {0, s->Ntot[0] - 1, 0, s->Ntot[1] - 1, 0, s->Ntot[2] - 1};
+  or_apply_grid_bc(s);
+  interp_bc(s, s->prim, s->bfld, lims_act);
+  apply_bc_arrays(s, s->prim + (long)BX * s->ncell, 3);
+  prim2cons_lims(s, s->cons, s->prim, lims_tot);
+  s->time = 0.;
+  s->step = 0;
+}
+
+/* new_dt: src/integration/new_dt.pyx:18-135 (Cartesian widths) */
+double or_new_dt(or_sim *s)
+{
+  double cdsmax = 0.;
+  double w[NMODE];
+  for (int k = s->k1; k <= s->k2; ++k) {
+    double cx = 0., cy = 0., cz = 0.;
+    for (int j = s->j1; j <= s->j2; ++j)
+      for (int i = s->i1; i <= s->i2; ++i) {
+        for (int n = 0; n < NMODE; ++n) w[n] = A(s->prim, n, k, j, i);
+        double bx = w[BX], by = w[BY], bz = w[BZ];
+        double wy[NMODE], wz[NMODE];
+        const double *py = w, *pz = w;
+        if (!s->dt_fms_ref) {
+          /* corrected mode: fast speed along each axis */
+          memcpy(wy, w, sizeof(w));
+          memcpy(wz, w, sizeof(w));
+          wy[BX] = by; wy[BY] = bx;
+          wz[BX] = bz; wz[BZ] = bx;
+          py = wy;
+          pz = wz;
+        }
+        double vfx = or_fms(w, bx, s->gam);
+        cx = fabs(w[VX]) + vfx;
+        if (s->dims >= 2) {
+          double vfy = or_fms(py, by, s->gam);
+          cy = fabs(w[VY]) + vfy;
+        }
+        if (s->dims >= 3) {
+          double vfz = or_fms(pz, bz, s->gam);
+          cz = fabs(w[VZ]) + vfz;
+        }
+        double dsx = s->dlf[0][i], dsy = s->dlf[1][j], dsz = s->dlf[2][k];
+        double cds = fmax(fmax(cx / dsx, cy / dsy), cz / dsz);
+        if (cds > cdsmax) cdsmax = cds;
+      }
+  }
+  return s->cour / cdsmax;
+}
+
+/* One VL step with a given dt: src/integration/vanleer.pyx:100-163 */
+void or_vl_step(or_sim *s, double dt_full)
+{
+  int lims[6];
+  int lims_tot[6] = {0, s->Ntot[0] - 1, 0, s->Ntot[1] - 1, 0, s->Ntot[2] - 1};
+  double *w = s->prim, *u = s->cons, *b = s->bfld;
+  double *ws = s->prim, *us = s->cons_s, *bs = s->bfld_s;
+
+  /* predictor */
+  int order = 0;
+  double dt = 0.5 * dt_full;
+  integr_lims(s, order, lims);
+  godunov_fluxes(s, w, b, lims, order);
+  advance_hydro(s, us, u, lims, dt);
+  ec_from_prim(s, w, lims);
+  interp_ee1(s, lims);
+  advance_b(s, bs, b, lims, dt);
+  interp_bc(s, us, bs, lims);
+  cons2prim_lims(s, ws, us, lims);
+
+  /* corrector */
+  order = s->reconstr_order;
+  dt = dt_full;
+  integr_lims(s, order, lims);
+  godunov_fluxes(s, ws, bs, lims, order);
+  advance_hydro(s, u, u, lims, dt);
+  ec_from_prim(s, ws, lims);
+  interp_ee1(s, lims);
+  advance_b(s, b, b, lims, dt);
+  interp_bc(s, u, b, lims);
+  cons2prim_lims(s, w, u, lims);
+
+  or_apply_grid_bc(s);
+  prim2cons_lims(s, u, w, lims_tot);
+}
+
+/* vanleer.advance loop: src/integration/vanleer.pyx:87-167 */
+int or_advance(or_sim *s, double tmax, int max_steps)
+{
+  int n = 0;
+  while (s->time < tmax && (max_steps <= 0 || n < max_steps)) {
+    double dt = or_new_dt(s);
+    if (s->time + dt > tmax) dt = tmax - s->time;
+    s->dt = dt;
+    or_vl_step(s, dt);
+    s->time += dt;
+    s->step += 1;
+    ++n;
+  }
+  return n;
+}
+
+int or_advance_steps(or_sim *s, int nsteps)
+{
+  for (int n = 0; n < nsteps; ++n) {
+    double dt = or_new_dt(s);
+    s->dt = dt;
+    or_vl_step(s, dt);
+    s->time += dt;
+    s->step += 1;
+  }
+  return nsteps;
+}
+#undef A
