@@ -1,0 +1,225 @@
+"""ctypes binding of libbnzgpu.so (declarations in include/bnzgpu.h).
+
+The library is built in-tree by ``__graft_entry__.build()`` (or
+``make -C bonzo_amd/csrc``).  There is no fallback: if the shared object is
+missing or cannot be loaded, every entry point raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libbnzgpu.so")
+
+BNZ_OK, BNZ_EINVAL, BNZ_ENOMEM, BNZ_EHIP, BNZ_EUNSUPPORTED, BNZ_ESTATE = 0, -1, -2, -3, -4, -5
+RS = {"hll": 0, "hllc": 1, "hlld": 2, "hlla": 3, "hllt": 4}
+RCN = {"const": 0, "linear": 1, "weno": 2, "parab": 3}
+PREC = {"fp64": 0}
+
+ABI_SYMBOLS = (
+    "bnz_create", "bnz_destroy", "bnz_last_error", "bnz_get_shape", "bnz_upload",
+    "bnz_download", "bnz_init_state", "bnz_fill_ghosts", "bnz_new_dt", "bnz_vl_step",
+    "bnz_advance", "bnz_run_steps", "bnz_get_time", "bnz_set_time", "bnz_diag",
+    "bnz_synchronize", "bnz_set_kernel_timing", "bnz_kernel_timing", "bnz_gpu_flux_pencils",
+)
+
+
+class BnzParams(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+        ("rsolver", ctypes.c_int), ("reconstr", ctypes.c_int),
+        ("bc", ctypes.c_int * 6),
+        ("lmin", ctypes.c_double * 3), ("lmax", ctypes.c_double * 3),
+        ("gam", ctypes.c_double), ("cour", ctypes.c_double),
+        ("dt_fms_ref", ctypes.c_int), ("precision", ctypes.c_int), ("device", ctypes.c_int),
+    ]
+
+
+class BnzDiag(ctypes.Structure):
+    _fields_ = [
+        ("mass", ctypes.c_double), ("mom", ctypes.c_double * 3), ("energy", ctypes.c_double),
+        ("scalar", ctypes.c_double), ("divb_max", ctypes.c_double), ("bmax", ctypes.c_double),
+        ("pmin", ctypes.c_double), ("rhomin", ctypes.c_double), ("nonfinite", ctypes.c_long),
+    ]
+
+
+class BnzError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"bnz error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def load():
+    """Load libbnzgpu.so; raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build() "
+                           "or make -C bonzo_amd/csrc")
+    L = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    L.bnz_create.argtypes = [ctypes.POINTER(BnzParams), ctypes.POINTER(vp)]
+    L.bnz_destroy.argtypes = [vp]
+    L.bnz_last_error.restype = ctypes.c_char_p
+    L.bnz_last_error.argtypes = [vp]
+    L.bnz_get_shape.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+    L.bnz_upload.argtypes = [vp, _dp, _dp]
+    L.bnz_download.argtypes = [vp, _dp, _dp, _dp]
+    for name in ("bnz_init_state", "bnz_fill_ghosts", "bnz_synchronize"):
+        getattr(L, name).argtypes = [vp]
+    L.bnz_new_dt.argtypes = [vp, _dp]
+    L.bnz_vl_step.argtypes = [vp, ctypes.c_double]
+    L.bnz_advance.argtypes = [vp, ctypes.c_double, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    L.bnz_run_steps.argtypes = [vp, ctypes.c_int]
+    L.bnz_get_time.argtypes = [vp, _dp, ctypes.POINTER(ctypes.c_int), _dp]
+    L.bnz_set_time.argtypes = [vp, ctypes.c_double, ctypes.c_int]
+    L.bnz_diag.argtypes = [vp, ctypes.POINTER(BnzDiag)]
+    L.bnz_set_kernel_timing.argtypes = [vp, ctypes.c_int]
+    L.bnz_kernel_timing.argtypes = [vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_int, _dp,
+                                    ctypes.POINTER(ctypes.c_long)]
+    L.bnz_gpu_flux_pencils.argtypes = [ctypes.c_int, vp, vp, vp, vp, ctypes.c_long,
+                                       ctypes.c_double]
+    for name in ABI_SYMBOLS:
+        if name not in ("bnz_last_error",):
+            getattr(L, name).restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+class Context:
+    """One device grid: owns a bnz_ctx (device arrays, stream)."""
+
+    def __init__(self, nx, ny=1, nz=1, rsolver="hlld", reconstr="linear", bc=(0,) * 6,
+                 lmin=(0.0, 0.0, 0.0), lmax=(1.0, 1.0, 1.0), gam=5.0 / 3.0, cour=0.4,
+                 dt_fms_ref=True, precision="fp64", device=0):
+        L = load()
+        p = BnzParams()
+        p.nx, p.ny, p.nz = nx, ny, nz
+        p.rsolver = RS[rsolver] if isinstance(rsolver, str) else int(rsolver)
+        p.reconstr = RCN[reconstr] if isinstance(reconstr, str) else int(reconstr)
+        for q in range(6):
+            p.bc[q] = int(bc[q])
+        for q in range(3):
+            p.lmin[q] = lmin[q]
+            p.lmax[q] = lmax[q]
+        p.gam, p.cour = gam, cour
+        p.dt_fms_ref = int(bool(dt_fms_ref))
+        p.precision = PREC[precision]
+        p.device = device
+        h = ctypes.c_void_p()
+        rc = L.bnz_create(ctypes.byref(p), ctypes.byref(h))
+        if rc != BNZ_OK:
+            msg = L.bnz_last_error(h).decode() if h.value else "bnz_create failed"
+            if h.value:
+                L.bnz_destroy(h)
+            raise BnzError(rc, msg)
+        self._h = h
+        nt = (ctypes.c_int * 3)()
+        ng = ctypes.c_int()
+        self._check(L.bnz_get_shape(h, nt, ctypes.byref(ng)))
+        self.ntot = (nt[0], nt[1], nt[2])
+        self.ng = ng.value
+
+    def _check(self, rc):
+        if rc != BNZ_OK:
+            raise BnzError(rc, load().bnz_last_error(self._h).decode())
+
+    @property
+    def shape(self):
+        return (self.ntot[2], self.ntot[1], self.ntot[0])
+
+    def upload(self, prim, bfld):
+        prim = np.ascontiguousarray(prim, dtype=np.float64)
+        bfld = np.ascontiguousarray(bfld, dtype=np.float64)
+        assert prim.shape == (9,) + self.shape and bfld.shape == (3,) + self.shape
+        self._check(load().bnz_upload(self._h, _ptr(prim), _ptr(bfld)))
+
+    def download(self, cons=False):
+        prim = np.empty((9,) + self.shape)
+        bfld = np.empty((3,) + self.shape)
+        c = np.empty((9,) + self.shape) if cons else None
+        self._check(load().bnz_download(self._h, _ptr(prim), _ptr(bfld), _ptr(c)))
+        return (prim, bfld, c) if cons else (prim, bfld)
+
+    def init_state(self):
+        self._check(load().bnz_init_state(self._h))
+
+    def fill_ghosts(self):
+        self._check(load().bnz_fill_ghosts(self._h))
+
+    def new_dt(self):
+        d = ctypes.c_double()
+        self._check(load().bnz_new_dt(self._h, ctypes.byref(d)))
+        return d.value
+
+    def vl_step(self, dt):
+        self._check(load().bnz_vl_step(self._h, dt))
+
+    def advance(self, tmax, max_steps=0):
+        n = ctypes.c_int()
+        self._check(load().bnz_advance(self._h, tmax, max_steps, ctypes.byref(n)))
+        return n.value
+
+    def run_steps(self, n):
+        self._check(load().bnz_run_steps(self._h, n))
+
+    def synchronize(self):
+        self._check(load().bnz_synchronize(self._h))
+
+    def time(self):
+        t = ctypes.c_double()
+        s = ctypes.c_int()
+        d = ctypes.c_double()
+        self._check(load().bnz_get_time(self._h, ctypes.byref(t), ctypes.byref(s), ctypes.byref(d)))
+        return t.value, s.value, d.value
+
+    def set_time(self, t, step):
+        self._check(load().bnz_set_time(self._h, t, step))
+
+    def diag(self):
+        d = BnzDiag()
+        self._check(load().bnz_diag(self._h, ctypes.byref(d)))
+        return dict(mass=d.mass, mom=tuple(d.mom), energy=d.energy, scalar=d.scalar,
+                    divb_max=d.divb_max, bmax=d.bmax, pmin=d.pmin, rhomin=d.rhomin,
+                    nonfinite=d.nonfinite)
+
+    def set_kernel_timing(self, on):
+        self._check(load().bnz_set_kernel_timing(self._h, int(bool(on))))
+
+    def kernel_timing(self):
+        out = {}
+        q = 0
+        name = ctypes.create_string_buffer(64)
+        while True:
+            tot = ctypes.c_double()
+            n = ctypes.c_long()
+            rc = load().bnz_kernel_timing(self._h, q, name, 64, ctypes.byref(tot), ctypes.byref(n))
+            if rc != BNZ_OK:
+                break
+            out[name.value.decode()] = (tot.value, n.value)
+            q += 1
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            load().bnz_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

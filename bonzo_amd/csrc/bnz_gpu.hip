@@ -1,0 +1,1148 @@
+// bnz_gpu.hip -- MI355X (gfx950) kernels and C ABI of the Van-Leer MHD step.
+//
+// One VL step (src/integration/vanleer.pyx:87-167) is a chain of kernels on
+// one stream:
+//   dt_partial, dt_final                CFL dt, device resident
+//   predictor: flux_x, flux_y, flux_z, emf, update<pred>   (donor cell, dt/2)
+//   corrector: flux_x, flux_y, flux_z, emf, update<corr>   (PLM, dt)
+//   ghost, time_advance
+// Grid arrays keep the reference's SoA layout [var][k][j][i] in HBM so every
+// wavefront reads 64 consecutive doubles of one x-row.  The persistent state
+// is prim (9) + face B (3); conserved variables are recomputed from prim
+// where the reference reads its cons array (u_n = prim2cons(w_n) on all
+// cells, vanleer.pyx:163).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/bnzgpu.h"
+#include "bnz_phys.hpp"
+
+using namespace bnz;
+
+namespace {
+
+constexpr int NVAR_STATE = 9;
+constexpr int BLK = 256;
+
+// --------------------------------------------------------------------------
+// Geometry passed by value to every kernel.
+struct Geo {
+  int nt[3];       // Ntot per axis
+  int dims;
+  long sv;         // stride between variables (cells per array)
+  int sy, sz;      // strides of j and k
+  const double *dlf[3];   // cell widths (coord.pyx:254-257)
+  const double *dlfi[3];  // 1/width
+  const double *icm[3];   // lv - lf       (interp_bc weights, ct.pyx:110-123)
+  const double *icp[3];   // lf[i+1] - lv
+  double gam, gamm1i;
+};
+
+struct Lims {
+  int lo[3], hi[3];  // inclusive cell limits of get_integr_lims (integrator.pyx:375-383)
+};
+
+struct Box {
+  int lo[3];
+  int n[3];
+  __host__ __device__ long total() const { return (long)n[0] * n[1] * n[2]; }
+};
+
+__device__ __forceinline__ void box_idx(const Box &b, long t, int &i, int &j, int &k) {
+  int nx = b.n[0], ny = b.n[1];
+  i = b.lo[0] + (int)(t % nx);
+  long r = t / nx;
+  j = b.lo[1] + (int)(r % ny);
+  k = b.lo[2] + (int)(r / ny);
+}
+
+// Global variable read by local slot n of a sweep along DIR
+// (swap_mhd_vec, src/integration/godunov.pyx:297-324).
+template <int DIR>
+__device__ __forceinline__ constexpr int perm(int n) {
+  if (DIR == 0) return n;
+  if (DIR == 1) {
+    switch (n) {
+      case VX: return VY;
+      case VY: return VZ;
+      case VZ: return VX;
+      case BX: return BY;
+      case BY: return BZ;
+      case BZ: return BX;
+      default: return n;
+    }
+  }
+  switch (n) {
+    case VX: return VZ;
+    case VY: return VX;
+    case VZ: return VY;
+    case BX: return BZ;
+    case BY: return BX;
+    case BZ: return BY;
+    default: return n;
+  }
+}
+
+// --------------------------------------------------------------------------
+// Godunov fluxes of one direction (godunov.pyx:20-210 with the double-buffered
+// left state of D1).  One thread per face; face index c is the lower face of
+// cell c, between cells c-1 and c.  The flux is stored in global variable
+// order, flux[n][c], except the normal-field slot, which no consumer reads.
+template <int DIR, int ORDER, int RS>
+__global__ __launch_bounds__(BLK) void k_flux(Geo g, const double *__restrict__ w,
+                                              const double *__restrict__ bf,
+                                              double *__restrict__ fl, Box box) {
+  long t = (long)blockIdx.x * BLK + threadIdx.x;
+  if (t >= box.total()) return;
+  int i, j, k;
+  box_idx(box, t, i, j, k);
+  const long c = ((long)k * g.nt[1] + j) * g.nt[0] + i;
+  const long sd = (DIR == 0) ? 1 : (DIR == 1) ? g.sy : g.sz;
+  const double bn = bf[(long)DIR * g.sv + c];
+
+  double wl[NMODE], wr[NMODE];
+#pragma unroll
+  for (int n = 0; n < NMODE; ++n) {
+    if (n == BX) continue;
+    const double *p = w + (long)perm<DIR>(n) * g.sv + c;
+    if (ORDER == 0) {
+      wl[n] = p[-sd];
+      wr[n] = p[0];
+    } else {
+      const double wm2 = p[-2 * sd], wm1 = p[-sd], w0 = p[0], wp1 = p[sd];
+      // cell c-1, upper state; cell c, lower state (reconstr_c.c:54-89)
+      const double dl = 0.5 * vl_lim(wm1 - wm2, w0 - wm1);
+      const double dr = 0.5 * vl_lim(w0 - wm1, wp1 - w0);
+      wl[n] = wm1 + dl;
+      wr[n] = w0 - dr;
+    }
+  }
+  wl[BX] = bn;
+  wr[BX] = bn;
+
+  double f[NMODE];
+  int reg;
+  if (RS == BNZ_RS_HLLD)
+    reg = flux_hlld(f, wl, wr, g.gam, g.gamm1i);
+  else
+    reg = flux_hll(f, wl, wr, g.gam, g.gamm1i);
+  if (reg == 0) return;  // reference leaves the flux row untouched
+#pragma unroll
+  for (int n = 0; n < NMODE; ++n) {
+    if (n == BX) continue;
+    fl[(long)perm<DIR>(n) * g.sv + c] = f[n];
+  }
+}
+
+// --------------------------------------------------------------------------
+// Edge EMFs, interp_ee1 (src/integration/ct.pyx:153-215) with the cell-centred
+// E = -v x B of ec_from_prim (ct.pyx:128-148) computed on the fly.
+__device__ __forceinline__ double ec_comp(const double *w, long sv, long c, int comp) {
+  const double vx = w[VX * sv + c], vy = w[VY * sv + c], vz = w[VZ * sv + c];
+  const double bx = w[BX * sv + c], by = w[BY * sv + c], bz = w[BZ * sv + c];
+  if (comp == 0) return -(vy * bz - vz * by);
+  if (comp == 1) return (vx * bz - vz * bx);
+  return -(vx * by - vy * bx);
+}
+
+__global__ __launch_bounds__(BLK) void k_emf(Geo g, const double *__restrict__ w,
+                                             const double *__restrict__ fx,
+                                             const double *__restrict__ fy,
+                                             const double *__restrict__ fz,
+                                             double *__restrict__ ee, Box box) {
+  long t = (long)blockIdx.x * BLK + threadIdx.x;
+  if (t >= box.total()) return;
+  int i, j, k;
+  box_idx(box, t, i, j, k);
+  const long sv = g.sv;
+  const long c = ((long)k * g.nt[1] + j) * g.nt[0] + i;
+  const long sy = g.sy, sz = g.sz;
+  if (g.dims == 3) {
+    ee[0 * sv + c] = (0.5 * (-fy[BZ * sv + c - sz] - fy[BZ * sv + c] + fz[BY * sv + c - sy] +
+                             fz[BY * sv + c]) -
+                      0.25 * (ec_comp(w, sv, c - sy, 0) + ec_comp(w, sv, c, 0) +
+                              ec_comp(w, sv, c - sz - sy, 0) + ec_comp(w, sv, c - sz, 0)));
+    ee[1 * sv + c] = (0.5 * (-fz[BX * sv + c - 1] - fz[BX * sv + c] + fx[BZ * sv + c - sz] +
+                             fx[BZ * sv + c]) -
+                      0.25 * (ec_comp(w, sv, c - 1, 1) + ec_comp(w, sv, c, 1) +
+                              ec_comp(w, sv, c - sz - 1, 1) + ec_comp(w, sv, c - sz, 1)));
+    ee[2 * sv + c] = (0.5 * (-fx[BY * sv + c - sy] - fx[BY * sv + c] + fy[BX * sv + c - 1] +
+                             fy[BX * sv + c]) -
+                      0.25 * (ec_comp(w, sv, c - 1, 2) + ec_comp(w, sv, c, 2) +
+                              ec_comp(w, sv, c - sy - 1, 2) + ec_comp(w, sv, c - sy, 2)));
+  } else if (g.dims == 2) {
+    ee[0 * sv + c] = -fy[BZ * sv + c];
+    ee[1 * sv + c] = fx[BZ * sv + c];
+    ee[2 * sv + c] = (0.5 * (-fx[BY * sv + c - sy] - fx[BY * sv + c] + fy[BX * sv + c - 1] +
+                             fy[BX * sv + c]) -
+                      0.25 * (ec_comp(w, sv, c - 1, 2) + ec_comp(w, sv, c, 2) +
+                              ec_comp(w, sv, c - sy - 1, 2) + ec_comp(w, sv, c - sy, 2)));
+  } else {
+    ee[0 * sv + c] = -ec_comp(w, sv, c, 0);
+    ee[1 * sv + c] = fx[BZ * sv + c];
+    ee[2 * sv + c] = -fx[BY * sv + c];
+  }
+}
+
+// --------------------------------------------------------------------------
+// Face-field update by Stokes (advance_b, ct.pyx:15-91) for one face.
+__device__ __forceinline__ double bface_x(const Geo &g, const double *b0, const double *ee,
+                                          double dt, int i, int j, int k, long c) {
+  const long sv = g.sv;
+  const double dtda = dt / (g.dlf[1][j] * g.dlf[2][k]);
+  double v = b0[c];
+  if (g.dims >= 2) {
+    const double ds = g.dlf[2][k];
+    v -= dtda * (ds * ee[2 * sv + c + g.sy] - ds * ee[2 * sv + c]);
+  }
+  if (g.dims >= 3) {
+    const double ds = g.dlf[1][j];
+    v += dtda * (ds * ee[1 * sv + c + g.sz] - ds * ee[1 * sv + c]);
+  }
+  return v;
+}
+
+__device__ __forceinline__ double bface_y(const Geo &g, const double *b0, const double *ee,
+                                          double dt, int i, int j, int k, long c) {
+  const long sv = g.sv;
+  const double dtda = dt / (g.dlf[0][i] * g.dlf[2][k]);
+  double ds = g.dlf[2][k];
+  double v = b0[1 * sv + c] + dtda * (ds * ee[2 * sv + c + 1] - ds * ee[2 * sv + c]);
+  if (g.dims >= 3) {
+    ds = g.dlf[0][i];
+    v -= dtda * (ds * ee[0 * sv + c + g.sz] - ds * ee[0 * sv + c]);
+  }
+  return v;
+}
+
+__device__ __forceinline__ double bface_z(const Geo &g, const double *b0, const double *ee,
+                                          double dt, int i, int j, int k, long c) {
+  const long sv = g.sv;
+  const double dtda = dt / (g.dlf[0][i] * g.dlf[1][j]);
+  double ds = g.dlf[1][j];
+  double v = b0[2 * sv + c] - dtda * (ds * ee[1 * sv + c + 1] - ds * ee[1 * sv + c]);
+  if (g.dims >= 2) {
+    ds = g.dlf[0][i];
+    v += dtda * (ds * ee[0 * sv + c + g.sy] - ds * ee[0 * sv + c]);
+  }
+  return v;
+}
+
+// Stage update: advance_hydro (godunov.pyx:215-252), advance_b, interp_bc
+// (ct.pyx:96-123) and cons2prim (eos.pyx:13-41) fused, one thread per cell
+// of the whole grid.  u0 = prim2cons(w0) is the reference's cons array.
+// PRED: writes ws (a separate buffer); cells outside the limits take w0,
+// as the reference's ws aliases prim (vanleer.pyx:72).  Corrector: w1 may
+// alias w0 (each thread reads only its own cell of w0).
+template <bool PRED>
+__global__ __launch_bounds__(BLK) void k_update(Geo g, const double *w0, const double *__restrict__ fx,
+                                                const double *__restrict__ fy,
+                                                const double *__restrict__ fz,
+                                                const double *__restrict__ ee,
+                                                const double *__restrict__ b0,
+                                                double *__restrict__ b1, double *w1, Lims L,
+                                                const double *__restrict__ dts) {
+  long t = (long)blockIdx.x * BLK + threadIdx.x;
+  const long ncell = (long)g.nt[0] * g.nt[1] * g.nt[2];
+  if (t >= ncell) return;
+  const int i = (int)(t % g.nt[0]);
+  const long r = t / g.nt[0];
+  const int j = (int)(r % g.nt[1]);
+  const int k = (int)(r / g.nt[1]);
+  const long c = t;
+  const long sv = g.sv;
+  const double dt = PRED ? 0.5 * dts[0] : dts[0];
+
+  const bool inI = i >= L.lo[0] && i <= L.hi[0];
+  const bool inJ = j >= L.lo[1] && j <= L.hi[1];
+  const bool inK = k >= L.lo[2] && k <= L.hi[2];
+  const bool cell = inI && inJ && inK;
+  const bool fX = (i >= L.lo[0] && i <= L.hi[0] + 1) && inJ && inK;
+  const bool fY = inI && (j >= L.lo[1] && j <= L.hi[1] + (g.dims >= 2 ? 1 : 0)) && inK;
+  const bool fZ = inI && inJ && (k >= L.lo[2] && k <= L.hi[2] + (g.dims >= 3 ? 1 : 0));
+
+  double bxl = 0., byl = 0., bzl = 0.;
+  if (fX) {
+    bxl = bface_x(g, b0, ee, dt, i, j, k, c);
+    b1[c] = bxl;
+  }
+  if (fY) {
+    byl = bface_y(g, b0, ee, dt, i, j, k, c);
+    b1[sv + c] = byl;
+  }
+  if (fZ) {
+    bzl = bface_z(g, b0, ee, dt, i, j, k, c);
+    b1[2 * sv + c] = bzl;
+  }
+
+  if (!cell) {
+    if (PRED) {
+#pragma unroll
+      for (int n = 0; n < NMODE; ++n) w1[n * sv + c] = w0[n * sv + c];
+    }
+    return;
+  }
+
+  double wv[NMODE], u[NMODE];
+#pragma unroll
+  for (int n = 0; n < NMODE; ++n) wv[n] = w0[n * sv + c];
+  prim2cons(u, wv, g.gamm1i);
+
+  const double dlx = g.dlf[0][i], dly = g.dlf[1][j], dlz = g.dlf[2][k];
+  const double dtdv = dt / (dlx * dly * dlz);
+  const double ax = dly * dlz, ay = dlx * dlz, az = dlx * dly;
+#pragma unroll
+  for (int n = 0; n < NMODE - 3; ++n) {
+    double v = u[n] - dtdv * (ax * fx[n * sv + c + 1] - ax * fx[n * sv + c]);
+    if (g.dims >= 2) v -= dtdv * (ay * fy[n * sv + c + g.sy] - ay * fy[n * sv + c]);
+    if (g.dims >= 3) v -= dtdv * (az * fz[n * sv + c + g.sz] - az * fz[n * sv + c]);
+    u[n] = v;
+  }
+
+  // interp_bc with the upper faces recomputed in-thread
+  const double bxu = bface_x(g, b0, ee, dt, i + 1, j, k, c + 1);
+  u[BX] = (g.icm[0][i] * bxl + g.icp[0][i] * bxu) * g.dlfi[0][i];
+  if (g.dims >= 2) {
+    const double byu = bface_y(g, b0, ee, dt, i, j + 1, k, c + g.sy);
+    u[BY] = (g.icm[1][j] * byl + g.icp[1][j] * byu) * g.dlfi[1][j];
+  } else {
+    u[BY] = byl;
+  }
+  if (g.dims >= 3) {
+    const double bzu = bface_z(g, b0, ee, dt, i, j, k + 1, c + g.sz);
+    u[BZ] = (g.icm[2][k] * bzl + g.icp[2][k] * bzu) * g.dlfi[2][k];
+  } else {
+    u[BZ] = bzl;
+  }
+  cons2prim(wv, u, g.gam);
+#pragma unroll
+  for (int n = 0; n < NMODE; ++n) w1[n * sv + c] = wv[n];
+}
+
+// --------------------------------------------------------------------------
+// Ghost fill (src/bc/grid_bc.pyx:612-623, util_bc.pyx).  The z, y, x copy
+// sequence composes per-axis index maps (periodic wrap, outflow clamp), so
+// each ghost cell gathers directly from the active cell its maps name.
+__global__ __launch_bounds__(BLK) void k_ghost(Geo g, double *w, double *b,
+                                               const int *__restrict__ mx,
+                                               const int *__restrict__ my,
+                                               const int *__restrict__ mz) {
+  long t = (long)blockIdx.x * BLK + threadIdx.x;
+  const long ncell = (long)g.nt[0] * g.nt[1] * g.nt[2];
+  if (t >= ncell) return;
+  const int i = (int)(t % g.nt[0]);
+  const long r = t / g.nt[0];
+  const int j = (int)(r % g.nt[1]);
+  const int k = (int)(r / g.nt[1]);
+  const int si = mx[i], sj = my[j], sk = mz[k];
+  if (si == i && sj == j && sk == k) return;
+  const long s = ((long)sk * g.nt[1] + sj) * g.nt[0] + si;
+  const long sv = g.sv;
+#pragma unroll
+  for (int n = 0; n < NMODE; ++n) w[n * sv + t] = w[n * sv + s];
+#pragma unroll
+  for (int n = 0; n < 3; ++n) b[n * sv + t] = b[n * sv + s];
+}
+
+// interp_bc over the active cells into prim (init only, vanleer.pyx:44)
+__global__ __launch_bounds__(BLK) void k_interp_active(Geo g, double *w, const double *b, Box box) {
+  long t = (long)blockIdx.x * BLK + threadIdx.x;
+  if (t >= box.total()) return;
+  int i, j, k;
+  box_idx(box, t, i, j, k);
+  const long c = ((long)k * g.nt[1] + j) * g.nt[0] + i;
+  const long sv = g.sv;
+  w[BX * sv + c] = (g.icm[0][i] * b[c] + g.icp[0][i] * b[c + 1]) * g.dlfi[0][i];
+  if (g.dims >= 2)
+    w[BY * sv + c] = (g.icm[1][j] * b[sv + c] + g.icp[1][j] * b[sv + c + g.sy]) * g.dlfi[1][j];
+  else
+    w[BY * sv + c] = b[sv + c];
+  if (g.dims >= 3)
+    w[BZ * sv + c] = (g.icm[2][k] * b[2 * sv + c] + g.icp[2][k] * b[2 * sv + c + g.sz]) * g.dlfi[2][k];
+  else
+    w[BZ * sv + c] = b[2 * sv + c];
+}
+
+// cons = prim2cons(prim) over all cells (vanleer.pyx:163)
+__global__ __launch_bounds__(BLK) void k_prim2cons(Geo g, const double *w, double *u) {
+  long t = (long)blockIdx.x * BLK + threadIdx.x;
+  const long ncell = (long)g.nt[0] * g.nt[1] * g.nt[2];
+  if (t >= ncell) return;
+  double wv[NMODE], uv[NMODE];
+#pragma unroll
+  for (int n = 0; n < NMODE; ++n) wv[n] = w[n * g.sv + t];
+  prim2cons(uv, wv, g.gamm1i);
+#pragma unroll
+  for (int n = 0; n < NMODE; ++n) u[n * g.sv + t] = uv[n];
+}
+
+// --------------------------------------------------------------------------
+// CFL dt (new_dt.pyx:55-117): per-block max of max(cx/dx, cy/dy, cz/dz) over
+// active cells, wavefront shuffles then LDS across the block's 4 waves.
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+__global__ __launch_bounds__(BLK) void k_dt_partial(Geo g, const double *__restrict__ w, Box box,
+                                                    int fms_ref, double *__restrict__ part) {
+  __shared__ double red[BLK / 64];
+  long t = (long)blockIdx.x * BLK + threadIdx.x;
+  double cds = 0.;
+  if (t < box.total()) {
+    int i, j, k;
+    box_idx(box, t, i, j, k);
+    const long c = ((long)k * g.nt[1] + j) * g.nt[0] + i;
+    double wv[NMODE];
+#pragma unroll
+    for (int n = 0; n < NMODE; ++n) wv[n] = w[n * g.sv + c];
+    const double cx = fabs(wv[VX]) + fms(wv, g.gam);
+    double cy = 0., cz = 0.;
+    if (g.dims >= 2) {
+      double wy[NMODE];
+#pragma unroll
+      for (int n = 0; n < NMODE; ++n) wy[n] = wv[n];
+      if (!fms_ref) { wy[BX] = wv[BY]; wy[BY] = wv[BX]; }
+      cy = fabs(wv[VY]) + fms(wy, g.gam);
+    }
+    if (g.dims >= 3) {
+      double wz[NMODE];
+#pragma unroll
+      for (int n = 0; n < NMODE; ++n) wz[n] = wv[n];
+      if (!fms_ref) { wz[BX] = wv[BZ]; wz[BZ] = wv[BX]; }
+      cz = fabs(wv[VZ]) + fms(wz, g.gam);
+    }
+    cds = fmax(fmax(cx / g.dlf[0][i], cy / g.dlf[1][j]), cz / g.dlf[2][k]);
+    // NaN never wins the reference's `cds > cdsmaxloc` test
+    if (!(cds > 0.)) cds = 0.;
+  }
+  cds = wave_max(cds);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = cds;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = red[0];
+#pragma unroll
+    for (int q = 1; q < BLK / 64; ++q) m = fmax(m, red[q]);
+    part[blockIdx.x] = m;
+  }
+}
+
+// dev[0]=dt, dev[1]=t, dev[2]=tmax (<0: no clip), dev[3]=step counter
+__global__ __launch_bounds__(BLK) void k_dt_final(const double *__restrict__ part, int npart,
+                                                  double cour, double *dev, int use_fixed,
+                                                  double fixed_dt) {
+  __shared__ double red[BLK / 64];
+  double m = 0.;
+  for (int q = threadIdx.x; q < npart; q += BLK) m = fmax(m, part[q]);
+  m = wave_max(m);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double mm = red[0];
+    for (int q = 1; q < BLK / 64; ++q) mm = fmax(mm, red[q]);
+    double dt = use_fixed ? fixed_dt : cour / mm;
+    const double tnow = dev[1], tmax = dev[2];
+    if (!use_fixed && tmax >= 0. && tnow + dt > tmax) dt = tmax - tnow;
+    dev[0] = dt;
+  }
+}
+
+__global__ void k_time_advance(double *dev) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    dev[1] += dev[0];
+    dev[3] += 1.0;
+  }
+}
+
+// --------------------------------------------------------------------------
+// Diagnostics: sums and extrema over active cells, one partial per block.
+constexpr int NDIAG = 11;  // mass, mx, my, mz, E, psc, divb, bmax, -pmin, -rhomin, nonfinite
+__global__ __launch_bounds__(BLK) void k_diag(Geo g, const double *__restrict__ w,
+                                              const double *__restrict__ b, Box box,
+                                              double *__restrict__ part) {
+  __shared__ double red[NDIAG][BLK / 64];
+  long t = (long)blockIdx.x * BLK + threadIdx.x;
+  double v[NDIAG];
+#pragma unroll
+  for (int q = 0; q < NDIAG; ++q) v[q] = 0.;
+  v[8] = -1e300;
+  v[9] = -1e300;
+  if (t < box.total()) {
+    int i, j, k;
+    box_idx(box, t, i, j, k);
+    const long c = ((long)k * g.nt[1] + j) * g.nt[0] + i;
+    double wv[NMODE], u[NMODE];
+    bool finite = true;
+#pragma unroll
+    for (int n = 0; n < NMODE; ++n) {
+      wv[n] = w[n * g.sv + c];
+      finite = finite && isfinite(wv[n]);
+    }
+    prim2cons(u, wv, g.gamm1i);
+    const double vol = g.dlf[0][i] * g.dlf[1][j] * g.dlf[2][k];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) v[q] = u[q] * vol;
+    double div = (b[c + 1] - b[c]) * g.dlfi[0][i];
+    if (g.dims >= 2) div += (b[g.sv + c + g.sy] - b[g.sv + c]) * g.dlfi[1][j];
+    if (g.dims >= 3) div += (b[2 * g.sv + c + g.sz] - b[2 * g.sv + c]) * g.dlfi[2][k];
+    v[6] = fabs(div);
+    v[7] = sqrt(sqr(wv[BX]) + sqr(wv[BY]) + sqr(wv[BZ]));
+    v[8] = -wv[PR];
+    v[9] = -wv[RHO];
+    v[10] = finite ? 0. : 1.;
+  }
+#pragma unroll
+  for (int q = 0; q < NDIAG; ++q) {
+    double x = v[q];
+    const bool is_max = (q >= 6 && q <= 9);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      double y = __shfl_xor(x, off, 64);
+      x = is_max ? fmax(x, y) : x + y;
+    }
+    if ((threadIdx.x & 63) == 0) red[q][threadIdx.x >> 6] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < NDIAG) {
+    const int q = threadIdx.x;
+    const bool is_max = (q >= 6 && q <= 9);
+    double x = red[q][0];
+    for (int p = 1; p < BLK / 64; ++p) x = is_max ? fmax(x, red[q][p]) : x + red[q][p];
+    part[(long)blockIdx.x * NDIAG + q] = x;
+  }
+}
+
+// Pencil solver entry for unit tests: [9][n] local-frame states.
+template <int RS>
+__global__ __launch_bounds__(BLK) void k_pencil(const double *wl_, const double *wr_,
+                                                const double *bx, double *flux, long n,
+                                                double gam, double gamm1i) {
+  long t = (long)blockIdx.x * BLK + threadIdx.x;
+  if (t >= n) return;
+  double wl[NMODE], wr[NMODE], f[NMODE];
+#pragma unroll
+  for (int q = 0; q < NMODE; ++q) {
+    wl[q] = wl_[q * n + t];
+    wr[q] = wr_[q * n + t];
+  }
+  wl[BX] = bx[t];
+  wr[BX] = bx[t];
+  int reg = (RS == BNZ_RS_HLLD) ? flux_hlld(f, wl, wr, gam, gamm1i)
+                                : flux_hll(f, wl, wr, gam, gamm1i);
+  if (reg == 0) return;
+#pragma unroll
+  for (int q = 0; q < NMODE; ++q) flux[q * n + t] = f[q];
+}
+
+}  // namespace
+
+// ==========================================================================
+// Host side
+// ==========================================================================
+
+struct KTime {
+  std::string name;
+  double total_ms = 0.;
+  long launches = 0;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+};
+
+struct bnz_ctx {
+  bnz_params p{};
+  int dims = 1, ng = 3, order = 1;
+  int nt[3] = {1, 1, 1};
+  int i1, i2, j1, j2, k1, k2;
+  long ncell = 0;
+  hipStream_t stream = nullptr;
+  double *w = nullptr, *ws = nullptr, *b = nullptr, *bs = nullptr;
+  double *fx = nullptr, *fy = nullptr, *fz = nullptr, *ee = nullptr;
+  double *geo = nullptr;
+  int *maps = nullptr;
+  double *dev = nullptr;      // dt, t, tmax, step
+  double *part = nullptr;     // reduction partials
+  long npart_cap = 0;
+  std::vector<double> h_dlf[3], h_dlfi[3], h_icm[3], h_icp[3];
+  Geo g{};
+  bool ready = false;
+  bool timing = false;
+  std::vector<KTime> ktimes;
+  std::string err;
+};
+
+namespace {
+
+int fail(bnz_ctx *c, int code, const std::string &msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPCHK(ctx, call)                                                              \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(ctx, BNZ_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));   \
+  } while (0)
+
+int blocks_for(long n) { return (int)((n + BLK - 1) / BLK); }
+
+// Uniform Cartesian coordinates as src/coordinates/coord.pyx:201-257 and
+// src/coordinates/cartesian.pyx:18-43 compute them (one MPI block).
+void set_coordinates(bnz_ctx *c) {
+  for (int n = 0; n < 3; ++n) {
+    const int N = c->nt[n];
+    std::vector<double> lf(N + 1), lv(N);
+    if (N == 1) {
+      lf[0] = c->p.lmin[n];
+      lf[1] = c->p.lmax[n];
+    } else {
+      const int ind1 = (n == 0) ? c->i1 : (n == 1) ? c->j1 : c->k1;
+      const int ind2 = (n == 0) ? c->i2 : (n == 1) ? c->j2 : c->k2;
+      const int nact = (n == 0) ? c->p.nx : (n == 1) ? c->p.ny : c->p.nz;
+      const double dx = (c->p.lmax[n] - c->p.lmin[n]) / nact;
+      const double l1 = 0 * nact * dx;
+      for (int i = 0; i < N + 1; ++i) lf[i] = l1 + (i - ind1) * dx;
+      if (c->p.bc[2 * n] == 0)
+        for (int i = ind1 - 1; i > -1; --i) {
+          const int i_ = ind2 - i + (ind1 - 1);
+          const double d = lf[i_ + 1] - lf[i_];
+          lf[i] = lf[i + 1] - d;
+        }
+      if (c->p.bc[2 * n + 1] == 0)
+        for (int i = ind2 + 1; i < N; ++i) {
+          const int i_ = ind1 + i - (ind2 + 1);
+          const double d = lf[i_ + 1] - lf[i_];
+          lf[i + 1] = lf[i] + d;
+        }
+    }
+    c->h_dlf[n].resize(N);
+    c->h_dlfi[n].resize(N);
+    c->h_icm[n].resize(N);
+    c->h_icp[n].resize(N);
+    for (int i = 0; i < N; ++i) {
+      c->h_dlf[n][i] = lf[i + 1] - lf[i];
+      c->h_dlfi[n][i] = 1. / c->h_dlf[n][i];
+      lv[i] = 0.5 * (lf[i] + lf[i + 1]);
+      c->h_icm[n][i] = lv[i] - lf[i];
+      c->h_icp[n][i] = lf[i + 1] - lv[i];
+    }
+  }
+}
+
+// Per-axis ghost source maps (grid_bc_funcs.pyx:36-196; D14 for z1 outflow).
+std::vector<int> axis_map(int N, int a1, int a2, int ng, int f_lo, int f_hi, bool active) {
+  std::vector<int> m(N);
+  for (int i = 0; i < N; ++i) m[i] = i;
+  if (!active) return m;
+  const int n = a2 - a1 + 1;
+  for (int g = 0; g < ng; ++g) {
+    const int dl = a1 - ng + g;
+    m[dl] = (f_lo == 0) ? (a2 - ng + 1 + g) : a1;
+    const int dr = a2 + 1 + g;
+    m[dr] = (f_hi == 0) ? (a1 + g) : a2;
+  }
+  (void)n;
+  return m;
+}
+
+struct Launch {
+  bnz_ctx *c;
+  int idx;
+  hipEvent_t e0{}, e1{};
+  Launch(bnz_ctx *ctx, const char *name) : c(ctx), idx(-1) {
+    if (!c->timing) return;
+    for (size_t q = 0; q < c->ktimes.size(); ++q)
+      if (c->ktimes[q].name == name) idx = (int)q;
+    if (idx < 0) {
+      c->ktimes.push_back(KTime{});
+      c->ktimes.back().name = name;
+      idx = (int)c->ktimes.size() - 1;
+    }
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    (void)hipEventRecord(e0, c->stream);
+  }
+  ~Launch() {
+    if (idx < 0) return;
+    (void)hipEventRecord(e1, c->stream);
+    c->ktimes[idx].pending.emplace_back(e0, e1);
+  }
+};
+
+void collect_timing(bnz_ctx *c) {
+  for (auto &kt : c->ktimes) {
+    for (auto &pr : kt.pending) {
+      (void)hipEventSynchronize(pr.second);
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, pr.first, pr.second);
+      kt.total_ms += ms;
+      kt.launches += 1;
+      (void)hipEventDestroy(pr.first);
+      (void)hipEventDestroy(pr.second);
+    }
+    kt.pending.clear();
+  }
+}
+
+Lims integr_lims(const bnz_ctx *c, int order) {
+  Lims L;
+  for (int a = 0; a < 3; ++a) {
+    L.lo[a] = order + 1;
+    L.hi[a] = c->nt[a] - 2 - order;
+  }
+  if (c->dims < 2) L.lo[1] = L.hi[1] = 0;
+  if (c->dims < 3) L.lo[2] = L.hi[2] = 0;
+  return L;
+}
+
+Box active_box(const bnz_ctx *c) {
+  Box b;
+  b.lo[0] = c->i1; b.n[0] = c->i2 - c->i1 + 1;
+  b.lo[1] = c->j1; b.n[1] = c->j2 - c->j1 + 1;
+  b.lo[2] = c->k1; b.n[2] = c->k2 - c->k1 + 1;
+  return b;
+}
+
+template <int ORDER, int RS>
+void launch_fluxes(bnz_ctx *c, const double *w, const double *bf, const Lims &L) {
+  const int il = L.lo[0], iu = L.hi[0], jl = L.lo[1], ju = L.hi[1], kl = L.lo[2], ku = L.hi[2];
+  const Geo &g = c->g;
+  {  // x faces i in [il, iu+1], rows j in [jl-1, ju+1], k in [kl-1, ku+1]
+    Box bx;
+    bx.lo[0] = il; bx.n[0] = iu + 1 - il + 1;
+    bx.lo[1] = (c->dims >= 2) ? jl - 1 : 0; bx.n[1] = (c->dims >= 2) ? (ju + 1) - (jl - 1) + 1 : 1;
+    bx.lo[2] = (c->dims >= 3) ? kl - 1 : 0; bx.n[2] = (c->dims >= 3) ? (ku + 1) - (kl - 1) + 1 : 1;
+    Launch t(c, ORDER ? "flux_x_plm" : "flux_x_dc");
+    hipLaunchKernelGGL((k_flux<0, ORDER, RS>), dim3(blocks_for(bx.total())), dim3(BLK), 0,
+                       c->stream, g, w, bf, c->fx, bx);
+  }
+  if (c->dims >= 2) {  // y faces j in [jl, ju+1], i in [il-1, iu+1], k in [kl-1, ku+1]
+    Box by;
+    by.lo[0] = il - 1; by.n[0] = (iu + 1) - (il - 1) + 1;
+    by.lo[1] = jl; by.n[1] = ju + 1 - jl + 1;
+    by.lo[2] = (c->dims >= 3) ? kl - 1 : 0; by.n[2] = (c->dims >= 3) ? (ku + 1) - (kl - 1) + 1 : 1;
+    Launch t(c, ORDER ? "flux_y_plm" : "flux_y_dc");
+    hipLaunchKernelGGL((k_flux<1, ORDER, RS>), dim3(blocks_for(by.total())), dim3(BLK), 0,
+                       c->stream, g, w, bf, c->fy, by);
+  }
+  if (c->dims >= 3) {  // z faces k in [kl, ku+1], i in [il-1, iu+1], j in [jl-1, ju+1]
+    Box bz;
+    bz.lo[0] = il - 1; bz.n[0] = (iu + 1) - (il - 1) + 1;
+    bz.lo[1] = jl - 1; bz.n[1] = (ju + 1) - (jl - 1) + 1;
+    bz.lo[2] = kl; bz.n[2] = ku + 1 - kl + 1;
+    Launch t(c, ORDER ? "flux_z_plm" : "flux_z_dc");
+    hipLaunchKernelGGL((k_flux<2, ORDER, RS>), dim3(blocks_for(bz.total())), dim3(BLK), 0,
+                       c->stream, g, w, bf, c->fz, bz);
+  }
+}
+
+void launch_flux_dispatch(bnz_ctx *c, int order, const double *w, const double *bf,
+                          const Lims &L) {
+  const bool hlld = c->p.rsolver == BNZ_RS_HLLD;
+  if (order == 0) {
+    if (hlld) launch_fluxes<0, BNZ_RS_HLLD>(c, w, bf, L);
+    else launch_fluxes<0, BNZ_RS_HLL>(c, w, bf, L);
+  } else {
+    if (hlld) launch_fluxes<1, BNZ_RS_HLLD>(c, w, bf, L);
+    else launch_fluxes<1, BNZ_RS_HLL>(c, w, bf, L);
+  }
+}
+
+void launch_emf(bnz_ctx *c, const double *w, const Lims &L, const char *name) {
+  Box e;
+  e.lo[0] = L.lo[0]; e.n[0] = L.hi[0] + 1 - L.lo[0] + 1;
+  e.lo[1] = (c->dims >= 2) ? L.lo[1] : 0; e.n[1] = (c->dims >= 2) ? L.hi[1] + 1 - L.lo[1] + 1 : 1;
+  e.lo[2] = (c->dims >= 3) ? L.lo[2] : 0; e.n[2] = (c->dims >= 3) ? L.hi[2] + 1 - L.lo[2] + 1 : 1;
+  Launch t(c, name);
+  hipLaunchKernelGGL(k_emf, dim3(blocks_for(e.total())), dim3(BLK), 0, c->stream, c->g, w,
+                     c->fx, c->fy, c->fz, c->ee, e);
+}
+
+void launch_ghost(bnz_ctx *c) {
+  Launch t(c, "ghost");
+  hipLaunchKernelGGL(k_ghost, dim3(blocks_for(c->ncell)), dim3(BLK), 0, c->stream, c->g, c->w,
+                     c->b, c->maps, c->maps + c->nt[0], c->maps + c->nt[0] + c->nt[1]);
+}
+
+void launch_dt(bnz_ctx *c, int use_fixed, double fixed_dt) {
+  Box a = active_box(c);
+  const int nb = blocks_for(a.total());
+  {
+    Launch t(c, "dt_partial");
+    hipLaunchKernelGGL(k_dt_partial, dim3(nb), dim3(BLK), 0, c->stream, c->g, c->w, a,
+                       c->p.dt_fms_ref, c->part);
+  }
+  Launch t(c, "dt_final");
+  hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(BLK), 0, c->stream, c->part, nb, c->p.cour,
+                     c->dev, use_fixed, fixed_dt);
+}
+
+// One VL step after the dt kernels (vanleer.pyx:100-163).
+void launch_step_body(bnz_ctx *c) {
+  // predictor: donor cell, dt/2, lims of order 0
+  Lims L0 = integr_lims(c, 0);
+  launch_flux_dispatch(c, 0, c->w, c->b, L0);
+  launch_emf(c, c->w, L0, "emf_pred");
+  {
+    Launch t(c, "update_pred");
+    hipLaunchKernelGGL(k_update<true>, dim3(blocks_for(c->ncell)), dim3(BLK), 0, c->stream,
+                       c->g, c->w, c->fx, c->fy, c->fz, c->ee, c->b, c->bs, c->ws, L0, c->dev);
+  }
+  // corrector: reconstruction order, dt, reads ws/bs; writes w in place and
+  // the new face field into the spare face buffer
+  Lims L1 = integr_lims(c, c->order);
+  launch_flux_dispatch(c, c->order, c->ws, c->bs, L1);
+  launch_emf(c, c->ws, L1, "emf_corr");
+  {
+    // b_{n+1} goes into a buffer that is not b_n; the predictor's bs is dead
+    // once the corrector fluxes and EMFs are computed, so it is reused.
+    Launch t(c, "update_corr");
+    hipLaunchKernelGGL(k_update<false>, dim3(blocks_for(c->ncell)), dim3(BLK), 0, c->stream,
+                       c->g, c->w, c->fx, c->fy, c->fz, c->ee, c->b, c->bs, c->w, L1, c->dev);
+  }
+  std::swap(c->b, c->bs);
+  launch_ghost(c);
+  hipLaunchKernelGGL(k_time_advance, dim3(1), dim3(64), 0, c->stream, c->dev);
+}
+
+int check_launch(bnz_ctx *c) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(c, BNZ_EHIP, std::string("kernel launch: ") + hipGetErrorString(e));
+  return BNZ_OK;
+}
+
+}  // namespace
+
+// ==========================================================================
+// C ABI
+// ==========================================================================
+extern "C" {
+
+const char *bnz_last_error(const bnz_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int bnz_create(const bnz_params *p, bnz_ctx **out) {
+  if (!p || !out) return BNZ_EINVAL;
+  *out = nullptr;
+  bnz_ctx *c = new bnz_ctx();
+  c->p = *p;
+  auto bad = [&](int code, const std::string &m) {
+    // keep the context so the caller can read the message, then destroy it
+    c->err = m;
+    *out = c;
+    return code;
+  };
+  if (p->nx < 1 || p->ny < 1 || p->nz < 1) return bad(BNZ_EINVAL, "nx, ny, nz must be >= 1");
+  if (p->nz > 1 && p->ny == 1) return bad(BNZ_EINVAL, "nz > 1 needs ny > 1 (grid.pyx:114-122)");
+  if (!(p->gam > 1.0)) return bad(BNZ_EINVAL, "gam must be > 1");
+  if (!(p->cour > 0.0)) return bad(BNZ_EINVAL, "cour must be > 0");
+  if (p->precision != BNZ_PREC_FP64) return bad(BNZ_EUNSUPPORTED, "only FP64 precision is built");
+  if (p->rsolver == BNZ_RS_HLLC)
+    return bad(BNZ_EUNSUPPORTED, "hllc is a hydro-only solver (fluxes_c.c:230); MHD needs hll or hlld");
+  if (p->rsolver != BNZ_RS_HLLD && p->rsolver != BNZ_RS_HLL)
+    return bad(BNZ_EUNSUPPORTED, "rsolver not implemented on the device (hll, hlld are)");
+  if (p->reconstr != BNZ_RCN_CONST && p->reconstr != BNZ_RCN_LINEAR)
+    return bad(BNZ_EUNSUPPORTED, "reconstr must be const or linear (weno/parab are commented out in reconstr_c.c)");
+  for (int q = 0; q < 6; ++q)
+    if (p->bc[q] != BNZ_BC_PERIODIC && p->bc[q] != BNZ_BC_OUTFLOW)
+      return bad(BNZ_EUNSUPPORTED, "boundary flags must be 0 (periodic) or 1 (outflow)");
+  c->dims = (p->nz > 1) ? 3 : (p->ny > 1) ? 2 : 1;
+  c->order = (p->reconstr == BNZ_RCN_CONST) ? 0 : 1;
+  c->ng = (c->order == 0) ? 2 : 3;
+  c->nt[0] = p->nx + 2 * c->ng;
+  c->nt[1] = (c->dims >= 2) ? p->ny + 2 * c->ng : 1;
+  c->nt[2] = (c->dims >= 3) ? p->nz + 2 * c->ng : 1;
+  c->i1 = c->ng; c->i2 = p->nx + c->ng - 1;
+  c->j1 = (c->dims >= 2) ? c->ng : 0; c->j2 = (c->dims >= 2) ? p->ny + c->ng - 1 : 0;
+  c->k1 = (c->dims >= 3) ? c->ng : 0; c->k2 = (c->dims >= 3) ? p->nz + c->ng - 1 : 0;
+  c->ncell = (long)c->nt[0] * c->nt[1] * c->nt[2];
+  set_coordinates(c);
+
+  *out = c;
+  if (hipSetDevice(p->device) != hipSuccess) return fail(c, BNZ_EHIP, "hipSetDevice failed");
+  HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  const size_t cells = (size_t)c->ncell;
+  double **bufs9[] = {&c->w, &c->ws, &c->fx, &c->fy, &c->fz};
+  for (double **q : bufs9)
+    if (hipMalloc(q, NVAR_STATE * cells * sizeof(double)) != hipSuccess)
+      return fail(c, BNZ_ENOMEM, "device allocation of grid arrays failed");
+  double **bufs3[] = {&c->b, &c->bs, &c->ee};
+  for (double **q : bufs3)
+    if (hipMalloc(q, 3 * cells * sizeof(double)) != hipSuccess)
+      return fail(c, BNZ_ENOMEM, "device allocation of field arrays failed");
+  // zero everything once so stale reads of never-written ghost faces are defined
+  for (double **q : bufs9) HIPCHK(c, hipMemsetAsync(*q, 0, NVAR_STATE * cells * sizeof(double), c->stream));
+  for (double **q : bufs3) HIPCHK(c, hipMemsetAsync(*q, 0, 3 * cells * sizeof(double), c->stream));
+
+  size_t geo_n = 0;
+  for (int a = 0; a < 3; ++a) geo_n += 4 * (size_t)c->nt[a];
+  HIPCHK(c, hipMalloc(&c->geo, geo_n * sizeof(double)));
+  std::vector<double> hg;
+  hg.reserve(geo_n);
+  size_t off[3][4];
+  for (int a = 0; a < 3; ++a) {
+    off[a][0] = hg.size(); hg.insert(hg.end(), c->h_dlf[a].begin(), c->h_dlf[a].end());
+    off[a][1] = hg.size(); hg.insert(hg.end(), c->h_dlfi[a].begin(), c->h_dlfi[a].end());
+    off[a][2] = hg.size(); hg.insert(hg.end(), c->h_icm[a].begin(), c->h_icm[a].end());
+    off[a][3] = hg.size(); hg.insert(hg.end(), c->h_icp[a].begin(), c->h_icp[a].end());
+  }
+  HIPCHK(c, hipMemcpy(c->geo, hg.data(), geo_n * sizeof(double), hipMemcpyHostToDevice));
+
+  std::vector<int> maps;
+  auto mx = axis_map(c->nt[0], c->i1, c->i2, c->ng, p->bc[0], p->bc[1], true);
+  auto my = axis_map(c->nt[1], c->j1, c->j2, c->ng, p->bc[2], p->bc[3], c->dims >= 2);
+  auto mz = axis_map(c->nt[2], c->k1, c->k2, c->ng, p->bc[4], p->bc[5], c->dims >= 3);
+  maps.insert(maps.end(), mx.begin(), mx.end());
+  maps.insert(maps.end(), my.begin(), my.end());
+  maps.insert(maps.end(), mz.begin(), mz.end());
+  HIPCHK(c, hipMalloc(&c->maps, maps.size() * sizeof(int)));
+  HIPCHK(c, hipMemcpy(c->maps, maps.data(), maps.size() * sizeof(int), hipMemcpyHostToDevice));
+
+  HIPCHK(c, hipMalloc(&c->dev, 8 * sizeof(double)));
+  const double dev0[8] = {0., 0., -1., 0., 0., 0., 0., 0.};
+  HIPCHK(c, hipMemcpy(c->dev, dev0, sizeof(dev0), hipMemcpyHostToDevice));
+  Box a = active_box(c);
+  c->npart_cap = (long)blocks_for(a.total()) * NDIAG + 64;
+  HIPCHK(c, hipMalloc(&c->part, c->npart_cap * sizeof(double)));
+
+  Geo &g = c->g;
+  for (int q = 0; q < 3; ++q) g.nt[q] = c->nt[q];
+  g.dims = c->dims;
+  g.sv = c->ncell;
+  g.sy = c->nt[0];
+  g.sz = c->nt[0] * c->nt[1];
+  for (int q = 0; q < 3; ++q) {
+    g.dlf[q] = c->geo + off[q][0];
+    g.dlfi[q] = c->geo + off[q][1];
+    g.icm[q] = c->geo + off[q][2];
+    g.icp[q] = c->geo + off[q][3];
+  }
+  g.gam = p->gam;
+  g.gamm1i = 1. / (p->gam - 1.);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return BNZ_OK;
+}
+
+int bnz_destroy(bnz_ctx *c) {
+  if (!c) return BNZ_OK;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  collect_timing(c);
+  double *bufs[] = {c->w, c->ws, c->b, c->bs, c->fx, c->fy, c->fz, c->ee, c->geo, c->dev, c->part};
+  for (double *q : bufs)
+    if (q) (void)hipFree(q);
+  if (c->maps) (void)hipFree(c->maps);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return BNZ_OK;
+}
+
+int bnz_get_shape(const bnz_ctx *c, int *ntot3, int *ng) {
+  if (!c) return BNZ_EINVAL;
+  if (ntot3)
+    for (int q = 0; q < 3; ++q) ntot3[q] = c->nt[q];
+  if (ng) *ng = c->ng;
+  return BNZ_OK;
+}
+
+int bnz_upload(bnz_ctx *c, const double *prim, const double *bfld) {
+  if (!c || !c->stream) return BNZ_EINVAL;
+  if (prim) HIPCHK(c, hipMemcpy(c->w, prim, NMODE * c->ncell * sizeof(double), hipMemcpyHostToDevice));
+  if (bfld) HIPCHK(c, hipMemcpy(c->b, bfld, 3 * c->ncell * sizeof(double), hipMemcpyHostToDevice));
+  return BNZ_OK;
+}
+
+int bnz_download(bnz_ctx *c, double *prim, double *bfld, double *cons) {
+  if (!c || !c->stream) return BNZ_EINVAL;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (prim) HIPCHK(c, hipMemcpy(prim, c->w, NMODE * c->ncell * sizeof(double), hipMemcpyDeviceToHost));
+  if (bfld) HIPCHK(c, hipMemcpy(bfld, c->b, 3 * c->ncell * sizeof(double), hipMemcpyDeviceToHost));
+  if (cons) {
+    // ws is dead between steps; it holds prim2cons(prim) for the copy
+    hipLaunchKernelGGL(k_prim2cons, dim3(blocks_for(c->ncell)), dim3(BLK), 0, c->stream, c->g,
+                       c->w, c->ws);
+    int rc = check_launch(c);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(cons, c->ws, NMODE * c->ncell * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  return BNZ_OK;
+}
+
+int bnz_fill_ghosts(bnz_ctx *c) {
+  if (!c || !c->stream) return BNZ_EINVAL;
+  launch_ghost(c);
+  return check_launch(c);
+}
+
+int bnz_init_state(bnz_ctx *c) {
+  if (!c || !c->stream) return BNZ_EINVAL;
+  launch_ghost(c);
+  Box a = active_box(c);
+  hipLaunchKernelGGL(k_interp_active, dim3(blocks_for(a.total())), dim3(BLK), 0, c->stream,
+                     c->g, c->w, c->b, a);
+  launch_ghost(c);
+  const double dev0[4] = {0., 0., -1., 0.};
+  HIPCHK(c, hipMemcpyAsync(c->dev, dev0, sizeof(dev0), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->ready = true;
+  return check_launch(c);
+}
+
+int bnz_new_dt(bnz_ctx *c, double *dt) {
+  if (!c || !dt) return BNZ_EINVAL;
+  if (!c->ready) return fail(c, BNZ_ESTATE, "bnz_init_state has not run");
+  const double nomax = -1.;
+  HIPCHK(c, hipMemcpyAsync(c->dev + 2, &nomax, sizeof(double), hipMemcpyHostToDevice, c->stream));
+  launch_dt(c, 0, 0.);
+  int rc = check_launch(c);
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpyAsync(dt, c->dev, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return BNZ_OK;
+}
+
+int bnz_vl_step(bnz_ctx *c, double dt) {
+  if (!c) return BNZ_EINVAL;
+  if (!c->ready) return fail(c, BNZ_ESTATE, "bnz_init_state has not run");
+  if (!(dt > 0.)) return fail(c, BNZ_EINVAL, "dt must be > 0");
+  launch_dt(c, 1, dt);
+  launch_step_body(c);
+  return check_launch(c);
+}
+
+int bnz_run_steps(bnz_ctx *c, int nsteps) {
+  if (!c) return BNZ_EINVAL;
+  if (!c->ready) return fail(c, BNZ_ESTATE, "bnz_init_state has not run");
+  const double nomax = -1.;
+  HIPCHK(c, hipMemcpyAsync(c->dev + 2, &nomax, sizeof(double), hipMemcpyHostToDevice, c->stream));
+  for (int s = 0; s < nsteps; ++s) {
+    launch_dt(c, 0, 0.);
+    launch_step_body(c);
+  }
+  return check_launch(c);
+}
+
+int bnz_advance(bnz_ctx *c, double tmax, int max_steps, int *nsteps) {
+  if (!c) return BNZ_EINVAL;
+  if (!c->ready) return fail(c, BNZ_ESTATE, "bnz_init_state has not run");
+  HIPCHK(c, hipMemcpyAsync(c->dev + 2, &tmax, sizeof(double), hipMemcpyHostToDevice, c->stream));
+  int n = 0;
+  double t = 0.;
+  HIPCHK(c, hipMemcpyAsync(&t, c->dev + 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  while (t < tmax && (max_steps <= 0 || n < max_steps)) {
+    launch_dt(c, 0, 0.);
+    launch_step_body(c);
+    int rc = check_launch(c);
+    if (rc) return rc;
+    ++n;
+    HIPCHK(c, hipMemcpyAsync(&t, c->dev + 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  if (nsteps) *nsteps = n;
+  return BNZ_OK;
+}
+
+int bnz_get_time(bnz_ctx *c, double *time, int *step, double *last_dt) {
+  if (!c) return BNZ_EINVAL;
+  double d[4];
+  HIPCHK(c, hipMemcpyAsync(d, c->dev, sizeof(d), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (time) *time = d[1];
+  if (step) *step = (int)d[3];
+  if (last_dt) *last_dt = d[0];
+  return BNZ_OK;
+}
+
+int bnz_set_time(bnz_ctx *c, double time, int step) {
+  if (!c) return BNZ_EINVAL;
+  double d[2] = {time, 0.};
+  HIPCHK(c, hipMemcpyAsync(c->dev + 1, &d[0], sizeof(double), hipMemcpyHostToDevice, c->stream));
+  d[1] = (double)step;
+  HIPCHK(c, hipMemcpyAsync(c->dev + 3, &d[1], sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return BNZ_OK;
+}
+
+int bnz_diag(bnz_ctx *c, bnz_diag_out *out) {
+  if (!c || !out) return BNZ_EINVAL;
+  Box a = active_box(c);
+  const int nb = blocks_for(a.total());
+  hipLaunchKernelGGL(k_diag, dim3(nb), dim3(BLK), 0, c->stream, c->g, c->w, c->b, a, c->part);
+  int rc = check_launch(c);
+  if (rc) return rc;
+  std::vector<double> h((size_t)nb * NDIAG);
+  HIPCHK(c, hipMemcpyAsync(h.data(), c->part, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  double s[NDIAG] = {0};
+  s[6] = s[7] = 0.;
+  s[8] = s[9] = -1e300;
+  for (int q = 0; q < nb; ++q)
+    for (int v = 0; v < NDIAG; ++v) {
+      const double x = h[(size_t)q * NDIAG + v];
+      if (v >= 6 && v <= 9) s[v] = std::fmax(s[v], x);
+      else s[v] += x;
+    }
+  out->mass = s[0];
+  out->mom[0] = s[1];
+  out->mom[1] = s[2];
+  out->mom[2] = s[3];
+  out->energy = s[4];
+  out->scalar = s[5];
+  out->divb_max = s[6];
+  out->bmax = s[7];
+  out->pmin = -s[8];
+  out->rhomin = -s[9];
+  out->nonfinite = (long)s[10];
+  return BNZ_OK;
+}
+
+int bnz_synchronize(bnz_ctx *c) {
+  if (!c) return BNZ_EINVAL;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return BNZ_OK;
+}
+
+int bnz_set_kernel_timing(bnz_ctx *c, int on) {
+  if (!c) return BNZ_EINVAL;
+  c->timing = on != 0;
+  return BNZ_OK;
+}
+
+int bnz_kernel_timing(bnz_ctx *c, int idx, char *name, int name_len, double *total_ms,
+                      long *launches) {
+  if (!c) return BNZ_EINVAL;
+  collect_timing(c);
+  if (idx < 0 || idx >= (int)c->ktimes.size()) return BNZ_EINVAL;
+  if (name && name_len > 0) {
+    std::strncpy(name, c->ktimes[idx].name.c_str(), name_len - 1);
+    name[name_len - 1] = 0;
+  }
+  if (total_ms) *total_ms = c->ktimes[idx].total_ms;
+  if (launches) *launches = c->ktimes[idx].launches;
+  return BNZ_OK;
+}
+
+int bnz_gpu_flux_pencils(int rsolver, const double *wl, const double *wr, const double *bx,
+                         double *flux, long n, double gam) {
+  if (!wl || !wr || !bx || !flux || n < 0 || !(gam > 1.)) return BNZ_EINVAL;
+  if (n == 0) return BNZ_OK;
+  const double gm1i = 1. / (gam - 1.);
+  if (rsolver == BNZ_RS_HLLD)
+    hipLaunchKernelGGL(k_pencil<BNZ_RS_HLLD>, dim3(blocks_for(n)), dim3(BLK), 0, 0, wl, wr, bx,
+                       flux, n, gam, gm1i);
+  else if (rsolver == BNZ_RS_HLL)
+    hipLaunchKernelGGL(k_pencil<BNZ_RS_HLL>, dim3(blocks_for(n)), dim3(BLK), 0, 0, wl, wr, bx,
+                       flux, n, gam, gm1i);
+  else
+    return BNZ_EUNSUPPORTED;
+  if (hipGetLastError() != hipSuccess) return BNZ_EHIP;
+  if (hipDeviceSynchronize() != hipSuccess) return BNZ_EHIP;
+  return BNZ_OK;
+}
+
+}  // extern "C"

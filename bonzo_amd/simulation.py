@@ -1,0 +1,218 @@
+"""Host API of the MI355X VL step, shaped like the reference's.
+
+Mirrors the Python/Cython surface of sergekomarov/bonzo for this path:
+  * ``read_param(section, key, dtype, usr_dir)`` -- src/io/read_config.pyx:10-38
+  * ``BnzGrid`` (sizes, ghosts, index ranges, BC flags) -- src/coordinates/
+    grid.pyx:107-231, src/bc/grid_bc.pyx:45-50
+  * ``BnzIntegr`` with the tintegr/rsolver/reconstr selection of
+    src/integration/integrator.pyx:205-291 and enums of integrator.pxd:52-69
+  * ``Simulation(usr_dir)``, ``.advance(tmax)``, ``.prim/.cons/.bfld`` --
+    src/simulation.pyx:55-86, driving vanleer.init_sim/advance
+    (src/integration/vanleer.pyx:16-179).
+The step itself runs in libbnzgpu.so on the GPU; there is no CPU fallback.
+"""
+import configparser
+import os
+from enum import IntEnum
+
+import numpy as np
+
+from . import _lib, problems
+
+
+# ---------------------------------------------------------------- config
+def read_param(section, key, dtype, usr_dir):
+    """Value of ``key`` in ``[section]`` of ``usr_dir/config.cfg``; dtype is
+    'f', 'i' or 's' (src/io/read_config.pyx:10-38).  Unlike the reference,
+    a missing file raises instead of silently returning 0."""
+    path = os.path.join(usr_dir, "config.cfg")
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"cannot read parameter {key}: unable to open {path}")
+    cfg = configparser.ConfigParser(inline_comment_prefixes=("#",))
+    cfg.read(path)
+    if dtype == "f":
+        return cfg.getfloat(section, key)
+    if dtype == "i":
+        return cfg.getint(section, key)
+    return cfg.get(section, key)
+
+
+class _Params:
+    """Parameter source: a usr_dir with config.cfg, or a dict of sections."""
+
+    def __init__(self, usr_dir=None, config=None):
+        self.usr_dir = usr_dir
+        self.config = config or {}
+
+    def get(self, section, key, dtype, default=None):
+        if key in self.config.get(section, {}):
+            v = self.config[section][key]
+        elif self.usr_dir is not None:
+            try:
+                v = read_param(section, key, dtype, self.usr_dir)
+            except (configparser.NoOptionError, configparser.NoSectionError):
+                if default is None:
+                    raise
+                v = default
+        elif default is not None:
+            v = default
+        else:
+            raise KeyError(f"[{section}] {key} is not set")
+        return {"f": float, "i": int, "s": str}[dtype](v)
+
+
+# ---------------------------------------------------------------- enums
+class TIntegr(IntEnum):          # src/integration/integrator.pxd:52-55
+    TINT_VL = 0
+    TINT_RK3 = 1
+
+
+class RSolver(IntEnum):          # src/integration/integrator.pxd:57-63
+    RS_HLL = 0
+    RS_HLLC = 1
+    RS_HLLD = 2
+    RS_HLLA = 3
+    RS_HLLT = 4
+
+
+class Reconstr(IntEnum):         # src/integration/integrator.pxd:65-69
+    RCN_CONST = 0
+    RCN_LINEAR = 1
+    RCN_WENO = 2
+    RCN_PARAB = 3
+
+
+# ---------------------------------------------------------------- integrator
+class BnzIntegr:
+    """Integrator settings (src/integration/integrator.pyx:205-300).
+
+    Unknown ``tintegr``/``rsolver``/``reconstr`` names fall back to VL/HLL/
+    linear as in the reference.  Combinations the reference leaves with a
+    NULL function pointer (hllc or hlla with a magnetic field) raise here
+    instead of crashing later."""
+
+    def __init__(self, params):
+        self.cour = params.get("computation", "cour", "f")
+        t = params.get("computation", "tintegr", "s", "vl")
+        self.tintegr = TIntegr.TINT_RK3 if t == "rk3" else TIntegr.TINT_VL
+        r = params.get("computation", "rsolver", "s", "hll")
+        self.rsolver = {"hll": RSolver.RS_HLL, "hllc": RSolver.RS_HLLC, "hlld": RSolver.RS_HLLD,
+                        "hllt": RSolver.RS_HLLT, "hlla": RSolver.RS_HLLA}.get(r, RSolver.RS_HLL)
+        c = params.get("computation", "reconstr", "s", "linear")
+        self.reconstr = {"const": Reconstr.RCN_CONST, "linear": Reconstr.RCN_LINEAR,
+                         "parab": Reconstr.RCN_PARAB, "weno": Reconstr.RCN_WENO}.get(
+                             c, Reconstr.RCN_LINEAR)
+        self.reconstr_order = {Reconstr.RCN_CONST: 0, Reconstr.RCN_LINEAR: 1,
+                               Reconstr.RCN_WENO: 1, Reconstr.RCN_PARAB: 2}[self.reconstr]
+        self.char_proj = params.get("computation", "charact_proj", "i", 0)
+        self.gam = params.get("physics", "gam", "f")
+        self.tmax = params.get("computation", "tmax", "f", 0.0)
+        self.dt_fms_ref = bool(params.get("computation", "dt_fms_ref", "i", 1))
+        self.time = 0.0
+        self.step = 0
+        self.dt = 0.0
+        if self.rsolver in (RSolver.RS_HLLC, RSolver.RS_HLLA):
+            raise ValueError(f"rsolver {r} has no magnetic-field solver bound "
+                             "(src/integration/integrator.pyx:253-268)")
+        if self.tintegr == TIntegr.TINT_RK3:
+            raise NotImplementedError("tintegr=rk3 is not part of the device path")
+        if self.reconstr in (Reconstr.RCN_PARAB, Reconstr.RCN_WENO):
+            raise NotImplementedError("parab/weno are commented out in src/coordinates/reconstr_c.c")
+        if self.char_proj:
+            raise NotImplementedError("charact_proj=1 is not supported (src/mhd/eos_c.c:61-63)")
+
+
+# ---------------------------------------------------------------- grid
+class BnzGrid:
+    """Sizes, ghosts, index ranges and boundary flags of one uniform Cartesian
+    block (src/coordinates/grid.pyx:107-231; src/bc/grid_bc.pyx:45-50)."""
+
+    def __init__(self, params, integr):
+        self.Nact = [params.get("computation", k, "i") for k in ("Nx", "Ny", "Nz")]
+        self.dims = 3 if self.Nact[2] > 1 else (2 if self.Nact[1] > 1 else 1)
+        if self.Nact[2] > 1 and self.Nact[1] == 1:
+            raise ValueError("Nz > 1 needs Ny > 1")
+        self.ng = 2 if integr.reconstr == Reconstr.RCN_CONST else 3
+        self.Ntot = [self.Nact[0] + 2 * self.ng,
+                     self.Nact[1] + 2 * self.ng if self.dims >= 2 else 1,
+                     self.Nact[2] + 2 * self.ng if self.dims >= 3 else 1]
+        g = self.ng
+        self.i1, self.i2 = g, self.Nact[0] + g - 1
+        self.j1, self.j2 = (g, self.Nact[1] + g - 1) if self.dims >= 2 else (0, 0)
+        self.k1, self.k2 = (g, self.Nact[2] + g - 1) if self.dims >= 3 else (0, 0)
+        self.lmin = [params.get("computation", k, "f", 0.0) for k in ("xmin", "ymin", "zmin")]
+        self.lmax = [params.get("computation", k, "f", 1.0) for k in ("xmax", "ymax", "zmax")]
+        self.bc_flags = [params.get("physics", k, "i", 0)
+                         for k in ("bc_x1", "bc_x2", "bc_y1", "bc_y2", "bc_z1", "bc_z2")]
+        for f in self.bc_flags:
+            if f not in (0, 1):
+                raise NotImplementedError("only periodic (0) and outflow (1) boundaries are built")
+        self.shape = problems.GridShape(self.Nact[0], self.Nact[1], self.Nact[2], self.ng,
+                                        tuple(self.lmin), tuple(self.lmax))
+
+
+# ---------------------------------------------------------------- problems
+PROBLEMS = {
+    "otvortex": lambda grid, p: problems.orszag_tang(grid.Nact[0], grid.ng)[1:],
+    "turbulence": lambda grid, p: problems.turbulence(
+        grid.Nact[0], grid.ng, seed_b=p.get("problem", "seed_b", "i", 1001),
+        seed_v=p.get("problem", "seed_v", "i", 1002))[1:],
+    "brio_wu": lambda grid, p: problems.brio_wu(grid.Nact[0], 0, grid.ng, transverse=0)[1:],
+    "blast": lambda grid, p: problems.blast2d(grid.Nact[0], grid.ng,
+                                              seed=p.get("problem", "seed", "i", 5005))[1:],
+}
+
+
+class Simulation:
+    """``Simulation(usr_dir)`` of src/simulation.pyx:55-86 on the GPU.
+
+    ``usr_dir`` holds config.cfg; ``config`` (dict of sections) overrides or
+    replaces it.  ``set_problem(grid) -> (prim, bfld)`` supplies initial
+    conditions over the full grid (active cells set); by default the
+    ``[problem] name`` entry picks one of ``PROBLEMS``."""
+
+    def __init__(self, usr_dir=None, config=None, set_problem=None, device=0):
+        p = _Params(usr_dir, config)
+        self.integr = BnzIntegr(p)
+        self.grid = BnzGrid(p, self.integr)
+        rs = {RSolver.RS_HLL: "hll", RSolver.RS_HLLD: "hlld", RSolver.RS_HLLT: "hllt"}[
+            self.integr.rsolver]
+        self.ctx = _lib.Context(*self.grid.Nact, rsolver=rs,
+                                reconstr="const" if self.integr.reconstr_order == 0 else "linear",
+                                bc=self.grid.bc_flags, lmin=self.grid.lmin, lmax=self.grid.lmax,
+                                gam=self.integr.gam, cour=self.integr.cour,
+                                dt_fms_ref=self.integr.dt_fms_ref, device=device)
+        if set_problem is None:
+            name = p.get("problem", "name", "s", "otvortex")
+            prim, bfld = PROBLEMS[name](self.grid, p)
+        else:
+            prim, bfld = set_problem(self.grid)
+        # init_sim: ghosts, interp_bc, ghosts of cell B, prim2cons
+        # (src/integration/vanleer.pyx:38-48)
+        self.ctx.upload(prim, bfld)
+        self.ctx.init_state()
+
+    def advance(self, tmax, max_steps=0):
+        """vanleer.advance: steps until time >= tmax (src/integration/
+        vanleer.pyx:87-167).  Returns the number of steps taken."""
+        n = self.ctx.advance(tmax, max_steps)
+        self.integr.time, self.integr.step, self.integr.dt = self.ctx.time()
+        return n
+
+    @property
+    def prim(self):
+        return self.ctx.download()[0]
+
+    @property
+    def bfld(self):
+        return self.ctx.download()[1]
+
+    @property
+    def cons(self):
+        return self.ctx.download(cons=True)[2]
+
+    def diagnostics(self):
+        return self.ctx.diag()
+
+    def close(self):
+        self.ctx.close()

@@ -1,0 +1,121 @@
+/*
+ * bnzgpu.h -- C ABI of the MI355X Van-Leer MHD step (libbnzgpu.so).
+ *
+ * Drop-in boundary for the path that sergekomarov/bonzo's
+ * src/integration/vanleer.pyx:59-179 (advance loop) and the kernels below it
+ * implement.  Plain C types only; host arrays are C-contiguous doubles in the
+ * reference's layout [var][k][j][i] (x fastest) over the full grid including
+ * ghosts (src/coordinates/grid.pyx:28-52):
+ *   prim : 9 rows  rho, vx, vy, vz, p, passive scalar, Bx, By, Bz (cell)
+ *   cons : 9 rows  rho, mx, my, mz, E, passive scalar, Bx, By, Bz (cell)
+ *   bfld : 3 rows  face-normal Bx, By, Bz at the lower face of each cell
+ * Every call returns BNZ_OK (0) or a negative BNZ_E* code; bnz_last_error()
+ * gives the message.  A context owns its device buffers and one HIP stream;
+ * calls are stream-ordered, a context is not thread-safe, and host-visible
+ * results are synchronised only by download, new_dt, advance, get_time and
+ * diag.
+ */
+#ifndef BNZGPU_H
+#define BNZGPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct bnz_ctx bnz_ctx;
+
+enum {
+  BNZ_OK = 0,
+  BNZ_EINVAL = -1,       /* bad argument or configuration                   */
+  BNZ_ENOMEM = -2,       /* device allocation failed                        */
+  BNZ_EHIP = -3,         /* HIP runtime error                               */
+  BNZ_EUNSUPPORTED = -4, /* selection the device path does not implement    */
+  BNZ_ESTATE = -5        /* call out of order (e.g. step before upload)     */
+};
+
+/* Riemann solvers: numbering of RSolver, src/integration/integrator.pxd:59-65 */
+enum { BNZ_RS_HLL = 0, BNZ_RS_HLLC = 1, BNZ_RS_HLLD = 2, BNZ_RS_HLLA = 3, BNZ_RS_HLLT = 4 };
+/* Reconstruction: numbering of Reconstr, src/integration/integrator.pxd:67-71 */
+enum { BNZ_RCN_CONST = 0, BNZ_RCN_LINEAR = 1, BNZ_RCN_WENO = 2, BNZ_RCN_PARAB = 3 };
+/* Boundary flags: src/bc/grid_bc.pyx:45-50 (0 periodic, 1 outflow, 2 reflect, 3 user) */
+enum { BNZ_BC_PERIODIC = 0, BNZ_BC_OUTFLOW = 1, BNZ_BC_REFLECT = 2, BNZ_BC_USER = 3 };
+/* Arithmetic of the step */
+enum { BNZ_PREC_FP64 = 0 };
+
+typedef struct {
+  int nx, ny, nz;       /* active cells (ny=nz=1 in 1D, nz=1 in 2D), [computation] Nx,Ny,Nz */
+  int rsolver;          /* BNZ_RS_*,  [computation] rsolver                */
+  int reconstr;         /* BNZ_RCN_*, [computation] reconstr              */
+  int bc[6];            /* x1,x2,y1,y2,z1,z2: [physics] bc_x1 ... bc_z2     */
+  double lmin[3];       /* [computation] xmin,ymin,zmin                    */
+  double lmax[3];       /* [computation] xmax,ymax,zmax                    */
+  double gam;           /* [physics] gam                                   */
+  double cour;          /* [computation] cour                              */
+  int dt_fms_ref;       /* 1: dt with the x fast speed on every axis, as
+                           src/integration/new_dt.pyx:76-91 does; 0: per axis */
+  int precision;        /* BNZ_PREC_*                                      */
+  int device;           /* HIP device ordinal                              */
+} bnz_params;
+
+typedef struct {
+  double mass, mom[3], energy, scalar; /* sums over active cells times cell volume */
+  double divb_max;                     /* max |div B| over active cells (code units) */
+  double bmax;                         /* max |B_cell| over active cells             */
+  double pmin, rhomin;                 /* failure detection: minima over active cells */
+  long nonfinite;                      /* active cells with a non-finite prim value  */
+} bnz_diag_out;
+
+/* Replaces BnzGrid/BnzIntegr construction (src/coordinates/grid.pyx:59-82,
+   src/integration/integrator.pyx:23-28, init_method :205-291). */
+int bnz_create(const bnz_params *p, bnz_ctx **out);
+int bnz_destroy(bnz_ctx *ctx);
+const char *bnz_last_error(const bnz_ctx *ctx);
+/* Ntot per axis and ng (src/coordinates/grid.pyx:129-161) */
+int bnz_get_shape(const bnz_ctx *ctx, int *ntot3, int *ng);
+
+/* Host <-> device copies of the grid arrays (GridData, grid.pyx:28-52) */
+int bnz_upload(bnz_ctx *ctx, const double *prim, const double *bfld);
+int bnz_download(bnz_ctx *ctx, double *prim, double *bfld, double *cons);
+
+/* init_sim after set_problem: ghost fill, interp_bc, ghost fill of cell B,
+   prim2cons (src/integration/vanleer.pyx:38-48) */
+int bnz_init_state(bnz_ctx *ctx);
+/* GridBC.apply_grid_bc over all NVAR (src/bc/grid_bc.pyx:323-339) */
+int bnz_fill_ghosts(bnz_ctx *ctx);
+/* new_dt (src/integration/new_dt.pyx:18-135): cour / max (|v|+c_f)/dx */
+int bnz_new_dt(bnz_ctx *ctx, double *dt);
+/* One VL step with a caller-supplied dt: predictor, corrector, ghost fill,
+   prim2cons (src/integration/vanleer.pyx:100-163) */
+int bnz_vl_step(bnz_ctx *ctx, double dt);
+/* vanleer.advance (src/integration/vanleer.pyx:87-167): steps with the
+   device-computed CFL dt, clipped to tmax, until t >= tmax or max_steps
+   (max_steps <= 0: no limit).  *nsteps gets the number taken. */
+int bnz_advance(bnz_ctx *ctx, double tmax, int max_steps, int *nsteps);
+/* nsteps steps with the device CFL dt and no tmax clip, fully
+   stream-ordered (no host synchronisation). */
+int bnz_run_steps(bnz_ctx *ctx, int nsteps);
+int bnz_get_time(bnz_ctx *ctx, double *time, int *step, double *last_dt);
+int bnz_set_time(bnz_ctx *ctx, double time, int step);
+/* Conservation, div B and failure diagnostics (src/diagnostics.pyx:18-125
+   prints the energies; the div B and positivity checks have no reference
+   counterpart). */
+int bnz_diag(bnz_ctx *ctx, bnz_diag_out *out);
+int bnz_synchronize(bnz_ctx *ctx);
+
+/* Kernel timing on the context's stream: when on, every launch is bracketed
+   by HIP events and its duration accumulated per kernel name. */
+int bnz_set_kernel_timing(bnz_ctx *ctx, int on);
+int bnz_kernel_timing(bnz_ctx *ctx, int idx, char *name, int name_len, double *total_ms,
+                      long *launches);
+
+/* Riemann solver on n faces, device pointers, layout [9][n] in the local
+   frame of the face (slot 6 = normal field, overwritten by bx), on the
+   default stream; synchronous.  Same seam as the pencil solvers of
+   src/mhd/fluxes_c.h:3-21 (hll_flux, hlld_flux). */
+int bnz_gpu_flux_pencils(int rsolver, const double *wl, const double *wr, const double *bx,
+                         double *flux, long n, double gam);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,130 @@
+"""CPU-side checks: the C-ABI library loads and exports what include/bnzgpu.h
+declares, the host selection API keeps the reference's semantics, and the
+initial-condition generators produce solenoidal face fields.  No GPU calls."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from bonzo_amd import _lib, problems as P
+from bonzo_amd.simulation import BnzIntegr, Reconstr, RSolver, _Params, read_param
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    txt = open(os.path.join(ROOT, "include", "bnzgpu.h")).read()
+    return sorted(set(re.findall(r"\b(bnz_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_header_symbols_exported():
+    syms = header_symbols()
+    assert len(syms) >= 15
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = set(re.findall(r"\bT (bnz_[a-z0-9_]+)", out))
+    missing = [s for s in syms if s not in exported]
+    assert not missing, missing
+    L = _lib.load()
+    for s in syms:
+        assert hasattr(L, s)
+    assert set(_lib.ABI_SYMBOLS) == set(syms)
+
+
+def test_params_struct_layout_matches_header(tmp_path):
+    """ctypes mirrors of bnz_params / bnz_diag_out agree with the C compiler's
+    layout of include/bnzgpu.h."""
+    src = tmp_path / "layout.c"
+    src.write_text(
+        '#include <stdio.h>\n#include <stddef.h>\n#include "bnzgpu.h"\n'
+        'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu\\n", sizeof(bnz_params),'
+        ' offsetof(bnz_params, lmin), offsetof(bnz_params, gam), offsetof(bnz_params, device),'
+        ' sizeof(bnz_diag_out), offsetof(bnz_diag_out, divb_max),'
+        ' offsetof(bnz_diag_out, nonfinite)); return 0;}\n')
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)],
+                   check=True)
+    vals = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True,
+                                           check=True).stdout.split()]
+    mine = [ctypes.sizeof(_lib.BnzParams), _lib.BnzParams.lmin.offset, _lib.BnzParams.gam.offset,
+            _lib.BnzParams.device.offset, ctypes.sizeof(_lib.BnzDiag),
+            _lib.BnzDiag.divb_max.offset, _lib.BnzDiag.nonfinite.offset]
+    assert vals == mine
+
+
+def test_create_rejects_bad_selection_without_touching_gpu():
+    L = _lib.load()
+    p = _lib.BnzParams()
+    p.nx, p.ny, p.nz = 8, 8, 8
+    p.rsolver = _lib.RS["hllc"]
+    p.reconstr = 1
+    p.gam, p.cour = 5.0 / 3.0, 0.4
+    h = ctypes.c_void_p()
+    rc = L.bnz_create(ctypes.byref(p), ctypes.byref(h))
+    assert rc == _lib.BNZ_EUNSUPPORTED
+    assert b"hydro-only" in L.bnz_last_error(h)
+    L.bnz_destroy(h)
+    p.rsolver = _lib.RS["hlld"]
+    p.gam = 1.0
+    rc = L.bnz_create(ctypes.byref(p), ctypes.byref(h))
+    assert rc == _lib.BNZ_EINVAL
+    L.bnz_destroy(h)
+
+
+def test_selection_semantics_follow_reference(tmp_path):
+    cfg = tmp_path / "config.cfg"
+    cfg.write_text("[computation]\nNx: 16\nNy: 1\nNz: 1\ntmax: 0.1\ncour: 0.4\n"
+                   "tintegr: nonsense\nrsolver: weird\nreconstr: foo\ncharact_proj: 0\n"
+                   "[physics]\ngam: 1.4\n")
+    assert read_param("computation", "Nx", "i", str(tmp_path)) == 16
+    integr = BnzIntegr(_Params(str(tmp_path)))
+    # unknown names fall back to VL / HLL / linear (integrator.pyx:220-242)
+    assert integr.rsolver == RSolver.RS_HLL
+    assert integr.reconstr == Reconstr.RCN_LINEAR and integr.reconstr_order == 1
+    with pytest.raises(ValueError):
+        BnzIntegr(_Params(config={"computation": {"cour": 0.4, "rsolver": "hllc"},
+                                  "physics": {"gam": 1.4}}))
+    with pytest.raises(NotImplementedError):
+        BnzIntegr(_Params(config={"computation": {"cour": 0.4, "tintegr": "rk3"},
+                                  "physics": {"gam": 1.4}}))
+
+
+def _divb(s, bfld):
+    g = s.ng
+    nx, ny, nz = s.nact
+    sk, sj, si = s.active()
+    d = (bfld[0][sk, sj, g + 1:g + nx + 1] - bfld[0][sk, sj, si]) / s.dx(0)
+    if s.dims >= 2:
+        d = d + (np.roll(bfld[1][sk, sj, si], -1, axis=1) - bfld[1][sk, sj, si]) / s.dx(1)
+    if s.dims >= 3:
+        d = d + (np.roll(bfld[2][sk, sj, si], -1, axis=0) - bfld[2][sk, sj, si]) / s.dx(2)
+    return d
+
+
+def test_turbulence_ic_is_solenoidal_and_scaled():
+    s, prim, bfld = P.turbulence(32)
+    sk, sj, si = s.active()
+    # periodic wrap of the x faces for the check
+    b = bfld.copy()
+    b[0][sk, sj, s.ng + 32] = b[0][sk, sj, s.ng]
+    assert np.abs(_divb(s, b)).max() < 1e-11
+    v = prim[1:4][:, sk, sj, si]
+    assert abs(np.sqrt(np.mean(np.sum(v ** 2, axis=0))) - 0.5) < 1e-12
+    # solenoidal velocity: spectral divergence vanishes
+    vk = [np.fft.fftn(v[c]) for c in range(3)]
+    k = np.fft.fftfreq(32, d=1.0 / 32)
+    KZ, KY, KX = np.meshgrid(k, k, k, indexing="ij")
+    div = KX * vk[0] + KY * vk[1] + KZ * vk[2]
+    assert np.abs(div).max() < 1e-9 * np.abs(vk[0]).max()
+    assert np.allclose(prim[0][sk, sj, si], 1.0) and np.allclose(prim[4][sk, sj, si], 0.6)
+
+
+def test_orszag_tang_ic_is_solenoidal():
+    s, prim, bfld = P.orszag_tang(64)
+    sk, sj, si = s.active()
+    b = bfld.copy()
+    b[0][sk, sj, s.ng + 64] = b[0][sk, sj, s.ng]
+    assert np.abs(_divb(s, b)).max() < 1e-12
