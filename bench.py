@@ -68,6 +68,19 @@ def cpu_baseline(budget_s=15.0):
                       f"turbulence, {k} steps after 1 warm-up step, {el:.1f} s"}
 
 
+def copy_calibration(mib):
+    """A plain device-to-device copy of known size, five times, so that the
+    FETCH_SIZE/WRITE_SIZE counters can be scaled to bytes."""
+    import torch
+    n = mib * (1 << 20) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": 8 * n, "launches": 5}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,7 +88,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB mebibytes (rocprof counter scale)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the path is measured on one GPU's share "
               "(multi-GPU sharding is designed, not built)", file=sys.stderr)
