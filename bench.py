@@ -88,6 +88,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="skip the chunk autotuner")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB mebibytes (rocprof counter scale)")
     args = ap.parse_args()
@@ -101,13 +102,21 @@ def main():
 
     from bonzo_amd import problems
     from bonzo_amd._lib import Context
+    from bonzo_amd.autotune import autotune
 
     n = args.n
     s, prim, bfld = problems.turbulence(n)
     g = Context(n, n, n, rsolver="hlld", reconstr="linear", cour=0.4)
     g.upload(prim, bfld)
-    del prim, bfld
     g.init_state()
+    tuned, table = (0, 0, 0), {}
+    if not args.no_tune:
+        # tuning steps advance the state: restart from the initial condition
+        tuned, table = autotune(g)
+        g.upload(prim, bfld)
+        g.init_state()
+        g.set_time(0.0, 0)
+    del prim, bfld
     g.run_steps(args.warmup)
     g.synchronize()
 
@@ -148,7 +157,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"3D decaying MHD turbulence {n}^3 periodic, VL + PLM + HLLD + CT, "
                                "FP64 plain path, CFL 0.4, seeded k^-2 fields",
-                   "n": n, "precision": "fp64", "rsolver": "hlld", "reconstr": "linear"},
+                   "n": n, "precision": "fp64", "rsolver": "hlld", "reconstr": "linear",
+                   "chunks": {"flux": tuned[0], "emf": tuned[1], "update": tuned[2]}},
         "roofline": {"bound": "hbm", "kernel": dname, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "avg_launch_ms": davg_ms, "algorithmic_bytes_per_launch": bytes_per_launch},
@@ -156,6 +166,7 @@ def main():
                           "frac_of_8TBs": step_achieved / HBM_PEAK_GBS},
         "kernels_ms_per_launch": {k: v[0] / v[1] for k, v in sorted(kt.items())},
         "divb_max": d["divb_max"],
+        "autotune_ms": {str(c): {k: round(v, 3) for k, v in t.items()} for c, t in table.items()},
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()

@@ -21,7 +21,7 @@ ABI_SYMBOLS = (
     "bnz_create", "bnz_destroy", "bnz_last_error", "bnz_get_shape", "bnz_upload",
     "bnz_download", "bnz_init_state", "bnz_fill_ghosts", "bnz_new_dt", "bnz_vl_step",
     "bnz_advance", "bnz_run_steps", "bnz_get_time", "bnz_set_time", "bnz_diag",
-    "bnz_synchronize", "bnz_set_kernel_timing", "bnz_kernel_timing", "bnz_gpu_flux_pencils",
+    "bnz_synchronize", "bnz_set_tuning", "bnz_set_kernel_timing", "bnz_kernel_timing", "bnz_gpu_flux_pencils",
 )
 
 
@@ -81,6 +81,7 @@ def load():
     L.bnz_set_time.argtypes = [vp, ctypes.c_double, ctypes.c_int]
     L.bnz_diag.argtypes = [vp, ctypes.POINTER(BnzDiag)]
     L.bnz_set_kernel_timing.argtypes = [vp, ctypes.c_int]
+    L.bnz_set_tuning.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     L.bnz_kernel_timing.argtypes = [vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_int, _dp,
                                     ctypes.POINTER(ctypes.c_long)]
     L.bnz_gpu_flux_pencils.argtypes = [ctypes.c_int, vp, vp, vp, vp, ctypes.c_long,
@@ -195,6 +196,9 @@ class Context:
         return dict(mass=d.mass, mom=tuple(d.mom), energy=d.energy, scalar=d.scalar,
                     divb_max=d.divb_max, bmax=d.bmax, pmin=d.pmin, rhomin=d.rhomin,
                     nonfinite=d.nonfinite)
+
+    def set_tuning(self, flux_chunk=0, emf_chunk=0, update_chunk=0):
+        self._check(load().bnz_set_tuning(self._h, flux_chunk, emf_chunk, update_chunk))
 
     def set_kernel_timing(self, on):
         self._check(load().bnz_set_kernel_timing(self._h, int(bool(on))))

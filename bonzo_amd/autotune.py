@@ -1,0 +1,46 @@
+"""Autotuner for the marching-chunk lengths of the 3D kernels.
+
+Each candidate chunk length (0 = one thread per face/edge/cell) is applied
+to all three kernel classes, a few VL steps are run with per-launch HIP-event
+timing on the context's stream, and each class keeps the candidate with the
+smallest summed time:
+  flux   -- y and z Godunov-flux kernels (both stages)
+  emf    -- edge EMF kernels
+  update -- stage updates plus the CFL reduction (chunk > 0 fuses it into
+            the corrector update, so the standalone dt_partial kernel counts)
+The runs advance the context's state: callers tune first and then upload
+their initial state again, or tune a scratch context of the same shape.
+"""
+CANDIDATES = (0, 2, 4, 8, 16, 32)
+
+
+def _class_times(kt):
+    out = {"flux": 0.0, "emf": 0.0, "update": 0.0}
+    for name, (tot, n) in kt.items():
+        if name.startswith("flux_y") or name.startswith("flux_z"):
+            out["flux"] += tot
+        elif name.startswith("emf"):
+            out["emf"] += tot
+        elif name.startswith("update") or name.startswith("dt_partial"):
+            out["update"] += tot
+    return out
+
+
+def autotune(ctx, steps=2, candidates=CANDIDATES):
+    """Pick (flux_chunk, emf_chunk, update_chunk) for ``ctx`` and apply it.
+
+    Returns (best, table) where table[c] holds the per-class milliseconds of
+    candidate c.  The context must hold an initialised state."""
+    table = {}
+    for c in candidates:
+        ctx.set_tuning(c, c, c)
+        ctx.run_steps(1)                      # first launch of a shape pays setup
+        ctx.synchronize()
+        ctx.set_kernel_timing(True)
+        ctx.run_steps(steps)
+        ctx.synchronize()
+        ctx.set_kernel_timing(False)
+        table[c] = _class_times(ctx.kernel_timing())
+    best = tuple(min(candidates, key=lambda c: table[c][k]) for k in ("flux", "emf", "update"))
+    ctx.set_tuning(*best)
+    return best, table

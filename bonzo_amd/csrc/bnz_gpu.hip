@@ -15,6 +15,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -52,6 +53,17 @@ struct Box {
   int n[3];
   __host__ __device__ long total() const { return (long)n[0] * n[1] * n[2]; }
 };
+
+// Workgroups are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8).
+// Renumber them so that each XCD owns one contiguous run of logical blocks:
+// neighbouring rows and planes of a stencil then share that XCD's L2.
+constexpr int NXCD = 8;
+__device__ __forceinline__ long xcd_block() {
+  const long b = blockIdx.x, nb = gridDim.x;
+  const long q = nb / NXCD, r = nb % NXCD;
+  const long x = b % NXCD, idx = b / NXCD;
+  return x * q + (x < r ? x : r) + idx;
+}
 
 __device__ __forceinline__ void box_idx(const Box &b, long t, int &i, int &j, int &k) {
   int nx = b.n[0], ny = b.n[1];
@@ -97,7 +109,7 @@ template <int DIR, int ORDER, int RS>
 __global__ __launch_bounds__(BLK) void k_flux(Geo g, const double *__restrict__ w,
                                               const double *__restrict__ bf,
                                               double *__restrict__ fl, Box box) {
-  long t = (long)blockIdx.x * BLK + threadIdx.x;
+  long t = xcd_block() * BLK + threadIdx.x;
   if (t >= box.total()) return;
   int i, j, k;
   box_idx(box, t, i, j, k);
@@ -155,7 +167,7 @@ __global__ __launch_bounds__(BLK) void k_emf(Geo g, const double *__restrict__ w
                                              const double *__restrict__ fy,
                                              const double *__restrict__ fz,
                                              double *__restrict__ ee, Box box) {
-  long t = (long)blockIdx.x * BLK + threadIdx.x;
+  long t = xcd_block() * BLK + threadIdx.x;
   if (t >= box.total()) return;
   int i, j, k;
   box_idx(box, t, i, j, k);
@@ -247,7 +259,7 @@ __global__ __launch_bounds__(BLK) void k_update(Geo g, const double *w0, const d
                                                 const double *__restrict__ b0,
                                                 double *__restrict__ b1, double *w1, Lims L,
                                                 const double *__restrict__ dts) {
-  long t = (long)blockIdx.x * BLK + threadIdx.x;
+  long t = xcd_block() * BLK + threadIdx.x;
   const long ncell = (long)g.nt[0] * g.nt[1] * g.nt[2];
   if (t >= ncell) return;
   const int i = (int)(t % g.nt[0]);
@@ -325,6 +337,315 @@ __global__ __launch_bounds__(BLK) void k_update(Geo g, const double *w0, const d
 }
 
 // --------------------------------------------------------------------------
+// Marching variants for 3D grids.  A workgroup is 64 x-lanes by 4 rows of
+// one transverse axis; each thread walks a chunk of the marching axis and
+// keeps the stencil values it shares with the next cell in registers, so a
+// cell is read once and reconstructed once per sweep.  Values and operation
+// order are those of the per-face / per-cell kernels above.
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+struct MarchMap {
+  int ii, tt, m0, m1;
+};
+
+// Workgroup -> (x block, transverse block, chunk), XCD-major like xcd_block().
+__device__ __forceinline__ bool march_map(int nx, int nt, int nm, int chunk, MarchMap &mm) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int nbi = (nx + 63) / 64, nch = (nm + chunk - 1) / chunk;
+  long L = xcd_block();
+  const int bc = (int)(L % nch);  // neighbouring chunks run back to back
+  L /= nch;
+  const int bi = (int)(L % nbi);
+  const int bt = (int)(L / nbi);
+  mm.ii = bi * 64 + lane;
+  mm.tt = bt * 4 + wv;
+  mm.m0 = bc * chunk;
+  mm.m1 = min(mm.m0 + chunk, nm);
+  return mm.ii < nx && mm.tt < nt && mm.m0 < nm;
+}
+
+// Godunov fluxes along DIR (1: y, 2: z) marching over the faces of one
+// (x, transverse) column.  box: face index ranges as for k_flux.
+template <int DIR, int ORDER, int RS>
+__global__ __launch_bounds__(BLK) void k_flux_march(Geo g, const double *__restrict__ w,
+                                                    const double *__restrict__ bf,
+                                                    double *__restrict__ fl, Box box, int chunk) {
+  constexpr int TD = (DIR == 1) ? 2 : 1;
+  MarchMap mm;
+  if (!march_map(box.n[0], box.n[TD], box.n[DIR], chunk, mm)) return;
+  int idx[3];
+  idx[0] = box.lo[0] + mm.ii;
+  idx[TD] = box.lo[TD] + mm.tt;
+  idx[DIR] = box.lo[DIR] + mm.m0;
+  long c = ((long)idx[2] * g.nt[1] + idx[1]) * g.nt[0] + idx[0];
+  const long sd = (DIR == 1) ? g.sy : g.sz;
+  const long sv = g.sv;
+
+  double wm[NMODE], w0[NMODE], dm[NMODE];
+#pragma unroll
+  for (int n = 0; n < NMODE; ++n) {
+    if (n == BX) continue;
+    const double *p = w + (long)perm<DIR>(n) * sv + c;
+    wm[n] = p[-sd];
+    if (ORDER == 1) {
+      const double wm2 = p[-2 * sd];
+      w0[n] = p[0];
+      dm[n] = 0.5 * vl_lim(wm[n] - wm2, w0[n] - wm[n]);
+    }
+  }
+  for (int m = mm.m0; m < mm.m1; ++m, c += sd) {
+    double wl[NMODE], wr[NMODE];
+#pragma unroll
+    for (int n = 0; n < NMODE; ++n) {
+      if (n == BX) continue;
+      const double *p = w + (long)perm<DIR>(n) * sv + c;
+      if (ORDER == 0) {
+        const double wc = p[0];
+        wl[n] = wm[n];
+        wr[n] = wc;
+        wm[n] = wc;
+      } else {
+        const double wp = p[sd];
+        const double d0 = 0.5 * vl_lim(w0[n] - wm[n], wp - w0[n]);
+        wl[n] = wm[n] + dm[n];
+        wr[n] = w0[n] - d0;
+        wm[n] = w0[n];
+        w0[n] = wp;
+        dm[n] = d0;
+      }
+    }
+    const double bn = bf[(long)DIR * sv + c];
+    wl[BX] = bn;
+    wr[BX] = bn;
+    double f[NMODE];
+    const int reg = (RS == BNZ_RS_HLLD) ? flux_hlld(f, wl, wr, g.gam, g.gamm1i)
+                                        : flux_hll(f, wl, wr, g.gam, g.gamm1i);
+    if (reg == 0) continue;
+#pragma unroll
+    for (int n = 0; n < NMODE; ++n) {
+      if (n == BX) continue;
+      fl[(long)perm<DIR>(n) * sv + c] = f[n];
+    }
+  }
+}
+
+// interp_ee1 3D branch marching in z; box: edge ranges as for k_emf.
+__global__ __launch_bounds__(BLK) void k_emf_march(Geo g, const double *__restrict__ w,
+                                                   const double *__restrict__ fx,
+                                                   const double *__restrict__ fy,
+                                                   const double *__restrict__ fz,
+                                                   double *__restrict__ ee, Box box, int chunk) {
+  MarchMap mm;
+  if (!march_map(box.n[0], box.n[1], box.n[2], chunk, mm)) return;
+  const int i = box.lo[0] + mm.ii, j = box.lo[1] + mm.tt;
+  int k = box.lo[2] + mm.m0;
+  const long sv = g.sv, sy = g.sy, sz = g.sz;
+  long c = ((long)k * g.nt[1] + j) * g.nt[0] + i;
+  // plane k-1 values carried between iterations
+  double pfyz = fy[BZ * sv + c - sz], pfxz = fx[BZ * sv + c - sz];
+  double pe0a = ec_comp(w, sv, c - sz - sy, 0), pe0b = ec_comp(w, sv, c - sz, 0);
+  double pe1a = ec_comp(w, sv, c - sz - 1, 1), pe1b = ec_comp(w, sv, c - sz, 1);
+  for (int m = mm.m0; m < mm.m1; ++m, c += sz) {
+    const double fyz = fy[BZ * sv + c], fxz = fx[BZ * sv + c];
+    const double e0a = ec_comp(w, sv, c - sy, 0), e0b = ec_comp(w, sv, c, 0);
+    const double e1a = ec_comp(w, sv, c - 1, 1), e1b = ec_comp(w, sv, c, 1);
+    ee[0 * sv + c] = (0.5 * (-pfyz - fyz + fz[BY * sv + c - sy] + fz[BY * sv + c]) -
+                      0.25 * (e0a + e0b + pe0a + pe0b));
+    ee[1 * sv + c] = (0.5 * (-fz[BX * sv + c - 1] - fz[BX * sv + c] + pfxz + fxz) -
+                      0.25 * (e1a + e1b + pe1a + pe1b));
+    ee[2 * sv + c] = (0.5 * (-fx[BY * sv + c - sy] - fx[BY * sv + c] + fy[BX * sv + c - 1] +
+                             fy[BX * sv + c]) -
+                      0.25 * (ec_comp(w, sv, c - 1, 2) + ec_comp(w, sv, c, 2) +
+                              ec_comp(w, sv, c - sy - 1, 2) + ec_comp(w, sv, c - sy, 2)));
+    pfyz = fyz;
+    pfxz = fxz;
+    pe0a = e0a;
+    pe0b = e0b;
+    pe1a = e1a;
+    pe1b = e1b;
+  }
+}
+
+__device__ __forceinline__ double cds_cell(const Geo &g, const double *wv, int fms_ref, int i,
+                                           int j, int k) {
+  const double cx = fabs(wv[VX]) + fms(wv, g.gam);
+  double cy = 0., cz = 0.;
+  if (g.dims >= 2) {
+    double wy[NMODE];
+#pragma unroll
+    for (int n = 0; n < NMODE; ++n) wy[n] = wv[n];
+    if (!fms_ref) { wy[BX] = wv[BY]; wy[BY] = wv[BX]; }
+    cy = fabs(wv[VY]) + fms(wy, g.gam);
+  }
+  if (g.dims >= 3) {
+    double wz[NMODE];
+#pragma unroll
+    for (int n = 0; n < NMODE; ++n) wz[n] = wv[n];
+    if (!fms_ref) { wz[BX] = wv[BZ]; wz[BZ] = wv[BX]; }
+    cz = fabs(wv[VZ]) + fms(wz, g.gam);
+  }
+  double cds = fmax(fmax(cx / g.dlf[0][i], cy / g.dlf[1][j]), cz / g.dlf[2][k]);
+  return (cds > 0.) ? cds : 0.;  // NaN never wins the reference's `>` test
+}
+
+// Stage update for 3D grids marching in z over the whole grid (same work
+// and predicates as k_update).  DTRED: also reduce the next step's CFL
+// maximum over the active cells of the new prim into part[blockIdx.x].
+template <bool PRED, bool DTRED>
+__global__ __launch_bounds__(BLK) void k_update_march(
+    Geo g, const double *w0, const double *__restrict__ fx, const double *__restrict__ fy,
+    const double *__restrict__ fz, const double *__restrict__ ee, const double *__restrict__ b0,
+    double *__restrict__ b1, double *w1, Lims L, const double *__restrict__ dts, int chunk,
+    Box act, int fms_ref, double *__restrict__ part) {
+  __shared__ double red[BLK / 64];
+  double cdsmax = 0.;
+  MarchMap mm;
+  const bool live = march_map(g.nt[0], g.nt[1], g.nt[2], chunk, mm);
+  if (live) {
+    const long sv = g.sv, sy = g.sy, sz = g.sz;
+    const int i = mm.ii, j = mm.tt;
+    const double dt = PRED ? 0.5 * dts[0] : dts[0];
+    const bool hasI = i + 1 < g.nt[0], hasJ = j + 1 < g.nt[1];
+    const bool inI = i >= L.lo[0] && i <= L.hi[0];
+    const bool inJ = j >= L.lo[1] && j <= L.hi[1];
+    const bool inIf = i >= L.lo[0] && i <= L.hi[0] + 1;
+    const bool inJf = j >= L.lo[1] && j <= L.hi[1] + 1;
+    int k = mm.m0;
+    long c = ((long)k * g.nt[1] + j) * g.nt[0] + i;
+    // values at face/edge level k, carried from the previous iteration
+    double fzl[6], e0l = ee[c], e0l1 = hasJ ? ee[c + sy] : 0.;
+    double e1l = ee[sv + c], e1l1 = hasI ? ee[sv + c + 1] : 0., bzl = b0[2 * sv + c];
+#pragma unroll
+    for (int n = 0; n < 6; ++n) fzl[n] = fz[n * sv + c];
+    const double dlx = g.dlf[0][i], dly = g.dlf[1][j];
+    for (int m = mm.m0; m < mm.m1; ++m, ++k, c += sz) {
+      const bool hasK = k + 1 < g.nt[2];
+      double fzu[6], e0u = 0., e0u1 = 0., e1u = 0., e1u1 = 0., bzu = 0.;
+      if (hasK) {
+#pragma unroll
+        for (int n = 0; n < 6; ++n) fzu[n] = fz[n * sv + c + sz];
+        e0u = ee[c + sz];
+        e0u1 = hasJ ? ee[c + sz + sy] : 0.;
+        e1u = ee[sv + c + sz];
+        e1u1 = hasI ? ee[sv + c + sz + 1] : 0.;
+        bzu = b0[2 * sv + c + sz];
+      } else {
+#pragma unroll
+        for (int n = 0; n < 6; ++n) fzu[n] = 0.;
+      }
+      const bool inK = k >= L.lo[2] && k <= L.hi[2];
+      const bool inKf = k >= L.lo[2] && k <= L.hi[2] + 1;
+      const bool cell = inI && inJ && inK;
+      const bool fX = inIf && inJ && inK, fY = inI && inJf && inK, fZ = inI && inJ && inKf;
+      const double dlz = g.dlf[2][k];
+      const double e2 = ee[2 * sv + c];
+      const double e2i = hasI ? ee[2 * sv + c + 1] : 0.;
+      const double e2j = hasJ ? ee[2 * sv + c + sy] : 0.;
+      const double e2ij = (hasI && hasJ) ? ee[2 * sv + c + sy + 1] : 0.;
+      double bxl = 0., byl = 0., bzlo = 0.;
+      // advance_b (ct.pyx:15-91) for the lower faces of this cell
+      if (fX || cell) {
+        const double dtda = dt / (dly * dlz);
+        double v = b0[c];
+        v -= dtda * (dlz * e2j - dlz * e2);
+        v += dtda * (dly * e1u - dly * e1l);
+        bxl = v;
+        if (fX) b1[c] = v;
+      }
+      if (fY || cell) {
+        const double dtda = dt / (dlx * dlz);
+        double v = b0[sv + c] + dtda * (dlz * e2i - dlz * e2);
+        v -= dtda * (dlx * e0u - dlx * e0l);
+        byl = v;
+        if (fY) b1[sv + c] = v;
+      }
+      if (fZ || cell) {
+        const double dtda = dt / (dlx * dly);
+        double v = bzl - dtda * (dly * e1l1 - dly * e1l);
+        v += dtda * (dlx * e0l1 - dlx * e0l);
+        bzlo = v;
+        if (fZ) b1[2 * sv + c] = v;
+      }
+      if (!cell) {
+        if (PRED) {
+#pragma unroll
+          for (int n = 0; n < NMODE; ++n) w1[n * sv + c] = w0[n * sv + c];
+        }
+      } else {
+        double wv[NMODE], u[NMODE];
+#pragma unroll
+        for (int n = 0; n < NMODE; ++n) wv[n] = w0[n * sv + c];
+        prim2cons(u, wv, g.gamm1i);
+        const double dtdv = dt / (dlx * dly * dlz);
+        const double ax = dly * dlz, ay = dlx * dlz, az = dlx * dly;
+#pragma unroll
+        for (int n = 0; n < NMODE - 3; ++n) {
+          double v = u[n] - dtdv * (ax * fx[n * sv + c + 1] - ax * fx[n * sv + c]);
+          v -= dtdv * (ay * fy[n * sv + c + sy] - ay * fy[n * sv + c]);
+          v -= dtdv * (az * fzu[n] - az * fzl[n]);
+          u[n] = v;
+        }
+        // upper faces for interp_bc (same arithmetic as the lower ones)
+        double bxu, byu, bzu1;
+        {
+          const double dtda = dt / (dly * dlz);
+          double v = b0[c + 1];
+          v -= dtda * (dlz * e2ij - dlz * e2i);
+          v += dtda * (dly * e1u1 - dly * e1l1);
+          bxu = v;
+        }
+        {
+          const double dtda = dt / (dlx * dlz);
+          double v = b0[sv + c + sy] + dtda * (dlz * e2ij - dlz * e2j);
+          v -= dtda * (dlx * e0u1 - dlx * e0l1);
+          byu = v;
+        }
+        {
+          const double dtda = dt / (dlx * dly);
+          double v = bzu - dtda * (dly * e1u1 - dly * e1u);
+          v += dtda * (dlx * e0u1 - dlx * e0u);
+          bzu1 = v;
+        }
+        u[BX] = (g.icm[0][i] * bxl + g.icp[0][i] * bxu) * g.dlfi[0][i];
+        u[BY] = (g.icm[1][j] * byl + g.icp[1][j] * byu) * g.dlfi[1][j];
+        u[BZ] = (g.icm[2][k] * bzlo + g.icp[2][k] * bzu1) * g.dlfi[2][k];
+        cons2prim(wv, u, g.gam);
+#pragma unroll
+        for (int n = 0; n < NMODE; ++n) w1[n * sv + c] = wv[n];
+        if (DTRED) {
+          if (i >= act.lo[0] && i < act.lo[0] + act.n[0] && j >= act.lo[1] &&
+              j < act.lo[1] + act.n[1] && k >= act.lo[2] && k < act.lo[2] + act.n[2])
+            cdsmax = fmax(cdsmax, cds_cell(g, wv, fms_ref, i, j, k));
+        }
+      }
+#pragma unroll
+      for (int n = 0; n < 6; ++n) fzl[n] = fzu[n];
+      e0l = e0u;
+      e0l1 = e0u1;
+      e1l = e1u;
+      e1l1 = e1u1;
+      bzl = bzu;
+    }
+  }
+  if (DTRED) {
+    cdsmax = wave_max(cdsmax);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = cdsmax;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double mx = red[0];
+#pragma unroll
+      for (int q = 1; q < BLK / 64; ++q) mx = fmax(mx, red[q]);
+      part[blockIdx.x] = mx;
+    }
+  }
+}
+
+// --------------------------------------------------------------------------
 // Ghost fill (src/bc/grid_bc.pyx:612-623, util_bc.pyx).  The z, y, x copy
 // sequence composes per-axis index maps (periodic wrap, outflow clamp), so
 // each ghost cell gathers directly from the active cell its maps name.
@@ -332,7 +653,7 @@ __global__ __launch_bounds__(BLK) void k_ghost(Geo g, double *w, double *b,
                                                const int *__restrict__ mx,
                                                const int *__restrict__ my,
                                                const int *__restrict__ mz) {
-  long t = (long)blockIdx.x * BLK + threadIdx.x;
+  long t = xcd_block() * BLK + threadIdx.x;
   const long ncell = (long)g.nt[0] * g.nt[1] * g.nt[2];
   if (t >= ncell) return;
   const int i = (int)(t % g.nt[0]);
@@ -351,7 +672,7 @@ __global__ __launch_bounds__(BLK) void k_ghost(Geo g, double *w, double *b,
 
 // interp_bc over the active cells into prim (init only, vanleer.pyx:44)
 __global__ __launch_bounds__(BLK) void k_interp_active(Geo g, double *w, const double *b, Box box) {
-  long t = (long)blockIdx.x * BLK + threadIdx.x;
+  long t = xcd_block() * BLK + threadIdx.x;
   if (t >= box.total()) return;
   int i, j, k;
   box_idx(box, t, i, j, k);
@@ -370,7 +691,7 @@ __global__ __launch_bounds__(BLK) void k_interp_active(Geo g, double *w, const d
 
 // cons = prim2cons(prim) over all cells (vanleer.pyx:163)
 __global__ __launch_bounds__(BLK) void k_prim2cons(Geo g, const double *w, double *u) {
-  long t = (long)blockIdx.x * BLK + threadIdx.x;
+  long t = xcd_block() * BLK + threadIdx.x;
   const long ncell = (long)g.nt[0] * g.nt[1] * g.nt[2];
   if (t >= ncell) return;
   double wv[NMODE], uv[NMODE];
@@ -384,16 +705,11 @@ __global__ __launch_bounds__(BLK) void k_prim2cons(Geo g, const double *w, doubl
 // --------------------------------------------------------------------------
 // CFL dt (new_dt.pyx:55-117): per-block max of max(cx/dx, cy/dy, cz/dz) over
 // active cells, wavefront shuffles then LDS across the block's 4 waves.
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
-  return v;
-}
 
 __global__ __launch_bounds__(BLK) void k_dt_partial(Geo g, const double *__restrict__ w, Box box,
                                                     int fms_ref, double *__restrict__ part) {
   __shared__ double red[BLK / 64];
-  long t = (long)blockIdx.x * BLK + threadIdx.x;
+  long t = xcd_block() * BLK + threadIdx.x;
   double cds = 0.;
   if (t < box.total()) {
     int i, j, k;
@@ -438,15 +754,23 @@ __global__ __launch_bounds__(BLK) void k_dt_final(const double *__restrict__ par
                                                   double cour, double *dev, int use_fixed,
                                                   double fixed_dt) {
   __shared__ double red[BLK / 64];
-  double m = 0.;
-  for (int q = threadIdx.x; q < npart; q += BLK) m = fmax(m, part[q]);
-  m = wave_max(m);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  double m[8] = {0., 0., 0., 0., 0., 0., 0., 0.};
+  int q = threadIdx.x;
+  for (; q + 7 * BLK < npart; q += 8 * BLK) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) m[u] = fmax(m[u], part[q + u * BLK]);
+  }
+  for (; q < npart; q += BLK) m[0] = fmax(m[0], part[q]);
+  double mm = m[0];
+#pragma unroll
+  for (int u = 1; u < 8; ++u) mm = fmax(mm, m[u]);
+  mm = wave_max(mm);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mm;
   __syncthreads();
   if (threadIdx.x == 0) {
-    double mm = red[0];
-    for (int q = 1; q < BLK / 64; ++q) mm = fmax(mm, red[q]);
-    double dt = use_fixed ? fixed_dt : cour / mm;
+    double x = red[0];
+    for (int r = 1; r < BLK / 64; ++r) x = fmax(x, red[r]);
+    double dt = use_fixed ? fixed_dt : cour / x;
     const double tnow = dev[1], tmax = dev[2];
     if (!use_fixed && tmax >= 0. && tnow + dt > tmax) dt = tmax - tnow;
     dev[0] = dt;
@@ -467,7 +791,7 @@ __global__ __launch_bounds__(BLK) void k_diag(Geo g, const double *__restrict__ 
                                               const double *__restrict__ b, Box box,
                                               double *__restrict__ part) {
   __shared__ double red[NDIAG][BLK / 64];
-  long t = (long)blockIdx.x * BLK + threadIdx.x;
+  long t = xcd_block() * BLK + threadIdx.x;
   double v[NDIAG];
 #pragma unroll
   for (int q = 0; q < NDIAG; ++q) v[q] = 0.;
@@ -523,7 +847,7 @@ template <int RS>
 __global__ __launch_bounds__(BLK) void k_pencil(const double *wl_, const double *wr_,
                                                 const double *bx, double *flux, long n,
                                                 double gam, double gamm1i) {
-  long t = (long)blockIdx.x * BLK + threadIdx.x;
+  long t = xcd_block() * BLK + threadIdx.x;
   if (t >= n) return;
   double wl[NMODE], wr[NMODE], f[NMODE];
 #pragma unroll
@@ -571,6 +895,10 @@ struct bnz_ctx {
   Geo g{};
   bool ready = false;
   bool timing = false;
+  // tuning: chunk lengths of the marching kernels (3D)
+  int flux_chunk = 0, emf_chunk = 0, update_chunk = 0;
+  // CFL partial maxima left by the last corrector for the next step
+  int dt_parts = 0;
   std::vector<KTime> ktimes;
   std::string err;
 };
@@ -590,6 +918,11 @@ int fail(bnz_ctx *c, int code, const std::string &msg) {
   } while (0)
 
 int blocks_for(long n) { return (int)((n + BLK - 1) / BLK); }
+
+// Workgroups of a marching kernel: 64 x-lanes x 4 transverse rows x chunks.
+int march_blocks(int nx, int nt, int nm, int chunk) {
+  return ((nx + 63) / 64) * ((nt + 3) / 4) * ((nm + chunk - 1) / chunk);
+}
 
 // Uniform Cartesian coordinates as src/coordinates/coord.pyx:201-257 and
 // src/coordinates/cartesian.pyx:18-43 compute them (one MPI block).
@@ -727,8 +1060,13 @@ void launch_fluxes(bnz_ctx *c, const double *w, const double *bf, const Lims &L)
     by.lo[1] = jl; by.n[1] = ju + 1 - jl + 1;
     by.lo[2] = (c->dims >= 3) ? kl - 1 : 0; by.n[2] = (c->dims >= 3) ? (ku + 1) - (kl - 1) + 1 : 1;
     Launch t(c, ORDER ? "flux_y_plm" : "flux_y_dc");
-    hipLaunchKernelGGL((k_flux<1, ORDER, RS>), dim3(blocks_for(by.total())), dim3(BLK), 0,
-                       c->stream, g, w, bf, c->fy, by);
+    if (c->dims >= 3 && c->flux_chunk > 0)
+      hipLaunchKernelGGL((k_flux_march<1, ORDER, RS>),
+                         dim3(march_blocks(by.n[0], by.n[2], by.n[1], c->flux_chunk)), dim3(BLK), 0,
+                         c->stream, g, w, bf, c->fy, by, c->flux_chunk);
+    else
+      hipLaunchKernelGGL((k_flux<1, ORDER, RS>), dim3(blocks_for(by.total())), dim3(BLK), 0,
+                         c->stream, g, w, bf, c->fy, by);
   }
   if (c->dims >= 3) {  // z faces k in [kl, ku+1], i in [il-1, iu+1], j in [jl-1, ju+1]
     Box bz;
@@ -736,8 +1074,13 @@ void launch_fluxes(bnz_ctx *c, const double *w, const double *bf, const Lims &L)
     bz.lo[1] = jl - 1; bz.n[1] = (ju + 1) - (jl - 1) + 1;
     bz.lo[2] = kl; bz.n[2] = ku + 1 - kl + 1;
     Launch t(c, ORDER ? "flux_z_plm" : "flux_z_dc");
-    hipLaunchKernelGGL((k_flux<2, ORDER, RS>), dim3(blocks_for(bz.total())), dim3(BLK), 0,
-                       c->stream, g, w, bf, c->fz, bz);
+    if (c->flux_chunk > 0)
+      hipLaunchKernelGGL((k_flux_march<2, ORDER, RS>),
+                         dim3(march_blocks(bz.n[0], bz.n[1], bz.n[2], c->flux_chunk)), dim3(BLK),
+                         0, c->stream, g, w, bf, c->fz, bz, c->flux_chunk);
+    else
+      hipLaunchKernelGGL((k_flux<2, ORDER, RS>), dim3(blocks_for(bz.total())), dim3(BLK), 0,
+                         c->stream, g, w, bf, c->fz, bz);
   }
 }
 
@@ -759,8 +1102,13 @@ void launch_emf(bnz_ctx *c, const double *w, const Lims &L, const char *name) {
   e.lo[1] = (c->dims >= 2) ? L.lo[1] : 0; e.n[1] = (c->dims >= 2) ? L.hi[1] + 1 - L.lo[1] + 1 : 1;
   e.lo[2] = (c->dims >= 3) ? L.lo[2] : 0; e.n[2] = (c->dims >= 3) ? L.hi[2] + 1 - L.lo[2] + 1 : 1;
   Launch t(c, name);
-  hipLaunchKernelGGL(k_emf, dim3(blocks_for(e.total())), dim3(BLK), 0, c->stream, c->g, w,
-                     c->fx, c->fy, c->fz, c->ee, e);
+  if (c->dims >= 3 && c->emf_chunk > 0)
+    hipLaunchKernelGGL(k_emf_march, dim3(march_blocks(e.n[0], e.n[1], e.n[2], c->emf_chunk)),
+                       dim3(BLK), 0, c->stream, c->g, w, c->fx, c->fy, c->fz, c->ee, e,
+                       c->emf_chunk);
+  else
+    hipLaunchKernelGGL(k_emf, dim3(blocks_for(e.total())), dim3(BLK), 0, c->stream, c->g, w,
+                       c->fx, c->fy, c->fz, c->ee, e);
 }
 
 void launch_ghost(bnz_ctx *c) {
@@ -770,9 +1118,10 @@ void launch_ghost(bnz_ctx *c) {
 }
 
 void launch_dt(bnz_ctx *c, int use_fixed, double fixed_dt) {
-  Box a = active_box(c);
-  const int nb = blocks_for(a.total());
-  {
+  int nb = c->dt_parts;
+  if (nb <= 0) {
+    Box a = active_box(c);
+    nb = blocks_for(a.total());
     Launch t(c, "dt_partial");
     hipLaunchKernelGGL(k_dt_partial, dim3(nb), dim3(BLK), 0, c->stream, c->g, c->w, a,
                        c->p.dt_fms_ref, c->part);
@@ -783,28 +1132,47 @@ void launch_dt(bnz_ctx *c, int use_fixed, double fixed_dt) {
 }
 
 // One VL step after the dt kernels (vanleer.pyx:100-163).
+void launch_update(bnz_ctx *c, bool pred, const Lims &L) {
+  double *w1 = pred ? c->ws : c->w;
+  if (c->dims >= 3 && c->update_chunk > 0) {
+    const int nb = march_blocks(c->nt[0], c->nt[1], c->nt[2], c->update_chunk);
+    Launch t(c, pred ? "update_pred" : "update_corr");
+    if (pred)
+      hipLaunchKernelGGL((k_update_march<true, false>), dim3(nb), dim3(BLK), 0, c->stream, c->g,
+                         c->w, c->fx, c->fy, c->fz, c->ee, c->b, c->bs, w1, L, c->dev,
+                         c->update_chunk, active_box(c), c->p.dt_fms_ref, c->part);
+    else
+      hipLaunchKernelGGL((k_update_march<false, true>), dim3(nb), dim3(BLK), 0, c->stream, c->g,
+                         c->w, c->fx, c->fy, c->fz, c->ee, c->b, c->bs, w1, L, c->dev,
+                         c->update_chunk, active_box(c), c->p.dt_fms_ref, c->part);
+    // the corrector leaves the next step's CFL partials in part[0..nb)
+    c->dt_parts = pred ? 0 : nb;
+  } else {
+    Launch t(c, pred ? "update_pred" : "update_corr");
+    if (pred)
+      hipLaunchKernelGGL(k_update<true>, dim3(blocks_for(c->ncell)), dim3(BLK), 0, c->stream,
+                         c->g, c->w, c->fx, c->fy, c->fz, c->ee, c->b, c->bs, w1, L, c->dev);
+    else
+      hipLaunchKernelGGL(k_update<false>, dim3(blocks_for(c->ncell)), dim3(BLK), 0, c->stream,
+                         c->g, c->w, c->fx, c->fy, c->fz, c->ee, c->b, c->bs, w1, L, c->dev);
+    c->dt_parts = 0;
+  }
+}
+
+// One VL step after the dt kernels (vanleer.pyx:100-163).
 void launch_step_body(bnz_ctx *c) {
-  // predictor: donor cell, dt/2, lims of order 0
+  // predictor: donor cell, dt/2, lims of order 0; ws and bs are new buffers
   Lims L0 = integr_lims(c, 0);
   launch_flux_dispatch(c, 0, c->w, c->b, L0);
   launch_emf(c, c->w, L0, "emf_pred");
-  {
-    Launch t(c, "update_pred");
-    hipLaunchKernelGGL(k_update<true>, dim3(blocks_for(c->ncell)), dim3(BLK), 0, c->stream,
-                       c->g, c->w, c->fx, c->fy, c->fz, c->ee, c->b, c->bs, c->ws, L0, c->dev);
-  }
-  // corrector: reconstruction order, dt, reads ws/bs; writes w in place and
-  // the new face field into the spare face buffer
+  launch_update(c, true, L0);
+  // corrector: reconstruction order, dt; prim is updated in place (each cell
+  // reads only its own w_n), b_{n+1} goes to the predictor's face buffer,
+  // which is dead once the corrector fluxes and EMFs exist
   Lims L1 = integr_lims(c, c->order);
   launch_flux_dispatch(c, c->order, c->ws, c->bs, L1);
   launch_emf(c, c->ws, L1, "emf_corr");
-  {
-    // b_{n+1} goes into a buffer that is not b_n; the predictor's bs is dead
-    // once the corrector fluxes and EMFs are computed, so it is reused.
-    Launch t(c, "update_corr");
-    hipLaunchKernelGGL(k_update<false>, dim3(blocks_for(c->ncell)), dim3(BLK), 0, c->stream,
-                       c->g, c->w, c->fx, c->fy, c->fz, c->ee, c->b, c->bs, c->w, L1, c->dev);
-  }
+  launch_update(c, false, L1);
   std::swap(c->b, c->bs);
   launch_ghost(c);
   hipLaunchKernelGGL(k_time_advance, dim3(1), dim3(64), 0, c->stream, c->dev);
@@ -906,7 +1274,8 @@ int bnz_create(const bnz_params *p, bnz_ctx **out) {
   const double dev0[8] = {0., 0., -1., 0., 0., 0., 0., 0.};
   HIPCHK(c, hipMemcpy(c->dev, dev0, sizeof(dev0), hipMemcpyHostToDevice));
   Box a = active_box(c);
-  c->npart_cap = (long)blocks_for(a.total()) * NDIAG + 64;
+  c->npart_cap = std::max((long)blocks_for(a.total()) * NDIAG + 64,
+                          (long)march_blocks(c->nt[0], c->nt[1], c->nt[2], 1));
   HIPCHK(c, hipMalloc(&c->part, c->npart_cap * sizeof(double)));
 
   Geo &g = c->g;
@@ -950,6 +1319,7 @@ int bnz_get_shape(const bnz_ctx *c, int *ntot3, int *ng) {
 
 int bnz_upload(bnz_ctx *c, const double *prim, const double *bfld) {
   if (!c || !c->stream) return BNZ_EINVAL;
+  c->dt_parts = 0;
   if (prim) HIPCHK(c, hipMemcpy(c->w, prim, NMODE * c->ncell * sizeof(double), hipMemcpyHostToDevice));
   if (bfld) HIPCHK(c, hipMemcpy(c->b, bfld, 3 * c->ncell * sizeof(double), hipMemcpyHostToDevice));
   return BNZ_OK;
@@ -980,6 +1350,7 @@ int bnz_fill_ghosts(bnz_ctx *c) {
 
 int bnz_init_state(bnz_ctx *c) {
   if (!c || !c->stream) return BNZ_EINVAL;
+  c->dt_parts = 0;
   launch_ghost(c);
   Box a = active_box(c);
   hipLaunchKernelGGL(k_interp_active, dim3(blocks_for(a.total())), dim3(BLK), 0, c->stream,
@@ -1070,6 +1441,7 @@ int bnz_set_time(bnz_ctx *c, double time, int step) {
 
 int bnz_diag(bnz_ctx *c, bnz_diag_out *out) {
   if (!c || !out) return BNZ_EINVAL;
+  c->dt_parts = 0;  // the partial buffer is reused below
   Box a = active_box(c);
   const int nb = blocks_for(a.total());
   hipLaunchKernelGGL(k_diag, dim3(nb), dim3(BLK), 0, c->stream, c->g, c->w, c->b, a, c->part);
@@ -1107,8 +1479,29 @@ int bnz_synchronize(bnz_ctx *c) {
   return BNZ_OK;
 }
 
+int bnz_set_tuning(bnz_ctx *c, int flux_chunk, int emf_chunk, int update_chunk) {
+  if (!c || flux_chunk < 0 || emf_chunk < 0 || update_chunk < 0) return BNZ_EINVAL;
+  c->flux_chunk = flux_chunk;
+  c->emf_chunk = emf_chunk;
+  c->update_chunk = update_chunk;
+  // partial count of a fused CFL reduction depends on the update chunk
+  c->dt_parts = 0;
+  const long need = (long)march_blocks(c->nt[0], c->nt[1], c->nt[2], std::max(update_chunk, 1));
+  if (need > c->npart_cap) {
+    double *np = nullptr;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMalloc(&np, need * sizeof(double)));
+    (void)hipFree(c->part);
+    c->part = np;
+    c->npart_cap = need;
+  }
+  return BNZ_OK;
+}
+
 int bnz_set_kernel_timing(bnz_ctx *c, int on) {
   if (!c) return BNZ_EINVAL;
+  collect_timing(c);
+  if (on) c->ktimes.clear();
   c->timing = on != 0;
   return BNZ_OK;
 }

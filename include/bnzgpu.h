@@ -102,8 +102,15 @@ int bnz_set_time(bnz_ctx *ctx, double time, int step);
 int bnz_diag(bnz_ctx *ctx, bnz_diag_out *out);
 int bnz_synchronize(bnz_ctx *ctx);
 
+/* Chunk lengths (cells along the marching axis per thread) of the 3D
+   marching kernels: y/z fluxes, edge EMFs and the stage update; 0 selects
+   the one-thread-per-face/edge/cell kernel (the default).  Chosen by the
+   autotuner (bonzo_amd/autotune.py); no counterpart in the reference. */
+int bnz_set_tuning(bnz_ctx *ctx, int flux_chunk, int emf_chunk, int update_chunk);
+
 /* Kernel timing on the context's stream: when on, every launch is bracketed
-   by HIP events and its duration accumulated per kernel name. */
+   by HIP events and its duration accumulated per kernel name; switching it
+   on clears the table. */
 int bnz_set_kernel_timing(bnz_ctx *ctx, int on);
 int bnz_kernel_timing(bnz_ctx *ctx, int idx, char *name, int name_len, double *total_ms,
                       long *launches);

@@ -198,3 +198,27 @@ def test_divb_and_conservation_128():
     for q in range(3):
         assert abs(d1["mom"][q] - d0["mom"][q]) <= 1e-13
     assert d1["nonfinite"] == 0
+
+
+@pytest.mark.parametrize("chunks", [(0, 0, 0), (1, 1, 1), (7, 5, 3), (64, 64, 64), (0, 16, 4)])
+def test_marching_chunks_do_not_change_results(chunks):
+    """The 3D marching kernels give the oracle's result for any chunk length
+    (chunk boundaries restart the register window)."""
+    from bonzo_amd._lib import Context
+    from oracle.oracle import OracleSim
+    say(f"case: turbulence 20^3, chunks {chunks}, 4 steps")
+    s, prim, bfld = P.turbulence(20, seed_b=61, seed_v=62)
+    o = OracleSim(20, 20, 20)
+    o.prim[:] = prim
+    o.bfld[:] = bfld
+    o.init()
+    o.advance_steps(4)
+    g = Context(20, 20, 20)
+    g.set_tuning(*chunks)
+    g.upload(prim, bfld)
+    g.init_state()
+    g.run_steps(4)
+    gp, gb = g.download()
+    sk, sj, si = s.active()
+    assert np.array_equal(gp[:, sk, sj, si], o.prim[:, sk, sj, si])
+    assert np.array_equal(gb[:, sk, sj, si], o.bfld[:, sk, sj, si])
