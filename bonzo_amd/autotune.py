@@ -1,6 +1,8 @@
 """Autotuner for the marching-chunk lengths of the 3D kernels.
 
-Each candidate chunk length (0 = one thread per face/edge/cell) is applied
+Each candidate chunk length (0 = one thread per face/edge/cell in x-fastest
+order, 1 = one thread per face/edge/cell on 64x4 tiles ordered along the
+marching axis, >1 = register-window marching over that many cells) is applied
 to all three kernel classes, a few VL steps are run with per-launch HIP-event
 timing on the context's stream, and each class keeps the candidate with the
 smallest summed time:
@@ -11,7 +13,7 @@ smallest summed time:
 The runs advance the context's state: callers tune first and then upload
 their initial state again, or tune a scratch context of the same shape.
 """
-CANDIDATES = (0, 2, 4, 8, 16, 32)
+CANDIDATES = (0, 1, 2, 4, 8, 16)
 
 
 def _class_times(kt):

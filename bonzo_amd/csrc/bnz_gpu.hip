@@ -73,6 +73,33 @@ __device__ __forceinline__ void box_idx(const Box &b, long t, int &i, int &j, in
   k = b.lo[2] + (int)(r / ny);
 }
 
+// Thread -> cell of box b.  Flat: x fastest over the whole box.  Tiled
+// (TA, MA): a workgroup is 64 x-lanes by 4 rows of axis TA at one index of
+// axis MA, and consecutive workgroups step along MA, so stencil neighbours
+// along MA are read by workgroups that run back to back on the same XCD.
+template <bool TILED, int TA, int MA>
+__device__ __forceinline__ bool cell_index(const Box &b, int &i, int &j, int &k) {
+  if (!TILED) {
+    const long t = xcd_block() * BLK + threadIdx.x;
+    if (t >= b.total()) return false;
+    box_idx(b, t, i, j, k);
+    return true;
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int nbi = (b.n[0] + 63) / 64, nm = b.n[MA];
+  long L = xcd_block();
+  const int m = (int)(L % nm);
+  L /= nm;
+  const int bi = (int)(L % nbi), bt = (int)(L / nbi);
+  const int ii = bi * 64 + lane, tt = bt * 4 + wv;
+  if (ii >= b.n[0] || tt >= b.n[TA]) return false;
+  i = b.lo[0] + ii;
+  const int vt = b.lo[TA] + tt, vm = b.lo[MA] + m;
+  j = (TA == 1) ? vt : vm;
+  k = (TA == 2) ? vt : vm;
+  return true;
+}
+
 // Global variable read by local slot n of a sweep along DIR
 // (swap_mhd_vec, src/integration/godunov.pyx:297-324).
 template <int DIR>
@@ -105,14 +132,12 @@ __device__ __forceinline__ constexpr int perm(int n) {
 // left state of D1).  One thread per face; face index c is the lower face of
 // cell c, between cells c-1 and c.  The flux is stored in global variable
 // order, flux[n][c], except the normal-field slot, which no consumer reads.
-template <int DIR, int ORDER, int RS>
+template <int DIR, int ORDER, int RS, bool TILED = false>
 __global__ __launch_bounds__(BLK) void k_flux(Geo g, const double *__restrict__ w,
                                               const double *__restrict__ bf,
                                               double *__restrict__ fl, Box box) {
-  long t = xcd_block() * BLK + threadIdx.x;
-  if (t >= box.total()) return;
   int i, j, k;
-  box_idx(box, t, i, j, k);
+  if (!cell_index<TILED, (DIR == 1 ? 2 : 1), (DIR == 1 ? 1 : 2)>(box, i, j, k)) return;
   const long c = ((long)k * g.nt[1] + j) * g.nt[0] + i;
   const long sd = (DIR == 0) ? 1 : (DIR == 1) ? g.sy : g.sz;
   const double bn = bf[(long)DIR * g.sv + c];
@@ -162,15 +187,14 @@ __device__ __forceinline__ double ec_comp(const double *w, long sv, long c, int 
   return -(vx * by - vy * bx);
 }
 
+template <bool TILED = false>
 __global__ __launch_bounds__(BLK) void k_emf(Geo g, const double *__restrict__ w,
                                              const double *__restrict__ fx,
                                              const double *__restrict__ fy,
                                              const double *__restrict__ fz,
                                              double *__restrict__ ee, Box box) {
-  long t = xcd_block() * BLK + threadIdx.x;
-  if (t >= box.total()) return;
   int i, j, k;
-  box_idx(box, t, i, j, k);
+  if (!cell_index<TILED, 1, 2>(box, i, j, k)) return;
   const long sv = g.sv;
   const long c = ((long)k * g.nt[1] + j) * g.nt[0] + i;
   const long sy = g.sy, sz = g.sz;
@@ -245,28 +269,66 @@ __device__ __forceinline__ double bface_z(const Geo &g, const double *b0, const 
   return v;
 }
 
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// Maximum over the workgroup into part[blockIdx.x] (wave shuffles, then LDS).
+__device__ __forceinline__ void block_max_to(double v, double *part) {
+  __shared__ double red[BLK / 64];
+  v = wave_max(v);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = red[0];
+#pragma unroll
+    for (int q = 1; q < BLK / 64; ++q) m = fmax(m, red[q]);
+    part[blockIdx.x] = m;
+  }
+}
+
+// CFL speed of one active cell (new_dt.pyx:74-91).
+__device__ __forceinline__ double cds_cell(const Geo &g, const double *wv, int fms_ref, int i,
+                                           int j, int k) {
+  const double cx = fabs(wv[VX]) + fms(wv, g.gam);
+  double cy = 0., cz = 0.;
+  if (g.dims >= 2) {
+    double wy[NMODE];
+#pragma unroll
+    for (int n = 0; n < NMODE; ++n) wy[n] = wv[n];
+    if (!fms_ref) { wy[BX] = wv[BY]; wy[BY] = wv[BX]; }
+    cy = fabs(wv[VY]) + fms(wy, g.gam);
+  }
+  if (g.dims >= 3) {
+    double wz[NMODE];
+#pragma unroll
+    for (int n = 0; n < NMODE; ++n) wz[n] = wv[n];
+    if (!fms_ref) { wz[BX] = wv[BZ]; wz[BZ] = wv[BX]; }
+    cz = fabs(wv[VZ]) + fms(wz, g.gam);
+  }
+  double cds = fmax(fmax(cx / g.dlf[0][i], cy / g.dlf[1][j]), cz / g.dlf[2][k]);
+  return (cds > 0.) ? cds : 0.;  // NaN never wins the reference's `>` test
+}
+
 // Stage update: advance_hydro (godunov.pyx:215-252), advance_b, interp_bc
 // (ct.pyx:96-123) and cons2prim (eos.pyx:13-41) fused, one thread per cell
 // of the whole grid.  u0 = prim2cons(w0) is the reference's cons array.
 // PRED: writes ws (a separate buffer); cells outside the limits take w0,
 // as the reference's ws aliases prim (vanleer.pyx:72).  Corrector: w1 may
 // alias w0 (each thread reads only its own cell of w0).
-template <bool PRED>
-__global__ __launch_bounds__(BLK) void k_update(Geo g, const double *w0, const double *__restrict__ fx,
-                                                const double *__restrict__ fy,
-                                                const double *__restrict__ fz,
-                                                const double *__restrict__ ee,
-                                                const double *__restrict__ b0,
-                                                double *__restrict__ b1, double *w1, Lims L,
-                                                const double *__restrict__ dts) {
-  long t = xcd_block() * BLK + threadIdx.x;
-  const long ncell = (long)g.nt[0] * g.nt[1] * g.nt[2];
-  if (t >= ncell) return;
-  const int i = (int)(t % g.nt[0]);
-  const long r = t / g.nt[0];
-  const int j = (int)(r % g.nt[1]);
-  const int k = (int)(r / g.nt[1]);
-  const long c = t;
+template <bool PRED, bool DTRED>
+__device__ __forceinline__ double update_cell(const Geo &g, const double *w0,
+                                              const double *__restrict__ fx,
+                                              const double *__restrict__ fy,
+                                              const double *__restrict__ fz,
+                                              const double *__restrict__ ee,
+                                              const double *__restrict__ b0,
+                                              double *__restrict__ b1, double *w1, const Lims &L,
+                                              const double *__restrict__ dts, const Box &act,
+                                              int fms_ref, int i, int j, int k) {
+  const long c = ((long)k * g.nt[1] + j) * g.nt[0] + i;
   const long sv = g.sv;
   const double dt = PRED ? 0.5 * dts[0] : dts[0];
 
@@ -297,7 +359,7 @@ __global__ __launch_bounds__(BLK) void k_update(Geo g, const double *w0, const d
 #pragma unroll
       for (int n = 0; n < NMODE; ++n) w1[n * sv + c] = w0[n * sv + c];
     }
-    return;
+    return 0.;
   }
 
   double wv[NMODE], u[NMODE];
@@ -334,6 +396,27 @@ __global__ __launch_bounds__(BLK) void k_update(Geo g, const double *w0, const d
   cons2prim(wv, u, g.gam);
 #pragma unroll
   for (int n = 0; n < NMODE; ++n) w1[n * sv + c] = wv[n];
+  if (DTRED && i >= act.lo[0] && i < act.lo[0] + act.n[0] && j >= act.lo[1] &&
+      j < act.lo[1] + act.n[1] && k >= act.lo[2] && k < act.lo[2] + act.n[2])
+    return cds_cell(g, wv, fms_ref, i, j, k);
+  return 0.;
+}
+
+template <bool PRED, bool DTRED = false, bool TILED = false>
+__global__ __launch_bounds__(BLK) void k_update(Geo g, const double *w0, const double *__restrict__ fx,
+                                                const double *__restrict__ fy,
+                                                const double *__restrict__ fz,
+                                                const double *__restrict__ ee,
+                                                const double *__restrict__ b0,
+                                                double *__restrict__ b1, double *w1, Lims L,
+                                                const double *__restrict__ dts, Box grid, Box act,
+                                                int fms_ref, double *__restrict__ part) {
+  int i, j, k;
+  double cdsmax = 0.;
+  if (cell_index<TILED, 1, 2>(grid, i, j, k))
+    cdsmax = update_cell<PRED, DTRED>(g, w0, fx, fy, fz, ee, b0, b1, w1, L, dts, act, fms_ref, i,
+                                      j, k);
+  if (DTRED) block_max_to(cdsmax, part);
 }
 
 // --------------------------------------------------------------------------
@@ -343,11 +426,6 @@ __global__ __launch_bounds__(BLK) void k_update(Geo g, const double *w0, const d
 // cell is read once and reconstructed once per sweep.  Values and operation
 // order are those of the per-face / per-cell kernels above.
 
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
-  return v;
-}
 
 struct MarchMap {
   int ii, tt, m0, m1;
@@ -471,27 +549,6 @@ __global__ __launch_bounds__(BLK) void k_emf_march(Geo g, const double *__restri
   }
 }
 
-__device__ __forceinline__ double cds_cell(const Geo &g, const double *wv, int fms_ref, int i,
-                                           int j, int k) {
-  const double cx = fabs(wv[VX]) + fms(wv, g.gam);
-  double cy = 0., cz = 0.;
-  if (g.dims >= 2) {
-    double wy[NMODE];
-#pragma unroll
-    for (int n = 0; n < NMODE; ++n) wy[n] = wv[n];
-    if (!fms_ref) { wy[BX] = wv[BY]; wy[BY] = wv[BX]; }
-    cy = fabs(wv[VY]) + fms(wy, g.gam);
-  }
-  if (g.dims >= 3) {
-    double wz[NMODE];
-#pragma unroll
-    for (int n = 0; n < NMODE; ++n) wz[n] = wv[n];
-    if (!fms_ref) { wz[BX] = wv[BZ]; wz[BZ] = wv[BX]; }
-    cz = fabs(wv[VZ]) + fms(wz, g.gam);
-  }
-  double cds = fmax(fmax(cx / g.dlf[0][i], cy / g.dlf[1][j]), cz / g.dlf[2][k]);
-  return (cds > 0.) ? cds : 0.;  // NaN never wins the reference's `>` test
-}
 
 // Stage update for 3D grids marching in z over the whole grid (same work
 // and predicates as k_update).  DTRED: also reduce the next step's CFL
@@ -919,6 +976,11 @@ int fail(bnz_ctx *c, int code, const std::string &msg) {
 
 int blocks_for(long n) { return (int)((n + BLK - 1) / BLK); }
 
+// Workgroups of a tiled per-cell kernel over box b (see cell_index).
+int tiled_blocks(const Box &b, int ta, int ma) {
+  return ((b.n[0] + 63) / 64) * ((b.n[ta] + 3) / 4) * b.n[ma];
+}
+
 // Workgroups of a marching kernel: 64 x-lanes x 4 transverse rows x chunks.
 int march_blocks(int nx, int nt, int nm, int chunk) {
   return ((nx + 63) / 64) * ((nt + 3) / 4) * ((nm + chunk - 1) / chunk);
@@ -1051,7 +1113,7 @@ void launch_fluxes(bnz_ctx *c, const double *w, const double *bf, const Lims &L)
     bx.lo[1] = (c->dims >= 2) ? jl - 1 : 0; bx.n[1] = (c->dims >= 2) ? (ju + 1) - (jl - 1) + 1 : 1;
     bx.lo[2] = (c->dims >= 3) ? kl - 1 : 0; bx.n[2] = (c->dims >= 3) ? (ku + 1) - (kl - 1) + 1 : 1;
     Launch t(c, ORDER ? "flux_x_plm" : "flux_x_dc");
-    hipLaunchKernelGGL((k_flux<0, ORDER, RS>), dim3(blocks_for(bx.total())), dim3(BLK), 0,
+    hipLaunchKernelGGL((k_flux<0, ORDER, RS, false>), dim3(blocks_for(bx.total())), dim3(BLK), 0,
                        c->stream, g, w, bf, c->fx, bx);
   }
   if (c->dims >= 2) {  // y faces j in [jl, ju+1], i in [il-1, iu+1], k in [kl-1, ku+1]
@@ -1060,12 +1122,15 @@ void launch_fluxes(bnz_ctx *c, const double *w, const double *bf, const Lims &L)
     by.lo[1] = jl; by.n[1] = ju + 1 - jl + 1;
     by.lo[2] = (c->dims >= 3) ? kl - 1 : 0; by.n[2] = (c->dims >= 3) ? (ku + 1) - (kl - 1) + 1 : 1;
     Launch t(c, ORDER ? "flux_y_plm" : "flux_y_dc");
-    if (c->dims >= 3 && c->flux_chunk > 0)
+    if (c->dims >= 3 && c->flux_chunk > 1)
       hipLaunchKernelGGL((k_flux_march<1, ORDER, RS>),
                          dim3(march_blocks(by.n[0], by.n[2], by.n[1], c->flux_chunk)), dim3(BLK), 0,
                          c->stream, g, w, bf, c->fy, by, c->flux_chunk);
+    else if (c->dims >= 3 && c->flux_chunk == 1)
+      hipLaunchKernelGGL((k_flux<1, ORDER, RS, true>), dim3(tiled_blocks(by, 2, 1)), dim3(BLK), 0,
+                         c->stream, g, w, bf, c->fy, by);
     else
-      hipLaunchKernelGGL((k_flux<1, ORDER, RS>), dim3(blocks_for(by.total())), dim3(BLK), 0,
+      hipLaunchKernelGGL((k_flux<1, ORDER, RS, false>), dim3(blocks_for(by.total())), dim3(BLK), 0,
                          c->stream, g, w, bf, c->fy, by);
   }
   if (c->dims >= 3) {  // z faces k in [kl, ku+1], i in [il-1, iu+1], j in [jl-1, ju+1]
@@ -1074,12 +1139,15 @@ void launch_fluxes(bnz_ctx *c, const double *w, const double *bf, const Lims &L)
     bz.lo[1] = jl - 1; bz.n[1] = (ju + 1) - (jl - 1) + 1;
     bz.lo[2] = kl; bz.n[2] = ku + 1 - kl + 1;
     Launch t(c, ORDER ? "flux_z_plm" : "flux_z_dc");
-    if (c->flux_chunk > 0)
+    if (c->flux_chunk == 1)
+      hipLaunchKernelGGL((k_flux<2, ORDER, RS, true>), dim3(tiled_blocks(bz, 1, 2)), dim3(BLK), 0,
+                         c->stream, g, w, bf, c->fz, bz);
+    else if (c->flux_chunk > 1)
       hipLaunchKernelGGL((k_flux_march<2, ORDER, RS>),
                          dim3(march_blocks(bz.n[0], bz.n[1], bz.n[2], c->flux_chunk)), dim3(BLK),
                          0, c->stream, g, w, bf, c->fz, bz, c->flux_chunk);
     else
-      hipLaunchKernelGGL((k_flux<2, ORDER, RS>), dim3(blocks_for(bz.total())), dim3(BLK), 0,
+      hipLaunchKernelGGL((k_flux<2, ORDER, RS, false>), dim3(blocks_for(bz.total())), dim3(BLK), 0,
                          c->stream, g, w, bf, c->fz, bz);
   }
 }
@@ -1102,13 +1170,16 @@ void launch_emf(bnz_ctx *c, const double *w, const Lims &L, const char *name) {
   e.lo[1] = (c->dims >= 2) ? L.lo[1] : 0; e.n[1] = (c->dims >= 2) ? L.hi[1] + 1 - L.lo[1] + 1 : 1;
   e.lo[2] = (c->dims >= 3) ? L.lo[2] : 0; e.n[2] = (c->dims >= 3) ? L.hi[2] + 1 - L.lo[2] + 1 : 1;
   Launch t(c, name);
-  if (c->dims >= 3 && c->emf_chunk > 0)
+  if (c->dims >= 3 && c->emf_chunk == 1)
+    hipLaunchKernelGGL(k_emf<true>, dim3(tiled_blocks(e, 1, 2)), dim3(BLK), 0, c->stream, c->g,
+                       w, c->fx, c->fy, c->fz, c->ee, e);
+  else if (c->dims >= 3 && c->emf_chunk > 1)
     hipLaunchKernelGGL(k_emf_march, dim3(march_blocks(e.n[0], e.n[1], e.n[2], c->emf_chunk)),
                        dim3(BLK), 0, c->stream, c->g, w, c->fx, c->fy, c->fz, c->ee, e,
                        c->emf_chunk);
   else
-    hipLaunchKernelGGL(k_emf, dim3(blocks_for(e.total())), dim3(BLK), 0, c->stream, c->g, w,
-                       c->fx, c->fy, c->fz, c->ee, e);
+    hipLaunchKernelGGL(k_emf<false>, dim3(blocks_for(e.total())), dim3(BLK), 0, c->stream, c->g,
+                       w, c->fx, c->fy, c->fz, c->ee, e);
 }
 
 void launch_ghost(bnz_ctx *c) {
@@ -1134,29 +1205,48 @@ void launch_dt(bnz_ctx *c, int use_fixed, double fixed_dt) {
 // One VL step after the dt kernels (vanleer.pyx:100-163).
 void launch_update(bnz_ctx *c, bool pred, const Lims &L) {
   double *w1 = pred ? c->ws : c->w;
-  if (c->dims >= 3 && c->update_chunk > 0) {
-    const int nb = march_blocks(c->nt[0], c->nt[1], c->nt[2], c->update_chunk);
-    Launch t(c, pred ? "update_pred" : "update_corr");
+  Box grid;
+  for (int q = 0; q < 3; ++q) {
+    grid.lo[q] = 0;
+    grid.n[q] = c->nt[q];
+  }
+  const Box act = active_box(c);
+  const int fr = c->p.dt_fms_ref;
+  Launch t(c, pred ? "update_pred" : "update_corr");
+  int nb;
+  if (c->dims >= 3 && c->update_chunk > 1) {
+    nb = march_blocks(c->nt[0], c->nt[1], c->nt[2], c->update_chunk);
     if (pred)
       hipLaunchKernelGGL((k_update_march<true, false>), dim3(nb), dim3(BLK), 0, c->stream, c->g,
                          c->w, c->fx, c->fy, c->fz, c->ee, c->b, c->bs, w1, L, c->dev,
-                         c->update_chunk, active_box(c), c->p.dt_fms_ref, c->part);
+                         c->update_chunk, act, fr, c->part);
     else
       hipLaunchKernelGGL((k_update_march<false, true>), dim3(nb), dim3(BLK), 0, c->stream, c->g,
                          c->w, c->fx, c->fy, c->fz, c->ee, c->b, c->bs, w1, L, c->dev,
-                         c->update_chunk, active_box(c), c->p.dt_fms_ref, c->part);
-    // the corrector leaves the next step's CFL partials in part[0..nb)
-    c->dt_parts = pred ? 0 : nb;
-  } else {
-    Launch t(c, pred ? "update_pred" : "update_corr");
+                         c->update_chunk, act, fr, c->part);
+  } else if (c->dims >= 3 && c->update_chunk == 1) {
+    nb = tiled_blocks(grid, 1, 2);
     if (pred)
-      hipLaunchKernelGGL(k_update<true>, dim3(blocks_for(c->ncell)), dim3(BLK), 0, c->stream,
-                         c->g, c->w, c->fx, c->fy, c->fz, c->ee, c->b, c->bs, w1, L, c->dev);
+      hipLaunchKernelGGL((k_update<true, false, true>), dim3(nb), dim3(BLK), 0, c->stream, c->g,
+                         c->w, c->fx, c->fy, c->fz, c->ee, c->b, c->bs, w1, L, c->dev, grid, act,
+                         fr, c->part);
     else
-      hipLaunchKernelGGL(k_update<false>, dim3(blocks_for(c->ncell)), dim3(BLK), 0, c->stream,
-                         c->g, c->w, c->fx, c->fy, c->fz, c->ee, c->b, c->bs, w1, L, c->dev);
-    c->dt_parts = 0;
+      hipLaunchKernelGGL((k_update<false, true, true>), dim3(nb), dim3(BLK), 0, c->stream, c->g,
+                         c->w, c->fx, c->fy, c->fz, c->ee, c->b, c->bs, w1, L, c->dev, grid, act,
+                         fr, c->part);
+  } else {
+    nb = blocks_for(c->ncell);
+    if (pred)
+      hipLaunchKernelGGL((k_update<true, false, false>), dim3(nb), dim3(BLK), 0, c->stream, c->g,
+                         c->w, c->fx, c->fy, c->fz, c->ee, c->b, c->bs, w1, L, c->dev, grid, act,
+                         fr, c->part);
+    else
+      hipLaunchKernelGGL((k_update<false, true, false>), dim3(nb), dim3(BLK), 0, c->stream, c->g,
+                         c->w, c->fx, c->fy, c->fz, c->ee, c->b, c->bs, w1, L, c->dev, grid, act,
+                         fr, c->part);
   }
+  // the corrector leaves the next step's CFL partials in part[0..nb)
+  c->dt_parts = pred ? 0 : nb;
 }
 
 // One VL step after the dt kernels (vanleer.pyx:100-163).
@@ -1274,8 +1364,16 @@ int bnz_create(const bnz_params *p, bnz_ctx **out) {
   const double dev0[8] = {0., 0., -1., 0., 0., 0., 0., 0.};
   HIPCHK(c, hipMemcpy(c->dev, dev0, sizeof(dev0), hipMemcpyHostToDevice));
   Box a = active_box(c);
-  c->npart_cap = std::max((long)blocks_for(a.total()) * NDIAG + 64,
-                          (long)march_blocks(c->nt[0], c->nt[1], c->nt[2], 1));
+  {
+    Box grid;
+    for (int q = 0; q < 3; ++q) {
+      grid.lo[q] = 0;
+      grid.n[q] = c->nt[q];
+    }
+    c->npart_cap = std::max({(long)blocks_for(a.total()) * NDIAG + 64, (long)blocks_for(c->ncell),
+                             (long)tiled_blocks(grid, 1, 2),
+                             (long)march_blocks(c->nt[0], c->nt[1], c->nt[2], 1)});
+  }
   HIPCHK(c, hipMalloc(&c->part, c->npart_cap * sizeof(double)));
 
   Geo &g = c->g;
