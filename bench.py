@@ -1,13 +1,17 @@
 """Benchmark of the MI355X VL MHD step: 3D decaying MHD turbulence, 256^3,
-periodic, HLLD + PLM + CT, FP64 (cell-updates/s on one GPU).
+periodic, HLLD + PLM + CT.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--no-cpu]
+                    [--no-tune] [--no-fp32c] [--accuracy-steps 100]
 
-Prints one JSON line.  A step is one full Van-Leer step (predictor +
-corrector + ghost fill + CFL dt) over the whole grid; inputs are resident in
-HBM before the timed region starts.  The CPU baseline is the oracle (a C
-restatement of the reference's algorithm with its OpenMP structure) on the
-host cores, over a bounded sample of the same workload.
+Prints one JSON line.  ``value`` is the FP64 plain path (cell-updates/s);
+the FP32-compensated path is timed on the same initial state and reported
+beside it with its speed-up and its per-variable relative L1 error against
+the FP64 path after ``--accuracy-steps`` steps.  A step is one full Van-Leer
+step (CFL dt, predictor, corrector, ghost fill) over the whole grid; inputs
+are resident in HBM before the timed region starts.  The CPU baseline is the
+oracle (a C restatement of the reference's algorithm with its OpenMP
+structure) on the host cores, over a bounded sample of the same workload.
 """
 import argparse
 import json
@@ -21,16 +25,27 @@ sys.path.insert(0, ROOT)
 METRIC = "cell-updates/s at 256^3 HLLD+CT, FP32-comp vs FP64 on 1 MI355X; %HBM roofline"
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic HBM bytes per unit of each kernel, FP64 (8-byte words):
-#   flux_*: per face, read 9 prim + 1 face-normal B, write 8 flux components
-#   emf_*:  per edge triple, read 6 flux components + 6 prim (v, B), write 3
-#   update_*: per cell, read 9 prim + 18 lower-face fluxes + 3 EMFs + 3 faces,
-#             write 9 prim + 3 faces
-KERNEL_BYTES = {"flux": 18 * 8, "emf": 15 * 8, "update": 45 * 8}
-STEP_BYTES_FP64 = 360.0  # 5 state volumes of 9 words per cell-update
+# Algorithmic HBM bytes per unit of each kernel (4- or 8-byte words):
+#   flux_*:   per face, read 9 prim + 1 face-normal B, write 8 flux components
+#   emf_*:    per edge triple, read 6 flux components + 6 prim (v, B), write 3
+#   update_*: per cell, FP64: read 9 prim + 18 lower-face fluxes + 3 EMFs +
+#             3 faces, write 9 prim + 3 faces (45 words of 8 bytes);
+#             FP32C predictor: read 9 prim + 6 hi words + 18 fluxes + 3 EMFs
+#             (4 bytes) + 3 faces (8 bytes), write 9 prim + 3 faces (4 bytes);
+#             FP32C corrector: read 6 hi + 6 lo + 18 fluxes + 3 EMFs (4 bytes)
+#             + 3 faces (8 bytes), write 6 hi + 6 lo + 9 prim (4) + 3 faces (8)
+KERNEL_BYTES = {
+    "fp64": {"flux": 18 * 8, "emf": 15 * 8, "update_pred": 45 * 8, "update_corr": 45 * 8},
+    "fp32c": {"flux": 18 * 4, "emf": 15 * 4, "update_pred": 48 * 4 + 3 * 8,
+              "update_corr": 54 * 4 + 6 * 8},
+}
+# Algorithmic bytes per cell-update of a whole VL step: 5 state volumes of 9
+# words (FP64 360 B; FP32C 252 B with hi + lo on the stage-2 read of S_n and
+# the write of S_{n+1}).
+STEP_BYTES = {"fp64": 360.0, "fp32c": 252.0}
 
 
-def face_units(name, n, ng=3):
+def kernel_units(name, n, ng=3):
     """Units (faces/edges/cells) one launch of a kernel processes at n^3."""
     nt = n + 2 * ng
     order = 1 if name.endswith("plm") or name.endswith("corr") else 0
@@ -43,9 +58,13 @@ def face_units(name, n, ng=3):
     return nt ** 3
 
 
-def cpu_baseline(budget_s=15.0):
-    import numpy as np  # noqa: F401
+def kernel_bytes(prec, name, n):
+    table = KERNEL_BYTES[prec]
+    key = name if name in table else name.split("_")[0]
+    return table[key] * kernel_units(name, n)
 
+
+def cpu_baseline(budget_s=15.0):
     from bonzo_amd import problems
     from oracle.oracle import OracleSim
     n = 64
@@ -81,6 +100,55 @@ def copy_calibration(mib):
     print(json.dumps({"copy_calibration_bytes": 8 * n, "launches": 5}), flush=True)
 
 
+def run_path(prec, n, prim, bfld, args):
+    """Time one precision on the shared initial state; returns timings and the
+    state after ``accuracy_steps`` steps (counting warm-up and timed steps)."""
+    from bonzo_amd._lib import Context
+    from bonzo_amd.autotune import autotune
+    g = Context(n, n, n, rsolver="hlld", reconstr="linear", cour=0.4, precision=prec)
+    g.upload(prim, bfld)
+    g.init_state()
+    tuned, table = (0, 0, 0), {}
+    if prec == "fp64" and not args.no_tune:
+        # tuning steps advance the state: restart from the initial condition
+        tuned, table = autotune(g)
+        g.upload(prim, bfld)
+        g.init_state()
+    g.run_steps(args.warmup)
+    g.synchronize()
+    # timed region: K steps with the device-resident dt, bracketed by stream
+    # syncs; every launch is bracketed by HIP events on the library's stream
+    g.set_kernel_timing(True)
+    g.synchronize()
+    t0 = time.perf_counter()
+    g.run_steps(args.steps)
+    g.synchronize()
+    el = time.perf_counter() - t0
+    g.set_kernel_timing(False)
+    kt = g.kernel_timing()
+    done = args.warmup + args.steps
+    rest = max(0, args.accuracy_steps - done)
+    g.run_steps(rest)
+    d = g.diag()
+    state = g.download() if args.accuracy_steps > 0 else None
+    g.close()
+    value = n ** 3 * args.steps / el
+    dname, (dtot, dl) = max(kt.items(), key=lambda kv: kv[1][0])
+    davg = dtot / dl
+    b = kernel_bytes(prec, dname, n)
+    ach = b / (davg * 1e-3) / 1e9
+    return {
+        "value": value, "ms_per_step": 1e3 * el / args.steps, "tuned": tuned, "table": table,
+        "kernels": {k: v[0] / v[1] for k, v in sorted(kt.items())},
+        "roofline": {"bound": "hbm", "kernel": dname, "achieved": ach, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                     "avg_launch_ms": davg, "algorithmic_bytes_per_launch": b},
+        "step_GBs": STEP_BYTES[prec] * value / 1e9,
+        "divb_max": d["divb_max"], "hlld_guards": d["hlld_guards"],
+        "steps_total": max(done, args.accuracy_steps), "state": state,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,6 +157,9 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the chunk autotuner")
+    ap.add_argument("--no-fp32c", action="store_true", help="time the FP64 path only")
+    ap.add_argument("--accuracy-steps", type=int, default=100,
+                    help="steps after which FP32C is compared with FP64 (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB mebibytes (rocprof counter scale)")
     args = ap.parse_args()
@@ -100,74 +171,71 @@ def main():
               "(multi-GPU sharding is designed, not built)", file=sys.stderr)
         sys.exit(2)
 
+    import numpy as np
+
     from bonzo_amd import problems
-    from bonzo_amd._lib import Context
-    from bonzo_amd.autotune import autotune
 
     n = args.n
     s, prim, bfld = problems.turbulence(n)
-    g = Context(n, n, n, rsolver="hlld", reconstr="linear", cour=0.4)
-    g.upload(prim, bfld)
-    g.init_state()
-    tuned, table = (0, 0, 0), {}
-    if not args.no_tune:
-        # tuning steps advance the state: restart from the initial condition
-        tuned, table = autotune(g)
-        g.upload(prim, bfld)
-        g.init_state()
-        g.set_time(0.0, 0)
+    r64 = run_path("fp64", n, prim, bfld, args)
+    r32 = None if args.no_fp32c else run_path("fp32c", n, prim, bfld, args)
     del prim, bfld
-    g.run_steps(args.warmup)
-    g.synchronize()
 
-    # timed region: K steps, device-resident dt, bracketed by stream syncs;
-    # every launch is bracketed by HIP events on the library's stream
-    g.set_kernel_timing(True)
-    g.synchronize()
-    t0 = time.perf_counter()
-    g.run_steps(args.steps)
-    g.synchronize()
-    el = time.perf_counter() - t0
-    g.set_kernel_timing(False)
-    kt = g.kernel_timing()
-    d = g.diag()
-    g.close()
-
-    ms = 1e3 * el / args.steps
-    value = n ** 3 * args.steps / el
-    dom = max(kt.items(), key=lambda kv: kv[1][0])
-    dname, (dtot, dlaunch) = dom
-    davg_ms = dtot / dlaunch
-    kind = dname.split("_")[0]
-    bytes_per_launch = KERNEL_BYTES[kind] * face_units(dname, n)
-    achieved = bytes_per_launch / (davg_ms * 1e-3) / 1e9
-    step_achieved = STEP_BYTES_FP64 * value / 1e9
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": r64["value"],
         "unit": "cell-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms,
+        "ms_per_step": r64["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"3D decaying MHD turbulence {n}^3 periodic, VL + PLM + HLLD + CT, "
-                               "FP64 plain path, CFL 0.4, seeded k^-2 fields",
+                               "FP64 plain path, CFL 0.4, seeded k^-2 fields "
+                               "(rho=1, p=0.6, Bx=0.2, dB_rms=0.05, Mach 0.5)",
                    "n": n, "precision": "fp64", "rsolver": "hlld", "reconstr": "linear",
-                   "chunks": {"flux": tuned[0], "emf": tuned[1], "update": tuned[2]}},
-        "roofline": {"bound": "hbm", "kernel": dname, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "avg_launch_ms": davg_ms, "algorithmic_bytes_per_launch": bytes_per_launch},
-        "step_roofline": {"bytes_per_cell_update": STEP_BYTES_FP64, "achieved_GBs": step_achieved,
-                          "frac_of_8TBs": step_achieved / HBM_PEAK_GBS},
-        "kernels_ms_per_launch": {k: v[0] / v[1] for k, v in sorted(kt.items())},
-        "divb_max": d["divb_max"],
-        "autotune_ms": {str(c): {k: round(v, 3) for k, v in t.items()} for c, t in table.items()},
+                   "chunks": dict(zip(("flux", "emf", "update"), r64["tuned"]))},
+        "roofline": r64["roofline"],
+        "step_roofline": {"bytes_per_cell_update": STEP_BYTES["fp64"],
+                          "achieved_GBs": r64["step_GBs"],
+                          "frac_of_8TBs": r64["step_GBs"] / HBM_PEAK_GBS},
+        "kernels_ms_per_launch": r64["kernels"],
+        "divb_max": r64["divb_max"],
+        "autotune_ms": {str(c): {k: round(v, 3) for k, v in t.items()}
+                        for c, t in r64["table"].items()},
     }
+    if r32 is not None:
+        fp32c = {"value": r32["value"], "ms_per_step": r32["ms_per_step"],
+                 "speedup_vs_fp64": r32["value"] / r64["value"],
+                 "dtype": "f32 (hi+lo state; FP64 update, pressure recovery, CT increments, dt)",
+                 "roofline": r32["roofline"],
+                 "step_roofline": {"bytes_per_cell_update": STEP_BYTES["fp32c"],
+                                   "achieved_GBs": r32["step_GBs"],
+                                   "frac_of_8TBs": r32["step_GBs"] / HBM_PEAK_GBS},
+                 "kernels_ms_per_launch": r32["kernels"], "divb_max": r32["divb_max"],
+                 "hlld_guards": r32["hlld_guards"]}
+        if r64["state"] is not None and r32["state"] is not None:
+            sk, sj, si = s.active()
+            names = ["rho", "vx", "vy", "vz", "p", "psc", "bxc", "byc", "bzc"]
+            l1 = {}
+            for v in range(9):
+                a = r32["state"][0][v][sk, sj, si]
+                b = r64["state"][0][v][sk, sj, si]
+                den = float(np.sum(np.abs(b)))
+                if den > 0:
+                    l1[names[v]] = float(np.sum(np.abs(a - b)) / den)
+            for c, nm in enumerate(("bxf", "byf", "bzf")):
+                a = r32["state"][1][c][sk, sj, si]
+                b = r64["state"][1][c][sk, sj, si]
+                l1[nm] = float(np.sum(np.abs(a - b)) / np.sum(np.abs(b)))
+            fp32c["rel_l1_vs_fp64"] = l1
+            fp32c["rel_l1_max"] = max(l1.values())
+            fp32c["accuracy_steps"] = r64["steps_total"]
+        out["fp32c"] = fp32c
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

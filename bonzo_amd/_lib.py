@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(_HERE, "libbnzgpu.so")
 BNZ_OK, BNZ_EINVAL, BNZ_ENOMEM, BNZ_EHIP, BNZ_EUNSUPPORTED, BNZ_ESTATE = 0, -1, -2, -3, -4, -5
 RS = {"hll": 0, "hllc": 1, "hlld": 2, "hlla": 3, "hllt": 4}
 RCN = {"const": 0, "linear": 1, "weno": 2, "parab": 3}
-PREC = {"fp64": 0}
+PREC = {"fp64": 0, "fp32c": 1}
 
 ABI_SYMBOLS = (
     "bnz_create", "bnz_destroy", "bnz_last_error", "bnz_get_shape", "bnz_upload",
@@ -41,6 +41,7 @@ class BnzDiag(ctypes.Structure):
         ("mass", ctypes.c_double), ("mom", ctypes.c_double * 3), ("energy", ctypes.c_double),
         ("scalar", ctypes.c_double), ("divb_max", ctypes.c_double), ("bmax", ctypes.c_double),
         ("pmin", ctypes.c_double), ("rhomin", ctypes.c_double), ("nonfinite", ctypes.c_long),
+        ("hlld_guards", ctypes.c_long),
     ]
 
 
@@ -195,7 +196,7 @@ class Context:
         self._check(load().bnz_diag(self._h, ctypes.byref(d)))
         return dict(mass=d.mass, mom=tuple(d.mom), energy=d.energy, scalar=d.scalar,
                     divb_max=d.divb_max, bmax=d.bmax, pmin=d.pmin, rhomin=d.rhomin,
-                    nonfinite=d.nonfinite)
+                    nonfinite=d.nonfinite, hlld_guards=d.hlld_guards)
 
     def set_tuning(self, flux_chunk=0, emf_chunk=0, update_chunk=0):
         self._check(load().bnz_set_tuning(self._h, flux_chunk, emf_chunk, update_chunk))

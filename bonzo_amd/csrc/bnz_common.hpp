@@ -1,0 +1,313 @@
+// bnz_common.hpp -- types and helpers shared by the FP64 (bnz_gpu.hip) and
+// FP32-compensated (bnz_gpu32.hip) translation units of libbnzgpu.so.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/bnzgpu.h"
+#include "bnz_phys.hpp"
+
+namespace bnz {
+
+constexpr int NVAR_STATE = 9;
+constexpr int BLK = 256;
+
+// --------------------------------------------------------------------------
+// Geometry passed by value to every kernel.
+struct Geo {
+  int nt[3];       // Ntot per axis
+  int dims;
+  long sv;         // stride between variables (cells per array)
+  int sy, sz;      // strides of j and k
+  const double *dlf[3];   // cell widths (coord.pyx:254-257)
+  const double *dlfi[3];  // 1/width
+  const double *icm[3];   // lv - lf       (interp_bc weights, ct.pyx:110-123)
+  const double *icp[3];   // lf[i+1] - lv
+  double gam, gamm1i;
+};
+
+struct Lims {
+  int lo[3], hi[3];  // inclusive cell limits of get_integr_lims (integrator.pyx:375-383)
+};
+
+struct Box {
+  int lo[3];
+  int n[3];
+  __host__ __device__ long total() const { return (long)n[0] * n[1] * n[2]; }
+};
+
+// Workgroups are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8).
+// Renumber them so that each XCD owns one contiguous run of logical blocks:
+// neighbouring rows and planes of a stencil then share that XCD's L2.
+constexpr int NXCD = 8;
+__device__ __forceinline__ long xcd_block() {
+  const long b = blockIdx.x, nb = gridDim.x;
+  const long q = nb / NXCD, r = nb % NXCD;
+  const long x = b % NXCD, idx = b / NXCD;
+  return x * q + (x < r ? x : r) + idx;
+}
+
+__device__ __forceinline__ void box_idx(const Box &b, long t, int &i, int &j, int &k) {
+  int nx = b.n[0], ny = b.n[1];
+  i = b.lo[0] + (int)(t % nx);
+  long r = t / nx;
+  j = b.lo[1] + (int)(r % ny);
+  k = b.lo[2] + (int)(r / ny);
+}
+
+// Thread -> cell of box b.  Flat: x fastest over the whole box.  Tiled
+// (TA, MA): a workgroup is 64 x-lanes by 4 rows of axis TA at one index of
+// axis MA, and consecutive workgroups step along MA, so stencil neighbours
+// along MA are read by workgroups that run back to back on the same XCD.
+template <bool TILED, int TA, int MA>
+__device__ __forceinline__ bool cell_index(const Box &b, int &i, int &j, int &k) {
+  if (!TILED) {
+    const long t = xcd_block() * BLK + threadIdx.x;
+    if (t >= b.total()) return false;
+    box_idx(b, t, i, j, k);
+    return true;
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int nbi = (b.n[0] + 63) / 64, nm = b.n[MA];
+  long L = xcd_block();
+  const int m = (int)(L % nm);
+  L /= nm;
+  const int bi = (int)(L % nbi), bt = (int)(L / nbi);
+  const int ii = bi * 64 + lane, tt = bt * 4 + wv;
+  if (ii >= b.n[0] || tt >= b.n[TA]) return false;
+  i = b.lo[0] + ii;
+  const int vt = b.lo[TA] + tt, vm = b.lo[MA] + m;
+  j = (TA == 1) ? vt : vm;
+  k = (TA == 2) ? vt : vm;
+  return true;
+}
+
+// Global variable read by local slot n of a sweep along DIR
+// (swap_mhd_vec, src/integration/godunov.pyx:297-324).
+template <int DIR>
+__device__ __forceinline__ constexpr int perm(int n) {
+  if (DIR == 0) return n;
+  if (DIR == 1) {
+    switch (n) {
+      case VX: return VY;
+      case VY: return VZ;
+      case VZ: return VX;
+      case BX: return BY;
+      case BY: return BZ;
+      case BZ: return BX;
+      default: return n;
+    }
+  }
+  switch (n) {
+    case VX: return VZ;
+    case VY: return VX;
+    case VZ: return VY;
+    case BX: return BZ;
+    case BY: return BX;
+    case BZ: return BY;
+    default: return n;
+  }
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// Maximum over the workgroup into part[blockIdx.x] (wave shuffles, then LDS).
+__device__ __forceinline__ void block_max_to(double v, double *part) {
+  __shared__ double red[BLK / 64];
+  v = wave_max(v);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = red[0];
+#pragma unroll
+    for (int q = 1; q < BLK / 64; ++q) m = fmax(m, red[q]);
+    part[blockIdx.x] = m;
+  }
+}
+
+// CFL speed of one active cell (new_dt.pyx:74-91).
+__device__ __forceinline__ double cds_cell(const Geo &g, const double *wv, int fms_ref, int i,
+                                           int j, int k) {
+  const double cx = fabs(wv[VX]) + fms(wv, g.gam);
+  double cy = 0., cz = 0.;
+  if (g.dims >= 2) {
+    double wy[NMODE];
+#pragma unroll
+    for (int n = 0; n < NMODE; ++n) wy[n] = wv[n];
+    if (!fms_ref) { wy[BX] = wv[BY]; wy[BY] = wv[BX]; }
+    cy = fabs(wv[VY]) + fms(wy, g.gam);
+  }
+  if (g.dims >= 3) {
+    double wz[NMODE];
+#pragma unroll
+    for (int n = 0; n < NMODE; ++n) wz[n] = wv[n];
+    if (!fms_ref) { wz[BX] = wv[BZ]; wz[BZ] = wv[BX]; }
+    cz = fabs(wv[VZ]) + fms(wz, g.gam);
+  }
+  double cds = fmax(fmax(cx / g.dlf[0][i], cy / g.dlf[1][j]), cz / g.dlf[2][k]);
+  return (cds > 0.) ? cds : 0.;  // NaN never wins the reference's `>` test
+}
+
+}  // namespace bnz
+
+// ==========================================================================
+// Host side
+// ==========================================================================
+
+struct KTime {
+  std::string name;
+  double total_ms = 0.;
+  long launches = 0;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+};
+
+struct bnz_ctx {
+  bnz_params p{};
+  int dims = 1, ng = 3, order = 1;
+  int nt[3] = {1, 1, 1};
+  int i1, i2, j1, j2, k1, k2;
+  long ncell = 0;
+  hipStream_t stream = nullptr;
+  double *w = nullptr, *ws = nullptr, *b = nullptr, *bs = nullptr;
+  double *fx = nullptr, *fy = nullptr, *fz = nullptr, *ee = nullptr;
+  double *geo = nullptr;
+  int *maps = nullptr;
+  double *dev = nullptr;      // dt, t, tmax, step
+  double *part = nullptr;     // reduction partials
+  long npart_cap = 0;
+  std::vector<double> h_dlf[3], h_dlfi[3], h_icm[3], h_icp[3];
+  bnz::Geo g{};
+  // FP32-compensated state (BNZ_PREC_FP32C): conserved hydro variables as
+  // FP32 hi + lo pairs, face fields in b/bs (FP64), FP32 primitives of the
+  // current and predicted state, FP32 predicted faces, fluxes and EMFs; w
+  // above is the FP64 prim mirror used for upload, init and download.
+  bool f32 = false;
+  float *uh = nullptr, *ul = nullptr;     // [6][cells]
+  float *w32 = nullptr, *ws32 = nullptr;  // [9][cells]
+  float *bs32 = nullptr;                  // [3][cells] predicted faces
+  float *fx32 = nullptr, *fy32 = nullptr, *fz32 = nullptr;  // [9][cells]
+  float *ee32 = nullptr;                  // [3][cells]
+  bool ready = false;
+  bool timing = false;
+  // tuning: chunk lengths of the marching kernels (3D)
+  int flux_chunk = 0, emf_chunk = 0, update_chunk = 0;
+  // CFL partial maxima left by the last corrector for the next step
+  int dt_parts = 0;
+  std::vector<KTime> ktimes;
+  std::string err;
+};
+
+
+namespace bnz {
+
+inline int fail(bnz_ctx *c, int code, const std::string &msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPCHK(ctx, call)                                                              \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(ctx, BNZ_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));   \
+  } while (0)
+
+inline int blocks_for(long n) { return (int)((n + BLK - 1) / BLK); }
+
+// Workgroups of a tiled per-cell kernel over box b (see cell_index).
+inline int tiled_blocks(const Box &b, int ta, int ma) {
+  return ((b.n[0] + 63) / 64) * ((b.n[ta] + 3) / 4) * b.n[ma];
+}
+
+// Workgroups of a marching kernel: 64 x-lanes x 4 transverse rows x chunks.
+inline int march_blocks(int nx, int nt, int nm, int chunk) {
+  return ((nx + 63) / 64) * ((nt + 3) / 4) * ((nm + chunk - 1) / chunk);
+}
+
+struct Launch {
+  bnz_ctx *c;
+  int idx;
+  hipEvent_t e0{}, e1{};
+  Launch(bnz_ctx *ctx, const char *name) : c(ctx), idx(-1) {
+    if (!c->timing) return;
+    for (size_t q = 0; q < c->ktimes.size(); ++q)
+      if (c->ktimes[q].name == name) idx = (int)q;
+    if (idx < 0) {
+      c->ktimes.push_back(KTime{});
+      c->ktimes.back().name = name;
+      idx = (int)c->ktimes.size() - 1;
+    }
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    (void)hipEventRecord(e0, c->stream);
+  }
+  ~Launch() {
+    if (idx < 0) return;
+    (void)hipEventRecord(e1, c->stream);
+    c->ktimes[idx].pending.emplace_back(e0, e1);
+  }
+};
+
+inline void collect_timing(bnz_ctx *c) {
+  for (auto &kt : c->ktimes) {
+    for (auto &pr : kt.pending) {
+      (void)hipEventSynchronize(pr.second);
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, pr.first, pr.second);
+      kt.total_ms += ms;
+      kt.launches += 1;
+      (void)hipEventDestroy(pr.first);
+      (void)hipEventDestroy(pr.second);
+    }
+    kt.pending.clear();
+  }
+}
+
+inline Lims integr_lims(const bnz_ctx *c, int order) {
+  Lims L;
+  for (int a = 0; a < 3; ++a) {
+    L.lo[a] = order + 1;
+    L.hi[a] = c->nt[a] - 2 - order;
+  }
+  if (c->dims < 2) L.lo[1] = L.hi[1] = 0;
+  if (c->dims < 3) L.lo[2] = L.hi[2] = 0;
+  return L;
+}
+
+inline Box active_box(const bnz_ctx *c) {
+  Box b;
+  b.lo[0] = c->i1; b.n[0] = c->i2 - c->i1 + 1;
+  b.lo[1] = c->j1; b.n[1] = c->j2 - c->j1 + 1;
+  b.lo[2] = c->k1; b.n[2] = c->k2 - c->k1 + 1;
+  return b;
+}
+
+inline int check_launch(bnz_ctx *c) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(c, BNZ_EHIP, std::string("kernel launch: ") + hipGetErrorString(e));
+  return BNZ_OK;
+}
+
+
+// FP32-compensated step (bnz_gpu32.hip)
+int f32_alloc(bnz_ctx *c);
+void f32_free(bnz_ctx *c);
+int f32_from64(bnz_ctx *c);   // state from the FP64 prim/face arrays (all cells)
+int f32_to64(bnz_ctx *c);     // FP64 prim/face arrays from the state
+void f32_ghost(bnz_ctx *c);
+void f32_dt(bnz_ctx *c, int use_fixed, double fixed_dt);
+void f32_step_body(bnz_ctx *c);
+// shared launches (bnz_gpu.hip)
+void launch_dt_final(bnz_ctx *c, int npart, int use_fixed, double fixed_dt);
+void launch_time_advance(bnz_ctx *c);
+void launch_ghost64(bnz_ctx *c);
+
+}  // namespace bnz

@@ -20,112 +20,11 @@
 #include <string>
 #include <vector>
 
-#include "../../include/bnzgpu.h"
-#include "bnz_phys.hpp"
+#include "bnz_common.hpp"
 
 using namespace bnz;
 
 namespace {
-
-constexpr int NVAR_STATE = 9;
-constexpr int BLK = 256;
-
-// --------------------------------------------------------------------------
-// Geometry passed by value to every kernel.
-struct Geo {
-  int nt[3];       // Ntot per axis
-  int dims;
-  long sv;         // stride between variables (cells per array)
-  int sy, sz;      // strides of j and k
-  const double *dlf[3];   // cell widths (coord.pyx:254-257)
-  const double *dlfi[3];  // 1/width
-  const double *icm[3];   // lv - lf       (interp_bc weights, ct.pyx:110-123)
-  const double *icp[3];   // lf[i+1] - lv
-  double gam, gamm1i;
-};
-
-struct Lims {
-  int lo[3], hi[3];  // inclusive cell limits of get_integr_lims (integrator.pyx:375-383)
-};
-
-struct Box {
-  int lo[3];
-  int n[3];
-  __host__ __device__ long total() const { return (long)n[0] * n[1] * n[2]; }
-};
-
-// Workgroups are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8).
-// Renumber them so that each XCD owns one contiguous run of logical blocks:
-// neighbouring rows and planes of a stencil then share that XCD's L2.
-constexpr int NXCD = 8;
-__device__ __forceinline__ long xcd_block() {
-  const long b = blockIdx.x, nb = gridDim.x;
-  const long q = nb / NXCD, r = nb % NXCD;
-  const long x = b % NXCD, idx = b / NXCD;
-  return x * q + (x < r ? x : r) + idx;
-}
-
-__device__ __forceinline__ void box_idx(const Box &b, long t, int &i, int &j, int &k) {
-  int nx = b.n[0], ny = b.n[1];
-  i = b.lo[0] + (int)(t % nx);
-  long r = t / nx;
-  j = b.lo[1] + (int)(r % ny);
-  k = b.lo[2] + (int)(r / ny);
-}
-
-// Thread -> cell of box b.  Flat: x fastest over the whole box.  Tiled
-// (TA, MA): a workgroup is 64 x-lanes by 4 rows of axis TA at one index of
-// axis MA, and consecutive workgroups step along MA, so stencil neighbours
-// along MA are read by workgroups that run back to back on the same XCD.
-template <bool TILED, int TA, int MA>
-__device__ __forceinline__ bool cell_index(const Box &b, int &i, int &j, int &k) {
-  if (!TILED) {
-    const long t = xcd_block() * BLK + threadIdx.x;
-    if (t >= b.total()) return false;
-    box_idx(b, t, i, j, k);
-    return true;
-  }
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int nbi = (b.n[0] + 63) / 64, nm = b.n[MA];
-  long L = xcd_block();
-  const int m = (int)(L % nm);
-  L /= nm;
-  const int bi = (int)(L % nbi), bt = (int)(L / nbi);
-  const int ii = bi * 64 + lane, tt = bt * 4 + wv;
-  if (ii >= b.n[0] || tt >= b.n[TA]) return false;
-  i = b.lo[0] + ii;
-  const int vt = b.lo[TA] + tt, vm = b.lo[MA] + m;
-  j = (TA == 1) ? vt : vm;
-  k = (TA == 2) ? vt : vm;
-  return true;
-}
-
-// Global variable read by local slot n of a sweep along DIR
-// (swap_mhd_vec, src/integration/godunov.pyx:297-324).
-template <int DIR>
-__device__ __forceinline__ constexpr int perm(int n) {
-  if (DIR == 0) return n;
-  if (DIR == 1) {
-    switch (n) {
-      case VX: return VY;
-      case VY: return VZ;
-      case VZ: return VX;
-      case BX: return BY;
-      case BY: return BZ;
-      case BZ: return BX;
-      default: return n;
-    }
-  }
-  switch (n) {
-    case VX: return VZ;
-    case VY: return VX;
-    case VZ: return VY;
-    case BX: return BZ;
-    case BY: return BX;
-    case BZ: return BY;
-    default: return n;
-  }
-}
 
 // --------------------------------------------------------------------------
 // Godunov fluxes of one direction (godunov.pyx:20-210 with the double-buffered
@@ -269,48 +168,6 @@ __device__ __forceinline__ double bface_z(const Geo &g, const double *b0, const 
   return v;
 }
 
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
-  return v;
-}
-
-// Maximum over the workgroup into part[blockIdx.x] (wave shuffles, then LDS).
-__device__ __forceinline__ void block_max_to(double v, double *part) {
-  __shared__ double red[BLK / 64];
-  v = wave_max(v);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double m = red[0];
-#pragma unroll
-    for (int q = 1; q < BLK / 64; ++q) m = fmax(m, red[q]);
-    part[blockIdx.x] = m;
-  }
-}
-
-// CFL speed of one active cell (new_dt.pyx:74-91).
-__device__ __forceinline__ double cds_cell(const Geo &g, const double *wv, int fms_ref, int i,
-                                           int j, int k) {
-  const double cx = fabs(wv[VX]) + fms(wv, g.gam);
-  double cy = 0., cz = 0.;
-  if (g.dims >= 2) {
-    double wy[NMODE];
-#pragma unroll
-    for (int n = 0; n < NMODE; ++n) wy[n] = wv[n];
-    if (!fms_ref) { wy[BX] = wv[BY]; wy[BY] = wv[BX]; }
-    cy = fabs(wv[VY]) + fms(wy, g.gam);
-  }
-  if (g.dims >= 3) {
-    double wz[NMODE];
-#pragma unroll
-    for (int n = 0; n < NMODE; ++n) wz[n] = wv[n];
-    if (!fms_ref) { wz[BX] = wv[BZ]; wz[BZ] = wv[BX]; }
-    cz = fabs(wv[VZ]) + fms(wz, g.gam);
-  }
-  double cds = fmax(fmax(cx / g.dlf[0][i], cy / g.dlf[1][j]), cz / g.dlf[2][k]);
-  return (cds > 0.) ? cds : 0.;  // NaN never wins the reference's `>` test
-}
 
 // Stage update: advance_hydro (godunov.pyx:215-252), advance_b, interp_bc
 // (ct.pyx:96-123) and cons2prim (eos.pyx:13-41) fused, one thread per cell
@@ -927,64 +784,7 @@ __global__ __launch_bounds__(BLK) void k_pencil(const double *wl_, const double 
 // Host side
 // ==========================================================================
 
-struct KTime {
-  std::string name;
-  double total_ms = 0.;
-  long launches = 0;
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
-};
-
-struct bnz_ctx {
-  bnz_params p{};
-  int dims = 1, ng = 3, order = 1;
-  int nt[3] = {1, 1, 1};
-  int i1, i2, j1, j2, k1, k2;
-  long ncell = 0;
-  hipStream_t stream = nullptr;
-  double *w = nullptr, *ws = nullptr, *b = nullptr, *bs = nullptr;
-  double *fx = nullptr, *fy = nullptr, *fz = nullptr, *ee = nullptr;
-  double *geo = nullptr;
-  int *maps = nullptr;
-  double *dev = nullptr;      // dt, t, tmax, step
-  double *part = nullptr;     // reduction partials
-  long npart_cap = 0;
-  std::vector<double> h_dlf[3], h_dlfi[3], h_icm[3], h_icp[3];
-  Geo g{};
-  bool ready = false;
-  bool timing = false;
-  // tuning: chunk lengths of the marching kernels (3D)
-  int flux_chunk = 0, emf_chunk = 0, update_chunk = 0;
-  // CFL partial maxima left by the last corrector for the next step
-  int dt_parts = 0;
-  std::vector<KTime> ktimes;
-  std::string err;
-};
-
 namespace {
-
-int fail(bnz_ctx *c, int code, const std::string &msg) {
-  if (c) c->err = msg;
-  return code;
-}
-
-#define HIPCHK(ctx, call)                                                              \
-  do {                                                                                 \
-    hipError_t e_ = (call);                                                            \
-    if (e_ != hipSuccess)                                                              \
-      return fail(ctx, BNZ_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));   \
-  } while (0)
-
-int blocks_for(long n) { return (int)((n + BLK - 1) / BLK); }
-
-// Workgroups of a tiled per-cell kernel over box b (see cell_index).
-int tiled_blocks(const Box &b, int ta, int ma) {
-  return ((b.n[0] + 63) / 64) * ((b.n[ta] + 3) / 4) * b.n[ma];
-}
-
-// Workgroups of a marching kernel: 64 x-lanes x 4 transverse rows x chunks.
-int march_blocks(int nx, int nt, int nm, int chunk) {
-  return ((nx + 63) / 64) * ((nt + 3) / 4) * ((nm + chunk - 1) / chunk);
-}
 
 // Uniform Cartesian coordinates as src/coordinates/coord.pyx:201-257 and
 // src/coordinates/cartesian.pyx:18-43 compute them (one MPI block).
@@ -1043,64 +843,6 @@ std::vector<int> axis_map(int N, int a1, int a2, int ng, int f_lo, int f_hi, boo
   }
   (void)n;
   return m;
-}
-
-struct Launch {
-  bnz_ctx *c;
-  int idx;
-  hipEvent_t e0{}, e1{};
-  Launch(bnz_ctx *ctx, const char *name) : c(ctx), idx(-1) {
-    if (!c->timing) return;
-    for (size_t q = 0; q < c->ktimes.size(); ++q)
-      if (c->ktimes[q].name == name) idx = (int)q;
-    if (idx < 0) {
-      c->ktimes.push_back(KTime{});
-      c->ktimes.back().name = name;
-      idx = (int)c->ktimes.size() - 1;
-    }
-    (void)hipEventCreate(&e0);
-    (void)hipEventCreate(&e1);
-    (void)hipEventRecord(e0, c->stream);
-  }
-  ~Launch() {
-    if (idx < 0) return;
-    (void)hipEventRecord(e1, c->stream);
-    c->ktimes[idx].pending.emplace_back(e0, e1);
-  }
-};
-
-void collect_timing(bnz_ctx *c) {
-  for (auto &kt : c->ktimes) {
-    for (auto &pr : kt.pending) {
-      (void)hipEventSynchronize(pr.second);
-      float ms = 0.f;
-      (void)hipEventElapsedTime(&ms, pr.first, pr.second);
-      kt.total_ms += ms;
-      kt.launches += 1;
-      (void)hipEventDestroy(pr.first);
-      (void)hipEventDestroy(pr.second);
-    }
-    kt.pending.clear();
-  }
-}
-
-Lims integr_lims(const bnz_ctx *c, int order) {
-  Lims L;
-  for (int a = 0; a < 3; ++a) {
-    L.lo[a] = order + 1;
-    L.hi[a] = c->nt[a] - 2 - order;
-  }
-  if (c->dims < 2) L.lo[1] = L.hi[1] = 0;
-  if (c->dims < 3) L.lo[2] = L.hi[2] = 0;
-  return L;
-}
-
-Box active_box(const bnz_ctx *c) {
-  Box b;
-  b.lo[0] = c->i1; b.n[0] = c->i2 - c->i1 + 1;
-  b.lo[1] = c->j1; b.n[1] = c->j2 - c->j1 + 1;
-  b.lo[2] = c->k1; b.n[2] = c->k2 - c->k1 + 1;
-  return b;
 }
 
 template <int ORDER, int RS>
@@ -1268,10 +1010,34 @@ void launch_step_body(bnz_ctx *c) {
   hipLaunchKernelGGL(k_time_advance, dim3(1), dim3(64), 0, c->stream, c->dev);
 }
 
-int check_launch(bnz_ctx *c) {
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return fail(c, BNZ_EHIP, std::string("kernel launch: ") + hipGetErrorString(e));
-  return BNZ_OK;
+}  // namespace
+
+namespace bnz {
+
+void launch_dt_final(bnz_ctx *c, int npart, int use_fixed, double fixed_dt) {
+  Launch t(c, "dt_final");
+  hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(BLK), 0, c->stream, c->part, npart, c->p.cour,
+                     c->dev, use_fixed, fixed_dt);
+}
+
+void launch_time_advance(bnz_ctx *c) {
+  hipLaunchKernelGGL(k_time_advance, dim3(1), dim3(64), 0, c->stream, c->dev);
+}
+
+void launch_ghost64(bnz_ctx *c) { launch_ghost(c); }
+
+}  // namespace bnz
+
+namespace {
+
+// Precision dispatch of the step-level calls.
+void any_dt(bnz_ctx *c, int use_fixed, double fixed_dt) {
+  if (c->f32) f32_dt(c, use_fixed, fixed_dt);
+  else launch_dt(c, use_fixed, fixed_dt);
+}
+void any_step(bnz_ctx *c) {
+  if (c->f32) f32_step_body(c);
+  else launch_step_body(c);
 }
 
 }  // namespace
@@ -1298,7 +1064,8 @@ int bnz_create(const bnz_params *p, bnz_ctx **out) {
   if (p->nz > 1 && p->ny == 1) return bad(BNZ_EINVAL, "nz > 1 needs ny > 1 (grid.pyx:114-122)");
   if (!(p->gam > 1.0)) return bad(BNZ_EINVAL, "gam must be > 1");
   if (!(p->cour > 0.0)) return bad(BNZ_EINVAL, "cour must be > 0");
-  if (p->precision != BNZ_PREC_FP64) return bad(BNZ_EUNSUPPORTED, "only FP64 precision is built");
+  if (p->precision != BNZ_PREC_FP64 && p->precision != BNZ_PREC_FP32C)
+    return bad(BNZ_EINVAL, "precision must be BNZ_PREC_FP64 or BNZ_PREC_FP32C");
   if (p->rsolver == BNZ_RS_HLLC)
     return bad(BNZ_EUNSUPPORTED, "hllc is a hydro-only solver (fluxes_c.c:230); MHD needs hll or hlld");
   if (p->rsolver != BNZ_RS_HLLD && p->rsolver != BNZ_RS_HLL)
@@ -1324,17 +1091,27 @@ int bnz_create(const bnz_params *p, bnz_ctx **out) {
   if (hipSetDevice(p->device) != hipSuccess) return fail(c, BNZ_EHIP, "hipSetDevice failed");
   HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   const size_t cells = (size_t)c->ncell;
-  double **bufs9[] = {&c->w, &c->ws, &c->fx, &c->fy, &c->fz};
+  c->f32 = (p->precision == BNZ_PREC_FP32C);
+  // FP64 prim and faces always exist (upload, init, download); the FP64
+  // work arrays only for the FP64 step
+  std::vector<double **> bufs9 = {&c->w}, bufs3 = {&c->b, &c->bs};
+  if (!c->f32) {
+    bufs9.insert(bufs9.end(), {&c->ws, &c->fx, &c->fy, &c->fz});
+    bufs3.insert(bufs3.end(), {&c->ee});
+  }
   for (double **q : bufs9)
     if (hipMalloc(q, NVAR_STATE * cells * sizeof(double)) != hipSuccess)
       return fail(c, BNZ_ENOMEM, "device allocation of grid arrays failed");
-  double **bufs3[] = {&c->b, &c->bs, &c->ee};
   for (double **q : bufs3)
     if (hipMalloc(q, 3 * cells * sizeof(double)) != hipSuccess)
       return fail(c, BNZ_ENOMEM, "device allocation of field arrays failed");
   // zero everything once so stale reads of never-written ghost faces are defined
   for (double **q : bufs9) HIPCHK(c, hipMemsetAsync(*q, 0, NVAR_STATE * cells * sizeof(double), c->stream));
   for (double **q : bufs3) HIPCHK(c, hipMemsetAsync(*q, 0, 3 * cells * sizeof(double), c->stream));
+  if (c->f32) {
+    int rc = f32_alloc(c);
+    if (rc) return rc;
+  }
 
   size_t geo_n = 0;
   for (int a = 0; a < 3; ++a) geo_n += 4 * (size_t)c->nt[a];
@@ -1402,6 +1179,7 @@ int bnz_destroy(bnz_ctx *c) {
   for (double *q : bufs)
     if (q) (void)hipFree(q);
   if (c->maps) (void)hipFree(c->maps);
+  f32_free(c);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return BNZ_OK;
@@ -1425,24 +1203,36 @@ int bnz_upload(bnz_ctx *c, const double *prim, const double *bfld) {
 
 int bnz_download(bnz_ctx *c, double *prim, double *bfld, double *cons) {
   if (!c || !c->stream) return BNZ_EINVAL;
+  if (c->f32 && c->ready) {
+    int rc = f32_to64(c);
+    if (rc) return rc;
+  }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (prim) HIPCHK(c, hipMemcpy(prim, c->w, NMODE * c->ncell * sizeof(double), hipMemcpyDeviceToHost));
   if (bfld) HIPCHK(c, hipMemcpy(bfld, c->b, 3 * c->ncell * sizeof(double), hipMemcpyDeviceToHost));
   if (cons) {
-    // ws is dead between steps; it holds prim2cons(prim) for the copy
+    // ws is dead between steps; it holds prim2cons(prim) for the copy (the
+    // FP32 step has no FP64 ws, so a temporary is used)
+    double *tmp = c->ws;
+    if (!tmp) HIPCHK(c, hipMalloc(&tmp, NMODE * c->ncell * sizeof(double)));
     hipLaunchKernelGGL(k_prim2cons, dim3(blocks_for(c->ncell)), dim3(BLK), 0, c->stream, c->g,
-                       c->w, c->ws);
+                       c->w, tmp);
     int rc = check_launch(c);
+    if (!rc) {
+      hipError_t e1 = hipStreamSynchronize(c->stream);
+      hipError_t e2 = hipMemcpy(cons, tmp, NMODE * c->ncell * sizeof(double), hipMemcpyDeviceToHost);
+      if (e1 != hipSuccess || e2 != hipSuccess) rc = fail(c, BNZ_EHIP, "copy of cons failed");
+    }
+    if (tmp != c->ws) (void)hipFree(tmp);
     if (rc) return rc;
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    HIPCHK(c, hipMemcpy(cons, c->ws, NMODE * c->ncell * sizeof(double), hipMemcpyDeviceToHost));
   }
   return BNZ_OK;
 }
 
 int bnz_fill_ghosts(bnz_ctx *c) {
   if (!c || !c->stream) return BNZ_EINVAL;
-  launch_ghost(c);
+  if (c->f32 && c->ready) f32_ghost(c);
+  else launch_ghost(c);
   return check_launch(c);
 }
 
@@ -1454,6 +1244,10 @@ int bnz_init_state(bnz_ctx *c) {
   hipLaunchKernelGGL(k_interp_active, dim3(blocks_for(a.total())), dim3(BLK), 0, c->stream,
                      c->g, c->w, c->b, a);
   launch_ghost(c);
+  if (c->f32) {
+    int rc = f32_from64(c);
+    if (rc) return rc;
+  }
   const double dev0[4] = {0., 0., -1., 0.};
   HIPCHK(c, hipMemcpyAsync(c->dev, dev0, sizeof(dev0), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1466,7 +1260,7 @@ int bnz_new_dt(bnz_ctx *c, double *dt) {
   if (!c->ready) return fail(c, BNZ_ESTATE, "bnz_init_state has not run");
   const double nomax = -1.;
   HIPCHK(c, hipMemcpyAsync(c->dev + 2, &nomax, sizeof(double), hipMemcpyHostToDevice, c->stream));
-  launch_dt(c, 0, 0.);
+  any_dt(c, 0, 0.);
   int rc = check_launch(c);
   if (rc) return rc;
   HIPCHK(c, hipMemcpyAsync(dt, c->dev, sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -1478,8 +1272,8 @@ int bnz_vl_step(bnz_ctx *c, double dt) {
   if (!c) return BNZ_EINVAL;
   if (!c->ready) return fail(c, BNZ_ESTATE, "bnz_init_state has not run");
   if (!(dt > 0.)) return fail(c, BNZ_EINVAL, "dt must be > 0");
-  launch_dt(c, 1, dt);
-  launch_step_body(c);
+  any_dt(c, 1, dt);
+  any_step(c);
   return check_launch(c);
 }
 
@@ -1489,8 +1283,8 @@ int bnz_run_steps(bnz_ctx *c, int nsteps) {
   const double nomax = -1.;
   HIPCHK(c, hipMemcpyAsync(c->dev + 2, &nomax, sizeof(double), hipMemcpyHostToDevice, c->stream));
   for (int s = 0; s < nsteps; ++s) {
-    launch_dt(c, 0, 0.);
-    launch_step_body(c);
+    any_dt(c, 0, 0.);
+    any_step(c);
   }
   return check_launch(c);
 }
@@ -1504,8 +1298,8 @@ int bnz_advance(bnz_ctx *c, double tmax, int max_steps, int *nsteps) {
   HIPCHK(c, hipMemcpyAsync(&t, c->dev + 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   while (t < tmax && (max_steps <= 0 || n < max_steps)) {
-    launch_dt(c, 0, 0.);
-    launch_step_body(c);
+    any_dt(c, 0, 0.);
+    any_step(c);
     int rc = check_launch(c);
     if (rc) return rc;
     ++n;
@@ -1540,6 +1334,10 @@ int bnz_set_time(bnz_ctx *c, double time, int step) {
 int bnz_diag(bnz_ctx *c, bnz_diag_out *out) {
   if (!c || !out) return BNZ_EINVAL;
   c->dt_parts = 0;  // the partial buffer is reused below
+  if (c->f32 && c->ready) {
+    int rc0 = f32_to64(c);
+    if (rc0) return rc0;
+  }
   Box a = active_box(c);
   const int nb = blocks_for(a.total());
   hipLaunchKernelGGL(k_diag, dim3(nb), dim3(BLK), 0, c->stream, c->g, c->w, c->b, a, c->part);
@@ -1568,6 +1366,9 @@ int bnz_diag(bnz_ctx *c, bnz_diag_out *out) {
   out->pmin = -s[8];
   out->rhomin = -s[9];
   out->nonfinite = (long)s[10];
+  int ng = 0;
+  HIPCHK(c, hipMemcpy(&ng, (int *)(c->dev + 4), sizeof(int), hipMemcpyDeviceToHost));
+  out->hlld_guards = ng;
   return BNZ_OK;
 }
 

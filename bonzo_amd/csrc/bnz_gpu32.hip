@@ -1,0 +1,536 @@
+// bnz_gpu32.hip -- FP32-compensated Van-Leer step (BNZ_PREC_FP32C).
+//
+// Same sequence as the FP64 step (src/integration/vanleer.pyx:100-163) with
+// the arithmetic split by need:
+//   FP32   reconstruction, Riemann solvers, EMFs (hardware rcp/sqrt, FMA)
+//          and the predictor's half-step update;
+//   FP64   the corrector's conserved and face-field updates (applied to the
+//          hi + lo pairs of the state, whose low words carry the rounding
+//          residual from step to step), the pressure recovery
+//          p = (gam-1)(E - (B^2 + rho v^2)/2), which cancels at low beta,
+//          and the CFL maximum.
+// Face fields are stored in FP64 (the same 8 bytes as a hi + lo pair, without
+// the pair's 48-bit rounding of every face each step); their increments are
+// FP64 functions of FP32 edge EMFs that each edge shares with the faces
+// around it, so div B stays at FP64 round-off.  The conserved state is the
+// hi + lo pair (not primitive variables): the low words are only meaningful
+// on the quantities that the update accumulates, and the predictor reads the
+// hi words alone.
+#include "bnz_common.hpp"
+
+using namespace bnz;
+
+namespace {
+
+__device__ __forceinline__ void split(double v, float &hi, float &lo) {
+  hi = (float)v;
+  lo = (float)(v - (double)hi);
+}
+__device__ __forceinline__ double join(float hi, float lo) { return (double)hi + (double)lo; }
+
+// Godunov fluxes, FP32 (same stencil and ranges as k_flux).
+template <int DIR, int ORDER, int RS, typename BF>
+__global__ __launch_bounds__(BLK) void k32_flux(Geo g, const float *__restrict__ w,
+                                                const BF *__restrict__ bf,
+                                                float *__restrict__ fl, Box box, int *nguard) {
+  int i, j, k;
+  if (!cell_index<false, 1, 2>(box, i, j, k)) return;
+  const long c = ((long)k * g.nt[1] + j) * g.nt[0] + i;
+  const long sd = (DIR == 0) ? 1 : (DIR == 1) ? g.sy : g.sz;
+  const float bn = (float)bf[(long)DIR * g.sv + c];
+  float wl[NMODE], wr[NMODE];
+#pragma unroll
+  for (int n = 0; n < NMODE; ++n) {
+    if (n == BX) continue;
+    const float *p = w + (long)perm<DIR>(n) * g.sv + c;
+    if (ORDER == 0) {
+      wl[n] = p[-sd];
+      wr[n] = p[0];
+    } else {
+      const float wm2 = p[-2 * sd], wm1 = p[-sd], w0 = p[0], wp1 = p[sd];
+      wl[n] = wm1 + 0.5f * vl_lim(wm1 - wm2, w0 - wm1);
+      wr[n] = w0 - 0.5f * vl_lim(w0 - wm1, wp1 - w0);
+    }
+  }
+  wl[BX] = bn;
+  wr[BX] = bn;
+  float f[NMODE];
+  int ng = 0;
+  const float gam = (float)g.gam, gm1i = (float)g.gamm1i;
+  const int reg = (RS == BNZ_RS_HLLD) ? flux_hlld(f, wl, wr, gam, gm1i, &ng)
+                                      : flux_hll(f, wl, wr, gam, gm1i);
+  if (ng) atomicAdd(nguard, ng);
+  if (reg == 0) return;
+#pragma unroll
+  for (int n = 0; n < NMODE; ++n) {
+    if (n == BX) continue;
+    fl[(long)perm<DIR>(n) * g.sv + c] = f[n];
+  }
+}
+
+__device__ __forceinline__ float ec32(const float *w, long sv, long c, int comp) {
+  const float vx = w[VX * sv + c], vy = w[VY * sv + c], vz = w[VZ * sv + c];
+  const float bx = w[BX * sv + c], by = w[BY * sv + c], bz = w[BZ * sv + c];
+  if (comp == 0) return -(vy * bz - vz * by);
+  if (comp == 1) return (vx * bz - vz * bx);
+  return -(vx * by - vy * bx);
+}
+
+// Edge EMFs, FP32 (interp_ee1, src/integration/ct.pyx:153-215).
+__global__ __launch_bounds__(BLK) void k32_emf(Geo g, const float *__restrict__ w,
+                                               const float *__restrict__ fx,
+                                               const float *__restrict__ fy,
+                                               const float *__restrict__ fz,
+                                               float *__restrict__ ee, Box box) {
+  int i, j, k;
+  if (!cell_index<false, 1, 2>(box, i, j, k)) return;
+  const long sv = g.sv, sy = g.sy, sz = g.sz;
+  const long c = ((long)k * g.nt[1] + j) * g.nt[0] + i;
+  if (g.dims == 3) {
+    ee[c] = 0.5f * (-fy[BZ * sv + c - sz] - fy[BZ * sv + c] + fz[BY * sv + c - sy] + fz[BY * sv + c]) -
+            0.25f * (ec32(w, sv, c - sy, 0) + ec32(w, sv, c, 0) + ec32(w, sv, c - sz - sy, 0) +
+                     ec32(w, sv, c - sz, 0));
+    ee[sv + c] = 0.5f * (-fz[BX * sv + c - 1] - fz[BX * sv + c] + fx[BZ * sv + c - sz] + fx[BZ * sv + c]) -
+                 0.25f * (ec32(w, sv, c - 1, 1) + ec32(w, sv, c, 1) + ec32(w, sv, c - sz - 1, 1) +
+                          ec32(w, sv, c - sz, 1));
+  } else if (g.dims == 2) {
+    ee[c] = -fy[BZ * sv + c];
+    ee[sv + c] = fx[BZ * sv + c];
+  } else {
+    ee[c] = -ec32(w, sv, c, 0);
+    ee[sv + c] = fx[BZ * sv + c];
+    ee[2 * sv + c] = -fx[BY * sv + c];
+    return;
+  }
+  ee[2 * sv + c] = 0.5f * (-fx[BY * sv + c - sy] - fx[BY * sv + c] + fy[BX * sv + c - 1] + fy[BX * sv + c]) -
+                   0.25f * (ec32(w, sv, c - 1, 2) + ec32(w, sv, c, 2) + ec32(w, sv, c - sy - 1, 2) +
+                            ec32(w, sv, c - sy, 2));
+}
+
+// Face-field update by Stokes (ct.pyx:15-91) of one face in working type R
+// from face values b (R) and FP32 EMFs.
+template <typename R, typename BF>
+__device__ __forceinline__ R face_x(const Geo &g, BF b, const float *ee, double dt, int i, int j,
+                                    int k, long c) {
+  const long sv = g.sv;
+  const R dtda = (R)(dt / (g.dlf[1][j] * g.dlf[2][k]));
+  R v = b(0, c);
+  if (g.dims >= 2) {
+    const R ds = (R)g.dlf[2][k];
+    v -= dtda * (ds * (R)ee[2 * sv + c + g.sy] - ds * (R)ee[2 * sv + c]);
+  }
+  if (g.dims >= 3) {
+    const R ds = (R)g.dlf[1][j];
+    v += dtda * (ds * (R)ee[sv + c + g.sz] - ds * (R)ee[sv + c]);
+  }
+  return v;
+}
+template <typename R, typename BF>
+__device__ __forceinline__ R face_y(const Geo &g, BF b, const float *ee, double dt, int i, int j,
+                                    int k, long c) {
+  const long sv = g.sv;
+  const R dtda = (R)(dt / (g.dlf[0][i] * g.dlf[2][k]));
+  R ds = (R)g.dlf[2][k];
+  R v = b(1, c) + dtda * (ds * (R)ee[2 * sv + c + 1] - ds * (R)ee[2 * sv + c]);
+  if (g.dims >= 3) {
+    ds = (R)g.dlf[0][i];
+    v -= dtda * (ds * (R)ee[c + g.sz] - ds * (R)ee[c]);
+  }
+  return v;
+}
+template <typename R, typename BF>
+__device__ __forceinline__ R face_z(const Geo &g, BF b, const float *ee, double dt, int i, int j,
+                                    int k, long c) {
+  const long sv = g.sv;
+  const R dtda = (R)(dt / (g.dlf[0][i] * g.dlf[1][j]));
+  R ds = (R)g.dlf[1][j];
+  R v = b(2, c) - dtda * (ds * (R)ee[sv + c + 1] - ds * (R)ee[sv + c]);
+  if (g.dims >= 2) {
+    ds = (R)g.dlf[0][i];
+    v += dtda * (ds * (R)ee[c + g.sy] - ds * (R)ee[c]);
+  }
+  return v;
+}
+
+struct Faces32 {
+  const double *b;
+  long sv;
+  __device__ float operator()(int a, long c) const { return (float)b[a * sv + c]; }
+};
+struct Faces64 {
+  const double *b;
+  long sv;
+  __device__ double operator()(int a, long c) const { return b[a * sv + c]; }
+};
+
+struct Pred32 {
+  const float *w32, *uh;
+  const double *b;
+  const float *fx, *fy, *fz, *ee;
+  float *bs, *ws;
+};
+
+// Predictor (dt/2): FP32 update of the hi words, FP64 pressure recovery.
+__global__ __launch_bounds__(BLK) void k32_update_pred(Geo g, Pred32 a, Lims L,
+                                                       const double *__restrict__ dts) {
+  const long t = xcd_block() * BLK + threadIdx.x;
+  const long ncell = (long)g.nt[0] * g.nt[1] * g.nt[2];
+  if (t >= ncell) return;
+  const int i = (int)(t % g.nt[0]);
+  const long r = t / g.nt[0];
+  const int j = (int)(r % g.nt[1]), k = (int)(r / g.nt[1]);
+  const long c = t, sv = g.sv;
+  const double dt = 0.5 * dts[0];
+  const bool inI = i >= L.lo[0] && i <= L.hi[0], inJ = j >= L.lo[1] && j <= L.hi[1];
+  const bool inK = k >= L.lo[2] && k <= L.hi[2];
+  const bool cell = inI && inJ && inK;
+  const bool fX = (i >= L.lo[0] && i <= L.hi[0] + 1) && inJ && inK;
+  const bool fY = inI && (j >= L.lo[1] && j <= L.hi[1] + (g.dims >= 2 ? 1 : 0)) && inK;
+  const bool fZ = inI && inJ && (k >= L.lo[2] && k <= L.hi[2] + (g.dims >= 3 ? 1 : 0));
+  const Faces32 B{a.b, sv};
+  float bxl = 0.f, byl = 0.f, bzl = 0.f;
+  if (fX) a.bs[c] = bxl = face_x<float>(g, B, a.ee, dt, i, j, k, c);
+  if (fY) a.bs[sv + c] = byl = face_y<float>(g, B, a.ee, dt, i, j, k, c);
+  if (fZ) a.bs[2 * sv + c] = bzl = face_z<float>(g, B, a.ee, dt, i, j, k, c);
+  if (!cell) {
+#pragma unroll
+    for (int n = 0; n < NMODE; ++n) a.ws[n * sv + c] = a.w32[n * sv + c];
+    return;
+  }
+  const float dlx = (float)g.dlf[0][i], dly = (float)g.dlf[1][j], dlz = (float)g.dlf[2][k];
+  const float dtdv = (float)(dt / (g.dlf[0][i] * g.dlf[1][j] * g.dlf[2][k]));
+  const float ax = dly * dlz, ay = dlx * dlz, az = dlx * dly;
+  double u[NMODE], wv[NMODE];
+#pragma unroll
+  for (int n = 0; n < NMODE - 3; ++n) {
+    float v = a.uh[n * sv + c] - dtdv * (ax * a.fx[n * sv + c + 1] - ax * a.fx[n * sv + c]);
+    if (g.dims >= 2) v -= dtdv * (ay * a.fy[n * sv + c + g.sy] - ay * a.fy[n * sv + c]);
+    if (g.dims >= 3) v -= dtdv * (az * a.fz[n * sv + c + g.sz] - az * a.fz[n * sv + c]);
+    u[n] = v;
+  }
+  const float bxu = face_x<float>(g, B, a.ee, dt, i + 1, j, k, c + 1);
+  u[BX] = ((float)g.icm[0][i] * bxl + (float)g.icp[0][i] * bxu) * (float)g.dlfi[0][i];
+  if (g.dims >= 2) {
+    const float byu = face_y<float>(g, B, a.ee, dt, i, j + 1, k, c + g.sy);
+    u[BY] = ((float)g.icm[1][j] * byl + (float)g.icp[1][j] * byu) * (float)g.dlfi[1][j];
+  } else {
+    u[BY] = byl;
+  }
+  if (g.dims >= 3) {
+    const float bzu = face_z<float>(g, B, a.ee, dt, i, j, k + 1, c + g.sz);
+    u[BZ] = ((float)g.icm[2][k] * bzl + (float)g.icp[2][k] * bzu) * (float)g.dlfi[2][k];
+  } else {
+    u[BZ] = bzl;
+  }
+  cons2prim(wv, u, g.gam);
+#pragma unroll
+  for (int n = 0; n < NMODE; ++n) a.ws[n * sv + c] = (float)wv[n];
+}
+
+struct Corr32 {
+  float *uh, *ul;
+  const double *b;
+  const float *fx, *fy, *fz, *ee;
+  double *b1;
+  float *w32;
+};
+
+// Corrector (dt): FP64 update of the hi + lo state, FP64 pressure recovery,
+// fused CFL maximum of the new state.
+__global__ __launch_bounds__(BLK) void k32_update_corr(Geo g, Corr32 a, Lims L,
+                                                       const double *__restrict__ dts, Box act,
+                                                       int fms_ref, double *__restrict__ part) {
+  const long t = xcd_block() * BLK + threadIdx.x;
+  const long ncell = (long)g.nt[0] * g.nt[1] * g.nt[2];
+  double cdsmax = 0.;
+  if (t < ncell) {
+    const int i = (int)(t % g.nt[0]);
+    const long r = t / g.nt[0];
+    const int j = (int)(r % g.nt[1]), k = (int)(r / g.nt[1]);
+    const long c = t, sv = g.sv;
+    const double dt = dts[0];
+    const bool inI = i >= L.lo[0] && i <= L.hi[0], inJ = j >= L.lo[1] && j <= L.hi[1];
+    const bool inK = k >= L.lo[2] && k <= L.hi[2];
+    const bool cell = inI && inJ && inK;
+    const bool fX = (i >= L.lo[0] && i <= L.hi[0] + 1) && inJ && inK;
+    const bool fY = inI && (j >= L.lo[1] && j <= L.hi[1] + (g.dims >= 2 ? 1 : 0)) && inK;
+    const bool fZ = inI && inJ && (k >= L.lo[2] && k <= L.hi[2] + (g.dims >= 3 ? 1 : 0));
+    const Faces64 B{a.b, sv};
+    double bxl = 0., byl = 0., bzl = 0.;
+    if (fX) a.b1[c] = bxl = face_x<double>(g, B, a.ee, dt, i, j, k, c);
+    if (fY) a.b1[sv + c] = byl = face_y<double>(g, B, a.ee, dt, i, j, k, c);
+    if (fZ) a.b1[2 * sv + c] = bzl = face_z<double>(g, B, a.ee, dt, i, j, k, c);
+    if (cell) {
+      const double dlx = g.dlf[0][i], dly = g.dlf[1][j], dlz = g.dlf[2][k];
+      const double dtdv = dt / (dlx * dly * dlz);
+      const double ax = dly * dlz, ay = dlx * dlz, az = dlx * dly;
+      double u[NMODE], wv[NMODE];
+#pragma unroll
+      for (int n = 0; n < NMODE - 3; ++n) {
+        double v = join(a.uh[n * sv + c], a.ul[n * sv + c]) -
+                   dtdv * (ax * (double)a.fx[n * sv + c + 1] - ax * (double)a.fx[n * sv + c]);
+        if (g.dims >= 2)
+          v -= dtdv * (ay * (double)a.fy[n * sv + c + g.sy] - ay * (double)a.fy[n * sv + c]);
+        if (g.dims >= 3)
+          v -= dtdv * (az * (double)a.fz[n * sv + c + g.sz] - az * (double)a.fz[n * sv + c]);
+        u[n] = v;
+        split(v, a.uh[n * sv + c], a.ul[n * sv + c]);
+      }
+      const double bxu = face_x<double>(g, B, a.ee, dt, i + 1, j, k, c + 1);
+      u[BX] = (g.icm[0][i] * bxl + g.icp[0][i] * bxu) * g.dlfi[0][i];
+      if (g.dims >= 2) {
+        const double byu = face_y<double>(g, B, a.ee, dt, i, j + 1, k, c + g.sy);
+        u[BY] = (g.icm[1][j] * byl + g.icp[1][j] * byu) * g.dlfi[1][j];
+      } else {
+        u[BY] = byl;
+      }
+      if (g.dims >= 3) {
+        const double bzu = face_z<double>(g, B, a.ee, dt, i, j, k + 1, c + g.sz);
+        u[BZ] = (g.icm[2][k] * bzl + g.icp[2][k] * bzu) * g.dlfi[2][k];
+      } else {
+        u[BZ] = bzl;
+      }
+      cons2prim(wv, u, g.gam);
+#pragma unroll
+      for (int n = 0; n < NMODE; ++n) a.w32[n * sv + c] = (float)wv[n];
+      if (i >= act.lo[0] && i < act.lo[0] + act.n[0] && j >= act.lo[1] &&
+          j < act.lo[1] + act.n[1] && k >= act.lo[2] && k < act.lo[2] + act.n[2])
+        cdsmax = cds_cell(g, wv, fms_ref, i, j, k);
+    }
+  }
+  block_max_to(cdsmax, part);
+}
+
+// CFL maximum from the FP32 primitives (first step after an upload).
+__global__ __launch_bounds__(BLK) void k32_dt_partial(Geo g, const float *__restrict__ w, Box box,
+                                                      int fms_ref, double *__restrict__ part) {
+  int i, j, k;
+  double cds = 0.;
+  if (cell_index<false, 1, 2>(box, i, j, k)) {
+    const long c = ((long)k * g.nt[1] + j) * g.nt[0] + i;
+    double wv[NMODE];
+#pragma unroll
+    for (int n = 0; n < NMODE; ++n) wv[n] = (double)w[n * g.sv + c];
+    cds = cds_cell(g, wv, fms_ref, i, j, k);
+  }
+  block_max_to(cds, part);
+}
+
+struct Ghost32 {
+  float *a[21];   // uh, ul, w32
+  double *d[3];   // face fields
+};
+
+__global__ __launch_bounds__(BLK) void k32_ghost(Geo g, Ghost32 p, const int *__restrict__ mx,
+                                                 const int *__restrict__ my,
+                                                 const int *__restrict__ mz) {
+  const long t = xcd_block() * BLK + threadIdx.x;
+  const long ncell = (long)g.nt[0] * g.nt[1] * g.nt[2];
+  if (t >= ncell) return;
+  const int i = (int)(t % g.nt[0]);
+  const long r = t / g.nt[0];
+  const int j = (int)(r % g.nt[1]), k = (int)(r / g.nt[1]);
+  const int si = mx[i], sj = my[j], sk = mz[k];
+  if (si == i && sj == j && sk == k) return;
+  const long s = ((long)sk * g.nt[1] + sj) * g.nt[0] + si;
+#pragma unroll
+  for (int n = 0; n < 21; ++n) p.a[n][t] = p.a[n][s];
+#pragma unroll
+  for (int n = 0; n < 3; ++n) p.d[n][t] = p.d[n][s];
+}
+
+// FP64 prim (all cells) -> FP32-compensated state.
+__global__ __launch_bounds__(BLK) void k32_from64(Geo g, const double *__restrict__ w, float *uh,
+                                                  float *ul, float *w32) {
+  const long t = xcd_block() * BLK + threadIdx.x;
+  const long ncell = (long)g.nt[0] * g.nt[1] * g.nt[2];
+  if (t >= ncell) return;
+  const long sv = g.sv;
+  double wv[NMODE], u[NMODE];
+#pragma unroll
+  for (int n = 0; n < NMODE; ++n) {
+    wv[n] = w[n * sv + t];
+    w32[n * sv + t] = (float)wv[n];
+  }
+  prim2cons(u, wv, g.gamm1i);
+#pragma unroll
+  for (int n = 0; n < NMODE - 3; ++n) split(u[n], uh[n * sv + t], ul[n * sv + t]);
+}
+
+// FP32-compensated state -> FP64 prim of the active cells (the caller fills
+// the ghosts).
+__global__ __launch_bounds__(BLK) void k32_to64(Geo g, const float *uh, const float *ul,
+                                                const double *b, double *w, Box act) {
+  const long t = xcd_block() * BLK + threadIdx.x;
+  const long ncell = (long)g.nt[0] * g.nt[1] * g.nt[2];
+  if (t >= ncell) return;
+  const long sv = g.sv;
+  const int i = (int)(t % g.nt[0]);
+  const long r = t / g.nt[0];
+  const int j = (int)(r % g.nt[1]), k = (int)(r / g.nt[1]);
+  if (i < act.lo[0] || i >= act.lo[0] + act.n[0] || j < act.lo[1] || j >= act.lo[1] + act.n[1] ||
+      k < act.lo[2] || k >= act.lo[2] + act.n[2])
+    return;
+  double u[NMODE], wv[NMODE];
+#pragma unroll
+  for (int n = 0; n < NMODE - 3; ++n) u[n] = join(uh[n * sv + t], ul[n * sv + t]);
+  u[BX] = (g.icm[0][i] * b[t] + g.icp[0][i] * b[t + 1]) * g.dlfi[0][i];
+  if (g.dims >= 2)
+    u[BY] = (g.icm[1][j] * b[sv + t] + g.icp[1][j] * b[sv + t + g.sy]) * g.dlfi[1][j];
+  else
+    u[BY] = b[sv + t];
+  if (g.dims >= 3)
+    u[BZ] = (g.icm[2][k] * b[2 * sv + t] + g.icp[2][k] * b[2 * sv + t + g.sz]) * g.dlfi[2][k];
+  else
+    u[BZ] = b[2 * sv + t];
+  cons2prim(wv, u, g.gam);
+#pragma unroll
+  for (int n = 0; n < NMODE; ++n) w[n * sv + t] = wv[n];
+}
+
+template <int ORDER, int RS, typename BF>
+void launch_fluxes32(bnz_ctx *c, const float *w, const BF *bf, const Lims &L) {
+  const int il = L.lo[0], iu = L.hi[0], jl = L.lo[1], ju = L.hi[1], kl = L.lo[2], ku = L.hi[2];
+  int *ng = (int *)(c->dev + 4);
+  {
+    Box bx;
+    bx.lo[0] = il; bx.n[0] = iu + 1 - il + 1;
+    bx.lo[1] = (c->dims >= 2) ? jl - 1 : 0; bx.n[1] = (c->dims >= 2) ? (ju + 1) - (jl - 1) + 1 : 1;
+    bx.lo[2] = (c->dims >= 3) ? kl - 1 : 0; bx.n[2] = (c->dims >= 3) ? (ku + 1) - (kl - 1) + 1 : 1;
+    Launch t(c, ORDER ? "flux_x_plm" : "flux_x_dc");
+    hipLaunchKernelGGL((k32_flux<0, ORDER, RS, BF>), dim3(blocks_for(bx.total())), dim3(BLK), 0,
+                       c->stream, c->g, w, bf, c->fx32, bx, ng);
+  }
+  if (c->dims >= 2) {
+    Box by;
+    by.lo[0] = il - 1; by.n[0] = (iu + 1) - (il - 1) + 1;
+    by.lo[1] = jl; by.n[1] = ju + 1 - jl + 1;
+    by.lo[2] = (c->dims >= 3) ? kl - 1 : 0; by.n[2] = (c->dims >= 3) ? (ku + 1) - (kl - 1) + 1 : 1;
+    Launch t(c, ORDER ? "flux_y_plm" : "flux_y_dc");
+    hipLaunchKernelGGL((k32_flux<1, ORDER, RS, BF>), dim3(blocks_for(by.total())), dim3(BLK), 0,
+                       c->stream, c->g, w, bf, c->fy32, by, ng);
+  }
+  if (c->dims >= 3) {
+    Box bz;
+    bz.lo[0] = il - 1; bz.n[0] = (iu + 1) - (il - 1) + 1;
+    bz.lo[1] = jl - 1; bz.n[1] = (ju + 1) - (jl - 1) + 1;
+    bz.lo[2] = kl; bz.n[2] = ku + 1 - kl + 1;
+    Launch t(c, ORDER ? "flux_z_plm" : "flux_z_dc");
+    hipLaunchKernelGGL((k32_flux<2, ORDER, RS, BF>), dim3(blocks_for(bz.total())), dim3(BLK), 0,
+                       c->stream, c->g, w, bf, c->fz32, bz, ng);
+  }
+}
+
+template <typename BF>
+void fluxes32(bnz_ctx *c, int order, const float *w, const BF *bf, const Lims &L) {
+  const bool hlld = c->p.rsolver == BNZ_RS_HLLD;
+  if (order == 0) {
+    if (hlld) launch_fluxes32<0, BNZ_RS_HLLD>(c, w, bf, L);
+    else launch_fluxes32<0, BNZ_RS_HLL>(c, w, bf, L);
+  } else {
+    if (hlld) launch_fluxes32<1, BNZ_RS_HLLD>(c, w, bf, L);
+    else launch_fluxes32<1, BNZ_RS_HLL>(c, w, bf, L);
+  }
+}
+
+void emf32(bnz_ctx *c, const float *w, const Lims &L, const char *name) {
+  Box e;
+  e.lo[0] = L.lo[0]; e.n[0] = L.hi[0] + 1 - L.lo[0] + 1;
+  e.lo[1] = (c->dims >= 2) ? L.lo[1] : 0; e.n[1] = (c->dims >= 2) ? L.hi[1] + 1 - L.lo[1] + 1 : 1;
+  e.lo[2] = (c->dims >= 3) ? L.lo[2] : 0; e.n[2] = (c->dims >= 3) ? L.hi[2] + 1 - L.lo[2] + 1 : 1;
+  Launch t(c, name);
+  hipLaunchKernelGGL(k32_emf, dim3(blocks_for(e.total())), dim3(BLK), 0, c->stream, c->g, w,
+                     c->fx32, c->fy32, c->fz32, c->ee32, e);
+}
+
+void ghost32(bnz_ctx *c) {
+  Ghost32 p;
+  int q = 0;
+  for (int n = 0; n < 6; ++n) p.a[q++] = c->uh + n * c->ncell;
+  for (int n = 0; n < 6; ++n) p.a[q++] = c->ul + n * c->ncell;
+  for (int n = 0; n < 9; ++n) p.a[q++] = c->w32 + n * c->ncell;
+  for (int n = 0; n < 3; ++n) p.d[n] = c->b + n * c->ncell;
+  Launch t(c, "ghost");
+  hipLaunchKernelGGL(k32_ghost, dim3(blocks_for(c->ncell)), dim3(BLK), 0, c->stream, c->g, p,
+                     c->maps, c->maps + c->nt[0], c->maps + c->nt[0] + c->nt[1]);
+}
+
+}  // namespace
+
+namespace bnz {
+
+int f32_alloc(bnz_ctx *c) {
+  const size_t n = (size_t)c->ncell * sizeof(float);
+  struct {
+    float **p;
+    int rows;
+  } bufs[] = {{&c->uh, 6},   {&c->ul, 6},   {&c->w32, 9},  {&c->ws32, 9}, {&c->bs32, 3},
+              {&c->fx32, 9}, {&c->fy32, 9}, {&c->fz32, 9}, {&c->ee32, 3}};
+  for (auto &q : bufs) {
+    if (hipMalloc(q.p, q.rows * n) != hipSuccess)
+      return fail(c, BNZ_ENOMEM, "device allocation of FP32 arrays failed");
+    HIPCHK(c, hipMemsetAsync(*q.p, 0, q.rows * n, c->stream));
+  }
+  return BNZ_OK;
+}
+
+void f32_free(bnz_ctx *c) {
+  float *bufs[] = {c->uh, c->ul, c->w32, c->ws32, c->bs32, c->fx32, c->fy32, c->fz32, c->ee32};
+  for (float *q : bufs)
+    if (q) (void)hipFree(q);
+}
+
+int f32_from64(bnz_ctx *c) {
+  hipLaunchKernelGGL(k32_from64, dim3(blocks_for(c->ncell)), dim3(BLK), 0, c->stream, c->g, c->w,
+                     c->uh, c->ul, c->w32);
+  c->dt_parts = 0;
+  return check_launch(c);
+}
+
+int f32_to64(bnz_ctx *c) {
+  hipLaunchKernelGGL(k32_to64, dim3(blocks_for(c->ncell)), dim3(BLK), 0, c->stream, c->g, c->uh,
+                     c->ul, c->b, c->w, active_box(c));
+  launch_ghost64(c);
+  return check_launch(c);
+}
+
+void f32_ghost(bnz_ctx *c) { ghost32(c); }
+
+void f32_dt(bnz_ctx *c, int use_fixed, double fixed_dt) {
+  int nb = c->dt_parts;
+  if (nb <= 0) {
+    Box a = active_box(c);
+    nb = blocks_for(a.total());
+    Launch t(c, "dt_partial");
+    hipLaunchKernelGGL(k32_dt_partial, dim3(nb), dim3(BLK), 0, c->stream, c->g, c->w32, a,
+                       c->p.dt_fms_ref, c->part);
+  }
+  launch_dt_final(c, nb, use_fixed, fixed_dt);
+}
+
+void f32_step_body(bnz_ctx *c) {
+  const int nb = blocks_for(c->ncell);
+  Lims L0 = integr_lims(c, 0);
+  fluxes32(c, 0, c->w32, c->b, L0);
+  emf32(c, c->w32, L0, "emf_pred");
+  {
+    Pred32 a{c->w32, c->uh, c->b, c->fx32, c->fy32, c->fz32, c->ee32, c->bs32, c->ws32};
+    Launch t(c, "update_pred");
+    hipLaunchKernelGGL(k32_update_pred, dim3(nb), dim3(BLK), 0, c->stream, c->g, a, L0, c->dev);
+  }
+  Lims L1 = integr_lims(c, c->order);
+  fluxes32(c, c->order, c->ws32, c->bs32, L1);
+  emf32(c, c->ws32, L1, "emf_corr");
+  {
+    Corr32 a{c->uh, c->ul, c->b, c->fx32, c->fy32, c->fz32, c->ee32, c->bs, c->w32};
+    Launch t(c, "update_corr");
+    hipLaunchKernelGGL(k32_update_corr, dim3(nb), dim3(BLK), 0, c->stream, c->g, a, L1, c->dev,
+                       active_box(c), c->p.dt_fms_ref, c->part);
+  }
+  c->dt_parts = nb;
+  std::swap(c->b, c->bs);
+  ghost32(c);
+  launch_time_advance(c);
+}
+
+}  // namespace bnz

@@ -1,8 +1,12 @@
-// bnz_phys.hpp -- device-side ideal-MHD point physics for the VL step (FP64).
+// bnz_phys.hpp -- device-side ideal-MHD point physics of the VL step.
 //
-// Operation order follows the reference expression by expression
-// (src/mhd/eos_c.h, src/mhd/fluxes_c.c, src/coordinates/reconstr_c.h) so that
-// an IEEE build (-ffp-contract=off) reproduces the CPU oracle bit for bit.
+// Templated on the working type T.  T = double follows the reference
+// expression by expression (src/mhd/eos_c.h, src/mhd/fluxes_c.c,
+// src/coordinates/reconstr_c.h) with IEEE division and square root, so an
+// -ffp-contract=off build reproduces the CPU oracle bit for bit.  T = float is
+// the fast arithmetic of the FP32-compensated step: hardware reciprocal and
+// square root (about 1 ulp), FMA contraction allowed by its translation unit,
+// and guards where HLLD's denominators vanish.
 // States are 9-slot register arrays in the local frame of the sweep:
 //   0 rho, 1 vx|mx, 2 vy|my, 3 vz|mz, 4 p|E, 5 passive scalar, 6 Bx, 7 By, 8 Bz
 // (src/defs_c.h:68-82).
@@ -15,27 +19,44 @@ enum { RHO = 0, MX = 1, MY = 2, MZ = 3, EN = 4, PSC = 5, BX = 6, BY = 7, BZ = 8 
 enum { VX = 1, VY = 2, VZ = 3, PR = 4 };
 constexpr int NMODE = 9;
 
-__device__ __forceinline__ double sqr(double x) { return x * x; }
+template <typename T> struct Ar;
+template <> struct Ar<double> {
+  static __device__ __forceinline__ double div(double a, double b) { return a / b; }
+  static __device__ __forceinline__ double root(double a) { return sqrt(a); }
+  static constexpr bool guarded = false;
+};
+template <> struct Ar<float> {
+  static __device__ __forceinline__ float div(float a, float b) {
+    return a * __builtin_amdgcn_rcpf(b);
+  }
+  static __device__ __forceinline__ float root(float a) { return __builtin_amdgcn_sqrtf(a); }
+  static constexpr bool guarded = true;
+};
+
+template <typename T> __device__ __forceinline__ T sqr(T x) { return x * x; }
 // SIGN(0)=0 (src/defs_c.h:150)
-__device__ __forceinline__ double sign0(double a) { return (double)((0.0 < a) - (a < 0.0)); }
+template <typename T> __device__ __forceinline__ T sign0(T a) {
+  return (T)((T(0) < a) - (a < T(0)));
+}
 
 // src/mhd/eos_c.h:60-95; gamm1i = 1/(gam-1) is passed in (same value the
 // reference recomputes on every call).
-__device__ __forceinline__ void prim2cons(double *u, const double *w, double gamm1i) {
+template <typename T>
+__device__ __forceinline__ void prim2cons(T *u, const T *w, T gamm1i) {
   u[RHO] = w[RHO];
   u[MX] = w[RHO] * w[VX];
   u[MY] = w[RHO] * w[VY];
   u[MZ] = w[RHO] * w[VZ];
-  double v2 = sqr(w[VX]) + sqr(w[VY]) + sqr(w[VZ]);
-  double b2 = sqr(w[BX]) + sqr(w[BY]) + sqr(w[BZ]);
-  u[EN] = 0.5 * (w[RHO] * v2 + b2) + w[PR] * gamm1i;
+  T v2 = sqr(w[VX]) + sqr(w[VY]) + sqr(w[VZ]);
+  T b2 = sqr(w[BX]) + sqr(w[BY]) + sqr(w[BZ]);
+  u[EN] = T(0.5) * (w[RHO] * v2 + b2) + w[PR] * gamm1i;
   u[PSC] = w[PSC];
   u[BX] = w[BX];
   u[BY] = w[BY];
   u[BZ] = w[BZ];
 }
 
-// src/mhd/eos_c.h:20-55
+// src/mhd/eos_c.h:20-55 (FP64; the FP32 path recovers pressure with it too)
 __device__ __forceinline__ void cons2prim(double *w, const double *u, double gam) {
   double rhoi = 1. / u[RHO];
   w[RHO] = u[RHO];
@@ -52,124 +73,148 @@ __device__ __forceinline__ void cons2prim(double *w, const double *u, double gam
 }
 
 // src/mhd/eos_c.h:416-461 with a=1 (no CGL/TWOTEMP)
-__device__ __forceinline__ void primcons2flux(double *f, const double *w, const double *u) {
-  double bx2 = sqr(w[BX]);
-  double b2 = bx2 + sqr(w[BY]) + sqr(w[BZ]);
-  double bxby = w[BX] * w[BY];
-  double bxbz = w[BX] * w[BZ];
-  double ptot = 0.;
-  ptot += 0.5 * b2;
+template <typename T>
+__device__ __forceinline__ void primcons2flux(T *f, const T *w, const T *u) {
+  T bx2 = sqr(w[BX]);
+  T b2 = bx2 + sqr(w[BY]) + sqr(w[BZ]);
+  T bxby = w[BX] * w[BY];
+  T bxbz = w[BX] * w[BZ];
+  T ptot = T(0);
+  ptot += T(0.5) * b2;
   ptot += w[PR];
 #pragma unroll
   for (int n = 0; n < BX; ++n) f[n] = u[n] * w[VX];
   f[MX] += ptot;
   f[EN] += ptot * w[VX];
-  f[MX] -= 1. * bx2;
-  f[MY] -= 1. * bxby;
-  f[MZ] -= 1. * bxbz;
-  f[EN] -= 1. * (w[VX] * bx2 + w[VY] * bxby + w[VZ] * bxbz);
-  f[BX] = 0.;
+  f[MX] -= T(1) * bx2;
+  f[MY] -= T(1) * bxby;
+  f[MZ] -= T(1) * bxbz;
+  f[EN] -= T(1) * (w[VX] * bx2 + w[VY] * bxby + w[VZ] * bxbz);
+  f[BX] = T(0);
   f[BY] = w[VX] * w[BY] - w[VY] * w[BX];
   f[BZ] = w[VX] * w[BZ] - w[VZ] * w[BX];
 }
 
 // Fast magnetosonic speed, src/mhd/eos_c.h:468-530.  Reads w[BX] (the
 // reference ignores its bx argument).
-__device__ __forceinline__ double fms(const double *w, double gam) {
-  double bx2 = sqr(w[BX]);
-  double b2 = bx2 + sqr(w[BY]) + sqr(w[BZ]);
-  double gamp = gam * w[PR];
-  double c1 = gamp + b2;
-  double c2 = sqrt(fabs(c1 * c1 - 4. * gamp * bx2));
-  return sqrt(0.5 / w[RHO] * (c1 + c2));
+template <typename T>
+__device__ __forceinline__ T fms(const T *w, T gam) {
+  T bx2 = sqr(w[BX]);
+  T b2 = bx2 + sqr(w[BY]) + sqr(w[BZ]);
+  T gamp = gam * w[PR];
+  T c1 = gamp + b2;
+  T c2 = Ar<T>::root(fabs(c1 * c1 - T(4) * gamp * bx2));
+  return Ar<T>::root(Ar<T>::div(T(0.5), w[RHO]) * (c1 + c2));
 }
 
 // van Leer limiter, src/coordinates/reconstr_c.h:39-44
-__device__ __forceinline__ double vl_lim(double a, double b) {
-  double ab = a * b;
-  return (ab > 0.) ? (2. * ab / (a + b)) : 0.;
+template <typename T>
+__device__ __forceinline__ T vl_lim(T a, T b) {
+  T ab = a * b;
+  return (ab > T(0)) ? Ar<T>::div(T(2) * ab, a + b) : T(0);
 }
 
 // sLsR_simple, src/mhd/fluxes_c.c:25-42 (FMAX/FMIN as fmax/fmin)
-__device__ __forceinline__ void speeds_simple(double &sL, double &sR, const double *wL,
-                                              const double *wR, double gam) {
-  double cfL = fms(wL, gam);
-  double cfR = fms(wR, gam);
-  double cf_max = fmax(cfL, cfR);
+template <typename T>
+__device__ __forceinline__ void speeds_simple(T &sL, T &sR, const T *wL, const T *wR, T gam) {
+  T cfL = fms(wL, gam);
+  T cfR = fms(wR, gam);
+  T cf_max = fmax(cfL, cfR);
   sL = fmin(wL[VX], wR[VX]) - cf_max;
   sR = fmax(wL[VX], wR[VX]) + cf_max;
 }
 
 // HLL flux, src/mhd/fluxes_c.c:175-223.  Always writes (returns 1).
-__device__ __forceinline__ int flux_hll(double *flux, const double *wL, const double *wR,
-                                       double gam, double gamm1i) {
-  double uL[NMODE], uR[NMODE], fL[NMODE], fR[NMODE];
+template <typename T>
+__device__ __forceinline__ int flux_hll(T *flux, const T *wL, const T *wR, T gam, T gamm1i) {
+  T uL[NMODE], uR[NMODE], fL[NMODE], fR[NMODE];
   prim2cons(uL, wL, gamm1i);
   prim2cons(uR, wR, gamm1i);
   primcons2flux(fL, wL, uL);
   primcons2flux(fR, wR, uR);
-  double sL, sR;
+  T sL, sR;
   speeds_simple(sL, sR, wL, wR, gam);
-  double aL = fmin(sL, 0.);
-  double aR = fmax(sR, 0.);
-  double aLaR = aL * aR;
-  double aRmaLi = 1. / (aR - aL);
+  T aL = fmin(sL, T(0));
+  T aR = fmax(sR, T(0));
+  T aLaR = aL * aR;
+  T aRmaLi = Ar<T>::div(T(1), aR - aL);
 #pragma unroll
   for (int n = 0; n < NMODE; ++n)
     flux[n] = (aR * fL[n] - aL * fR[n] + aLaR * (uR[n] - uL[n])) * aRmaLi;
-  flux[BX] = 0.;
+  flux[BX] = T(0);
   return 1;
 }
 
 // HLLD flux, src/mhd/fluxes_c.c:616-834.  Returns the region (1..6) chosen
 // by the reference's chain of independent ifs (last true wins), or 0 when
 // no condition holds (only with NaN speeds): the caller then keeps the old
-// flux, as the reference leaves the flux row untouched.
-__device__ __forceinline__ int flux_hlld(double *flux, const double *wL, const double *wR,
-                                        double gam, double gamm1i) {
-  const double bx = wL[BX];
-  double uL[NMODE], uR[NMODE], fL[NMODE], fR[NMODE];
+// flux, as the reference leaves the flux row untouched.  For T = float the
+// star-state denominators rho(S-v)(S-S_M) - Bx^2 are guarded (Miyoshi &
+// Kusano 2005, sec. 4: the transverse fields then stay those of the outer
+// state) and *nguard counts the faces where that happened.
+template <typename T>
+__device__ __forceinline__ int flux_hlld(T *flux, const T *wL, const T *wR, T gam, T gamm1i,
+                                         int *nguard = nullptr) {
+  const T bx = wL[BX];
+  T uL[NMODE], uR[NMODE], fL[NMODE], fR[NMODE];
   prim2cons(uL, wL, gamm1i);
   prim2cons(uR, wR, gamm1i);
   primcons2flux(fL, wL, uL);
   primcons2flux(fR, wR, uR);
-  double sL, sR;
+  T sL, sR;
   speeds_simple(sL, sR, wL, wR, gam);
 
-  double b2hL = 0.5 * (sqr(bx) + sqr(wL[BY]) + sqr(wL[BZ]));
-  double b2hR = 0.5 * (sqr(bx) + sqr(wR[BY]) + sqr(wR[BZ]));
-  double ptL = wL[PR] + b2hL;
-  double ptR = wR[PR] + b2hR;
+  T b2hL = T(0.5) * (sqr(bx) + sqr(wL[BY]) + sqr(wL[BZ]));
+  T b2hR = T(0.5) * (sqr(bx) + sqr(wR[BY]) + sqr(wR[BZ]));
+  T ptL = wL[PR] + b2hL;
+  T ptR = wR[PR] + b2hR;
 
-  double deni = 1. / (sR * uR[RHO] - sL * uL[RHO] - (fR[RHO] - fL[RHO]));
-  double sM = (sR * uR[MX] - sL * uL[MX] - (fR[MX] - fL[MX])) * deni;
+  T deni = Ar<T>::div(T(1), sR * uR[RHO] - sL * uL[RHO] - (fR[RHO] - fL[RHO]));
+  T sM = (sR * uR[MX] - sL * uL[MX] - (fR[MX] - fL[MX])) * deni;
 
-  double sLvxL = sL - wL[VX];
-  double sRvxR = sR - wR[VX];
-  double sLsMi = 1. / (sL - sM);
-  double sRsMi = 1. / (sR - sM);
-  double xL = sLvxL * sLsMi;
-  double xR = sRvxR * sRsMi;
+  T sLvxL = sL - wL[VX];
+  T sRvxR = sR - wR[VX];
+  T sLsMi = Ar<T>::div(T(1), sL - sM);
+  T sRsMi = Ar<T>::div(T(1), sR - sM);
+  T xL = sLvxL * sLsMi;
+  T xR = sRvxR * sRsMi;
 
-  double uLs[NMODE], uRs[NMODE];
+  T uLs[NMODE], uRs[NMODE];
   uLs[RHO] = uL[RHO] * xL;
   uRs[RHO] = uR[RHO] * xR;
   uLs[PSC] = uL[PSC] * xL;
   uRs[PSC] = uR[PSC] * xR;
 
-  double pts = wL[RHO] * sLvxL * (sM - wL[VX]) + ptL;
+  T pts = wL[RHO] * sLvxL * (sM - wL[VX]) + ptL;
 
-  double rhosLvxL = wL[RHO] * sLvxL;
-  double rhosRvxR = wR[RHO] * sRvxR;
-  double bx2 = sqr(bx);
-  double wvbL = 1. / (rhosLvxL * (sL - sM) - bx2);
-  double wvbR = 1. / (rhosRvxR * (sR - sM) - bx2);
-  double wvL = (bx * (sM - wL[VX])) * wvbL;
-  double wvR = (bx * (sM - wR[VX])) * wvbR;
-  double vyLs = wL[VY] - wL[BY] * wvL;
-  double vyRs = wR[VY] - wR[BY] * wvR;
-  double vzLs = wL[VZ] - wL[BZ] * wvL;
-  double vzRs = wR[VZ] - wR[BZ] * wvR;
+  T rhosLvxL = wL[RHO] * sLvxL;
+  T rhosRvxR = wR[RHO] * sRvxR;
+  T bx2 = sqr(bx);
+  T denL = rhosLvxL * (sL - sM) - bx2;
+  T denR = rhosRvxR * (sR - sM) - bx2;
+  T wvbL = Ar<T>::div(T(1), denL);
+  T wvbR = Ar<T>::div(T(1), denR);
+  T wvL = (bx * (sM - wL[VX])) * wvbL;
+  T wvR = (bx * (sM - wR[VX])) * wvbR;
+  T wbL = (rhosLvxL * sLvxL - bx2) * wvbL;
+  T wbR = (rhosRvxR * sRvxR - bx2) * wvbR;
+  if (Ar<T>::guarded) {
+    const T tol = T(1e-5);
+    if (fabs(denL) <= tol * (fabs(rhosLvxL * (sL - sM)) + bx2)) {
+      wvL = T(0);
+      wbL = T(1);
+      if (nguard) ++*nguard;
+    }
+    if (fabs(denR) <= tol * (fabs(rhosRvxR * (sR - sM)) + bx2)) {
+      wvR = T(0);
+      wbR = T(1);
+      if (nguard) ++*nguard;
+    }
+  }
+  T vyLs = wL[VY] - wL[BY] * wvL;
+  T vyRs = wR[VY] - wR[BY] * wvR;
+  T vzLs = wL[VZ] - wL[BZ] * wvL;
+  T vzRs = wR[VZ] - wR[BZ] * wvR;
 
   uLs[MX] = uLs[RHO] * sM;
   uRs[MX] = uRs[RHO] * sM;
@@ -178,57 +223,54 @@ __device__ __forceinline__ int flux_hlld(double *flux, const double *wL, const d
   uLs[MZ] = uLs[RHO] * vzLs;
   uRs[MZ] = uRs[RHO] * vzRs;
 
-  double wbL = (rhosLvxL * sLvxL - bx2) * wvbL;
-  double wbR = (rhosRvxR * sRvxR - bx2) * wvbR;
   uLs[BY] = wL[BY] * wbL;
   uRs[BY] = wR[BY] * wbR;
   uLs[BZ] = wL[BZ] * wbL;
   uRs[BZ] = wR[BZ] * wbR;
 
-  double vsbsL = sM * bx + vyLs * uLs[BY] + vzLs * uLs[BZ];
-  double vsbsR = sM * bx + vyRs * uRs[BY] + vzRs * uRs[BZ];
+  T vsbsL = sM * bx + vyLs * uLs[BY] + vzLs * uLs[BZ];
+  T vsbsR = sM * bx + vyRs * uRs[BY] + vzRs * uRs[BZ];
   uLs[EN] = (sLvxL * uL[EN] - ptL * wL[VX] + pts * sM +
              bx * (wL[VX] * bx + wL[VY] * wL[BY] + wL[VZ] * wL[BZ] - vsbsL)) * sLsMi;
   uRs[EN] = (sRvxR * uR[EN] - ptR * wR[VX] + pts * sM +
              bx * (wR[VX] * bx + wR[VY] * wR[BY] + wR[VZ] * wR[BZ] - vsbsR)) * sRsMi;
 
-  double sqrt_rhoLs = sqrt(uLs[RHO]);
-  double sqrt_rhoRs = sqrt(uRs[RHO]);
-  double sLs = sM - fabs(bx) / sqrt_rhoLs;
-  double sRs = sM + fabs(bx) / sqrt_rhoRs;
-  double rhoLRi = 1. / (sqrt_rhoLs + sqrt_rhoRs);
-  double sgnb = sign0(bx);
+  T sqrt_rhoLs = Ar<T>::root(uLs[RHO]);
+  T sqrt_rhoRs = Ar<T>::root(uRs[RHO]);
+  T sLs = sM - Ar<T>::div(fabs(bx), sqrt_rhoLs);
+  T sRs = sM + Ar<T>::div(fabs(bx), sqrt_rhoRs);
+  T rhoLRi = Ar<T>::div(T(1), sqrt_rhoLs + sqrt_rhoRs);
+  T sgnb = sign0(bx);
 
-  double vyss = (sqrt_rhoLs * vyLs + sqrt_rhoRs * vyRs + (uRs[BY] - uLs[BY]) * sgnb) * rhoLRi;
-  double vzss = (sqrt_rhoLs * vzLs + sqrt_rhoRs * vzRs + (uRs[BZ] - uLs[BZ]) * sgnb) * rhoLRi;
-  double byss = (sqrt_rhoLs * uRs[BY] + sqrt_rhoRs * uLs[BY] +
-                 sqrt_rhoLs * sqrt_rhoRs * (vyRs - vyLs) * sgnb) * rhoLRi;
-  double bzss = (sqrt_rhoLs * uRs[BZ] + sqrt_rhoRs * uLs[BZ] +
-                 sqrt_rhoLs * sqrt_rhoRs * (vzRs - vzLs) * sgnb) * rhoLRi;
-  double vssbss = sM * bx + vyss * byss + vzss * bzss;
+  T vyss = (sqrt_rhoLs * vyLs + sqrt_rhoRs * vyRs + (uRs[BY] - uLs[BY]) * sgnb) * rhoLRi;
+  T vzss = (sqrt_rhoLs * vzLs + sqrt_rhoRs * vzRs + (uRs[BZ] - uLs[BZ]) * sgnb) * rhoLRi;
+  T byss = (sqrt_rhoLs * uRs[BY] + sqrt_rhoRs * uLs[BY] +
+            sqrt_rhoLs * sqrt_rhoRs * (vyRs - vyLs) * sgnb) * rhoLRi;
+  T bzss = (sqrt_rhoLs * uRs[BZ] + sqrt_rhoRs * uLs[BZ] +
+            sqrt_rhoLs * sqrt_rhoRs * (vzRs - vzLs) * sgnb) * rhoLRi;
+  T vssbss = sM * bx + vyss * byss + vzss * bzss;
 
   int reg = 0;
-  if (sL > 0.) reg = 1;
-  if (sL <= 0. && 0. < sLs) reg = 2;
-  if (sLs <= 0. && 0. < sM) reg = 3;
-  if (sM <= 0. && 0. < sRs) reg = 4;
-  if (sRs <= 0. && 0. < sR) reg = 5;
-  if (sR <= 0.) reg = 6;
+  if (sL > T(0)) reg = 1;
+  if (sL <= T(0) && T(0) < sLs) reg = 2;
+  if (sLs <= T(0) && T(0) < sM) reg = 3;
+  if (sM <= T(0) && T(0) < sRs) reg = 4;
+  if (sRs <= T(0) && T(0) < sR) reg = 5;
+  if (sR <= T(0)) reg = 6;
   if (reg == 0) return 0;
 
   // Evaluate only the chosen side; each region's expression keeps the
   // reference's association: ((f + s*(u* - u)) + s*(u** - u*)).
   const bool left = reg <= 3;
-  const double *f = left ? fL : fR;
-  const double *u = left ? uL : uR;
-  const double *us = left ? uLs : uRs;
-  const double s1 = left ? sL : sR;
-  const double s2 = left ? sLs : sRs;
-  const double sq = left ? sqrt_rhoLs : sqrt_rhoRs;
-  const double vyS = left ? vyLs : vyRs, vzS = left ? vzLs : vzRs;
-  const double vsbs = left ? vsbsL : vsbsR;
-  const double rhoS = us[RHO];
-  double uss[NMODE];
+  const T *f = left ? fL : fR;
+  const T *u = left ? uL : uR;
+  const T *us = left ? uLs : uRs;
+  const T s1 = left ? sL : sR;
+  const T s2 = left ? sLs : sRs;
+  const T sq = left ? sqrt_rhoLs : sqrt_rhoRs;
+  const T vsbs = left ? vsbsL : vsbsR;
+  const T rhoS = us[RHO];
+  T uss[NMODE];
   uss[RHO] = rhoS;
   uss[MX] = us[MX];
   uss[PSC] = us[PSC];
@@ -237,18 +279,16 @@ __device__ __forceinline__ int flux_hlld(double *flux, const double *wL, const d
   uss[BY] = byss;
   uss[BZ] = bzss;
   uss[EN] = left ? (us[EN] - sq * (vsbs - vssbss) * sgnb) : (us[EN] + sq * (vsbs - vssbss) * sgnb);
-  (void)vyS;
-  (void)vzS;
   const int lvl = left ? (reg == 1 ? 0 : reg == 2 ? 1 : 2) : (reg == 6 ? 0 : reg == 5 ? 1 : 2);
 #pragma unroll
   for (int n = 0; n < NMODE; ++n) {
     if (n == BX) continue;
-    double v = f[n];
+    T v = f[n];
     if (lvl >= 1) v = f[n] + s1 * (us[n] - u[n]);
     if (lvl >= 2) v = v + s2 * (uss[n] - us[n]);
     flux[n] = v;
   }
-  flux[BX] = 0.;
+  flux[BX] = T(0);
   return reg;
 }
 
