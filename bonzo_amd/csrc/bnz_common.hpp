@@ -179,7 +179,8 @@ struct bnz_ctx {
   double *w = nullptr, *ws = nullptr, *b = nullptr, *bs = nullptr;
   double *fx = nullptr, *fy = nullptr, *fz = nullptr, *ee = nullptr;
   double *geo = nullptr;
-  int *maps = nullptr;
+  int2 *ghost = nullptr;      // (ghost cell, source cell) of every ghost cell
+  int nghost = 0;
   double *dev = nullptr;      // dt, t, tmax, step
   double *part = nullptr;     // reduction partials
   long npart_cap = 0;

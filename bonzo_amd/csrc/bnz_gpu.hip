@@ -564,24 +564,15 @@ __global__ __launch_bounds__(BLK) void k_update_march(
 // sequence composes per-axis index maps (periodic wrap, outflow clamp), so
 // each ghost cell gathers directly from the active cell its maps name.
 __global__ __launch_bounds__(BLK) void k_ghost(Geo g, double *w, double *b,
-                                               const int *__restrict__ mx,
-                                               const int *__restrict__ my,
-                                               const int *__restrict__ mz) {
-  long t = xcd_block() * BLK + threadIdx.x;
-  const long ncell = (long)g.nt[0] * g.nt[1] * g.nt[2];
-  if (t >= ncell) return;
-  const int i = (int)(t % g.nt[0]);
-  const long r = t / g.nt[0];
-  const int j = (int)(r % g.nt[1]);
-  const int k = (int)(r / g.nt[1]);
-  const int si = mx[i], sj = my[j], sk = mz[k];
-  if (si == i && sj == j && sk == k) return;
-  const long s = ((long)sk * g.nt[1] + sj) * g.nt[0] + si;
+                                               const int2 *__restrict__ pairs, int npairs) {
+  const long t = xcd_block() * BLK + threadIdx.x;
+  if (t >= npairs) return;
+  const int2 ds = pairs[t];  // (ghost cell, active source cell)
   const long sv = g.sv;
 #pragma unroll
-  for (int n = 0; n < NMODE; ++n) w[n * sv + t] = w[n * sv + s];
+  for (int n = 0; n < NMODE; ++n) w[n * sv + ds.x] = w[n * sv + ds.y];
 #pragma unroll
-  for (int n = 0; n < 3; ++n) b[n * sv + t] = b[n * sv + s];
+  for (int n = 0; n < 3; ++n) b[n * sv + ds.x] = b[n * sv + ds.y];
 }
 
 // interp_bc over the active cells into prim (init only, vanleer.pyx:44)
@@ -926,8 +917,8 @@ void launch_emf(bnz_ctx *c, const double *w, const Lims &L, const char *name) {
 
 void launch_ghost(bnz_ctx *c) {
   Launch t(c, "ghost");
-  hipLaunchKernelGGL(k_ghost, dim3(blocks_for(c->ncell)), dim3(BLK), 0, c->stream, c->g, c->w,
-                     c->b, c->maps, c->maps + c->nt[0], c->maps + c->nt[0] + c->nt[1]);
+  hipLaunchKernelGGL(k_ghost, dim3(blocks_for(c->nghost)), dim3(BLK), 0, c->stream, c->g, c->w,
+                     c->b, c->ghost, c->nghost);
 }
 
 void launch_dt(bnz_ctx *c, int use_fixed, double fixed_dt) {
@@ -1085,6 +1076,7 @@ int bnz_create(const bnz_params *p, bnz_ctx **out) {
   c->j1 = (c->dims >= 2) ? c->ng : 0; c->j2 = (c->dims >= 2) ? p->ny + c->ng - 1 : 0;
   c->k1 = (c->dims >= 3) ? c->ng : 0; c->k2 = (c->dims >= 3) ? p->nz + c->ng - 1 : 0;
   c->ncell = (long)c->nt[0] * c->nt[1] * c->nt[2];
+  if (c->ncell >= (1L << 31)) return bad(BNZ_EINVAL, "grid too large: cell indices are 32-bit");
   set_coordinates(c);
 
   *out = c;
@@ -1127,15 +1119,27 @@ int bnz_create(const bnz_params *p, bnz_ctx **out) {
   }
   HIPCHK(c, hipMemcpy(c->geo, hg.data(), geo_n * sizeof(double), hipMemcpyHostToDevice));
 
-  std::vector<int> maps;
-  auto mx = axis_map(c->nt[0], c->i1, c->i2, c->ng, p->bc[0], p->bc[1], true);
-  auto my = axis_map(c->nt[1], c->j1, c->j2, c->ng, p->bc[2], p->bc[3], c->dims >= 2);
-  auto mz = axis_map(c->nt[2], c->k1, c->k2, c->ng, p->bc[4], p->bc[5], c->dims >= 3);
-  maps.insert(maps.end(), mx.begin(), mx.end());
-  maps.insert(maps.end(), my.begin(), my.end());
-  maps.insert(maps.end(), mz.begin(), mz.end());
-  HIPCHK(c, hipMalloc(&c->maps, maps.size() * sizeof(int)));
-  HIPCHK(c, hipMemcpy(c->maps, maps.data(), maps.size() * sizeof(int), hipMemcpyHostToDevice));
+  {
+    // (ghost, source) pairs of every ghost cell, from the per-axis maps
+    auto mx = axis_map(c->nt[0], c->i1, c->i2, c->ng, p->bc[0], p->bc[1], true);
+    auto my = axis_map(c->nt[1], c->j1, c->j2, c->ng, p->bc[2], p->bc[3], c->dims >= 2);
+    auto mz = axis_map(c->nt[2], c->k1, c->k2, c->ng, p->bc[4], p->bc[5], c->dims >= 3);
+    std::vector<int2> pairs;
+    pairs.reserve((size_t)(c->ncell - (long)p->nx * p->ny * p->nz));
+    for (int k = 0; k < c->nt[2]; ++k)
+      for (int j = 0; j < c->nt[1]; ++j)
+        for (int i = 0; i < c->nt[0]; ++i) {
+          const int si = mx[i], sj = my[j], sk = mz[k];
+          if (si == i && sj == j && sk == k) continue;
+          const long d = ((long)k * c->nt[1] + j) * c->nt[0] + i;
+          const long s = ((long)sk * c->nt[1] + sj) * c->nt[0] + si;
+          pairs.push_back(make_int2((int)d, (int)s));
+        }
+    c->nghost = (int)pairs.size();
+    HIPCHK(c, hipMalloc(&c->ghost, std::max<size_t>(1, pairs.size()) * sizeof(int2)));
+    if (!pairs.empty())
+      HIPCHK(c, hipMemcpy(c->ghost, pairs.data(), pairs.size() * sizeof(int2), hipMemcpyHostToDevice));
+  }
 
   HIPCHK(c, hipMalloc(&c->dev, 8 * sizeof(double)));
   const double dev0[8] = {0., 0., -1., 0., 0., 0., 0., 0.};
@@ -1178,7 +1182,7 @@ int bnz_destroy(bnz_ctx *c) {
   double *bufs[] = {c->w, c->ws, c->b, c->bs, c->fx, c->fy, c->fz, c->ee, c->geo, c->dev, c->part};
   for (double *q : bufs)
     if (q) (void)hipFree(q);
-  if (c->maps) (void)hipFree(c->maps);
+  if (c->ghost) (void)hipFree(c->ghost);
   f32_free(c);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;

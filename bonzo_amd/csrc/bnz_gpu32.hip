@@ -321,22 +321,15 @@ struct Ghost32 {
   double *d[3];   // face fields
 };
 
-__global__ __launch_bounds__(BLK) void k32_ghost(Geo g, Ghost32 p, const int *__restrict__ mx,
-                                                 const int *__restrict__ my,
-                                                 const int *__restrict__ mz) {
+__global__ __launch_bounds__(BLK) void k32_ghost(Geo g, Ghost32 p, const int2 *__restrict__ pairs,
+                                                 int npairs) {
   const long t = xcd_block() * BLK + threadIdx.x;
-  const long ncell = (long)g.nt[0] * g.nt[1] * g.nt[2];
-  if (t >= ncell) return;
-  const int i = (int)(t % g.nt[0]);
-  const long r = t / g.nt[0];
-  const int j = (int)(r % g.nt[1]), k = (int)(r / g.nt[1]);
-  const int si = mx[i], sj = my[j], sk = mz[k];
-  if (si == i && sj == j && sk == k) return;
-  const long s = ((long)sk * g.nt[1] + sj) * g.nt[0] + si;
+  if (t >= npairs) return;
+  const int2 ds = pairs[t];
 #pragma unroll
-  for (int n = 0; n < 21; ++n) p.a[n][t] = p.a[n][s];
+  for (int n = 0; n < 21; ++n) p.a[n][ds.x] = p.a[n][ds.y];
 #pragma unroll
-  for (int n = 0; n < 3; ++n) p.d[n][t] = p.d[n][s];
+  for (int n = 0; n < 3; ++n) p.d[n][ds.x] = p.d[n][ds.y];
 }
 
 // FP64 prim (all cells) -> FP32-compensated state.
@@ -451,8 +444,8 @@ void ghost32(bnz_ctx *c) {
   for (int n = 0; n < 9; ++n) p.a[q++] = c->w32 + n * c->ncell;
   for (int n = 0; n < 3; ++n) p.d[n] = c->b + n * c->ncell;
   Launch t(c, "ghost");
-  hipLaunchKernelGGL(k32_ghost, dim3(blocks_for(c->ncell)), dim3(BLK), 0, c->stream, c->g, p,
-                     c->maps, c->maps + c->nt[0], c->maps + c->nt[0] + c->nt[1]);
+  hipLaunchKernelGGL(k32_ghost, dim3(blocks_for(c->nghost)), dim3(BLK), 0, c->stream, c->g, p,
+                     c->ghost, c->nghost);
 }
 
 }  // namespace
