@@ -21,7 +21,11 @@ ABI_SYMBOLS = (
     "bnz_create", "bnz_destroy", "bnz_last_error", "bnz_get_shape", "bnz_upload",
     "bnz_download", "bnz_init_state", "bnz_fill_ghosts", "bnz_new_dt", "bnz_vl_step",
     "bnz_advance", "bnz_run_steps", "bnz_get_time", "bnz_set_time", "bnz_diag",
-    "bnz_synchronize", "bnz_set_tuning", "bnz_set_kernel_timing", "bnz_kernel_timing", "bnz_gpu_flux_pencils",
+    "bnz_synchronize", "bnz_set_tuning", "bnz_set_kernel_timing", "bnz_kernel_timing",
+    "bnz_gpu_flux_pencils",
+    "bnz_batch_create", "bnz_batch_destroy", "bnz_batch_last_error", "bnz_batch_box_shape",
+    "bnz_batch_upload", "bnz_batch_download", "bnz_batch_init_state", "bnz_batch_advance",
+    "bnz_batch_get_times", "bnz_batch_synchronize",
 )
 
 
@@ -42,6 +46,15 @@ class BnzDiag(ctypes.Structure):
         ("scalar", ctypes.c_double), ("divb_max", ctypes.c_double), ("bmax", ctypes.c_double),
         ("pmin", ctypes.c_double), ("rhomin", ctypes.c_double), ("nonfinite", ctypes.c_long),
         ("hlld_guards", ctypes.c_long),
+    ]
+
+
+class BnzBox(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+        ("bc", ctypes.c_int * 6),
+        ("lmin", ctypes.c_double * 3), ("lmax", ctypes.c_double * 3),
+        ("tmax", ctypes.c_double),
     ]
 
 
@@ -87,8 +100,20 @@ def load():
                                     ctypes.POINTER(ctypes.c_long)]
     L.bnz_gpu_flux_pencils.argtypes = [ctypes.c_int, vp, vp, vp, vp, ctypes.c_long,
                                        ctypes.c_double]
+    L.bnz_batch_create.argtypes = [ctypes.POINTER(BnzParams), ctypes.POINTER(BnzBox),
+                                   ctypes.c_int, ctypes.POINTER(vp)]
+    L.bnz_batch_destroy.argtypes = [vp]
+    L.bnz_batch_last_error.restype = ctypes.c_char_p
+    L.bnz_batch_last_error.argtypes = [vp]
+    L.bnz_batch_box_shape.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    L.bnz_batch_upload.argtypes = [vp, ctypes.c_int, _dp, _dp]
+    L.bnz_batch_download.argtypes = [vp, ctypes.c_int, _dp, _dp]
+    L.bnz_batch_init_state.argtypes = [vp]
+    L.bnz_batch_synchronize.argtypes = [vp]
+    L.bnz_batch_advance.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    L.bnz_batch_get_times.argtypes = [vp, _dp, ctypes.POINTER(ctypes.c_int)]
     for name in ABI_SYMBOLS:
-        if name not in ("bnz_last_error",):
+        if name not in ("bnz_last_error", "bnz_batch_last_error"):
             getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L
@@ -101,6 +126,25 @@ def _ptr(a):
     return a.ctypes.data_as(_dp)
 
 
+def make_params(nx=1, ny=1, nz=1, rsolver="hlld", reconstr="linear", bc=(0,) * 6,
+                lmin=(0.0, 0.0, 0.0), lmax=(1.0, 1.0, 1.0), gam=5.0 / 3.0, cour=0.4,
+                dt_fms_ref=True, precision="fp64", device=0):
+    p = BnzParams()
+    p.nx, p.ny, p.nz = nx, ny, nz
+    p.rsolver = RS[rsolver] if isinstance(rsolver, str) else int(rsolver)
+    p.reconstr = RCN[reconstr] if isinstance(reconstr, str) else int(reconstr)
+    for q in range(6):
+        p.bc[q] = int(bc[q])
+    for q in range(3):
+        p.lmin[q] = lmin[q]
+        p.lmax[q] = lmax[q]
+    p.gam, p.cour = gam, cour
+    p.dt_fms_ref = int(bool(dt_fms_ref))
+    p.precision = PREC[precision]
+    p.device = device
+    return p
+
+
 class Context:
     """One device grid: owns a bnz_ctx (device arrays, stream)."""
 
@@ -108,19 +152,8 @@ class Context:
                  lmin=(0.0, 0.0, 0.0), lmax=(1.0, 1.0, 1.0), gam=5.0 / 3.0, cour=0.4,
                  dt_fms_ref=True, precision="fp64", device=0):
         L = load()
-        p = BnzParams()
-        p.nx, p.ny, p.nz = nx, ny, nz
-        p.rsolver = RS[rsolver] if isinstance(rsolver, str) else int(rsolver)
-        p.reconstr = RCN[reconstr] if isinstance(reconstr, str) else int(reconstr)
-        for q in range(6):
-            p.bc[q] = int(bc[q])
-        for q in range(3):
-            p.lmin[q] = lmin[q]
-            p.lmax[q] = lmax[q]
-        p.gam, p.cour = gam, cour
-        p.dt_fms_ref = int(bool(dt_fms_ref))
-        p.precision = PREC[precision]
-        p.device = device
+        p = make_params(nx, ny, nz, rsolver, reconstr, bc, lmin, lmax, gam, cour, dt_fms_ref,
+                        precision, device)
         h = ctypes.c_void_p()
         rc = L.bnz_create(ctypes.byref(p), ctypes.byref(h))
         if rc != BNZ_OK:
@@ -221,6 +254,92 @@ class Context:
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
             load().bnz_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Independent boxes of uneven size advanced together: owns a bnz_batch.
+
+    ``boxes`` is a sequence of dicts with keys nx, ny, nz (active cells),
+    bc (6 flags, periodic 0 / outflow 1), lmin, lmax (extent) and tmax.
+    The remaining keyword arguments are the common selection of Context.
+    """
+
+    def __init__(self, boxes, rsolver="hlld", reconstr="linear", gam=5.0 / 3.0, cour=0.4,
+                 dt_fms_ref=True, precision="fp64", device=0):
+        L = load()
+        p = make_params(rsolver=rsolver, reconstr=reconstr, gam=gam, cour=cour,
+                        dt_fms_ref=dt_fms_ref, precision=precision, device=device)
+        arr = (BnzBox * len(boxes))()
+        for q, b in enumerate(boxes):
+            arr[q].nx, arr[q].ny, arr[q].nz = b["nx"], b.get("ny", 1), b.get("nz", 1)
+            bc = b.get("bc", (0,) * 6)
+            for s in range(6):
+                arr[q].bc[s] = int(bc[s])
+            for a in range(3):
+                arr[q].lmin[a] = b.get("lmin", (0.0, 0.0, 0.0))[a]
+                arr[q].lmax[a] = b.get("lmax", (1.0, 1.0, 1.0))[a]
+            arr[q].tmax = b["tmax"]
+        h = ctypes.c_void_p()
+        rc = L.bnz_batch_create(ctypes.byref(p), arr, len(boxes), ctypes.byref(h))
+        if rc != BNZ_OK:
+            msg = L.bnz_batch_last_error(h).decode() if h.value else "bnz_batch_create failed"
+            if h.value:
+                L.bnz_batch_destroy(h)
+            raise BnzError(rc, msg)
+        self._h = h
+        self.nbox = len(boxes)
+        self.shapes = []
+        nt = (ctypes.c_int * 3)()
+        for q in range(self.nbox):
+            self._check(L.bnz_batch_box_shape(h, q, nt))
+            self.shapes.append((nt[2], nt[1], nt[0]))
+
+    def _check(self, rc):
+        if rc != BNZ_OK:
+            raise BnzError(rc, load().bnz_batch_last_error(self._h).decode())
+
+    def upload(self, box, prim, bfld):
+        prim = np.ascontiguousarray(prim, dtype=np.float64)
+        bfld = np.ascontiguousarray(bfld, dtype=np.float64)
+        sh = self.shapes[box]
+        assert prim.shape == (9,) + sh and bfld.shape == (3,) + sh
+        self._check(load().bnz_batch_upload(self._h, box, _ptr(prim), _ptr(bfld)))
+
+    def download(self, box):
+        sh = self.shapes[box]
+        prim = np.empty((9,) + sh)
+        bfld = np.empty((3,) + sh)
+        self._check(load().bnz_batch_download(self._h, box, _ptr(prim), _ptr(bfld)))
+        return prim, bfld
+
+    def init_state(self):
+        self._check(load().bnz_batch_init_state(self._h))
+
+    def advance(self, max_steps=0, use_graph=True):
+        n = ctypes.c_int()
+        self._check(load().bnz_batch_advance(self._h, max_steps, int(bool(use_graph)),
+                                             ctypes.byref(n)))
+        return n.value
+
+    def times(self):
+        t = np.empty(self.nbox)
+        s = (ctypes.c_int * self.nbox)()
+        self._check(load().bnz_batch_get_times(self._h, _ptr(t), s))
+        return t, np.array(list(s), dtype=np.int64)
+
+    def synchronize(self):
+        self._check(load().bnz_batch_synchronize(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            load().bnz_batch_destroy(self._h)
         self._h = None
 
     def __del__(self):

@@ -1,0 +1,719 @@
+// bnz_batch.hip -- many independent boxes of uneven size advanced together,
+// one launch per stage over all boxes (FP64).
+//
+// Every box is a complete grid of its own (sizes, ghosts, boundary flags,
+// t_max) stored at an offset in shared pools.  The work of each stage
+// (faces, edges or cells of every box) is cut into work blocks of BLK items
+// that never straddle two boxes, so a workgroup finds its box with one
+// uniform load and skips a finished box as a whole.  Launches are persistent:
+// a fixed grid walks the work blocks.  The per-item arithmetic is the
+// single-grid arithmetic (bnz_kern64.hpp), so a box reproduces its
+// single-grid run bit for bit.  Each box keeps its own CFL dt, time and
+// "running" flag on the device, so a VL step -- and, in pairs, the
+// face-buffer ping-pong -- is captured once in a hipGraph and replayed until
+// every box has reached its t_max (vanleer.pyx:87-96 per box).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "bnz_kern64.hpp"
+
+using namespace bnz;
+
+namespace {
+
+// work lists: fluxes x/y/z and EMFs of the predictor and of the corrector,
+// all cells (stage updates), active cells (init and the first CFL maximum)
+enum { KFX0 = 0, KFY0, KFZ0, KEM0, KFX1, KFY1, KFZ1, KEM1, KUPD, KACT, NK };
+
+struct BoxDev {
+  Geo g;         // strides and geometry of the box; g.sv is the pool stride
+  long off;      // first cell of the box in the pools
+  Lims L[2];     // predictor / corrector limits (integrator.pyx:375-383)
+  Box act;       // active cells
+  Box items[NK];
+  int blk0[NK];  // first work block of the box in each list
+};
+
+struct Pools {
+  double *w, *ws, *b, *bs, *fx, *fy, *fz, *ee;
+};
+
+struct BoxState {
+  double *dt, *t, *tmax;
+  unsigned long long *cmax;  // CFL maximum, as the bits of a non-negative double
+  int *run;                  // box advances in the current step
+  int *steps;
+  int *nrun;                 // boxes still short of t_max after the last step
+};
+
+// work list kt as seen by a kernel
+struct WList {
+  const int *bmap;  // work block -> box
+  int nblk;
+  int kt;
+};
+
+constexpr int PERSIST = 256 * 8;  // workgroups of a persistent launch
+
+template <int DIR, int ORDER, int RS>
+__global__ __launch_bounds__(BLK) void kb_flux(const BoxDev *__restrict__ boxes, WList wl,
+                                               BoxState st, const double *__restrict__ w,
+                                               const double *__restrict__ bf,
+                                               double *__restrict__ fl) {
+  for (int wb = blockIdx.x; wb < wl.nblk; wb += gridDim.x) {
+    const int b = wl.bmap[wb];
+    if (!st.run[b]) continue;
+    const BoxDev &d = boxes[b];
+    const long t = (long)(wb - d.blk0[wl.kt]) * BLK + threadIdx.x;
+    const Box &it = d.items[wl.kt];
+    if (t >= it.total()) continue;
+    int i, j, k;
+    box_idx(it, t, i, j, k);
+    flux_face<DIR, ORDER, RS>(d.g, w + d.off, bf + d.off, fl + d.off, i, j, k);
+  }
+}
+
+__global__ __launch_bounds__(BLK) void kb_emf(const BoxDev *__restrict__ boxes, WList wl,
+                                              BoxState st, const double *__restrict__ w,
+                                              const double *__restrict__ fx,
+                                              const double *__restrict__ fy,
+                                              const double *__restrict__ fz,
+                                              double *__restrict__ ee) {
+  for (int wb = blockIdx.x; wb < wl.nblk; wb += gridDim.x) {
+    const int b = wl.bmap[wb];
+    if (!st.run[b]) continue;
+    const BoxDev &d = boxes[b];
+    const long t = (long)(wb - d.blk0[wl.kt]) * BLK + threadIdx.x;
+    const Box &it = d.items[wl.kt];
+    if (t >= it.total()) continue;
+    int i, j, k;
+    box_idx(it, t, i, j, k);
+    const long o = d.off;
+    emf_edge(d.g, w + o, fx + o, fy + o, fz + o, ee + o, i, j, k);
+  }
+}
+
+// Stage update of every cell of the running boxes.  The corrector also
+// folds the CFL maximum of the new state into the box's cmax (one atomic per
+// wave), and carries the faces of a finished box into the buffer that
+// becomes current after the face-buffer swap.
+template <bool PRED>
+__global__ __launch_bounds__(BLK) void kb_update(const BoxDev *__restrict__ boxes, WList wl,
+                                                 BoxState st, Pools p, int fms_ref) {
+  for (int wb = blockIdx.x; wb < wl.nblk; wb += gridDim.x) {
+    const int b = wl.bmap[wb];
+    const BoxDev &d = boxes[b];
+    const long t = (long)(wb - d.blk0[wl.kt]) * BLK + threadIdx.x;
+    const Box &it = d.items[wl.kt];
+    const bool valid = t < it.total();
+    const long o = d.off;
+    if (!st.run[b]) {
+      if (!PRED && valid) {
+        const long sv = d.g.sv;
+#pragma unroll
+        for (int n = 0; n < 3; ++n) p.bs[n * sv + o + t] = p.b[n * sv + o + t];
+      }
+      continue;
+    }
+    double cds = 0.;
+    if (valid) {
+      int i, j, k;
+      box_idx(it, t, i, j, k);
+      if (PRED)
+        update_cell<true, false>(d.g, p.w + o, p.fx + o, p.fy + o, p.fz + o, p.ee + o, p.b + o,
+                                 p.bs + o, p.ws + o, d.L[0], st.dt + b, d.act, fms_ref, i, j, k);
+      else
+        cds = update_cell<false, true>(d.g, p.w + o, p.fx + o, p.fy + o, p.fz + o, p.ee + o,
+                                       p.b + o, p.bs + o, p.w + o, d.L[1], st.dt + b, d.act,
+                                       fms_ref, i, j, k);
+    }
+    if (!PRED) {
+      cds = wave_max(cds);
+      if ((threadIdx.x & 63) == 0 && cds > 0.)
+        atomicMax(st.cmax + b, (unsigned long long)__double_as_longlong(cds));
+    }
+  }
+}
+
+// CFL maximum of the current prim over the active cells (after init).
+__global__ __launch_bounds__(BLK) void kb_cfl(const BoxDev *__restrict__ boxes, WList wl,
+                                              BoxState st, const double *__restrict__ w,
+                                              int fms_ref) {
+  for (int wb = blockIdx.x; wb < wl.nblk; wb += gridDim.x) {
+    const int b = wl.bmap[wb];
+    const BoxDev &d = boxes[b];
+    const long t = (long)(wb - d.blk0[wl.kt]) * BLK + threadIdx.x;
+    const Box &it = d.items[wl.kt];
+    double cds = 0.;
+    if (t < it.total()) {
+      int i, j, k;
+      box_idx(it, t, i, j, k);
+      const long c = d.off + ((long)k * d.g.nt[1] + j) * d.g.nt[0] + i;
+      double wv[NMODE];
+#pragma unroll
+      for (int n = 0; n < NMODE; ++n) wv[n] = w[n * d.g.sv + c];
+      cds = cds_cell(d.g, wv, fms_ref, i, j, k);
+    }
+    cds = wave_max(cds);
+    if ((threadIdx.x & 63) == 0 && cds > 0.)
+      atomicMax(st.cmax + b, (unsigned long long)__double_as_longlong(cds));
+  }
+}
+
+// interp_bc over the active cells (init only, vanleer.pyx:44)
+__global__ __launch_bounds__(BLK) void kb_interp(const BoxDev *__restrict__ boxes, WList wl,
+                                                 double *w, const double *b) {
+  for (int wb = blockIdx.x; wb < wl.nblk; wb += gridDim.x) {
+    const BoxDev &d = boxes[wl.bmap[wb]];
+    const long t = (long)(wb - d.blk0[wl.kt]) * BLK + threadIdx.x;
+    const Box &it = d.items[wl.kt];
+    if (t >= it.total()) continue;
+    int i, j, k;
+    box_idx(it, t, i, j, k);
+    const Geo &g = d.g;
+    const long c = d.off + ((long)k * g.nt[1] + j) * g.nt[0] + i, sv = g.sv;
+    w[BX * sv + c] = (g.icm[0][i] * b[c] + g.icp[0][i] * b[c + 1]) * g.dlfi[0][i];
+    if (g.dims >= 2)
+      w[BY * sv + c] = (g.icm[1][j] * b[sv + c] + g.icp[1][j] * b[sv + c + g.sy]) * g.dlfi[1][j];
+    else
+      w[BY * sv + c] = b[sv + c];
+    if (g.dims >= 3)
+      w[BZ * sv + c] =
+          (g.icm[2][k] * b[2 * sv + c] + g.icp[2][k] * b[2 * sv + c + g.sz]) * g.dlfi[2][k];
+    else
+      w[BZ * sv + c] = b[2 * sv + c];
+  }
+}
+
+// Per-box dt from the CFL maximum, clipped to t_max; a box at t_max stops
+// (vanleer.pyx:87-96).  Clears the maxima for the next accumulation.
+__global__ __launch_bounds__(BLK) void kb_dt(BoxState st, int nbox, double cour) {
+  for (int b = threadIdx.x; b < nbox; b += BLK) {
+    const double m = __longlong_as_double((long long)st.cmax[b]);
+    st.cmax[b] = 0ull;
+    const double t = st.t[b], tmax = st.tmax[b];
+    int run = 0;
+    if (t < tmax) {
+      double dt = cour / m;
+      if (t + dt > tmax) dt = tmax - t;
+      st.dt[b] = dt;
+      run = 1;
+    }
+    st.run[b] = run;
+  }
+}
+
+// t += dt of the running boxes; counts the boxes still short of t_max.
+__global__ __launch_bounds__(BLK) void kb_time(BoxState st, int nbox) {
+  __shared__ int cnt[BLK / 64];
+  int n = 0;
+  for (int b = threadIdx.x; b < nbox; b += BLK) {
+    double t = st.t[b];
+    if (st.run[b]) {
+      t += st.dt[b];
+      st.t[b] = t;
+      st.steps[b] += 1;
+    }
+    n += (t < st.tmax[b]) ? 1 : 0;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off, 64);
+  if ((threadIdx.x & 63) == 0) cnt[threadIdx.x >> 6] = n;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int tot = 0;
+#pragma unroll
+    for (int q = 0; q < BLK / 64; ++q) tot += cnt[q];
+    *st.nrun = tot;
+  }
+}
+
+__global__ __launch_bounds__(BLK) void kb_ghost(long sv, double *w, double *b,
+                                                const int2 *__restrict__ pairs, long npairs) {
+  for (long t = (long)blockIdx.x * BLK + threadIdx.x; t < npairs; t += (long)gridDim.x * BLK) {
+    const int2 ds = pairs[t];  // (ghost cell, source cell), pool indices
+#pragma unroll
+    for (int n = 0; n < NMODE; ++n) w[n * sv + ds.x] = w[n * sv + ds.y];
+#pragma unroll
+    for (int n = 0; n < 3; ++n) b[n * sv + ds.x] = b[n * sv + ds.y];
+  }
+}
+
+__global__ void kb_zero(unsigned long long *a, int n) {
+  for (int q = threadIdx.x + blockIdx.x * blockDim.x; q < n; q += blockDim.x * gridDim.x)
+    a[q] = 0ull;
+}
+
+}  // namespace
+
+// ==========================================================================
+// Host side
+// ==========================================================================
+
+struct bnz_batch {
+  bnz_params p{};
+  int nbox = 0, order = 1, ng = 3;
+  std::vector<BoxDev> hbox;
+  int nblk[NK] = {};
+  int bm_off[NK] = {};
+  long ncell = 0, nghost = 0;
+  hipStream_t stream = nullptr;
+  BoxDev *boxes = nullptr;
+  int *bmap = nullptr;
+  double *geo = nullptr;
+  int2 *ghost = nullptr;
+  Pools P{};
+  BoxState S{};
+  double *sbuf = nullptr;      // backing store of BoxState
+  hipGraph_t graph = nullptr;  // two captured steps
+  hipGraphExec_t exec = nullptr;
+  double *graph_b = nullptr;   // current face buffer when the graph was captured
+  bool ready = false;
+  std::string err;
+};
+
+namespace {
+
+int bfail(bnz_batch *c, int code, const std::string &m) {
+  if (c) c->err = m;
+  return code;
+}
+
+#define BCHK(ctx, call)                                                                 \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return bfail(ctx, BNZ_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));   \
+  } while (0)
+
+// Coordinates of one axis of one box (coord.pyx:201-257, cartesian.pyx:18-43),
+// the same construction as the single grid's.
+void axis_coords(int N, int ind1, int ind2, int nact, double lmin, double lmax, int bclo,
+                 int bchi, std::vector<double> *out) {
+  std::vector<double> lf(N + 1);
+  if (N == 1) {
+    lf[0] = lmin;
+    lf[1] = lmax;
+  } else {
+    const double dx = (lmax - lmin) / nact;
+    const double l1 = 0 * nact * dx;
+    for (int i = 0; i < N + 1; ++i) lf[i] = l1 + (i - ind1) * dx;
+    if (bclo == 0)
+      for (int i = ind1 - 1; i > -1; --i) {
+        const int i_ = ind2 - i + (ind1 - 1);
+        const double d = lf[i_ + 1] - lf[i_];
+        lf[i] = lf[i + 1] - d;
+      }
+    if (bchi == 0)
+      for (int i = ind2 + 1; i < N; ++i) {
+        const int i_ = ind1 + i - (ind2 + 1);
+        const double d = lf[i_ + 1] - lf[i_];
+        lf[i + 1] = lf[i] + d;
+      }
+  }
+  for (int q = 0; q < 4; ++q) out[q].resize(N);
+  for (int i = 0; i < N; ++i) {
+    const double dlf = lf[i + 1] - lf[i];
+    const double lv = 0.5 * (lf[i] + lf[i + 1]);
+    out[0][i] = dlf;
+    out[1][i] = 1. / dlf;
+    out[2][i] = lv - lf[i];
+    out[3][i] = lf[i + 1] - lv;
+  }
+}
+
+Box mkbox(int i0, int ni, int j0, int nj, int k0, int nk) {
+  Box b;
+  b.lo[0] = i0; b.n[0] = ni;
+  b.lo[1] = j0; b.n[1] = nj;
+  b.lo[2] = k0; b.n[2] = nk;
+  return b;
+}
+
+int grid_for(long nblk) { return (int)std::max<long>(1, std::min<long>(nblk, PERSIST)); }
+
+WList wlist(const bnz_batch *c, int kt) { return WList{c->bmap + c->bm_off[kt], c->nblk[kt], kt}; }
+
+template <int ORDER, int RS>
+void b_fluxes(bnz_batch *c, const double *w, const double *bf, int k0) {
+  double *fl[3] = {c->P.fx, c->P.fy, c->P.fz};
+  if (c->nblk[k0] > 0)
+    hipLaunchKernelGGL((kb_flux<0, ORDER, RS>), dim3(grid_for(c->nblk[k0])), dim3(BLK), 0,
+                       c->stream, c->boxes, wlist(c, k0), c->S, w, bf, fl[0]);
+  if (c->nblk[k0 + 1] > 0)
+    hipLaunchKernelGGL((kb_flux<1, ORDER, RS>), dim3(grid_for(c->nblk[k0 + 1])), dim3(BLK), 0,
+                       c->stream, c->boxes, wlist(c, k0 + 1), c->S, w, bf, fl[1]);
+  if (c->nblk[k0 + 2] > 0)
+    hipLaunchKernelGGL((kb_flux<2, ORDER, RS>), dim3(grid_for(c->nblk[k0 + 2])), dim3(BLK), 0,
+                       c->stream, c->boxes, wlist(c, k0 + 2), c->S, w, bf, fl[2]);
+}
+
+void b_flux_dispatch(bnz_batch *c, int order, const double *w, const double *bf, int k0) {
+  const bool hlld = c->p.rsolver == BNZ_RS_HLLD;
+  if (order == 0) {
+    if (hlld) b_fluxes<0, BNZ_RS_HLLD>(c, w, bf, k0);
+    else b_fluxes<0, BNZ_RS_HLL>(c, w, bf, k0);
+  } else {
+    if (hlld) b_fluxes<1, BNZ_RS_HLLD>(c, w, bf, k0);
+    else b_fluxes<1, BNZ_RS_HLL>(c, w, bf, k0);
+  }
+}
+
+void b_ghost(bnz_batch *c) {
+  if (c->nghost <= 0) return;
+  hipLaunchKernelGGL(kb_ghost, dim3(grid_for((c->nghost + BLK - 1) / BLK)), dim3(BLK), 0,
+                     c->stream, c->ncell, c->P.w, c->P.b, c->ghost, c->nghost);
+}
+
+// One VL step of every running box (vanleer.pyx:94-167): the same launch
+// sequence as the single grid's step.
+void b_step(bnz_batch *c) {
+  const int fr = c->p.dt_fms_ref;
+  hipLaunchKernelGGL(kb_dt, dim3(1), dim3(BLK), 0, c->stream, c->S, c->nbox, c->p.cour);
+  // predictor: donor cell, dt/2, limits of order 0
+  b_flux_dispatch(c, 0, c->P.w, c->P.b, KFX0);
+  hipLaunchKernelGGL(kb_emf, dim3(grid_for(c->nblk[KEM0])), dim3(BLK), 0, c->stream, c->boxes,
+                     wlist(c, KEM0), c->S, c->P.w, c->P.fx, c->P.fy, c->P.fz, c->P.ee);
+  hipLaunchKernelGGL(kb_update<true>, dim3(grid_for(c->nblk[KUPD])), dim3(BLK), 0, c->stream,
+                     c->boxes, wlist(c, KUPD), c->S, c->P, fr);
+  // corrector: reconstruction order, dt; b_{n+1} goes to the predictor's
+  // face buffer, which then becomes the current one
+  b_flux_dispatch(c, c->order, c->P.ws, c->P.bs, KFX1);
+  hipLaunchKernelGGL(kb_emf, dim3(grid_for(c->nblk[KEM1])), dim3(BLK), 0, c->stream, c->boxes,
+                     wlist(c, KEM1), c->S, c->P.ws, c->P.fx, c->P.fy, c->P.fz, c->P.ee);
+  hipLaunchKernelGGL(kb_update<false>, dim3(grid_for(c->nblk[KUPD])), dim3(BLK), 0, c->stream,
+                     c->boxes, wlist(c, KUPD), c->S, c->P, fr);
+  std::swap(c->P.b, c->P.bs);
+  b_ghost(c);
+  hipLaunchKernelGGL(kb_time, dim3(1), dim3(BLK), 0, c->stream, c->S, c->nbox);
+}
+
+int blaunch_check(bnz_batch *c) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    return bfail(c, BNZ_EHIP, std::string("kernel launch: ") + hipGetErrorString(e));
+  return BNZ_OK;
+}
+
+void drop_graph(bnz_batch *c) {
+  if (c->exec) (void)hipGraphExecDestroy(c->exec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->exec = nullptr;
+  c->graph = nullptr;
+  c->graph_b = nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *bnz_batch_last_error(const bnz_batch *c) { return c ? c->err.c_str() : "null batch"; }
+
+int bnz_batch_create(const bnz_params *p, const bnz_box *boxes, int nbox, bnz_batch **out) {
+  if (!out) return BNZ_EINVAL;
+  *out = nullptr;
+  if (!p || !boxes || nbox < 1) return BNZ_EINVAL;
+  bnz_batch *c = new bnz_batch();
+  *out = c;
+  c->p = *p;
+  c->nbox = nbox;
+  if (p->precision != BNZ_PREC_FP64)
+    return bfail(c, BNZ_EUNSUPPORTED, "batched boxes run the FP64 step");
+  if (p->rsolver != BNZ_RS_HLLD && p->rsolver != BNZ_RS_HLL)
+    return bfail(c, BNZ_EUNSUPPORTED, "rsolver must be hll or hlld");
+  if (p->reconstr != BNZ_RCN_CONST && p->reconstr != BNZ_RCN_LINEAR)
+    return bfail(c, BNZ_EUNSUPPORTED, "reconstr must be const or linear");
+  if (!(p->gam > 1.) || !(p->cour > 0.)) return bfail(c, BNZ_EINVAL, "need gam > 1 and cour > 0");
+  c->order = (p->reconstr == BNZ_RCN_CONST) ? 0 : 1;
+  c->ng = c->order ? 3 : 2;  // grid.pyx:132-150
+  const int ng = c->ng;
+  std::vector<double> hgeo;
+  std::vector<size_t> geo_off;  // 12 per box: (dlf, dlfi, icm, icp) x 3 axes
+  std::vector<int2> pairs;
+  c->hbox.resize(nbox);
+  long off = 0;
+  for (int b = 0; b < nbox; ++b) {
+    const bnz_box &s = boxes[b];
+    if (s.nx < 1 || s.ny < 1 || s.nz < 1 || (s.nz > 1 && s.ny == 1))
+      return bfail(c, BNZ_EINVAL, "box " + std::to_string(b) + ": bad size");
+    for (int q = 0; q < 6; ++q)
+      if (s.bc[q] != BNZ_BC_PERIODIC && s.bc[q] != BNZ_BC_OUTFLOW)
+        return bfail(c, BNZ_EUNSUPPORTED, "box " + std::to_string(b) +
+                                              ": boundary flags must be periodic or outflow");
+    for (int a = 0; a < 3; ++a)
+      if (!(s.lmax[a] > s.lmin[a]))
+        return bfail(c, BNZ_EINVAL, "box " + std::to_string(b) + ": need lmax > lmin");
+    BoxDev &d = c->hbox[b];
+    std::memset(&d, 0, sizeof(d));
+    const int dims = (s.nz > 1) ? 3 : (s.ny > 1) ? 2 : 1;
+    const int nt[3] = {s.nx + 2 * ng, dims >= 2 ? s.ny + 2 * ng : 1, dims >= 3 ? s.nz + 2 * ng : 1};
+    const int a1[3] = {ng, dims >= 2 ? ng : 0, dims >= 3 ? ng : 0};
+    const int a2[3] = {s.nx + ng - 1, dims >= 2 ? s.ny + ng - 1 : 0, dims >= 3 ? s.nz + ng - 1 : 0};
+    const int nact[3] = {s.nx, s.ny, s.nz};
+    for (int q = 0; q < 3; ++q) d.g.nt[q] = nt[q];
+    d.g.dims = dims;
+    d.g.sy = nt[0];
+    d.g.sz = nt[0] * nt[1];
+    d.g.gam = p->gam;
+    d.g.gamm1i = 1. / (p->gam - 1.);
+    d.off = off;
+    for (int a = 0; a < 3; ++a) {
+      std::vector<double> v[4];
+      axis_coords(nt[a], a1[a], a2[a], nact[a], s.lmin[a], s.lmax[a], s.bc[2 * a],
+                  s.bc[2 * a + 1], v);
+      for (int q = 0; q < 4; ++q) {
+        geo_off.push_back(hgeo.size());
+        hgeo.insert(hgeo.end(), v[q].begin(), v[q].end());
+      }
+    }
+    // work items of each list: the single grid's launch boxes
+    for (int o = 0; o < 2; ++o) {
+      const int ord = o == 0 ? 0 : c->order;
+      Lims &L = d.L[o];
+      for (int a = 0; a < 3; ++a) {
+        L.lo[a] = ord + 1;
+        L.hi[a] = nt[a] - 2 - ord;
+      }
+      if (dims < 2) L.lo[1] = L.hi[1] = 0;
+      if (dims < 3) L.lo[2] = L.hi[2] = 0;
+      const int il = L.lo[0], iu = L.hi[0], jl = L.lo[1], ju = L.hi[1], kl = L.lo[2], ku = L.hi[2];
+      const int kf = o == 0 ? KFX0 : KFX1;
+      const int j0 = dims >= 2 ? jl - 1 : 0, nj = dims >= 2 ? ju - jl + 3 : 1;
+      const int k0 = dims >= 3 ? kl - 1 : 0, nk = dims >= 3 ? ku - kl + 3 : 1;
+      d.items[kf] = mkbox(il, iu + 2 - il, j0, nj, k0, nk);
+      d.items[kf + 1] = dims >= 2 ? mkbox(il - 1, iu - il + 3, jl, ju + 2 - jl, k0, nk)
+                                  : mkbox(0, 0, 0, 0, 0, 0);
+      d.items[kf + 2] = dims >= 3 ? mkbox(il - 1, iu - il + 3, jl - 1, ju - jl + 3, kl, ku + 2 - kl)
+                                  : mkbox(0, 0, 0, 0, 0, 0);
+      d.items[kf + 3] = mkbox(il, iu + 2 - il, dims >= 2 ? jl : 0, dims >= 2 ? ju + 2 - jl : 1,
+                              dims >= 3 ? kl : 0, dims >= 3 ? ku + 2 - kl : 1);
+    }
+    d.items[KUPD] = mkbox(0, nt[0], 0, nt[1], 0, nt[2]);
+    d.act = mkbox(a1[0], s.nx, a1[1], dims >= 2 ? s.ny : 1, a1[2], dims >= 3 ? s.nz : 1);
+    d.items[KACT] = d.act;
+    // ghost (cell, source) pairs: per-axis source maps composed
+    // (grid_bc_funcs.pyx:36-196; z1 outflow copies k1 as intended)
+    std::vector<int> m[3];
+    for (int a = 0; a < 3; ++a) {
+      m[a].resize(nt[a]);
+      for (int i = 0; i < nt[a]; ++i) m[a][i] = i;
+      if (a >= dims) continue;
+      for (int gg = 0; gg < ng; ++gg) {
+        m[a][a1[a] - ng + gg] = (s.bc[2 * a] == BNZ_BC_PERIODIC) ? (a2[a] - ng + 1 + gg) : a1[a];
+        m[a][a2[a] + 1 + gg] = (s.bc[2 * a + 1] == BNZ_BC_PERIODIC) ? (a1[a] + gg) : a2[a];
+      }
+    }
+    for (int k = 0; k < nt[2]; ++k)
+      for (int j = 0; j < nt[1]; ++j)
+        for (int i = 0; i < nt[0]; ++i) {
+          const int si = m[0][i], sj = m[1][j], sk = m[2][k];
+          if (si == i && sj == j && sk == k) continue;
+          const long dc = off + ((long)k * nt[1] + j) * nt[0] + i;
+          const long sc = off + ((long)sk * nt[1] + sj) * nt[0] + si;
+          pairs.push_back(make_int2((int)dc, (int)sc));
+        }
+    off += (long)nt[0] * nt[1] * nt[2];
+    if (off >= (1L << 31))
+      return bfail(c, BNZ_EINVAL, "batch too large: pool cell indices are 32-bit");
+  }
+  c->ncell = off;
+  c->nghost = (long)pairs.size();
+  // work blocks: box b owns ceil(items / BLK) consecutive blocks of each list
+  std::vector<int> hmap;
+  for (int kt = 0; kt < NK; ++kt) {
+    c->bm_off[kt] = (int)hmap.size();
+    int nb = 0;
+    for (int b = 0; b < nbox; ++b) {
+      BoxDev &d = c->hbox[b];
+      d.blk0[kt] = nb;
+      const long blocks = (d.items[kt].total() + BLK - 1) / BLK;
+      for (long q = 0; q < blocks; ++q) hmap.push_back(b);
+      nb += (int)blocks;
+    }
+    c->nblk[kt] = nb;
+  }
+
+  if (hipSetDevice(p->device) != hipSuccess) return bfail(c, BNZ_EHIP, "hipSetDevice failed");
+  BCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  const size_t cells = (size_t)c->ncell;
+  struct {
+    double **q;
+    int rows;
+  } bufs[] = {{&c->P.w, NMODE}, {&c->P.ws, NMODE}, {&c->P.b, 3},     {&c->P.bs, 3},
+              {&c->P.fx, NMODE}, {&c->P.fy, NMODE}, {&c->P.fz, NMODE}, {&c->P.ee, 3}};
+  for (auto &q : bufs) {
+    if (hipMalloc(q.q, q.rows * cells * sizeof(double)) != hipSuccess)
+      return bfail(c, BNZ_ENOMEM, "device allocation of the box pools failed");
+    // zeroed once so reads of never-written ghost faces are defined
+    BCHK(c, hipMemsetAsync(*q.q, 0, q.rows * cells * sizeof(double), c->stream));
+  }
+  BCHK(c, hipMalloc(&c->geo, hgeo.size() * sizeof(double)));
+  BCHK(c, hipMemcpy(c->geo, hgeo.data(), hgeo.size() * sizeof(double), hipMemcpyHostToDevice));
+  for (int b = 0; b < nbox; ++b) {
+    Geo &g = c->hbox[b].g;
+    g.sv = c->ncell;
+    for (int a = 0; a < 3; ++a) {
+      g.dlf[a] = c->geo + geo_off[b * 12 + a * 4 + 0];
+      g.dlfi[a] = c->geo + geo_off[b * 12 + a * 4 + 1];
+      g.icm[a] = c->geo + geo_off[b * 12 + a * 4 + 2];
+      g.icp[a] = c->geo + geo_off[b * 12 + a * 4 + 3];
+    }
+  }
+  BCHK(c, hipMalloc(&c->boxes, nbox * sizeof(BoxDev)));
+  BCHK(c, hipMemcpy(c->boxes, c->hbox.data(), nbox * sizeof(BoxDev), hipMemcpyHostToDevice));
+  BCHK(c, hipMalloc(&c->bmap, std::max<size_t>(1, hmap.size()) * sizeof(int)));
+  BCHK(c, hipMemcpy(c->bmap, hmap.data(), hmap.size() * sizeof(int), hipMemcpyHostToDevice));
+  BCHK(c, hipMalloc(&c->ghost, std::max<size_t>(1, pairs.size()) * sizeof(int2)));
+  if (!pairs.empty())
+    BCHK(c, hipMemcpy(c->ghost, pairs.data(), pairs.size() * sizeof(int2), hipMemcpyHostToDevice));
+  // box scalars: dt, t, tmax, cmax (8 bytes each), run, steps (4 bytes), nrun
+  const size_t nb = nbox;
+  const size_t words = 4 * nb + (2 * nb + 1 + 1) / 2;
+  BCHK(c, hipMalloc(&c->sbuf, words * sizeof(double)));
+  c->S.dt = c->sbuf;
+  c->S.t = c->sbuf + nb;
+  c->S.tmax = c->sbuf + 2 * nb;
+  c->S.cmax = (unsigned long long *)(c->sbuf + 3 * nb);
+  c->S.run = (int *)(c->sbuf + 4 * nb);
+  c->S.steps = c->S.run + nb;
+  c->S.nrun = c->S.steps + nb;
+  std::vector<double> h(words, 0.);
+  for (int b = 0; b < nbox; ++b) h[2 * nb + b] = boxes[b].tmax;
+  BCHK(c, hipMemcpy(c->sbuf, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+  BCHK(c, hipStreamSynchronize(c->stream));
+  return BNZ_OK;
+}
+
+int bnz_batch_destroy(bnz_batch *c) {
+  if (!c) return BNZ_OK;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  drop_graph(c);
+  double *bufs[] = {c->P.w, c->P.ws, c->P.b,  c->P.bs, c->P.fx,
+                    c->P.fy, c->P.fz, c->P.ee, c->geo,  c->sbuf};
+  for (double *q : bufs)
+    if (q) (void)hipFree(q);
+  if (c->boxes) (void)hipFree(c->boxes);
+  if (c->bmap) (void)hipFree(c->bmap);
+  if (c->ghost) (void)hipFree(c->ghost);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return BNZ_OK;
+}
+
+int bnz_batch_box_shape(const bnz_batch *c, int box, int *ntot3) {
+  if (!c || box < 0 || box >= c->nbox || !ntot3 || c->hbox.size() != (size_t)c->nbox)
+    return BNZ_EINVAL;
+  for (int q = 0; q < 3; ++q) ntot3[q] = c->hbox[box].g.nt[q];
+  return BNZ_OK;
+}
+
+int bnz_batch_upload(bnz_batch *c, int box, const double *prim, const double *bfld) {
+  if (!c || box < 0 || box >= c->nbox) return BNZ_EINVAL;
+  if (!c->stream) return bfail(c, BNZ_ESTATE, "batch was not created");
+  const BoxDev &d = c->hbox[box];
+  const size_t n = (size_t)d.g.nt[0] * d.g.nt[1] * d.g.nt[2];
+  for (int v = 0; v < NMODE && prim; ++v)
+    BCHK(c, hipMemcpyAsync(c->P.w + v * c->ncell + d.off, prim + v * n, n * sizeof(double),
+                           hipMemcpyHostToDevice, c->stream));
+  for (int v = 0; v < 3 && bfld; ++v)
+    BCHK(c, hipMemcpyAsync(c->P.b + v * c->ncell + d.off, bfld + v * n, n * sizeof(double),
+                           hipMemcpyHostToDevice, c->stream));
+  BCHK(c, hipStreamSynchronize(c->stream));
+  c->ready = false;
+  return BNZ_OK;
+}
+
+int bnz_batch_download(bnz_batch *c, int box, double *prim, double *bfld) {
+  if (!c || box < 0 || box >= c->nbox) return BNZ_EINVAL;
+  if (!c->stream) return bfail(c, BNZ_ESTATE, "batch was not created");
+  BCHK(c, hipStreamSynchronize(c->stream));
+  const BoxDev &d = c->hbox[box];
+  const size_t n = (size_t)d.g.nt[0] * d.g.nt[1] * d.g.nt[2];
+  for (int v = 0; v < NMODE && prim; ++v)
+    BCHK(c, hipMemcpy(prim + v * n, c->P.w + v * c->ncell + d.off, n * sizeof(double),
+                      hipMemcpyDeviceToHost));
+  for (int v = 0; v < 3 && bfld; ++v)
+    BCHK(c, hipMemcpy(bfld + v * n, c->P.b + v * c->ncell + d.off, n * sizeof(double),
+                      hipMemcpyDeviceToHost));
+  return BNZ_OK;
+}
+
+int bnz_batch_init_state(bnz_batch *c) {
+  if (!c) return BNZ_EINVAL;
+  if (!c->stream) return bfail(c, BNZ_ESTATE, "batch was not created");
+  const int nb = c->nbox;
+  b_ghost(c);
+  hipLaunchKernelGGL(kb_interp, dim3(grid_for(c->nblk[KACT])), dim3(BLK), 0, c->stream, c->boxes,
+                     wlist(c, KACT), c->P.w, c->P.b);
+  b_ghost(c);
+  std::vector<double> z(nb, 0.);
+  std::vector<int> zi(nb + 1, 0);
+  BCHK(c, hipMemcpyAsync(c->S.t, z.data(), nb * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  BCHK(c, hipMemcpyAsync(c->S.steps, zi.data(), (nb + 1) * sizeof(int), hipMemcpyHostToDevice,
+                         c->stream));
+  hipLaunchKernelGGL(kb_zero, dim3(1), dim3(BLK), 0, c->stream, c->S.cmax, nb);
+  hipLaunchKernelGGL(kb_cfl, dim3(grid_for(c->nblk[KACT])), dim3(BLK), 0, c->stream, c->boxes,
+                     wlist(c, KACT), c->S, c->P.w, c->p.dt_fms_ref);
+  int rc = blaunch_check(c);
+  if (rc) return rc;
+  BCHK(c, hipStreamSynchronize(c->stream));
+  c->ready = true;
+  return BNZ_OK;
+}
+
+int bnz_batch_advance(bnz_batch *c, int max_steps, int use_graph, int *nsteps) {
+  if (!c) return BNZ_EINVAL;
+  if (!c->ready) return bfail(c, BNZ_ESTATE, "bnz_batch_init_state has not run");
+  if (use_graph && c->exec && c->graph_b != c->P.b) drop_graph(c);
+  if (use_graph && !c->exec) {
+    c->graph_b = c->P.b;
+    BCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    b_step(c);
+    b_step(c);  // two swaps: the face buffers are back where capture began
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (e != hipSuccess) {
+      c->graph_b = nullptr;
+      return bfail(c, BNZ_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    }
+    c->graph = g;
+    BCHK(c, hipGraphInstantiate(&c->exec, c->graph, nullptr, nullptr, 0));
+  }
+  int n = 0, nrun = 1;
+  while (nrun > 0 && (max_steps <= 0 || n < max_steps)) {
+    if (use_graph) {
+      BCHK(c, hipGraphLaunch(c->exec, c->stream));
+      n += 2;
+    } else {
+      b_step(c);
+      n += 1;
+    }
+    int rc = blaunch_check(c);
+    if (rc) return rc;
+    BCHK(c, hipMemcpyAsync(&nrun, c->S.nrun, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    BCHK(c, hipStreamSynchronize(c->stream));
+  }
+  if (nsteps) *nsteps = n;
+  return BNZ_OK;
+}
+
+int bnz_batch_get_times(bnz_batch *c, double *t, int *steps) {
+  if (!c) return BNZ_EINVAL;
+  if (!c->stream) return bfail(c, BNZ_ESTATE, "batch was not created");
+  BCHK(c, hipStreamSynchronize(c->stream));
+  if (t) BCHK(c, hipMemcpy(t, c->S.t, c->nbox * sizeof(double), hipMemcpyDeviceToHost));
+  if (steps) BCHK(c, hipMemcpy(steps, c->S.steps, c->nbox * sizeof(int), hipMemcpyDeviceToHost));
+  return BNZ_OK;
+}
+
+int bnz_batch_synchronize(bnz_batch *c) {
+  if (!c) return BNZ_EINVAL;
+  if (!c->stream) return bfail(c, BNZ_ESTATE, "batch was not created");
+  BCHK(c, hipStreamSynchronize(c->stream));
+  return BNZ_OK;
+}
+
+}  // extern "C"

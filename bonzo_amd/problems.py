@@ -255,13 +255,16 @@ def linear_wave(n, wave="fast", amp=1e-6, ng=3):
     return s, prim, bfld
 
 
-def blast2d(n, ng=3, seed=5005, p_in=10.0, p_out=0.1, radius=0.1):
+def blast2d(n, ng=3, seed=5005, p_in=10.0, p_out=0.1, radius=0.1, theta=None, bmag=None):
     """2D MHD blast on the periodic unit square: rho=1, p=p_out, uniform B at
     a random angle with |B| in [0.5, 2], p=p_in inside r<radius of the centre
-    (cf. src/problem/blast_MHD.pyx:24-59)."""
+    (cf. src/problem/blast_MHD.pyx:24-59).  theta and bmag, when given,
+    replace the seeded draws."""
     rng = np.random.Generator(np.random.PCG64(seed))
-    theta = rng.uniform(0.0, 2 * math.pi)
-    bmag = rng.uniform(0.5, 2.0)
+    th = rng.uniform(0.0, 2 * math.pi)
+    bm = rng.uniform(0.5, 2.0)
+    theta = th if theta is None else theta
+    bmag = bm if bmag is None else bmag
     s = GridShape(n, n, 1, ng)
     prim, bfld = s.empty()
     x = s.centres(0)
@@ -300,3 +303,26 @@ def riemann_tube(n, rng, ng=3):
     _put(bfld[1], s, np.where(left, L["by"], R["by"]))
     _put(bfld[2], s, np.where(left, L["bz"], R["bz"]))
     return s, prim, bfld
+
+
+def tube_sweep(nbox=512, seed=4004, nmin=64, nmax=4096, ng=3):
+    """Batch of seeded 1D Riemann problems of uneven length: lengths drawn
+    uniformly from [nmin, nmax], then one riemann_tube per box from the same
+    generator.  Returns a list of (shape, prim, bfld)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    lengths = rng.integers(nmin, nmax + 1, size=nbox)
+    return [riemann_tube(int(n), rng, ng) for n in lengths]
+
+
+def blast_sweep(nbox=64, seed=5005, nmin=64, nmax=512, ng=3):
+    """Batch of 2D periodic blast boxes of uneven side drawn uniformly from
+    [nmin, nmax], each with its own field angle in [0, 2pi) and |B| in
+    [0.5, 2].  Returns a list of (shape, prim, bfld)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sides = rng.integers(nmin, nmax + 1, size=nbox)
+    out = []
+    for n in sides:
+        theta = rng.uniform(0.0, 2 * math.pi)
+        bmag = rng.uniform(0.5, 2.0)
+        out.append(blast2d(int(n), ng, theta=theta, bmag=bmag))
+    return out

@@ -126,6 +126,46 @@ int bnz_kernel_timing(bnz_ctx *ctx, int idx, char *name, int name_len, double *t
 int bnz_gpu_flux_pencils(int rsolver, const double *wl, const double *wr, const double *bx,
                          double *flux, long n, double gam);
 
+/* ------------------------------------------------------------------------
+ * Batches of independent boxes of uneven size (parameter sweeps).  Each box
+ * is the grid that one bnz_create with the same bnz_params would build (its
+ * own sizes, extent, boundary flags and t_max); all boxes advance together,
+ * one launch per stage over every box, with a per-box CFL dt, time and
+ * "running" flag on the device, so that each box reproduces the single-grid
+ * run of vanleer.advance (src/integration/vanleer.pyx:87-167) to its own
+ * t_max bit for bit.  The reference has no batched form: a sweep there is one
+ * process per parameter set.
+ */
+typedef struct bnz_batch bnz_batch;
+
+typedef struct {
+  int nx, ny, nz;   /* active cells, as bnz_params                         */
+  int bc[6];        /* BNZ_BC_PERIODIC or BNZ_BC_OUTFLOW per side           */
+  double lmin[3];   /* extent                                              */
+  double lmax[3];
+  double tmax;      /* [time] tmax of this box                             */
+} bnz_box;
+
+/* Common selection from p (rsolver, reconstr, gam, cour, dt_fms_ref,
+   precision, device; its sizes, extent and flags are ignored). */
+int bnz_batch_create(const bnz_params *p, const bnz_box *boxes, int nbox, bnz_batch **out);
+int bnz_batch_destroy(bnz_batch *b);
+const char *bnz_batch_last_error(const bnz_batch *b);
+/* Ntot per axis of one box; its host arrays are [var][Ntot_z][Ntot_y][Ntot_x] */
+int bnz_batch_box_shape(const bnz_batch *b, int box, int *ntot3);
+int bnz_batch_upload(bnz_batch *b, int box, const double *prim, const double *bfld);
+int bnz_batch_download(bnz_batch *b, int box, double *prim, double *bfld);
+/* init_sim for every box and the first CFL maxima (vanleer.pyx:38-48) */
+int bnz_batch_init_state(bnz_batch *b);
+/* Steps until every box has reached its t_max, or max_steps (> 0) step
+   launches.  use_graph: replay a hipGraph holding two captured steps (steps
+   are then taken in pairs; a box that finishes in the first idles in the
+   second).  *nsteps gets the number of step launches. */
+int bnz_batch_advance(bnz_batch *b, int max_steps, int use_graph, int *nsteps);
+/* Per-box time and step count (arrays of nbox) */
+int bnz_batch_get_times(bnz_batch *b, double *t, int *steps);
+int bnz_batch_synchronize(bnz_batch *b);
+
 #ifdef __cplusplus
 }
 #endif

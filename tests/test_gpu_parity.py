@@ -97,10 +97,10 @@ def compare(res, label):
 
 
 def test_pencil_solvers_match_oracle():
-    import torch
     from bonzo_amd._lib import load
     from oracle import oracle as orc
     from tests import mathref as M
+    from tests.devmem import DeviceArray
     say("case: pencil hll/hlld, 4096 faces")
     rng = np.random.default_rng(21)
     wl, wr, bx = M.random_states(rng, 4096, bx_min=0.0)
@@ -108,14 +108,12 @@ def test_pencil_solvers_match_oracle():
     wr[1] += rng.uniform(-3, 3, 4096)
     for rs, rid in (("hll", 0), ("hlld", 2)):
         ref = orc.riemann(rs, wl, wr, bx, 5.0 / 3.0)
-        dwl = torch.tensor(wl, device="cuda")
-        dwr = torch.tensor(wr, device="cuda")
-        dbx = torch.tensor(bx, device="cuda")
-        out = torch.zeros((9, 4096), dtype=torch.float64, device="cuda")
-        rc = load().bnz_gpu_flux_pencils(rid, dwl.data_ptr(), dwr.data_ptr(), dbx.data_ptr(),
-                                         out.data_ptr(), 4096, 5.0 / 3.0)
+        bufs = [DeviceArray(a) for a in (wl, wr, bx, np.zeros((9, 4096)))]
+        rc = load().bnz_gpu_flux_pencils(rid, *(b.ptr for b in bufs), 4096, 5.0 / 3.0)
         assert rc == 0
-        got = out.cpu().numpy()
+        got = bufs[3].get()
+        for b in bufs:
+            b.free()
         ref[6] = 0.0
         assert np.array_equal(got, ref), f"{rs}: max diff {np.max(np.abs(got - ref))}"
 

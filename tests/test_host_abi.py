@@ -40,10 +40,11 @@ def test_params_struct_layout_matches_header(tmp_path):
     src = tmp_path / "layout.c"
     src.write_text(
         '#include <stdio.h>\n#include <stddef.h>\n#include "bnzgpu.h"\n'
-        'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu\\n", sizeof(bnz_params),'
+        'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(bnz_params),'
         ' offsetof(bnz_params, lmin), offsetof(bnz_params, gam), offsetof(bnz_params, device),'
         ' sizeof(bnz_diag_out), offsetof(bnz_diag_out, divb_max),'
-        ' offsetof(bnz_diag_out, nonfinite)); return 0;}\n')
+        ' offsetof(bnz_diag_out, nonfinite), sizeof(bnz_box), offsetof(bnz_box, lmin),'
+        ' offsetof(bnz_box, tmax)); return 0;}\n')
     exe = tmp_path / "layout"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)],
                    check=True)
@@ -51,7 +52,8 @@ def test_params_struct_layout_matches_header(tmp_path):
                                            check=True).stdout.split()]
     mine = [ctypes.sizeof(_lib.BnzParams), _lib.BnzParams.lmin.offset, _lib.BnzParams.gam.offset,
             _lib.BnzParams.device.offset, ctypes.sizeof(_lib.BnzDiag),
-            _lib.BnzDiag.divb_max.offset, _lib.BnzDiag.nonfinite.offset]
+            _lib.BnzDiag.divb_max.offset, _lib.BnzDiag.nonfinite.offset,
+            ctypes.sizeof(_lib.BnzBox), _lib.BnzBox.lmin.offset, _lib.BnzBox.tmax.offset]
     assert vals == mine
 
 
@@ -72,6 +74,29 @@ def test_create_rejects_bad_selection_without_touching_gpu():
     rc = L.bnz_create(ctypes.byref(p), ctypes.byref(h))
     assert rc == _lib.BNZ_EINVAL
     L.bnz_destroy(h)
+
+
+def test_batch_create_rejects_bad_selection_without_touching_gpu():
+    L = _lib.load()
+    p = _lib.make_params(precision="fp32c")
+    boxes = (_lib.BnzBox * 2)()
+    for b in boxes:
+        b.nx, b.ny, b.nz, b.tmax = 16, 1, 1, 0.1
+        for a in range(3):
+            b.lmax[a] = 1.0
+    h = ctypes.c_void_p()
+    assert L.bnz_batch_create(ctypes.byref(p), boxes, 2, ctypes.byref(h)) == _lib.BNZ_EUNSUPPORTED
+    assert b"FP64" in L.bnz_batch_last_error(h)
+    L.bnz_batch_destroy(h)
+    p = _lib.make_params(rsolver="hllc")
+    assert L.bnz_batch_create(ctypes.byref(p), boxes, 2, ctypes.byref(h)) == _lib.BNZ_EUNSUPPORTED
+    L.bnz_batch_destroy(h)
+    p = _lib.make_params()
+    boxes[1].bc[0] = 2  # reflect
+    assert L.bnz_batch_create(ctypes.byref(p), boxes, 2, ctypes.byref(h)) == _lib.BNZ_EUNSUPPORTED
+    assert b"box 1" in L.bnz_batch_last_error(h)
+    L.bnz_batch_destroy(h)
+    assert L.bnz_batch_create(ctypes.byref(p), boxes, 0, ctypes.byref(h)) == _lib.BNZ_EINVAL
 
 
 def test_selection_semantics_follow_reference(tmp_path):
