@@ -26,6 +26,8 @@ METRIC = "cell-updates/s at 256^3 HLLD+CT, FP32-comp vs FP64 on 1 MI355X; %HBM r
 HBM_PEAK_GBS = 8000.0
 
 # Algorithmic HBM bytes per unit of each kernel (4- or 8-byte words):
+#   flux3_*:  per cell of the union of the face ranges, read 9 prim + 3
+#             face-normal B; plus per face written, 8 flux components
 #   flux_*:   per face, read 9 prim + 1 face-normal B, write 8 flux components
 #   emf_*:    per edge triple, read 6 flux components + 6 prim (v, B), write 3
 #   update_*: per cell, FP64: read 9 prim + 18 lower-face fluxes + 3 EMFs +
@@ -51,6 +53,8 @@ def kernel_units(name, n, ng=3):
     order = 1 if name.endswith("plm") or name.endswith("corr") else 0
     lo, hi = order + 1, nt - 2 - order
     m = hi - lo + 1
+    if name.startswith("flux3"):
+        return (m + 2) ** 3
     if name.startswith("flux"):
         return (m + 1) * (m + 2) * (m + 2)
     if name.startswith("emf"):
@@ -59,6 +63,10 @@ def kernel_units(name, n, ng=3):
 
 
 def kernel_bytes(prec, name, n):
+    word = 8 if prec == "fp64" else 4
+    if name.startswith("flux3"):
+        faces = 3 * kernel_units(name.replace("flux3", "flux"), n)
+        return kernel_units(name, n) * 12 * word + faces * 8 * word
     table = KERNEL_BYTES[prec]
     key = name if name in table else name.split("_")[0]
     return table[key] * kernel_units(name, n)
@@ -149,6 +157,55 @@ def run_path(prec, n, prim, bfld, args):
     }
 
 
+def run_batch_workload(cases, reps, use_graph=True):
+    """Time a batch of boxes from their initial states to their t_max:
+    ``reps`` timed repetitions after one warm-up (which also captures the
+    graph); cell-updates = sum over boxes of active cells x steps taken."""
+    from bonzo_amd._lib import Batch
+    b = Batch([dict(nx=s.nx, ny=s.ny, nz=s.nz, bc=bc, lmin=s.lmin, lmax=s.lmax, tmax=tm)
+               for (s, _, _), bc, tm in cases], rsolver="hlld", reconstr="linear", cour=0.4)
+
+    def reset():
+        for q, ((s, prim, bfld), _, _) in enumerate(cases):
+            b.upload(q, prim, bfld)
+        b.init_state()
+
+    reset()
+    b.advance(max_steps=2, use_graph=use_graph)
+    walls = []
+    for _ in range(reps):
+        reset()
+        b.synchronize()
+        t0 = time.perf_counter()
+        b.advance(use_graph=use_graph)  # returns after the device has finished
+        walls.append(time.perf_counter() - t0)
+    t, steps = b.times()
+    b.close()
+    cells = [s.nx * s.ny * s.nz for (s, _, _), _, _ in cases]
+    cu = float(sum(c * int(n) for c, n in zip(cells, steps)))
+    med = sorted(walls)[len(walls) // 2]
+    return {"value": cu / med, "unit": "cell-updates/s", "wall_s_median": med,
+            "wall_s": walls, "cell_updates": cu, "boxes": len(cases), "cells": sum(cells),
+            "steps_min": int(steps.min()), "steps_max": int(steps.max()),
+            "all_at_tmax": bool(all(t[q] == cases[q][2] for q in range(len(cases)))),
+            "graph": use_graph}
+
+
+def batch_benches(reps):
+    from bonzo_amd import problems
+    tubes = [(c, (1,) * 6, 0.1) for c in problems.tube_sweep(512, seed=4004)]
+    out = {"tubes": run_batch_workload(tubes, reps)}
+    out["tubes"]["workload"] = ("512 seeded 1D MHD Riemann tubes, lengths 64-4096 cells on [0,1], "
+                                "outflow, VL + PLM + HLLD, CFL 0.4, FP64, each to t=0.1")
+    out["tubes_no_graph"] = run_batch_workload(tubes, 1, use_graph=False)
+    del tubes
+    blasts = [(c, (0,) * 6, 0.2) for c in problems.blast_sweep(64, seed=5005)]
+    out["blast2d"] = run_batch_workload(blasts, max(1, reps // 2 + 1))
+    out["blast2d"]["workload"] = ("64 periodic 2D MHD blast boxes, sides 64-512 on [0,1]^2, "
+                                  "VL + PLM + HLLD + CT, CFL 0.4, FP64, each to t=0.2")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -160,6 +217,11 @@ def main():
     ap.add_argument("--no-fp32c", action="store_true", help="time the FP64 path only")
     ap.add_argument("--accuracy-steps", type=int, default=100,
                     help="steps after which FP32C is compared with FP64 (0: skip)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-box workloads")
+    ap.add_argument("--batch-reps", type=int, default=5,
+                    help="timed repetitions of the batched tube sweep (blast boxes: half)")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="only run the batched-box workloads")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB mebibytes (rocprof counter scale)")
     args = ap.parse_args()
@@ -175,6 +237,9 @@ def main():
 
     from bonzo_amd import problems
 
+    if args.batch_only:
+        print(json.dumps({"batched": batch_benches(args.batch_reps)}), flush=True)
+        return
     n = args.n
     s, prim, bfld = problems.turbulence(n)
     r64 = run_path("fp64", n, prim, bfld, args)
@@ -236,6 +301,8 @@ def main():
             fp32c["rel_l1_max"] = max(l1.values())
             fp32c["accuracy_steps"] = r64["steps_total"]
         out["fp32c"] = fp32c
+    if not args.no_batch:
+        out["batched"] = batch_benches(args.batch_reps)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

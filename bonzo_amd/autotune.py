@@ -1,25 +1,24 @@
-"""Autotuner for the marching-chunk lengths of the 3D kernels.
+"""Autotuner for the work shapes of the 3D kernels.
 
-Each candidate chunk length (0 = one thread per face/edge/cell in x-fastest
-order, 1 = one thread per face/edge/cell on 64x4 tiles ordered along the
-marching axis, >1 = register-window marching over that many cells) is applied
-to all three kernel classes, a few VL steps are run with per-launch HIP-event
-timing on the context's stream, and each class keeps the candidate with the
-smallest summed time:
-  flux   -- y and z Godunov-flux kernels (both stages)
+Each candidate shape (0 = one thread per face/edge/cell in x-fastest order,
+1 = 64x4 tiles ordered along z, n > 1 = slab columns of n rows walked plane
+by plane, so that the next plane's rows are still in the XCD's L2 when they
+are reused) is applied to all three kernel classes, a few VL steps are run
+with per-launch HIP-event timing on the context's stream, and each class
+keeps the candidate with the smallest summed time:
+  flux   -- Godunov fluxes of both stages (all directions in one launch)
   emf    -- edge EMF kernels
-  update -- stage updates plus the CFL reduction (chunk > 0 fuses it into
-            the corrector update, so the standalone dt_partial kernel counts)
+  update -- stage updates (the corrector carries the fused CFL reduction)
 The runs advance the context's state: callers tune first and then upload
 their initial state again, or tune a scratch context of the same shape.
 """
-CANDIDATES = (0, 1, 2, 4, 8, 16)
+CANDIDATES = (0, 1, 4, 8, 16, 32)
 
 
 def _class_times(kt):
     out = {"flux": 0.0, "emf": 0.0, "update": 0.0}
     for name, (tot, n) in kt.items():
-        if name.startswith("flux_y") or name.startswith("flux_z"):
+        if name.startswith("flux"):
             out["flux"] += tot
         elif name.startswith("emf"):
             out["emf"] += tot

@@ -24,9 +24,10 @@ using namespace bnz;
 
 namespace {
 
-// work lists: fluxes x/y/z and EMFs of the predictor and of the corrector,
-// all cells (stage updates), active cells (init and the first CFL maximum)
-enum { KFX0 = 0, KFY0, KFZ0, KEM0, KFX1, KFY1, KFZ1, KEM1, KUPD, KACT, NK };
+// work lists: fluxes (all directions, one item per cell of the union of the
+// face ranges) and EMFs of the predictor and of the corrector, all cells
+// (stage updates), active cells (init and the first CFL maximum)
+enum { KF0 = 0, KEM0, KF1, KEM1, KUPD, KACT, NK };
 
 struct BoxDev {
   Geo g;         // strides and geometry of the box; g.sv is the pool stride
@@ -58,11 +59,12 @@ struct WList {
 
 constexpr int PERSIST = 256 * 8;  // workgroups of a persistent launch
 
-template <int DIR, int ORDER, int RS>
+template <int ORDER, int RS>
 __global__ __launch_bounds__(BLK) void kb_flux(const BoxDev *__restrict__ boxes, WList wl,
                                                BoxState st, const double *__restrict__ w,
                                                const double *__restrict__ bf,
-                                               double *__restrict__ fl) {
+                                               double *__restrict__ fx, double *__restrict__ fy,
+                                               double *__restrict__ fz) {
   for (int wb = blockIdx.x; wb < wl.nblk; wb += gridDim.x) {
     const int b = wl.bmap[wb];
     if (!st.run[b]) continue;
@@ -72,7 +74,9 @@ __global__ __launch_bounds__(BLK) void kb_flux(const BoxDev *__restrict__ boxes,
     if (t >= it.total()) continue;
     int i, j, k;
     box_idx(it, t, i, j, k);
-    flux_face<DIR, ORDER, RS>(d.g, w + d.off, bf + d.off, fl + d.off, i, j, k);
+    const long o = d.off;
+    flux_cell<ORDER, RS>(d.g, w + o, bf + o, fx + o, fy + o, fz + o, d.L[wl.kt == KF0 ? 0 : 1], i,
+                         j, k);
   }
 }
 
@@ -338,17 +342,9 @@ int grid_for(long nblk) { return (int)std::max<long>(1, std::min<long>(nblk, PER
 WList wlist(const bnz_batch *c, int kt) { return WList{c->bmap + c->bm_off[kt], c->nblk[kt], kt}; }
 
 template <int ORDER, int RS>
-void b_fluxes(bnz_batch *c, const double *w, const double *bf, int k0) {
-  double *fl[3] = {c->P.fx, c->P.fy, c->P.fz};
-  if (c->nblk[k0] > 0)
-    hipLaunchKernelGGL((kb_flux<0, ORDER, RS>), dim3(grid_for(c->nblk[k0])), dim3(BLK), 0,
-                       c->stream, c->boxes, wlist(c, k0), c->S, w, bf, fl[0]);
-  if (c->nblk[k0 + 1] > 0)
-    hipLaunchKernelGGL((kb_flux<1, ORDER, RS>), dim3(grid_for(c->nblk[k0 + 1])), dim3(BLK), 0,
-                       c->stream, c->boxes, wlist(c, k0 + 1), c->S, w, bf, fl[1]);
-  if (c->nblk[k0 + 2] > 0)
-    hipLaunchKernelGGL((kb_flux<2, ORDER, RS>), dim3(grid_for(c->nblk[k0 + 2])), dim3(BLK), 0,
-                       c->stream, c->boxes, wlist(c, k0 + 2), c->S, w, bf, fl[2]);
+void b_fluxes(bnz_batch *c, const double *w, const double *bf, int kt) {
+  hipLaunchKernelGGL((kb_flux<ORDER, RS>), dim3(grid_for(c->nblk[kt])), dim3(BLK), 0, c->stream,
+                     c->boxes, wlist(c, kt), c->S, w, bf, c->P.fx, c->P.fy, c->P.fz);
 }
 
 void b_flux_dispatch(bnz_batch *c, int order, const double *w, const double *bf, int k0) {
@@ -374,14 +370,14 @@ void b_step(bnz_batch *c) {
   const int fr = c->p.dt_fms_ref;
   hipLaunchKernelGGL(kb_dt, dim3(1), dim3(BLK), 0, c->stream, c->S, c->nbox, c->p.cour);
   // predictor: donor cell, dt/2, limits of order 0
-  b_flux_dispatch(c, 0, c->P.w, c->P.b, KFX0);
+  b_flux_dispatch(c, 0, c->P.w, c->P.b, KF0);
   hipLaunchKernelGGL(kb_emf, dim3(grid_for(c->nblk[KEM0])), dim3(BLK), 0, c->stream, c->boxes,
                      wlist(c, KEM0), c->S, c->P.w, c->P.fx, c->P.fy, c->P.fz, c->P.ee);
   hipLaunchKernelGGL(kb_update<true>, dim3(grid_for(c->nblk[KUPD])), dim3(BLK), 0, c->stream,
                      c->boxes, wlist(c, KUPD), c->S, c->P, fr);
   // corrector: reconstruction order, dt; b_{n+1} goes to the predictor's
   // face buffer, which then becomes the current one
-  b_flux_dispatch(c, c->order, c->P.ws, c->P.bs, KFX1);
+  b_flux_dispatch(c, c->order, c->P.ws, c->P.bs, KF1);
   hipLaunchKernelGGL(kb_emf, dim3(grid_for(c->nblk[KEM1])), dim3(BLK), 0, c->stream, c->boxes,
                      wlist(c, KEM1), c->S, c->P.ws, c->P.fx, c->P.fy, c->P.fz, c->P.ee);
   hipLaunchKernelGGL(kb_update<false>, dim3(grid_for(c->nblk[KUPD])), dim3(BLK), 0, c->stream,
@@ -480,16 +476,10 @@ int bnz_batch_create(const bnz_params *p, const bnz_box *boxes, int nbox, bnz_ba
       if (dims < 2) L.lo[1] = L.hi[1] = 0;
       if (dims < 3) L.lo[2] = L.hi[2] = 0;
       const int il = L.lo[0], iu = L.hi[0], jl = L.lo[1], ju = L.hi[1], kl = L.lo[2], ku = L.hi[2];
-      const int kf = o == 0 ? KFX0 : KFX1;
-      const int j0 = dims >= 2 ? jl - 1 : 0, nj = dims >= 2 ? ju - jl + 3 : 1;
-      const int k0 = dims >= 3 ? kl - 1 : 0, nk = dims >= 3 ? ku - kl + 3 : 1;
-      d.items[kf] = mkbox(il, iu + 2 - il, j0, nj, k0, nk);
-      d.items[kf + 1] = dims >= 2 ? mkbox(il - 1, iu - il + 3, jl, ju + 2 - jl, k0, nk)
-                                  : mkbox(0, 0, 0, 0, 0, 0);
-      d.items[kf + 2] = dims >= 3 ? mkbox(il - 1, iu - il + 3, jl - 1, ju - jl + 3, kl, ku + 2 - kl)
-                                  : mkbox(0, 0, 0, 0, 0, 0);
-      d.items[kf + 3] = mkbox(il, iu + 2 - il, dims >= 2 ? jl : 0, dims >= 2 ? ju + 2 - jl : 1,
-                              dims >= 3 ? kl : 0, dims >= 3 ? ku + 2 - kl : 1);
+      d.items[o == 0 ? KF0 : KF1] = flux3_box(dims, L);
+      d.items[o == 0 ? KEM0 : KEM1] =
+          mkbox(il, iu + 2 - il, dims >= 2 ? jl : 0, dims >= 2 ? ju + 2 - jl : 1,
+                dims >= 3 ? kl : 0, dims >= 3 ? ku + 2 - kl : 1);
     }
     d.items[KUPD] = mkbox(0, nt[0], 0, nt[1], 0, nt[2]);
     d.act = mkbox(a1[0], s.nx, a1[1], dims >= 2 ? s.ny : 1, a1[2], dims >= 3 ? s.nz : 1);
@@ -682,11 +672,16 @@ int bnz_batch_advance(bnz_batch *c, int max_steps, int use_graph, int *nsteps) {
     c->graph = g;
     BCHK(c, hipGraphInstantiate(&c->exec, c->graph, nullptr, nullptr, 0));
   }
+  // the host reads the running count after every REPLAYS graph replays (a
+  // replay after the last box finished only launches idle kernels)
+  constexpr int REPLAYS = 4;
   int n = 0, nrun = 1;
   while (nrun > 0 && (max_steps <= 0 || n < max_steps)) {
     if (use_graph) {
-      BCHK(c, hipGraphLaunch(c->exec, c->stream));
-      n += 2;
+      for (int r = 0; r < REPLAYS && (max_steps <= 0 || n < max_steps); ++r) {
+        BCHK(c, hipGraphLaunch(c->exec, c->stream));
+        n += 2;
+      }
     } else {
       b_step(c);
       n += 1;

@@ -59,30 +59,53 @@ __device__ __forceinline__ void box_idx(const Box &b, long t, int &i, int &j, in
   k = b.lo[2] + (int)(r / ny);
 }
 
-// Thread -> cell of box b.  Flat: x fastest over the whole box.  Tiled
-// (TA, MA): a workgroup is 64 x-lanes by 4 rows of axis TA at one index of
-// axis MA, and consecutive workgroups step along MA, so stencil neighbours
-// along MA are read by workgroups that run back to back on the same XCD.
+// Thread -> cell of box b.
+//   flat (ty <= 0): x fastest over the whole box;
+//   TILED (TA, MA): a workgroup is 64 x-lanes by 4 rows of axis TA at one
+//     index of axis MA, and consecutive workgroups step along MA, so stencil
+//     neighbours along MA are read by workgroups that run back to back on
+//     the same XCD;
+//   slab (ty > 1): the box is cut into columns of ty rows (y); a column is
+//     walked plane by plane (z), x fastest within a plane, so the rows of
+//     plane k+1 that plane k reads as neighbours are still in the XCD's L2
+//     when plane k+1 is processed.
 template <bool TILED, int TA, int MA>
-__device__ __forceinline__ bool cell_index(const Box &b, int &i, int &j, int &k) {
-  if (!TILED) {
+__device__ __forceinline__ bool cell_index(const Box &b, int ty, int &i, int &j, int &k) {
+  if (TILED) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int nbi = (b.n[0] + 63) / 64, nm = b.n[MA];
+    long L = xcd_block();
+    const int m = (int)(L % nm);
+    L /= nm;
+    const int bi = (int)(L % nbi), bt = (int)(L / nbi);
+    const int ii = bi * 64 + lane, tt = bt * 4 + wv;
+    if (ii >= b.n[0] || tt >= b.n[TA]) return false;
+    i = b.lo[0] + ii;
+    const int vt = b.lo[TA] + tt, vm = b.lo[MA] + m;
+    j = (TA == 1) ? vt : vm;
+    k = (TA == 2) ? vt : vm;
+    return true;
+  }
+  if (ty <= 0) {
     const long t = xcd_block() * BLK + threadIdx.x;
     if (t >= b.total()) return false;
     box_idx(b, t, i, j, k);
     return true;
   }
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int nbi = (b.n[0] + 63) / 64, nm = b.n[MA];
-  long L = xcd_block();
-  const int m = (int)(L % nm);
-  L /= nm;
-  const int bi = (int)(L % nbi), bt = (int)(L / nbi);
-  const int ii = bi * 64 + lane, tt = bt * 4 + wv;
-  if (ii >= b.n[0] || tt >= b.n[TA]) return false;
-  i = b.lo[0] + ii;
-  const int vt = b.lo[TA] + tt, vm = b.lo[MA] + m;
-  j = (TA == 1) ? vt : vm;
-  k = (TA == 2) ? vt : vm;
+  const int per = ty * b.n[0];                // cells of one column plane
+  const int bpp = (per + BLK - 1) / BLK;      // workgroups per column plane
+  const long L = xcd_block();
+  const long bcol = (long)bpp * b.n[2];       // workgroups per column
+  const int col = (int)(L / bcol);
+  const long rem = L - col * bcol;
+  const int kk = (int)(rem / bpp);
+  const int t = (int)(rem - (long)kk * bpp) * BLK + threadIdx.x;
+  if (t >= per) return false;
+  const int jj = col * ty + t / b.n[0];
+  if (jj >= b.n[1]) return false;
+  i = b.lo[0] + t % b.n[0];
+  j = b.lo[1] + jj;
+  k = b.lo[2] + kk;
   return true;
 }
 
@@ -198,7 +221,8 @@ struct bnz_ctx {
   float *ee32 = nullptr;                  // [3][cells]
   bool ready = false;
   bool timing = false;
-  // tuning: chunk lengths of the marching kernels (3D)
+  // tuning: work shapes of the flux, EMF and update kernels (3D), see
+  // work_shape()
   int flux_chunk = 0, emf_chunk = 0, update_chunk = 0;
   // CFL partial maxima left by the last corrector for the next step
   int dt_parts = 0;
@@ -228,9 +252,22 @@ inline int tiled_blocks(const Box &b, int ta, int ma) {
   return ((b.n[0] + 63) / 64) * ((b.n[ta] + 3) / 4) * b.n[ma];
 }
 
-// Workgroups of a marching kernel: 64 x-lanes x 4 transverse rows x chunks.
-inline int march_blocks(int nx, int nt, int nm, int chunk) {
-  return ((nx + 63) / 64) * ((nt + 3) / 4) * ((nm + chunk - 1) / chunk);
+// Workgroups of a slab-column launch over box b with columns of ty rows.
+inline int slab_blocks(const Box &b, int ty) {
+  return ((b.n[1] + ty - 1) / ty) * b.n[2] * ((ty * b.n[0] + BLK - 1) / BLK);
+}
+
+// Work shape of a kernel class from its tuning value (3D only): 0 flat,
+// 1 tiles of 64 x 4 along z, > 1 slab columns of that many rows.
+struct Shape {
+  bool tiled;
+  int ty;
+  int blocks;
+};
+inline Shape work_shape(const Box &b, int dims, int chunk) {
+  if (dims < 3 || chunk <= 0) return Shape{false, 0, blocks_for(b.total())};
+  if (chunk == 1) return Shape{true, 0, tiled_blocks(b, 1, 2)};
+  return Shape{false, chunk, slab_blocks(b, chunk)};
 }
 
 struct Launch {
@@ -281,6 +318,20 @@ inline Lims integr_lims(const bnz_ctx *c, int order) {
   if (c->dims < 2) L.lo[1] = L.hi[1] = 0;
   if (c->dims < 3) L.lo[2] = L.hi[2] = 0;
   return L;
+}
+
+// Union of the x, y and z face ranges of a stage with limits L: x faces
+// i in [il, iu+1], j in [jl-1, ju+1], k in [kl-1, ku+1]; y faces i in
+// [il-1, iu+1], j in [jl, ju+1]; z faces k in [kl, ku+1] (godunov.pyx:66-206).
+inline Box flux3_box(int dims, const Lims &L) {
+  Box b;
+  b.lo[0] = dims >= 2 ? L.lo[0] - 1 : L.lo[0];
+  b.n[0] = L.hi[0] + 1 - b.lo[0] + 1;
+  b.lo[1] = dims >= 2 ? L.lo[1] - 1 : 0;
+  b.n[1] = dims >= 2 ? L.hi[1] + 1 - b.lo[1] + 1 : 1;
+  b.lo[2] = dims >= 3 ? L.lo[2] - 1 : 0;
+  b.n[2] = dims >= 3 ? L.hi[2] + 1 - b.lo[2] + 1 : 1;
+  return b;
 }
 
 inline Box active_box(const bnz_ctx *c) {

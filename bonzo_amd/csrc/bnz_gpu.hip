@@ -27,34 +27,39 @@ using namespace bnz;
 namespace {
 
 // --------------------------------------------------------------------------
-// Godunov fluxes of one direction (godunov.pyx:20-210 with the double-buffered
-// left state of D1).  One thread per face; face index c is the lower face of
-// cell c, between cells c-1 and c.  The flux is stored in global variable
-// order, flux[n][c], except the normal-field slot, which no consumer reads.
-template <int DIR, int ORDER, int RS, bool TILED = false>
-__global__ __launch_bounds__(BLK) void k_flux(Geo g, const double *__restrict__ w,
-                                              const double *__restrict__ bf,
-                                              double *__restrict__ fl, Box box) {
+// Work shapes (3D): ty == 0 one thread per item in x-fastest order; TILED
+// 64 x 4 tiles ordered along z; ty > 1 slab columns of ty rows walked plane
+// by plane (see cell_index).
+//
+// All three face directions of a stage in one launch: one thread per cell of
+// the union of the face ranges computes the x, y and z faces whose lower
+// cell it is, so the prim stencil of a cell is fetched once per stage.
+template <int ORDER, int RS, bool TILED = false>
+__global__ __launch_bounds__(BLK) void k_flux3(Geo g, const double *__restrict__ w,
+                                               const double *__restrict__ bf,
+                                               double *__restrict__ fx, double *__restrict__ fy,
+                                               double *__restrict__ fz, Box box, Lims L,
+                                               int ty) {
   int i, j, k;
-  if (!cell_index<TILED, (DIR == 1 ? 2 : 1), (DIR == 1 ? 1 : 2)>(box, i, j, k)) return;
-  flux_face<DIR, ORDER, RS>(g, w, bf, fl, i, j, k);
+  if (!cell_index<TILED, 1, 2>(box, ty, i, j, k)) return;
+  flux_cell<ORDER, RS>(g, w, bf, fx, fy, fz, L, i, j, k);
 }
 
 // --------------------------------------------------------------------------
 // Edge EMFs, interp_ee1 (src/integration/ct.pyx:153-215) with the cell-centred
 // E = -v x B of ec_from_prim (ct.pyx:128-148) computed on the fly.
-template <bool TILED = false>
+template <int DIMS, bool TILED = false>
 __global__ __launch_bounds__(BLK) void k_emf(Geo g, const double *__restrict__ w,
                                              const double *__restrict__ fx,
                                              const double *__restrict__ fy,
                                              const double *__restrict__ fz,
-                                             double *__restrict__ ee, Box box) {
+                                             double *__restrict__ ee, Box box, int ty) {
   int i, j, k;
-  if (!cell_index<TILED, 1, 2>(box, i, j, k)) return;
-  emf_edge(g, w, fx, fy, fz, ee, i, j, k);
+  if (!cell_index<TILED, 1, 2>(box, ty, i, j, k)) return;
+  emf_edge_d<DIMS>(g, w, fx, fy, fz, ee, i, j, k);
 }
 
-template <bool PRED, bool DTRED = false, bool TILED = false>
+template <int DIMS, bool PRED, bool DTRED = false, bool TILED = false>
 __global__ __launch_bounds__(BLK) void k_update(Geo g, const double *w0, const double *__restrict__ fx,
                                                 const double *__restrict__ fy,
                                                 const double *__restrict__ fz,
@@ -62,296 +67,14 @@ __global__ __launch_bounds__(BLK) void k_update(Geo g, const double *w0, const d
                                                 const double *__restrict__ b0,
                                                 double *__restrict__ b1, double *w1, Lims L,
                                                 const double *__restrict__ dts, Box grid, Box act,
-                                                int fms_ref, double *__restrict__ part) {
+                                                int fms_ref, double *__restrict__ part,
+                                                int ty) {
   int i, j, k;
   double cdsmax = 0.;
-  if (cell_index<TILED, 1, 2>(grid, i, j, k))
-    cdsmax = update_cell<PRED, DTRED>(g, w0, fx, fy, fz, ee, b0, b1, w1, L, dts, act, fms_ref, i,
-                                      j, k);
+  if (cell_index<TILED, 1, 2>(grid, ty, i, j, k))
+    cdsmax = update_cell_d<DIMS, PRED, DTRED>(g, w0, fx, fy, fz, ee, b0, b1, w1, L, dts, act,
+                                              fms_ref, i, j, k);
   if (DTRED) block_max_to(cdsmax, part);
-}
-
-// --------------------------------------------------------------------------
-// Marching variants for 3D grids.  A workgroup is 64 x-lanes by 4 rows of
-// one transverse axis; each thread walks a chunk of the marching axis and
-// keeps the stencil values it shares with the next cell in registers, so a
-// cell is read once and reconstructed once per sweep.  Values and operation
-// order are those of the per-face / per-cell kernels above.
-
-
-struct MarchMap {
-  int ii, tt, m0, m1;
-};
-
-// Workgroup -> (x block, transverse block, chunk), XCD-major like xcd_block().
-__device__ __forceinline__ bool march_map(int nx, int nt, int nm, int chunk, MarchMap &mm) {
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int nbi = (nx + 63) / 64, nch = (nm + chunk - 1) / chunk;
-  long L = xcd_block();
-  const int bc = (int)(L % nch);  // neighbouring chunks run back to back
-  L /= nch;
-  const int bi = (int)(L % nbi);
-  const int bt = (int)(L / nbi);
-  mm.ii = bi * 64 + lane;
-  mm.tt = bt * 4 + wv;
-  mm.m0 = bc * chunk;
-  mm.m1 = min(mm.m0 + chunk, nm);
-  return mm.ii < nx && mm.tt < nt && mm.m0 < nm;
-}
-
-// Godunov fluxes along DIR (1: y, 2: z) marching over the faces of one
-// (x, transverse) column.  box: face index ranges as for k_flux.
-template <int DIR, int ORDER, int RS>
-__global__ __launch_bounds__(BLK) void k_flux_march(Geo g, const double *__restrict__ w,
-                                                    const double *__restrict__ bf,
-                                                    double *__restrict__ fl, Box box, int chunk) {
-  constexpr int TD = (DIR == 1) ? 2 : 1;
-  MarchMap mm;
-  if (!march_map(box.n[0], box.n[TD], box.n[DIR], chunk, mm)) return;
-  int idx[3];
-  idx[0] = box.lo[0] + mm.ii;
-  idx[TD] = box.lo[TD] + mm.tt;
-  idx[DIR] = box.lo[DIR] + mm.m0;
-  long c = ((long)idx[2] * g.nt[1] + idx[1]) * g.nt[0] + idx[0];
-  const long sd = (DIR == 1) ? g.sy : g.sz;
-  const long sv = g.sv;
-
-  double wm[NMODE], w0[NMODE], dm[NMODE];
-#pragma unroll
-  for (int n = 0; n < NMODE; ++n) {
-    if (n == BX) continue;
-    const double *p = w + (long)perm<DIR>(n) * sv + c;
-    wm[n] = p[-sd];
-    if (ORDER == 1) {
-      const double wm2 = p[-2 * sd];
-      w0[n] = p[0];
-      dm[n] = 0.5 * vl_lim(wm[n] - wm2, w0[n] - wm[n]);
-    }
-  }
-  for (int m = mm.m0; m < mm.m1; ++m, c += sd) {
-    double wl[NMODE], wr[NMODE];
-#pragma unroll
-    for (int n = 0; n < NMODE; ++n) {
-      if (n == BX) continue;
-      const double *p = w + (long)perm<DIR>(n) * sv + c;
-      if (ORDER == 0) {
-        const double wc = p[0];
-        wl[n] = wm[n];
-        wr[n] = wc;
-        wm[n] = wc;
-      } else {
-        const double wp = p[sd];
-        const double d0 = 0.5 * vl_lim(w0[n] - wm[n], wp - w0[n]);
-        wl[n] = wm[n] + dm[n];
-        wr[n] = w0[n] - d0;
-        wm[n] = w0[n];
-        w0[n] = wp;
-        dm[n] = d0;
-      }
-    }
-    const double bn = bf[(long)DIR * sv + c];
-    wl[BX] = bn;
-    wr[BX] = bn;
-    double f[NMODE];
-    const int reg = (RS == BNZ_RS_HLLD) ? flux_hlld(f, wl, wr, g.gam, g.gamm1i)
-                                        : flux_hll(f, wl, wr, g.gam, g.gamm1i);
-    if (reg == 0) continue;
-#pragma unroll
-    for (int n = 0; n < NMODE; ++n) {
-      if (n == BX) continue;
-      fl[(long)perm<DIR>(n) * sv + c] = f[n];
-    }
-  }
-}
-
-// interp_ee1 3D branch marching in z; box: edge ranges as for k_emf.
-__global__ __launch_bounds__(BLK) void k_emf_march(Geo g, const double *__restrict__ w,
-                                                   const double *__restrict__ fx,
-                                                   const double *__restrict__ fy,
-                                                   const double *__restrict__ fz,
-                                                   double *__restrict__ ee, Box box, int chunk) {
-  MarchMap mm;
-  if (!march_map(box.n[0], box.n[1], box.n[2], chunk, mm)) return;
-  const int i = box.lo[0] + mm.ii, j = box.lo[1] + mm.tt;
-  int k = box.lo[2] + mm.m0;
-  const long sv = g.sv, sy = g.sy, sz = g.sz;
-  long c = ((long)k * g.nt[1] + j) * g.nt[0] + i;
-  // plane k-1 values carried between iterations
-  double pfyz = fy[BZ * sv + c - sz], pfxz = fx[BZ * sv + c - sz];
-  double pe0a = ec_comp(w, sv, c - sz - sy, 0), pe0b = ec_comp(w, sv, c - sz, 0);
-  double pe1a = ec_comp(w, sv, c - sz - 1, 1), pe1b = ec_comp(w, sv, c - sz, 1);
-  for (int m = mm.m0; m < mm.m1; ++m, c += sz) {
-    const double fyz = fy[BZ * sv + c], fxz = fx[BZ * sv + c];
-    const double e0a = ec_comp(w, sv, c - sy, 0), e0b = ec_comp(w, sv, c, 0);
-    const double e1a = ec_comp(w, sv, c - 1, 1), e1b = ec_comp(w, sv, c, 1);
-    ee[0 * sv + c] = (0.5 * (-pfyz - fyz + fz[BY * sv + c - sy] + fz[BY * sv + c]) -
-                      0.25 * (e0a + e0b + pe0a + pe0b));
-    ee[1 * sv + c] = (0.5 * (-fz[BX * sv + c - 1] - fz[BX * sv + c] + pfxz + fxz) -
-                      0.25 * (e1a + e1b + pe1a + pe1b));
-    ee[2 * sv + c] = (0.5 * (-fx[BY * sv + c - sy] - fx[BY * sv + c] + fy[BX * sv + c - 1] +
-                             fy[BX * sv + c]) -
-                      0.25 * (ec_comp(w, sv, c - 1, 2) + ec_comp(w, sv, c, 2) +
-                              ec_comp(w, sv, c - sy - 1, 2) + ec_comp(w, sv, c - sy, 2)));
-    pfyz = fyz;
-    pfxz = fxz;
-    pe0a = e0a;
-    pe0b = e0b;
-    pe1a = e1a;
-    pe1b = e1b;
-  }
-}
-
-
-// Stage update for 3D grids marching in z over the whole grid (same work
-// and predicates as k_update).  DTRED: also reduce the next step's CFL
-// maximum over the active cells of the new prim into part[blockIdx.x].
-template <bool PRED, bool DTRED>
-__global__ __launch_bounds__(BLK) void k_update_march(
-    Geo g, const double *w0, const double *__restrict__ fx, const double *__restrict__ fy,
-    const double *__restrict__ fz, const double *__restrict__ ee, const double *__restrict__ b0,
-    double *__restrict__ b1, double *w1, Lims L, const double *__restrict__ dts, int chunk,
-    Box act, int fms_ref, double *__restrict__ part) {
-  __shared__ double red[BLK / 64];
-  double cdsmax = 0.;
-  MarchMap mm;
-  const bool live = march_map(g.nt[0], g.nt[1], g.nt[2], chunk, mm);
-  if (live) {
-    const long sv = g.sv, sy = g.sy, sz = g.sz;
-    const int i = mm.ii, j = mm.tt;
-    const double dt = PRED ? 0.5 * dts[0] : dts[0];
-    const bool hasI = i + 1 < g.nt[0], hasJ = j + 1 < g.nt[1];
-    const bool inI = i >= L.lo[0] && i <= L.hi[0];
-    const bool inJ = j >= L.lo[1] && j <= L.hi[1];
-    const bool inIf = i >= L.lo[0] && i <= L.hi[0] + 1;
-    const bool inJf = j >= L.lo[1] && j <= L.hi[1] + 1;
-    int k = mm.m0;
-    long c = ((long)k * g.nt[1] + j) * g.nt[0] + i;
-    // values at face/edge level k, carried from the previous iteration
-    double fzl[6], e0l = ee[c], e0l1 = hasJ ? ee[c + sy] : 0.;
-    double e1l = ee[sv + c], e1l1 = hasI ? ee[sv + c + 1] : 0., bzl = b0[2 * sv + c];
-#pragma unroll
-    for (int n = 0; n < 6; ++n) fzl[n] = fz[n * sv + c];
-    const double dlx = g.dlf[0][i], dly = g.dlf[1][j];
-    for (int m = mm.m0; m < mm.m1; ++m, ++k, c += sz) {
-      const bool hasK = k + 1 < g.nt[2];
-      double fzu[6], e0u = 0., e0u1 = 0., e1u = 0., e1u1 = 0., bzu = 0.;
-      if (hasK) {
-#pragma unroll
-        for (int n = 0; n < 6; ++n) fzu[n] = fz[n * sv + c + sz];
-        e0u = ee[c + sz];
-        e0u1 = hasJ ? ee[c + sz + sy] : 0.;
-        e1u = ee[sv + c + sz];
-        e1u1 = hasI ? ee[sv + c + sz + 1] : 0.;
-        bzu = b0[2 * sv + c + sz];
-      } else {
-#pragma unroll
-        for (int n = 0; n < 6; ++n) fzu[n] = 0.;
-      }
-      const bool inK = k >= L.lo[2] && k <= L.hi[2];
-      const bool inKf = k >= L.lo[2] && k <= L.hi[2] + 1;
-      const bool cell = inI && inJ && inK;
-      const bool fX = inIf && inJ && inK, fY = inI && inJf && inK, fZ = inI && inJ && inKf;
-      const double dlz = g.dlf[2][k];
-      const double e2 = ee[2 * sv + c];
-      const double e2i = hasI ? ee[2 * sv + c + 1] : 0.;
-      const double e2j = hasJ ? ee[2 * sv + c + sy] : 0.;
-      const double e2ij = (hasI && hasJ) ? ee[2 * sv + c + sy + 1] : 0.;
-      double bxl = 0., byl = 0., bzlo = 0.;
-      // advance_b (ct.pyx:15-91) for the lower faces of this cell
-      if (fX || cell) {
-        const double dtda = dt / (dly * dlz);
-        double v = b0[c];
-        v -= dtda * (dlz * e2j - dlz * e2);
-        v += dtda * (dly * e1u - dly * e1l);
-        bxl = v;
-        if (fX) b1[c] = v;
-      }
-      if (fY || cell) {
-        const double dtda = dt / (dlx * dlz);
-        double v = b0[sv + c] + dtda * (dlz * e2i - dlz * e2);
-        v -= dtda * (dlx * e0u - dlx * e0l);
-        byl = v;
-        if (fY) b1[sv + c] = v;
-      }
-      if (fZ || cell) {
-        const double dtda = dt / (dlx * dly);
-        double v = bzl - dtda * (dly * e1l1 - dly * e1l);
-        v += dtda * (dlx * e0l1 - dlx * e0l);
-        bzlo = v;
-        if (fZ) b1[2 * sv + c] = v;
-      }
-      if (!cell) {
-        if (PRED) {
-#pragma unroll
-          for (int n = 0; n < NMODE; ++n) w1[n * sv + c] = w0[n * sv + c];
-        }
-      } else {
-        double wv[NMODE], u[NMODE];
-#pragma unroll
-        for (int n = 0; n < NMODE; ++n) wv[n] = w0[n * sv + c];
-        prim2cons(u, wv, g.gamm1i);
-        const double dtdv = dt / (dlx * dly * dlz);
-        const double ax = dly * dlz, ay = dlx * dlz, az = dlx * dly;
-#pragma unroll
-        for (int n = 0; n < NMODE - 3; ++n) {
-          double v = u[n] - dtdv * (ax * fx[n * sv + c + 1] - ax * fx[n * sv + c]);
-          v -= dtdv * (ay * fy[n * sv + c + sy] - ay * fy[n * sv + c]);
-          v -= dtdv * (az * fzu[n] - az * fzl[n]);
-          u[n] = v;
-        }
-        // upper faces for interp_bc (same arithmetic as the lower ones)
-        double bxu, byu, bzu1;
-        {
-          const double dtda = dt / (dly * dlz);
-          double v = b0[c + 1];
-          v -= dtda * (dlz * e2ij - dlz * e2i);
-          v += dtda * (dly * e1u1 - dly * e1l1);
-          bxu = v;
-        }
-        {
-          const double dtda = dt / (dlx * dlz);
-          double v = b0[sv + c + sy] + dtda * (dlz * e2ij - dlz * e2j);
-          v -= dtda * (dlx * e0u1 - dlx * e0l1);
-          byu = v;
-        }
-        {
-          const double dtda = dt / (dlx * dly);
-          double v = bzu - dtda * (dly * e1u1 - dly * e1u);
-          v += dtda * (dlx * e0u1 - dlx * e0u);
-          bzu1 = v;
-        }
-        u[BX] = (g.icm[0][i] * bxl + g.icp[0][i] * bxu) * g.dlfi[0][i];
-        u[BY] = (g.icm[1][j] * byl + g.icp[1][j] * byu) * g.dlfi[1][j];
-        u[BZ] = (g.icm[2][k] * bzlo + g.icp[2][k] * bzu1) * g.dlfi[2][k];
-        cons2prim(wv, u, g.gam);
-#pragma unroll
-        for (int n = 0; n < NMODE; ++n) w1[n * sv + c] = wv[n];
-        if (DTRED) {
-          if (i >= act.lo[0] && i < act.lo[0] + act.n[0] && j >= act.lo[1] &&
-              j < act.lo[1] + act.n[1] && k >= act.lo[2] && k < act.lo[2] + act.n[2])
-            cdsmax = fmax(cdsmax, cds_cell(g, wv, fms_ref, i, j, k));
-        }
-      }
-#pragma unroll
-      for (int n = 0; n < 6; ++n) fzl[n] = fzu[n];
-      e0l = e0u;
-      e0l1 = e0u1;
-      e1l = e1u;
-      e1l1 = e1u1;
-      bzl = bzu;
-    }
-  }
-  if (DTRED) {
-    cdsmax = wave_max(cdsmax);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = cdsmax;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double mx = red[0];
-#pragma unroll
-      for (int q = 1; q < BLK / 64; ++q) mx = fmax(mx, red[q]);
-      part[blockIdx.x] = mx;
-    }
-  }
 }
 
 // --------------------------------------------------------------------------
@@ -633,51 +356,16 @@ std::vector<int> axis_map(int N, int a1, int a2, int ng, int f_lo, int f_hi, boo
 
 template <int ORDER, int RS>
 void launch_fluxes(bnz_ctx *c, const double *w, const double *bf, const Lims &L) {
-  const int il = L.lo[0], iu = L.hi[0], jl = L.lo[1], ju = L.hi[1], kl = L.lo[2], ku = L.hi[2];
-  const Geo &g = c->g;
-  {  // x faces i in [il, iu+1], rows j in [jl-1, ju+1], k in [kl-1, ku+1]
-    Box bx;
-    bx.lo[0] = il; bx.n[0] = iu + 1 - il + 1;
-    bx.lo[1] = (c->dims >= 2) ? jl - 1 : 0; bx.n[1] = (c->dims >= 2) ? (ju + 1) - (jl - 1) + 1 : 1;
-    bx.lo[2] = (c->dims >= 3) ? kl - 1 : 0; bx.n[2] = (c->dims >= 3) ? (ku + 1) - (kl - 1) + 1 : 1;
-    Launch t(c, ORDER ? "flux_x_plm" : "flux_x_dc");
-    hipLaunchKernelGGL((k_flux<0, ORDER, RS, false>), dim3(blocks_for(bx.total())), dim3(BLK), 0,
-                       c->stream, g, w, bf, c->fx, bx);
-  }
-  if (c->dims >= 2) {  // y faces j in [jl, ju+1], i in [il-1, iu+1], k in [kl-1, ku+1]
-    Box by;
-    by.lo[0] = il - 1; by.n[0] = (iu + 1) - (il - 1) + 1;
-    by.lo[1] = jl; by.n[1] = ju + 1 - jl + 1;
-    by.lo[2] = (c->dims >= 3) ? kl - 1 : 0; by.n[2] = (c->dims >= 3) ? (ku + 1) - (kl - 1) + 1 : 1;
-    Launch t(c, ORDER ? "flux_y_plm" : "flux_y_dc");
-    if (c->dims >= 3 && c->flux_chunk > 1)
-      hipLaunchKernelGGL((k_flux_march<1, ORDER, RS>),
-                         dim3(march_blocks(by.n[0], by.n[2], by.n[1], c->flux_chunk)), dim3(BLK), 0,
-                         c->stream, g, w, bf, c->fy, by, c->flux_chunk);
-    else if (c->dims >= 3 && c->flux_chunk == 1)
-      hipLaunchKernelGGL((k_flux<1, ORDER, RS, true>), dim3(tiled_blocks(by, 2, 1)), dim3(BLK), 0,
-                         c->stream, g, w, bf, c->fy, by);
-    else
-      hipLaunchKernelGGL((k_flux<1, ORDER, RS, false>), dim3(blocks_for(by.total())), dim3(BLK), 0,
-                         c->stream, g, w, bf, c->fy, by);
-  }
-  if (c->dims >= 3) {  // z faces k in [kl, ku+1], i in [il-1, iu+1], j in [jl-1, ju+1]
-    Box bz;
-    bz.lo[0] = il - 1; bz.n[0] = (iu + 1) - (il - 1) + 1;
-    bz.lo[1] = jl - 1; bz.n[1] = (ju + 1) - (jl - 1) + 1;
-    bz.lo[2] = kl; bz.n[2] = ku + 1 - kl + 1;
-    Launch t(c, ORDER ? "flux_z_plm" : "flux_z_dc");
-    if (c->flux_chunk == 1)
-      hipLaunchKernelGGL((k_flux<2, ORDER, RS, true>), dim3(tiled_blocks(bz, 1, 2)), dim3(BLK), 0,
-                         c->stream, g, w, bf, c->fz, bz);
-    else if (c->flux_chunk > 1)
-      hipLaunchKernelGGL((k_flux_march<2, ORDER, RS>),
-                         dim3(march_blocks(bz.n[0], bz.n[1], bz.n[2], c->flux_chunk)), dim3(BLK),
-                         0, c->stream, g, w, bf, c->fz, bz, c->flux_chunk);
-    else
-      hipLaunchKernelGGL((k_flux<2, ORDER, RS, false>), dim3(blocks_for(bz.total())), dim3(BLK), 0,
-                         c->stream, g, w, bf, c->fz, bz);
-  }
+  // all face directions in one launch over the union of the face ranges
+  const Box u = flux3_box(c->dims, L);
+  const Shape sh = work_shape(u, c->dims, c->flux_chunk);
+  Launch t(c, ORDER ? "flux3_plm" : "flux3_dc");
+  if (sh.tiled)
+    hipLaunchKernelGGL((k_flux3<ORDER, RS, true>), dim3(sh.blocks), dim3(BLK), 0, c->stream, c->g,
+                       w, bf, c->fx, c->fy, c->fz, u, L, 0);
+  else
+    hipLaunchKernelGGL((k_flux3<ORDER, RS, false>), dim3(sh.blocks), dim3(BLK), 0, c->stream,
+                       c->g, w, bf, c->fx, c->fy, c->fz, u, L, sh.ty);
 }
 
 void launch_flux_dispatch(bnz_ctx *c, int order, const double *w, const double *bf,
@@ -697,17 +385,15 @@ void launch_emf(bnz_ctx *c, const double *w, const Lims &L, const char *name) {
   e.lo[0] = L.lo[0]; e.n[0] = L.hi[0] + 1 - L.lo[0] + 1;
   e.lo[1] = (c->dims >= 2) ? L.lo[1] : 0; e.n[1] = (c->dims >= 2) ? L.hi[1] + 1 - L.lo[1] + 1 : 1;
   e.lo[2] = (c->dims >= 3) ? L.lo[2] : 0; e.n[2] = (c->dims >= 3) ? L.hi[2] + 1 - L.lo[2] + 1 : 1;
+  const Shape sh = work_shape(e, c->dims, c->emf_chunk);
   Launch t(c, name);
-  if (c->dims >= 3 && c->emf_chunk == 1)
-    hipLaunchKernelGGL(k_emf<true>, dim3(tiled_blocks(e, 1, 2)), dim3(BLK), 0, c->stream, c->g,
-                       w, c->fx, c->fy, c->fz, c->ee, e);
-  else if (c->dims >= 3 && c->emf_chunk > 1)
-    hipLaunchKernelGGL(k_emf_march, dim3(march_blocks(e.n[0], e.n[1], e.n[2], c->emf_chunk)),
-                       dim3(BLK), 0, c->stream, c->g, w, c->fx, c->fy, c->fz, c->ee, e,
-                       c->emf_chunk);
-  else
-    hipLaunchKernelGGL(k_emf<false>, dim3(blocks_for(e.total())), dim3(BLK), 0, c->stream, c->g,
-                       w, c->fx, c->fy, c->fz, c->ee, e);
+#define EMF_ARGS                                                                              \
+  dim3(sh.blocks), dim3(BLK), 0, c->stream, c->g, w, c->fx, c->fy, c->fz, c->ee, e, sh.ty
+  if (sh.tiled) hipLaunchKernelGGL((k_emf<3, true>), EMF_ARGS);
+  else if (c->dims == 3) hipLaunchKernelGGL((k_emf<3, false>), EMF_ARGS);
+  else if (c->dims == 2) hipLaunchKernelGGL((k_emf<2, false>), EMF_ARGS);
+  else hipLaunchKernelGGL((k_emf<1, false>), EMF_ARGS);
+#undef EMF_ARGS
 }
 
 void launch_ghost(bnz_ctx *c) {
@@ -740,41 +426,27 @@ void launch_update(bnz_ctx *c, bool pred, const Lims &L) {
   }
   const Box act = active_box(c);
   const int fr = c->p.dt_fms_ref;
+  const Shape sh = work_shape(grid, c->dims, c->update_chunk);
   Launch t(c, pred ? "update_pred" : "update_corr");
-  int nb;
-  if (c->dims >= 3 && c->update_chunk > 1) {
-    nb = march_blocks(c->nt[0], c->nt[1], c->nt[2], c->update_chunk);
-    if (pred)
-      hipLaunchKernelGGL((k_update_march<true, false>), dim3(nb), dim3(BLK), 0, c->stream, c->g,
-                         c->w, c->fx, c->fy, c->fz, c->ee, c->b, c->bs, w1, L, c->dev,
-                         c->update_chunk, act, fr, c->part);
-    else
-      hipLaunchKernelGGL((k_update_march<false, true>), dim3(nb), dim3(BLK), 0, c->stream, c->g,
-                         c->w, c->fx, c->fy, c->fz, c->ee, c->b, c->bs, w1, L, c->dev,
-                         c->update_chunk, act, fr, c->part);
-  } else if (c->dims >= 3 && c->update_chunk == 1) {
-    nb = tiled_blocks(grid, 1, 2);
-    if (pred)
-      hipLaunchKernelGGL((k_update<true, false, true>), dim3(nb), dim3(BLK), 0, c->stream, c->g,
-                         c->w, c->fx, c->fy, c->fz, c->ee, c->b, c->bs, w1, L, c->dev, grid, act,
-                         fr, c->part);
-    else
-      hipLaunchKernelGGL((k_update<false, true, true>), dim3(nb), dim3(BLK), 0, c->stream, c->g,
-                         c->w, c->fx, c->fy, c->fz, c->ee, c->b, c->bs, w1, L, c->dev, grid, act,
-                         fr, c->part);
+#define UPD_ARGS                                                                              \
+  dim3(sh.blocks), dim3(BLK), 0, c->stream, c->g, c->w, c->fx, c->fy, c->fz, c->ee, c->b, c->bs, \
+      w1, L, c->dev, grid, act, fr, c->part, sh.ty
+  if (sh.tiled) {  // 3D only
+    if (pred) hipLaunchKernelGGL((k_update<3, true, false, true>), UPD_ARGS);
+    else hipLaunchKernelGGL((k_update<3, false, true, true>), UPD_ARGS);
+  } else if (c->dims == 3) {
+    if (pred) hipLaunchKernelGGL((k_update<3, true, false, false>), UPD_ARGS);
+    else hipLaunchKernelGGL((k_update<3, false, true, false>), UPD_ARGS);
+  } else if (c->dims == 2) {
+    if (pred) hipLaunchKernelGGL((k_update<2, true, false, false>), UPD_ARGS);
+    else hipLaunchKernelGGL((k_update<2, false, true, false>), UPD_ARGS);
   } else {
-    nb = blocks_for(c->ncell);
-    if (pred)
-      hipLaunchKernelGGL((k_update<true, false, false>), dim3(nb), dim3(BLK), 0, c->stream, c->g,
-                         c->w, c->fx, c->fy, c->fz, c->ee, c->b, c->bs, w1, L, c->dev, grid, act,
-                         fr, c->part);
-    else
-      hipLaunchKernelGGL((k_update<false, true, false>), dim3(nb), dim3(BLK), 0, c->stream, c->g,
-                         c->w, c->fx, c->fy, c->fz, c->ee, c->b, c->bs, w1, L, c->dev, grid, act,
-                         fr, c->part);
+    if (pred) hipLaunchKernelGGL((k_update<1, true, false, false>), UPD_ARGS);
+    else hipLaunchKernelGGL((k_update<1, false, true, false>), UPD_ARGS);
   }
-  // the corrector leaves the next step's CFL partials in part[0..nb)
-  c->dt_parts = pred ? 0 : nb;
+#undef UPD_ARGS
+  // the corrector leaves the next step's CFL partials in part[0..blocks)
+  c->dt_parts = pred ? 0 : sh.blocks;
 }
 
 // One VL step after the dt kernels (vanleer.pyx:100-163).
@@ -947,8 +619,7 @@ int bnz_create(const bnz_params *p, bnz_ctx **out) {
       grid.n[q] = c->nt[q];
     }
     c->npart_cap = std::max({(long)blocks_for(a.total()) * NDIAG + 64, (long)blocks_for(c->ncell),
-                             (long)tiled_blocks(grid, 1, 2),
-                             (long)march_blocks(c->nt[0], c->nt[1], c->nt[2], 1)});
+                             (long)tiled_blocks(grid, 1, 2)});
   }
   HIPCHK(c, hipMalloc(&c->part, c->npart_cap * sizeof(double)));
 
@@ -1182,9 +853,14 @@ int bnz_set_tuning(bnz_ctx *c, int flux_chunk, int emf_chunk, int update_chunk) 
   c->flux_chunk = flux_chunk;
   c->emf_chunk = emf_chunk;
   c->update_chunk = update_chunk;
-  // partial count of a fused CFL reduction depends on the update chunk
+  // partial count of the fused CFL reduction depends on the update shape
   c->dt_parts = 0;
-  const long need = (long)march_blocks(c->nt[0], c->nt[1], c->nt[2], std::max(update_chunk, 1));
+  Box grid;
+  for (int q = 0; q < 3; ++q) {
+    grid.lo[q] = 0;
+    grid.n[q] = c->nt[q];
+  }
+  const long need = work_shape(grid, c->dims, update_chunk).blocks;
   if (need > c->npart_cap) {
     double *np = nullptr;
     HIPCHK(c, hipStreamSynchronize(c->stream));

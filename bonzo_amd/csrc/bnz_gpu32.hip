@@ -28,13 +28,12 @@ __device__ __forceinline__ void split(double v, float &hi, float &lo) {
 }
 __device__ __forceinline__ double join(float hi, float lo) { return (double)hi + (double)lo; }
 
-// Godunov fluxes, FP32 (same stencil and ranges as k_flux).
+// Godunov flux of one face, FP32 (same stencil as the FP64 flux_face); ng
+// counts guarded HLLD star denominators.
 template <int DIR, int ORDER, int RS, typename BF>
-__global__ __launch_bounds__(BLK) void k32_flux(Geo g, const float *__restrict__ w,
-                                                const BF *__restrict__ bf,
-                                                float *__restrict__ fl, Box box, int *nguard) {
-  int i, j, k;
-  if (!cell_index<false, 1, 2>(box, i, j, k)) return;
+__device__ __forceinline__ void flux32_face(const Geo &g, const float *__restrict__ w,
+                                            const BF *__restrict__ bf, float *__restrict__ fl,
+                                            int i, int j, int k, int &ng) {
   const long c = ((long)k * g.nt[1] + j) * g.nt[0] + i;
   const long sd = (DIR == 0) ? 1 : (DIR == 1) ? g.sy : g.sz;
   const float bn = (float)bf[(long)DIR * g.sv + c];
@@ -55,17 +54,32 @@ __global__ __launch_bounds__(BLK) void k32_flux(Geo g, const float *__restrict__
   wl[BX] = bn;
   wr[BX] = bn;
   float f[NMODE];
-  int ng = 0;
   const float gam = (float)g.gam, gm1i = (float)g.gamm1i;
   const int reg = (RS == BNZ_RS_HLLD) ? flux_hlld(f, wl, wr, gam, gm1i, &ng)
                                       : flux_hll(f, wl, wr, gam, gm1i);
-  if (ng) atomicAdd(nguard, ng);
   if (reg == 0) return;
 #pragma unroll
   for (int n = 0; n < NMODE; ++n) {
     if (n == BX) continue;
     fl[(long)perm<DIR>(n) * g.sv + c] = f[n];
   }
+}
+
+// All face directions of a stage in one launch, one thread per cell of the
+// union of the face ranges (as the FP64 k_flux3).
+template <int ORDER, int RS, typename BF>
+__global__ __launch_bounds__(BLK) void k32_flux3(Geo g, const float *__restrict__ w,
+                                                 const BF *__restrict__ bf,
+                                                 float *__restrict__ fx, float *__restrict__ fy,
+                                                 float *__restrict__ fz, Box box, Lims L,
+                                                 int *nguard) {
+  int i, j, k;
+  if (!cell_index<false, 1, 2>(box, 0, i, j, k)) return;
+  int ng = 0;
+  if (i >= L.lo[0]) flux32_face<0, ORDER, RS>(g, w, bf, fx, i, j, k, ng);
+  if (g.dims >= 2 && j >= L.lo[1]) flux32_face<1, ORDER, RS>(g, w, bf, fy, i, j, k, ng);
+  if (g.dims >= 3 && k >= L.lo[2]) flux32_face<2, ORDER, RS>(g, w, bf, fz, i, j, k, ng);
+  if (ng) atomicAdd(nguard, ng);
 }
 
 __device__ __forceinline__ float ec32(const float *w, long sv, long c, int comp) {
@@ -76,24 +90,26 @@ __device__ __forceinline__ float ec32(const float *w, long sv, long c, int comp)
   return -(vx * by - vy * bx);
 }
 
-// Edge EMFs, FP32 (interp_ee1, src/integration/ct.pyx:153-215).
+// Edge EMFs, FP32 (interp_ee1, src/integration/ct.pyx:153-215), for a grid
+// of DIMS dimensions.
+template <int DIMS>
 __global__ __launch_bounds__(BLK) void k32_emf(Geo g, const float *__restrict__ w,
                                                const float *__restrict__ fx,
                                                const float *__restrict__ fy,
                                                const float *__restrict__ fz,
                                                float *__restrict__ ee, Box box) {
   int i, j, k;
-  if (!cell_index<false, 1, 2>(box, i, j, k)) return;
+  if (!cell_index<false, 1, 2>(box, 0, i, j, k)) return;
   const long sv = g.sv, sy = g.sy, sz = g.sz;
   const long c = ((long)k * g.nt[1] + j) * g.nt[0] + i;
-  if (g.dims == 3) {
+  if (DIMS == 3) {
     ee[c] = 0.5f * (-fy[BZ * sv + c - sz] - fy[BZ * sv + c] + fz[BY * sv + c - sy] + fz[BY * sv + c]) -
             0.25f * (ec32(w, sv, c - sy, 0) + ec32(w, sv, c, 0) + ec32(w, sv, c - sz - sy, 0) +
                      ec32(w, sv, c - sz, 0));
     ee[sv + c] = 0.5f * (-fz[BX * sv + c - 1] - fz[BX * sv + c] + fx[BZ * sv + c - sz] + fx[BZ * sv + c]) -
                  0.25f * (ec32(w, sv, c - 1, 1) + ec32(w, sv, c, 1) + ec32(w, sv, c - sz - 1, 1) +
                           ec32(w, sv, c - sz, 1));
-  } else if (g.dims == 2) {
+  } else if (DIMS == 2) {
     ee[c] = -fy[BZ * sv + c];
     ee[sv + c] = fx[BZ * sv + c];
   } else {
@@ -107,50 +123,83 @@ __global__ __launch_bounds__(BLK) void k32_emf(Geo g, const float *__restrict__ 
                             ec32(w, sv, c - sy, 2));
 }
 
-// Face-field update by Stokes (ct.pyx:15-91) of one face in working type R
-// from face values b (R) and FP32 EMFs.
-template <typename R, typename BF>
-__device__ __forceinline__ R face_x(const Geo &g, BF b, const float *ee, double dt, int i, int j,
-                                    int k, long c) {
+// Face-field update by Stokes (ct.pyx:15-91) in working type R from face
+// values b and FP32 EMFs: the lower faces of cell c (lo) and its upper faces
+// (up).  Neighbour offsets ox, oy, oz are zero where the cell needs no
+// neighbour (all loads stay in bounds; see the FP64 update_cell_d).
+template <int DIMS, typename R, typename BF>
+__device__ __forceinline__ void faces_d(const Geo &g, BF b, const float *__restrict__ ee,
+                                        double dt, int i, int j, int k, long c, long ox, long oy,
+                                        long oz, R *lo, R *up) {
   const long sv = g.sv;
-  const R dtda = (R)(dt / (g.dlf[1][j] * g.dlf[2][k]));
-  R v = b(0, c);
-  if (g.dims >= 2) {
-    const R ds = (R)g.dlf[2][k];
-    v -= dtda * (ds * (R)ee[2 * sv + c + g.sy] - ds * (R)ee[2 * sv + c]);
+  const float *ex = ee, *ey = ee + sv, *ez = ee + 2 * sv;
+  const R ez00 = ez[c], ez10 = ez[c + ox], ez01 = ez[c + oy], ez11 = ez[c + ox + oy];
+  const R ey00 = ey[c], ey10 = ey[c + ox], ey01 = ey[c + oz], ey11 = ey[c + ox + oz];
+  const R ex00 = ex[c], ex10 = ex[c + oy], ex01 = ex[c + oz], ex11 = ex[c + oy + oz];
+  const R bxc = b(0, c), byc = b(1, c), bzc = b(2, c);
+  const R bxp = b(0, c + ox), byp = b(1, c + oy), bzp = b(2, c + oz);
+  const double dlx = g.dlf[0][i], dly = g.dlf[1][j], dlz = g.dlf[2][k];
+  {
+    const R dtda = (R)(dt / (dly * dlz));
+    R v = bxc, vu = bxp;
+    if (DIMS >= 2) {
+      const R ds = (R)dlz;
+      v -= dtda * (ds * ez01 - ds * ez00);
+      vu -= dtda * (ds * ez11 - ds * ez10);
+    }
+    if (DIMS >= 3) {
+      const R ds = (R)dly;
+      v += dtda * (ds * ey01 - ds * ey00);
+      vu += dtda * (ds * ey11 - ds * ey10);
+    }
+    lo[0] = v;
+    up[0] = vu;
   }
-  if (g.dims >= 3) {
-    const R ds = (R)g.dlf[1][j];
-    v += dtda * (ds * (R)ee[sv + c + g.sz] - ds * (R)ee[sv + c]);
+  {
+    const R dtda = (R)(dt / (dlx * dlz));
+    R ds = (R)dlz;
+    R v = byc + dtda * (ds * ez10 - ds * ez00);
+    R vu = byp + dtda * (ds * ez11 - ds * ez01);
+    if (DIMS >= 3) {
+      ds = (R)dlx;
+      v -= dtda * (ds * ex01 - ds * ex00);
+      vu -= dtda * (ds * ex11 - ds * ex10);
+    }
+    lo[1] = v;
+    up[1] = vu;
   }
-  return v;
+  {
+    const R dtda = (R)(dt / (dlx * dly));
+    R ds = (R)dly;
+    R v = bzc - dtda * (ds * ey10 - ds * ey00);
+    R vu = bzp - dtda * (ds * ey11 - ds * ey01);
+    if (DIMS >= 2) {
+      ds = (R)dlx;
+      v += dtda * (ds * ex10 - ds * ex00);
+      vu += dtda * (ds * ex11 - ds * ex01);
+    }
+    lo[2] = v;
+    up[2] = vu;
+  }
 }
-template <typename R, typename BF>
-__device__ __forceinline__ R face_y(const Geo &g, BF b, const float *ee, double dt, int i, int j,
-                                    int k, long c) {
-  const long sv = g.sv;
-  const R dtda = (R)(dt / (g.dlf[0][i] * g.dlf[2][k]));
-  R ds = (R)g.dlf[2][k];
-  R v = b(1, c) + dtda * (ds * (R)ee[2 * sv + c + 1] - ds * (R)ee[2 * sv + c]);
-  if (g.dims >= 3) {
-    ds = (R)g.dlf[0][i];
-    v -= dtda * (ds * (R)ee[c + g.sz] - ds * (R)ee[c]);
+
+// Cell roles of the stage update (as update_cell_d) and neighbour offsets.
+template <int DIMS>
+struct Roles {
+  bool cell, fX, fY, fZ;
+  long ox, oy, oz;
+  __device__ Roles(const Geo &g, const Lims &L, int i, int j, int k) {
+    const bool inI = i >= L.lo[0] && i <= L.hi[0], inJ = j >= L.lo[1] && j <= L.hi[1];
+    const bool inK = k >= L.lo[2] && k <= L.hi[2];
+    cell = inI && inJ && inK;
+    fX = (i >= L.lo[0] && i <= L.hi[0] + 1) && inJ && inK;
+    fY = inI && (j >= L.lo[1] && j <= L.hi[1] + (DIMS >= 2 ? 1 : 0)) && inK;
+    fZ = inI && inJ && (k >= L.lo[2] && k <= L.hi[2] + (DIMS >= 3 ? 1 : 0));
+    ox = (cell || fY || fZ) ? 1 : 0;
+    oy = (DIMS >= 2 && (cell || fX || fZ)) ? g.sy : 0;
+    oz = (DIMS >= 3 && (cell || fX || fY)) ? g.sz : 0;
   }
-  return v;
-}
-template <typename R, typename BF>
-__device__ __forceinline__ R face_z(const Geo &g, BF b, const float *ee, double dt, int i, int j,
-                                    int k, long c) {
-  const long sv = g.sv;
-  const R dtda = (R)(dt / (g.dlf[0][i] * g.dlf[1][j]));
-  R ds = (R)g.dlf[1][j];
-  R v = b(2, c) - dtda * (ds * (R)ee[sv + c + 1] - ds * (R)ee[sv + c]);
-  if (g.dims >= 2) {
-    ds = (R)g.dlf[0][i];
-    v += dtda * (ds * (R)ee[c + g.sy] - ds * (R)ee[c]);
-  }
-  return v;
-}
+};
 
 struct Faces32 {
   const double *b;
@@ -171,6 +220,8 @@ struct Pred32 {
 };
 
 // Predictor (dt/2): FP32 update of the hi words, FP64 pressure recovery.
+// All operands are loaded up front; only the stores depend on the roles.
+template <int DIMS>
 __global__ __launch_bounds__(BLK) void k32_update_pred(Geo g, Pred32 a, Lims L,
                                                        const double *__restrict__ dts) {
   const long t = xcd_block() * BLK + threadIdx.x;
@@ -181,50 +232,58 @@ __global__ __launch_bounds__(BLK) void k32_update_pred(Geo g, Pred32 a, Lims L,
   const int j = (int)(r % g.nt[1]), k = (int)(r / g.nt[1]);
   const long c = t, sv = g.sv;
   const double dt = 0.5 * dts[0];
-  const bool inI = i >= L.lo[0] && i <= L.hi[0], inJ = j >= L.lo[1] && j <= L.hi[1];
-  const bool inK = k >= L.lo[2] && k <= L.hi[2];
-  const bool cell = inI && inJ && inK;
-  const bool fX = (i >= L.lo[0] && i <= L.hi[0] + 1) && inJ && inK;
-  const bool fY = inI && (j >= L.lo[1] && j <= L.hi[1] + (g.dims >= 2 ? 1 : 0)) && inK;
-  const bool fZ = inI && inJ && (k >= L.lo[2] && k <= L.hi[2] + (g.dims >= 3 ? 1 : 0));
-  const Faces32 B{a.b, sv};
-  float bxl = 0.f, byl = 0.f, bzl = 0.f;
-  if (fX) a.bs[c] = bxl = face_x<float>(g, B, a.ee, dt, i, j, k, c);
-  if (fY) a.bs[sv + c] = byl = face_y<float>(g, B, a.ee, dt, i, j, k, c);
-  if (fZ) a.bs[2 * sv + c] = bzl = face_z<float>(g, B, a.ee, dt, i, j, k, c);
-  if (!cell) {
+  const Roles<DIMS> ro(g, L, i, j, k);
+  float w0[NMODE], uh[NMODE - 3], fxl[NMODE - 3], fxu[NMODE - 3], fyl[NMODE - 3],
+      fyu[NMODE - 3], fzl[NMODE - 3], fzu[NMODE - 3];
 #pragma unroll
-    for (int n = 0; n < NMODE; ++n) a.ws[n * sv + c] = a.w32[n * sv + c];
-    return;
+  for (int n = 0; n < NMODE; ++n) w0[n] = a.w32[n * sv + c];
+#pragma unroll
+  for (int n = 0; n < NMODE - 3; ++n) {
+    uh[n] = a.uh[n * sv + c];
+    fxl[n] = a.fx[n * sv + c];
+    fxu[n] = a.fx[n * sv + c + ro.ox];
+    if (DIMS >= 2) {
+      fyl[n] = a.fy[n * sv + c];
+      fyu[n] = a.fy[n * sv + c + ro.oy];
+    }
+    if (DIMS >= 3) {
+      fzl[n] = a.fz[n * sv + c];
+      fzu[n] = a.fz[n * sv + c + ro.oz];
+    }
   }
+  float lo[3], up[3];
+  faces_d<DIMS, float>(g, Faces32{a.b, sv}, a.ee, dt, i, j, k, c, ro.ox, ro.oy, ro.oz, lo, up);
   const float dlx = (float)g.dlf[0][i], dly = (float)g.dlf[1][j], dlz = (float)g.dlf[2][k];
   const float dtdv = (float)(dt / (g.dlf[0][i] * g.dlf[1][j] * g.dlf[2][k]));
   const float ax = dly * dlz, ay = dlx * dlz, az = dlx * dly;
   double u[NMODE], wv[NMODE];
 #pragma unroll
   for (int n = 0; n < NMODE - 3; ++n) {
-    float v = a.uh[n * sv + c] - dtdv * (ax * a.fx[n * sv + c + 1] - ax * a.fx[n * sv + c]);
-    if (g.dims >= 2) v -= dtdv * (ay * a.fy[n * sv + c + g.sy] - ay * a.fy[n * sv + c]);
-    if (g.dims >= 3) v -= dtdv * (az * a.fz[n * sv + c + g.sz] - az * a.fz[n * sv + c]);
+    float v = uh[n] - dtdv * (ax * fxu[n] - ax * fxl[n]);
+    if (DIMS >= 2) v -= dtdv * (ay * fyu[n] - ay * fyl[n]);
+    if (DIMS >= 3) v -= dtdv * (az * fzu[n] - az * fzl[n]);
     u[n] = v;
   }
-  const float bxu = face_x<float>(g, B, a.ee, dt, i + 1, j, k, c + 1);
-  u[BX] = ((float)g.icm[0][i] * bxl + (float)g.icp[0][i] * bxu) * (float)g.dlfi[0][i];
-  if (g.dims >= 2) {
-    const float byu = face_y<float>(g, B, a.ee, dt, i, j + 1, k, c + g.sy);
-    u[BY] = ((float)g.icm[1][j] * byl + (float)g.icp[1][j] * byu) * (float)g.dlfi[1][j];
-  } else {
-    u[BY] = byl;
-  }
-  if (g.dims >= 3) {
-    const float bzu = face_z<float>(g, B, a.ee, dt, i, j, k + 1, c + g.sz);
-    u[BZ] = ((float)g.icm[2][k] * bzl + (float)g.icp[2][k] * bzu) * (float)g.dlfi[2][k];
-  } else {
-    u[BZ] = bzl;
-  }
+  u[BX] = ((float)g.icm[0][i] * lo[0] + (float)g.icp[0][i] * up[0]) * (float)g.dlfi[0][i];
+  if (DIMS >= 2)
+    u[BY] = ((float)g.icm[1][j] * lo[1] + (float)g.icp[1][j] * up[1]) * (float)g.dlfi[1][j];
+  else
+    u[BY] = lo[1];
+  if (DIMS >= 3)
+    u[BZ] = ((float)g.icm[2][k] * lo[2] + (float)g.icp[2][k] * up[2]) * (float)g.dlfi[2][k];
+  else
+    u[BZ] = lo[2];
   cons2prim(wv, u, g.gam);
+  if (ro.fX) a.bs[c] = lo[0];
+  if (ro.fY) a.bs[sv + c] = lo[1];
+  if (ro.fZ) a.bs[2 * sv + c] = lo[2];
+  if (ro.cell) {
 #pragma unroll
-  for (int n = 0; n < NMODE; ++n) a.ws[n * sv + c] = (float)wv[n];
+    for (int n = 0; n < NMODE; ++n) a.ws[n * sv + c] = (float)wv[n];
+  } else {
+#pragma unroll
+    for (int n = 0; n < NMODE; ++n) a.ws[n * sv + c] = w0[n];
+  }
 }
 
 struct Corr32 {
@@ -236,7 +295,9 @@ struct Corr32 {
 };
 
 // Corrector (dt): FP64 update of the hi + lo state, FP64 pressure recovery,
-// fused CFL maximum of the new state.
+// fused CFL maximum of the new state.  Operands loaded up front as in the
+// predictor.
+template <int DIMS>
 __global__ __launch_bounds__(BLK) void k32_update_corr(Geo g, Corr32 a, Lims L,
                                                        const double *__restrict__ dts, Box act,
                                                        int fms_ref, double *__restrict__ part) {
@@ -249,48 +310,59 @@ __global__ __launch_bounds__(BLK) void k32_update_corr(Geo g, Corr32 a, Lims L,
     const int j = (int)(r % g.nt[1]), k = (int)(r / g.nt[1]);
     const long c = t, sv = g.sv;
     const double dt = dts[0];
-    const bool inI = i >= L.lo[0] && i <= L.hi[0], inJ = j >= L.lo[1] && j <= L.hi[1];
-    const bool inK = k >= L.lo[2] && k <= L.hi[2];
-    const bool cell = inI && inJ && inK;
-    const bool fX = (i >= L.lo[0] && i <= L.hi[0] + 1) && inJ && inK;
-    const bool fY = inI && (j >= L.lo[1] && j <= L.hi[1] + (g.dims >= 2 ? 1 : 0)) && inK;
-    const bool fZ = inI && inJ && (k >= L.lo[2] && k <= L.hi[2] + (g.dims >= 3 ? 1 : 0));
-    const Faces64 B{a.b, sv};
-    double bxl = 0., byl = 0., bzl = 0.;
-    if (fX) a.b1[c] = bxl = face_x<double>(g, B, a.ee, dt, i, j, k, c);
-    if (fY) a.b1[sv + c] = byl = face_y<double>(g, B, a.ee, dt, i, j, k, c);
-    if (fZ) a.b1[2 * sv + c] = bzl = face_z<double>(g, B, a.ee, dt, i, j, k, c);
-    if (cell) {
-      const double dlx = g.dlf[0][i], dly = g.dlf[1][j], dlz = g.dlf[2][k];
-      const double dtdv = dt / (dlx * dly * dlz);
-      const double ax = dly * dlz, ay = dlx * dlz, az = dlx * dly;
-      double u[NMODE], wv[NMODE];
+    const Roles<DIMS> ro(g, L, i, j, k);
+    float uh[NMODE - 3], ul[NMODE - 3], fxl[NMODE - 3], fxu[NMODE - 3], fyl[NMODE - 3],
+        fyu[NMODE - 3], fzl[NMODE - 3], fzu[NMODE - 3];
+#pragma unroll
+    for (int n = 0; n < NMODE - 3; ++n) {
+      uh[n] = a.uh[n * sv + c];
+      ul[n] = a.ul[n * sv + c];
+      fxl[n] = a.fx[n * sv + c];
+      fxu[n] = a.fx[n * sv + c + ro.ox];
+      if (DIMS >= 2) {
+        fyl[n] = a.fy[n * sv + c];
+        fyu[n] = a.fy[n * sv + c + ro.oy];
+      }
+      if (DIMS >= 3) {
+        fzl[n] = a.fz[n * sv + c];
+        fzu[n] = a.fz[n * sv + c + ro.oz];
+      }
+    }
+    double lo[3], up[3];
+    faces_d<DIMS, double>(g, Faces64{a.b, sv}, a.ee, dt, i, j, k, c, ro.ox, ro.oy, ro.oz, lo,
+                          up);
+    const double dlx = g.dlf[0][i], dly = g.dlf[1][j], dlz = g.dlf[2][k];
+    const double dtdv = dt / (dlx * dly * dlz);
+    const double ax = dly * dlz, ay = dlx * dlz, az = dlx * dly;
+    double u[NMODE], wv[NMODE];
+    float nh[NMODE - 3], nl[NMODE - 3];
+#pragma unroll
+    for (int n = 0; n < NMODE - 3; ++n) {
+      double v = join(uh[n], ul[n]) - dtdv * (ax * (double)fxu[n] - ax * (double)fxl[n]);
+      if (DIMS >= 2) v -= dtdv * (ay * (double)fyu[n] - ay * (double)fyl[n]);
+      if (DIMS >= 3) v -= dtdv * (az * (double)fzu[n] - az * (double)fzl[n]);
+      u[n] = v;
+      split(v, nh[n], nl[n]);
+    }
+    u[BX] = (g.icm[0][i] * lo[0] + g.icp[0][i] * up[0]) * g.dlfi[0][i];
+    if (DIMS >= 2)
+      u[BY] = (g.icm[1][j] * lo[1] + g.icp[1][j] * up[1]) * g.dlfi[1][j];
+    else
+      u[BY] = lo[1];
+    if (DIMS >= 3)
+      u[BZ] = (g.icm[2][k] * lo[2] + g.icp[2][k] * up[2]) * g.dlfi[2][k];
+    else
+      u[BZ] = lo[2];
+    cons2prim(wv, u, g.gam);
+    if (ro.fX) a.b1[c] = lo[0];
+    if (ro.fY) a.b1[sv + c] = lo[1];
+    if (ro.fZ) a.b1[2 * sv + c] = lo[2];
+    if (ro.cell) {
 #pragma unroll
       for (int n = 0; n < NMODE - 3; ++n) {
-        double v = join(a.uh[n * sv + c], a.ul[n * sv + c]) -
-                   dtdv * (ax * (double)a.fx[n * sv + c + 1] - ax * (double)a.fx[n * sv + c]);
-        if (g.dims >= 2)
-          v -= dtdv * (ay * (double)a.fy[n * sv + c + g.sy] - ay * (double)a.fy[n * sv + c]);
-        if (g.dims >= 3)
-          v -= dtdv * (az * (double)a.fz[n * sv + c + g.sz] - az * (double)a.fz[n * sv + c]);
-        u[n] = v;
-        split(v, a.uh[n * sv + c], a.ul[n * sv + c]);
+        a.uh[n * sv + c] = nh[n];
+        a.ul[n * sv + c] = nl[n];
       }
-      const double bxu = face_x<double>(g, B, a.ee, dt, i + 1, j, k, c + 1);
-      u[BX] = (g.icm[0][i] * bxl + g.icp[0][i] * bxu) * g.dlfi[0][i];
-      if (g.dims >= 2) {
-        const double byu = face_y<double>(g, B, a.ee, dt, i, j + 1, k, c + g.sy);
-        u[BY] = (g.icm[1][j] * byl + g.icp[1][j] * byu) * g.dlfi[1][j];
-      } else {
-        u[BY] = byl;
-      }
-      if (g.dims >= 3) {
-        const double bzu = face_z<double>(g, B, a.ee, dt, i, j, k + 1, c + g.sz);
-        u[BZ] = (g.icm[2][k] * bzl + g.icp[2][k] * bzu) * g.dlfi[2][k];
-      } else {
-        u[BZ] = bzl;
-      }
-      cons2prim(wv, u, g.gam);
 #pragma unroll
       for (int n = 0; n < NMODE; ++n) a.w32[n * sv + c] = (float)wv[n];
       if (i >= act.lo[0] && i < act.lo[0] + act.n[0] && j >= act.lo[1] &&
@@ -306,7 +378,7 @@ __global__ __launch_bounds__(BLK) void k32_dt_partial(Geo g, const float *__rest
                                                       int fms_ref, double *__restrict__ part) {
   int i, j, k;
   double cds = 0.;
-  if (cell_index<false, 1, 2>(box, i, j, k)) {
+  if (cell_index<false, 1, 2>(box, 0, i, j, k)) {
     const long c = ((long)k * g.nt[1] + j) * g.nt[0] + i;
     double wv[NMODE];
 #pragma unroll
@@ -383,35 +455,11 @@ __global__ __launch_bounds__(BLK) void k32_to64(Geo g, const float *uh, const fl
 
 template <int ORDER, int RS, typename BF>
 void launch_fluxes32(bnz_ctx *c, const float *w, const BF *bf, const Lims &L) {
-  const int il = L.lo[0], iu = L.hi[0], jl = L.lo[1], ju = L.hi[1], kl = L.lo[2], ku = L.hi[2];
   int *ng = (int *)(c->dev + 4);
-  {
-    Box bx;
-    bx.lo[0] = il; bx.n[0] = iu + 1 - il + 1;
-    bx.lo[1] = (c->dims >= 2) ? jl - 1 : 0; bx.n[1] = (c->dims >= 2) ? (ju + 1) - (jl - 1) + 1 : 1;
-    bx.lo[2] = (c->dims >= 3) ? kl - 1 : 0; bx.n[2] = (c->dims >= 3) ? (ku + 1) - (kl - 1) + 1 : 1;
-    Launch t(c, ORDER ? "flux_x_plm" : "flux_x_dc");
-    hipLaunchKernelGGL((k32_flux<0, ORDER, RS, BF>), dim3(blocks_for(bx.total())), dim3(BLK), 0,
-                       c->stream, c->g, w, bf, c->fx32, bx, ng);
-  }
-  if (c->dims >= 2) {
-    Box by;
-    by.lo[0] = il - 1; by.n[0] = (iu + 1) - (il - 1) + 1;
-    by.lo[1] = jl; by.n[1] = ju + 1 - jl + 1;
-    by.lo[2] = (c->dims >= 3) ? kl - 1 : 0; by.n[2] = (c->dims >= 3) ? (ku + 1) - (kl - 1) + 1 : 1;
-    Launch t(c, ORDER ? "flux_y_plm" : "flux_y_dc");
-    hipLaunchKernelGGL((k32_flux<1, ORDER, RS, BF>), dim3(blocks_for(by.total())), dim3(BLK), 0,
-                       c->stream, c->g, w, bf, c->fy32, by, ng);
-  }
-  if (c->dims >= 3) {
-    Box bz;
-    bz.lo[0] = il - 1; bz.n[0] = (iu + 1) - (il - 1) + 1;
-    bz.lo[1] = jl - 1; bz.n[1] = (ju + 1) - (jl - 1) + 1;
-    bz.lo[2] = kl; bz.n[2] = ku + 1 - kl + 1;
-    Launch t(c, ORDER ? "flux_z_plm" : "flux_z_dc");
-    hipLaunchKernelGGL((k32_flux<2, ORDER, RS, BF>), dim3(blocks_for(bz.total())), dim3(BLK), 0,
-                       c->stream, c->g, w, bf, c->fz32, bz, ng);
-  }
+  const Box u = flux3_box(c->dims, L);
+  Launch t(c, ORDER ? "flux3_plm" : "flux3_dc");
+  hipLaunchKernelGGL((k32_flux3<ORDER, RS, BF>), dim3(blocks_for(u.total())), dim3(BLK), 0,
+                     c->stream, c->g, w, bf, c->fx32, c->fy32, c->fz32, u, L, ng);
 }
 
 template <typename BF>
@@ -432,8 +480,12 @@ void emf32(bnz_ctx *c, const float *w, const Lims &L, const char *name) {
   e.lo[1] = (c->dims >= 2) ? L.lo[1] : 0; e.n[1] = (c->dims >= 2) ? L.hi[1] + 1 - L.lo[1] + 1 : 1;
   e.lo[2] = (c->dims >= 3) ? L.lo[2] : 0; e.n[2] = (c->dims >= 3) ? L.hi[2] + 1 - L.lo[2] + 1 : 1;
   Launch t(c, name);
-  hipLaunchKernelGGL(k32_emf, dim3(blocks_for(e.total())), dim3(BLK), 0, c->stream, c->g, w,
-                     c->fx32, c->fy32, c->fz32, c->ee32, e);
+#define E32_ARGS \
+  dim3(blocks_for(e.total())), dim3(BLK), 0, c->stream, c->g, w, c->fx32, c->fy32, c->fz32, c->ee32, e
+  if (c->dims == 3) hipLaunchKernelGGL(k32_emf<3>, E32_ARGS);
+  else if (c->dims == 2) hipLaunchKernelGGL(k32_emf<2>, E32_ARGS);
+  else hipLaunchKernelGGL(k32_emf<1>, E32_ARGS);
+#undef E32_ARGS
 }
 
 void ghost32(bnz_ctx *c) {
@@ -509,7 +561,11 @@ void f32_step_body(bnz_ctx *c) {
   {
     Pred32 a{c->w32, c->uh, c->b, c->fx32, c->fy32, c->fz32, c->ee32, c->bs32, c->ws32};
     Launch t(c, "update_pred");
-    hipLaunchKernelGGL(k32_update_pred, dim3(nb), dim3(BLK), 0, c->stream, c->g, a, L0, c->dev);
+#define P32_ARGS dim3(nb), dim3(BLK), 0, c->stream, c->g, a, L0, c->dev
+    if (c->dims == 3) hipLaunchKernelGGL(k32_update_pred<3>, P32_ARGS);
+    else if (c->dims == 2) hipLaunchKernelGGL(k32_update_pred<2>, P32_ARGS);
+    else hipLaunchKernelGGL(k32_update_pred<1>, P32_ARGS);
+#undef P32_ARGS
   }
   Lims L1 = integr_lims(c, c->order);
   fluxes32(c, c->order, c->ws32, c->bs32, L1);
@@ -517,8 +573,12 @@ void f32_step_body(bnz_ctx *c) {
   {
     Corr32 a{c->uh, c->ul, c->b, c->fx32, c->fy32, c->fz32, c->ee32, c->bs, c->w32};
     Launch t(c, "update_corr");
-    hipLaunchKernelGGL(k32_update_corr, dim3(nb), dim3(BLK), 0, c->stream, c->g, a, L1, c->dev,
-                       active_box(c), c->p.dt_fms_ref, c->part);
+#define C32_ARGS \
+  dim3(nb), dim3(BLK), 0, c->stream, c->g, a, L1, c->dev, active_box(c), c->p.dt_fms_ref, c->part
+    if (c->dims == 3) hipLaunchKernelGGL(k32_update_corr<3>, C32_ARGS);
+    else if (c->dims == 2) hipLaunchKernelGGL(k32_update_corr<2>, C32_ARGS);
+    else hipLaunchKernelGGL(k32_update_corr<1>, C32_ARGS);
+#undef C32_ARGS
   }
   c->dt_parts = nb;
   std::swap(c->b, c->bs);

@@ -53,6 +53,21 @@ __device__ __forceinline__ void flux_face(const Geo &g, const double *__restrict
   }
 }
 
+// Every face flux of one stage whose lower cell is (i, j, k): the x, y and
+// z faces of godunov_fluxes' three sweeps (godunov.pyx:20-210), each only
+// where that sweep's range includes it.  Over flux3_box(L) this covers
+// exactly the faces of the three per-direction launches.
+template <int ORDER, int RS>
+__device__ __forceinline__ void flux_cell(const Geo &g, const double *__restrict__ w,
+                                          const double *__restrict__ bf,
+                                          double *__restrict__ fx, double *__restrict__ fy,
+                                          double *__restrict__ fz, const Lims &L, int i, int j,
+                                          int k) {
+  if (i >= L.lo[0]) flux_face<0, ORDER, RS>(g, w, bf, fx, i, j, k);
+  if (g.dims >= 2 && j >= L.lo[1]) flux_face<1, ORDER, RS>(g, w, bf, fy, i, j, k);
+  if (g.dims >= 3 && k >= L.lo[2]) flux_face<2, ORDER, RS>(g, w, bf, fz, i, j, k);
+}
+
 __device__ __forceinline__ double ec_comp(const double *w, long sv, long c, int comp) {
   const double vx = w[VX * sv + c], vy = w[VY * sv + c], vz = w[VZ * sv + c];
   const double bx = w[BX * sv + c], by = w[BY * sv + c], bz = w[BZ * sv + c];
@@ -61,16 +76,18 @@ __device__ __forceinline__ double ec_comp(const double *w, long sv, long c, int 
   return -(vx * by - vy * bx);
 }
 
-// Edge EMFs, interp_ee1 (src/integration/ct.pyx:153-215).
-__device__ __forceinline__ void emf_edge(const Geo &g, const double *__restrict__ w,
-                                         const double *__restrict__ fx,
-                                         const double *__restrict__ fy,
-                                         const double *__restrict__ fz,
-                                         double *__restrict__ ee, int i, int j, int k) {
+// Edge EMFs, interp_ee1 (src/integration/ct.pyx:153-215), for a grid of
+// DIMS dimensions: the three edges whose lower corner is cell (i, j, k).
+template <int DIMS>
+__device__ __forceinline__ void emf_edge_d(const Geo &g, const double *__restrict__ w,
+                                           const double *__restrict__ fx,
+                                           const double *__restrict__ fy,
+                                           const double *__restrict__ fz,
+                                           double *__restrict__ ee, int i, int j, int k) {
   const long sv = g.sv;
   const long c = ((long)k * g.nt[1] + j) * g.nt[0] + i;
   const long sy = g.sy, sz = g.sz;
-  if (g.dims == 3) {
+  if (DIMS == 3) {
     ee[0 * sv + c] = (0.5 * (-fy[BZ * sv + c - sz] - fy[BZ * sv + c] + fz[BY * sv + c - sy] +
                              fz[BY * sv + c]) -
                       0.25 * (ec_comp(w, sv, c - sy, 0) + ec_comp(w, sv, c, 0) +
@@ -83,7 +100,7 @@ __device__ __forceinline__ void emf_edge(const Geo &g, const double *__restrict_
                              fy[BX * sv + c]) -
                       0.25 * (ec_comp(w, sv, c - 1, 2) + ec_comp(w, sv, c, 2) +
                               ec_comp(w, sv, c - sy - 1, 2) + ec_comp(w, sv, c - sy, 2)));
-  } else if (g.dims == 2) {
+  } else if (DIMS == 2) {
     ee[0 * sv + c] = -fy[BZ * sv + c];
     ee[1 * sv + c] = fx[BZ * sv + c];
     ee[2 * sv + c] = (0.5 * (-fx[BY * sv + c - sy] - fx[BY * sv + c] + fy[BX * sv + c - 1] +
@@ -97,57 +114,174 @@ __device__ __forceinline__ void emf_edge(const Geo &g, const double *__restrict_
   }
 }
 
+__device__ __forceinline__ void emf_edge(const Geo &g, const double *__restrict__ w,
+                                         const double *__restrict__ fx,
+                                         const double *__restrict__ fy,
+                                         const double *__restrict__ fz,
+                                         double *__restrict__ ee, int i, int j, int k) {
+  if (g.dims == 3) emf_edge_d<3>(g, w, fx, fy, fz, ee, i, j, k);
+  else if (g.dims == 2) emf_edge_d<2>(g, w, fx, fy, fz, ee, i, j, k);
+  else emf_edge_d<1>(g, w, fx, fy, fz, ee, i, j, k);
+}
+
 // --------------------------------------------------------------------------
-// Face-field update by Stokes (advance_b, ct.pyx:15-91) for one face.
-__device__ __forceinline__ double bface_x(const Geo &g, const double *b0, const double *ee,
-                                          double dt, int i, int j, int k, long c) {
-  const long sv = g.sv;
-  const double dtda = dt / (g.dlf[1][j] * g.dlf[2][k]);
-  double v = b0[c];
-  if (g.dims >= 2) {
-    const double ds = g.dlf[2][k];
-    v -= dtda * (ds * ee[2 * sv + c + g.sy] - ds * ee[2 * sv + c]);
-  }
-  if (g.dims >= 3) {
-    const double ds = g.dlf[1][j];
-    v += dtda * (ds * ee[1 * sv + c + g.sz] - ds * ee[1 * sv + c]);
-  }
-  return v;
-}
-
-__device__ __forceinline__ double bface_y(const Geo &g, const double *b0, const double *ee,
-                                          double dt, int i, int j, int k, long c) {
-  const long sv = g.sv;
-  const double dtda = dt / (g.dlf[0][i] * g.dlf[2][k]);
-  double ds = g.dlf[2][k];
-  double v = b0[1 * sv + c] + dtda * (ds * ee[2 * sv + c + 1] - ds * ee[2 * sv + c]);
-  if (g.dims >= 3) {
-    ds = g.dlf[0][i];
-    v -= dtda * (ds * ee[0 * sv + c + g.sz] - ds * ee[0 * sv + c]);
-  }
-  return v;
-}
-
-__device__ __forceinline__ double bface_z(const Geo &g, const double *b0, const double *ee,
-                                          double dt, int i, int j, int k, long c) {
-  const long sv = g.sv;
-  const double dtda = dt / (g.dlf[0][i] * g.dlf[1][j]);
-  double ds = g.dlf[1][j];
-  double v = b0[2 * sv + c] - dtda * (ds * ee[1 * sv + c + 1] - ds * ee[1 * sv + c]);
-  if (g.dims >= 2) {
-    ds = g.dlf[0][i];
-    v += dtda * (ds * ee[0 * sv + c + g.sy] - ds * ee[0 * sv + c]);
-  }
-  return v;
-}
-
-
-// Stage update: advance_hydro (godunov.pyx:215-252), advance_b, interp_bc
-// (ct.pyx:96-123) and cons2prim (eos.pyx:13-41) fused, one thread per cell
-// of the whole grid.  u0 = prim2cons(w0) is the reference's cons array.
-// PRED: writes ws (a separate buffer); cells outside the limits take w0,
-// as the reference's ws aliases prim (vanleer.pyx:72).  Corrector: w1 may
+// Stage update: advance_hydro (godunov.pyx:215-252), advance_b (ct.pyx:15-91),
+// interp_bc (ct.pyx:96-123) and cons2prim (eos.pyx:13-41) fused, one thread
+// per cell of the whole grid.  u0 = prim2cons(w0) is the reference's cons
+// array.  PRED: writes ws (a separate buffer); cells outside the limits take
+// w0, as the reference's ws aliases prim (vanleer.pyx:72).  Corrector: w1 may
 // alias w0 (each thread reads only its own cell of w0).
+//
+// Every operand is loaded up front and unconditionally: neighbour offsets
+// are zero where the cell needs no neighbour on that side, which keeps all
+// loads in bounds, and only the stores depend on the cell's role.  The loads
+// then form one block the compiler can issue together instead of a chain of
+// dependent round trips.  Face values by Stokes (advance_b):
+//   Bx(i)   = b - dt/(dy dz) (dz Ez(j+1) - dz Ez(j)) + dt/(dy dz) (dy Ey(k+1) - dy Ey(k))
+//   By(j)   = b + dt/(dx dz) (dz Ez(i+1) - dz Ez(i)) - dt/(dx dz) (dx Ex(k+1) - dx Ex(k))
+//   Bz(k)   = b - dt/(dx dy) (dy Ey(i+1) - dy Ey(i)) + dt/(dx dy) (dx Ex(j+1) - dx Ex(j))
+template <int DIMS, bool PRED, bool DTRED>
+__device__ __forceinline__ double update_cell_d(const Geo &g, const double *w0,
+                                                const double *__restrict__ fx,
+                                                const double *__restrict__ fy,
+                                                const double *__restrict__ fz,
+                                                const double *__restrict__ ee,
+                                                const double *__restrict__ b0,
+                                                double *__restrict__ b1, double *w1,
+                                                const Lims &L, const double *__restrict__ dts,
+                                                const Box &act, int fms_ref, int i, int j, int k) {
+  const long c = ((long)k * g.nt[1] + j) * g.nt[0] + i;
+  const long sv = g.sv;
+  const double dt = PRED ? 0.5 * dts[0] : dts[0];
+
+  const bool inI = i >= L.lo[0] && i <= L.hi[0];
+  const bool inJ = j >= L.lo[1] && j <= L.hi[1];
+  const bool inK = k >= L.lo[2] && k <= L.hi[2];
+  const bool cell = inI && inJ && inK;
+  const bool fX = (i >= L.lo[0] && i <= L.hi[0] + 1) && inJ && inK;
+  const bool fY = inI && (j >= L.lo[1] && j <= L.hi[1] + (DIMS >= 2 ? 1 : 0)) && inK;
+  const bool fZ = inI && inJ && (k >= L.lo[2] && k <= L.hi[2] + (DIMS >= 3 ? 1 : 0));
+  // neighbour offsets: each is used only by roles that keep that index
+  // inside the limits, so c + ox + oy + oz is a cell of the grid
+  const long ox = (cell || fY || fZ) ? 1 : 0;
+  const long oy = (DIMS >= 2 && (cell || fX || fZ)) ? g.sy : 0;
+  const long oz = (DIMS >= 3 && (cell || fX || fY)) ? g.sz : 0;
+
+  // ---- loads
+  double wv[NMODE];
+#pragma unroll
+  for (int n = 0; n < NMODE; ++n) wv[n] = w0[n * sv + c];
+  const double bxc = b0[c], byc = b0[sv + c], bzc = b0[2 * sv + c];
+  const double bxp = b0[c + ox], byp = b0[sv + c + oy], bzp = b0[2 * sv + c + oz];
+  const double *ex = ee, *ey = ee + sv, *ez = ee + 2 * sv;
+  const double ez00 = ez[c], ez10 = ez[c + ox], ez01 = ez[c + oy], ez11 = ez[c + ox + oy];
+  const double ey00 = ey[c], ey10 = ey[c + ox], ey01 = ey[c + oz], ey11 = ey[c + ox + oz];
+  const double ex00 = ex[c], ex10 = ex[c + oy], ex01 = ex[c + oz], ex11 = ex[c + oy + oz];
+  double fxl[NMODE - 3], fxu[NMODE - 3], fyl[NMODE - 3], fyu[NMODE - 3], fzl[NMODE - 3],
+      fzu[NMODE - 3];
+#pragma unroll
+  for (int n = 0; n < NMODE - 3; ++n) {
+    fxl[n] = fx[n * sv + c];
+    fxu[n] = fx[n * sv + c + ox];
+    if (DIMS >= 2) {
+      fyl[n] = fy[n * sv + c];
+      fyu[n] = fy[n * sv + c + oy];
+    }
+    if (DIMS >= 3) {
+      fzl[n] = fz[n * sv + c];
+      fzu[n] = fz[n * sv + c + oz];
+    }
+  }
+  const double dlx = g.dlf[0][i], dly = g.dlf[1][j], dlz = g.dlf[2][k];
+
+  // ---- faces (advance_b); the lower faces of the cell and, for the cell
+  // interpolation, its upper faces
+  double bxl, bxu, byl, byu = 0., bzl, bzu = 0.;
+  {
+    const double dtda = dt / (dly * dlz);
+    double v = bxc, vu = bxp;
+    if (DIMS >= 2) {
+      const double ds = dlz;
+      v -= dtda * (ds * ez01 - ds * ez00);
+      vu -= dtda * (ds * ez11 - ds * ez10);
+    }
+    if (DIMS >= 3) {
+      const double ds = dly;
+      v += dtda * (ds * ey01 - ds * ey00);
+      vu += dtda * (ds * ey11 - ds * ey10);
+    }
+    bxl = v;
+    bxu = vu;
+  }
+  {
+    const double dtda = dt / (dlx * dlz);
+    double ds = dlz;
+    double v = byc + dtda * (ds * ez10 - ds * ez00);
+    double vu = byp + dtda * (ds * ez11 - ds * ez01);
+    if (DIMS >= 3) {
+      ds = dlx;
+      v -= dtda * (ds * ex01 - ds * ex00);
+      vu -= dtda * (ds * ex11 - ds * ex10);
+    }
+    byl = v;
+    if (DIMS >= 2) byu = vu;
+  }
+  {
+    const double dtda = dt / (dlx * dly);
+    double ds = dly;
+    double v = bzc - dtda * (ds * ey10 - ds * ey00);
+    double vu = bzp - dtda * (ds * ey11 - ds * ey01);
+    if (DIMS >= 2) {
+      ds = dlx;
+      v += dtda * (ds * ex10 - ds * ex00);
+      vu += dtda * (ds * ex11 - ds * ex01);
+    }
+    bzl = v;
+    if (DIMS >= 3) bzu = vu;
+  }
+
+  // ---- conserved update (advance_hydro) and cell-centred field (interp_bc)
+  double u[NMODE];
+  prim2cons(u, wv, g.gamm1i);
+  const double dtdv = dt / (dlx * dly * dlz);
+  const double ax = dly * dlz, ay = dlx * dlz, az = dlx * dly;
+#pragma unroll
+  for (int n = 0; n < NMODE - 3; ++n) {
+    double v = u[n] - dtdv * (ax * fxu[n] - ax * fxl[n]);
+    if (DIMS >= 2) v -= dtdv * (ay * fyu[n] - ay * fyl[n]);
+    if (DIMS >= 3) v -= dtdv * (az * fzu[n] - az * fzl[n]);
+    u[n] = v;
+  }
+  u[BX] = (g.icm[0][i] * bxl + g.icp[0][i] * bxu) * g.dlfi[0][i];
+  if (DIMS >= 2)
+    u[BY] = (g.icm[1][j] * byl + g.icp[1][j] * byu) * g.dlfi[1][j];
+  else
+    u[BY] = byl;
+  if (DIMS >= 3)
+    u[BZ] = (g.icm[2][k] * bzl + g.icp[2][k] * bzu) * g.dlfi[2][k];
+  else
+    u[BZ] = bzl;
+  double wn[NMODE];
+  cons2prim(wn, u, g.gam);
+
+  // ---- stores
+  if (fX) b1[c] = bxl;
+  if (fY) b1[sv + c] = byl;
+  if (fZ) b1[2 * sv + c] = bzl;
+  if (cell) {
+#pragma unroll
+    for (int n = 0; n < NMODE; ++n) w1[n * sv + c] = wn[n];
+  } else if (PRED) {
+#pragma unroll
+    for (int n = 0; n < NMODE; ++n) w1[n * sv + c] = wv[n];
+  }
+  if (DTRED && cell && i >= act.lo[0] && i < act.lo[0] + act.n[0] && j >= act.lo[1] &&
+      j < act.lo[1] + act.n[1] && k >= act.lo[2] && k < act.lo[2] + act.n[2])
+    return cds_cell(g, wn, fms_ref, i, j, k);
+  return 0.;
+}
+
+// Same with the dimensionality read from g (batched boxes of mixed dims).
 template <bool PRED, bool DTRED>
 __device__ __forceinline__ double update_cell(const Geo &g, const double *w0,
                                               const double *__restrict__ fx,
@@ -158,79 +292,14 @@ __device__ __forceinline__ double update_cell(const Geo &g, const double *w0,
                                               double *__restrict__ b1, double *w1, const Lims &L,
                                               const double *__restrict__ dts, const Box &act,
                                               int fms_ref, int i, int j, int k) {
-  const long c = ((long)k * g.nt[1] + j) * g.nt[0] + i;
-  const long sv = g.sv;
-  const double dt = PRED ? 0.5 * dts[0] : dts[0];
-
-  const bool inI = i >= L.lo[0] && i <= L.hi[0];
-  const bool inJ = j >= L.lo[1] && j <= L.hi[1];
-  const bool inK = k >= L.lo[2] && k <= L.hi[2];
-  const bool cell = inI && inJ && inK;
-  const bool fX = (i >= L.lo[0] && i <= L.hi[0] + 1) && inJ && inK;
-  const bool fY = inI && (j >= L.lo[1] && j <= L.hi[1] + (g.dims >= 2 ? 1 : 0)) && inK;
-  const bool fZ = inI && inJ && (k >= L.lo[2] && k <= L.hi[2] + (g.dims >= 3 ? 1 : 0));
-
-  double bxl = 0., byl = 0., bzl = 0.;
-  if (fX) {
-    bxl = bface_x(g, b0, ee, dt, i, j, k, c);
-    b1[c] = bxl;
-  }
-  if (fY) {
-    byl = bface_y(g, b0, ee, dt, i, j, k, c);
-    b1[sv + c] = byl;
-  }
-  if (fZ) {
-    bzl = bface_z(g, b0, ee, dt, i, j, k, c);
-    b1[2 * sv + c] = bzl;
-  }
-
-  if (!cell) {
-    if (PRED) {
-#pragma unroll
-      for (int n = 0; n < NMODE; ++n) w1[n * sv + c] = w0[n * sv + c];
-    }
-    return 0.;
-  }
-
-  double wv[NMODE], u[NMODE];
-#pragma unroll
-  for (int n = 0; n < NMODE; ++n) wv[n] = w0[n * sv + c];
-  prim2cons(u, wv, g.gamm1i);
-
-  const double dlx = g.dlf[0][i], dly = g.dlf[1][j], dlz = g.dlf[2][k];
-  const double dtdv = dt / (dlx * dly * dlz);
-  const double ax = dly * dlz, ay = dlx * dlz, az = dlx * dly;
-#pragma unroll
-  for (int n = 0; n < NMODE - 3; ++n) {
-    double v = u[n] - dtdv * (ax * fx[n * sv + c + 1] - ax * fx[n * sv + c]);
-    if (g.dims >= 2) v -= dtdv * (ay * fy[n * sv + c + g.sy] - ay * fy[n * sv + c]);
-    if (g.dims >= 3) v -= dtdv * (az * fz[n * sv + c + g.sz] - az * fz[n * sv + c]);
-    u[n] = v;
-  }
-
-  // interp_bc with the upper faces recomputed in-thread
-  const double bxu = bface_x(g, b0, ee, dt, i + 1, j, k, c + 1);
-  u[BX] = (g.icm[0][i] * bxl + g.icp[0][i] * bxu) * g.dlfi[0][i];
-  if (g.dims >= 2) {
-    const double byu = bface_y(g, b0, ee, dt, i, j + 1, k, c + g.sy);
-    u[BY] = (g.icm[1][j] * byl + g.icp[1][j] * byu) * g.dlfi[1][j];
-  } else {
-    u[BY] = byl;
-  }
-  if (g.dims >= 3) {
-    const double bzu = bface_z(g, b0, ee, dt, i, j, k + 1, c + g.sz);
-    u[BZ] = (g.icm[2][k] * bzl + g.icp[2][k] * bzu) * g.dlfi[2][k];
-  } else {
-    u[BZ] = bzl;
-  }
-  cons2prim(wv, u, g.gam);
-#pragma unroll
-  for (int n = 0; n < NMODE; ++n) w1[n * sv + c] = wv[n];
-  if (DTRED && i >= act.lo[0] && i < act.lo[0] + act.n[0] && j >= act.lo[1] &&
-      j < act.lo[1] + act.n[1] && k >= act.lo[2] && k < act.lo[2] + act.n[2])
-    return cds_cell(g, wv, fms_ref, i, j, k);
-  return 0.;
+  if (g.dims == 3)
+    return update_cell_d<3, PRED, DTRED>(g, w0, fx, fy, fz, ee, b0, b1, w1, L, dts, act, fms_ref,
+                                         i, j, k);
+  if (g.dims == 2)
+    return update_cell_d<2, PRED, DTRED>(g, w0, fx, fy, fz, ee, b0, b1, w1, L, dts, act, fms_ref,
+                                         i, j, k);
+  return update_cell_d<1, PRED, DTRED>(g, w0, fx, fy, fz, ee, b0, b1, w1, L, dts, act, fms_ref,
+                                       i, j, k);
 }
-
 
 }  // namespace bnz

@@ -111,7 +111,10 @@ __device__ __forceinline__ T fms(const T *w, T gam) {
 template <typename T>
 __device__ __forceinline__ T vl_lim(T a, T b) {
   T ab = a * b;
-  return (ab > T(0)) ? Ar<T>::div(T(2) * ab, a + b) : T(0);
+  // branch-free: the division always runs, on (0, 1) where the limiter is 0,
+  // so the stencil loads around it stay in one basic block
+  const bool pos = ab > T(0);
+  return Ar<T>::div(pos ? T(2) * ab : T(0), pos ? a + b : T(1));
 }
 
 // sLsR_simple, src/mhd/fluxes_c.c:25-42 (FMAX/FMIN as fmax/fmin)

@@ -106,9 +106,10 @@ int bnz_set_time(bnz_ctx *ctx, double time, int step);
 int bnz_diag(bnz_ctx *ctx, bnz_diag_out *out);
 int bnz_synchronize(bnz_ctx *ctx);
 
-/* Chunk lengths (cells along the marching axis per thread) of the 3D
-   marching kernels: y/z fluxes, edge EMFs and the stage update; 0 selects
-   the one-thread-per-face/edge/cell kernel (the default).  Chosen by the
+/* Work shapes of the 3D Godunov-flux, edge-EMF and stage-update kernels:
+   0 one thread per face/edge/cell in x-fastest order (the default), 1 tiles
+   of 64 x 4 cells ordered along z, n > 1 slab columns of n rows walked
+   plane by plane.  Results do not depend on the shape.  Chosen by the
    autotuner (bonzo_amd/autotune.py); no counterpart in the reference. */
 int bnz_set_tuning(bnz_ctx *ctx, int flux_chunk, int emf_chunk, int update_chunk);
 
@@ -159,8 +160,9 @@ int bnz_batch_download(bnz_batch *b, int box, double *prim, double *bfld);
 int bnz_batch_init_state(bnz_batch *b);
 /* Steps until every box has reached its t_max, or max_steps (> 0) step
    launches.  use_graph: replay a hipGraph holding two captured steps (steps
-   are then taken in pairs; a box that finishes in the first idles in the
-   second).  *nsteps gets the number of step launches. */
+   are then launched in pairs, up to four pairs between host checks; a box
+   at its t_max idles through the rest).  *nsteps gets the number of step
+   launches; bnz_batch_get_times gives the steps each box took. */
 int bnz_batch_advance(bnz_batch *b, int max_steps, int use_graph, int *nsteps);
 /* Per-box time and step count (arrays of nbox) */
 int bnz_batch_get_times(bnz_batch *b, double *t, int *steps);

@@ -198,13 +198,15 @@ def test_divb_and_conservation_128():
     assert d1["nonfinite"] == 0
 
 
-@pytest.mark.parametrize("chunks", [(0, 0, 0), (1, 1, 1), (7, 5, 3), (64, 64, 64), (0, 16, 4)])
-def test_marching_chunks_do_not_change_results(chunks):
-    """The 3D marching kernels give the oracle's result for any chunk length
-    (chunk boundaries restart the register window)."""
+@pytest.mark.parametrize("chunks", [(0, 0, 0), (1, 1, 1), (7, 5, 3), (64, 64, 64), (0, 16, 4),
+                                    (2, 2, 2), (26, 26, 26)])
+def test_work_shapes_do_not_change_results(chunks):
+    """Every work shape of the 3D flux, EMF and update kernels (flat, 64x4
+    tiles, slab columns of any height, including partial and single columns)
+    gives the oracle's result bit for bit."""
     from bonzo_amd._lib import Context
     from oracle.oracle import OracleSim
-    say(f"case: turbulence 20^3, chunks {chunks}, 4 steps")
+    say(f"case: turbulence 20^3, shapes {chunks}, 4 steps")
     s, prim, bfld = P.turbulence(20, seed_b=61, seed_v=62)
     o = OracleSim(20, 20, 20)
     o.prim[:] = prim
@@ -220,3 +222,22 @@ def test_marching_chunks_do_not_change_results(chunks):
     sk, sj, si = s.active()
     assert np.array_equal(gp[:, sk, sj, si], o.prim[:, sk, sj, si])
     assert np.array_equal(gb[:, sk, sj, si], o.bfld[:, sk, sj, si])
+
+
+def test_work_shapes_non_cubic():
+    """Slab columns on a non-cubic grid with uneven column counts."""
+    from bonzo_amd._lib import Context
+    say("case: Brio-Wu along y on 6x40x6, outflow, shapes (5, 7, 3), 10 steps")
+    res = {}
+    for shapes in ((0, 0, 0), (5, 7, 3), (1, 1, 1)):
+        s, prim, bfld = P.brio_wu(40, axis=1, transverse=6)
+        g = Context(s.nx, s.ny, s.nz, bc=(1,) * 6)
+        g.set_tuning(*shapes)
+        g.upload(prim, bfld)
+        g.init_state()
+        g.run_steps(10)
+        res[shapes] = g.download()
+        g.close()
+    for shapes in ((5, 7, 3), (1, 1, 1)):
+        assert np.array_equal(res[shapes][0], res[(0, 0, 0)][0])
+        assert np.array_equal(res[shapes][1], res[(0, 0, 0)][1])
