@@ -246,6 +246,26 @@ __global__ __launch_bounds__(BLK) void kb_ghost(long sv, double *w, double *b,
   }
 }
 
+// Ghost cells a reflecting side touches (GhostOp), pool indices.
+__global__ __launch_bounds__(BLK) void kb_ghost_ops(long sv, double *w, double *b,
+                                                    const GhostOp *__restrict__ ops, long nops) {
+  for (long t = (long)blockIdx.x * BLK + threadIdx.x; t < nops; t += (long)gridDim.x * BLK) {
+    const GhostOp o = ops[t];
+    if (!(o.bits & (1u << 15))) {
+#pragma unroll
+      for (int n = 0; n < NMODE; ++n) {
+        const double v = w[n * sv + o.src];
+        w[n * sv + o.dst] = ((o.bits >> n) & 1u) ? -v : v;
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const double v = ((o.bits >> (12 + a)) & 1u) ? 0. : b[a * sv + o.srcf[a]];
+      b[a * sv + o.dst] = ((o.bits >> (9 + a)) & 1u) ? -v : v;
+    }
+  }
+}
+
 __global__ void kb_zero(unsigned long long *a, int n) {
   for (int q = threadIdx.x + blockIdx.x * blockDim.x; q < n; q += blockDim.x * gridDim.x)
     a[q] = 0ull;
@@ -263,12 +283,13 @@ struct bnz_batch {
   std::vector<BoxDev> hbox;
   int nblk[NK] = {};
   int bm_off[NK] = {};
-  long ncell = 0, nghost = 0;
+  long ncell = 0, nghost = 0, ngops = 0;
   hipStream_t stream = nullptr;
   BoxDev *boxes = nullptr;
   int *bmap = nullptr;
   double *geo = nullptr;
   int2 *ghost = nullptr;
+  GhostOp *gops = nullptr;
   Pools P{};
   BoxState S{};
   double *sbuf = nullptr;      // backing store of BoxState
@@ -359,9 +380,12 @@ void b_flux_dispatch(bnz_batch *c, int order, const double *w, const double *bf,
 }
 
 void b_ghost(bnz_batch *c) {
-  if (c->nghost <= 0) return;
-  hipLaunchKernelGGL(kb_ghost, dim3(grid_for((c->nghost + BLK - 1) / BLK)), dim3(BLK), 0,
-                     c->stream, c->ncell, c->P.w, c->P.b, c->ghost, c->nghost);
+  if (c->nghost > 0)
+    hipLaunchKernelGGL(kb_ghost, dim3(grid_for((c->nghost + BLK - 1) / BLK)), dim3(BLK), 0,
+                       c->stream, c->ncell, c->P.w, c->P.b, c->ghost, c->nghost);
+  if (c->ngops > 0)
+    hipLaunchKernelGGL(kb_ghost_ops, dim3(grid_for((c->ngops + BLK - 1) / BLK)), dim3(BLK), 0,
+                       c->stream, c->ncell, c->P.w, c->P.b, c->gops, c->ngops);
 }
 
 // One VL step of every running box (vanleer.pyx:94-167): the same launch
@@ -429,6 +453,7 @@ int bnz_batch_create(const bnz_params *p, const bnz_box *boxes, int nbox, bnz_ba
   std::vector<double> hgeo;
   std::vector<size_t> geo_off;  // 12 per box: (dlf, dlfi, icm, icp) x 3 axes
   std::vector<int2> pairs;
+  std::vector<GhostOp> gops;
   c->hbox.resize(nbox);
   long off = 0;
   for (int b = 0; b < nbox; ++b) {
@@ -436,9 +461,10 @@ int bnz_batch_create(const bnz_params *p, const bnz_box *boxes, int nbox, bnz_ba
     if (s.nx < 1 || s.ny < 1 || s.nz < 1 || (s.nz > 1 && s.ny == 1))
       return bfail(c, BNZ_EINVAL, "box " + std::to_string(b) + ": bad size");
     for (int q = 0; q < 6; ++q)
-      if (s.bc[q] != BNZ_BC_PERIODIC && s.bc[q] != BNZ_BC_OUTFLOW)
-        return bfail(c, BNZ_EUNSUPPORTED, "box " + std::to_string(b) +
-                                              ": boundary flags must be periodic or outflow");
+      if (s.bc[q] != BNZ_BC_PERIODIC && s.bc[q] != BNZ_BC_OUTFLOW && s.bc[q] != BNZ_BC_REFLECT)
+        return bfail(c, BNZ_EUNSUPPORTED,
+                     "box " + std::to_string(b) +
+                         ": boundary flags must be periodic, outflow or reflect");
     for (int a = 0; a < 3; ++a)
       if (!(s.lmax[a] > s.lmin[a]))
         return bfail(c, BNZ_EINVAL, "box " + std::to_string(b) + ": need lmax > lmin");
@@ -449,6 +475,10 @@ int bnz_batch_create(const bnz_params *p, const bnz_box *boxes, int nbox, bnz_ba
     const int a1[3] = {ng, dims >= 2 ? ng : 0, dims >= 3 ? ng : 0};
     const int a2[3] = {s.nx + ng - 1, dims >= 2 ? s.ny + ng - 1 : 0, dims >= 3 ? s.nz + ng - 1 : 0};
     const int nact[3] = {s.nx, s.ny, s.nz};
+    for (int a = 0; a < dims; ++a)
+      if ((s.bc[2 * a] == BNZ_BC_REFLECT || s.bc[2 * a + 1] == BNZ_BC_REFLECT) && nact[a] < ng)
+        return bfail(c, BNZ_EINVAL,
+                     "box " + std::to_string(b) + ": a reflecting axis needs at least ng cells");
     for (int q = 0; q < 3; ++q) d.g.nt[q] = nt[q];
     d.g.dims = dims;
     d.g.sy = nt[0];
@@ -484,33 +514,14 @@ int bnz_batch_create(const bnz_params *p, const bnz_box *boxes, int nbox, bnz_ba
     d.items[KUPD] = mkbox(0, nt[0], 0, nt[1], 0, nt[2]);
     d.act = mkbox(a1[0], s.nx, a1[1], dims >= 2 ? s.ny : 1, a1[2], dims >= 3 ? s.nz : 1);
     d.items[KACT] = d.act;
-    // ghost (cell, source) pairs: per-axis source maps composed
-    // (grid_bc_funcs.pyx:36-196; z1 outflow copies k1 as intended)
-    std::vector<int> m[3];
-    for (int a = 0; a < 3; ++a) {
-      m[a].resize(nt[a]);
-      for (int i = 0; i < nt[a]; ++i) m[a][i] = i;
-      if (a >= dims) continue;
-      for (int gg = 0; gg < ng; ++gg) {
-        m[a][a1[a] - ng + gg] = (s.bc[2 * a] == BNZ_BC_PERIODIC) ? (a2[a] - ng + 1 + gg) : a1[a];
-        m[a][a2[a] + 1 + gg] = (s.bc[2 * a + 1] == BNZ_BC_PERIODIC) ? (a1[a] + gg) : a2[a];
-      }
-    }
-    for (int k = 0; k < nt[2]; ++k)
-      for (int j = 0; j < nt[1]; ++j)
-        for (int i = 0; i < nt[0]; ++i) {
-          const int si = m[0][i], sj = m[1][j], sk = m[2][k];
-          if (si == i && sj == j && sk == k) continue;
-          const long dc = off + ((long)k * nt[1] + j) * nt[0] + i;
-          const long sc = off + ((long)sk * nt[1] + sj) * nt[0] + si;
-          pairs.push_back(make_int2((int)dc, (int)sc));
-        }
+    build_ghosts(nt, dims, ng, s.bc, off, pairs, gops);
     off += (long)nt[0] * nt[1] * nt[2];
     if (off >= (1L << 31))
       return bfail(c, BNZ_EINVAL, "batch too large: pool cell indices are 32-bit");
   }
   c->ncell = off;
   c->nghost = (long)pairs.size();
+  c->ngops = (long)gops.size();
   // work blocks: box b owns ceil(items / BLK) consecutive blocks of each list
   std::vector<int> hmap;
   for (int kt = 0; kt < NK; ++kt) {
@@ -559,6 +570,9 @@ int bnz_batch_create(const bnz_params *p, const bnz_box *boxes, int nbox, bnz_ba
   BCHK(c, hipMalloc(&c->ghost, std::max<size_t>(1, pairs.size()) * sizeof(int2)));
   if (!pairs.empty())
     BCHK(c, hipMemcpy(c->ghost, pairs.data(), pairs.size() * sizeof(int2), hipMemcpyHostToDevice));
+  BCHK(c, hipMalloc(&c->gops, std::max<size_t>(1, gops.size()) * sizeof(GhostOp)));
+  if (!gops.empty())
+    BCHK(c, hipMemcpy(c->gops, gops.data(), gops.size() * sizeof(GhostOp), hipMemcpyHostToDevice));
   // box scalars: dt, t, tmax, cmax (8 bytes each), run, steps (4 bytes), nrun
   const size_t nb = nbox;
   const size_t words = 4 * nb + (2 * nb + 1 + 1) / 2;
@@ -588,6 +602,7 @@ int bnz_batch_destroy(bnz_batch *c) {
   if (c->boxes) (void)hipFree(c->boxes);
   if (c->bmap) (void)hipFree(c->bmap);
   if (c->ghost) (void)hipFree(c->ghost);
+  if (c->gops) (void)hipFree(c->gops);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return BNZ_OK;

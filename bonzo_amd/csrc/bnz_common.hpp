@@ -30,6 +30,14 @@ struct Geo {
   double gam, gamm1i;
 };
 
+struct GhostOp {
+  int dst;       // cell written
+  int src;       // source of the cell variables
+  int srcf[3];   // source of each face component
+  unsigned bits; // bit v < 9: negate slot v; 9 + a: negate face a;
+                 // 12 + a: face a = 0; 15: cell variables untouched
+};
+
 struct Lims {
   int lo[3], hi[3];  // inclusive cell limits of get_integr_lims (integrator.pyx:375-383)
 };
@@ -202,8 +210,10 @@ struct bnz_ctx {
   double *w = nullptr, *ws = nullptr, *b = nullptr, *bs = nullptr;
   double *fx = nullptr, *fy = nullptr, *fz = nullptr, *ee = nullptr;
   double *geo = nullptr;
-  int2 *ghost = nullptr;      // (ghost cell, source cell) of every ghost cell
+  int2 *ghost = nullptr;      // (ghost cell, source cell) pairs, see build_ghosts
   int nghost = 0;
+  bnz::GhostOp *gops = nullptr;  // ghost cells touched by a reflecting side
+  int ngops = 0;
   double *dev = nullptr;      // dt, t, tmax, step
   double *part = nullptr;     // reduction partials
   long npart_cap = 0;
@@ -348,6 +358,103 @@ inline int check_launch(bnz_ctx *c) {
   return BNZ_OK;
 }
 
+
+// --------------------------------------------------------------------------
+// Ghost fill of one grid (GridBC.apply_grid_bc, src/bc/grid_bc.pyx:323-339)
+// as two host-built lists.  Each axis has a map from a ghost index to the
+// active index it is filled from (periodic copy_layer_*, outflow prolong_*,
+// reflect copy_reflect_layer_*: util_bc.pyx:9-90); the reference fills z,
+// then y, then x over the full extent of the other axes, which composes the
+// per-axis maps, so a ghost cell takes its value straight from one active
+// cell.
+//   pairs: ghost cells untouched by a reflecting side, (cell, source), all
+//          variables copied as they are;
+//   ops:   cells a reflecting side touches: one source for the cell
+//          variables and one per face component (the face-normal component
+//          is mirrored about the wall face, not about the wall cell), sign
+//          flips (normal velocity or momentum and normal cell field flip,
+//          the face-normal field flips across its own wall) and the zeroed
+//          wall face (set_layer_*, util_bc.pyx:172-205).
+// Sources are active cells that the fill never writes, so every entry runs
+// independently.  Cell indices are offset by off (a box in a pool).
+
+
+inline void build_ghosts(const int nt[3], int dims, int ng, const int bc[6], long off,
+                         std::vector<int2> &pairs, std::vector<GhostOp> &ops) {
+  std::vector<int> mc[3], mf[3];
+  std::vector<char> fc[3], ff[3], zf[3], refl[3];
+  for (int a = 0; a < 3; ++a) {
+    const int N = nt[a];
+    mc[a].resize(N);
+    mf[a].resize(N);
+    fc[a].assign(N, 0);
+    ff[a].assign(N, 0);
+    zf[a].assign(N, 0);
+    refl[a].assign(N, 0);
+    for (int i = 0; i < N; ++i) mc[a][i] = mf[a][i] = i;
+    if (a >= dims) continue;
+    const int a1 = ng, a2 = N - ng - 1;
+    for (int side = 0; side < 2; ++side) {
+      const int flag = bc[2 * a + side];
+      for (int g = 0; g < ng; ++g) {
+        const int idx = side == 0 ? a1 - ng + g : a2 + 1 + g;
+        if (flag == BNZ_BC_PERIODIC) {
+          mc[a][idx] = mf[a][idx] = side == 0 ? a2 - ng + 1 + g : a1 + g;
+        } else if (flag == BNZ_BC_OUTFLOW) {  // D14: z1 copies k1
+          mc[a][idx] = mf[a][idx] = side == 0 ? a1 : a2;
+        } else {  // reflect: mirror about the wall
+          mc[a][idx] = side == 0 ? 2 * a1 - 1 - idx : 2 * a2 + 1 - idx;
+          fc[a][idx] = 1;
+          refl[a][idx] = 1;
+        }
+      }
+      if (flag == BNZ_BC_REFLECT) {  // face-normal component: wall face zeroed
+        const int wall = side == 0 ? a1 : a2 + 1;
+        zf[a][wall] = 1;
+        refl[a][wall] = 1;
+        mf[a][wall] = wall;
+        if (side == 0) {
+          for (int idx = 0; idx < a1; ++idx) {
+            mf[a][idx] = 2 * a1 - idx;
+            ff[a][idx] = 1;
+          }
+        } else {
+          for (int idx = a2 + 2; idx < N; ++idx) {
+            mf[a][idx] = 2 * a2 + 2 - idx;
+            ff[a][idx] = 1;
+          }
+        }
+      }
+    }
+  }
+  auto cell = [&](int i, int j, int k) { return off + ((long)k * nt[1] + j) * nt[0] + i; };
+  for (int k = 0; k < nt[2]; ++k)
+    for (int j = 0; j < nt[1]; ++j)
+      for (int i = 0; i < nt[0]; ++i) {
+        const int id[3] = {i, j, k};
+        const int si = mc[0][i], sj = mc[1][j], sk = mc[2][k];
+        if (!refl[0][i] && !refl[1][j] && !refl[2][k]) {
+          if (si == i && sj == j && sk == k) continue;
+          pairs.push_back(make_int2((int)cell(i, j, k), (int)cell(si, sj, sk)));
+          continue;
+        }
+        GhostOp o;
+        o.dst = (int)cell(i, j, k);
+        o.src = (int)cell(si, sj, sk);
+        o.bits = 0;
+        for (int a = 0; a < 3; ++a)
+          if (fc[a][id[a]]) o.bits ^= (1u << (VX + a)) | (1u << (BX + a));
+        for (int f = 0; f < 3; ++f) {
+          int s3[3] = {si, sj, sk};
+          s3[f] = mf[f][id[f]];
+          o.srcf[f] = (int)cell(s3[0], s3[1], s3[2]);
+          if (ff[f][id[f]]) o.bits |= 1u << (9 + f);
+          if (zf[f][id[f]]) o.bits |= 1u << (12 + f);
+        }
+        if (o.src == o.dst && (o.bits & 0x1ffu) == 0) o.bits |= 1u << 15;
+        ops.push_back(o);
+      }
+}
 
 // FP32-compensated step (bnz_gpu32.hip)
 int f32_alloc(bnz_ctx *c);

@@ -93,6 +93,28 @@ __global__ __launch_bounds__(BLK) void k_ghost(Geo g, double *w, double *b,
   for (int n = 0; n < 3; ++n) b[n * sv + ds.x] = b[n * sv + ds.y];
 }
 
+// Ghost cells a reflecting side touches (see GhostOp): per-class sources,
+// sign flips and zeroed wall faces.
+__global__ __launch_bounds__(BLK) void k_ghost_ops(Geo g, double *w, double *b,
+                                                   const GhostOp *__restrict__ ops, int nops) {
+  const long t = xcd_block() * BLK + threadIdx.x;
+  if (t >= nops) return;
+  const GhostOp o = ops[t];
+  const long sv = g.sv;
+  if (!(o.bits & (1u << 15))) {
+#pragma unroll
+    for (int n = 0; n < NMODE; ++n) {
+      const double v = w[n * sv + o.src];
+      w[n * sv + o.dst] = ((o.bits >> n) & 1u) ? -v : v;
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const double v = ((o.bits >> (12 + a)) & 1u) ? 0. : b[a * sv + o.srcf[a]];
+    b[a * sv + o.dst] = ((o.bits >> (9 + a)) & 1u) ? -v : v;
+  }
+}
+
 // interp_bc over the active cells into prim (init only, vanleer.pyx:44)
 __global__ __launch_bounds__(BLK) void k_interp_active(Geo g, double *w, const double *b, Box box) {
   long t = xcd_block() * BLK + threadIdx.x;
@@ -338,22 +360,6 @@ void set_coordinates(bnz_ctx *c) {
   }
 }
 
-// Per-axis ghost source maps (grid_bc_funcs.pyx:36-196; D14 for z1 outflow).
-std::vector<int> axis_map(int N, int a1, int a2, int ng, int f_lo, int f_hi, bool active) {
-  std::vector<int> m(N);
-  for (int i = 0; i < N; ++i) m[i] = i;
-  if (!active) return m;
-  const int n = a2 - a1 + 1;
-  for (int g = 0; g < ng; ++g) {
-    const int dl = a1 - ng + g;
-    m[dl] = (f_lo == 0) ? (a2 - ng + 1 + g) : a1;
-    const int dr = a2 + 1 + g;
-    m[dr] = (f_hi == 0) ? (a1 + g) : a2;
-  }
-  (void)n;
-  return m;
-}
-
 template <int ORDER, int RS>
 void launch_fluxes(bnz_ctx *c, const double *w, const double *bf, const Lims &L) {
   // all face directions in one launch over the union of the face ranges
@@ -398,8 +404,12 @@ void launch_emf(bnz_ctx *c, const double *w, const Lims &L, const char *name) {
 
 void launch_ghost(bnz_ctx *c) {
   Launch t(c, "ghost");
-  hipLaunchKernelGGL(k_ghost, dim3(blocks_for(c->nghost)), dim3(BLK), 0, c->stream, c->g, c->w,
-                     c->b, c->ghost, c->nghost);
+  if (c->nghost > 0)
+    hipLaunchKernelGGL(k_ghost, dim3(blocks_for(c->nghost)), dim3(BLK), 0, c->stream, c->g, c->w,
+                       c->b, c->ghost, c->nghost);
+  if (c->ngops > 0)
+    hipLaunchKernelGGL(k_ghost_ops, dim3(blocks_for(c->ngops)), dim3(BLK), 0, c->stream, c->g,
+                       c->w, c->b, c->gops, c->ngops);
 }
 
 void launch_dt(bnz_ctx *c, int use_fixed, double fixed_dt) {
@@ -531,8 +541,9 @@ int bnz_create(const bnz_params *p, bnz_ctx **out) {
   if (p->reconstr != BNZ_RCN_CONST && p->reconstr != BNZ_RCN_LINEAR)
     return bad(BNZ_EUNSUPPORTED, "reconstr must be const or linear (weno/parab are commented out in reconstr_c.c)");
   for (int q = 0; q < 6; ++q)
-    if (p->bc[q] != BNZ_BC_PERIODIC && p->bc[q] != BNZ_BC_OUTFLOW)
-      return bad(BNZ_EUNSUPPORTED, "boundary flags must be 0 (periodic) or 1 (outflow)");
+    if (p->bc[q] != BNZ_BC_PERIODIC && p->bc[q] != BNZ_BC_OUTFLOW && p->bc[q] != BNZ_BC_REFLECT)
+      return bad(BNZ_EUNSUPPORTED,
+                 "boundary flags must be 0 (periodic), 1 (outflow) or 2 (reflect)");
   c->dims = (p->nz > 1) ? 3 : (p->ny > 1) ? 2 : 1;
   c->order = (p->reconstr == BNZ_RCN_CONST) ? 0 : 1;
   c->ng = (c->order == 0) ? 2 : 3;
@@ -544,6 +555,12 @@ int bnz_create(const bnz_params *p, bnz_ctx **out) {
   c->k1 = (c->dims >= 3) ? c->ng : 0; c->k2 = (c->dims >= 3) ? p->nz + c->ng - 1 : 0;
   c->ncell = (long)c->nt[0] * c->nt[1] * c->nt[2];
   if (c->ncell >= (1L << 31)) return bad(BNZ_EINVAL, "grid too large: cell indices are 32-bit");
+  {
+    const int nact[3] = {p->nx, p->ny, p->nz};
+    for (int a = 0; a < c->dims; ++a)
+      if ((p->bc[2 * a] == BNZ_BC_REFLECT || p->bc[2 * a + 1] == BNZ_BC_REFLECT) && nact[a] < c->ng)
+        return bad(BNZ_EINVAL, "a reflecting axis needs at least ng active cells");
+  }
   set_coordinates(c);
 
   *out = c;
@@ -588,24 +605,17 @@ int bnz_create(const bnz_params *p, bnz_ctx **out) {
 
   {
     // (ghost, source) pairs of every ghost cell, from the per-axis maps
-    auto mx = axis_map(c->nt[0], c->i1, c->i2, c->ng, p->bc[0], p->bc[1], true);
-    auto my = axis_map(c->nt[1], c->j1, c->j2, c->ng, p->bc[2], p->bc[3], c->dims >= 2);
-    auto mz = axis_map(c->nt[2], c->k1, c->k2, c->ng, p->bc[4], p->bc[5], c->dims >= 3);
     std::vector<int2> pairs;
-    pairs.reserve((size_t)(c->ncell - (long)p->nx * p->ny * p->nz));
-    for (int k = 0; k < c->nt[2]; ++k)
-      for (int j = 0; j < c->nt[1]; ++j)
-        for (int i = 0; i < c->nt[0]; ++i) {
-          const int si = mx[i], sj = my[j], sk = mz[k];
-          if (si == i && sj == j && sk == k) continue;
-          const long d = ((long)k * c->nt[1] + j) * c->nt[0] + i;
-          const long s = ((long)sk * c->nt[1] + sj) * c->nt[0] + si;
-          pairs.push_back(make_int2((int)d, (int)s));
-        }
+    std::vector<GhostOp> ops;
+    build_ghosts(c->nt, c->dims, c->ng, p->bc, 0, pairs, ops);
     c->nghost = (int)pairs.size();
+    c->ngops = (int)ops.size();
     HIPCHK(c, hipMalloc(&c->ghost, std::max<size_t>(1, pairs.size()) * sizeof(int2)));
     if (!pairs.empty())
       HIPCHK(c, hipMemcpy(c->ghost, pairs.data(), pairs.size() * sizeof(int2), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMalloc(&c->gops, std::max<size_t>(1, ops.size()) * sizeof(GhostOp)));
+    if (!ops.empty())
+      HIPCHK(c, hipMemcpy(c->gops, ops.data(), ops.size() * sizeof(GhostOp), hipMemcpyHostToDevice));
   }
 
   HIPCHK(c, hipMalloc(&c->dev, 8 * sizeof(double)));
@@ -649,6 +659,7 @@ int bnz_destroy(bnz_ctx *c) {
   for (double *q : bufs)
     if (q) (void)hipFree(q);
   if (c->ghost) (void)hipFree(c->ghost);
+  if (c->gops) (void)hipFree(c->gops);
   f32_free(c);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;

@@ -90,37 +90,60 @@ __device__ __forceinline__ float ec32(const float *w, long sv, long c, int comp)
   return -(vx * by - vy * bx);
 }
 
+// Index of the reflecting wall plane of each side (x1, x2, y1, y2, z1, z2),
+// or -1.
+struct Walls {
+  int at[6];
+};
+
 // Edge EMFs, FP32 (interp_ee1, src/integration/ct.pyx:153-215), for a grid
-// of DIMS dimensions.
+// of DIMS dimensions.  On a reflecting wall the two EMF components lying in
+// the wall are set to zero (v_n = 0 and B_n = 0 there, so the tangential E
+// vanishes): the wall face of the normal field, which the boundary fill
+// zeroes after every update, then receives no FP32 round-off from them and
+// div B stays at FP64 round-off (D16; the FP64 path keeps the reference's
+// computed EMFs, which cancel there to FP64 round-off).
 template <int DIMS>
 __global__ __launch_bounds__(BLK) void k32_emf(Geo g, const float *__restrict__ w,
                                                const float *__restrict__ fx,
                                                const float *__restrict__ fy,
                                                const float *__restrict__ fz,
-                                               float *__restrict__ ee, Box box) {
+                                               float *__restrict__ ee, Box box, Walls wl) {
   int i, j, k;
   if (!cell_index<false, 1, 2>(box, 0, i, j, k)) return;
   const long sv = g.sv, sy = g.sy, sz = g.sz;
   const long c = ((long)k * g.nt[1] + j) * g.nt[0] + i;
+  const bool wx = i == wl.at[0] || i == wl.at[1];
+  const bool wy = DIMS >= 2 && (j == wl.at[2] || j == wl.at[3]);
+  const bool wz = DIMS >= 3 && (k == wl.at[4] || k == wl.at[5]);
+  float ex, ey, ez;
   if (DIMS == 3) {
-    ee[c] = 0.5f * (-fy[BZ * sv + c - sz] - fy[BZ * sv + c] + fz[BY * sv + c - sy] + fz[BY * sv + c]) -
-            0.25f * (ec32(w, sv, c - sy, 0) + ec32(w, sv, c, 0) + ec32(w, sv, c - sz - sy, 0) +
-                     ec32(w, sv, c - sz, 0));
-    ee[sv + c] = 0.5f * (-fz[BX * sv + c - 1] - fz[BX * sv + c] + fx[BZ * sv + c - sz] + fx[BZ * sv + c]) -
-                 0.25f * (ec32(w, sv, c - 1, 1) + ec32(w, sv, c, 1) + ec32(w, sv, c - sz - 1, 1) +
-                          ec32(w, sv, c - sz, 1));
+    ex = 0.5f * (-fy[BZ * sv + c - sz] - fy[BZ * sv + c] + fz[BY * sv + c - sy] + fz[BY * sv + c]) -
+         0.25f * (ec32(w, sv, c - sy, 0) + ec32(w, sv, c, 0) + ec32(w, sv, c - sz - sy, 0) +
+                  ec32(w, sv, c - sz, 0));
+    ey = 0.5f * (-fz[BX * sv + c - 1] - fz[BX * sv + c] + fx[BZ * sv + c - sz] + fx[BZ * sv + c]) -
+         0.25f * (ec32(w, sv, c - 1, 1) + ec32(w, sv, c, 1) + ec32(w, sv, c - sz - 1, 1) +
+                  ec32(w, sv, c - sz, 1));
   } else if (DIMS == 2) {
-    ee[c] = -fy[BZ * sv + c];
-    ee[sv + c] = fx[BZ * sv + c];
+    ex = -fy[BZ * sv + c];
+    ey = fx[BZ * sv + c];
   } else {
-    ee[c] = -ec32(w, sv, c, 0);
-    ee[sv + c] = fx[BZ * sv + c];
-    ee[2 * sv + c] = -fx[BY * sv + c];
-    return;
+    ex = -ec32(w, sv, c, 0);
+    ey = fx[BZ * sv + c];
   }
-  ee[2 * sv + c] = 0.5f * (-fx[BY * sv + c - sy] - fx[BY * sv + c] + fy[BX * sv + c - 1] + fy[BX * sv + c]) -
-                   0.25f * (ec32(w, sv, c - 1, 2) + ec32(w, sv, c, 2) + ec32(w, sv, c - sy - 1, 2) +
-                            ec32(w, sv, c - sy, 2));
+  if (DIMS >= 2)
+    ez = 0.5f * (-fx[BY * sv + c - sy] - fx[BY * sv + c] + fy[BX * sv + c - 1] + fy[BX * sv + c]) -
+         0.25f * (ec32(w, sv, c - 1, 2) + ec32(w, sv, c, 2) + ec32(w, sv, c - sy - 1, 2) +
+                  ec32(w, sv, c - sy, 2));
+  else
+    ez = -fx[BY * sv + c];
+  // components lying in a reflecting wall plane
+  if (wy || wz) ex = 0.f;
+  if (wx || wz) ey = 0.f;
+  if (wx || wy) ez = 0.f;
+  ee[c] = ex;
+  ee[sv + c] = ey;
+  ee[2 * sv + c] = ez;
 }
 
 // Face-field update by Stokes (ct.pyx:15-91) in working type R from face
@@ -220,7 +243,8 @@ struct Pred32 {
 };
 
 // Predictor (dt/2): FP32 update of the hi words, FP64 pressure recovery.
-// All operands are loaded up front; only the stores depend on the roles.
+// All operands are loaded up front; only the stores (and the copy of w into
+// ws outside the limits) depend on the roles.
 template <int DIMS>
 __global__ __launch_bounds__(BLK) void k32_update_pred(Geo g, Pred32 a, Lims L,
                                                        const double *__restrict__ dts) {
@@ -233,10 +257,8 @@ __global__ __launch_bounds__(BLK) void k32_update_pred(Geo g, Pred32 a, Lims L,
   const long c = t, sv = g.sv;
   const double dt = 0.5 * dts[0];
   const Roles<DIMS> ro(g, L, i, j, k);
-  float w0[NMODE], uh[NMODE - 3], fxl[NMODE - 3], fxu[NMODE - 3], fyl[NMODE - 3],
-      fyu[NMODE - 3], fzl[NMODE - 3], fzu[NMODE - 3];
-#pragma unroll
-  for (int n = 0; n < NMODE; ++n) w0[n] = a.w32[n * sv + c];
+  float uh[NMODE - 3], fxl[NMODE - 3], fxu[NMODE - 3], fyl[NMODE - 3], fyu[NMODE - 3],
+      fzl[NMODE - 3], fzu[NMODE - 3];
 #pragma unroll
   for (int n = 0; n < NMODE - 3; ++n) {
     uh[n] = a.uh[n * sv + c];
@@ -280,9 +302,9 @@ __global__ __launch_bounds__(BLK) void k32_update_pred(Geo g, Pred32 a, Lims L,
   if (ro.cell) {
 #pragma unroll
     for (int n = 0; n < NMODE; ++n) a.ws[n * sv + c] = (float)wv[n];
-  } else {
+  } else {  // outside the limits ws keeps w (read only here: the thin outer shell)
 #pragma unroll
-    for (int n = 0; n < NMODE; ++n) a.ws[n * sv + c] = w0[n];
+    for (int n = 0; n < NMODE; ++n) a.ws[n * sv + c] = a.w32[n * sv + c];
   }
 }
 
@@ -404,6 +426,29 @@ __global__ __launch_bounds__(BLK) void k32_ghost(Geo g, Ghost32 p, const int2 *_
   for (int n = 0; n < 3; ++n) p.d[n][ds.x] = p.d[n][ds.y];
 }
 
+// Ghost cells a reflecting side touches (GhostOp); the sign bit of slot v
+// applies to conserved slot v of uh/ul (momentum flips with velocity) and to
+// prim slot v of w32.
+__global__ __launch_bounds__(BLK) void k32_ghost_ops(Geo g, Ghost32 p,
+                                                     const GhostOp *__restrict__ ops, int nops) {
+  const long t = xcd_block() * BLK + threadIdx.x;
+  if (t >= nops) return;
+  const GhostOp o = ops[t];
+  if (!(o.bits & (1u << 15))) {
+#pragma unroll
+    for (int n = 0; n < 21; ++n) {
+      const int slot = n < 12 ? n % 6 : n - 12;
+      const float v = p.a[n][o.src];
+      p.a[n][o.dst] = ((o.bits >> slot) & 1u) ? -v : v;
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const double v = ((o.bits >> (12 + a)) & 1u) ? 0. : p.d[a][o.srcf[a]];
+    p.d[a][o.dst] = ((o.bits >> (9 + a)) & 1u) ? -v : v;
+  }
+}
+
 // FP64 prim (all cells) -> FP32-compensated state.
 __global__ __launch_bounds__(BLK) void k32_from64(Geo g, const double *__restrict__ w, float *uh,
                                                   float *ul, float *w32) {
@@ -480,8 +525,14 @@ void emf32(bnz_ctx *c, const float *w, const Lims &L, const char *name) {
   e.lo[1] = (c->dims >= 2) ? L.lo[1] : 0; e.n[1] = (c->dims >= 2) ? L.hi[1] + 1 - L.lo[1] + 1 : 1;
   e.lo[2] = (c->dims >= 3) ? L.lo[2] : 0; e.n[2] = (c->dims >= 3) ? L.hi[2] + 1 - L.lo[2] + 1 : 1;
   Launch t(c, name);
+  Walls wl;
+  const int lo[3] = {c->i1, c->j1, c->k1}, hi[3] = {c->i2, c->j2, c->k2};
+  for (int a = 0; a < 3; ++a) {
+    wl.at[2 * a] = (a < c->dims && c->p.bc[2 * a] == BNZ_BC_REFLECT) ? lo[a] : -1;
+    wl.at[2 * a + 1] = (a < c->dims && c->p.bc[2 * a + 1] == BNZ_BC_REFLECT) ? hi[a] + 1 : -1;
+  }
 #define E32_ARGS \
-  dim3(blocks_for(e.total())), dim3(BLK), 0, c->stream, c->g, w, c->fx32, c->fy32, c->fz32, c->ee32, e
+  dim3(blocks_for(e.total())), dim3(BLK), 0, c->stream, c->g, w, c->fx32, c->fy32, c->fz32, c->ee32, e, wl
   if (c->dims == 3) hipLaunchKernelGGL(k32_emf<3>, E32_ARGS);
   else if (c->dims == 2) hipLaunchKernelGGL(k32_emf<2>, E32_ARGS);
   else hipLaunchKernelGGL(k32_emf<1>, E32_ARGS);
@@ -496,8 +547,12 @@ void ghost32(bnz_ctx *c) {
   for (int n = 0; n < 9; ++n) p.a[q++] = c->w32 + n * c->ncell;
   for (int n = 0; n < 3; ++n) p.d[n] = c->b + n * c->ncell;
   Launch t(c, "ghost");
-  hipLaunchKernelGGL(k32_ghost, dim3(blocks_for(c->nghost)), dim3(BLK), 0, c->stream, c->g, p,
-                     c->ghost, c->nghost);
+  if (c->nghost > 0)
+    hipLaunchKernelGGL(k32_ghost, dim3(blocks_for(c->nghost)), dim3(BLK), 0, c->stream, c->g, p,
+                       c->ghost, c->nghost);
+  if (c->ngops > 0)
+    hipLaunchKernelGGL(k32_ghost_ops, dim3(blocks_for(c->ngops)), dim3(BLK), 0, c->stream, c->g,
+                       p, c->gops, c->ngops);
 }
 
 }  // namespace

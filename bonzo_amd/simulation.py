@@ -145,8 +145,9 @@ class BnzGrid:
         self.bc_flags = [params.get("physics", k, "i", 0)
                          for k in ("bc_x1", "bc_x2", "bc_y1", "bc_y2", "bc_z1", "bc_z2")]
         for f in self.bc_flags:
-            if f not in (0, 1):
-                raise NotImplementedError("only periodic (0) and outflow (1) boundaries are built")
+            if f not in (0, 1, 2):
+                raise NotImplementedError("periodic (0), outflow (1) and reflect (2) boundaries "
+                                          "are built; user boundaries (3) are not")
         self.shape = problems.GridShape(self.Nact[0], self.Nact[1], self.Nact[2], self.ng,
                                         tuple(self.lmin), tuple(self.lmax))
 

@@ -141,7 +141,7 @@ typedef struct bnz_batch bnz_batch;
 
 typedef struct {
   int nx, ny, nz;   /* active cells, as bnz_params                         */
-  int bc[6];        /* BNZ_BC_PERIODIC or BNZ_BC_OUTFLOW per side           */
+  int bc[6];        /* BNZ_BC_PERIODIC, _OUTFLOW or _REFLECT per side       */
   double lmin[3];   /* extent                                              */
   double lmax[3];
   double tmax;      /* [time] tmax of this box                             */

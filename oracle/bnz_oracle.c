@@ -25,6 +25,8 @@
  *       arguments (src/mhd/fluxes_c.c:38-40,208-209); fmax/fmin are used.
  *   2D: advance_b's Bz loop runs k in [0,1] with Ntot_z=1 (ct.pyx:75), an
  *       out-of-bounds write; only k=0 is updated here.
+ *   D15 reflect boundaries follow the evident intent of grid_bc_funcs.pyx
+ *       (see bc_axis).
  */
 #include "bnz_oracle.h"
 
@@ -935,50 +937,96 @@ static void prim2cons_lims(or_sim *s, double *u, const double *w, const int *lim
 
 /* ------------------------------------------------------------------------ */
 /* Ghost fill: src/bc/grid_bc.pyx:612-623 (z, y, x; left then right),        */
-/* periodic copy_layer_* and outflow prolong_* from src/bc/util_bc.pyx.      */
+/* periodic copy_layer_* and outflow prolong_* from src/bc/util_bc.pyx,      */
+/* reflect from grid_bc_funcs.pyx:203-330 with util_bc.pyx:48-90,172-205.    */
 /* ------------------------------------------------------------------------ */
 
-static void bc_axis(or_sim *s, double *A3, int ax, int side)
+/* One side of one axis for array A3, which is prim slot n (face = 0) or face
+   component n (face = 1).
+   Reflect (D15): the shipped reflect functions test an undefined name
+   (bvar[n], grid_bc_funcs.pyx:213) and, through an if / if-else chain
+   (:213-221), overwrite the sign-flipped copy of the normal velocity and
+   normal cell field with an unflipped one.  The evident intent is followed:
+   normal velocity and normal cell-centred field are mirrored with a sign
+   flip, every other cell variable and the transverse face fields mirrored
+   as they are (copy_reflect_layer_*, util_bc.pyx:48-90); the face-normal
+   field is zeroed on the wall face (set_layer_*, util_bc.pyx:172-205) and
+   mirrored about it with a sign flip, over ng faces on the left and ng-1 on
+   the right (:218, :239). */
+static void bc_axis(or_sim *s, double *A3, int ax, int side, int face, int n)
 {
   int ng = s->ng;
   int nx = s->Ntot[0], ny = s->Ntot[1], nz = s->Ntot[2];
   int a1 = (ax == 0) ? s->i1 : (ax == 1) ? s->j1 : s->k1;
   int a2 = (ax == 0) ? s->i2 : (ax == 1) ? s->j2 : s->k2;
   int flag = s->bc[ax][side];
-  int dst0, src0, width = ng, single_src = 0;
-  if (flag == 0) { /* grid_bc_funcs.pyx:36-105 */
-    if (side == 0) { dst0 = a1 - ng; src0 = a2 - ng + 1; }
-    else { dst0 = a2 + 1; src0 = a1; }
-  } else { /* outflow, grid_bc_funcs.pyx:112-196, util_bc.pyx:208-249 (D14) */
-    single_src = 1;
-    if (side == 0) { dst0 = a1 - ng; src0 = a1; }
-    else { dst0 = a2 + 1; src0 = a2; }
-  }
 #define AT(k, j, i) A3[((long)(k) * ny + (j)) * nx + (i)]
-  for (int g = 0; g < width; ++g) {
-    int d = dst0 + g, sidx = single_src ? src0 : src0 + g;
-    if (ax == 0) {
-      for (int k = 0; k < nz; ++k)
-        for (int j = 0; j < ny; ++j) AT(k, j, d) = AT(k, j, sidx);
-    } else if (ax == 1) {
-      for (int k = 0; k < nz; ++k)
-        for (int i = 0; i < nx; ++i) AT(k, d, i) = AT(k, sidx, i);
+#define LAYER(d, expr)                                                      \
+  do {                                                                      \
+    if (ax == 0) {                                                          \
+      for (int k = 0; k < nz; ++k)                                          \
+        for (int j = 0; j < ny; ++j) AT(k, j, d) = (expr(AT(k, j, src)));   \
+    } else if (ax == 1) {                                                   \
+      for (int k = 0; k < nz; ++k)                                          \
+        for (int i = 0; i < nx; ++i) AT(k, d, i) = (expr(AT(k, src, i)));   \
+    } else {                                                                \
+      for (int j = 0; j < ny; ++j)                                          \
+        for (int i = 0; i < nx; ++i) AT(d, j, i) = (expr(AT(src, j, i)));   \
+    }                                                                       \
+  } while (0)
+#define PLUS(x) (x)
+#define MINUS(x) (-1 * (x))
+#define ZERO(x) (0.0)
+  if (flag == 2) {
+    if (face && n == ax) {
+      int src = 0;
+      int wall = side == 0 ? a1 : a2 + 1;
+      LAYER(wall, ZERO);
+      int cnt = side == 0 ? ng : ng - 1;
+      for (int g = 0; g < cnt; ++g) {
+        int d = side == 0 ? a1 - 1 - g : a2 + 2 + g;
+        src = side == 0 ? a1 + 1 + g : a2 - g;
+        LAYER(d, MINUS);
+      }
     } else {
-      for (int j = 0; j < ny; ++j)
-        for (int i = 0; i < nx; ++i) AT(d, j, i) = AT(sidx, j, i);
+      int odd = !face && (n == VX + ax || n == BX + ax);
+      for (int g = 0; g < ng; ++g) {
+        int d = side == 0 ? a1 - 1 - g : a2 + 1 + g;
+        int src = side == 0 ? a1 + g : a2 - g;
+        if (odd) LAYER(d, MINUS);
+        else LAYER(d, PLUS);
+      }
+    }
+  } else {
+    for (int g = 0; g < ng; ++g) {
+      int d, src;
+      if (flag == 0) { /* periodic, grid_bc_funcs.pyx:36-105 */
+        d = side == 0 ? a1 - ng + g : a2 + 1 + g;
+        src = side == 0 ? a2 - ng + 1 + g : a1 + g;
+      } else { /* outflow, grid_bc_funcs.pyx:112-196, util_bc.pyx:208-249 (D14) */
+        d = side == 0 ? a1 - ng + g : a2 + 1 + g;
+        src = side == 0 ? a1 : a2;
+      }
+      LAYER(d, PLUS);
     }
   }
+#undef ZERO
+#undef MINUS
+#undef PLUS
+#undef LAYER
 #undef AT
 }
 
-static void apply_bc_arrays(or_sim *s, double *base, int narr)
+/* narr arrays from base; the first is prim slot (or face component) n0 */
+static void apply_bc_arrays(or_sim *s, double *base, int narr, int face, int n0)
 {
-  for (int n = 0; n < narr; ++n) {
-    double *A3 = base + (long)n * s->ncell;
-    if (s->dims >= 3) { bc_axis(s, A3, 2, 0); bc_axis(s, A3, 2, 1); }
-    if (s->dims >= 2) { bc_axis(s, A3, 1, 0); bc_axis(s, A3, 1, 1); }
-    bc_axis(s, A3, 0, 0);
-    bc_axis(s, A3, 0, 1);
+  for (int q = 0; q < narr; ++q) {
+    double *A3 = base + (long)q * s->ncell;
+    int n = n0 + q;
+    if (s->dims >= 3) { bc_axis(s, A3, 2, 0, face, n); bc_axis(s, A3, 2, 1, face, n); }
+    if (s->dims >= 2) { bc_axis(s, A3, 1, 0, face, n); bc_axis(s, A3, 1, 1, face, n); }
+    bc_axis(s, A3, 0, 0, face, n);
+    bc_axis(s, A3, 0, 1, face, n);
   }
 }
 
@@ -988,8 +1036,8 @@ static void apply_bc_arrays(or_sim *s, double *base, int narr)
    copies of different variables are independent. */
 void or_apply_grid_bc(or_sim *s)
 {
-  apply_bc_arrays(s, s->prim, NMODE);
-  apply_bc_arrays(s, s->bfld, 3);
+  apply_bc_arrays(s, s->prim, NMODE, 0, 0);
+  apply_bc_arrays(s, s->bfld, 3, 1, 0);
 }
 
 /* init_sim after set_problem: src/integration/vanleer.pyx:38-48 */
@@ -999,7 +1047,7 @@ void or_init(or_sim *s)
   int lims_tot[6] = {0, s->Ntot[0] - 1, 0, s->Ntot[1] - 1, 0, s->Ntot[2] - 1};
   or_apply_grid_bc(s);
   interp_bc(s, s->prim, s->bfld, lims_act);
-  apply_bc_arrays(s, s->prim + (long)BX * s->ncell, 3);
+  apply_bc_arrays(s, s->prim + (long)BX * s->ncell, 3, 0, BX);
   prim2cons_lims(s, s->cons, s->prim, lims_tot);
   s->time = 0.;
   s->step = 0;

@@ -142,3 +142,17 @@ def test_tube_sweep_conserves_and_reaches_tmax():
         sk, sj, si = s.active()
         assert np.all(np.isfinite(out[q][0][:, sk, sj, si]))
         assert np.all(out[q][0][0][sk, sj, si] > 0) and np.all(out[q][0][4][sk, sj, si] > 0)
+
+
+def test_batch_reflecting_boxes_bitwise_vs_single():
+    say("case: reflecting blast 40^2, walled 2D state, tube with a wall, mixed flags")
+    from tests.test_oracle_pins import _wall_state
+    rng = np.random.Generator(np.random.PCG64(123))
+    from tests.test_gpu_parity import _blast_guide_field
+    tube = P.riemann_tube(150, rng)
+    tube[2][0][:] = 0.0  # no normal field through the wall
+    cases = [(_blast_guide_field(40), (2,) * 6, 0.02),
+             (_wall_state(24, 20, 0), (2, 2, 0, 0, 0, 0), 0.03),
+             (tube, (2, 1, 0, 0, 0, 0), 0.05),
+             (P.turbulence(12, seed_b=3, seed_v=4), (1, 1, 2, 2, 0, 0), 0.02)]
+    check_against_single(cases)

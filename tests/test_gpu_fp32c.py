@@ -103,3 +103,17 @@ def test_fp32c_init_and_download_round_trip():
     p32, b32 = g.download()
     assert np.array_equal(b32, b64)
     assert np.max(np.abs(p32 - p64)) <= 1e-13 * max(1.0, np.max(np.abs(p64)))
+
+
+def test_reflecting_walls_fp32c():
+    """FP32C ghost fill with reflecting walls (sign flips on hi, lo and FP32
+    prim, zeroed wall faces) against the FP64 path on a smooth 2D state."""
+    from tests.test_oracle_pins import _wall_state
+    say("case: smooth 2D state 64x48, reflecting x walls, 40 steps, fp32c vs fp64")
+    s, out = pair(_wall_state(64, 48, 0), 40, bc=(2, 2, 0, 0, 0, 0))
+    e = errors(s, out)
+    say("  rel L1 per variable: " + " ".join(f"{x:.2e}" for x in e))
+    assert max(e[v] for v in (0, 1, 2, 3, 4, 6, 7, 8, 9, 10, 11)) <= 1e-5
+    assert out["fp32c"]["diag"]["divb_max"] <= 1e-12
+    g = s.ng
+    assert np.all(out["fp32c"]["bfld"][0][0, g:g + 48, g] == 0)

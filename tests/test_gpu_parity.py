@@ -241,3 +241,45 @@ def test_work_shapes_non_cubic():
     for shapes in ((5, 7, 3), (1, 1, 1)):
         assert np.array_equal(res[shapes][0], res[(0, 0, 0)][0])
         assert np.array_equal(res[shapes][1], res[(0, 0, 0)][1])
+
+
+def _blast_guide_field(n):
+    """Blast with a uniform guide field along z only: no field crosses the
+    walls of a closed box (a field normal to a reflecting wall is
+    incompatible with it: the boundary zeroes it on the wall face)."""
+    s, prim, bfld = P.blast2d(n, seed=21)
+    bfld[0][:] = 0.0
+    bfld[1][:] = 0.0
+    bfld[2][s.active()] = 1.0
+    return s, prim, bfld
+
+
+def _tube_tangential_field(n):
+    """Brio-Wu tube with the normal field removed (walls at both ends)."""
+    s, prim, bfld = P.brio_wu(n, axis=0, transverse=0)
+    bfld[0][:] = 0.0
+    return s, prim, bfld
+
+
+@pytest.mark.parametrize("case", ["blast2d_walls", "turb3d_mixed", "tube1d_walls", "wall2d_y"])
+def test_reflecting_boundaries_bitwise(case):
+    """Reflecting walls (mirror ghosts, sign-flipped normal velocity and
+    field, zeroed wall face of the face-normal field) against the oracle,
+    alone and mixed with periodic and outflow sides, on states whose field
+    does not cross the walls."""
+    from tests.test_oracle_pins import _wall_state
+    if case == "blast2d_walls":
+        say("case: blast 2D 48^2 with a z guide field in a reflecting box, 60 steps")
+        res = run_pair(_blast_guide_field(48), steps=60, bc=(2,) * 6)
+    elif case == "turb3d_mixed":
+        say("case: turbulence 16^3, x periodic, y reflect, z outflow, 8 steps")
+        res = run_pair(P.turbulence(16, seed_b=41, seed_v=42), steps=8, bc=(0, 0, 2, 2, 1, 1))
+    elif case == "tube1d_walls":
+        say("case: 1D tube 200 cells, tangential field, reflecting ends, 300 steps")
+        res = run_pair(_tube_tangential_field(200), steps=300, bc=(2, 2, 0, 0, 0, 0))
+    else:
+        say("case: 2D state with walls in y (face-normal field zero there), 20 steps")
+        res = run_pair(_wall_state(24, 20, 1), steps=20, bc=(0, 0, 2, 2, 0, 0))
+    assert compare(res, case)
+    sk, sj, si = res["s"].active()
+    assert np.all(np.isfinite(res["gp"][:, sk, sj, si]))

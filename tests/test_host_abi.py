@@ -92,7 +92,7 @@ def test_batch_create_rejects_bad_selection_without_touching_gpu():
     assert L.bnz_batch_create(ctypes.byref(p), boxes, 2, ctypes.byref(h)) == _lib.BNZ_EUNSUPPORTED
     L.bnz_batch_destroy(h)
     p = _lib.make_params()
-    boxes[1].bc[0] = 2  # reflect
+    boxes[1].bc[0] = 3  # user boundary: not built
     assert L.bnz_batch_create(ctypes.byref(p), boxes, 2, ctypes.byref(h)) == _lib.BNZ_EUNSUPPORTED
     assert b"box 1" in L.bnz_batch_last_error(h)
     L.bnz_batch_destroy(h)

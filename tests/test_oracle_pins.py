@@ -11,6 +11,7 @@ import math
 import numpy as np
 import pytest
 
+from bonzo_amd import problems as P
 from oracle import oracle as orc
 from tests import mathref as M
 
@@ -385,3 +386,106 @@ def test_linear_fast_wave_second_order():
         sk, sj, si = s.active()
         errs.append(np.mean(np.abs(o.prim[0][sk, sj, si] - prim[0][sk, sj, si])))
     assert errs[0] / errs[1] > 3.2 and errs[1] / errs[2] > 3.2
+
+
+def _wall_state(n, m, axis, g=3):
+    """2D state on [0,1]^2, n x m cells, periodic along the other axis, whose
+    face-normal field vanishes on the two walls of ``axis`` (B from a corner
+    vector potential that is zero there)."""
+    s = P.GridShape(n, m, 1, g)
+    prim, bfld = s.empty()
+    sk, sj, si = s.active()
+    X, Y = np.meshgrid(s.centres(0), s.centres(1), indexing="xy")
+    prim[0][0, sj, si] = 1 + 0.5 * np.exp(-((X - .3) ** 2 + (Y - .6) ** 2) / .02)
+    prim[4][0, sj, si] = .5 + 2 * np.exp(-((X - .25) ** 2 + (Y - .4) ** 2) / .01)
+    prim[1][0, sj, si] = .3 * np.sin(2 * np.pi * Y) + .2 * X
+    prim[2][0, sj, si] = .2 * np.cos(2 * np.pi * X) - .1 * Y
+    prim[3][0, sj, si] = .1 * np.sin(2 * np.pi * (X + Y))
+    prim[5][0, sj, si] = X
+    XF, YF = np.meshgrid(np.arange(n + 1) / n, np.arange(m + 1) / m, indexing="xy")
+    if axis == 0:
+        az = .2 * np.sin(np.pi * XF) * np.cos(2 * np.pi * YF)
+        az[:, 0] = 0.0
+        az[:, n] = 0.0
+        az[m, :] = az[0, :]
+    else:
+        az = .2 * np.cos(2 * np.pi * XF) * np.sin(np.pi * YF)
+        az[0, :] = 0.0
+        az[m, :] = 0.0
+        az[:, n] = az[:, 0]
+    bfld[0][0, g:g + m, g:g + n + 1] = (az[1:, :] - az[:-1, :]) * m
+    bfld[1][0, g:g + m + 1, g:g + n] = -(az[:, 1:] - az[:, :-1]) * n
+    bfld[2][0, sj, si] = 0.3 + 0.1 * X
+    return s, prim, bfld
+
+
+def _mirror_double(s, prim, bfld, axis):
+    """The same state on a box twice as long along ``axis``, extended by its
+    mirror image (normal velocity and normal field odd, the rest even)."""
+    g, n, m = s.ng, s.nx, s.ny
+    if axis == 1:  # work on transposed (x <-> y) views
+        def tr(a):
+            return np.swapaxes(a, -1, -2)
+    else:
+        def tr(a):
+            return a
+    nn, mm = (n, m) if axis == 0 else (m, n)
+    d = P.GridShape(2 * n, m, 1, g, lmax=(2., 1., 1.)) if axis == 0 else \
+        P.GridShape(n, 2 * m, 1, g, lmax=(1., 2., 1.))
+    dp, db = d.empty()
+    vn, bnc, fn, ft = (1, 6, 0, 1) if axis == 0 else (2, 7, 1, 0)
+    P_, B_, DP, DB = tr(prim), tr(bfld), tr(dp), tr(db)
+    DP[:, 0, g:g + mm, g:g + nn] = P_[:, 0, g:g + mm, g:g + nn]
+    mir = P_[:, 0, g:g + mm, g:g + nn][:, :, ::-1].copy()
+    mir[vn] *= -1
+    mir[bnc] *= -1
+    DP[:, 0, g:g + mm, g + nn:g + 2 * nn] = mir
+    DB[fn][0, g:g + mm, g:g + nn + 1] = B_[fn][0, g:g + mm, g:g + nn + 1]
+    DB[fn][0, g:g + mm, g + nn + 1:g + 2 * nn + 1] = -B_[fn][0, g:g + mm, g:g + nn][:, ::-1]
+    DB[ft][0, g:g + mm + 1, g:g + nn] = B_[ft][0, g:g + mm + 1, g:g + nn]
+    DB[ft][0, g:g + mm + 1, g + nn:g + 2 * nn] = B_[ft][0, g:g + mm + 1, g:g + nn][:, ::-1]
+    DB[2][0, g:g + mm, g:g + nn] = B_[2][0, g:g + mm, g:g + nn]
+    DB[2][0, g:g + mm, g + nn:g + 2 * nn] = B_[2][0, g:g + mm, g:g + nn][:, ::-1]
+    return d, dp, db
+
+
+@pytest.mark.parametrize("axis", [0, 1])
+def test_reflect_walls_match_mirror_doubled_periodic_box(axis):
+    """A box with reflecting walls on one axis evolves like the periodic box
+    of twice its length holding the state and its mirror image: the ghost
+    map, the sign flips of the normal velocity and field, and the zeroed
+    wall face of the face-normal field must all be right for the two runs to
+    agree to round-off (the mirrored run sums the same terms in another
+    order, so agreement is to round-off, not bitwise)."""
+    n, m, steps = 24, 20, 15
+    s, prim, bfld = _wall_state(n, m, axis)
+    bc = [0] * 6
+    bc[2 * axis] = bc[2 * axis + 1] = 2
+    o = orc.OracleSim(n, m, 1, bc=tuple(bc))
+    o.prim[:] = prim
+    o.bfld[:] = bfld
+    o.init()
+    o.advance_steps(steps)
+    d, dp, db = _mirror_double(s, prim, bfld, axis)
+    od = orc.OracleSim(d.nx, d.ny, 1, lmax=d.lmax)
+    od.prim[:] = dp
+    od.bfld[:] = db
+    od.init()
+    od.advance_steps(steps)
+    g = s.ng
+    sl = (0, slice(g, g + m), slice(g, g + n))
+    for v in range(9):
+        a, b = o.prim[v][sl], od.prim[v][sl]
+        den = max(np.sum(np.abs(b)), 1e-300)
+        assert np.sum(np.abs(a - b)) / den <= 1e-12, (v, np.max(np.abs(a - b)))
+    for c in range(3):
+        a, b = o.bfld[c][sl], od.bfld[c][sl]
+        den = max(np.sum(np.abs(b)), 1e-300)
+        assert np.sum(np.abs(a - b)) / den <= 1e-12, ("face", c)
+    # wall faces of the face-normal field stay exactly zero in the walled box
+    if axis == 0:
+        assert np.all(o.bfld[0][0, g:g + m, g] == 0) and np.all(o.bfld[0][0, g:g + m, g + n] == 0)
+    else:
+        assert np.all(o.bfld[1][0, g, g:g + n] == 0) and np.all(o.bfld[1][0, g + m, g:g + n] == 0)
+    # the run moved the state (so the comparison is not of two initial states)
+    assert np.max(np.abs(o.prim[0][sl] - prim[0][sl])) > 1e-3
