@@ -117,7 +117,10 @@ def run_path(prec, n, prim, bfld, args):
     g.upload(prim, bfld)
     g.init_state()
     tuned, table = (0, 0, 0), {}
-    if prec == "fp64" and not args.no_tune:
+    if args.shapes:
+        tuned = tuple(int(x) for x in args.shapes.split(","))
+        g.set_tuning(*tuned)
+    elif prec == "fp64" and not args.no_tune:
         # tuning steps advance the state: restart from the initial condition
         tuned, table = autotune(g)
         g.upload(prim, bfld)
@@ -213,7 +216,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-tune", action="store_true", help="skip the chunk autotuner")
+    ap.add_argument("--no-tune", action="store_true", help="skip the work-shape autotuner")
+    ap.add_argument("--shapes", default="", metavar="F,E,U",
+                    help="fixed work shapes of the flux, EMF and update kernels (no autotuning)")
     ap.add_argument("--no-fp32c", action="store_true", help="time the FP64 path only")
     ap.add_argument("--accuracy-steps", type=int, default=100,
                     help="steps after which FP32C is compared with FP64 (0: skip)")
@@ -263,7 +268,7 @@ def main():
                                "FP64 plain path, CFL 0.4, seeded k^-2 fields "
                                "(rho=1, p=0.6, Bx=0.2, dB_rms=0.05, Mach 0.5)",
                    "n": n, "precision": "fp64", "rsolver": "hlld", "reconstr": "linear",
-                   "chunks": dict(zip(("flux", "emf", "update"), r64["tuned"]))},
+                   "shapes": dict(zip(("flux", "emf", "update"), r64["tuned"]))},
         "roofline": r64["roofline"],
         "step_roofline": {"bytes_per_cell_update": STEP_BYTES["fp64"],
                           "achieved_GBs": r64["step_GBs"],

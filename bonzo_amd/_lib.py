@@ -20,8 +20,9 @@ PREC = {"fp64": 0, "fp32c": 1}
 ABI_SYMBOLS = (
     "bnz_create", "bnz_destroy", "bnz_last_error", "bnz_get_shape", "bnz_upload",
     "bnz_download", "bnz_init_state", "bnz_fill_ghosts", "bnz_new_dt", "bnz_vl_step",
-    "bnz_advance", "bnz_run_steps", "bnz_get_time", "bnz_set_time", "bnz_diag",
+    "bnz_advance", "bnz_run_steps", "bnz_get_time", "bnz_set_time", "bnz_diag", "bnz_history",
     "bnz_synchronize", "bnz_set_tuning", "bnz_set_kernel_timing", "bnz_kernel_timing",
+    "bnz_state_size", "bnz_save_state", "bnz_load_state",
     "bnz_gpu_flux_pencils",
     "bnz_batch_create", "bnz_batch_destroy", "bnz_batch_last_error", "bnz_batch_box_shape",
     "bnz_batch_upload", "bnz_batch_download", "bnz_batch_init_state", "bnz_batch_advance",
@@ -94,6 +95,10 @@ def load():
     L.bnz_get_time.argtypes = [vp, _dp, ctypes.POINTER(ctypes.c_int), _dp]
     L.bnz_set_time.argtypes = [vp, ctypes.c_double, ctypes.c_int]
     L.bnz_diag.argtypes = [vp, ctypes.POINTER(BnzDiag)]
+    L.bnz_history.argtypes = [vp, _dp]
+    L.bnz_state_size.argtypes = [vp, ctypes.POINTER(ctypes.c_long)]
+    L.bnz_save_state.argtypes = [vp, vp, ctypes.c_long]
+    L.bnz_load_state.argtypes = [vp, vp, ctypes.c_long]
     L.bnz_set_kernel_timing.argtypes = [vp, ctypes.c_int]
     L.bnz_set_tuning.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     L.bnz_kernel_timing.argtypes = [vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_int, _dp,
@@ -230,6 +235,24 @@ class Context:
         return dict(mass=d.mass, mom=tuple(d.mom), energy=d.energy, scalar=d.scalar,
                     divb_max=d.divb_max, bmax=d.bmax, pmin=d.pmin, rhomin=d.rhomin,
                     nonfinite=d.nonfinite, hlld_guards=d.hlld_guards)
+
+    def history(self):
+        """One history row (step, time, 14 cell means; see bnz_history)."""
+        out = np.zeros(16)
+        self._check(load().bnz_history(self._h, _ptr(out)))
+        return out
+
+    def save_state(self):
+        """Exact device state (bnz_save_state) as a uint8 array."""
+        n = ctypes.c_long()
+        self._check(load().bnz_state_size(self._h, ctypes.byref(n)))
+        buf = np.empty(n.value, dtype=np.uint8)
+        self._check(load().bnz_save_state(self._h, buf.ctypes.data, n.value))
+        return buf
+
+    def load_state(self, buf):
+        buf = np.ascontiguousarray(buf, dtype=np.uint8)
+        self._check(load().bnz_load_state(self._h, buf.ctypes.data, buf.size))
 
     def set_tuning(self, flux_chunk=0, emf_chunk=0, update_chunk=0):
         self._check(load().bnz_set_tuning(self._h, flux_chunk, emf_chunk, update_chunk))

@@ -239,10 +239,15 @@ __global__ __launch_bounds__(BLK) void kb_ghost(long sv, double *w, double *b,
                                                 const int2 *__restrict__ pairs, long npairs) {
   for (long t = (long)blockIdx.x * BLK + threadIdx.x; t < npairs; t += (long)gridDim.x * BLK) {
     const int2 ds = pairs[t];  // (ghost cell, source cell), pool indices
+    double v[NMODE + 3];
 #pragma unroll
-    for (int n = 0; n < NMODE; ++n) w[n * sv + ds.x] = w[n * sv + ds.y];
+    for (int n = 0; n < NMODE; ++n) v[n] = w[n * sv + ds.y];
 #pragma unroll
-    for (int n = 0; n < 3; ++n) b[n * sv + ds.x] = b[n * sv + ds.y];
+    for (int n = 0; n < 3; ++n) v[NMODE + n] = b[n * sv + ds.y];
+#pragma unroll
+    for (int n = 0; n < NMODE; ++n) w[n * sv + ds.x] = v[n];
+#pragma unroll
+    for (int n = 0; n < 3; ++n) b[n * sv + ds.x] = v[NMODE + n];
   }
 }
 
@@ -369,12 +374,14 @@ void b_fluxes(bnz_batch *c, const double *w, const double *bf, int kt) {
 }
 
 void b_flux_dispatch(bnz_batch *c, int order, const double *w, const double *bf, int k0) {
-  const bool hlld = c->p.rsolver == BNZ_RS_HLLD;
+  const int rs = c->p.rsolver;
   if (order == 0) {
-    if (hlld) b_fluxes<0, BNZ_RS_HLLD>(c, w, bf, k0);
+    if (rs == BNZ_RS_HLLD) b_fluxes<0, BNZ_RS_HLLD>(c, w, bf, k0);
+    else if (rs == BNZ_RS_HLLT) b_fluxes<0, BNZ_RS_HLLT>(c, w, bf, k0);
     else b_fluxes<0, BNZ_RS_HLL>(c, w, bf, k0);
   } else {
-    if (hlld) b_fluxes<1, BNZ_RS_HLLD>(c, w, bf, k0);
+    if (rs == BNZ_RS_HLLD) b_fluxes<1, BNZ_RS_HLLD>(c, w, bf, k0);
+    else if (rs == BNZ_RS_HLLT) b_fluxes<1, BNZ_RS_HLLT>(c, w, bf, k0);
     else b_fluxes<1, BNZ_RS_HLL>(c, w, bf, k0);
   }
 }
@@ -442,8 +449,8 @@ int bnz_batch_create(const bnz_params *p, const bnz_box *boxes, int nbox, bnz_ba
   c->nbox = nbox;
   if (p->precision != BNZ_PREC_FP64)
     return bfail(c, BNZ_EUNSUPPORTED, "batched boxes run the FP64 step");
-  if (p->rsolver != BNZ_RS_HLLD && p->rsolver != BNZ_RS_HLL)
-    return bfail(c, BNZ_EUNSUPPORTED, "rsolver must be hll or hlld");
+  if (p->rsolver != BNZ_RS_HLLD && p->rsolver != BNZ_RS_HLL && p->rsolver != BNZ_RS_HLLT)
+    return bfail(c, BNZ_EUNSUPPORTED, "rsolver must be hll, hlld or hllt");
   if (p->reconstr != BNZ_RCN_CONST && p->reconstr != BNZ_RCN_LINEAR)
     return bfail(c, BNZ_EUNSUPPORTED, "reconstr must be const or linear");
   if (!(p->gam > 1.) || !(p->cour > 0.)) return bfail(c, BNZ_EINVAL, "need gam > 1 and cour > 0");

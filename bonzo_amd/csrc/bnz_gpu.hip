@@ -87,10 +87,17 @@ __global__ __launch_bounds__(BLK) void k_ghost(Geo g, double *w, double *b,
   if (t >= npairs) return;
   const int2 ds = pairs[t];  // (ghost cell, active source cell)
   const long sv = g.sv;
+  // sources are active cells the fill never writes: load everything, then
+  // store, so the copies do not wait on one another
+  double v[NMODE + 3];
 #pragma unroll
-  for (int n = 0; n < NMODE; ++n) w[n * sv + ds.x] = w[n * sv + ds.y];
+  for (int n = 0; n < NMODE; ++n) v[n] = w[n * sv + ds.y];
 #pragma unroll
-  for (int n = 0; n < 3; ++n) b[n * sv + ds.x] = b[n * sv + ds.y];
+  for (int n = 0; n < 3; ++n) v[NMODE + n] = b[n * sv + ds.y];
+#pragma unroll
+  for (int n = 0; n < NMODE; ++n) w[n * sv + ds.x] = v[n];
+#pragma unroll
+  for (int n = 0; n < 3; ++n) b[n * sv + ds.x] = v[NMODE + n];
 }
 
 // Ghost cells a reflecting side touches (see GhostOp): per-class sources,
@@ -101,18 +108,18 @@ __global__ __launch_bounds__(BLK) void k_ghost_ops(Geo g, double *w, double *b,
   if (t >= nops) return;
   const GhostOp o = ops[t];
   const long sv = g.sv;
-  if (!(o.bits & (1u << 15))) {
+  const bool cells = !(o.bits & (1u << 15));
+  double v[NMODE], f[3];
 #pragma unroll
-    for (int n = 0; n < NMODE; ++n) {
-      const double v = w[n * sv + o.src];
-      w[n * sv + o.dst] = ((o.bits >> n) & 1u) ? -v : v;
-    }
+  for (int n = 0; n < NMODE; ++n) v[n] = cells ? w[n * sv + o.src] : 0.;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) f[a] = ((o.bits >> (12 + a)) & 1u) ? 0. : b[a * sv + o.srcf[a]];
+  if (cells) {
+#pragma unroll
+    for (int n = 0; n < NMODE; ++n) w[n * sv + o.dst] = ((o.bits >> n) & 1u) ? -v[n] : v[n];
   }
 #pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    const double v = ((o.bits >> (12 + a)) & 1u) ? 0. : b[a * sv + o.srcf[a]];
-    b[a * sv + o.dst] = ((o.bits >> (9 + a)) & 1u) ? -v : v;
-  }
+  for (int a = 0; a < 3; ++a) b[a * sv + o.dst] = ((o.bits >> (9 + a)) & 1u) ? -f[a] : f[a];
 }
 
 // interp_bc over the active cells into prim (init only, vanleer.pyx:44)
@@ -231,6 +238,60 @@ __global__ void k_time_advance(double *dev) {
 
 // --------------------------------------------------------------------------
 // Diagnostics: sums and extrema over active cells, one partial per block.
+// History sums over the active cells (get_hst_vars_grid, src/io/output.pyx:
+// 20-138) from u = prim2cons(prim): rho, Ek, Et, Em, E, mx, my, mz, Bx, By,
+// Bz, Bx^2, By^2, Bz^2; one partial per block.
+constexpr int NHIST = 14;
+__global__ __launch_bounds__(BLK) void k_hist(Geo g, const double *__restrict__ w, Box box,
+                                              double *__restrict__ part) {
+  __shared__ double red[NHIST][BLK / 64];
+  const long t = xcd_block() * BLK + threadIdx.x;
+  double v[NHIST];
+#pragma unroll
+  for (int q = 0; q < NHIST; ++q) v[q] = 0.;
+  if (t < box.total()) {
+    int i, j, k;
+    box_idx(box, t, i, j, k);
+    const long c = ((long)k * g.nt[1] + j) * g.nt[0] + i;
+    double wv[NMODE], u[NMODE];
+#pragma unroll
+    for (int n = 0; n < NMODE; ++n) wv[n] = w[n * g.sv + c];
+    prim2cons(u, wv, g.gamm1i);
+    const double rhoi = 1. / u[RHO];
+    const double engk = 0.5 * rhoi * (sqr(u[MX]) + sqr(u[MY]) + sqr(u[MZ]));
+    const double bx2 = sqr(u[BX]), by2 = sqr(u[BY]), bz2 = sqr(u[BZ]);
+    const double engm = 0.5 * (bx2 + by2 + bz2);
+    v[0] = u[RHO];
+    v[1] = engk;
+    v[2] = u[EN] - engk - engm;
+    v[3] = engm;
+    v[4] = u[EN];
+    v[5] = u[MX];
+    v[6] = u[MY];
+    v[7] = u[MZ];
+    v[8] = u[BX];
+    v[9] = u[BY];
+    v[10] = u[BZ];
+    v[11] = bx2;
+    v[12] = by2;
+    v[13] = bz2;
+  }
+#pragma unroll
+  for (int q = 0; q < NHIST; ++q) {
+    double x = v[q];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    if ((threadIdx.x & 63) == 0) red[q][threadIdx.x >> 6] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < NHIST) {
+    double x = 0.;
+#pragma unroll
+    for (int r = 0; r < BLK / 64; ++r) x += red[threadIdx.x][r];
+    part[(long)blockIdx.x * NHIST + threadIdx.x] = x;
+  }
+}
+
 constexpr int NDIAG = 11;  // mass, mx, my, mz, E, psc, divb, bmax, -pmin, -rhomin, nonfinite
 __global__ __launch_bounds__(BLK) void k_diag(Geo g, const double *__restrict__ w,
                                               const double *__restrict__ b, Box box,
@@ -302,8 +363,9 @@ __global__ __launch_bounds__(BLK) void k_pencil(const double *wl_, const double 
   }
   wl[BX] = bx[t];
   wr[BX] = bx[t];
-  int reg = (RS == BNZ_RS_HLLD) ? flux_hlld(f, wl, wr, gam, gamm1i)
-                                : flux_hll(f, wl, wr, gam, gamm1i);
+  int reg = (RS == BNZ_RS_HLLD)   ? flux_hlld(f, wl, wr, gam, gamm1i)
+            : (RS == BNZ_RS_HLLT) ? flux_hllt(f, wl, wr, gam, gamm1i)
+                                  : flux_hll(f, wl, wr, gam, gamm1i);
   if (reg == 0) return;
 #pragma unroll
   for (int q = 0; q < NMODE; ++q) flux[q * n + t] = f[q];
@@ -376,12 +438,14 @@ void launch_fluxes(bnz_ctx *c, const double *w, const double *bf, const Lims &L)
 
 void launch_flux_dispatch(bnz_ctx *c, int order, const double *w, const double *bf,
                           const Lims &L) {
-  const bool hlld = c->p.rsolver == BNZ_RS_HLLD;
+  const int rs = c->p.rsolver;
   if (order == 0) {
-    if (hlld) launch_fluxes<0, BNZ_RS_HLLD>(c, w, bf, L);
+    if (rs == BNZ_RS_HLLD) launch_fluxes<0, BNZ_RS_HLLD>(c, w, bf, L);
+    else if (rs == BNZ_RS_HLLT) launch_fluxes<0, BNZ_RS_HLLT>(c, w, bf, L);
     else launch_fluxes<0, BNZ_RS_HLL>(c, w, bf, L);
   } else {
-    if (hlld) launch_fluxes<1, BNZ_RS_HLLD>(c, w, bf, L);
+    if (rs == BNZ_RS_HLLD) launch_fluxes<1, BNZ_RS_HLLD>(c, w, bf, L);
+    else if (rs == BNZ_RS_HLLT) launch_fluxes<1, BNZ_RS_HLLT>(c, w, bf, L);
     else launch_fluxes<1, BNZ_RS_HLL>(c, w, bf, L);
   }
 }
@@ -513,6 +577,30 @@ void any_step(bnz_ctx *c) {
 // ==========================================================================
 // C ABI
 // ==========================================================================
+// Exact state for checkpoint/resume: a header, then the arrays the next step
+// reads (FP64: prim and face B; FP32C: conserved hi, lo, FP32 prim, face B),
+// all cells.  The header carries time, step, last dt and the CFL maximum the
+// last corrector left for the next step, so a resumed run takes the same dt.
+struct StateHeader {
+  char magic[8];
+  int precision, nt[3];
+  double time, step, dt, cflmax;
+};
+
+namespace {
+struct StatePart {
+  void *dev;
+  size_t bytes;
+};
+std::vector<StatePart> state_parts(bnz_ctx *c) {
+  const size_t n = (size_t)c->ncell;
+  if (c->f32)
+    return {{c->uh, 6 * n * sizeof(float)}, {c->ul, 6 * n * sizeof(float)},
+            {c->w32, 9 * n * sizeof(float)}, {c->b, 3 * n * sizeof(double)}};
+  return {{c->w, 9 * n * sizeof(double)}, {c->b, 3 * n * sizeof(double)}};
+}
+}  // namespace
+
 extern "C" {
 
 const char *bnz_last_error(const bnz_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
@@ -536,8 +624,10 @@ int bnz_create(const bnz_params *p, bnz_ctx **out) {
     return bad(BNZ_EINVAL, "precision must be BNZ_PREC_FP64 or BNZ_PREC_FP32C");
   if (p->rsolver == BNZ_RS_HLLC)
     return bad(BNZ_EUNSUPPORTED, "hllc is a hydro-only solver (fluxes_c.c:230); MHD needs hll or hlld");
-  if (p->rsolver != BNZ_RS_HLLD && p->rsolver != BNZ_RS_HLL)
-    return bad(BNZ_EUNSUPPORTED, "rsolver not implemented on the device (hll, hlld are)");
+  if (p->rsolver != BNZ_RS_HLLD && p->rsolver != BNZ_RS_HLL && p->rsolver != BNZ_RS_HLLT)
+    return bad(BNZ_EUNSUPPORTED, "rsolver not implemented on the device (hll, hlld, hllt are)");
+  if (p->rsolver == BNZ_RS_HLLT && p->precision != BNZ_PREC_FP64)
+    return bad(BNZ_EUNSUPPORTED, "hllt runs in FP64 only");
   if (p->reconstr != BNZ_RCN_CONST && p->reconstr != BNZ_RCN_LINEAR)
     return bad(BNZ_EUNSUPPORTED, "reconstr must be const or linear (weno/parab are commented out in reconstr_c.c)");
   for (int q = 0; q < 6; ++q)
@@ -628,7 +718,8 @@ int bnz_create(const bnz_params *p, bnz_ctx **out) {
       grid.lo[q] = 0;
       grid.n[q] = c->nt[q];
     }
-    c->npart_cap = std::max({(long)blocks_for(a.total()) * NDIAG + 64, (long)blocks_for(c->ncell),
+    c->npart_cap = std::max({(long)blocks_for(a.total()) * std::max(NDIAG, NHIST) + 64,
+                             (long)blocks_for(c->ncell),
                              (long)tiled_blocks(grid, 1, 2)});
   }
   HIPCHK(c, hipMalloc(&c->part, c->npart_cap * sizeof(double)));
@@ -812,6 +903,76 @@ int bnz_set_time(bnz_ctx *c, double time, int step) {
   return BNZ_OK;
 }
 
+int bnz_state_size(const bnz_ctx *c, long *bytes) {
+  if (!c || !bytes) return BNZ_EINVAL;
+  size_t t = sizeof(StateHeader);
+  for (auto &q : state_parts(const_cast<bnz_ctx *>(c))) t += q.bytes;
+  *bytes = (long)t;
+  return BNZ_OK;
+}
+
+int bnz_save_state(bnz_ctx *c, void *host, long bytes) {
+  if (!c || !host) return BNZ_EINVAL;
+  if (!c->ready) return fail(c, BNZ_ESTATE, "bnz_init_state has not run");
+  long need = 0;
+  bnz_state_size(c, &need);
+  if (bytes < need) return fail(c, BNZ_EINVAL, "state buffer too small");
+  StateHeader h;
+  std::memcpy(h.magic, "BNZSTAT1", 8);
+  h.precision = c->p.precision;
+  for (int a = 0; a < 3; ++a) h.nt[a] = c->nt[a];
+  double d[4];
+  HIPCHK(c, hipMemcpyAsync(d, c->dev, sizeof(d), hipMemcpyDeviceToHost, c->stream));
+  h.cflmax = -1.;
+  std::vector<double> parts((size_t)std::max(c->dt_parts, 0));
+  if (c->dt_parts > 0)
+    HIPCHK(c, hipMemcpyAsync(parts.data(), c->part, parts.size() * sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (double x : parts) h.cflmax = std::fmax(h.cflmax, x);
+  h.dt = d[0];
+  h.time = d[1];
+  h.step = d[3];
+  char *out = (char *)host;
+  std::memcpy(out, &h, sizeof(h));
+  out += sizeof(h);
+  for (auto &q : state_parts(c)) {
+    HIPCHK(c, hipMemcpy(out, q.dev, q.bytes, hipMemcpyDeviceToHost));
+    out += q.bytes;
+  }
+  return BNZ_OK;
+}
+
+int bnz_load_state(bnz_ctx *c, const void *host, long bytes) {
+  if (!c || !host) return BNZ_EINVAL;
+  long need = 0;
+  bnz_state_size(c, &need);
+  if (bytes < need) return fail(c, BNZ_EINVAL, "state buffer too small for this context");
+  StateHeader h;
+  std::memcpy(&h, host, sizeof(h));
+  if (std::memcmp(h.magic, "BNZSTAT1", 8) != 0) return fail(c, BNZ_EINVAL, "not a state buffer");
+  if (h.precision != c->p.precision || h.nt[0] != c->nt[0] || h.nt[1] != c->nt[1] ||
+      h.nt[2] != c->nt[2])
+    return fail(c, BNZ_EINVAL, "state was saved by a context of another shape or precision");
+  const char *in = (const char *)host + sizeof(h);
+  for (auto &q : state_parts(c)) {
+    HIPCHK(c, hipMemcpyAsync(q.dev, in, q.bytes, hipMemcpyHostToDevice, c->stream));
+    in += q.bytes;
+  }
+  const double d[4] = {h.dt, h.time, -1., h.step};
+  HIPCHK(c, hipMemcpyAsync(c->dev, d, sizeof(d), hipMemcpyHostToDevice, c->stream));
+  if (h.cflmax >= 0.) {
+    HIPCHK(c, hipMemcpyAsync(c->part, &h.cflmax, sizeof(double), hipMemcpyHostToDevice,
+                             c->stream));
+    c->dt_parts = 1;
+  } else {
+    c->dt_parts = 0;
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->ready = true;
+  return BNZ_OK;
+}
+
 int bnz_diag(bnz_ctx *c, bnz_diag_out *out) {
   if (!c || !out) return BNZ_EINVAL;
   c->dt_parts = 0;  // the partial buffer is reused below
@@ -850,6 +1011,36 @@ int bnz_diag(bnz_ctx *c, bnz_diag_out *out) {
   int ng = 0;
   HIPCHK(c, hipMemcpy(&ng, (int *)(c->dev + 4), sizeof(int), hipMemcpyDeviceToHost));
   out->hlld_guards = ng;
+  return BNZ_OK;
+}
+
+int bnz_history(bnz_ctx *c, double *out16) {
+  if (!c || !out16) return BNZ_EINVAL;
+  if (!c->ready) return fail(c, BNZ_ESTATE, "bnz_init_state has not run");
+  c->dt_parts = 0;  // the partial buffer is reused below
+  if (c->f32) {
+    int rc0 = f32_to64(c);
+    if (rc0) return rc0;
+  }
+  Box a = active_box(c);
+  const int nb = blocks_for(a.total());
+  hipLaunchKernelGGL(k_hist, dim3(nb), dim3(BLK), 0, c->stream, c->g, c->w, a, c->part);
+  int rc = check_launch(c);
+  if (rc) return rc;
+  std::vector<double> h((size_t)nb * NHIST);
+  double d[4];
+  HIPCHK(c, hipMemcpyAsync(h.data(), c->part, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipMemcpyAsync(d, c->dev, sizeof(d), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  double sum[NHIST] = {0};
+  for (int q = 0; q < nb; ++q)
+    for (int v = 0; v < NHIST; ++v) sum[v] += h[(size_t)q * NHIST + v];
+  const double ncells = (double)c->p.nx * c->p.ny * c->p.nz;
+  out16[0] = d[3];
+  out16[1] = d[1];
+  for (int v = 0; v < NHIST; ++v) out16[2 + v] = sum[v] / ncells;
+  for (int q = 13; q < 16; ++q) out16[q] = std::sqrt(std::fabs(out16[q] - out16[q - 3] * out16[q - 3]));
   return BNZ_OK;
 }
 
@@ -915,6 +1106,9 @@ int bnz_gpu_flux_pencils(int rsolver, const double *wl, const double *wr, const 
                        flux, n, gam, gm1i);
   else if (rsolver == BNZ_RS_HLL)
     hipLaunchKernelGGL(k_pencil<BNZ_RS_HLL>, dim3(blocks_for(n)), dim3(BLK), 0, 0, wl, wr, bx,
+                       flux, n, gam, gm1i);
+  else if (rsolver == BNZ_RS_HLLT)
+    hipLaunchKernelGGL(k_pencil<BNZ_RS_HLLT>, dim3(blocks_for(n)), dim3(BLK), 0, 0, wl, wr, bx,
                        flux, n, gam, gm1i);
   else
     return BNZ_EUNSUPPORTED;

@@ -420,10 +420,17 @@ __global__ __launch_bounds__(BLK) void k32_ghost(Geo g, Ghost32 p, const int2 *_
   const long t = xcd_block() * BLK + threadIdx.x;
   if (t >= npairs) return;
   const int2 ds = pairs[t];
+  // sources are active cells the fill never writes: load all, then store
+  float v[21];
+  double f[3];
 #pragma unroll
-  for (int n = 0; n < 21; ++n) p.a[n][ds.x] = p.a[n][ds.y];
+  for (int n = 0; n < 21; ++n) v[n] = p.a[n][ds.y];
 #pragma unroll
-  for (int n = 0; n < 3; ++n) p.d[n][ds.x] = p.d[n][ds.y];
+  for (int n = 0; n < 3; ++n) f[n] = p.d[n][ds.y];
+#pragma unroll
+  for (int n = 0; n < 21; ++n) p.a[n][ds.x] = v[n];
+#pragma unroll
+  for (int n = 0; n < 3; ++n) p.d[n][ds.x] = f[n];
 }
 
 // Ghost cells a reflecting side touches (GhostOp); the sign bit of slot v
@@ -434,19 +441,22 @@ __global__ __launch_bounds__(BLK) void k32_ghost_ops(Geo g, Ghost32 p,
   const long t = xcd_block() * BLK + threadIdx.x;
   if (t >= nops) return;
   const GhostOp o = ops[t];
-  if (!(o.bits & (1u << 15))) {
+  const bool cells = !(o.bits & (1u << 15));
+  float v[21];
+  double f[3];
+#pragma unroll
+  for (int n = 0; n < 21; ++n) v[n] = cells ? p.a[n][o.src] : 0.f;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) f[a] = ((o.bits >> (12 + a)) & 1u) ? 0. : p.d[a][o.srcf[a]];
+  if (cells) {
 #pragma unroll
     for (int n = 0; n < 21; ++n) {
       const int slot = n < 12 ? n % 6 : n - 12;
-      const float v = p.a[n][o.src];
-      p.a[n][o.dst] = ((o.bits >> slot) & 1u) ? -v : v;
+      p.a[n][o.dst] = ((o.bits >> slot) & 1u) ? -v[n] : v[n];
     }
   }
 #pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    const double v = ((o.bits >> (12 + a)) & 1u) ? 0. : p.d[a][o.srcf[a]];
-    p.d[a][o.dst] = ((o.bits >> (9 + a)) & 1u) ? -v : v;
-  }
+  for (int a = 0; a < 3; ++a) p.d[a][o.dst] = ((o.bits >> (9 + a)) & 1u) ? -f[a] : f[a];
 }
 
 // FP64 prim (all cells) -> FP32-compensated state.

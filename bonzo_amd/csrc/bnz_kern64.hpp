@@ -43,6 +43,8 @@ __device__ __forceinline__ void flux_face(const Geo &g, const double *__restrict
   int reg;
   if (RS == BNZ_RS_HLLD)
     reg = flux_hlld(f, wl, wr, g.gam, g.gamm1i);
+  else if (RS == BNZ_RS_HLLT)
+    reg = flux_hllt(f, wl, wr, g.gam, g.gamm1i);
   else
     reg = flux_hll(f, wl, wr, g.gam, g.gamm1i);
   if (reg == 0) return;  // reference leaves the flux row untouched

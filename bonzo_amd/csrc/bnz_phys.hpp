@@ -295,4 +295,89 @@ __device__ __forceinline__ int flux_hlld(T *flux, const T *wL, const T *wR, T ga
   return reg;
 }
 
+// HLLT (src/mhd/fluxes_c.c:447-607): HLL-averaged star density and normal
+// momentum, Alfven fan with the star speeds sM -/+ |bx|/sqrt(rho*), central
+// state from consistency.  The reference never sets the energy and Bx slots
+// of its star states (fluxes_c.c:504-573), so its energy flux in regions 2-4
+// reads uninitialised memory; zero is used for those slots, as in the
+// oracle.  Regions: independent tests, the last true one wins; 0 leaves the
+// flux row untouched.  FP64 only.
+template <typename T>
+__device__ __forceinline__ int flux_hllt(T *flux, const T *wL, const T *wR, T gam, T gamm1i) {
+  const T bx = wL[BX];
+  T uL[NMODE], uR[NMODE], fL[NMODE], fR[NMODE];
+  prim2cons(uL, wL, gamm1i);
+  prim2cons(uR, wR, gamm1i);
+  primcons2flux(fL, wL, uL);
+  primcons2flux(fR, wR, uR);
+  T sL, sR;
+  speeds_simple(sL, sR, wL, wR, gam);
+  const T sRsLi = Ar<T>::div(T(1), sR - sL);
+  const T rhos = (sR * uR[RHO] - sL * uL[RHO] - (fR[RHO] - fL[RHO])) * sRsLi;
+  const T mxs = (sR * uR[MX] - sL * uL[MX] - (fR[MX] - fL[MX])) * sRsLi;
+  const T frhos = (sR * fL[RHO] - sL * fR[RHO] + sR * sL * (uR[RHO] - uL[RHO])) * sRsLi;
+  const T us = Ar<T>::div(frhos, rhos);
+  const T pscs = (sR * uR[PSC] - sL * uL[PSC] - (fR[PSC] - fL[PSC])) * sRsLi;
+  T uLs[NMODE], uRs[NMODE], uCs[NMODE];
+  uLs[RHO] = rhos;
+  uRs[RHO] = rhos;
+  uLs[MX] = mxs;
+  uRs[MX] = mxs;
+  uLs[PSC] = pscs;
+  uRs[PSC] = pscs;
+  uLs[MY] = rhos * wL[VY];
+  uRs[MY] = rhos * wR[VY];
+  uLs[MZ] = rhos * wL[VZ];
+  uRs[MZ] = rhos * wR[VZ];
+  T sLs = us, sRs = us;
+  const T rhosi = Ar<T>::div(T(1), rhos);
+  const T sqrt_rhosi = Ar<T>::root(rhosi);
+  sLs -= fabs(bx) * sqrt_rhosi;
+  sRs += fabs(bx) * sqrt_rhosi;
+  const T xL = Ar<T>::div(T(1), (sL - sLs) * (sL - sRs));
+  const T xR = Ar<T>::div(T(1), (sR - sLs) * (sR - sRs));
+  T yL = bx * xL * (us - wL[VX]);
+  T yR = bx * xR * (us - wR[VX]);
+  uLs[MY] -= yL * wL[BY];
+  uRs[MY] -= yR * wR[BY];
+  uLs[MZ] -= yL * wL[BZ];
+  uRs[MZ] -= yR * wR[BZ];
+  yL = (wL[RHO] * sqr(sL - wL[VX]) - sqr(bx)) * rhosi * xL;
+  yR = (wR[RHO] * sqr(sR - wR[VX]) - sqr(bx)) * rhosi * xR;
+  uLs[BY] = yL * wL[BY];
+  uRs[BY] = yR * wR[BY];
+  uLs[BZ] = yL * wL[BZ];
+  uRs[BZ] = yR * wR[BZ];
+  uCs[RHO] = rhos;
+  uCs[MX] = mxs;
+  uCs[PSC] = pscs;
+  const T sgnb = sign0(bx);
+  const T x = Ar<T>::div(T(0.5) * sgnb, sqrt_rhosi);
+  uCs[MY] = T(0.5) * (uLs[MY] + uRs[MY]) + x * (uRs[BY] - uLs[BY]);
+  uCs[MZ] = T(0.5) * (uLs[MZ] + uRs[MZ]) + x * (uRs[BZ] - uLs[BZ]);
+  uCs[BY] = T(0.5) * (uLs[BY] + uRs[BY]) + x * rhosi * (uRs[MY] - uLs[MY]);
+  uCs[BZ] = T(0.5) * (uLs[BZ] + uRs[BZ]) + x * rhosi * (uRs[MZ] - uLs[MZ]);
+  uLs[EN] = uRs[EN] = uCs[EN] = T(0);
+  uLs[BX] = uRs[BX] = uCs[BX] = T(0);
+  int reg = 0;
+  if (sL > T(0)) reg = 1;
+  if (sL <= T(0) && T(0) < sLs) reg = 2;
+  if (sLs <= T(0) && T(0) < sRs) reg = 3;
+  if (sRs <= T(0) && T(0) < sR) reg = 4;
+  if (sR <= T(0)) reg = 5;
+  if (reg == 0) return 0;
+#pragma unroll
+  for (int n = 0; n < NMODE; ++n) {
+    T v;
+    if (reg == 1) v = fL[n];
+    else if (reg == 2) v = fL[n] + sL * (uLs[n] - uL[n]);
+    else if (reg == 3) v = fL[n] + sL * (uLs[n] - uL[n]) + sLs * (uCs[n] - uLs[n]);
+    else if (reg == 4) v = fR[n] + sR * (uRs[n] - uR[n]);
+    else v = fR[n];
+    flux[n] = v;
+  }
+  flux[BX] = T(0);
+  return reg;
+}
+
 }  // namespace bnz

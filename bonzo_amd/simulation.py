@@ -174,6 +174,7 @@ class Simulation:
 
     def __init__(self, usr_dir=None, config=None, set_problem=None, device=0):
         p = _Params(usr_dir, config)
+        self.usr_dir = usr_dir
         self.integr = BnzIntegr(p)
         self.grid = BnzGrid(p, self.integr)
         rs = {RSolver.RS_HLL: "hll", RSolver.RS_HLLD: "hlld", RSolver.RS_HLLT: "hllt"}[
@@ -182,7 +183,9 @@ class Simulation:
                                 reconstr="const" if self.integr.reconstr_order == 0 else "linear",
                                 bc=self.grid.bc_flags, lmin=self.grid.lmin, lmax=self.grid.lmax,
                                 gam=self.integr.gam, cour=self.integr.cour,
-                                dt_fms_ref=self.integr.dt_fms_ref, device=device)
+                                dt_fms_ref=self.integr.dt_fms_ref,
+                                precision=p.get("computation", "precision", "s", "fp64"),
+                                device=device)
         if set_problem is None:
             name = p.get("problem", "name", "s", "otvortex")
             prim, bfld = PROBLEMS[name](self.grid, p)
@@ -214,6 +217,61 @@ class Simulation:
 
     def diagnostics(self):
         return self.ctx.diag()
+
+    def history(self):
+        """History row of get_hst_vars_grid (src/io/output.pyx:20-138):
+        step, time, cell means of rho, Ek, Et, Em, E, mx, my, mz, Bx, By,
+        Bz and the rms deviations of Bx, By, Bz, computed on the device."""
+        return self.ctx.history()
+
+    def write_history(self, path=None):
+        """Append one history row as write_history does (src/io/output.pyx:
+        221-243): each value as '{:<14e}', one row per call, to
+        ``usr_dir/out/hst.txt`` unless ``path`` is given."""
+        if path is None:
+            if self.usr_dir is None:
+                raise ValueError("no usr_dir: pass the history file path")
+            os.makedirs(os.path.join(self.usr_dir, "out"), exist_ok=True)
+            path = os.path.join(self.usr_dir, "out", "hst.txt")
+        row = self.history()
+        with open(path, "a") as f:
+            for v in row:
+                f.write("{:<14e}".format(v))
+            f.write("\n")
+        return row
+
+    GRID_NAMES = ("rho", "vx", "vy", "vz", "p", "psc", "bxc", "byc", "bzc")
+    FACE_NAMES = ("bxf", "byf", "bzf")
+
+    def save_restart(self, path):
+        """Restart file.  The reference writes the grid datasets and
+        attributes to rst/rst_grid.h5 (src/io/restart.pyx:23-86, dataset
+        names of src/io/output.pyx:248-385); HDF5 is not available here, so
+        an .npz holds the same names (rho ... bzc, bxf, byf, bzf, time, step,
+        dt, xf, yf, zf) and, under 'state', the exact device state
+        (bnz_save_state) that load_restart resumes from bit for bit."""
+        prim, bfld = self.ctx.download()
+        t, step, dt = self.ctx.time()
+        data = {n: prim[q] for q, n in enumerate(self.GRID_NAMES)}
+        data.update({n: bfld[q] for q, n in enumerate(self.FACE_NAMES)})
+        g = self.grid
+        for a, name in enumerate(("xf", "yf", "zf")):
+            n = g.shape.ntot[a]
+            if n == 1:
+                data[name] = np.array([g.lmin[a], g.lmax[a]])
+            else:
+                dx = (g.lmax[a] - g.lmin[a]) / g.Nact[a]
+                data[name] = (np.arange(n + 1) - g.ng) * dx
+        data.update(time=np.float64(t), step=np.int64(step), dt=np.float64(dt),
+                    state=self.ctx.save_state())
+        np.savez(path, **data)
+
+    def load_restart(self, path):
+        """Resume from a file written by save_restart (same grid and
+        precision)."""
+        with np.load(path, allow_pickle=False) as f:
+            self.ctx.load_state(f["state"])
+        self.integr.time, self.integr.step, self.integr.dt = self.ctx.time()
 
     def close(self):
         self.ctx.close()

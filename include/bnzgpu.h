@@ -104,7 +104,24 @@ int bnz_set_time(bnz_ctx *ctx, double time, int step);
    prints the energies; the div B and positivity checks have no reference
    counterpart). */
 int bnz_diag(bnz_ctx *ctx, bnz_diag_out *out);
+/* One history row of get_hst_vars_grid (src/io/output.pyx:20-138, the
+   columns write_history appends to out/hst.txt, :221-243): step, time, then
+   cell means over the active cells of rho, kinetic, thermal, magnetic and
+   total energy density, mx, my, mz, Bx, By, Bz, and the rms deviations
+   sqrt(|<B^2> - <B>^2|) of Bx, By, Bz.  16 doubles. */
+int bnz_history(bnz_ctx *ctx, double *out16);
 int bnz_synchronize(bnz_ctx *ctx);
+
+/* Exact state for checkpoint/resume (the reference writes restart files of
+   the grid data, src/io/restart.pyx:23-86): a header (time, step, last dt,
+   the CFL maximum pending for the next step) and the arrays the next step
+   reads -- FP64: prim and face B; FP32C: conserved hi and lo words, FP32
+   prim and face B -- so that a context that loads it continues bit for bit
+   like the one that saved it.  load_state needs a context of the same
+   shape and precision and replaces upload + init_state. */
+int bnz_state_size(const bnz_ctx *ctx, long *bytes);
+int bnz_save_state(bnz_ctx *ctx, void *host, long bytes);
+int bnz_load_state(bnz_ctx *ctx, const void *host, long bytes);
 
 /* Work shapes of the 3D Godunov-flux, edge-EMF and stage-update kernels:
    0 one thread per face/edge/cell in x-fastest order (the default), 1 tiles

@@ -466,6 +466,35 @@ double *or_bfld(or_sim *s) { return s->bfld; }
 double or_time(const or_sim *s) { return s->time; }
 int or_step(const or_sim *s) { return s->step; }
 double or_last_dt(const or_sim *s) { return s->dt; }
+
+/* get_hst_vars_grid (src/io/output.pyx:20-138) with MFIELD: sums over the
+   active cells of the conserved state, in loop order, divided by the cell
+   count; the last three columns become sqrt(|<B^2> - <B>^2|). */
+void or_history(or_sim *s, double *out)
+{
+  double sum[14];
+  for (int q = 0; q < 14; ++q) sum[q] = 0.;
+  double w[NMODE], u[NMODE];
+  for (int k = s->k1; k <= s->k2; ++k)
+    for (int j = s->j1; j <= s->j2; ++j)
+      for (int i = s->i1; i <= s->i2; ++i) {
+        for (int n = 0; n < NMODE; ++n) w[n] = s->prim[IDX(s, n, k, j, i)];
+        or_prim2cons(u, w, s->gam);
+        double rhoi = 1. / u[RHO];
+        double engk = 0.5 * rhoi * (sqr(u[MX]) + sqr(u[MY]) + sqr(u[MZ]));
+        double bx2 = sqr(u[BX]), by2 = sqr(u[BY]), bz2 = sqr(u[BZ]);
+        double engm = 0.5 * (bx2 + by2 + bz2);
+        double engt = u[EN] - engk - engm;
+        double v[14] = {u[RHO], engk, engt, engm, u[EN], u[MX], u[MY], u[MZ],
+                        u[BX], u[BY], u[BZ], bx2, by2, bz2};
+        for (int q = 0; q < 14; ++q) sum[q] += v[q];
+      }
+  double ncells = (double)s->Nact[0] * s->Nact[1] * s->Nact[2];
+  out[0] = s->step;
+  out[1] = s->time;
+  for (int q = 0; q < 14; ++q) out[2 + q] = sum[q] / ncells;
+  for (int q = 13; q < 16; ++q) out[q] = sqrt(fabs(out[q] - sqr(out[q - 3])));
+}
 void or_set_dt_fms_ref(or_sim *s, int on) { s->dt_fms_ref = on; }
 
 double *or_coord(or_sim *s, int which, int axis)

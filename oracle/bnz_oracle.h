@@ -46,7 +46,7 @@ void or_reconstr_linear(double **wl, double **wr, double ***w, double ***scr, in
 /* ---- whole-grid simulation object ---- */
 typedef struct or_sim or_sim;
 
-/* bc flags per the reference: 0 periodic, 1 outflow (src/bc/grid_bc.pyx:45-50) */
+/* bc flags per the reference: 0 periodic, 1 outflow, 2 reflect (src/bc/grid_bc.pyx:45-50) */
 or_sim *or_create(int nx, int ny, int nz, int reconstr_order,
                   const double *lmin, const double *lmax,
                   double gam, double cour, int rsolver, const int *bc_flags,
@@ -68,6 +68,9 @@ int     or_advance_steps(or_sim *s, int nsteps);
 double  or_time(const or_sim *s);
 int     or_step(const or_sim *s);
 double  or_last_dt(const or_sim *s);
+/* history row of get_hst_vars_grid (src/io/output.pyx:20-138): step, time,
+   cell means of rho, Ek, Et, Em, E, mx, my, mz, Bx, By, Bz, rms Bx, By, Bz */
+void    or_history(or_sim *s, double *out16);
 
 #ifdef __cplusplus
 }

@@ -64,6 +64,8 @@ def lib():
         L.or_step.argtypes = [ctypes.c_void_p]
         L.or_last_dt.restype = ctypes.c_double
         L.or_last_dt.argtypes = [ctypes.c_void_p]
+        L.or_history.argtypes = [ctypes.c_void_p, _dp]
+        L.or_history.restype = None
         for name in ("or_cons2prim", "or_prim2cons"):
             getattr(L, name).argtypes = [_dp, _dp, ctypes.c_double]
         L.or_primcons2flux.argtypes = [_dp, _dp, _dp, ctypes.c_double]
@@ -221,6 +223,11 @@ class OracleSim:
     @property
     def last_dt(self):
         return lib().or_last_dt(self._h)
+
+    def history(self):
+        out = np.zeros(16)
+        lib().or_history(self._h, _ptr(out))
+        return out
 
     def close(self):
         if getattr(self, "_h", None):
