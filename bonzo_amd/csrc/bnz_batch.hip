@@ -18,46 +18,13 @@
 #include <string>
 #include <vector>
 
+#include "bnz_batch.hpp"
 #include "bnz_kern64.hpp"
 
 using namespace bnz;
+using namespace bnzb;
 
 namespace {
-
-// work lists: fluxes (all directions, one item per cell of the union of the
-// face ranges) and EMFs of the predictor and of the corrector, all cells
-// (stage updates), active cells (init and the first CFL maximum)
-enum { KF0 = 0, KEM0, KF1, KEM1, KUPD, KACT, NK };
-
-struct BoxDev {
-  Geo g;         // strides and geometry of the box; g.sv is the pool stride
-  long off;      // first cell of the box in the pools
-  Lims L[2];     // predictor / corrector limits (integrator.pyx:375-383)
-  Box act;       // active cells
-  Box items[NK];
-  int blk0[NK];  // first work block of the box in each list
-};
-
-struct Pools {
-  double *w, *ws, *b, *bs, *fx, *fy, *fz, *ee;
-};
-
-struct BoxState {
-  double *dt, *t, *tmax;
-  unsigned long long *cmax;  // CFL maximum, as the bits of a non-negative double
-  int *run;                  // box advances in the current step
-  int *steps;
-  int *nrun;                 // boxes still short of t_max after the last step
-};
-
-// work list kt as seen by a kernel
-struct WList {
-  const int *bmap;  // work block -> box
-  int nblk;
-  int kt;
-};
-
-constexpr int PERSIST = 256 * 8;  // workgroups of a persistent launch
 
 template <int ORDER, int RS>
 __global__ __launch_bounds__(BLK) void kb_flux(const BoxDev *__restrict__ boxes, WList wl,
@@ -282,28 +249,6 @@ __global__ void kb_zero(unsigned long long *a, int n) {
 // Host side
 // ==========================================================================
 
-struct bnz_batch {
-  bnz_params p{};
-  int nbox = 0, order = 1, ng = 3;
-  std::vector<BoxDev> hbox;
-  int nblk[NK] = {};
-  int bm_off[NK] = {};
-  long ncell = 0, nghost = 0, ngops = 0;
-  hipStream_t stream = nullptr;
-  BoxDev *boxes = nullptr;
-  int *bmap = nullptr;
-  double *geo = nullptr;
-  int2 *ghost = nullptr;
-  GhostOp *gops = nullptr;
-  Pools P{};
-  BoxState S{};
-  double *sbuf = nullptr;      // backing store of BoxState
-  hipGraph_t graph = nullptr;  // two captured steps
-  hipGraphExec_t exec = nullptr;
-  double *graph_b = nullptr;   // current face buffer when the graph was captured
-  bool ready = false;
-  std::string err;
-};
 
 namespace {
 
@@ -363,9 +308,7 @@ Box mkbox(int i0, int ni, int j0, int nj, int k0, int nk) {
   return b;
 }
 
-int grid_for(long nblk) { return (int)std::max<long>(1, std::min<long>(nblk, PERSIST)); }
 
-WList wlist(const bnz_batch *c, int kt) { return WList{c->bmap + c->bm_off[kt], c->nblk[kt], kt}; }
 
 template <int ORDER, int RS>
 void b_fluxes(bnz_batch *c, const double *w, const double *bf, int kt) {
@@ -400,6 +343,11 @@ void b_ghost(bnz_batch *c) {
 void b_step(bnz_batch *c) {
   const int fr = c->p.dt_fms_ref;
   hipLaunchKernelGGL(kb_dt, dim3(1), dim3(BLK), 0, c->stream, c->S, c->nbox, c->p.cour);
+  if (c->f32) {
+    b32_step(c);
+    hipLaunchKernelGGL(kb_time, dim3(1), dim3(BLK), 0, c->stream, c->S, c->nbox);
+    return;
+  }
   // predictor: donor cell, dt/2, limits of order 0
   b_flux_dispatch(c, 0, c->P.w, c->P.b, KF0);
   hipLaunchKernelGGL(kb_emf, dim3(grid_for(c->nblk[KEM0])), dim3(BLK), 0, c->stream, c->boxes,
@@ -447,10 +395,13 @@ int bnz_batch_create(const bnz_params *p, const bnz_box *boxes, int nbox, bnz_ba
   *out = c;
   c->p = *p;
   c->nbox = nbox;
-  if (p->precision != BNZ_PREC_FP64)
-    return bfail(c, BNZ_EUNSUPPORTED, "batched boxes run the FP64 step");
+  if (p->precision != BNZ_PREC_FP64 && p->precision != BNZ_PREC_FP32C)
+    return bfail(c, BNZ_EINVAL, "precision must be BNZ_PREC_FP64 or BNZ_PREC_FP32C");
+  c->f32 = p->precision == BNZ_PREC_FP32C;
   if (p->rsolver != BNZ_RS_HLLD && p->rsolver != BNZ_RS_HLL && p->rsolver != BNZ_RS_HLLT)
     return bfail(c, BNZ_EUNSUPPORTED, "rsolver must be hll, hlld or hllt");
+  if (p->rsolver == BNZ_RS_HLLT && c->f32)
+    return bfail(c, BNZ_EUNSUPPORTED, "hllt runs in FP64 only");
   if (p->reconstr != BNZ_RCN_CONST && p->reconstr != BNZ_RCN_LINEAR)
     return bfail(c, BNZ_EUNSUPPORTED, "reconstr must be const or linear");
   if (!(p->gam > 1.) || !(p->cour > 0.)) return bfail(c, BNZ_EINVAL, "need gam > 1 and cour > 0");
@@ -521,6 +472,10 @@ int bnz_batch_create(const bnz_params *p, const bnz_box *boxes, int nbox, bnz_ba
     d.items[KUPD] = mkbox(0, nt[0], 0, nt[1], 0, nt[2]);
     d.act = mkbox(a1[0], s.nx, a1[1], dims >= 2 ? s.ny : 1, a1[2], dims >= 3 ? s.nz : 1);
     d.items[KACT] = d.act;
+    for (int a = 0; a < 3; ++a) {  // reflecting wall planes (FP32C EMFs)
+      d.walls[2 * a] = (a < dims && s.bc[2 * a] == BNZ_BC_REFLECT) ? a1[a] : -1;
+      d.walls[2 * a + 1] = (a < dims && s.bc[2 * a + 1] == BNZ_BC_REFLECT) ? a2[a] + 1 : -1;
+    }
     build_ghosts(nt, dims, ng, s.bc, off, pairs, gops);
     off += (long)nt[0] * nt[1] * nt[2];
     if (off >= (1L << 31))
@@ -550,13 +505,32 @@ int bnz_batch_create(const bnz_params *p, const bnz_box *boxes, int nbox, bnz_ba
   struct {
     double **q;
     int rows;
-  } bufs[] = {{&c->P.w, NMODE}, {&c->P.ws, NMODE}, {&c->P.b, 3},     {&c->P.bs, 3},
-              {&c->P.fx, NMODE}, {&c->P.fy, NMODE}, {&c->P.fz, NMODE}, {&c->P.ee, 3}};
-  for (auto &q : bufs) {
-    if (hipMalloc(q.q, q.rows * cells * sizeof(double)) != hipSuccess)
+  } bufs[] = {{&c->P.w, NMODE}, {&c->P.b, 3},          {&c->P.bs, 3},
+              {&c->P.ws, NMODE}, {&c->P.fx, NMODE}, {&c->P.fy, NMODE},
+              {&c->P.fz, NMODE}, {&c->P.ee, 3}};
+  // FP32C keeps prim (upload, download) and the faces in FP64; its work
+  // arrays are the FP32 pools below
+  const int nb64 = c->f32 ? 3 : 8;
+  for (int q = 0; q < nb64; ++q) {
+    if (hipMalloc(bufs[q].q, bufs[q].rows * cells * sizeof(double)) != hipSuccess)
       return bfail(c, BNZ_ENOMEM, "device allocation of the box pools failed");
     // zeroed once so reads of never-written ghost faces are defined
-    BCHK(c, hipMemsetAsync(*q.q, 0, q.rows * cells * sizeof(double), c->stream));
+    BCHK(c, hipMemsetAsync(*bufs[q].q, 0, bufs[q].rows * cells * sizeof(double), c->stream));
+  }
+  if (c->f32) {
+    struct {
+      float **q;
+      int rows;
+    } b32[] = {{&c->Q.uh, 6},   {&c->Q.ul, 6},  {&c->Q.w32, 9}, {&c->Q.ws32, 9},
+               {&c->Q.bs32, 3}, {&c->Q.fx, 9},  {&c->Q.fy, 9},  {&c->Q.fz, 9},
+               {&c->Q.ee, 3}};
+    for (auto &q : b32) {
+      if (hipMalloc(q.q, q.rows * cells * sizeof(float)) != hipSuccess)
+        return bfail(c, BNZ_ENOMEM, "device allocation of the FP32 box pools failed");
+      BCHK(c, hipMemsetAsync(*q.q, 0, q.rows * cells * sizeof(float), c->stream));
+    }
+    BCHK(c, hipMalloc(&c->nguard, sizeof(int)));
+    BCHK(c, hipMemsetAsync(c->nguard, 0, sizeof(int), c->stream));
   }
   BCHK(c, hipMalloc(&c->geo, hgeo.size() * sizeof(double)));
   BCHK(c, hipMemcpy(c->geo, hgeo.data(), hgeo.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -606,6 +580,11 @@ int bnz_batch_destroy(bnz_batch *c) {
                     c->P.fy, c->P.fz, c->P.ee, c->geo,  c->sbuf};
   for (double *q : bufs)
     if (q) (void)hipFree(q);
+  float *b32[] = {c->Q.uh, c->Q.ul, c->Q.w32, c->Q.ws32, c->Q.bs32,
+                  c->Q.fx, c->Q.fy, c->Q.fz,  c->Q.ee};
+  for (float *q : b32)
+    if (q) (void)hipFree(q);
+  if (c->nguard) (void)hipFree(c->nguard);
   if (c->boxes) (void)hipFree(c->boxes);
   if (c->bmap) (void)hipFree(c->bmap);
   if (c->ghost) (void)hipFree(c->ghost);
@@ -635,12 +614,20 @@ int bnz_batch_upload(bnz_batch *c, int box, const double *prim, const double *bf
                            hipMemcpyHostToDevice, c->stream));
   BCHK(c, hipStreamSynchronize(c->stream));
   c->ready = false;
+  c->w_stale = false;
   return BNZ_OK;
 }
 
 int bnz_batch_download(bnz_batch *c, int box, double *prim, double *bfld) {
   if (!c || box < 0 || box >= c->nbox) return BNZ_EINVAL;
   if (!c->stream) return bfail(c, BNZ_ESTATE, "batch was not created");
+  if (c->f32 && c->w_stale) {  // FP32C state -> FP64 prim of every box, then its ghosts
+    b32_to64(c);
+    b_ghost(c);
+    int rc = blaunch_check(c);
+    if (rc) return rc;
+    c->w_stale = false;
+  }
   BCHK(c, hipStreamSynchronize(c->stream));
   const BoxDev &d = c->hbox[box];
   const size_t n = (size_t)d.g.nt[0] * d.g.nt[1] * d.g.nt[2];
@@ -667,8 +654,12 @@ int bnz_batch_init_state(bnz_batch *c) {
   BCHK(c, hipMemcpyAsync(c->S.steps, zi.data(), (nb + 1) * sizeof(int), hipMemcpyHostToDevice,
                          c->stream));
   hipLaunchKernelGGL(kb_zero, dim3(1), dim3(BLK), 0, c->stream, c->S.cmax, nb);
-  hipLaunchKernelGGL(kb_cfl, dim3(grid_for(c->nblk[KACT])), dim3(BLK), 0, c->stream, c->boxes,
-                     wlist(c, KACT), c->S, c->P.w, c->p.dt_fms_ref);
+  if (c->f32)  // FP32C state from the FP64 prim, first CFL maxima from its FP32 prim
+    b32_init(c);
+  else
+    hipLaunchKernelGGL(kb_cfl, dim3(grid_for(c->nblk[KACT])), dim3(BLK), 0, c->stream,
+                       c->boxes, wlist(c, KACT), c->S, c->P.w, c->p.dt_fms_ref);
+  c->w_stale = false;
   int rc = blaunch_check(c);
   if (rc) return rc;
   BCHK(c, hipStreamSynchronize(c->stream));
@@ -714,6 +705,7 @@ int bnz_batch_advance(bnz_batch *c, int max_steps, int use_graph, int *nsteps) {
     BCHK(c, hipStreamSynchronize(c->stream));
   }
   if (nsteps) *nsteps = n;
+  c->w_stale = c->f32;
   return BNZ_OK;
 }
 

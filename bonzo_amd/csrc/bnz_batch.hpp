@@ -1,0 +1,96 @@
+// bnz_batch.hpp -- types shared by the FP64 (bnz_batch.hip) and
+// FP32-compensated (bnz_batch32.hip) translation units of the batched-box
+// step: box descriptors, work lists, pools and the batch object.
+#pragma once
+#include <string>
+#include <vector>
+
+#include "bnz_common.hpp"
+
+namespace bnzb {
+using namespace bnz;
+
+// work lists: fluxes (all directions, one item per cell of the union of the
+// face ranges) and EMFs of the predictor and of the corrector, all cells
+// (stage updates), active cells (init and the first CFL maximum)
+enum { KF0 = 0, KEM0, KF1, KEM1, KUPD, KACT, NK };
+
+struct BoxDev {
+  Geo g;         // strides and geometry of the box; g.sv is the pool stride
+  long off;      // first cell of the box in the pools
+  Lims L[2];     // predictor / corrector limits (integrator.pyx:375-383)
+  Box act;       // active cells
+  Box items[NK];
+  int blk0[NK];  // first work block of the box in each list
+  int walls[6];  // reflecting wall plane of each side, or -1 (FP32C EMFs)
+};
+
+struct Pools {
+  double *w, *ws, *b, *bs, *fx, *fy, *fz, *ee;
+};
+
+// FP32-compensated pools (face fields stay in Pools::b / bs, FP64)
+struct Pools32 {
+  float *uh, *ul, *w32, *ws32, *bs32, *fx, *fy, *fz, *ee;
+};
+
+struct BoxState {
+  double *dt, *t, *tmax;
+  unsigned long long *cmax;  // CFL maximum, as the bits of a non-negative double
+  int *run;                  // box advances in the current step
+  int *steps;
+  int *nrun;                 // boxes still short of t_max after the last step
+};
+
+// work list kt as seen by a kernel
+struct WList {
+  const int *bmap;  // work block -> box
+  int nblk;
+  int kt;
+};
+
+constexpr int PERSIST = 256 * 8;  // workgroups of a persistent launch
+
+inline int grid_for(long nblk) { return (int)std::max<long>(1, std::min<long>(nblk, PERSIST)); }
+
+}  // namespace bnzb
+
+struct bnz_batch {
+  bnz_params p{};
+  int nbox = 0, order = 1, ng = 3;
+  std::vector<bnzb::BoxDev> hbox;
+  int nblk[bnzb::NK] = {};
+  int bm_off[bnzb::NK] = {};
+  long ncell = 0, nghost = 0, ngops = 0;
+  hipStream_t stream = nullptr;
+  bnzb::BoxDev *boxes = nullptr;
+  int *bmap = nullptr;
+  double *geo = nullptr;
+  int2 *ghost = nullptr;
+  bnz::GhostOp *gops = nullptr;
+  bnzb::Pools P{};
+  bool f32 = false;
+  bnzb::Pools32 Q{};
+  int *nguard = nullptr;  // guarded FP32 HLLD faces
+  bool w_stale = false;   // FP32C: the FP64 prim pool lags the state
+  bnzb::BoxState S{};
+  double *sbuf = nullptr;      // backing store of BoxState
+  hipGraph_t graph = nullptr;  // two captured steps
+  hipGraphExec_t exec = nullptr;
+  double *graph_b = nullptr;   // current face buffer when the graph was captured
+  bool ready = false;
+  std::string err;
+};
+
+namespace bnzb {
+
+inline WList wlist(const bnz_batch *c, int kt) {
+  return WList{c->bmap + c->bm_off[kt], c->nblk[kt], kt};
+}
+
+// FP32-compensated step (bnz_batch32.hip)
+void b32_step(bnz_batch *c);
+void b32_init(bnz_batch *c);   // FP64 prim pool -> FP32C state; first CFL maxima
+void b32_to64(bnz_batch *c);   // FP32C state -> FP64 prim pool (active cells)
+
+}  // namespace bnzb

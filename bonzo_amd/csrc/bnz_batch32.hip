@@ -1,0 +1,262 @@
+// bnz_batch32.hip -- FP32-compensated kernels of the batched-box step.
+//
+// The same work lists, box descriptors and per-box dt / t / run flags as the
+// FP64 batch (bnz_batch.hip), with the per-item bodies of the single-grid
+// FP32C step (bnz_kern32.hpp), so that a box of an FP32C batch follows
+// exactly the arithmetic of an FP32C single grid.  Built, like
+// bnz_gpu32.hip, with FMA contraction allowed.
+#include <hip/hip_runtime.h>
+
+#include "bnz_batch.hpp"
+#include "bnz_kern32.hpp"
+
+using namespace bnz;
+using namespace bnzb;
+
+namespace {
+
+__device__ __forceinline__ Walls box_walls(const BoxDev &d) {
+  Walls wl;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) wl.at[q] = d.walls[q];
+  return wl;
+}
+
+template <int ORDER, int RS, typename BF>
+__global__ __launch_bounds__(BLK) void kb32_flux(const BoxDev *__restrict__ boxes, WList wl,
+                                                 BoxState st, const float *__restrict__ w,
+                                                 const BF *__restrict__ bf,
+                                                 float *__restrict__ fx, float *__restrict__ fy,
+                                                 float *__restrict__ fz, int *nguard) {
+  int ng = 0;
+  for (int wb = blockIdx.x; wb < wl.nblk; wb += gridDim.x) {
+    const int b = wl.bmap[wb];
+    if (!st.run[b]) continue;
+    const BoxDev &d = boxes[b];
+    const long t = (long)(wb - d.blk0[wl.kt]) * BLK + threadIdx.x;
+    const Box &it = d.items[wl.kt];
+    if (t >= it.total()) continue;
+    int i, j, k;
+    box_idx(it, t, i, j, k);
+    const long o = d.off;
+    flux32_cell<ORDER, RS>(d.g, w + o, bf + o, fx + o, fy + o, fz + o,
+                           d.L[wl.kt == KF0 ? 0 : 1], i, j, k, ng);
+  }
+  if (ng) atomicAdd(nguard, ng);
+}
+
+__global__ __launch_bounds__(BLK) void kb32_emf(const BoxDev *__restrict__ boxes, WList wl,
+                                                BoxState st, const float *__restrict__ w,
+                                                const float *__restrict__ fx,
+                                                const float *__restrict__ fy,
+                                                const float *__restrict__ fz,
+                                                float *__restrict__ ee) {
+  for (int wb = blockIdx.x; wb < wl.nblk; wb += gridDim.x) {
+    const int b = wl.bmap[wb];
+    if (!st.run[b]) continue;
+    const BoxDev &d = boxes[b];
+    const long t = (long)(wb - d.blk0[wl.kt]) * BLK + threadIdx.x;
+    const Box &it = d.items[wl.kt];
+    if (t >= it.total()) continue;
+    int i, j, k;
+    box_idx(it, t, i, j, k);
+    const long o = d.off;
+    const Walls wa = box_walls(d);
+    if (d.g.dims == 3)
+      emf32_edge<3>(d.g, w + o, fx + o, fy + o, fz + o, ee + o, wa, i, j, k);
+    else if (d.g.dims == 2)
+      emf32_edge<2>(d.g, w + o, fx + o, fy + o, fz + o, ee + o, wa, i, j, k);
+    else
+      emf32_edge<1>(d.g, w + o, fx + o, fy + o, fz + o, ee + o, wa, i, j, k);
+  }
+}
+
+// Stage update of every cell of the running boxes (update32_pred /
+// update32_corr); the corrector folds the CFL maximum into the box's cmax
+// and carries the faces of a finished box into the buffer that becomes
+// current after the face-buffer swap.
+template <bool PRED>
+__global__ __launch_bounds__(BLK) void kb32_update(const BoxDev *__restrict__ boxes, WList wl,
+                                                   BoxState st, Pools P, Pools32 Q,
+                                                   int fms_ref) {
+  for (int wb = blockIdx.x; wb < wl.nblk; wb += gridDim.x) {
+    const int b = wl.bmap[wb];
+    const BoxDev &d = boxes[b];
+    const long t = (long)(wb - d.blk0[wl.kt]) * BLK + threadIdx.x;
+    const Box &it = d.items[wl.kt];
+    const bool valid = t < it.total();
+    const long o = d.off;
+    if (!st.run[b]) {
+      if (!PRED && valid) {
+        const long sv = d.g.sv;
+#pragma unroll
+        for (int n = 0; n < 3; ++n) P.bs[n * sv + o + t] = P.b[n * sv + o + t];
+      }
+      continue;
+    }
+    double cds = 0.;
+    if (valid) {
+      int i, j, k;
+      box_idx(it, t, i, j, k);
+      if (PRED) {
+        const Pred32 a{Q.w32 + o, Q.uh + o, P.b + o, Q.fx + o, Q.fy + o,
+                       Q.fz + o,  Q.ee + o, Q.bs32 + o, Q.ws32 + o};
+        if (d.g.dims == 3) update32_pred<3>(d.g, a, d.L[0], st.dt + b, i, j, k);
+        else if (d.g.dims == 2) update32_pred<2>(d.g, a, d.L[0], st.dt + b, i, j, k);
+        else update32_pred<1>(d.g, a, d.L[0], st.dt + b, i, j, k);
+      } else {
+        const Corr32 a{Q.uh + o, Q.ul + o, P.b + o, Q.fx + o, Q.fy + o,
+                       Q.fz + o, Q.ee + o, P.bs + o, Q.w32 + o};
+        if (d.g.dims == 3)
+          cds = update32_corr<3>(d.g, a, d.L[1], st.dt + b, d.act, fms_ref, i, j, k);
+        else if (d.g.dims == 2)
+          cds = update32_corr<2>(d.g, a, d.L[1], st.dt + b, d.act, fms_ref, i, j, k);
+        else
+          cds = update32_corr<1>(d.g, a, d.L[1], st.dt + b, d.act, fms_ref, i, j, k);
+      }
+    }
+    if (!PRED) {
+      cds = wave_max(cds);
+      if ((threadIdx.x & 63) == 0 && cds > 0.)
+        atomicMax(st.cmax + b, (unsigned long long)__double_as_longlong(cds));
+    }
+  }
+}
+
+// FP64 prim -> FP32C state over every cell of every box (init).
+__global__ __launch_bounds__(BLK) void kb32_from64(const BoxDev *__restrict__ boxes, WList wl,
+                                                   Pools P, Pools32 Q) {
+  for (int wb = blockIdx.x; wb < wl.nblk; wb += gridDim.x) {
+    const BoxDev &d = boxes[wl.bmap[wb]];
+    const long t = (long)(wb - d.blk0[wl.kt]) * BLK + threadIdx.x;
+    if (t >= d.items[wl.kt].total()) continue;
+    const long o = d.off;
+    from64_cell(d.g, P.w + o, Q.uh + o, Q.ul + o, Q.w32 + o, t);
+  }
+}
+
+// CFL maximum of the FP32 prims over the active cells (first step).
+__global__ __launch_bounds__(BLK) void kb32_cfl(const BoxDev *__restrict__ boxes, WList wl,
+                                                BoxState st, const float *__restrict__ w32,
+                                                int fms_ref) {
+  for (int wb = blockIdx.x; wb < wl.nblk; wb += gridDim.x) {
+    const int b = wl.bmap[wb];
+    const BoxDev &d = boxes[b];
+    const long t = (long)(wb - d.blk0[wl.kt]) * BLK + threadIdx.x;
+    const Box &it = d.items[wl.kt];
+    double cds = 0.;
+    if (t < it.total()) {
+      int i, j, k;
+      box_idx(it, t, i, j, k);
+      cds = cfl32_cell(d.g, w32 + d.off, fms_ref, i, j, k);
+    }
+    cds = wave_max(cds);
+    if ((threadIdx.x & 63) == 0 && cds > 0.)
+      atomicMax(st.cmax + b, (unsigned long long)__double_as_longlong(cds));
+  }
+}
+
+// FP32C state -> FP64 prim of the active cells of every box (download).
+__global__ __launch_bounds__(BLK) void kb32_to64(const BoxDev *__restrict__ boxes, WList wl,
+                                                 Pools P, Pools32 Q) {
+  for (int wb = blockIdx.x; wb < wl.nblk; wb += gridDim.x) {
+    const BoxDev &d = boxes[wl.bmap[wb]];
+    const long t = (long)(wb - d.blk0[wl.kt]) * BLK + threadIdx.x;
+    const Box &it = d.items[wl.kt];
+    if (t >= it.total()) continue;
+    int i, j, k;
+    box_idx(it, t, i, j, k);
+    const long o = d.off;
+    to64_cell(d.g, Q.uh + o, Q.ul + o, P.b + o, P.w + o, i, j, k);
+  }
+}
+
+__global__ __launch_bounds__(BLK) void kb32_ghost(Ghost32 p, const int2 *__restrict__ pairs,
+                                                  long npairs) {
+  for (long t = (long)blockIdx.x * BLK + threadIdx.x; t < npairs; t += (long)gridDim.x * BLK) {
+    const int2 ds = pairs[t];
+    ghost32_pair(p, ds.x, ds.y);
+  }
+}
+
+__global__ __launch_bounds__(BLK) void kb32_ghost_ops(Ghost32 p, const GhostOp *__restrict__ ops,
+                                                      long nops) {
+  for (long t = (long)blockIdx.x * BLK + threadIdx.x; t < nops; t += (long)gridDim.x * BLK)
+    ghost32_op(p, ops[t]);
+}
+
+template <int ORDER, int RS, typename BF>
+void flux_launch(bnz_batch *c, const float *w, const BF *bf, int kt) {
+  hipLaunchKernelGGL((kb32_flux<ORDER, RS, BF>), dim3(grid_for(c->nblk[kt])), dim3(BLK), 0,
+                     c->stream, c->boxes, wlist(c, kt), c->S, w, bf, c->Q.fx, c->Q.fy, c->Q.fz,
+                     c->nguard);
+}
+
+template <typename BF>
+void fluxes(bnz_batch *c, int order, const float *w, const BF *bf, int kt) {
+  const int rs = c->p.rsolver;
+  if (order == 0) {
+    if (rs == BNZ_RS_HLLD) flux_launch<0, BNZ_RS_HLLD>(c, w, bf, kt);
+    else flux_launch<0, BNZ_RS_HLL>(c, w, bf, kt);
+  } else {
+    if (rs == BNZ_RS_HLLD) flux_launch<1, BNZ_RS_HLLD>(c, w, bf, kt);
+    else flux_launch<1, BNZ_RS_HLL>(c, w, bf, kt);
+  }
+}
+
+void emfs(bnz_batch *c, const float *w, int kt) {
+  hipLaunchKernelGGL(kb32_emf, dim3(grid_for(c->nblk[kt])), dim3(BLK), 0, c->stream, c->boxes,
+                     wlist(c, kt), c->S, w, c->Q.fx, c->Q.fy, c->Q.fz, c->Q.ee);
+}
+
+Ghost32 ghost_arrays(bnz_batch *c) {
+  Ghost32 p;
+  int q = 0;
+  const long n = c->ncell;
+  for (int v = 0; v < 6; ++v) p.a[q++] = c->Q.uh + v * n;
+  for (int v = 0; v < 6; ++v) p.a[q++] = c->Q.ul + v * n;
+  for (int v = 0; v < 9; ++v) p.a[q++] = c->Q.w32 + v * n;
+  for (int v = 0; v < 3; ++v) p.d[v] = c->P.b + v * n;
+  return p;
+}
+
+}  // namespace
+
+namespace bnzb {
+
+// The FP32C stages of one VL step of every running box, between the dt
+// and time kernels of the caller (the sequence of the single-grid
+// f32_step_body).
+void b32_step(bnz_batch *c) {
+  const int fr = c->p.dt_fms_ref;
+  fluxes(c, 0, c->Q.w32, (const double *)c->P.b, KF0);
+  emfs(c, c->Q.w32, KEM0);
+  hipLaunchKernelGGL(kb32_update<true>, dim3(grid_for(c->nblk[KUPD])), dim3(BLK), 0, c->stream,
+                     c->boxes, wlist(c, KUPD), c->S, c->P, c->Q, fr);
+  fluxes(c, c->order, c->Q.ws32, (const float *)c->Q.bs32, KF1);
+  emfs(c, c->Q.ws32, KEM1);
+  hipLaunchKernelGGL(kb32_update<false>, dim3(grid_for(c->nblk[KUPD])), dim3(BLK), 0,
+                     c->stream, c->boxes, wlist(c, KUPD), c->S, c->P, c->Q, fr);
+  std::swap(c->P.b, c->P.bs);
+  const Ghost32 p = ghost_arrays(c);
+  if (c->nghost > 0)
+    hipLaunchKernelGGL(kb32_ghost, dim3(grid_for((c->nghost + BLK - 1) / BLK)), dim3(BLK), 0,
+                       c->stream, p, c->ghost, c->nghost);
+  if (c->ngops > 0)
+    hipLaunchKernelGGL(kb32_ghost_ops, dim3(grid_for((c->ngops + BLK - 1) / BLK)), dim3(BLK), 0,
+                       c->stream, p, c->gops, c->ngops);
+}
+
+void b32_init(bnz_batch *c) {
+  hipLaunchKernelGGL(kb32_from64, dim3(grid_for(c->nblk[KUPD])), dim3(BLK), 0, c->stream,
+                     c->boxes, wlist(c, KUPD), c->P, c->Q);
+  hipLaunchKernelGGL(kb32_cfl, dim3(grid_for(c->nblk[KACT])), dim3(BLK), 0, c->stream, c->boxes,
+                     wlist(c, KACT), c->S, c->Q.w32, c->p.dt_fms_ref);
+}
+
+void b32_to64(bnz_batch *c) {
+  hipLaunchKernelGGL(kb32_to64, dim3(grid_for(c->nblk[KACT])), dim3(BLK), 0, c->stream,
+                     c->boxes, wlist(c, KACT), c->P, c->Q);
+}
+
+}  // namespace bnzb

@@ -1,0 +1,464 @@
+// bnz_kern32.hpp -- per-face, per-edge and per-cell bodies of the
+// FP32-compensated VL step, shared by the single-grid kernels
+// (bnz_gpu32.hip) and the batched kernels (bnz_batch.hip), so that a box of
+// a batch follows exactly the arithmetic of a single grid.  Compiled with
+// FMA contraction allowed (the FP32C translation units).
+#pragma once
+#include "bnz_common.hpp"
+
+namespace bnz {
+
+__device__ __forceinline__ void split(double v, float &hi, float &lo) {
+  hi = (float)v;
+  lo = (float)(v - (double)hi);
+}
+__device__ __forceinline__ double join(float hi, float lo) { return (double)hi + (double)lo; }
+
+
+// Godunov flux of one face, FP32 (same stencil as the FP64 flux_face); ng
+// counts guarded HLLD star denominators.
+template <int DIR, int ORDER, int RS, typename BF>
+__device__ __forceinline__ void flux32_face(const Geo &g, const float *__restrict__ w,
+                                            const BF *__restrict__ bf, float *__restrict__ fl,
+                                            int i, int j, int k, int &ng) {
+  const long c = ((long)k * g.nt[1] + j) * g.nt[0] + i;
+  const long sd = (DIR == 0) ? 1 : (DIR == 1) ? g.sy : g.sz;
+  const float bn = (float)bf[(long)DIR * g.sv + c];
+  float wl[NMODE], wr[NMODE];
+#pragma unroll
+  for (int n = 0; n < NMODE; ++n) {
+    if (n == BX) continue;
+    const float *p = w + (long)perm<DIR>(n) * g.sv + c;
+    if (ORDER == 0) {
+      wl[n] = p[-sd];
+      wr[n] = p[0];
+    } else {
+      const float wm2 = p[-2 * sd], wm1 = p[-sd], w0 = p[0], wp1 = p[sd];
+      wl[n] = wm1 + 0.5f * vl_lim(wm1 - wm2, w0 - wm1);
+      wr[n] = w0 - 0.5f * vl_lim(w0 - wm1, wp1 - w0);
+    }
+  }
+  wl[BX] = bn;
+  wr[BX] = bn;
+  float f[NMODE];
+  const float gam = (float)g.gam, gm1i = (float)g.gamm1i;
+  const int reg = (RS == BNZ_RS_HLLD) ? flux_hlld(f, wl, wr, gam, gm1i, &ng)
+                                      : flux_hll(f, wl, wr, gam, gm1i);
+  if (reg == 0) return;
+#pragma unroll
+  for (int n = 0; n < NMODE; ++n) {
+    if (n == BX) continue;
+    fl[(long)perm<DIR>(n) * g.sv + c] = f[n];
+  }
+}
+
+
+// All face fluxes of one stage whose lower cell is (i, j, k) (as the FP64
+// flux_cell); ng counts guarded HLLD star denominators.
+template <int ORDER, int RS, typename BF>
+__device__ __forceinline__ void flux32_cell(const Geo &g, const float *__restrict__ w,
+                                            const BF *__restrict__ bf, float *__restrict__ fx,
+                                            float *__restrict__ fy, float *__restrict__ fz,
+                                            const Lims &L, int i, int j, int k, int &ng) {
+  if (i >= L.lo[0]) flux32_face<0, ORDER, RS>(g, w, bf, fx, i, j, k, ng);
+  if (g.dims >= 2 && j >= L.lo[1]) flux32_face<1, ORDER, RS>(g, w, bf, fy, i, j, k, ng);
+  if (g.dims >= 3 && k >= L.lo[2]) flux32_face<2, ORDER, RS>(g, w, bf, fz, i, j, k, ng);
+}
+
+
+__device__ __forceinline__ float ec32(const float *w, long sv, long c, int comp) {
+  const float vx = w[VX * sv + c], vy = w[VY * sv + c], vz = w[VZ * sv + c];
+  const float bx = w[BX * sv + c], by = w[BY * sv + c], bz = w[BZ * sv + c];
+  if (comp == 0) return -(vy * bz - vz * by);
+  if (comp == 1) return (vx * bz - vz * bx);
+  return -(vx * by - vy * bx);
+}
+
+// Index of the reflecting wall plane of each side (x1, x2, y1, y2, z1, z2),
+// or -1.
+struct Walls {
+  int at[6];
+};
+
+
+// Edge EMFs, FP32 (interp_ee1, src/integration/ct.pyx:153-215), for a grid
+// of DIMS dimensions.  On a reflecting wall the two EMF components lying in
+// the wall are set to zero (v_n = 0 and B_n = 0 there, so the tangential E
+// vanishes): the wall face of the normal field, which the boundary fill
+// zeroes after every update, then receives no FP32 round-off from them and
+// div B stays at FP64 round-off (D16; the FP64 path keeps the reference's
+// computed EMFs, which cancel there to FP64 round-off).
+template <int DIMS>
+__device__ __forceinline__ void emf32_edge(const Geo &g, const float *__restrict__ w,
+                                           const float *__restrict__ fx,
+                                           const float *__restrict__ fy,
+                                           const float *__restrict__ fz,
+                                           float *__restrict__ ee, const Walls &wl, int i, int j,
+                                           int k) {
+  const long sv = g.sv, sy = g.sy, sz = g.sz;
+  const long c = ((long)k * g.nt[1] + j) * g.nt[0] + i;
+  const bool wx = i == wl.at[0] || i == wl.at[1];
+  const bool wy = DIMS >= 2 && (j == wl.at[2] || j == wl.at[3]);
+  const bool wz = DIMS >= 3 && (k == wl.at[4] || k == wl.at[5]);
+  float ex, ey, ez;
+  if (DIMS == 3) {
+    ex = 0.5f * (-fy[BZ * sv + c - sz] - fy[BZ * sv + c] + fz[BY * sv + c - sy] + fz[BY * sv + c]) -
+         0.25f * (ec32(w, sv, c - sy, 0) + ec32(w, sv, c, 0) + ec32(w, sv, c - sz - sy, 0) +
+                  ec32(w, sv, c - sz, 0));
+    ey = 0.5f * (-fz[BX * sv + c - 1] - fz[BX * sv + c] + fx[BZ * sv + c - sz] + fx[BZ * sv + c]) -
+         0.25f * (ec32(w, sv, c - 1, 1) + ec32(w, sv, c, 1) + ec32(w, sv, c - sz - 1, 1) +
+                  ec32(w, sv, c - sz, 1));
+  } else if (DIMS == 2) {
+    ex = -fy[BZ * sv + c];
+    ey = fx[BZ * sv + c];
+  } else {
+    ex = -ec32(w, sv, c, 0);
+    ey = fx[BZ * sv + c];
+  }
+  if (DIMS >= 2)
+    ez = 0.5f * (-fx[BY * sv + c - sy] - fx[BY * sv + c] + fy[BX * sv + c - 1] + fy[BX * sv + c]) -
+         0.25f * (ec32(w, sv, c - 1, 2) + ec32(w, sv, c, 2) + ec32(w, sv, c - sy - 1, 2) +
+                  ec32(w, sv, c - sy, 2));
+  else
+    ez = -fx[BY * sv + c];
+  // components lying in a reflecting wall plane
+  if (wy || wz) ex = 0.f;
+  if (wx || wz) ey = 0.f;
+  if (wx || wy) ez = 0.f;
+  ee[c] = ex;
+  ee[sv + c] = ey;
+  ee[2 * sv + c] = ez;
+}
+
+
+// Face-field update by Stokes (ct.pyx:15-91) in working type R from face
+// values b and FP32 EMFs: the lower faces of cell c (lo) and its upper faces
+// (up).  Neighbour offsets ox, oy, oz are zero where the cell needs no
+// neighbour (all loads stay in bounds; see the FP64 update_cell_d).
+template <int DIMS, typename R, typename BF>
+__device__ __forceinline__ void faces_d(const Geo &g, BF b, const float *__restrict__ ee,
+                                        double dt, int i, int j, int k, long c, long ox, long oy,
+                                        long oz, R *lo, R *up) {
+  const long sv = g.sv;
+  const float *ex = ee, *ey = ee + sv, *ez = ee + 2 * sv;
+  const R ez00 = ez[c], ez10 = ez[c + ox], ez01 = ez[c + oy], ez11 = ez[c + ox + oy];
+  const R ey00 = ey[c], ey10 = ey[c + ox], ey01 = ey[c + oz], ey11 = ey[c + ox + oz];
+  const R ex00 = ex[c], ex10 = ex[c + oy], ex01 = ex[c + oz], ex11 = ex[c + oy + oz];
+  const R bxc = b(0, c), byc = b(1, c), bzc = b(2, c);
+  const R bxp = b(0, c + ox), byp = b(1, c + oy), bzp = b(2, c + oz);
+  const double dlx = g.dlf[0][i], dly = g.dlf[1][j], dlz = g.dlf[2][k];
+  {
+    const R dtda = (R)(dt / (dly * dlz));
+    R v = bxc, vu = bxp;
+    if (DIMS >= 2) {
+      const R ds = (R)dlz;
+      v -= dtda * (ds * ez01 - ds * ez00);
+      vu -= dtda * (ds * ez11 - ds * ez10);
+    }
+    if (DIMS >= 3) {
+      const R ds = (R)dly;
+      v += dtda * (ds * ey01 - ds * ey00);
+      vu += dtda * (ds * ey11 - ds * ey10);
+    }
+    lo[0] = v;
+    up[0] = vu;
+  }
+  {
+    const R dtda = (R)(dt / (dlx * dlz));
+    R ds = (R)dlz;
+    R v = byc + dtda * (ds * ez10 - ds * ez00);
+    R vu = byp + dtda * (ds * ez11 - ds * ez01);
+    if (DIMS >= 3) {
+      ds = (R)dlx;
+      v -= dtda * (ds * ex01 - ds * ex00);
+      vu -= dtda * (ds * ex11 - ds * ex10);
+    }
+    lo[1] = v;
+    up[1] = vu;
+  }
+  {
+    const R dtda = (R)(dt / (dlx * dly));
+    R ds = (R)dly;
+    R v = bzc - dtda * (ds * ey10 - ds * ey00);
+    R vu = bzp - dtda * (ds * ey11 - ds * ey01);
+    if (DIMS >= 2) {
+      ds = (R)dlx;
+      v += dtda * (ds * ex10 - ds * ex00);
+      vu += dtda * (ds * ex11 - ds * ex01);
+    }
+    lo[2] = v;
+    up[2] = vu;
+  }
+}
+
+// Cell roles of the stage update (as update_cell_d) and neighbour offsets.
+template <int DIMS>
+struct Roles {
+  bool cell, fX, fY, fZ;
+  long ox, oy, oz;
+  __device__ Roles(const Geo &g, const Lims &L, int i, int j, int k) {
+    const bool inI = i >= L.lo[0] && i <= L.hi[0], inJ = j >= L.lo[1] && j <= L.hi[1];
+    const bool inK = k >= L.lo[2] && k <= L.hi[2];
+    cell = inI && inJ && inK;
+    fX = (i >= L.lo[0] && i <= L.hi[0] + 1) && inJ && inK;
+    fY = inI && (j >= L.lo[1] && j <= L.hi[1] + (DIMS >= 2 ? 1 : 0)) && inK;
+    fZ = inI && inJ && (k >= L.lo[2] && k <= L.hi[2] + (DIMS >= 3 ? 1 : 0));
+    ox = (cell || fY || fZ) ? 1 : 0;
+    oy = (DIMS >= 2 && (cell || fX || fZ)) ? g.sy : 0;
+    oz = (DIMS >= 3 && (cell || fX || fY)) ? g.sz : 0;
+  }
+};
+
+struct Faces32 {
+  const double *b;
+  long sv;
+  __device__ float operator()(int a, long c) const { return (float)b[a * sv + c]; }
+};
+struct Faces64 {
+  const double *b;
+  long sv;
+  __device__ double operator()(int a, long c) const { return b[a * sv + c]; }
+};
+
+struct Pred32 {
+  const float *w32, *uh;
+  const double *b;
+  const float *fx, *fy, *fz, *ee;
+  float *bs, *ws;
+};
+
+
+// Predictor (dt/2): FP32 update of the hi words, FP64 pressure recovery.
+// All operands are loaded up front; only the stores (and the copy of w into
+// ws outside the limits) depend on the roles.
+template <int DIMS>
+__device__ __forceinline__ void update32_pred(const Geo &g, const Pred32 &a, const Lims &L,
+                                              const double *__restrict__ dts, int i, int j,
+                                              int k) {
+  const long c = ((long)k * g.nt[1] + j) * g.nt[0] + i, sv = g.sv;
+  const double dt = 0.5 * dts[0];
+  const Roles<DIMS> ro(g, L, i, j, k);
+  float uh[NMODE - 3], fxl[NMODE - 3], fxu[NMODE - 3], fyl[NMODE - 3], fyu[NMODE - 3],
+      fzl[NMODE - 3], fzu[NMODE - 3];
+#pragma unroll
+  for (int n = 0; n < NMODE - 3; ++n) {
+    uh[n] = a.uh[n * sv + c];
+    fxl[n] = a.fx[n * sv + c];
+    fxu[n] = a.fx[n * sv + c + ro.ox];
+    if (DIMS >= 2) {
+      fyl[n] = a.fy[n * sv + c];
+      fyu[n] = a.fy[n * sv + c + ro.oy];
+    }
+    if (DIMS >= 3) {
+      fzl[n] = a.fz[n * sv + c];
+      fzu[n] = a.fz[n * sv + c + ro.oz];
+    }
+  }
+  float lo[3], up[3];
+  faces_d<DIMS, float>(g, Faces32{a.b, sv}, a.ee, dt, i, j, k, c, ro.ox, ro.oy, ro.oz, lo, up);
+  const float dlx = (float)g.dlf[0][i], dly = (float)g.dlf[1][j], dlz = (float)g.dlf[2][k];
+  const float dtdv = (float)(dt / (g.dlf[0][i] * g.dlf[1][j] * g.dlf[2][k]));
+  const float ax = dly * dlz, ay = dlx * dlz, az = dlx * dly;
+  double u[NMODE], wv[NMODE];
+#pragma unroll
+  for (int n = 0; n < NMODE - 3; ++n) {
+    float v = uh[n] - dtdv * (ax * fxu[n] - ax * fxl[n]);
+    if (DIMS >= 2) v -= dtdv * (ay * fyu[n] - ay * fyl[n]);
+    if (DIMS >= 3) v -= dtdv * (az * fzu[n] - az * fzl[n]);
+    u[n] = v;
+  }
+  u[BX] = ((float)g.icm[0][i] * lo[0] + (float)g.icp[0][i] * up[0]) * (float)g.dlfi[0][i];
+  if (DIMS >= 2)
+    u[BY] = ((float)g.icm[1][j] * lo[1] + (float)g.icp[1][j] * up[1]) * (float)g.dlfi[1][j];
+  else
+    u[BY] = lo[1];
+  if (DIMS >= 3)
+    u[BZ] = ((float)g.icm[2][k] * lo[2] + (float)g.icp[2][k] * up[2]) * (float)g.dlfi[2][k];
+  else
+    u[BZ] = lo[2];
+  cons2prim(wv, u, g.gam);
+  if (ro.fX) a.bs[c] = lo[0];
+  if (ro.fY) a.bs[sv + c] = lo[1];
+  if (ro.fZ) a.bs[2 * sv + c] = lo[2];
+  if (ro.cell) {
+#pragma unroll
+    for (int n = 0; n < NMODE; ++n) a.ws[n * sv + c] = (float)wv[n];
+  } else {  // outside the limits ws keeps w (read only here: the thin outer shell)
+#pragma unroll
+    for (int n = 0; n < NMODE; ++n) a.ws[n * sv + c] = a.w32[n * sv + c];
+  }
+}
+
+
+struct Corr32 {
+  float *uh, *ul;
+  const double *b;
+  const float *fx, *fy, *fz, *ee;
+  double *b1;
+  float *w32;
+};
+
+
+// Corrector (dt): FP64 update of the hi + lo state, FP64 pressure recovery,
+// returns the CFL speed of the new state (0 outside the active cells).  Operands loaded up front as in the
+// predictor.
+template <int DIMS>
+__device__ __forceinline__ double update32_corr(const Geo &g, const Corr32 &a, const Lims &L,
+                                                const double *__restrict__ dts, const Box &act,
+                                                int fms_ref, int i, int j, int k) {
+  const long c = ((long)k * g.nt[1] + j) * g.nt[0] + i, sv = g.sv;
+  double cdsmax = 0.;
+  const double dt = dts[0];
+  const Roles<DIMS> ro(g, L, i, j, k);
+  float uh[NMODE - 3], ul[NMODE - 3], fxl[NMODE - 3], fxu[NMODE - 3], fyl[NMODE - 3],
+      fyu[NMODE - 3], fzl[NMODE - 3], fzu[NMODE - 3];
+#pragma unroll
+  for (int n = 0; n < NMODE - 3; ++n) {
+    uh[n] = a.uh[n * sv + c];
+    ul[n] = a.ul[n * sv + c];
+    fxl[n] = a.fx[n * sv + c];
+    fxu[n] = a.fx[n * sv + c + ro.ox];
+    if (DIMS >= 2) {
+      fyl[n] = a.fy[n * sv + c];
+      fyu[n] = a.fy[n * sv + c + ro.oy];
+    }
+    if (DIMS >= 3) {
+      fzl[n] = a.fz[n * sv + c];
+      fzu[n] = a.fz[n * sv + c + ro.oz];
+    }
+  }
+  double lo[3], up[3];
+  faces_d<DIMS, double>(g, Faces64{a.b, sv}, a.ee, dt, i, j, k, c, ro.ox, ro.oy, ro.oz, lo,
+                        up);
+  const double dlx = g.dlf[0][i], dly = g.dlf[1][j], dlz = g.dlf[2][k];
+  const double dtdv = dt / (dlx * dly * dlz);
+  const double ax = dly * dlz, ay = dlx * dlz, az = dlx * dly;
+  double u[NMODE], wv[NMODE];
+  float nh[NMODE - 3], nl[NMODE - 3];
+#pragma unroll
+  for (int n = 0; n < NMODE - 3; ++n) {
+    double v = join(uh[n], ul[n]) - dtdv * (ax * (double)fxu[n] - ax * (double)fxl[n]);
+    if (DIMS >= 2) v -= dtdv * (ay * (double)fyu[n] - ay * (double)fyl[n]);
+    if (DIMS >= 3) v -= dtdv * (az * (double)fzu[n] - az * (double)fzl[n]);
+    u[n] = v;
+    split(v, nh[n], nl[n]);
+  }
+  u[BX] = (g.icm[0][i] * lo[0] + g.icp[0][i] * up[0]) * g.dlfi[0][i];
+  if (DIMS >= 2)
+    u[BY] = (g.icm[1][j] * lo[1] + g.icp[1][j] * up[1]) * g.dlfi[1][j];
+  else
+    u[BY] = lo[1];
+  if (DIMS >= 3)
+    u[BZ] = (g.icm[2][k] * lo[2] + g.icp[2][k] * up[2]) * g.dlfi[2][k];
+  else
+    u[BZ] = lo[2];
+  cons2prim(wv, u, g.gam);
+  if (ro.fX) a.b1[c] = lo[0];
+  if (ro.fY) a.b1[sv + c] = lo[1];
+  if (ro.fZ) a.b1[2 * sv + c] = lo[2];
+  if (ro.cell) {
+#pragma unroll
+    for (int n = 0; n < NMODE - 3; ++n) {
+      a.uh[n * sv + c] = nh[n];
+      a.ul[n * sv + c] = nl[n];
+    }
+#pragma unroll
+    for (int n = 0; n < NMODE; ++n) a.w32[n * sv + c] = (float)wv[n];
+    if (i >= act.lo[0] && i < act.lo[0] + act.n[0] && j >= act.lo[1] &&
+        j < act.lo[1] + act.n[1] && k >= act.lo[2] && k < act.lo[2] + act.n[2])
+      cdsmax = cds_cell(g, wv, fms_ref, i, j, k);
+  }
+  return cdsmax;
+}
+
+
+// CFL speed of one active cell from the FP32 primitives.
+__device__ __forceinline__ double cfl32_cell(const Geo &g, const float *__restrict__ w,
+                                             int fms_ref, int i, int j, int k) {
+  const long c = ((long)k * g.nt[1] + j) * g.nt[0] + i;
+  double wv[NMODE];
+#pragma unroll
+  for (int n = 0; n < NMODE; ++n) wv[n] = (double)w[n * g.sv + c];
+  return cds_cell(g, wv, fms_ref, i, j, k);
+}
+
+struct Ghost32 {
+  float *a[21];   // uh, ul, w32
+  double *d[3];   // face fields
+};
+
+// One (ghost, source) pair: sources are active cells the fill never writes,
+// so every value is loaded before any is stored.
+__device__ __forceinline__ void ghost32_pair(const Ghost32 &p, int dst, int src) {
+  float v[21];
+  double f[3];
+#pragma unroll
+  for (int n = 0; n < 21; ++n) v[n] = p.a[n][src];
+#pragma unroll
+  for (int n = 0; n < 3; ++n) f[n] = p.d[n][src];
+#pragma unroll
+  for (int n = 0; n < 21; ++n) p.a[n][dst] = v[n];
+#pragma unroll
+  for (int n = 0; n < 3; ++n) p.d[n][dst] = f[n];
+}
+
+// A ghost cell a reflecting side touches (GhostOp); the sign bit of slot v
+// applies to conserved slot v of uh/ul (momentum flips with velocity) and to
+// prim slot v of w32.
+__device__ __forceinline__ void ghost32_op(const Ghost32 &p, const GhostOp &o) {
+  const bool cells = !(o.bits & (1u << 15));
+  float v[21];
+  double f[3];
+#pragma unroll
+  for (int n = 0; n < 21; ++n) v[n] = cells ? p.a[n][o.src] : 0.f;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) f[a] = ((o.bits >> (12 + a)) & 1u) ? 0. : p.d[a][o.srcf[a]];
+  if (cells) {
+#pragma unroll
+    for (int n = 0; n < 21; ++n) {
+      const int slot = n < 12 ? n % 6 : n - 12;
+      p.a[n][o.dst] = ((o.bits >> slot) & 1u) ? -v[n] : v[n];
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) p.d[a][o.dst] = ((o.bits >> (9 + a)) & 1u) ? -f[a] : f[a];
+}
+
+// FP64 prim of cell c -> FP32-compensated state (conserved hi + lo, FP32 prim).
+__device__ __forceinline__ void from64_cell(const Geo &g, const double *__restrict__ w,
+                                            float *uh, float *ul, float *w32, long c) {
+  const long sv = g.sv;
+  double wv[NMODE], u[NMODE];
+#pragma unroll
+  for (int n = 0; n < NMODE; ++n) {
+    wv[n] = w[n * sv + c];
+    w32[n * sv + c] = (float)wv[n];
+  }
+  prim2cons(u, wv, g.gamm1i);
+#pragma unroll
+  for (int n = 0; n < NMODE - 3; ++n) split(u[n], uh[n * sv + c], ul[n * sv + c]);
+}
+
+// FP32-compensated state of active cell (i, j, k) -> FP64 prim.
+__device__ __forceinline__ void to64_cell(const Geo &g, const float *uh, const float *ul,
+                                          const double *b, double *w, int i, int j, int k) {
+  const long sv = g.sv;
+  const long t = ((long)k * g.nt[1] + j) * g.nt[0] + i;
+  double u[NMODE], wv[NMODE];
+#pragma unroll
+  for (int n = 0; n < NMODE - 3; ++n) u[n] = join(uh[n * sv + t], ul[n * sv + t]);
+  u[BX] = (g.icm[0][i] * b[t] + g.icp[0][i] * b[t + 1]) * g.dlfi[0][i];
+  if (g.dims >= 2)
+    u[BY] = (g.icm[1][j] * b[sv + t] + g.icp[1][j] * b[sv + t + g.sy]) * g.dlfi[1][j];
+  else
+    u[BY] = b[sv + t];
+  if (g.dims >= 3)
+    u[BZ] = (g.icm[2][k] * b[2 * sv + t] + g.icp[2][k] * b[2 * sv + t + g.sz]) * g.dlfi[2][k];
+  else
+    u[BZ] = b[2 * sv + t];
+  cons2prim(wv, u, g.gam);
+#pragma unroll
+  for (int n = 0; n < NMODE; ++n) w[n * sv + t] = wv[n];
+}
+
+}  // namespace bnz

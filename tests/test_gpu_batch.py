@@ -26,9 +26,10 @@ def box_desc(s, bc, tmax):
     return dict(nx=s.nx, ny=s.ny, nz=s.nz, bc=bc, lmin=s.lmin, lmax=s.lmax, tmax=tmax)
 
 
-def single(s, prim, bfld, bc, tmax, rsolver, reconstr, max_steps=0):
+def single(s, prim, bfld, bc, tmax, rsolver, reconstr, max_steps=0, precision="fp64"):
     from bonzo_amd._lib import Context
-    g = Context(s.nx, s.ny, s.nz, rsolver=rsolver, reconstr=reconstr, bc=bc)
+    g = Context(s.nx, s.ny, s.nz, rsolver=rsolver, reconstr=reconstr, bc=bc,
+                precision=precision)
     g.upload(prim, bfld)
     g.init_state()
     n = g.advance(tmax, max_steps)
@@ -38,11 +39,12 @@ def single(s, prim, bfld, bc, tmax, rsolver, reconstr, max_steps=0):
     return p_, b_, t, n
 
 
-def run_batch(cases, rsolver="hlld", reconstr="linear", use_graph=True, max_steps=0):
+def run_batch(cases, rsolver="hlld", reconstr="linear", use_graph=True, max_steps=0,
+              precision="fp64"):
     """cases: list of ((shape, prim, bfld), bc, tmax)."""
     from bonzo_amd._lib import Batch
     b = Batch([box_desc(c[0][0], c[1], c[2]) for c in cases], rsolver=rsolver,
-              reconstr=reconstr)
+              reconstr=reconstr, precision=precision)
     for q, ((s, prim, bfld), _, _) in enumerate(cases):
         b.upload(q, prim, bfld)
     b.init_state()
@@ -53,11 +55,13 @@ def run_batch(cases, rsolver="hlld", reconstr="linear", use_graph=True, max_step
     return out, t, steps, n
 
 
-def check_against_single(cases, rsolver="hlld", reconstr="linear", use_graph=True):
-    out, t, steps, n = run_batch(cases, rsolver, reconstr, use_graph)
+def check_against_single(cases, rsolver="hlld", reconstr="linear", use_graph=True,
+                         precision="fp64"):
+    out, t, steps, n = run_batch(cases, rsolver, reconstr, use_graph, precision=precision)
     say(f"  batch: {n} step launches, per-box steps {list(steps)}")
     for q, ((s, prim, bfld), bc, tmax) in enumerate(cases):
-        p1, b1, t1, n1 = single(s, prim, bfld, bc, tmax, rsolver, reconstr)
+        p1, b1, t1, n1 = single(s, prim, bfld, bc, tmax, rsolver, reconstr,
+                                precision=precision)
         assert steps[q] == n1, (q, steps[q], n1)
         assert t[q] == t1, (q, t[q], t1)
         assert np.array_equal(out[q][0], p1), (q, np.max(np.abs(out[q][0] - p1)))
@@ -156,3 +160,20 @@ def test_batch_reflecting_boxes_bitwise_vs_single():
              (tube, (2, 1, 0, 0, 0, 0), 0.05),
              (P.turbulence(12, seed_b=3, seed_v=4), (1, 1, 2, 2, 0, 0), 0.02)]
     check_against_single(cases)
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_batch_fp32c_bitwise_vs_single_fp32c(use_graph):
+    """FP32-compensated batches run the single-grid FP32C bodies: every box
+    equals its FP32C single-grid run (tubes, 2D blasts, a 3D box, walls)."""
+    say(f"case: FP32C batch of tubes, 2D blasts, 3D turbulence, walls; graph={use_graph}")
+    from tests.test_gpu_parity import _blast_guide_field
+    from tests.test_oracle_pins import _wall_state
+    rng = np.random.Generator(np.random.PCG64(4005))
+    cases = [(P.riemann_tube(n, rng), (1,) * 6, tm)
+             for n, tm in ((96, 0.05), (300, 0.1), (64, 0.02))]
+    cases += [(P.blast2d(40, seed=14), (0,) * 6, 0.02),
+              (_blast_guide_field(32), (2,) * 6, 0.03),
+              (_wall_state(24, 20, 0), (2, 2, 0, 0, 0, 0), 0.03),
+              (P.turbulence(12, seed_b=15, seed_v=16), (0,) * 6, 0.02)]
+    check_against_single(cases, use_graph=use_graph, precision="fp32c")

@@ -78,7 +78,7 @@ def test_create_rejects_bad_selection_without_touching_gpu():
 
 def test_batch_create_rejects_bad_selection_without_touching_gpu():
     L = _lib.load()
-    p = _lib.make_params(precision="fp32c")
+    p = _lib.make_params(precision="fp32c", rsolver="hllt")
     boxes = (_lib.BnzBox * 2)()
     for b in boxes:
         b.nx, b.ny, b.nz, b.tmax = 16, 1, 1, 0.1
