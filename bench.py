@@ -160,13 +160,14 @@ def run_path(prec, n, prim, bfld, args):
     }
 
 
-def run_batch_workload(cases, reps, use_graph=True):
+def run_batch_workload(cases, reps, use_graph=True, precision="fp64", keep_state=False):
     """Time a batch of boxes from their initial states to their t_max:
     ``reps`` timed repetitions after one warm-up (which also captures the
     graph); cell-updates = sum over boxes of active cells x steps taken."""
     from bonzo_amd._lib import Batch
     b = Batch([dict(nx=s.nx, ny=s.ny, nz=s.nz, bc=bc, lmin=s.lmin, lmax=s.lmax, tmax=tm)
-               for (s, _, _), bc, tm in cases], rsolver="hlld", reconstr="linear", cour=0.4)
+               for (s, _, _), bc, tm in cases], rsolver="hlld", reconstr="linear", cour=0.4,
+              precision=precision)
 
     def reset():
         for q, ((s, prim, bfld), _, _) in enumerate(cases):
@@ -183,29 +184,62 @@ def run_batch_workload(cases, reps, use_graph=True):
         b.advance(use_graph=use_graph)  # returns after the device has finished
         walls.append(time.perf_counter() - t0)
     t, steps = b.times()
+    state = [b.download(q) for q in range(len(cases))] if keep_state else None
     b.close()
     cells = [s.nx * s.ny * s.nz for (s, _, _), _, _ in cases]
     cu = float(sum(c * int(n) for c, n in zip(cells, steps)))
     med = sorted(walls)[len(walls) // 2]
-    return {"value": cu / med, "unit": "cell-updates/s", "wall_s_median": med,
-            "wall_s": walls, "cell_updates": cu, "boxes": len(cases), "cells": sum(cells),
-            "steps_min": int(steps.min()), "steps_max": int(steps.max()),
-            "all_at_tmax": bool(all(t[q] == cases[q][2] for q in range(len(cases)))),
-            "graph": use_graph}
+    out = {"value": cu / med, "unit": "cell-updates/s", "wall_s_median": med,
+           "wall_s": walls, "cell_updates": cu, "boxes": len(cases), "cells": sum(cells),
+           "steps_min": int(steps.min()), "steps_max": int(steps.max()),
+           "all_at_tmax": bool(all(t[q] == cases[q][2] for q in range(len(cases)))),
+           "graph": use_graph, "precision": precision}
+    return out, state
 
 
-def batch_benches(reps):
+def batch_rel_l1(cases, a, b):
+    """Per-variable relative L1 of batch states ``a`` against ``b`` over the
+    active cells of every box together."""
+    import numpy as np
+    names = ["rho", "vx", "vy", "vz", "p", "psc", "bxc", "byc", "bzc", "bxf", "byf", "bzf"]
+    num = np.zeros(12)
+    den = np.zeros(12)
+    for q, ((s, _, _), _, _) in enumerate(cases):
+        sk, sj, si = s.active()
+        for v in range(12):
+            x = a[q][v // 9][v % 9][sk, sj, si] if v < 9 else a[q][1][v - 9][sk, sj, si]
+            y = b[q][v // 9][v % 9][sk, sj, si] if v < 9 else b[q][1][v - 9][sk, sj, si]
+            num[v] += float(np.sum(np.abs(x - y)))
+            den[v] += float(np.sum(np.abs(y)))
+    return {names[v]: num[v] / den[v] for v in range(12) if den[v] > 0}
+
+
+def batch_benches(reps, fp32c=True):
     from bonzo_amd import problems
-    tubes = [(c, (1,) * 6, 0.1) for c in problems.tube_sweep(512, seed=4004)]
-    out = {"tubes": run_batch_workload(tubes, reps)}
-    out["tubes"]["workload"] = ("512 seeded 1D MHD Riemann tubes, lengths 64-4096 cells on [0,1], "
-                                "outflow, VL + PLM + HLLD, CFL 0.4, FP64, each to t=0.1")
-    out["tubes_no_graph"] = run_batch_workload(tubes, 1, use_graph=False)
-    del tubes
-    blasts = [(c, (0,) * 6, 0.2) for c in problems.blast_sweep(64, seed=5005)]
-    out["blast2d"] = run_batch_workload(blasts, max(1, reps // 2 + 1))
-    out["blast2d"]["workload"] = ("64 periodic 2D MHD blast boxes, sides 64-512 on [0,1]^2, "
-                                  "VL + PLM + HLLD + CT, CFL 0.4, FP64, each to t=0.2")
+    out = {}
+    sweeps = [("tubes", [(c, (1,) * 6, 0.1) for c in problems.tube_sweep(512, seed=4004)], reps,
+               "512 seeded 1D MHD Riemann tubes, lengths 64-4096 cells on [0,1], outflow, "
+               "VL + PLM + HLLD, CFL 0.4, each to t=0.1"),
+              ("blast2d", [(c, (0,) * 6, 0.2) for c in problems.blast_sweep(64, seed=5005)],
+               max(1, reps // 2 + 1),
+               "64 periodic 2D MHD blast boxes, sides 64-512 on [0,1]^2, VL + PLM + HLLD + CT, "
+               "CFL 0.4, each to t=0.2")]
+    for name, cases, nrep, desc in sweeps:
+        r64, s64 = run_batch_workload(cases, nrep, keep_state=fp32c)
+        r64["workload"] = desc + ", FP64"
+        out[name] = r64
+        if name == "tubes":
+            out["tubes_no_graph"], _ = run_batch_workload(cases, 1, use_graph=False)
+        if fp32c:
+            r32, s32 = run_batch_workload(cases, nrep, precision="fp32c", keep_state=True)
+            r32["workload"] = desc + ", FP32-compensated"
+            r32["speedup_vs_fp64"] = r32["value"] / r64["value"]
+            l1 = batch_rel_l1(cases, s32, s64)
+            r32["rel_l1_vs_fp64"] = l1
+            r32["rel_l1_max"] = max(l1.values())
+            out[name + "_fp32c"] = r32
+            del s32
+        del s64, cases
     return out
 
 
@@ -243,7 +277,8 @@ def main():
     from bonzo_amd import problems
 
     if args.batch_only:
-        print(json.dumps({"batched": batch_benches(args.batch_reps)}), flush=True)
+        print(json.dumps({"batched": batch_benches(args.batch_reps, not args.no_fp32c)}),
+              flush=True)
         return
     n = args.n
     s, prim, bfld = problems.turbulence(n)
@@ -307,7 +342,7 @@ def main():
             fp32c["accuracy_steps"] = r64["steps_total"]
         out["fp32c"] = fp32c
     if not args.no_batch:
-        out["batched"] = batch_benches(args.batch_reps)
+        out["batched"] = batch_benches(args.batch_reps, not args.no_fp32c)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

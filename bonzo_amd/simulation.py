@@ -13,6 +13,7 @@ The step itself runs in libbnzgpu.so on the GPU; there is no CPU fallback.
 """
 import configparser
 import os
+import warnings
 from enum import IntEnum
 
 import numpy as np
@@ -115,7 +116,11 @@ class BnzIntegr:
             raise ValueError(f"rsolver {r} has no magnetic-field solver bound "
                              "(src/integration/integrator.pyx:253-268)")
         if self.tintegr == TIntegr.TINT_RK3:
-            raise NotImplementedError("tintegr=rk3 is not part of the device path")
+            # The reference parses rk3 and allocates its extra stage arrays
+            # (integrator.pyx:339-361), but Simulation always drives
+            # vanleer.advance (simulation.pyx:67-75); _rk3.pyx is not wired.
+            warnings.warn("tintegr=rk3: the reference's driver runs the VL integrator "
+                          "(src/simulation.pyx:67-75); so does this one", stacklevel=2)
         if self.reconstr in (Reconstr.RCN_PARAB, Reconstr.RCN_WENO):
             raise NotImplementedError("parab/weno are commented out in src/coordinates/reconstr_c.c")
         if self.char_proj:

@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 from bonzo_amd import _lib, problems as P
-from bonzo_amd.simulation import BnzIntegr, Reconstr, RSolver, _Params, read_param
+from bonzo_amd.simulation import BnzIntegr, Reconstr, RSolver, TIntegr, _Params, read_param
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -112,9 +112,11 @@ def test_selection_semantics_follow_reference(tmp_path):
     with pytest.raises(ValueError):
         BnzIntegr(_Params(config={"computation": {"cour": 0.4, "rsolver": "hllc"},
                                   "physics": {"gam": 1.4}}))
-    with pytest.raises(NotImplementedError):
-        BnzIntegr(_Params(config={"computation": {"cour": 0.4, "tintegr": "rk3"},
-                                  "physics": {"gam": 1.4}}))
+    # rk3 is parsed, but the reference's Simulation always runs vanleer.advance
+    with pytest.warns(UserWarning, match="rk3"):
+        it = BnzIntegr(_Params(config={"computation": {"cour": 0.4, "tintegr": "rk3"},
+                                       "physics": {"gam": 1.4}}))
+    assert it.tintegr == TIntegr.TINT_RK3
 
 
 def _divb(s, bfld):
