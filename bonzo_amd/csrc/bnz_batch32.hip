@@ -4,7 +4,7 @@
 // FP64 batch (bnz_batch.hip), with the per-item bodies of the single-grid
 // FP32C step (bnz_kern32.hpp), so that a box of an FP32C batch follows
 // exactly the arithmetic of an FP32C single grid.  Built, like
-// bnz_gpu32.hip, with FMA contraction allowed.
+// bnz_gpu32.hip, with FMA contraction of single expressions (-ffp-contract=on).
 #include <hip/hip_runtime.h>
 
 #include "bnz_batch.hpp"

@@ -1,8 +1,8 @@
 // bnz_kern32.hpp -- per-face, per-edge and per-cell bodies of the
 // FP32-compensated VL step, shared by the single-grid kernels
-// (bnz_gpu32.hip) and the batched kernels (bnz_batch.hip), so that a box of
-// a batch follows exactly the arithmetic of a single grid.  Compiled with
-// FMA contraction allowed (the FP32C translation units).
+// (bnz_gpu32.hip) and the batched kernels (bnz_batch32.hip), so that a box
+// of a batch follows exactly the arithmetic of a single grid.  Compiled with
+// FMA contraction of single expressions only (-ffp-contract=on).
 #pragma once
 #include "bnz_common.hpp"
 

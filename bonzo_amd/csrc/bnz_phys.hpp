@@ -5,7 +5,7 @@
 // src/coordinates/reconstr_c.h) with IEEE division and square root, so an
 // -ffp-contract=off build reproduces the CPU oracle bit for bit.  T = float is
 // the fast arithmetic of the FP32-compensated step: hardware reciprocal and
-// square root (about 1 ulp), FMA contraction allowed by its translation unit,
+// square root (about 1 ulp), FMA contraction of single expressions,
 // and guards where HLLD's denominators vanish.
 // States are 9-slot register arrays in the local frame of the sweep:
 //   0 rho, 1 vx|mx, 2 vy|my, 3 vz|mz, 4 p|E, 5 passive scalar, 6 Bx, 7 By, 8 Bz
