@@ -41,6 +41,13 @@ KERNEL_BYTES = {
     "fp32c": {"flux": 18 * 4, "emf": 15 * 4, "update_pred": 48 * 4 + 3 * 8,
               "update_corr": 54 * 4 + 6 * 8},
 }
+# HBM bytes per launch of the dominant kernels at 256^3 (FETCH_SIZE x 2.0 +
+# WRITE_SIZE x 1.0, scaled by a plain device copy), from the rocprofv3 --pmc
+# runs summarised in the named profile; the bench itself runs no counters.
+PROFILED_TRAFFIC = {
+    ("fp64", "flux3_plm"): (6.452e9, "profiles/vl_step_kernels_turb256.md"),
+    ("fp32c", "update_corr"): (5.398e9, "profiles/vl_step_kernels_turb256.md"),
+}
 # Algorithmic bytes per cell-update of a whole VL step: 5 state volumes of 9
 # words (FP64 360 B; FP32C 252 B with hi + lo on the stage-2 read of S_n and
 # the write of S_{n+1}).
@@ -148,11 +155,13 @@ def run_path(prec, n, prim, bfld, args):
     davg = dtot / dl
     b = kernel_bytes(prec, dname, n)
     ach = b / (davg * 1e-3) / 1e9
+    prof = PROFILED_TRAFFIC.get((prec, dname), (None, None)) if n == 256 else (None, None)
     return {
         "value": value, "ms_per_step": 1e3 * el / args.steps, "tuned": tuned, "table": table,
         "kernels": {k: v[0] / v[1] for k, v in sorted(kt.items())},
         "roofline": {"bound": "hbm", "kernel": dname, "achieved": ach, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                     "traffic": prof[0], "traffic_source": prof[1],
                      "avg_launch_ms": davg, "algorithmic_bytes_per_launch": b},
         "step_GBs": STEP_BYTES[prec] * value / 1e9,
         "divb_max": d["divb_max"], "hlld_guards": d["hlld_guards"],
