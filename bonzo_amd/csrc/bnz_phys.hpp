@@ -19,16 +19,56 @@ enum { RHO = 0, MX = 1, MY = 2, MZ = 3, EN = 4, PSC = 5, BX = 6, BY = 7, BZ = 8 
 enum { VX = 1, VY = 2, VZ = 3, PR = 4 };
 constexpr int NMODE = 9;
 
+// FP64 division and square root by the Newton-Raphson steps of the
+// compiler's IEEE expansion for gfx950, without its range scaling and
+// special-value fix-up.  Division: hardware reciprocal, two refinements
+// r += r(1 - b r), quotient q = a r and Markstein's correction
+// q + r(a - b q) (P. Markstein, IA-64 and Elementary Functions, 2000).
+// Square root: reciprocal square root y, g = x y, h = y/2, one coupled
+// refinement of g and h and two corrections g += h(x - g g).  The expansion
+// rescales only operands far outside the range of the step's values (a
+// quotient beyond 2^767 or below 2^-1022, a numerator below 2^-970, a root
+// argument below 2^-767), so for every other operand the steps below are
+// the expansion's steps and give the IEEE result bit for bit.  Differences
+// are confined to: a numerator of -0 (gives +0), a zero denominator (NaN
+// instead of an infinity).  A zero root argument is passed through.
+__device__ __forceinline__ double nr_div(double a, double b) {
+  double r = __builtin_amdgcn_rcp(b);
+  double e = fma(-b, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-b, r, 1.0);
+  r = fma(r, e, r);
+  const double q = a * r;
+  const double rem = fma(-b, q, a);
+  return fma(rem, r, q);
+}
+__device__ __forceinline__ double nr_sqrt(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y;
+  double h = y * 0.5;
+  const double r = fma(-h, g, 0.5);
+  g = fma(g, r, g);
+  h = fma(h, r, h);
+  double d = fma(-g, g, x);
+  g = fma(d, h, g);
+  d = fma(-g, g, x);
+  g = fma(d, h, g);
+  return x == 0.0 ? x : g;
+}
+
 template <typename T> struct Ar;
 template <> struct Ar<double> {
-  static __device__ __forceinline__ double div(double a, double b) { return a / b; }
-  static __device__ __forceinline__ double root(double a) { return sqrt(a); }
+  static __device__ __forceinline__ double div(double a, double b) { return nr_div(a, b); }
+  // quotient whose numerator may be -0: the full IEEE expansion
+  static __device__ __forceinline__ double divz(double a, double b) { return a / b; }
+  static __device__ __forceinline__ double root(double a) { return nr_sqrt(a); }
   static constexpr bool guarded = false;
 };
 template <> struct Ar<float> {
   static __device__ __forceinline__ float div(float a, float b) {
     return a * __builtin_amdgcn_rcpf(b);
   }
+  static __device__ __forceinline__ float divz(float a, float b) { return div(a, b); }
   static __device__ __forceinline__ float root(float a) { return __builtin_amdgcn_sqrtf(a); }
   static constexpr bool guarded = true;
 };
@@ -316,7 +356,7 @@ __device__ __forceinline__ int flux_hllt(T *flux, const T *wL, const T *wR, T ga
   const T rhos = (sR * uR[RHO] - sL * uL[RHO] - (fR[RHO] - fL[RHO])) * sRsLi;
   const T mxs = (sR * uR[MX] - sL * uL[MX] - (fR[MX] - fL[MX])) * sRsLi;
   const T frhos = (sR * fL[RHO] - sL * fR[RHO] + sR * sL * (uR[RHO] - uL[RHO])) * sRsLi;
-  const T us = Ar<T>::div(frhos, rhos);
+  const T us = Ar<T>::divz(frhos, rhos);
   const T pscs = (sR * uR[PSC] - sL * uL[PSC] - (fR[PSC] - fL[PSC])) * sRsLi;
   T uLs[NMODE], uRs[NMODE], uCs[NMODE];
   uLs[RHO] = rhos;
