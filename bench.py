@@ -127,7 +127,7 @@ def run_path(prec, n, prim, bfld, args):
     if args.shapes:
         tuned = tuple(int(x) for x in args.shapes.split(","))
         g.set_tuning(*tuned)
-    elif prec == "fp64" and not args.no_tune:
+    elif not args.no_tune:
         # tuning steps advance the state: restart from the initial condition
         tuned, table = autotune(g)
         g.upload(prim, bfld)
@@ -330,6 +330,7 @@ def main():
                  "step_roofline": {"bytes_per_cell_update": STEP_BYTES["fp32c"],
                                    "achieved_GBs": r32["step_GBs"],
                                    "frac_of_8TBs": r32["step_GBs"] / HBM_PEAK_GBS},
+                 "shapes": dict(zip(("flux", "emf", "update"), r32["tuned"])),
                  "kernels_ms_per_launch": r32["kernels"], "divb_max": r32["divb_max"],
                  "hlld_guards": r32["hlld_guards"]}
         if r64["state"] is not None and r32["state"] is not None:

@@ -280,6 +280,11 @@ inline Shape work_shape(const Box &b, int dims, int chunk) {
   return Shape{false, chunk, slab_blocks(b, chunk)};
 }
 
+// FP32C kernels take flat or slab-column shapes; the tile shape runs flat.
+inline Shape work_shape32(const Box &b, int dims, int chunk) {
+  return work_shape(b, dims, chunk == 1 ? 0 : chunk);
+}
+
 struct Launch {
   bnz_ctx *c;
   int idx;

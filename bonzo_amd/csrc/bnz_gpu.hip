@@ -2,10 +2,10 @@
 //
 // One VL step (src/integration/vanleer.pyx:87-167) is a chain of kernels on
 // one stream:
-//   dt_partial, dt_final                CFL dt, device resident
-//   predictor: flux_x, flux_y, flux_z, emf, update<pred>   (donor cell, dt/2)
-//   corrector: flux_x, flux_y, flux_z, emf, update<corr>   (PLM, dt)
-//   ghost, time_advance
+//   dt_final (dt_partial on the first step)   CFL dt, device resident
+//   predictor: flux3, emf, update<pred>        (donor cell, dt/2)
+//   corrector: flux3, emf, update<corr>        (PLM, dt; CFL maximum fused)
+//   ghost (+ ghost_ops for reflecting sides), time_advance
 // Grid arrays keep the reference's SoA layout [var][k][j][i] in HBM so every
 // wavefront reads 64 consecutive doubles of one x-row.  The persistent state
 // is prim (9) + face B (3); conserved variables are recomputed from prim
@@ -1062,7 +1062,8 @@ int bnz_set_tuning(bnz_ctx *c, int flux_chunk, int emf_chunk, int update_chunk) 
     grid.lo[q] = 0;
     grid.n[q] = c->nt[q];
   }
-  const long need = work_shape(grid, c->dims, update_chunk).blocks;
+  const long need = c->f32 ? work_shape32(grid, c->dims, update_chunk).blocks
+                           : work_shape(grid, c->dims, update_chunk).blocks;
   if (need > c->npart_cap) {
     double *np = nullptr;
     HIPCHK(c, hipStreamSynchronize(c->stream));

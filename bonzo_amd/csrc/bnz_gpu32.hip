@@ -29,9 +29,9 @@ __global__ __launch_bounds__(BLK) void k32_flux3(Geo g, const float *__restrict_
                                                  const BF *__restrict__ bf,
                                                  float *__restrict__ fx, float *__restrict__ fy,
                                                  float *__restrict__ fz, Box box, Lims L,
-                                                 int *nguard) {
+                                                 int *nguard, int ty) {
   int i, j, k;
-  if (!cell_index<false, 1, 2>(box, 0, i, j, k)) return;
+  if (!cell_index<false, 1, 2>(box, ty, i, j, k)) return;
   int ng = 0;
   flux32_cell<ORDER, RS>(g, w, bf, fx, fy, fz, L, i, j, k, ng);
   if (ng) atomicAdd(nguard, ng);
@@ -43,22 +43,20 @@ __global__ __launch_bounds__(BLK) void k32_emf(Geo g, const float *__restrict__ 
                                                const float *__restrict__ fx,
                                                const float *__restrict__ fy,
                                                const float *__restrict__ fz,
-                                               float *__restrict__ ee, Box box, Walls wl) {
+                                               float *__restrict__ ee, Box box, Walls wl,
+                                               int ty) {
   int i, j, k;
-  if (!cell_index<false, 1, 2>(box, 0, i, j, k)) return;
+  if (!cell_index<false, 1, 2>(box, ty, i, j, k)) return;
   emf32_edge<DIMS>(g, w, fx, fy, fz, ee, wl, i, j, k);
 }
 
 // Predictor (update32_pred), one thread per cell of the grid.
 template <int DIMS>
 __global__ __launch_bounds__(BLK) void k32_update_pred(Geo g, Pred32 a, Lims L,
-                                                       const double *__restrict__ dts) {
-  const long t = xcd_block() * BLK + threadIdx.x;
-  const long ncell = (long)g.nt[0] * g.nt[1] * g.nt[2];
-  if (t >= ncell) return;
-  const int i = (int)(t % g.nt[0]);
-  const long r = t / g.nt[0];
-  const int j = (int)(r % g.nt[1]), k = (int)(r / g.nt[1]);
+                                                       const double *__restrict__ dts,
+                                                       Box grid, int ty) {
+  int i, j, k;
+  if (!cell_index<false, 1, 2>(grid, ty, i, j, k)) return;
   update32_pred<DIMS>(g, a, L, dts, i, j, k);
 }
 
@@ -66,16 +64,12 @@ __global__ __launch_bounds__(BLK) void k32_update_pred(Geo g, Pred32 a, Lims L,
 template <int DIMS>
 __global__ __launch_bounds__(BLK) void k32_update_corr(Geo g, Corr32 a, Lims L,
                                                        const double *__restrict__ dts, Box act,
-                                                       int fms_ref, double *__restrict__ part) {
-  const long t = xcd_block() * BLK + threadIdx.x;
-  const long ncell = (long)g.nt[0] * g.nt[1] * g.nt[2];
+                                                       int fms_ref, double *__restrict__ part,
+                                                       Box grid, int ty) {
+  int i, j, k;
   double cdsmax = 0.;
-  if (t < ncell) {
-    const int i = (int)(t % g.nt[0]);
-    const long r = t / g.nt[0];
-    const int j = (int)(r % g.nt[1]), k = (int)(r / g.nt[1]);
+  if (cell_index<false, 1, 2>(grid, ty, i, j, k))
     cdsmax = update32_corr<DIMS>(g, a, L, dts, act, fms_ref, i, j, k);
-  }
   block_max_to(cdsmax, part);
 }
 
@@ -125,9 +119,10 @@ template <int ORDER, int RS, typename BF>
 void launch_fluxes32(bnz_ctx *c, const float *w, const BF *bf, const Lims &L) {
   int *ng = (int *)(c->dev + 4);
   const Box u = flux3_box(c->dims, L);
+  const Shape sh = work_shape32(u, c->dims, c->flux_chunk);
   Launch t(c, ORDER ? "flux3_plm" : "flux3_dc");
-  hipLaunchKernelGGL((k32_flux3<ORDER, RS, BF>), dim3(blocks_for(u.total())), dim3(BLK), 0,
-                     c->stream, c->g, w, bf, c->fx32, c->fy32, c->fz32, u, L, ng);
+  hipLaunchKernelGGL((k32_flux3<ORDER, RS, BF>), dim3(sh.blocks), dim3(BLK), 0, c->stream, c->g,
+                     w, bf, c->fx32, c->fy32, c->fz32, u, L, ng, sh.ty);
 }
 
 template <typename BF>
@@ -154,8 +149,9 @@ void emf32(bnz_ctx *c, const float *w, const Lims &L, const char *name) {
     wl.at[2 * a] = (a < c->dims && c->p.bc[2 * a] == BNZ_BC_REFLECT) ? lo[a] : -1;
     wl.at[2 * a + 1] = (a < c->dims && c->p.bc[2 * a + 1] == BNZ_BC_REFLECT) ? hi[a] + 1 : -1;
   }
+  const Shape sh = work_shape32(e, c->dims, c->emf_chunk);
 #define E32_ARGS \
-  dim3(blocks_for(e.total())), dim3(BLK), 0, c->stream, c->g, w, c->fx32, c->fy32, c->fz32, c->ee32, e, wl
+  dim3(sh.blocks), dim3(BLK), 0, c->stream, c->g, w, c->fx32, c->fy32, c->fz32, c->ee32, e, wl, sh.ty
   if (c->dims == 3) hipLaunchKernelGGL(k32_emf<3>, E32_ARGS);
   else if (c->dims == 2) hipLaunchKernelGGL(k32_emf<2>, E32_ARGS);
   else hipLaunchKernelGGL(k32_emf<1>, E32_ARGS);
@@ -233,14 +229,20 @@ void f32_dt(bnz_ctx *c, int use_fixed, double fixed_dt) {
 }
 
 void f32_step_body(bnz_ctx *c) {
-  const int nb = blocks_for(c->ncell);
+  Box grid;
+  for (int q = 0; q < 3; ++q) {
+    grid.lo[q] = 0;
+    grid.n[q] = c->nt[q];
+  }
+  const Shape us = work_shape32(grid, c->dims, c->update_chunk);
+  const int nb = us.blocks;
   Lims L0 = integr_lims(c, 0);
   fluxes32(c, 0, c->w32, c->b, L0);
   emf32(c, c->w32, L0, "emf_pred");
   {
     Pred32 a{c->w32, c->uh, c->b, c->fx32, c->fy32, c->fz32, c->ee32, c->bs32, c->ws32};
     Launch t(c, "update_pred");
-#define P32_ARGS dim3(nb), dim3(BLK), 0, c->stream, c->g, a, L0, c->dev
+#define P32_ARGS dim3(nb), dim3(BLK), 0, c->stream, c->g, a, L0, c->dev, grid, us.ty
     if (c->dims == 3) hipLaunchKernelGGL(k32_update_pred<3>, P32_ARGS);
     else if (c->dims == 2) hipLaunchKernelGGL(k32_update_pred<2>, P32_ARGS);
     else hipLaunchKernelGGL(k32_update_pred<1>, P32_ARGS);
@@ -253,7 +255,8 @@ void f32_step_body(bnz_ctx *c) {
     Corr32 a{c->uh, c->ul, c->b, c->fx32, c->fy32, c->fz32, c->ee32, c->bs, c->w32};
     Launch t(c, "update_corr");
 #define C32_ARGS \
-  dim3(nb), dim3(BLK), 0, c->stream, c->g, a, L1, c->dev, active_box(c), c->p.dt_fms_ref, c->part
+  dim3(nb), dim3(BLK), 0, c->stream, c->g, a, L1, c->dev, active_box(c), c->p.dt_fms_ref, c->part, \
+      grid, us.ty
     if (c->dims == 3) hipLaunchKernelGGL(k32_update_corr<3>, C32_ARGS);
     else if (c->dims == 2) hipLaunchKernelGGL(k32_update_corr<2>, C32_ARGS);
     else hipLaunchKernelGGL(k32_update_corr<1>, C32_ARGS);
