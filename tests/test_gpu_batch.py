@@ -177,3 +177,12 @@ def test_batch_fp32c_bitwise_vs_single_fp32c(use_graph):
               (_wall_state(24, 20, 0), (2, 2, 0, 0, 0, 0), 0.03),
               (P.turbulence(12, seed_b=15, seed_v=16), (0,) * 6, 0.02)]
     check_against_single(cases, use_graph=use_graph, precision="fp32c")
+
+
+def test_batch_fp32c_hll_const_bitwise_vs_single_fp32c():
+    say("case: FP32C batch, HLL + donor cell (ng=2), 1D + 2D + 3D boxes, mixed flags")
+    rng = np.random.Generator(np.random.PCG64(4006))
+    cases = [(P.riemann_tube(120, rng, ng=2), (1,) * 6, 0.05),
+             (P.orszag_tang(36, ng=2), (0,) * 6, 0.04),
+             (P.turbulence(14, ng=2, seed_b=17, seed_v=18), (0, 0, 1, 1, 0, 0), 0.03)]
+    check_against_single(cases, rsolver="hll", reconstr="const", precision="fp32c")
