@@ -69,7 +69,7 @@ __global__ __launch_bounds__(BLK) void kb_emf(const BoxDev *__restrict__ boxes, 
 
 // Stage update of every cell of the running boxes.  The corrector also
 // folds the CFL maximum of the new state into the box's cmax (one atomic per
-// wave), and carries the faces of a finished box into the buffer that
+// work block), and carries the faces of a finished box into the buffer that
 // becomes current after the face-buffer swap.
 template <bool PRED>
 __global__ __launch_bounds__(BLK) void kb_update(const BoxDev *__restrict__ boxes, WList wl,
@@ -101,11 +101,7 @@ __global__ __launch_bounds__(BLK) void kb_update(const BoxDev *__restrict__ boxe
                                        p.b + o, p.bs + o, p.w + o, d.L[1], st.dt + b, d.act,
                                        fms_ref, i, j, k);
     }
-    if (!PRED) {
-      cds = wave_max(cds);
-      if ((threadIdx.x & 63) == 0 && cds > 0.)
-        atomicMax(st.cmax + b, (unsigned long long)__double_as_longlong(cds));
-    }
+    if (!PRED) block_max_atomic(cds, st.cmax + b);
   }
 }
 
@@ -128,9 +124,7 @@ __global__ __launch_bounds__(BLK) void kb_cfl(const BoxDev *__restrict__ boxes, 
       for (int n = 0; n < NMODE; ++n) wv[n] = w[n * d.g.sv + c];
       cds = cds_cell(d.g, wv, fms_ref, i, j, k);
     }
-    cds = wave_max(cds);
-    if ((threadIdx.x & 63) == 0 && cds > 0.)
-      atomicMax(st.cmax + b, (unsigned long long)__double_as_longlong(cds));
+    block_max_atomic(cds, st.cmax + b);
   }
 }
 

@@ -42,6 +42,24 @@ struct BoxState {
   int *nrun;                 // boxes still short of t_max after the last step
 };
 
+// Maximum of v over the workgroup folded into *addr with one atomic (the
+// bits of a non-negative double order like its value).  Every thread of the
+// workgroup calls it (a work block belongs to one box); the closing barrier
+// lets the next work block of a persistent loop reuse red.
+__device__ __forceinline__ void block_max_atomic(double v, unsigned long long *addr) {
+  __shared__ double red[BLK / 64];
+  v = bnz::wave_max(v);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = red[0];
+#pragma unroll
+    for (int q = 1; q < BLK / 64; ++q) m = fmax(m, red[q]);
+    if (m > 0.) atomicMax(addr, (unsigned long long)__double_as_longlong(m));
+  }
+  __syncthreads();
+}
+
 // work list kt as seen by a kernel
 struct WList {
   const int *bmap;  // work block -> box

@@ -115,11 +115,7 @@ __global__ __launch_bounds__(BLK) void kb32_update(const BoxDev *__restrict__ bo
           cds = update32_corr<1>(d.g, a, d.L[1], st.dt + b, d.act, fms_ref, i, j, k);
       }
     }
-    if (!PRED) {
-      cds = wave_max(cds);
-      if ((threadIdx.x & 63) == 0 && cds > 0.)
-        atomicMax(st.cmax + b, (unsigned long long)__double_as_longlong(cds));
-    }
+    if (!PRED) block_max_atomic(cds, st.cmax + b);
   }
 }
 
@@ -150,9 +146,7 @@ __global__ __launch_bounds__(BLK) void kb32_cfl(const BoxDev *__restrict__ boxes
       box_idx(it, t, i, j, k);
       cds = cfl32_cell(d.g, w32 + d.off, fms_ref, i, j, k);
     }
-    cds = wave_max(cds);
-    if ((threadIdx.x & 63) == 0 && cds > 0.)
-      atomicMax(st.cmax + b, (unsigned long long)__double_as_longlong(cds));
+    block_max_atomic(cds, st.cmax + b);
   }
 }
 
