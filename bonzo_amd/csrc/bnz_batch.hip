@@ -344,15 +344,17 @@ void b_step(bnz_batch *c) {
   }
   // predictor: donor cell, dt/2, limits of order 0
   b_flux_dispatch(c, 0, c->P.w, c->P.b, KF0);
-  hipLaunchKernelGGL(kb_emf, dim3(grid_for(c->nblk[KEM0])), dim3(BLK), 0, c->stream, c->boxes,
-                     wlist(c, KEM0), c->S, c->P.w, c->P.fx, c->P.fy, c->P.fz, c->P.ee);
+  if (c->nblk[KEM0] > 0)
+    hipLaunchKernelGGL(kb_emf, dim3(grid_for(c->nblk[KEM0])), dim3(BLK), 0, c->stream, c->boxes,
+                       wlist(c, KEM0), c->S, c->P.w, c->P.fx, c->P.fy, c->P.fz, c->P.ee);
   hipLaunchKernelGGL(kb_update<true>, dim3(grid_for(c->nblk[KUPD])), dim3(BLK), 0, c->stream,
                      c->boxes, wlist(c, KUPD), c->S, c->P, fr);
   // corrector: reconstruction order, dt; b_{n+1} goes to the predictor's
   // face buffer, which then becomes the current one
   b_flux_dispatch(c, c->order, c->P.ws, c->P.bs, KF1);
-  hipLaunchKernelGGL(kb_emf, dim3(grid_for(c->nblk[KEM1])), dim3(BLK), 0, c->stream, c->boxes,
-                     wlist(c, KEM1), c->S, c->P.ws, c->P.fx, c->P.fy, c->P.fz, c->P.ee);
+  if (c->nblk[KEM1] > 0)
+    hipLaunchKernelGGL(kb_emf, dim3(grid_for(c->nblk[KEM1])), dim3(BLK), 0, c->stream, c->boxes,
+                       wlist(c, KEM1), c->S, c->P.ws, c->P.fx, c->P.fy, c->P.fz, c->P.ee);
   hipLaunchKernelGGL(kb_update<false>, dim3(grid_for(c->nblk[KUPD])), dim3(BLK), 0, c->stream,
                      c->boxes, wlist(c, KUPD), c->S, c->P, fr);
   std::swap(c->P.b, c->P.bs);
@@ -459,9 +461,12 @@ int bnz_batch_create(const bnz_params *p, const bnz_box *boxes, int nbox, bnz_ba
       if (dims < 3) L.lo[2] = L.hi[2] = 0;
       const int il = L.lo[0], iu = L.hi[0], jl = L.lo[1], ju = L.hi[1], kl = L.lo[2], ku = L.hi[2];
       d.items[o == 0 ? KF0 : KF1] = flux3_box(dims, L);
+      // 1D boxes form their EMFs in the update (Emf1D, Emf32x1D): no EMF items
       d.items[o == 0 ? KEM0 : KEM1] =
-          mkbox(il, iu + 2 - il, dims >= 2 ? jl : 0, dims >= 2 ? ju + 2 - jl : 1,
-                dims >= 3 ? kl : 0, dims >= 3 ? ku + 2 - kl : 1);
+          dims == 1
+              ? mkbox(0, 0, 0, 0, 0, 0)
+              : mkbox(il, iu + 2 - il, dims >= 2 ? jl : 0, dims >= 2 ? ju + 2 - jl : 1,
+                      dims >= 3 ? kl : 0, dims >= 3 ? ku + 2 - kl : 1);
     }
     d.items[KUPD] = mkbox(0, nt[0], 0, nt[1], 0, nt[2]);
     d.act = mkbox(a1[0], s.nx, a1[1], dims >= 2 ? s.ny : 1, a1[2], dims >= 3 ? s.nz : 1);

@@ -98,21 +98,22 @@ __global__ __launch_bounds__(BLK) void kb32_update(const BoxDev *__restrict__ bo
     if (valid) {
       int i, j, k;
       box_idx(it, t, i, j, k);
+      const Walls wa = box_walls(d);
       if (PRED) {
         const Pred32 a{Q.w32 + o, Q.uh + o, P.b + o, Q.fx + o, Q.fy + o,
                        Q.fz + o,  Q.ee + o, Q.bs32 + o, Q.ws32 + o};
-        if (d.g.dims == 3) update32_pred<3>(d.g, a, d.L[0], st.dt + b, i, j, k);
-        else if (d.g.dims == 2) update32_pred<2>(d.g, a, d.L[0], st.dt + b, i, j, k);
-        else update32_pred<1>(d.g, a, d.L[0], st.dt + b, i, j, k);
+        if (d.g.dims == 3) update32_pred<3>(d.g, a, d.L[0], st.dt + b, wa, i, j, k);
+        else if (d.g.dims == 2) update32_pred<2>(d.g, a, d.L[0], st.dt + b, wa, i, j, k);
+        else update32_pred<1>(d.g, a, d.L[0], st.dt + b, wa, i, j, k);
       } else {
         const Corr32 a{Q.uh + o, Q.ul + o, P.b + o, Q.fx + o, Q.fy + o,
                        Q.fz + o, Q.ee + o, P.bs + o, Q.w32 + o};
         if (d.g.dims == 3)
-          cds = update32_corr<3>(d.g, a, d.L[1], st.dt + b, d.act, fms_ref, i, j, k);
+          cds = update32_corr<3>(d.g, a, d.L[1], st.dt + b, d.act, fms_ref, wa, i, j, k);
         else if (d.g.dims == 2)
-          cds = update32_corr<2>(d.g, a, d.L[1], st.dt + b, d.act, fms_ref, i, j, k);
+          cds = update32_corr<2>(d.g, a, d.L[1], st.dt + b, d.act, fms_ref, wa, i, j, k);
         else
-          cds = update32_corr<1>(d.g, a, d.L[1], st.dt + b, d.act, fms_ref, i, j, k);
+          cds = update32_corr<1>(d.g, a, d.L[1], st.dt + b, d.act, fms_ref, wa, i, j, k);
       }
     }
     if (!PRED) block_max_atomic(cds, st.cmax + b);
@@ -199,6 +200,7 @@ void fluxes(bnz_batch *c, int order, const float *w, const BF *bf, int kt) {
 }
 
 void emfs(bnz_batch *c, const float *w, int kt) {
+  if (c->nblk[kt] == 0) return;  // only 1D boxes: their EMFs are formed in the update
   hipLaunchKernelGGL(kb32_emf, dim3(grid_for(c->nblk[kt])), dim3(BLK), 0, c->stream, c->boxes,
                      wlist(c, kt), c->S, w, c->Q.fx, c->Q.fy, c->Q.fz, c->Q.ee);
 }

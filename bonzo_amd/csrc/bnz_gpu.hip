@@ -528,14 +528,14 @@ void launch_step_body(bnz_ctx *c) {
   // predictor: donor cell, dt/2, lims of order 0; ws and bs are new buffers
   Lims L0 = integr_lims(c, 0);
   launch_flux_dispatch(c, 0, c->w, c->b, L0);
-  launch_emf(c, c->w, L0, "emf_pred");
+  if (c->dims > 1) launch_emf(c, c->w, L0, "emf_pred");  // 1D: EMFs formed in the update
   launch_update(c, true, L0);
   // corrector: reconstruction order, dt; prim is updated in place (each cell
   // reads only its own w_n), b_{n+1} goes to the predictor's face buffer,
   // which is dead once the corrector fluxes and EMFs exist
   Lims L1 = integr_lims(c, c->order);
   launch_flux_dispatch(c, c->order, c->ws, c->bs, L1);
-  launch_emf(c, c->ws, L1, "emf_corr");
+  if (c->dims > 1) launch_emf(c, c->ws, L1, "emf_corr");
   launch_update(c, false, L1);
   std::swap(c->b, c->bs);
   launch_ghost(c);

@@ -54,10 +54,10 @@ __global__ __launch_bounds__(BLK) void k32_emf(Geo g, const float *__restrict__ 
 template <int DIMS>
 __global__ __launch_bounds__(BLK) void k32_update_pred(Geo g, Pred32 a, Lims L,
                                                        const double *__restrict__ dts,
-                                                       Box grid, int ty) {
+                                                       Box grid, int ty, Walls wl) {
   int i, j, k;
   if (!cell_index<false, 1, 2>(grid, ty, i, j, k)) return;
-  update32_pred<DIMS>(g, a, L, dts, i, j, k);
+  update32_pred<DIMS>(g, a, L, dts, wl, i, j, k);
 }
 
 // Corrector (update32_corr) with the CFL maximum reduced per block.
@@ -65,11 +65,11 @@ template <int DIMS>
 __global__ __launch_bounds__(BLK) void k32_update_corr(Geo g, Corr32 a, Lims L,
                                                        const double *__restrict__ dts, Box act,
                                                        int fms_ref, double *__restrict__ part,
-                                                       Box grid, int ty) {
+                                                       Box grid, int ty, Walls wl) {
   int i, j, k;
   double cdsmax = 0.;
   if (cell_index<false, 1, 2>(grid, ty, i, j, k))
-    cdsmax = update32_corr<DIMS>(g, a, L, dts, act, fms_ref, i, j, k);
+    cdsmax = update32_corr<DIMS>(g, a, L, dts, act, fms_ref, wl, i, j, k);
   block_max_to(cdsmax, part);
 }
 
@@ -137,18 +137,24 @@ void fluxes32(bnz_ctx *c, int order, const float *w, const BF *bf, const Lims &L
   }
 }
 
-void emf32(bnz_ctx *c, const float *w, const Lims &L, const char *name) {
-  Box e;
-  e.lo[0] = L.lo[0]; e.n[0] = L.hi[0] + 1 - L.lo[0] + 1;
-  e.lo[1] = (c->dims >= 2) ? L.lo[1] : 0; e.n[1] = (c->dims >= 2) ? L.hi[1] + 1 - L.lo[1] + 1 : 1;
-  e.lo[2] = (c->dims >= 3) ? L.lo[2] : 0; e.n[2] = (c->dims >= 3) ? L.hi[2] + 1 - L.lo[2] + 1 : 1;
-  Launch t(c, name);
+// Reflecting wall planes of the grid (their in-plane EMF components are zero).
+Walls walls32(const bnz_ctx *c) {
   Walls wl;
   const int lo[3] = {c->i1, c->j1, c->k1}, hi[3] = {c->i2, c->j2, c->k2};
   for (int a = 0; a < 3; ++a) {
     wl.at[2 * a] = (a < c->dims && c->p.bc[2 * a] == BNZ_BC_REFLECT) ? lo[a] : -1;
     wl.at[2 * a + 1] = (a < c->dims && c->p.bc[2 * a + 1] == BNZ_BC_REFLECT) ? hi[a] + 1 : -1;
   }
+  return wl;
+}
+
+void emf32(bnz_ctx *c, const float *w, const Lims &L, const char *name) {
+  Box e;
+  e.lo[0] = L.lo[0]; e.n[0] = L.hi[0] + 1 - L.lo[0] + 1;
+  e.lo[1] = (c->dims >= 2) ? L.lo[1] : 0; e.n[1] = (c->dims >= 2) ? L.hi[1] + 1 - L.lo[1] + 1 : 1;
+  e.lo[2] = (c->dims >= 3) ? L.lo[2] : 0; e.n[2] = (c->dims >= 3) ? L.hi[2] + 1 - L.lo[2] + 1 : 1;
+  Launch t(c, name);
+  const Walls wl = walls32(c);
   const Shape sh = work_shape32(e, c->dims, c->emf_chunk);
 #define E32_ARGS \
   dim3(sh.blocks), dim3(BLK), 0, c->stream, c->g, w, c->fx32, c->fy32, c->fz32, c->ee32, e, wl, sh.ty
@@ -237,12 +243,13 @@ void f32_step_body(bnz_ctx *c) {
   const Shape us = work_shape32(grid, c->dims, c->update_chunk);
   const int nb = us.blocks;
   Lims L0 = integr_lims(c, 0);
+  const Walls wl = walls32(c);
   fluxes32(c, 0, c->w32, c->b, L0);
-  emf32(c, c->w32, L0, "emf_pred");
+  if (c->dims > 1) emf32(c, c->w32, L0, "emf_pred");  // 1D: EMFs formed in the update
   {
     Pred32 a{c->w32, c->uh, c->b, c->fx32, c->fy32, c->fz32, c->ee32, c->bs32, c->ws32};
     Launch t(c, "update_pred");
-#define P32_ARGS dim3(nb), dim3(BLK), 0, c->stream, c->g, a, L0, c->dev, grid, us.ty
+#define P32_ARGS dim3(nb), dim3(BLK), 0, c->stream, c->g, a, L0, c->dev, grid, us.ty, wl
     if (c->dims == 3) hipLaunchKernelGGL(k32_update_pred<3>, P32_ARGS);
     else if (c->dims == 2) hipLaunchKernelGGL(k32_update_pred<2>, P32_ARGS);
     else hipLaunchKernelGGL(k32_update_pred<1>, P32_ARGS);
@@ -250,13 +257,13 @@ void f32_step_body(bnz_ctx *c) {
   }
   Lims L1 = integr_lims(c, c->order);
   fluxes32(c, c->order, c->ws32, c->bs32, L1);
-  emf32(c, c->ws32, L1, "emf_corr");
+  if (c->dims > 1) emf32(c, c->ws32, L1, "emf_corr");
   {
     Corr32 a{c->uh, c->ul, c->b, c->fx32, c->fy32, c->fz32, c->ee32, c->bs, c->w32};
     Launch t(c, "update_corr");
 #define C32_ARGS \
   dim3(nb), dim3(BLK), 0, c->stream, c->g, a, L1, c->dev, active_box(c), c->p.dt_fms_ref, c->part, \
-      grid, us.ty
+      grid, us.ty, wl
     if (c->dims == 3) hipLaunchKernelGGL(k32_update_corr<3>, C32_ARGS);
     else if (c->dims == 2) hipLaunchKernelGGL(k32_update_corr<2>, C32_ARGS);
     else hipLaunchKernelGGL(k32_update_corr<1>, C32_ARGS);

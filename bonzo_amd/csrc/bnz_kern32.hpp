@@ -131,19 +131,40 @@ __device__ __forceinline__ void emf32_edge(const Geo &g, const float *__restrict
 }
 
 
+// FP32 edge EMFs of the stage update: em(comp, off) is component comp at
+// the edge whose lower corner is cell c + off.  Emf32Global reads the ee
+// array of the EMF kernel.  On 1D grids Emf32x1D forms them from the x
+// fluxes as emf32_edge<1> does (ey = F_x[Bz], ez = -F_x[By], both zero on a
+// reflecting wall plane; a 1D cell reads no Ex), so 1D steps need no EMF
+// kernel.
+struct Emf32Global {
+  const float *ee;
+  long sv, c;
+  __device__ __forceinline__ float operator()(int comp, long off) const {
+    return ee[comp * sv + c + off];
+  }
+};
+struct Emf32x1D {
+  const float *fx;
+  long sv, c;
+  int i, wlo, whi;  // the cell's x index and the x wall planes (-1: none)
+  __device__ __forceinline__ float operator()(int comp, long off) const {
+    const int e = i + (int)off;  // 1D: off is 0 or 1
+    if (comp == 0 || e == wlo || e == whi) return 0.f;
+    return comp == 2 ? -fx[BY * sv + c + off] : fx[BZ * sv + c + off];
+  }
+};
+
 // Face-field update by Stokes (ct.pyx:15-91) in working type R from face
-// values b and FP32 EMFs: the lower faces of cell c (lo) and its upper faces
-// (up).  Neighbour offsets ox, oy, oz are zero where the cell needs no
+// values b and FP32 EMFs em: the lower faces of cell c (lo) and its upper
+// faces (up).  Neighbour offsets ox, oy, oz are zero where the cell needs no
 // neighbour (all loads stay in bounds; see the FP64 update_cell_d).
-template <int DIMS, typename R, typename BF>
-__device__ __forceinline__ void faces_d(const Geo &g, BF b, const float *__restrict__ ee,
-                                        double dt, int i, int j, int k, long c, long ox, long oy,
-                                        long oz, R *lo, R *up) {
-  const long sv = g.sv;
-  const float *ex = ee, *ey = ee + sv, *ez = ee + 2 * sv;
-  const R ez00 = ez[c], ez10 = ez[c + ox], ez01 = ez[c + oy], ez11 = ez[c + ox + oy];
-  const R ey00 = ey[c], ey10 = ey[c + ox], ey01 = ey[c + oz], ey11 = ey[c + ox + oz];
-  const R ex00 = ex[c], ex10 = ex[c + oy], ex01 = ex[c + oz], ex11 = ex[c + oy + oz];
+template <int DIMS, typename R, typename BF, class EM>
+__device__ __forceinline__ void faces_d(const Geo &g, BF b, const EM &em, double dt, int i, int j,
+                                        int k, long c, long ox, long oy, long oz, R *lo, R *up) {
+  const R ez00 = em(2, 0), ez10 = em(2, ox), ez01 = em(2, oy), ez11 = em(2, ox + oy);
+  const R ey00 = em(1, 0), ey10 = em(1, ox), ey01 = em(1, oz), ey11 = em(1, ox + oz);
+  const R ex00 = em(0, 0), ex10 = em(0, oy), ex01 = em(0, oz), ex11 = em(0, oy + oz);
   const R bxc = b(0, c), byc = b(1, c), bzc = b(2, c);
   const R bxp = b(0, c + ox), byp = b(1, c + oy), bzp = b(2, c + oz);
   const double dlx = g.dlf[0][i], dly = g.dlf[1][j], dlz = g.dlf[2][k];
@@ -220,6 +241,22 @@ struct Faces64 {
   __device__ double operator()(int a, long c) const { return b[a * sv + c]; }
 };
 
+// EMF source of a DIMS-dimensional update of cell (i, c): flux-derived in 1D.
+template <int DIMS>
+struct EmfSel32 {
+  static __device__ __forceinline__ Emf32Global get(const float *ee, const float *, long sv,
+                                                   long c, int, const Walls &) {
+    return Emf32Global{ee, sv, c};
+  }
+};
+template <>
+struct EmfSel32<1> {
+  static __device__ __forceinline__ Emf32x1D get(const float *, const float *fx, long sv, long c,
+                                                int i, const Walls &wl) {
+    return Emf32x1D{fx, sv, c, i, wl.at[0], wl.at[1]};
+  }
+};
+
 struct Pred32 {
   const float *w32, *uh;
   const double *b;
@@ -233,8 +270,8 @@ struct Pred32 {
 // ws outside the limits) depend on the roles.
 template <int DIMS>
 __device__ __forceinline__ void update32_pred(const Geo &g, const Pred32 &a, const Lims &L,
-                                              const double *__restrict__ dts, int i, int j,
-                                              int k) {
+                                              const double *__restrict__ dts, const Walls &wl,
+                                              int i, int j, int k) {
   const long c = ((long)k * g.nt[1] + j) * g.nt[0] + i, sv = g.sv;
   const double dt = 0.5 * dts[0];
   const Roles<DIMS> ro(g, L, i, j, k);
@@ -255,7 +292,8 @@ __device__ __forceinline__ void update32_pred(const Geo &g, const Pred32 &a, con
     }
   }
   float lo[3], up[3];
-  faces_d<DIMS, float>(g, Faces32{a.b, sv}, a.ee, dt, i, j, k, c, ro.ox, ro.oy, ro.oz, lo, up);
+  faces_d<DIMS, float>(g, Faces32{a.b, sv}, EmfSel32<DIMS>::get(a.ee, a.fx, sv, c, i, wl), dt, i,
+                       j, k, c, ro.ox, ro.oy, ro.oz, lo, up);
   const float dlx = (float)g.dlf[0][i], dly = (float)g.dlf[1][j], dlz = (float)g.dlf[2][k];
   const float dtdv = (float)(dt / (g.dlf[0][i] * g.dlf[1][j] * g.dlf[2][k]));
   const float ax = dly * dlz, ay = dlx * dlz, az = dlx * dly;
@@ -305,7 +343,8 @@ struct Corr32 {
 template <int DIMS>
 __device__ __forceinline__ double update32_corr(const Geo &g, const Corr32 &a, const Lims &L,
                                                 const double *__restrict__ dts, const Box &act,
-                                                int fms_ref, int i, int j, int k) {
+                                                int fms_ref, const Walls &wl, int i, int j,
+                                                int k) {
   const long c = ((long)k * g.nt[1] + j) * g.nt[0] + i, sv = g.sv;
   double cdsmax = 0.;
   const double dt = dts[0];
@@ -328,8 +367,8 @@ __device__ __forceinline__ double update32_corr(const Geo &g, const Corr32 &a, c
     }
   }
   double lo[3], up[3];
-  faces_d<DIMS, double>(g, Faces64{a.b, sv}, a.ee, dt, i, j, k, c, ro.ox, ro.oy, ro.oz, lo,
-                        up);
+  faces_d<DIMS, double>(g, Faces64{a.b, sv}, EmfSel32<DIMS>::get(a.ee, a.fx, sv, c, i, wl), dt,
+                        i, j, k, c, ro.ox, ro.oy, ro.oz, lo, up);
   const double dlx = g.dlf[0][i], dly = g.dlf[1][j], dlz = g.dlf[2][k];
   const double dtdv = dt / (dlx * dly * dlz);
   const double ax = dly * dlz, ay = dlx * dlz, az = dlx * dly;

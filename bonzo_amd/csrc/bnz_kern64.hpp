@@ -145,12 +145,32 @@ __device__ __forceinline__ void emf_edge(const Geo &g, const double *__restrict_
 //   Bx(i)   = b - dt/(dy dz) (dz Ez(j+1) - dz Ez(j)) + dt/(dy dz) (dy Ey(k+1) - dy Ey(k))
 //   By(j)   = b + dt/(dx dz) (dz Ez(i+1) - dz Ez(i)) - dt/(dx dz) (dx Ex(k+1) - dx Ex(k))
 //   Bz(k)   = b - dt/(dx dy) (dy Ey(i+1) - dy Ey(i)) + dt/(dx dy) (dx Ex(j+1) - dx Ex(j))
-template <int DIMS, bool PRED, bool DTRED>
-__device__ __forceinline__ double update_cell_d(const Geo &g, const double *w0,
+//
+// The edge EMFs come from em(comp, off): component comp at the edge whose
+// lower corner is cell c + off.  EmfGlobal reads the ee array of the EMF
+// kernel; on 1D grids Emf1D forms them from the x fluxes as that kernel
+// does (ey = F_x[Bz], ez = -F_x[By]; a 1D cell reads no Ex), so 1D steps
+// need no EMF kernel.
+struct EmfGlobal {
+  const double *ee;
+  long sv, c;
+  __device__ __forceinline__ double operator()(int comp, long off) const {
+    return ee[comp * sv + c + off];
+  }
+};
+struct Emf1D {
+  const double *fx;
+  long sv, c;
+  __device__ __forceinline__ double operator()(int comp, long off) const {
+    return comp == 2 ? -fx[BY * sv + c + off] : comp == 1 ? fx[BZ * sv + c + off] : 0.;
+  }
+};
+
+template <int DIMS, bool PRED, bool DTRED, class EMF>
+__device__ __forceinline__ double update_cell_e(const Geo &g, const double *w0,
                                                 const double *__restrict__ fx,
                                                 const double *__restrict__ fy,
-                                                const double *__restrict__ fz,
-                                                const double *__restrict__ ee,
+                                                const double *__restrict__ fz, const EMF &em,
                                                 const double *__restrict__ b0,
                                                 double *__restrict__ b1, double *w1,
                                                 const Lims &L, const double *__restrict__ dts,
@@ -178,10 +198,9 @@ __device__ __forceinline__ double update_cell_d(const Geo &g, const double *w0,
   for (int n = 0; n < NMODE; ++n) wv[n] = w0[n * sv + c];
   const double bxc = b0[c], byc = b0[sv + c], bzc = b0[2 * sv + c];
   const double bxp = b0[c + ox], byp = b0[sv + c + oy], bzp = b0[2 * sv + c + oz];
-  const double *ex = ee, *ey = ee + sv, *ez = ee + 2 * sv;
-  const double ez00 = ez[c], ez10 = ez[c + ox], ez01 = ez[c + oy], ez11 = ez[c + ox + oy];
-  const double ey00 = ey[c], ey10 = ey[c + ox], ey01 = ey[c + oz], ey11 = ey[c + ox + oz];
-  const double ex00 = ex[c], ex10 = ex[c + oy], ex01 = ex[c + oz], ex11 = ex[c + oy + oz];
+  const double ez00 = em(2, 0), ez10 = em(2, ox), ez01 = em(2, oy), ez11 = em(2, ox + oy);
+  const double ey00 = em(1, 0), ey10 = em(1, ox), ey01 = em(1, oz), ey11 = em(1, ox + oz);
+  const double ex00 = em(0, 0), ex10 = em(0, oy), ex01 = em(0, oz), ex11 = em(0, oy + oz);
   double fxl[NMODE - 3], fxu[NMODE - 3], fyl[NMODE - 3], fyu[NMODE - 3], fzl[NMODE - 3],
       fzu[NMODE - 3];
 #pragma unroll
@@ -284,6 +303,24 @@ __device__ __forceinline__ double update_cell_d(const Geo &g, const double *w0,
       j < act.lo[1] + act.n[1] && k >= act.lo[2] && k < act.lo[2] + act.n[2])
     return cds_cell(g, wn, fms_ref, i, j, k);
   return 0.;
+}
+
+template <int DIMS, bool PRED, bool DTRED>
+__device__ __forceinline__ double update_cell_d(const Geo &g, const double *w0,
+                                                const double *__restrict__ fx,
+                                                const double *__restrict__ fy,
+                                                const double *__restrict__ fz,
+                                                const double *__restrict__ ee,
+                                                const double *__restrict__ b0,
+                                                double *__restrict__ b1, double *w1,
+                                                const Lims &L, const double *__restrict__ dts,
+                                                const Box &act, int fms_ref, int i, int j, int k) {
+  const long c = ((long)k * g.nt[1] + j) * g.nt[0] + i;
+  if (DIMS == 1)
+    return update_cell_e<1, PRED, DTRED>(g, w0, fx, fy, fz, Emf1D{fx, g.sv, c}, b0, b1, w1, L,
+                                         dts, act, fms_ref, i, j, k);
+  return update_cell_e<DIMS, PRED, DTRED>(g, w0, fx, fy, fz, EmfGlobal{ee, g.sv, c}, b0, b1, w1,
+                                          L, dts, act, fms_ref, i, j, k);
 }
 
 // Same with the dimensionality read from g (batched boxes of mixed dims).

@@ -138,3 +138,19 @@ def test_fp32c_work_shapes_do_not_change_results():
             g.close()
         for r in res[1:]:
             assert np.array_equal(r, res[0])
+
+
+def test_brio_wu_walls_1d_fp32c():
+    """1D FP32C steps form their EMFs in the update (zero on a reflecting
+    wall plane) instead of an EMF kernel: a tube between reflecting walls
+    against the FP64 path."""
+    say("case: 1D tube 400 cells, reflecting walls, no normal field, 150 steps, fp32c vs fp64")
+    rng = np.random.Generator(np.random.PCG64(808))
+    s, prim, bfld = P.riemann_tube(400, rng)
+    prim[6][:] = 0.0  # no field through the walls
+    bfld[0][:] = 0.0
+    s, out = pair((s, prim, bfld), 150, bc=(2, 2, 0, 0, 0, 0))
+    e = errors(s, out)
+    say("  rel L1 per variable: " + " ".join(f"{x:.2e}" for x in e))
+    assert max(e[v] for v in (0, 1, 4, 7, 8, 10, 11)) <= 1e-5
+    assert out["fp32c"]["diag"]["nonfinite"] == 0
