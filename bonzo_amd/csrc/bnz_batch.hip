@@ -171,8 +171,12 @@ __global__ __launch_bounds__(BLK) void kb_dt(BoxState st, int nbox, double cour)
   }
 }
 
-// t += dt of the running boxes; counts the boxes still short of t_max.
-__global__ __launch_bounds__(BLK) void kb_time(BoxState st, int nbox) {
+// The end of one step and the dt of the next in one launch: t += dt of the
+// running boxes and the count of boxes still short of t_max, then the dt and
+// run flag of every box from the CFL maximum the corrector just folded (what
+// kb_dt at the start of the next step would compute from the same maximum
+// and time).
+__global__ __launch_bounds__(BLK) void kb_time_dt(BoxState st, int nbox, double cour) {
   __shared__ int cnt[BLK / 64];
   int n = 0;
   for (int b = threadIdx.x; b < nbox; b += BLK) {
@@ -182,7 +186,18 @@ __global__ __launch_bounds__(BLK) void kb_time(BoxState st, int nbox) {
       st.t[b] = t;
       st.steps[b] += 1;
     }
-    n += (t < st.tmax[b]) ? 1 : 0;
+    const double tmax = st.tmax[b];
+    n += (t < tmax) ? 1 : 0;
+    const double m = __longlong_as_double((long long)st.cmax[b]);
+    st.cmax[b] = 0ull;
+    int run = 0;
+    if (t < tmax) {
+      double dt = cour / m;
+      if (t + dt > tmax) dt = tmax - t;
+      st.dt[b] = dt;
+      run = 1;
+    }
+    st.run[b] = run;
   }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off, 64);
@@ -334,12 +349,13 @@ void b_ghost(bnz_batch *c) {
 
 // One VL step of every running box (vanleer.pyx:94-167): the same launch
 // sequence as the single grid's step.
+// One VL step of every running box; the box dt's were set by the previous
+// step's kb_time_dt (or by init_state's kb_dt).
 void b_step(bnz_batch *c) {
   const int fr = c->p.dt_fms_ref;
-  hipLaunchKernelGGL(kb_dt, dim3(1), dim3(BLK), 0, c->stream, c->S, c->nbox, c->p.cour);
   if (c->f32) {
     b32_step(c);
-    hipLaunchKernelGGL(kb_time, dim3(1), dim3(BLK), 0, c->stream, c->S, c->nbox);
+    hipLaunchKernelGGL(kb_time_dt, dim3(1), dim3(BLK), 0, c->stream, c->S, c->nbox, c->p.cour);
     return;
   }
   // predictor: donor cell, dt/2, limits of order 0
@@ -359,7 +375,7 @@ void b_step(bnz_batch *c) {
                      c->boxes, wlist(c, KUPD), c->S, c->P, fr);
   std::swap(c->P.b, c->P.bs);
   b_ghost(c);
-  hipLaunchKernelGGL(kb_time, dim3(1), dim3(BLK), 0, c->stream, c->S, c->nbox);
+  hipLaunchKernelGGL(kb_time_dt, dim3(1), dim3(BLK), 0, c->stream, c->S, c->nbox, c->p.cour);
 }
 
 int blaunch_check(bnz_batch *c) {
@@ -500,6 +516,8 @@ int bnz_batch_create(const bnz_params *p, const bnz_box *boxes, int nbox, bnz_ba
 
   if (hipSetDevice(p->device) != hipSuccess) return bfail(c, BNZ_EHIP, "hipSetDevice failed");
   BCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  BCHK(c, hipHostMalloc(&c->hrun, 2 * sizeof(int), hipHostMallocDefault));
+  for (int q = 0; q < 2; ++q) BCHK(c, hipEventCreateWithFlags(&c->ev[q], hipEventDisableTiming));
   const size_t cells = (size_t)c->ncell;
   struct {
     double **q;
@@ -588,6 +606,9 @@ int bnz_batch_destroy(bnz_batch *c) {
   if (c->bmap) (void)hipFree(c->bmap);
   if (c->ghost) (void)hipFree(c->ghost);
   if (c->gops) (void)hipFree(c->gops);
+  for (hipEvent_t e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->hrun) (void)hipHostFree(c->hrun);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return BNZ_OK;
@@ -658,6 +679,8 @@ int bnz_batch_init_state(bnz_batch *c) {
   else
     hipLaunchKernelGGL(kb_cfl, dim3(grid_for(c->nblk[KACT])), dim3(BLK), 0, c->stream,
                        c->boxes, wlist(c, KACT), c->S, c->P.w, c->p.dt_fms_ref);
+  // dt and run flags of the first step (later steps get theirs from kb_time_dt)
+  hipLaunchKernelGGL(kb_dt, dim3(1), dim3(BLK), 0, c->stream, c->S, c->nbox, c->p.cour);
   c->w_stale = false;
   int rc = blaunch_check(c);
   if (rc) return rc;
@@ -684,11 +707,14 @@ int bnz_batch_advance(bnz_batch *c, int max_steps, int use_graph, int *nsteps) {
     c->graph = g;
     BCHK(c, hipGraphInstantiate(&c->exec, c->graph, nullptr, nullptr, 0));
   }
-  // the host reads the running count after every REPLAYS graph replays (a
-  // replay after the last box finished only launches idle kernels)
+  // Steps go out in chunks (REPLAYS graph replays, or one eager step), each
+  // followed by a copy of the running-box count into a pinned slot.  The
+  // host reads a chunk's count while the next chunk runs, so the device does
+  // not drain between chunks; a chunk issued after the last box finished only
+  // launches idle kernels.
   constexpr int REPLAYS = 4;
-  int n = 0, nrun = 1;
-  while (nrun > 0 && (max_steps <= 0 || n < max_steps)) {
+  int n = 0;
+  auto chunk = [&](int slot) -> int {
     if (use_graph) {
       for (int r = 0; r < REPLAYS && (max_steps <= 0 || n < max_steps); ++r) {
         BCHK(c, hipGraphLaunch(c->exec, c->stream));
@@ -700,9 +726,22 @@ int bnz_batch_advance(bnz_batch *c, int max_steps, int use_graph, int *nsteps) {
     }
     int rc = blaunch_check(c);
     if (rc) return rc;
-    BCHK(c, hipMemcpyAsync(&nrun, c->S.nrun, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    BCHK(c, hipStreamSynchronize(c->stream));
+    BCHK(c, hipMemcpyAsync(c->hrun + slot, c->S.nrun, sizeof(int), hipMemcpyDeviceToHost,
+                           c->stream));
+    BCHK(c, hipEventRecord(c->ev[slot], c->stream));
+    return BNZ_OK;
+  };
+  int slot = 0;
+  int rc = chunk(slot);
+  if (rc) return rc;
+  for (;;) {
+    const bool more = max_steps <= 0 || n < max_steps;
+    if (more && (rc = chunk(slot ^ 1)) != 0) return rc;
+    BCHK(c, hipEventSynchronize(c->ev[slot]));
+    if (c->hrun[slot] == 0 || !more) break;
+    slot ^= 1;
   }
+  BCHK(c, hipStreamSynchronize(c->stream));
   if (nsteps) *nsteps = n;
   c->w_stale = c->f32;
   return BNZ_OK;

@@ -96,6 +96,8 @@ struct bnz_batch {
   hipGraph_t graph = nullptr;  // two captured steps
   hipGraphExec_t exec = nullptr;
   double *graph_b = nullptr;   // current face buffer when the graph was captured
+  int *hrun = nullptr;         // pinned host copies of the running-box count (2 slots)
+  hipEvent_t ev[2] = {nullptr, nullptr};
   bool ready = false;
   std::string err;
 };
