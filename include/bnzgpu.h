@@ -125,9 +125,10 @@ int bnz_load_state(bnz_ctx *ctx, const void *host, long bytes);
 
 /* Work shapes of the 3D Godunov-flux, edge-EMF and stage-update kernels:
    0 one thread per face/edge/cell in x-fastest order (the default), 1 tiles
-   of 64 x 4 cells ordered along z, n > 1 slab columns of n rows walked
-   plane by plane.  Results do not depend on the shape.  Chosen by the
-   autotuner (bonzo_amd/autotune.py); no counterpart in the reference. */
+   of 64 x 4 cells ordered along z (FP64; FP32C runs it flat), n > 1 slab
+   columns of n rows walked plane by plane.  Results do not depend on the
+   shape.  Chosen by the autotuner (bonzo_amd/autotune.py); no counterpart
+   in the reference. */
 int bnz_set_tuning(bnz_ctx *ctx, int flux_chunk, int emf_chunk, int update_chunk);
 
 /* Kernel timing on the context's stream: when on, every launch is bracketed
@@ -177,8 +178,10 @@ int bnz_batch_download(bnz_batch *b, int box, double *prim, double *bfld);
 int bnz_batch_init_state(bnz_batch *b);
 /* Steps until every box has reached its t_max, or max_steps (> 0) step
    launches.  use_graph: replay a hipGraph holding two captured steps (steps
-   are then launched in pairs, up to four pairs between host checks; a box
-   at its t_max idles through the rest).  *nsteps gets the number of step
+   are then launched in pairs).  Steps go out in chunks (four pairs, or one
+   eager step) and the host reads a chunk's count of running boxes while
+   the next chunk runs, so up to one chunk may run after the last box
+   finished (a box at its t_max idles).  *nsteps gets the number of step
    launches; bnz_batch_get_times gives the steps each box took. */
 int bnz_batch_advance(bnz_batch *b, int max_steps, int use_graph, int *nsteps);
 /* Per-box time and step count (arrays of nbox) */
