@@ -455,6 +455,7 @@ int bnz_batch_create(const bnz_params *p, const bnz_box *boxes, int nbox, bnz_ba
     d.g.sz = nt[0] * nt[1];
     d.g.gam = p->gam;
     d.g.gamm1i = 1. / (p->gam - 1.);
+    d.g.unif = 0;  // batch kernels read the per-cell geometry arrays
     d.off = off;
     for (int a = 0; a < 3; ++a) {
       std::vector<double> v[4];

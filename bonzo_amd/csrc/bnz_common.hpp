@@ -28,6 +28,27 @@ struct Geo {
   const double *icm[3];   // lv - lf       (interp_bc weights, ct.pyx:110-123)
   const double *icp[3];   // lf[i+1] - lv
   double gam, gamm1i;
+  // unif = 1 when each of the four arrays above is constant along every
+  // axis (bit for bit, e.g. 2^-8 wide cells); the constants are then here
+  int unif;
+  double udlf[3], udlfi[3], uicm[3], uicp[3];
+};
+
+// Geometry lookups of a kernel: GeoArr reads the per-cell arrays, GeoUni the
+// constants of a uniform grid (the same values, with no per-cell loads).
+struct GeoArr {
+  const Geo &g;
+  __device__ __forceinline__ double dlf(int a, int q) const { return g.dlf[a][q]; }
+  __device__ __forceinline__ double dlfi(int a, int q) const { return g.dlfi[a][q]; }
+  __device__ __forceinline__ double icm(int a, int q) const { return g.icm[a][q]; }
+  __device__ __forceinline__ double icp(int a, int q) const { return g.icp[a][q]; }
+};
+struct GeoUni {
+  const Geo &g;
+  __device__ __forceinline__ double dlf(int a, int) const { return g.udlf[a]; }
+  __device__ __forceinline__ double dlfi(int a, int) const { return g.udlfi[a]; }
+  __device__ __forceinline__ double icm(int a, int) const { return g.uicm[a]; }
+  __device__ __forceinline__ double icp(int a, int) const { return g.uicp[a]; }
 };
 
 struct GhostOp {
@@ -165,8 +186,9 @@ __device__ __forceinline__ void block_max_to(double v, double *part) {
 }
 
 // CFL speed of one active cell (new_dt.pyx:74-91).
-__device__ __forceinline__ double cds_cell(const Geo &g, const double *wv, int fms_ref, int i,
-                                           int j, int k) {
+template <class GM>
+__device__ __forceinline__ double cds_cell_g(const GM &gm, const Geo &g, const double *wv,
+                                             int fms_ref, int i, int j, int k) {
   const double cx = fabs(wv[VX]) + fms(wv, g.gam);
   double cy = 0., cz = 0.;
   if (g.dims >= 2) {
@@ -183,8 +205,12 @@ __device__ __forceinline__ double cds_cell(const Geo &g, const double *wv, int f
     if (!fms_ref) { wz[BX] = wv[BZ]; wz[BZ] = wv[BX]; }
     cz = fabs(wv[VZ]) + fms(wz, g.gam);
   }
-  double cds = fmax(fmax(cx / g.dlf[0][i], cy / g.dlf[1][j]), cz / g.dlf[2][k]);
+  double cds = fmax(fmax(cx / gm.dlf(0, i), cy / gm.dlf(1, j)), cz / gm.dlf(2, k));
   return (cds > 0.) ? cds : 0.;  // NaN never wins the reference's `>` test
+}
+__device__ __forceinline__ double cds_cell(const Geo &g, const double *wv, int fms_ref, int i,
+                                           int j, int k) {
+  return cds_cell_g(GeoArr{g}, g, wv, fms_ref, i, j, k);
 }
 
 }  // namespace bnz

@@ -59,7 +59,7 @@ __global__ __launch_bounds__(BLK) void k_emf(Geo g, const double *__restrict__ w
   emf_edge_d<DIMS>(g, w, fx, fy, fz, ee, i, j, k);
 }
 
-template <int DIMS, bool PRED, bool DTRED = false, bool TILED = false>
+template <int DIMS, bool PRED, bool DTRED = false, bool TILED = false, bool UG = false>
 __global__ __launch_bounds__(BLK) void k_update(Geo g, const double *w0, const double *__restrict__ fx,
                                                 const double *__restrict__ fy,
                                                 const double *__restrict__ fz,
@@ -72,8 +72,8 @@ __global__ __launch_bounds__(BLK) void k_update(Geo g, const double *w0, const d
   int i, j, k;
   double cdsmax = 0.;
   if (cell_index<TILED, 1, 2>(grid, ty, i, j, k))
-    cdsmax = update_cell_d<DIMS, PRED, DTRED>(g, w0, fx, fy, fz, ee, b0, b1, w1, L, dts, act,
-                                              fms_ref, i, j, k);
+    cdsmax = update_cell_d<DIMS, PRED, DTRED, UG>(g, w0, fx, fy, fz, ee, b0, b1, w1, L, dts, act,
+                                                  fms_ref, i, j, k);
   if (DTRED) block_max_to(cdsmax, part);
 }
 
@@ -505,9 +505,15 @@ void launch_update(bnz_ctx *c, bool pred, const Lims &L) {
 #define UPD_ARGS                                                                              \
   dim3(sh.blocks), dim3(BLK), 0, c->stream, c->g, c->w, c->fx, c->fy, c->fz, c->ee, c->b, c->bs, \
       w1, L, c->dev, grid, act, fr, c->part, sh.ty
+  // a uniform grid (every cell width bit for bit the same) takes its
+  // geometry from kernel arguments instead of per-cell loads
+  const bool ug = c->g.unif != 0;
   if (sh.tiled) {  // 3D only
     if (pred) hipLaunchKernelGGL((k_update<3, true, false, true>), UPD_ARGS);
     else hipLaunchKernelGGL((k_update<3, false, true, true>), UPD_ARGS);
+  } else if (c->dims == 3 && ug) {
+    if (pred) hipLaunchKernelGGL((k_update<3, true, false, false, true>), UPD_ARGS);
+    else hipLaunchKernelGGL((k_update<3, false, true, false, true>), UPD_ARGS);
   } else if (c->dims == 3) {
     if (pred) hipLaunchKernelGGL((k_update<3, true, false, false>), UPD_ARGS);
     else hipLaunchKernelGGL((k_update<3, false, true, false>), UPD_ARGS);
@@ -738,6 +744,16 @@ int bnz_create(const bnz_params *p, bnz_ctx **out) {
   }
   g.gam = p->gam;
   g.gamm1i = 1. / (p->gam - 1.);
+  g.unif = 1;
+  for (int q = 0; q < 3; ++q) {
+    const std::vector<double> *arr[4] = {&c->h_dlf[q], &c->h_dlfi[q], &c->h_icm[q], &c->h_icp[q]};
+    double *u[4] = {&g.udlf[q], &g.udlfi[q], &g.uicm[q], &g.uicp[q]};
+    for (int m = 0; m < 4; ++m) {
+      *u[m] = (*arr[m])[0];
+      for (double v : *arr[m])
+        if (v != *u[m]) g.unif = 0;
+    }
+  }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return BNZ_OK;
 }

@@ -16,6 +16,8 @@
 // hi + lo pair (not primitive variables): the low words are only meaningful
 // on the quantities that the update accumulates, and the predictor reads the
 // hi words alone.
+#include <type_traits>
+
 #include "bnz_kern32.hpp"
 
 using namespace bnz;
@@ -51,17 +53,18 @@ __global__ __launch_bounds__(BLK) void k32_emf(Geo g, const float *__restrict__ 
 }
 
 // Predictor (update32_pred), one thread per cell of the grid.
-template <int DIMS>
+template <int DIMS, bool UG = false>
 __global__ __launch_bounds__(BLK) void k32_update_pred(Geo g, Pred32 a, Lims L,
                                                        const double *__restrict__ dts,
                                                        Box grid, int ty, Walls wl) {
   int i, j, k;
   if (!cell_index<false, 1, 2>(grid, ty, i, j, k)) return;
-  update32_pred<DIMS>(g, a, L, dts, wl, i, j, k);
+  update32_pred<DIMS, typename std::conditional<UG, GeoUni, GeoArr>::type>(g, a, L, dts, wl, i, j,
+                                                                          k);
 }
 
 // Corrector (update32_corr) with the CFL maximum reduced per block.
-template <int DIMS>
+template <int DIMS, bool UG = false>
 __global__ __launch_bounds__(BLK) void k32_update_corr(Geo g, Corr32 a, Lims L,
                                                        const double *__restrict__ dts, Box act,
                                                        int fms_ref, double *__restrict__ part,
@@ -69,7 +72,8 @@ __global__ __launch_bounds__(BLK) void k32_update_corr(Geo g, Corr32 a, Lims L,
   int i, j, k;
   double cdsmax = 0.;
   if (cell_index<false, 1, 2>(grid, ty, i, j, k))
-    cdsmax = update32_corr<DIMS>(g, a, L, dts, act, fms_ref, wl, i, j, k);
+    cdsmax = update32_corr<DIMS, typename std::conditional<UG, GeoUni, GeoArr>::type>(
+        g, a, L, dts, act, fms_ref, wl, i, j, k);
   block_max_to(cdsmax, part);
 }
 
@@ -250,7 +254,8 @@ void f32_step_body(bnz_ctx *c) {
     Pred32 a{c->w32, c->uh, c->b, c->fx32, c->fy32, c->fz32, c->ee32, c->bs32, c->ws32};
     Launch t(c, "update_pred");
 #define P32_ARGS dim3(nb), dim3(BLK), 0, c->stream, c->g, a, L0, c->dev, grid, us.ty, wl
-    if (c->dims == 3) hipLaunchKernelGGL(k32_update_pred<3>, P32_ARGS);
+    if (c->dims == 3 && c->g.unif) hipLaunchKernelGGL((k32_update_pred<3, true>), P32_ARGS);
+    else if (c->dims == 3) hipLaunchKernelGGL(k32_update_pred<3>, P32_ARGS);
     else if (c->dims == 2) hipLaunchKernelGGL(k32_update_pred<2>, P32_ARGS);
     else hipLaunchKernelGGL(k32_update_pred<1>, P32_ARGS);
 #undef P32_ARGS
@@ -264,7 +269,8 @@ void f32_step_body(bnz_ctx *c) {
 #define C32_ARGS \
   dim3(nb), dim3(BLK), 0, c->stream, c->g, a, L1, c->dev, active_box(c), c->p.dt_fms_ref, c->part, \
       grid, us.ty, wl
-    if (c->dims == 3) hipLaunchKernelGGL(k32_update_corr<3>, C32_ARGS);
+    if (c->dims == 3 && c->g.unif) hipLaunchKernelGGL((k32_update_corr<3, true>), C32_ARGS);
+    else if (c->dims == 3) hipLaunchKernelGGL(k32_update_corr<3>, C32_ARGS);
     else if (c->dims == 2) hipLaunchKernelGGL(k32_update_corr<2>, C32_ARGS);
     else hipLaunchKernelGGL(k32_update_corr<1>, C32_ARGS);
 #undef C32_ARGS

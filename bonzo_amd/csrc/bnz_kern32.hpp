@@ -159,15 +159,15 @@ struct Emf32x1D {
 // values b and FP32 EMFs em: the lower faces of cell c (lo) and its upper
 // faces (up).  Neighbour offsets ox, oy, oz are zero where the cell needs no
 // neighbour (all loads stay in bounds; see the FP64 update_cell_d).
-template <int DIMS, typename R, typename BF, class EM>
-__device__ __forceinline__ void faces_d(const Geo &g, BF b, const EM &em, double dt, int i, int j,
+template <int DIMS, typename R, typename BF, class EM, class GM>
+__device__ __forceinline__ void faces_d(const GM &gm, BF b, const EM &em, double dt, int i, int j,
                                         int k, long c, long ox, long oy, long oz, R *lo, R *up) {
   const R ez00 = em(2, 0), ez10 = em(2, ox), ez01 = em(2, oy), ez11 = em(2, ox + oy);
   const R ey00 = em(1, 0), ey10 = em(1, ox), ey01 = em(1, oz), ey11 = em(1, ox + oz);
   const R ex00 = em(0, 0), ex10 = em(0, oy), ex01 = em(0, oz), ex11 = em(0, oy + oz);
   const R bxc = b(0, c), byc = b(1, c), bzc = b(2, c);
   const R bxp = b(0, c + ox), byp = b(1, c + oy), bzp = b(2, c + oz);
-  const double dlx = g.dlf[0][i], dly = g.dlf[1][j], dlz = g.dlf[2][k];
+  const double dlx = gm.dlf(0, i), dly = gm.dlf(1, j), dlz = gm.dlf(2, k);
   {
     const R dtda = (R)(dt / (dly * dlz));
     R v = bxc, vu = bxp;
@@ -268,7 +268,7 @@ struct Pred32 {
 // Predictor (dt/2): FP32 update of the hi words, FP64 pressure recovery.
 // All operands are loaded up front; only the stores (and the copy of w into
 // ws outside the limits) depend on the roles.
-template <int DIMS>
+template <int DIMS, class GM = GeoArr>
 __device__ __forceinline__ void update32_pred(const Geo &g, const Pred32 &a, const Lims &L,
                                               const double *__restrict__ dts, const Walls &wl,
                                               int i, int j, int k) {
@@ -292,10 +292,11 @@ __device__ __forceinline__ void update32_pred(const Geo &g, const Pred32 &a, con
     }
   }
   float lo[3], up[3];
-  faces_d<DIMS, float>(g, Faces32{a.b, sv}, EmfSel32<DIMS>::get(a.ee, a.fx, sv, c, i, wl), dt, i,
+  const GM gm{g};
+  faces_d<DIMS, float>(gm, Faces32{a.b, sv}, EmfSel32<DIMS>::get(a.ee, a.fx, sv, c, i, wl), dt, i,
                        j, k, c, ro.ox, ro.oy, ro.oz, lo, up);
-  const float dlx = (float)g.dlf[0][i], dly = (float)g.dlf[1][j], dlz = (float)g.dlf[2][k];
-  const float dtdv = (float)(dt / (g.dlf[0][i] * g.dlf[1][j] * g.dlf[2][k]));
+  const float dlx = (float)gm.dlf(0, i), dly = (float)gm.dlf(1, j), dlz = (float)gm.dlf(2, k);
+  const float dtdv = (float)(dt / (gm.dlf(0, i) * gm.dlf(1, j) * gm.dlf(2, k)));
   const float ax = dly * dlz, ay = dlx * dlz, az = dlx * dly;
   double u[NMODE], wv[NMODE];
 #pragma unroll
@@ -305,13 +306,13 @@ __device__ __forceinline__ void update32_pred(const Geo &g, const Pred32 &a, con
     if (DIMS >= 3) v -= dtdv * (az * fzu[n] - az * fzl[n]);
     u[n] = v;
   }
-  u[BX] = ((float)g.icm[0][i] * lo[0] + (float)g.icp[0][i] * up[0]) * (float)g.dlfi[0][i];
+  u[BX] = ((float)gm.icm(0, i) * lo[0] + (float)gm.icp(0, i) * up[0]) * (float)gm.dlfi(0, i);
   if (DIMS >= 2)
-    u[BY] = ((float)g.icm[1][j] * lo[1] + (float)g.icp[1][j] * up[1]) * (float)g.dlfi[1][j];
+    u[BY] = ((float)gm.icm(1, j) * lo[1] + (float)gm.icp(1, j) * up[1]) * (float)gm.dlfi(1, j);
   else
     u[BY] = lo[1];
   if (DIMS >= 3)
-    u[BZ] = ((float)g.icm[2][k] * lo[2] + (float)g.icp[2][k] * up[2]) * (float)g.dlfi[2][k];
+    u[BZ] = ((float)gm.icm(2, k) * lo[2] + (float)gm.icp(2, k) * up[2]) * (float)gm.dlfi(2, k);
   else
     u[BZ] = lo[2];
   cons2prim(wv, u, g.gam);
@@ -340,7 +341,7 @@ struct Corr32 {
 // Corrector (dt): FP64 update of the hi + lo state, FP64 pressure recovery,
 // returns the CFL speed of the new state (0 outside the active cells).  Operands loaded up front as in the
 // predictor.
-template <int DIMS>
+template <int DIMS, class GM = GeoArr>
 __device__ __forceinline__ double update32_corr(const Geo &g, const Corr32 &a, const Lims &L,
                                                 const double *__restrict__ dts, const Box &act,
                                                 int fms_ref, const Walls &wl, int i, int j,
@@ -367,9 +368,10 @@ __device__ __forceinline__ double update32_corr(const Geo &g, const Corr32 &a, c
     }
   }
   double lo[3], up[3];
-  faces_d<DIMS, double>(g, Faces64{a.b, sv}, EmfSel32<DIMS>::get(a.ee, a.fx, sv, c, i, wl), dt,
+  const GM gm{g};
+  faces_d<DIMS, double>(gm, Faces64{a.b, sv}, EmfSel32<DIMS>::get(a.ee, a.fx, sv, c, i, wl), dt,
                         i, j, k, c, ro.ox, ro.oy, ro.oz, lo, up);
-  const double dlx = g.dlf[0][i], dly = g.dlf[1][j], dlz = g.dlf[2][k];
+  const double dlx = gm.dlf(0, i), dly = gm.dlf(1, j), dlz = gm.dlf(2, k);
   const double dtdv = dt / (dlx * dly * dlz);
   const double ax = dly * dlz, ay = dlx * dlz, az = dlx * dly;
   double u[NMODE], wv[NMODE];
@@ -382,13 +384,13 @@ __device__ __forceinline__ double update32_corr(const Geo &g, const Corr32 &a, c
     u[n] = v;
     split(v, nh[n], nl[n]);
   }
-  u[BX] = (g.icm[0][i] * lo[0] + g.icp[0][i] * up[0]) * g.dlfi[0][i];
+  u[BX] = (gm.icm(0, i) * lo[0] + gm.icp(0, i) * up[0]) * gm.dlfi(0, i);
   if (DIMS >= 2)
-    u[BY] = (g.icm[1][j] * lo[1] + g.icp[1][j] * up[1]) * g.dlfi[1][j];
+    u[BY] = (gm.icm(1, j) * lo[1] + gm.icp(1, j) * up[1]) * gm.dlfi(1, j);
   else
     u[BY] = lo[1];
   if (DIMS >= 3)
-    u[BZ] = (g.icm[2][k] * lo[2] + g.icp[2][k] * up[2]) * g.dlfi[2][k];
+    u[BZ] = (gm.icm(2, k) * lo[2] + gm.icp(2, k) * up[2]) * gm.dlfi(2, k);
   else
     u[BZ] = lo[2];
   cons2prim(wv, u, g.gam);
@@ -405,7 +407,7 @@ __device__ __forceinline__ double update32_corr(const Geo &g, const Corr32 &a, c
     for (int n = 0; n < NMODE; ++n) a.w32[n * sv + c] = (float)wv[n];
     if (i >= act.lo[0] && i < act.lo[0] + act.n[0] && j >= act.lo[1] &&
         j < act.lo[1] + act.n[1] && k >= act.lo[2] && k < act.lo[2] + act.n[2])
-      cdsmax = cds_cell(g, wv, fms_ref, i, j, k);
+      cdsmax = cds_cell_g(gm, g, wv, fms_ref, i, j, k);
   }
   return cdsmax;
 }

@@ -3,6 +3,8 @@
 // kernels (bnz_batch.hip), so that a box of a batch follows exactly the
 // arithmetic of a single grid.
 #pragma once
+#include <type_traits>
+
 #include "bnz_common.hpp"
 
 namespace bnz {
@@ -166,8 +168,8 @@ struct Emf1D {
   }
 };
 
-template <int DIMS, bool PRED, bool DTRED, class EMF>
-__device__ __forceinline__ double update_cell_e(const Geo &g, const double *w0,
+template <int DIMS, bool PRED, bool DTRED, class EMF, class GM>
+__device__ __forceinline__ double update_cell_e(const Geo &g, const GM &gm, const double *w0,
                                                 const double *__restrict__ fx,
                                                 const double *__restrict__ fy,
                                                 const double *__restrict__ fz, const EMF &em,
@@ -216,7 +218,7 @@ __device__ __forceinline__ double update_cell_e(const Geo &g, const double *w0,
       fzu[n] = fz[n * sv + c + oz];
     }
   }
-  const double dlx = g.dlf[0][i], dly = g.dlf[1][j], dlz = g.dlf[2][k];
+  const double dlx = gm.dlf(0, i), dly = gm.dlf(1, j), dlz = gm.dlf(2, k);
 
   // ---- faces (advance_b); the lower faces of the cell and, for the cell
   // interpolation, its upper faces
@@ -276,13 +278,13 @@ __device__ __forceinline__ double update_cell_e(const Geo &g, const double *w0,
     if (DIMS >= 3) v -= dtdv * (az * fzu[n] - az * fzl[n]);
     u[n] = v;
   }
-  u[BX] = (g.icm[0][i] * bxl + g.icp[0][i] * bxu) * g.dlfi[0][i];
+  u[BX] = (gm.icm(0, i) * bxl + gm.icp(0, i) * bxu) * gm.dlfi(0, i);
   if (DIMS >= 2)
-    u[BY] = (g.icm[1][j] * byl + g.icp[1][j] * byu) * g.dlfi[1][j];
+    u[BY] = (gm.icm(1, j) * byl + gm.icp(1, j) * byu) * gm.dlfi(1, j);
   else
     u[BY] = byl;
   if (DIMS >= 3)
-    u[BZ] = (g.icm[2][k] * bzl + g.icp[2][k] * bzu) * g.dlfi[2][k];
+    u[BZ] = (gm.icm(2, k) * bzl + gm.icp(2, k) * bzu) * gm.dlfi(2, k);
   else
     u[BZ] = bzl;
   double wn[NMODE];
@@ -301,11 +303,11 @@ __device__ __forceinline__ double update_cell_e(const Geo &g, const double *w0,
   }
   if (DTRED && cell && i >= act.lo[0] && i < act.lo[0] + act.n[0] && j >= act.lo[1] &&
       j < act.lo[1] + act.n[1] && k >= act.lo[2] && k < act.lo[2] + act.n[2])
-    return cds_cell(g, wn, fms_ref, i, j, k);
+    return cds_cell_g(gm, g, wn, fms_ref, i, j, k);
   return 0.;
 }
 
-template <int DIMS, bool PRED, bool DTRED>
+template <int DIMS, bool PRED, bool DTRED, bool UG = false>
 __device__ __forceinline__ double update_cell_d(const Geo &g, const double *w0,
                                                 const double *__restrict__ fx,
                                                 const double *__restrict__ fy,
@@ -316,11 +318,13 @@ __device__ __forceinline__ double update_cell_d(const Geo &g, const double *w0,
                                                 const Lims &L, const double *__restrict__ dts,
                                                 const Box &act, int fms_ref, int i, int j, int k) {
   const long c = ((long)k * g.nt[1] + j) * g.nt[0] + i;
+  using GM = typename std::conditional<UG, GeoUni, GeoArr>::type;
+  const GM gm{g};
   if (DIMS == 1)
-    return update_cell_e<1, PRED, DTRED>(g, w0, fx, fy, fz, Emf1D{fx, g.sv, c}, b0, b1, w1, L,
-                                         dts, act, fms_ref, i, j, k);
-  return update_cell_e<DIMS, PRED, DTRED>(g, w0, fx, fy, fz, EmfGlobal{ee, g.sv, c}, b0, b1, w1,
-                                          L, dts, act, fms_ref, i, j, k);
+    return update_cell_e<1, PRED, DTRED>(g, gm, w0, fx, fy, fz, Emf1D{fx, g.sv, c}, b0, b1, w1,
+                                         L, dts, act, fms_ref, i, j, k);
+  return update_cell_e<DIMS, PRED, DTRED>(g, gm, w0, fx, fy, fz, EmfGlobal{ee, g.sv, c}, b0, b1,
+                                          w1, L, dts, act, fms_ref, i, j, k);
 }
 
 // Same with the dimensionality read from g (batched boxes of mixed dims).

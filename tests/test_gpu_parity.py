@@ -130,6 +130,15 @@ def test_turbulence_3d():
     compare(res, "turb24")
 
 
+def test_turbulence_3d_uniform_widths():
+    """32 cells on [0, 1] give cell widths of exactly 2^-5 on every axis, so
+    the update takes its geometry from kernel arguments (the uniform-grid
+    kernels); the result must still be the oracle's bit for bit."""
+    say("case: turbulence 32^3 (uniform widths), 5 steps, hlld+plm, periodic")
+    res = run_pair(P.turbulence(32, seed_b=33, seed_v=34), steps=5)
+    assert compare(res, "turb32")
+
+
 def test_brio_wu_1d_outflow_to_tmax():
     say("case: Brio-Wu 1D 400 cells, outflow, to t=0.08 with the tmax clip")
     res = run_pair(P.brio_wu(400, axis=0, transverse=0), tmax=0.08, bc=(1,) * 6)
