@@ -4,6 +4,7 @@ initial-condition generators produce solenoidal face fields.  No GPU calls."""
 import ctypes
 import os
 import re
+import sys
 import subprocess
 
 import numpy as np
@@ -155,3 +156,28 @@ def test_orszag_tang_ic_is_solenoidal():
     b = bfld.copy()
     b[0][sk, sj, s.ng + 64] = b[0][sk, sj, s.ng]
     assert np.abs(_divb(s, b)).max() < 1e-12
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU's share; --gpus N > 1 exits with a message
+    before touching any device."""
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
+
+
+def test_bench_kernel_accounting():
+    """Units and algorithmic bytes per launch of the bench's roofline figure:
+    flux3 over the union of the face ranges, updates over every cell."""
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    import bench
+    n = 256
+    nt = n + 6
+    m = (nt - 2 - 1) - 2 + 1          # PLM face range [2, nt - 3] per axis
+    assert bench.kernel_units("flux3_plm", n) == (m + 2) ** 3
+    assert bench.kernel_units("update_corr", n) == nt ** 3
+    assert bench.kernel_bytes("fp64", "update_corr", n) == 45 * 8 * nt ** 3
+    faces = 3 * (m + 1) * (m + 2) ** 2
+    assert bench.kernel_bytes("fp64", "flux3_plm", n) == (m + 2) ** 3 * 12 * 8 + faces * 8 * 8
