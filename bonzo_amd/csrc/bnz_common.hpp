@@ -1,5 +1,6 @@
-// bnz_common.hpp -- types and helpers shared by the FP64 (bnz_gpu.hip) and
-// FP32-compensated (bnz_gpu32.hip) translation units of libbnzgpu.so.
+// bnz_common.hpp -- types and helpers shared by every translation unit of
+// libbnzgpu.so: single grids in FP64 (bnz_gpu.hip) and FP32C (bnz_gpu32.hip),
+// batched boxes (bnz_batch.hip, bnz_batch32.hip).
 #pragma once
 #include <hip/hip_runtime.h>
 
