@@ -181,3 +181,18 @@ def test_bench_kernel_accounting():
     assert bench.kernel_bytes("fp64", "update_corr", n) == 45 * 8 * nt ** 3
     faces = 3 * (m + 1) * (m + 2) ** 2
     assert bench.kernel_bytes("fp64", "flux3_plm", n) == (m + 2) ** 3 * 12 * 8 + faces * 8 * 8
+
+
+def test_product_fails_loudly_without_the_library(monkeypatch):
+    """No CPU fallback: without libbnzgpu.so the package raises instead of
+    computing anything, and nothing under bonzo_amd/ imports the oracle."""
+    monkeypatch.setattr(_lib, "_lib", None)
+    monkeypatch.setattr(_lib, "LIB_PATH", "/nonexistent/libbnzgpu.so")
+    with pytest.raises(RuntimeError, match="not built"):
+        _lib.load()
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    for dirpath, _, files in os.walk(os.path.join(root, "bonzo_amd")):
+        for f in files:
+            if f.endswith(".py"):
+                text = open(os.path.join(dirpath, f)).read()
+                assert "import oracle" not in text and "from oracle" not in text, f
