@@ -263,6 +263,10 @@ struct bnz_ctx {
   int flux_chunk = 0, emf_chunk = 0, update_chunk = 0;
   // CFL partial maxima left by the last corrector for the next step
   int dt_parts = 0;
+  // bnz_advance: pinned copies of (t, tmax, step) per chunk of steps, and the
+  // events that mark them
+  double *hbuf = nullptr;
+  hipEvent_t hev[2] = {nullptr, nullptr};
   std::vector<KTime> ktimes;
   std::string err;
 };

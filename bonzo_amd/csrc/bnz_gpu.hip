@@ -39,7 +39,8 @@ __global__ __launch_bounds__(BLK) void k_flux3(Geo g, const double *__restrict__
                                                const double *__restrict__ bf,
                                                double *__restrict__ fx, double *__restrict__ fy,
                                                double *__restrict__ fz, Box box, Lims L,
-                                               int ty) {
+                                               int ty, const double *__restrict__ dts) {
+  if (dts[0] == 0.) return;  // a step past t_max (see k_update)
   int i, j, k;
   if (!cell_index<TILED, 1, 2>(box, ty, i, j, k)) return;
   flux_cell<ORDER, RS>(g, w, bf, fx, fy, fz, L, i, j, k);
@@ -71,7 +72,15 @@ __global__ __launch_bounds__(BLK) void k_update(Geo g, const double *w0, const d
                                                 int ty) {
   int i, j, k;
   double cdsmax = 0.;
-  if (cell_index<TILED, 1, 2>(grid, ty, i, j, k))
+  const bool have = cell_index<TILED, 1, 2>(grid, ty, i, j, k);
+  if (dts[0] == 0.) {  // a step past t_max: state unchanged, faces carried over
+    if (!PRED && have) {
+      const long c = ((long)k * g.nt[1] + j) * g.nt[0] + i;
+      for (int a = 0; a < 3; ++a) b1[a * g.sv + c] = b0[a * g.sv + c];
+    }
+    return;  // uniform over the workgroup; the CFL partials stay those of the state
+  }
+  if (have)
     cdsmax = update_cell_d<DIMS, PRED, DTRED, UG>(g, w0, fx, fy, fz, ee, b0, b1, w1, L, dts, act,
                                                   fms_ref, i, j, k);
   if (DTRED) block_max_to(cdsmax, part);
@@ -224,13 +233,19 @@ __global__ __launch_bounds__(BLK) void k_dt_final(const double *__restrict__ par
     for (int r = 1; r < BLK / 64; ++r) x = fmax(x, red[r]);
     double dt = use_fixed ? fixed_dt : cour / x;
     const double tnow = dev[1], tmax = dev[2];
-    if (!use_fixed && tmax >= 0. && tnow + dt > tmax) dt = tmax - tnow;
+    if (!use_fixed && tmax >= 0.) {
+      if (!(tnow < tmax))
+        dt = 0.;  // already at t_max: the step is a no-op (see k_update)
+      else if (tnow + dt > tmax)
+        dt = tmax - tnow;
+    }
     dev[0] = dt;
+    if (dt != 0.) dev[7] = dt;  // the dt of the last step taken
   }
 }
 
 __global__ void k_time_advance(double *dev) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
+  if (threadIdx.x == 0 && blockIdx.x == 0 && dev[0] != 0.) {
     dev[1] += dev[0];
     dev[3] += 1.0;
   }
@@ -430,10 +445,10 @@ void launch_fluxes(bnz_ctx *c, const double *w, const double *bf, const Lims &L)
   Launch t(c, ORDER ? "flux3_plm" : "flux3_dc");
   if (sh.tiled)
     hipLaunchKernelGGL((k_flux3<ORDER, RS, true>), dim3(sh.blocks), dim3(BLK), 0, c->stream, c->g,
-                       w, bf, c->fx, c->fy, c->fz, u, L, 0);
+                       w, bf, c->fx, c->fy, c->fz, u, L, 0, c->dev);
   else
     hipLaunchKernelGGL((k_flux3<ORDER, RS, false>), dim3(sh.blocks), dim3(BLK), 0, c->stream,
-                       c->g, w, bf, c->fx, c->fy, c->fz, u, L, sh.ty);
+                       c->g, w, bf, c->fx, c->fy, c->fz, u, L, sh.ty, c->dev);
 }
 
 void launch_flux_dispatch(bnz_ctx *c, int order, const double *w, const double *bf,
@@ -662,6 +677,8 @@ int bnz_create(const bnz_params *p, bnz_ctx **out) {
   *out = c;
   if (hipSetDevice(p->device) != hipSuccess) return fail(c, BNZ_EHIP, "hipSetDevice failed");
   HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  HIPCHK(c, hipHostMalloc(&c->hbuf, 8 * sizeof(double), hipHostMallocDefault));
+  for (int q = 0; q < 2; ++q) HIPCHK(c, hipEventCreateWithFlags(&c->hev[q], hipEventDisableTiming));
   const size_t cells = (size_t)c->ncell;
   c->f32 = (p->precision == BNZ_PREC_FP32C);
   // FP64 prim and faces always exist (upload, init, download); the FP64
@@ -768,6 +785,9 @@ int bnz_destroy(bnz_ctx *c) {
   if (c->ghost) (void)hipFree(c->ghost);
   if (c->gops) (void)hipFree(c->gops);
   f32_free(c);
+  for (hipEvent_t e : c->hev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->hbuf) (void)hipHostFree(c->hbuf);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return BNZ_OK;
@@ -881,18 +901,42 @@ int bnz_advance(bnz_ctx *c, double tmax, int max_steps, int *nsteps) {
   if (!c) return BNZ_EINVAL;
   if (!c->ready) return fail(c, BNZ_ESTATE, "bnz_init_state has not run");
   HIPCHK(c, hipMemcpyAsync(c->dev + 2, &tmax, sizeof(double), hipMemcpyHostToDevice, c->stream));
-  int n = 0;
-  double t = 0.;
-  HIPCHK(c, hipMemcpyAsync(&t, c->dev + 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  double d0[4];
+  HIPCHK(c, hipMemcpyAsync(d0, c->dev, sizeof(d0), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  while (t < tmax && (max_steps <= 0 || n < max_steps)) {
-    any_dt(c, 0, 0.);
-    any_step(c);
-    int rc = check_launch(c);
+  int n = 0;
+  if (d0[1] < tmax) {
+    // Steps go out in chunks; the host reads the time after a chunk while
+    // the next one runs.  A step that starts at t >= t_max gets dt = 0 and
+    // changes nothing (k_dt_final, k_update), so a chunk may overrun t_max.
+    constexpr int CHUNK = 8;
+    int launched = 0;
+    auto chunk = [&](int slot) -> int {
+      for (int r = 0; r < CHUNK && (max_steps <= 0 || launched < max_steps); ++r, ++launched) {
+        any_dt(c, 0, 0.);
+        any_step(c);
+      }
+      int rc = check_launch(c);
+      if (rc) return rc;
+      HIPCHK(c, hipMemcpyAsync(c->hbuf + 4 * slot, c->dev, 4 * sizeof(double),
+                               hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipEventRecord(c->hev[slot], c->stream));
+      return BNZ_OK;
+    };
+    int slot = 0;
+    int rc = chunk(slot);
     if (rc) return rc;
-    ++n;
-    HIPCHK(c, hipMemcpyAsync(&t, c->dev + 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    for (;;) {
+      const bool more = max_steps <= 0 || launched < max_steps;
+      if (more && (rc = chunk(slot ^ 1)) != 0) return rc;
+      HIPCHK(c, hipEventSynchronize(c->hev[slot]));
+      if (!(c->hbuf[4 * slot + 1] < tmax) || !more) break;
+      slot ^= 1;
+    }
+    double d1[4];
+    HIPCHK(c, hipMemcpyAsync(d1, c->dev, sizeof(d1), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    n = (int)(d1[3] - d0[3]);
   }
   if (nsteps) *nsteps = n;
   return BNZ_OK;
@@ -900,12 +944,12 @@ int bnz_advance(bnz_ctx *c, double tmax, int max_steps, int *nsteps) {
 
 int bnz_get_time(bnz_ctx *c, double *time, int *step, double *last_dt) {
   if (!c) return BNZ_EINVAL;
-  double d[4];
+  double d[8];
   HIPCHK(c, hipMemcpyAsync(d, c->dev, sizeof(d), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (time) *time = d[1];
   if (step) *step = (int)d[3];
-  if (last_dt) *last_dt = d[0];
+  if (last_dt) *last_dt = d[7];
   return BNZ_OK;
 }
 
@@ -937,7 +981,7 @@ int bnz_save_state(bnz_ctx *c, void *host, long bytes) {
   std::memcpy(h.magic, "BNZSTAT1", 8);
   h.precision = c->p.precision;
   for (int a = 0; a < 3; ++a) h.nt[a] = c->nt[a];
-  double d[4];
+  double d[8];
   HIPCHK(c, hipMemcpyAsync(d, c->dev, sizeof(d), hipMemcpyDeviceToHost, c->stream));
   h.cflmax = -1.;
   std::vector<double> parts((size_t)std::max(c->dt_parts, 0));
@@ -946,7 +990,7 @@ int bnz_save_state(bnz_ctx *c, void *host, long bytes) {
                              hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   for (double x : parts) h.cflmax = std::fmax(h.cflmax, x);
-  h.dt = d[0];
+  h.dt = d[7];
   h.time = d[1];
   h.step = d[3];
   char *out = (char *)host;
@@ -977,6 +1021,7 @@ int bnz_load_state(bnz_ctx *c, const void *host, long bytes) {
   }
   const double d[4] = {h.dt, h.time, -1., h.step};
   HIPCHK(c, hipMemcpyAsync(c->dev, d, sizeof(d), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->dev + 7, &h.dt, sizeof(double), hipMemcpyHostToDevice, c->stream));
   if (h.cflmax >= 0.) {
     HIPCHK(c, hipMemcpyAsync(c->part, &h.cflmax, sizeof(double), hipMemcpyHostToDevice,
                              c->stream));

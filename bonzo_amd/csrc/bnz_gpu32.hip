@@ -31,7 +31,9 @@ __global__ __launch_bounds__(BLK) void k32_flux3(Geo g, const float *__restrict_
                                                  const BF *__restrict__ bf,
                                                  float *__restrict__ fx, float *__restrict__ fy,
                                                  float *__restrict__ fz, Box box, Lims L,
-                                                 int *nguard, int ty) {
+                                                 int *nguard, int ty,
+                                                 const double *__restrict__ dts) {
+  if (dts[0] == 0.) return;  // a step past t_max (see k_update)
   int i, j, k;
   if (!cell_index<false, 1, 2>(box, ty, i, j, k)) return;
   int ng = 0;
@@ -58,6 +60,7 @@ __global__ __launch_bounds__(BLK) void k32_update_pred(Geo g, Pred32 a, Lims L,
                                                        const double *__restrict__ dts,
                                                        Box grid, int ty, Walls wl) {
   int i, j, k;
+  if (dts[0] == 0.) return;  // a step past t_max (see k_update)
   if (!cell_index<false, 1, 2>(grid, ty, i, j, k)) return;
   update32_pred<DIMS, typename std::conditional<UG, GeoUni, GeoArr>::type>(g, a, L, dts, wl, i, j,
                                                                           k);
@@ -71,7 +74,15 @@ __global__ __launch_bounds__(BLK) void k32_update_corr(Geo g, Corr32 a, Lims L,
                                                        Box grid, int ty, Walls wl) {
   int i, j, k;
   double cdsmax = 0.;
-  if (cell_index<false, 1, 2>(grid, ty, i, j, k))
+  const bool have = cell_index<false, 1, 2>(grid, ty, i, j, k);
+  if (dts[0] == 0.) {  // a step past t_max: state unchanged, faces carried over
+    if (have) {
+      const long c = ((long)k * g.nt[1] + j) * g.nt[0] + i;
+      for (int q = 0; q < 3; ++q) a.b1[q * g.sv + c] = a.b[q * g.sv + c];
+    }
+    return;
+  }
+  if (have)
     cdsmax = update32_corr<DIMS, typename std::conditional<UG, GeoUni, GeoArr>::type>(
         g, a, L, dts, act, fms_ref, wl, i, j, k);
   block_max_to(cdsmax, part);
@@ -126,7 +137,7 @@ void launch_fluxes32(bnz_ctx *c, const float *w, const BF *bf, const Lims &L) {
   const Shape sh = work_shape32(u, c->dims, c->flux_chunk);
   Launch t(c, ORDER ? "flux3_plm" : "flux3_dc");
   hipLaunchKernelGGL((k32_flux3<ORDER, RS, BF>), dim3(sh.blocks), dim3(BLK), 0, c->stream, c->g,
-                     w, bf, c->fx32, c->fy32, c->fz32, u, L, ng, sh.ty);
+                     w, bf, c->fx32, c->fy32, c->fz32, u, L, ng, sh.ty, c->dev);
 }
 
 template <typename BF>

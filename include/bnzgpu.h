@@ -89,11 +89,12 @@ int bnz_fill_ghosts(bnz_ctx *ctx);
 /* new_dt (src/integration/new_dt.pyx:18-135): cour / max (|v|+c_f)/dx */
 int bnz_new_dt(bnz_ctx *ctx, double *dt);
 /* One VL step with a caller-supplied dt: predictor, corrector, ghost fill,
-   prim2cons (src/integration/vanleer.pyx:100-163) */
+   prim2cons (src/integration/vanleer.pyx:100-163).  dt = 0 changes nothing. */
 int bnz_vl_step(bnz_ctx *ctx, double dt);
 /* vanleer.advance (src/integration/vanleer.pyx:87-167): steps with the
    device-computed CFL dt, clipped to tmax, until t >= tmax or max_steps
-   (max_steps <= 0: no limit).  *nsteps gets the number taken. */
+   (max_steps <= 0: no limit).  *nsteps gets the number taken.  Steps are
+   issued in chunks; one that would start at t >= tmax does nothing. */
 int bnz_advance(bnz_ctx *ctx, double tmax, int max_steps, int *nsteps);
 /* nsteps steps with the device CFL dt and no tmax clip, fully
    stream-ordered (no host synchronisation). */

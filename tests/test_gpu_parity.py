@@ -393,3 +393,32 @@ def test_simulation_restart_file(tmp_path):
         assert set(Simulation.GRID_NAMES + Simulation.FACE_NAMES) <= set(f.files)
     a.close()
     b.close()
+
+
+def test_advance_chunks_and_idle_steps_change_nothing():
+    """advance() issues steps in chunks and reads the time while the next
+    chunk runs; steps that start at t_max get dt = 0 and must leave the state,
+    time and step count alone.  Interrupted runs (max_steps) and repeated
+    calls at t_max give the uninterrupted run bit for bit, in both
+    precisions."""
+    from bonzo_amd._lib import Context
+    say("case: OT 32^2 and turbulence 16^3 to t=0.03: max_steps pieces, repeat at t_max, resume")
+    for prec in ("fp64", "fp32c"):
+        for s, prim, bfld in (P.orszag_tang(32), P.turbulence(16)):
+            ref = Context(s.nx, s.ny, s.nz, precision=prec)
+            ref.upload(prim, bfld)
+            ref.init_state()
+            n_ref = ref.advance(0.03)
+            st_ref, t_ref = ref.save_state(), ref.time()
+            g = Context(s.nx, s.ny, s.nz, precision=prec)
+            g.upload(prim, bfld)
+            g.init_state()
+            n = g.advance(0.03, max_steps=3) + g.advance(0.03, max_steps=5) + g.advance(0.03)
+            assert n == n_ref and g.time()[:2] == t_ref[:2]
+            assert g.time()[0] == 0.03
+            assert g.advance(0.03) == 0                 # already there: nothing happens
+            assert np.array_equal(g.save_state(), st_ref)
+            n2 = g.advance(0.04)
+            assert n2 > 0 and g.time()[0] == 0.04
+            ref.close()
+            g.close()
