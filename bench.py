@@ -92,7 +92,7 @@ def cpu_baseline(budget_s=15.0):
     t0 = time.perf_counter()
     o.advance_steps(1)
     one = time.perf_counter() - t0
-    k = max(1, min(50, int(budget_s / max(one, 1e-3))))
+    k = max(1, min(1000, int(budget_s / max(one, 1e-3))))
     t0 = time.perf_counter()
     o.advance_steps(k)
     el = time.perf_counter() - t0
