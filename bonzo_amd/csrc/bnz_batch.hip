@@ -446,9 +446,9 @@ int bnz_batch_create(const bnz_params *p, const bnz_box *boxes, int nbox, bnz_ba
     const int a2[3] = {s.nx + ng - 1, dims >= 2 ? s.ny + ng - 1 : 0, dims >= 3 ? s.nz + ng - 1 : 0};
     const int nact[3] = {s.nx, s.ny, s.nz};
     for (int a = 0; a < dims; ++a)
-      if ((s.bc[2 * a] == BNZ_BC_REFLECT || s.bc[2 * a + 1] == BNZ_BC_REFLECT) && nact[a] < ng)
-        return bfail(c, BNZ_EINVAL,
-                     "box " + std::to_string(b) + ": a reflecting axis needs at least ng cells");
+      if ((s.bc[2 * a] != BNZ_BC_OUTFLOW || s.bc[2 * a + 1] != BNZ_BC_OUTFLOW) && nact[a] < ng)
+        return bfail(c, BNZ_EINVAL, "box " + std::to_string(b) +
+                                        ": a periodic or reflecting axis needs at least ng cells");
     for (int q = 0; q < 3; ++q) d.g.nt[q] = nt[q];
     d.g.dims = dims;
     d.g.sy = nt[0];

@@ -669,8 +669,12 @@ int bnz_create(const bnz_params *p, bnz_ctx **out) {
   {
     const int nact[3] = {p->nx, p->ny, p->nz};
     for (int a = 0; a < c->dims; ++a)
-      if ((p->bc[2 * a] == BNZ_BC_REFLECT || p->bc[2 * a + 1] == BNZ_BC_REFLECT) && nact[a] < c->ng)
-        return bad(BNZ_EINVAL, "a reflecting axis needs at least ng active cells");
+      if ((p->bc[2 * a] != BNZ_BC_OUTFLOW || p->bc[2 * a + 1] != BNZ_BC_OUTFLOW) &&
+          nact[a] < c->ng)
+        // with fewer active cells than ghost layers the reference's periodic
+        // and reflect copies read ghost cells they have not filled yet
+        return bad(BNZ_EINVAL,
+                   "a periodic or reflecting axis needs at least ng active cells");
   }
   set_coordinates(c);
 

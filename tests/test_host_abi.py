@@ -75,6 +75,20 @@ def test_create_rejects_bad_selection_without_touching_gpu():
     rc = L.bnz_create(ctypes.byref(p), ctypes.byref(h))
     assert rc == _lib.BNZ_EINVAL
     L.bnz_destroy(h)
+    # fewer active cells than ghost layers on a periodic or reflecting axis:
+    # the reference's copies would read unfilled ghosts; outflow is fine
+    p.gam = 5.0 / 3.0
+    p.nx, p.ny, p.nz = 2, 1, 1
+    for flags, ok in (((0,) * 6, False), ((2, 2, 0, 0, 0, 0), False), ((1,) * 6, True)):
+        for q in range(6):
+            p.bc[q] = flags[q]
+        rc = L.bnz_create(ctypes.byref(p), ctypes.byref(h))
+        if ok:
+            assert rc in (_lib.BNZ_OK, _lib.BNZ_EHIP)   # validation passes (no GPU here)
+        else:
+            assert rc == _lib.BNZ_EINVAL
+            assert b"at least ng" in L.bnz_last_error(h)
+        L.bnz_destroy(h)
 
 
 def test_batch_create_rejects_bad_selection_without_touching_gpu():
