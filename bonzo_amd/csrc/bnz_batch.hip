@@ -26,13 +26,16 @@ using namespace bnzb;
 
 namespace {
 
-template <int ORDER, int RS>
+// The hot kernels (flux, EMF, update) are compiled per dimensionality and
+// launched over the work blocks of the boxes of that dimensionality: a 1D
+// box then runs 1D code with the registers 1D code needs.
+template <int ORDER, int RS, int DIMS>
 __global__ __launch_bounds__(BLK) void kb_flux(const BoxDev *__restrict__ boxes, WList wl,
                                                BoxState st, const double *__restrict__ w,
                                                const double *__restrict__ bf,
                                                double *__restrict__ fx, double *__restrict__ fy,
                                                double *__restrict__ fz) {
-  for (int wb = blockIdx.x; wb < wl.nblk; wb += gridDim.x) {
+  for (int wb = wl.lo + blockIdx.x; wb < wl.hi; wb += gridDim.x) {
     const int b = wl.bmap[wb];
     if (!st.run[b]) continue;
     const BoxDev &d = boxes[b];
@@ -42,18 +45,19 @@ __global__ __launch_bounds__(BLK) void kb_flux(const BoxDev *__restrict__ boxes,
     int i, j, k;
     box_idx(it, t, i, j, k);
     const long o = d.off;
-    flux_cell<ORDER, RS>(d.g, w + o, bf + o, fx + o, fy + o, fz + o, d.L[wl.kt == KF0 ? 0 : 1], i,
-                         j, k);
+    flux_cell<ORDER, RS, DIMS>(d.g, w + o, bf + o, fx + o, fy + o, fz + o,
+                               d.L[wl.kt == KF0 ? 0 : 1], i, j, k);
   }
 }
 
+template <int DIMS>
 __global__ __launch_bounds__(BLK) void kb_emf(const BoxDev *__restrict__ boxes, WList wl,
                                               BoxState st, const double *__restrict__ w,
                                               const double *__restrict__ fx,
                                               const double *__restrict__ fy,
                                               const double *__restrict__ fz,
                                               double *__restrict__ ee) {
-  for (int wb = blockIdx.x; wb < wl.nblk; wb += gridDim.x) {
+  for (int wb = wl.lo + blockIdx.x; wb < wl.hi; wb += gridDim.x) {
     const int b = wl.bmap[wb];
     if (!st.run[b]) continue;
     const BoxDev &d = boxes[b];
@@ -63,43 +67,37 @@ __global__ __launch_bounds__(BLK) void kb_emf(const BoxDev *__restrict__ boxes, 
     int i, j, k;
     box_idx(it, t, i, j, k);
     const long o = d.off;
-    emf_edge(d.g, w + o, fx + o, fy + o, fz + o, ee + o, i, j, k);
+    emf_edge_d<DIMS>(d.g, w + o, fx + o, fy + o, fz + o, ee + o, i, j, k);
   }
 }
 
 // Stage update of every cell of the running boxes.  The corrector also
 // folds the CFL maximum of the new state into the box's cmax (one atomic per
-// work block), and carries the faces of a finished box into the buffer that
-// becomes current after the face-buffer swap.
-template <bool PRED>
+// work block).  A finished box is left alone: its faces stay in the buffer
+// its last step wrote, which kb_time_dt tracks (BoxState::fpar) and
+// kb_face_fix makes current when an advance ends.
+template <bool PRED, int DIMS>
 __global__ __launch_bounds__(BLK) void kb_update(const BoxDev *__restrict__ boxes, WList wl,
                                                  BoxState st, Pools p, int fms_ref) {
-  for (int wb = blockIdx.x; wb < wl.nblk; wb += gridDim.x) {
+  for (int wb = wl.lo + blockIdx.x; wb < wl.hi; wb += gridDim.x) {
     const int b = wl.bmap[wb];
+    if (!st.run[b]) continue;
     const BoxDev &d = boxes[b];
     const long t = (long)(wb - d.blk0[wl.kt]) * BLK + threadIdx.x;
     const Box &it = d.items[wl.kt];
-    const bool valid = t < it.total();
     const long o = d.off;
-    if (!st.run[b]) {
-      if (!PRED && valid) {
-        const long sv = d.g.sv;
-#pragma unroll
-        for (int n = 0; n < 3; ++n) p.bs[n * sv + o + t] = p.b[n * sv + o + t];
-      }
-      continue;
-    }
     double cds = 0.;
-    if (valid) {
+    if (t < it.total()) {
       int i, j, k;
       box_idx(it, t, i, j, k);
       if (PRED)
-        update_cell<true, false>(d.g, p.w + o, p.fx + o, p.fy + o, p.fz + o, p.ee + o, p.b + o,
-                                 p.bs + o, p.ws + o, d.L[0], st.dt + b, d.act, fms_ref, i, j, k);
+        update_cell_d<DIMS, true, false>(d.g, p.w + o, p.fx + o, p.fy + o, p.fz + o, p.ee + o,
+                                         p.b + o, p.bs + o, p.ws + o, d.L[0], st.dt + b, d.act,
+                                         fms_ref, i, j, k);
       else
-        cds = update_cell<false, true>(d.g, p.w + o, p.fx + o, p.fy + o, p.fz + o, p.ee + o,
-                                       p.b + o, p.bs + o, p.w + o, d.L[1], st.dt + b, d.act,
-                                       fms_ref, i, j, k);
+        cds = update_cell_d<DIMS, false, true>(d.g, p.w + o, p.fx + o, p.fy + o, p.fz + o,
+                                               p.ee + o, p.b + o, p.bs + o, p.w + o, d.L[1],
+                                               st.dt + b, d.act, fms_ref, i, j, k);
     }
     if (!PRED) block_max_atomic(cds, st.cmax + b);
   }
@@ -109,7 +107,7 @@ __global__ __launch_bounds__(BLK) void kb_update(const BoxDev *__restrict__ boxe
 __global__ __launch_bounds__(BLK) void kb_cfl(const BoxDev *__restrict__ boxes, WList wl,
                                               BoxState st, const double *__restrict__ w,
                                               int fms_ref) {
-  for (int wb = blockIdx.x; wb < wl.nblk; wb += gridDim.x) {
+  for (int wb = wl.lo + blockIdx.x; wb < wl.hi; wb += gridDim.x) {
     const int b = wl.bmap[wb];
     const BoxDev &d = boxes[b];
     const long t = (long)(wb - d.blk0[wl.kt]) * BLK + threadIdx.x;
@@ -131,7 +129,7 @@ __global__ __launch_bounds__(BLK) void kb_cfl(const BoxDev *__restrict__ boxes, 
 // interp_bc over the active cells (init only, vanleer.pyx:44)
 __global__ __launch_bounds__(BLK) void kb_interp(const BoxDev *__restrict__ boxes, WList wl,
                                                  double *w, const double *b) {
-  for (int wb = blockIdx.x; wb < wl.nblk; wb += gridDim.x) {
+  for (int wb = wl.lo + blockIdx.x; wb < wl.hi; wb += gridDim.x) {
     const BoxDev &d = boxes[wl.bmap[wb]];
     const long t = (long)(wb - d.blk0[wl.kt]) * BLK + threadIdx.x;
     const Box &it = d.items[wl.kt];
@@ -185,6 +183,8 @@ __global__ __launch_bounds__(BLK) void kb_time_dt(BoxState st, int nbox, double 
       t += st.dt[b];
       st.t[b] = t;
       st.steps[b] += 1;
+    } else {
+      st.fpar[b] ^= 1;  // the face buffers swapped without this box
     }
     const double tmax = st.tmax[b];
     n += (t < tmax) ? 1 : 0;
@@ -245,6 +245,29 @@ __global__ __launch_bounds__(BLK) void kb_ghost_ops(long sv, double *w, double *
       b[a * sv + o.dst] = ((o.bits >> (9 + a)) & 1u) ? -v : v;
     }
   }
+}
+
+// Faces of the boxes that sat out an odd number of steps, from the other
+// buffer into the current one (all cells of the box), at the end of an
+// advance; kb_fpar_clear then resets the parities.
+__global__ __launch_bounds__(BLK) void kb_face_fix(const BoxDev *__restrict__ boxes, WList wl,
+                                                   BoxState st, double *__restrict__ b,
+                                                   const double *__restrict__ other) {
+  for (int wb = wl.lo + blockIdx.x; wb < wl.hi; wb += gridDim.x) {
+    const int bx = wl.bmap[wb];
+    if (!st.fpar[bx]) continue;
+    const BoxDev &d = boxes[bx];
+    const long t = (long)(wb - d.blk0[wl.kt]) * BLK + threadIdx.x;
+    if (t >= d.items[wl.kt].total()) continue;
+    const long sv = d.g.sv, c = d.off + t;
+#pragma unroll
+    for (int n = 0; n < 3; ++n) b[n * sv + c] = other[n * sv + c];
+  }
+}
+
+__global__ void kb_fpar_clear(int *fpar, int n) {
+  for (int q = threadIdx.x + blockIdx.x * blockDim.x; q < n; q += blockDim.x * gridDim.x)
+    fpar[q] = 0;
 }
 
 __global__ void kb_zero(unsigned long long *a, int n) {
@@ -321,8 +344,54 @@ Box mkbox(int i0, int ni, int j0, int nj, int k0, int nk) {
 
 template <int ORDER, int RS>
 void b_fluxes(bnz_batch *c, const double *w, const double *bf, int kt) {
-  hipLaunchKernelGGL((kb_flux<ORDER, RS>), dim3(grid_for(c->nblk[kt])), dim3(BLK), 0, c->stream,
-                     c->boxes, wlist(c, kt), c->S, w, bf, c->P.fx, c->P.fy, c->P.fz);
+  for (int d = 1; d <= 3; ++d) {
+    const WList wl = wlist_dims(c, kt, d);
+    if (wl.hi == wl.lo) continue;
+    const dim3 grid(grid_for(wl.hi - wl.lo));
+    if (d == 1)
+      hipLaunchKernelGGL((kb_flux<ORDER, RS, 1>), grid, dim3(BLK), 0, c->stream, c->boxes, wl,
+                         c->S, w, bf, c->P.fx, c->P.fy, c->P.fz);
+    else if (d == 2)
+      hipLaunchKernelGGL((kb_flux<ORDER, RS, 2>), grid, dim3(BLK), 0, c->stream, c->boxes, wl,
+                         c->S, w, bf, c->P.fx, c->P.fy, c->P.fz);
+    else
+      hipLaunchKernelGGL((kb_flux<ORDER, RS, 3>), grid, dim3(BLK), 0, c->stream, c->boxes, wl,
+                         c->S, w, bf, c->P.fx, c->P.fy, c->P.fz);
+  }
+}
+
+// EMFs of the 2D and 3D boxes (1D boxes form theirs in the update)
+void b_emf(bnz_batch *c, const double *w, int kt) {
+  for (int d = 2; d <= 3; ++d) {
+    const WList wl = wlist_dims(c, kt, d);
+    if (wl.hi == wl.lo) continue;
+    const dim3 grid(grid_for(wl.hi - wl.lo));
+    if (d == 2)
+      hipLaunchKernelGGL(kb_emf<2>, grid, dim3(BLK), 0, c->stream, c->boxes, wl, c->S, w,
+                         c->P.fx, c->P.fy, c->P.fz, c->P.ee);
+    else
+      hipLaunchKernelGGL(kb_emf<3>, grid, dim3(BLK), 0, c->stream, c->boxes, wl, c->S, w,
+                         c->P.fx, c->P.fy, c->P.fz, c->P.ee);
+  }
+}
+
+template <bool PRED>
+void b_update(bnz_batch *c) {
+  const int fr = c->p.dt_fms_ref;
+  for (int d = 1; d <= 3; ++d) {
+    const WList wl = wlist_dims(c, KUPD, d);
+    if (wl.hi == wl.lo) continue;
+    const dim3 grid(grid_for(wl.hi - wl.lo));
+    if (d == 1)
+      hipLaunchKernelGGL((kb_update<PRED, 1>), grid, dim3(BLK), 0, c->stream, c->boxes, wl, c->S,
+                         c->P, fr);
+    else if (d == 2)
+      hipLaunchKernelGGL((kb_update<PRED, 2>), grid, dim3(BLK), 0, c->stream, c->boxes, wl, c->S,
+                         c->P, fr);
+    else
+      hipLaunchKernelGGL((kb_update<PRED, 3>), grid, dim3(BLK), 0, c->stream, c->boxes, wl, c->S,
+                         c->P, fr);
+  }
 }
 
 void b_flux_dispatch(bnz_batch *c, int order, const double *w, const double *bf, int k0) {
@@ -352,7 +421,6 @@ void b_ghost(bnz_batch *c) {
 // One VL step of every running box; the box dt's were set by the previous
 // step's kb_time_dt (or by init_state's kb_dt).
 void b_step(bnz_batch *c) {
-  const int fr = c->p.dt_fms_ref;
   if (c->f32) {
     b32_step(c);
     hipLaunchKernelGGL(kb_time_dt, dim3(1), dim3(BLK), 0, c->stream, c->S, c->nbox, c->p.cour);
@@ -360,19 +428,13 @@ void b_step(bnz_batch *c) {
   }
   // predictor: donor cell, dt/2, limits of order 0
   b_flux_dispatch(c, 0, c->P.w, c->P.b, KF0);
-  if (c->nblk[KEM0] > 0)
-    hipLaunchKernelGGL(kb_emf, dim3(grid_for(c->nblk[KEM0])), dim3(BLK), 0, c->stream, c->boxes,
-                       wlist(c, KEM0), c->S, c->P.w, c->P.fx, c->P.fy, c->P.fz, c->P.ee);
-  hipLaunchKernelGGL(kb_update<true>, dim3(grid_for(c->nblk[KUPD])), dim3(BLK), 0, c->stream,
-                     c->boxes, wlist(c, KUPD), c->S, c->P, fr);
+  b_emf(c, c->P.w, KEM0);
+  b_update<true>(c);
   // corrector: reconstruction order, dt; b_{n+1} goes to the predictor's
   // face buffer, which then becomes the current one
   b_flux_dispatch(c, c->order, c->P.ws, c->P.bs, KF1);
-  if (c->nblk[KEM1] > 0)
-    hipLaunchKernelGGL(kb_emf, dim3(grid_for(c->nblk[KEM1])), dim3(BLK), 0, c->stream, c->boxes,
-                       wlist(c, KEM1), c->S, c->P.ws, c->P.fx, c->P.fy, c->P.fz, c->P.ee);
-  hipLaunchKernelGGL(kb_update<false>, dim3(grid_for(c->nblk[KUPD])), dim3(BLK), 0, c->stream,
-                     c->boxes, wlist(c, KUPD), c->S, c->P, fr);
+  b_emf(c, c->P.ws, KEM1);
+  b_update<false>(c);
   std::swap(c->P.b, c->P.bs);
   b_ghost(c);
   hipLaunchKernelGGL(kb_time_dt, dim3(1), dim3(BLK), 0, c->stream, c->S, c->nbox, c->p.cour);
@@ -500,19 +562,25 @@ int bnz_batch_create(const bnz_params *p, const bnz_box *boxes, int nbox, bnz_ba
   c->ncell = off;
   c->nghost = (long)pairs.size();
   c->ngops = (long)gops.size();
-  // work blocks: box b owns ceil(items / BLK) consecutive blocks of each list
+  // work blocks: box b owns ceil(items / BLK) consecutive blocks of each
+  // list; the 1D boxes come first, then the 2D and the 3D boxes
   std::vector<int> hmap;
   for (int kt = 0; kt < NK; ++kt) {
     c->bm_off[kt] = (int)hmap.size();
-    int nb = 0;
-    for (int b = 0; b < nbox; ++b) {
-      BoxDev &d = c->hbox[b];
-      d.blk0[kt] = nb;
-      const long blocks = (d.items[kt].total() + BLK - 1) / BLK;
-      for (long q = 0; q < blocks; ++q) hmap.push_back(b);
-      nb += (int)blocks;
+    long nb = 0;
+    for (int dims = 1; dims <= 3; ++dims) {
+      c->cls[kt][dims - 1] = (int)nb;
+      for (int b = 0; b < nbox; ++b) {
+        BoxDev &d = c->hbox[b];
+        if (d.g.dims != dims) continue;
+        d.blk0[kt] = (int)nb;
+        const long blocks = (d.items[kt].total() + BLK - 1) / BLK;
+        for (long q = 0; q < blocks; ++q) hmap.push_back(b);
+        nb += blocks;
+      }
     }
-    c->nblk[kt] = nb;
+    c->cls[kt][3] = (int)nb;
+    c->nblk[kt] = (int)nb;
   }
 
   if (hipSetDevice(p->device) != hipSuccess) return bfail(c, BNZ_EHIP, "hipSetDevice failed");
@@ -572,9 +640,10 @@ int bnz_batch_create(const bnz_params *p, const bnz_box *boxes, int nbox, bnz_ba
   BCHK(c, hipMalloc(&c->gops, std::max<size_t>(1, gops.size()) * sizeof(GhostOp)));
   if (!gops.empty())
     BCHK(c, hipMemcpy(c->gops, gops.data(), gops.size() * sizeof(GhostOp), hipMemcpyHostToDevice));
-  // box scalars: dt, t, tmax, cmax (8 bytes each), run, steps (4 bytes), nrun
+  // box scalars: dt, t, tmax, cmax (8 bytes each), run, steps, fpar (4 bytes),
+  // nrun
   const size_t nb = nbox;
-  const size_t words = 4 * nb + (2 * nb + 1 + 1) / 2;
+  const size_t words = 4 * nb + (3 * nb + 1 + 1) / 2;
   BCHK(c, hipMalloc(&c->sbuf, words * sizeof(double)));
   c->S.dt = c->sbuf;
   c->S.t = c->sbuf + nb;
@@ -582,7 +651,8 @@ int bnz_batch_create(const bnz_params *p, const bnz_box *boxes, int nbox, bnz_ba
   c->S.cmax = (unsigned long long *)(c->sbuf + 3 * nb);
   c->S.run = (int *)(c->sbuf + 4 * nb);
   c->S.steps = c->S.run + nb;
-  c->S.nrun = c->S.steps + nb;
+  c->S.fpar = c->S.steps + nb;
+  c->S.nrun = c->S.fpar + nb;
   std::vector<double> h(words, 0.);
   for (int b = 0; b < nbox; ++b) h[2 * nb + b] = boxes[b].tmax;
   BCHK(c, hipMemcpy(c->sbuf, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -670,9 +740,9 @@ int bnz_batch_init_state(bnz_batch *c) {
                      wlist(c, KACT), c->P.w, c->P.b);
   b_ghost(c);
   std::vector<double> z(nb, 0.);
-  std::vector<int> zi(nb + 1, 0);
+  std::vector<int> zi(2 * nb + 1, 0);  // steps, fpar, nrun
   BCHK(c, hipMemcpyAsync(c->S.t, z.data(), nb * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  BCHK(c, hipMemcpyAsync(c->S.steps, zi.data(), (nb + 1) * sizeof(int), hipMemcpyHostToDevice,
+  BCHK(c, hipMemcpyAsync(c->S.steps, zi.data(), (2 * nb + 1) * sizeof(int), hipMemcpyHostToDevice,
                          c->stream));
   hipLaunchKernelGGL(kb_zero, dim3(1), dim3(BLK), 0, c->stream, c->S.cmax, nb);
   if (c->f32)  // FP32C state from the FP64 prim, first CFL maxima from its FP32 prim
@@ -742,6 +812,12 @@ int bnz_batch_advance(bnz_batch *c, int max_steps, int use_graph, int *nsteps) {
     if (c->hrun[slot] == 0 || !more) break;
     slot ^= 1;
   }
+  // faces of the boxes that finished an odd number of steps ago
+  hipLaunchKernelGGL(kb_face_fix, dim3(grid_for(c->nblk[KUPD])), dim3(BLK), 0, c->stream,
+                     c->boxes, wlist(c, KUPD), c->S, c->P.b, c->P.bs);
+  hipLaunchKernelGGL(kb_fpar_clear, dim3(1), dim3(BLK), 0, c->stream, c->S.fpar, c->nbox);
+  rc = blaunch_check(c);
+  if (rc) return rc;
   BCHK(c, hipStreamSynchronize(c->stream));
   if (nsteps) *nsteps = n;
   c->w_stale = c->f32;

@@ -40,6 +40,8 @@ struct BoxState {
   int *run;                  // box advances in the current step
   int *steps;
   int *nrun;                 // boxes still short of t_max after the last step
+  int *fpar;                 // steps a finished box sat out, mod 2: its faces are
+                             // in the other face buffer when 1
 };
 
 // Maximum of v over the workgroup folded into *addr with one atomic (the
@@ -60,10 +62,12 @@ __device__ __forceinline__ void block_max_atomic(double v, unsigned long long *a
   __syncthreads();
 }
 
-// work list kt as seen by a kernel
+// Work list kt as seen by a kernel: work blocks [lo, hi).  Each list holds
+// the blocks of the 1D boxes, then those of the 2D and of the 3D boxes, so
+// the hot kernels run one launch per dimensionality present, compiled for it.
 struct WList {
   const int *bmap;  // work block -> box
-  int nblk;
+  int lo, hi;
   int kt;
 };
 
@@ -79,6 +83,7 @@ struct bnz_batch {
   std::vector<bnzb::BoxDev> hbox;
   int nblk[bnzb::NK] = {};
   int bm_off[bnzb::NK] = {};
+  int cls[bnzb::NK][4] = {};  // first work block of the 1D, 2D, 3D boxes; nblk
   long ncell = 0, nghost = 0, ngops = 0;
   hipStream_t stream = nullptr;
   bnzb::BoxDev *boxes = nullptr;
@@ -105,7 +110,12 @@ struct bnz_batch {
 namespace bnzb {
 
 inline WList wlist(const bnz_batch *c, int kt) {
-  return WList{c->bmap + c->bm_off[kt], c->nblk[kt], kt};
+  return WList{c->bmap + c->bm_off[kt], 0, c->nblk[kt], kt};
+}
+
+// the work blocks of list kt that belong to boxes of dimensionality dims
+inline WList wlist_dims(const bnz_batch *c, int kt, int dims) {
+  return WList{c->bmap + c->bm_off[kt], c->cls[kt][dims - 1], c->cls[kt][dims], kt};
 }
 
 // FP32-compensated step (bnz_batch32.hip)

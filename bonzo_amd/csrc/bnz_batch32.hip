@@ -22,14 +22,16 @@ __device__ __forceinline__ Walls box_walls(const BoxDev &d) {
   return wl;
 }
 
-template <int ORDER, int RS, typename BF>
+// The flux, EMF and update kernels are compiled per dimensionality and
+// launched over the work blocks of the boxes of that dimensionality.
+template <int ORDER, int RS, typename BF, int DIMS>
 __global__ __launch_bounds__(BLK) void kb32_flux(const BoxDev *__restrict__ boxes, WList wl,
                                                  BoxState st, const float *__restrict__ w,
                                                  const BF *__restrict__ bf,
                                                  float *__restrict__ fx, float *__restrict__ fy,
                                                  float *__restrict__ fz, int *nguard) {
   int ng = 0;
-  for (int wb = blockIdx.x; wb < wl.nblk; wb += gridDim.x) {
+  for (int wb = wl.lo + blockIdx.x; wb < wl.hi; wb += gridDim.x) {
     const int b = wl.bmap[wb];
     if (!st.run[b]) continue;
     const BoxDev &d = boxes[b];
@@ -39,19 +41,20 @@ __global__ __launch_bounds__(BLK) void kb32_flux(const BoxDev *__restrict__ boxe
     int i, j, k;
     box_idx(it, t, i, j, k);
     const long o = d.off;
-    flux32_cell<ORDER, RS>(d.g, w + o, bf + o, fx + o, fy + o, fz + o,
-                           d.L[wl.kt == KF0 ? 0 : 1], i, j, k, ng);
+    flux32_cell<ORDER, RS, BF, DIMS>(d.g, w + o, bf + o, fx + o, fy + o, fz + o,
+                                     d.L[wl.kt == KF0 ? 0 : 1], i, j, k, ng);
   }
   if (ng) atomicAdd(nguard, ng);
 }
 
+template <int DIMS>
 __global__ __launch_bounds__(BLK) void kb32_emf(const BoxDev *__restrict__ boxes, WList wl,
                                                 BoxState st, const float *__restrict__ w,
                                                 const float *__restrict__ fx,
                                                 const float *__restrict__ fy,
                                                 const float *__restrict__ fz,
                                                 float *__restrict__ ee) {
-  for (int wb = blockIdx.x; wb < wl.nblk; wb += gridDim.x) {
+  for (int wb = wl.lo + blockIdx.x; wb < wl.hi; wb += gridDim.x) {
     const int b = wl.bmap[wb];
     if (!st.run[b]) continue;
     const BoxDev &d = boxes[b];
@@ -61,59 +64,37 @@ __global__ __launch_bounds__(BLK) void kb32_emf(const BoxDev *__restrict__ boxes
     int i, j, k;
     box_idx(it, t, i, j, k);
     const long o = d.off;
-    const Walls wa = box_walls(d);
-    if (d.g.dims == 3)
-      emf32_edge<3>(d.g, w + o, fx + o, fy + o, fz + o, ee + o, wa, i, j, k);
-    else if (d.g.dims == 2)
-      emf32_edge<2>(d.g, w + o, fx + o, fy + o, fz + o, ee + o, wa, i, j, k);
-    else
-      emf32_edge<1>(d.g, w + o, fx + o, fy + o, fz + o, ee + o, wa, i, j, k);
+    emf32_edge<DIMS>(d.g, w + o, fx + o, fy + o, fz + o, ee + o, box_walls(d), i, j, k);
   }
 }
 
 // Stage update of every cell of the running boxes (update32_pred /
-// update32_corr); the corrector folds the CFL maximum into the box's cmax
-// and carries the faces of a finished box into the buffer that becomes
-// current after the face-buffer swap.
-template <bool PRED>
+// update32_corr); the corrector folds the CFL maximum into the box's cmax.
+// A finished box is left alone (its faces: BoxState::fpar, kb_face_fix).
+template <bool PRED, int DIMS>
 __global__ __launch_bounds__(BLK) void kb32_update(const BoxDev *__restrict__ boxes, WList wl,
                                                    BoxState st, Pools P, Pools32 Q,
                                                    int fms_ref) {
-  for (int wb = blockIdx.x; wb < wl.nblk; wb += gridDim.x) {
+  for (int wb = wl.lo + blockIdx.x; wb < wl.hi; wb += gridDim.x) {
     const int b = wl.bmap[wb];
+    if (!st.run[b]) continue;
     const BoxDev &d = boxes[b];
     const long t = (long)(wb - d.blk0[wl.kt]) * BLK + threadIdx.x;
     const Box &it = d.items[wl.kt];
-    const bool valid = t < it.total();
     const long o = d.off;
-    if (!st.run[b]) {
-      if (!PRED && valid) {
-        const long sv = d.g.sv;
-#pragma unroll
-        for (int n = 0; n < 3; ++n) P.bs[n * sv + o + t] = P.b[n * sv + o + t];
-      }
-      continue;
-    }
     double cds = 0.;
-    if (valid) {
+    if (t < it.total()) {
       int i, j, k;
       box_idx(it, t, i, j, k);
       const Walls wa = box_walls(d);
       if (PRED) {
         const Pred32 a{Q.w32 + o, Q.uh + o, P.b + o, Q.fx + o, Q.fy + o,
                        Q.fz + o,  Q.ee + o, Q.bs32 + o, Q.ws32 + o};
-        if (d.g.dims == 3) update32_pred<3>(d.g, a, d.L[0], st.dt + b, wa, i, j, k);
-        else if (d.g.dims == 2) update32_pred<2>(d.g, a, d.L[0], st.dt + b, wa, i, j, k);
-        else update32_pred<1>(d.g, a, d.L[0], st.dt + b, wa, i, j, k);
+        update32_pred<DIMS>(d.g, a, d.L[0], st.dt + b, wa, i, j, k);
       } else {
         const Corr32 a{Q.uh + o, Q.ul + o, P.b + o, Q.fx + o, Q.fy + o,
                        Q.fz + o, Q.ee + o, P.bs + o, Q.w32 + o};
-        if (d.g.dims == 3)
-          cds = update32_corr<3>(d.g, a, d.L[1], st.dt + b, d.act, fms_ref, wa, i, j, k);
-        else if (d.g.dims == 2)
-          cds = update32_corr<2>(d.g, a, d.L[1], st.dt + b, d.act, fms_ref, wa, i, j, k);
-        else
-          cds = update32_corr<1>(d.g, a, d.L[1], st.dt + b, d.act, fms_ref, wa, i, j, k);
+        cds = update32_corr<DIMS>(d.g, a, d.L[1], st.dt + b, d.act, fms_ref, wa, i, j, k);
       }
     }
     if (!PRED) block_max_atomic(cds, st.cmax + b);
@@ -123,7 +104,7 @@ __global__ __launch_bounds__(BLK) void kb32_update(const BoxDev *__restrict__ bo
 // FP64 prim -> FP32C state over every cell of every box (init).
 __global__ __launch_bounds__(BLK) void kb32_from64(const BoxDev *__restrict__ boxes, WList wl,
                                                    Pools P, Pools32 Q) {
-  for (int wb = blockIdx.x; wb < wl.nblk; wb += gridDim.x) {
+  for (int wb = wl.lo + blockIdx.x; wb < wl.hi; wb += gridDim.x) {
     const BoxDev &d = boxes[wl.bmap[wb]];
     const long t = (long)(wb - d.blk0[wl.kt]) * BLK + threadIdx.x;
     if (t >= d.items[wl.kt].total()) continue;
@@ -136,7 +117,7 @@ __global__ __launch_bounds__(BLK) void kb32_from64(const BoxDev *__restrict__ bo
 __global__ __launch_bounds__(BLK) void kb32_cfl(const BoxDev *__restrict__ boxes, WList wl,
                                                 BoxState st, const float *__restrict__ w32,
                                                 int fms_ref) {
-  for (int wb = blockIdx.x; wb < wl.nblk; wb += gridDim.x) {
+  for (int wb = wl.lo + blockIdx.x; wb < wl.hi; wb += gridDim.x) {
     const int b = wl.bmap[wb];
     const BoxDev &d = boxes[b];
     const long t = (long)(wb - d.blk0[wl.kt]) * BLK + threadIdx.x;
@@ -154,7 +135,7 @@ __global__ __launch_bounds__(BLK) void kb32_cfl(const BoxDev *__restrict__ boxes
 // FP32C state -> FP64 prim of the active cells of every box (download).
 __global__ __launch_bounds__(BLK) void kb32_to64(const BoxDev *__restrict__ boxes, WList wl,
                                                  Pools P, Pools32 Q) {
-  for (int wb = blockIdx.x; wb < wl.nblk; wb += gridDim.x) {
+  for (int wb = wl.lo + blockIdx.x; wb < wl.hi; wb += gridDim.x) {
     const BoxDev &d = boxes[wl.bmap[wb]];
     const long t = (long)(wb - d.blk0[wl.kt]) * BLK + threadIdx.x;
     const Box &it = d.items[wl.kt];
@@ -182,9 +163,20 @@ __global__ __launch_bounds__(BLK) void kb32_ghost_ops(Ghost32 p, const GhostOp *
 
 template <int ORDER, int RS, typename BF>
 void flux_launch(bnz_batch *c, const float *w, const BF *bf, int kt) {
-  hipLaunchKernelGGL((kb32_flux<ORDER, RS, BF>), dim3(grid_for(c->nblk[kt])), dim3(BLK), 0,
-                     c->stream, c->boxes, wlist(c, kt), c->S, w, bf, c->Q.fx, c->Q.fy, c->Q.fz,
-                     c->nguard);
+  for (int d = 1; d <= 3; ++d) {
+    const WList wl = wlist_dims(c, kt, d);
+    if (wl.hi == wl.lo) continue;
+    const dim3 grid(grid_for(wl.hi - wl.lo));
+    if (d == 1)
+      hipLaunchKernelGGL((kb32_flux<ORDER, RS, BF, 1>), grid, dim3(BLK), 0, c->stream, c->boxes,
+                         wl, c->S, w, bf, c->Q.fx, c->Q.fy, c->Q.fz, c->nguard);
+    else if (d == 2)
+      hipLaunchKernelGGL((kb32_flux<ORDER, RS, BF, 2>), grid, dim3(BLK), 0, c->stream, c->boxes,
+                         wl, c->S, w, bf, c->Q.fx, c->Q.fy, c->Q.fz, c->nguard);
+    else
+      hipLaunchKernelGGL((kb32_flux<ORDER, RS, BF, 3>), grid, dim3(BLK), 0, c->stream, c->boxes,
+                         wl, c->S, w, bf, c->Q.fx, c->Q.fy, c->Q.fz, c->nguard);
+  }
 }
 
 template <typename BF>
@@ -199,10 +191,38 @@ void fluxes(bnz_batch *c, int order, const float *w, const BF *bf, int kt) {
   }
 }
 
+// EMFs of the 2D and 3D boxes (1D boxes form theirs in the update)
 void emfs(bnz_batch *c, const float *w, int kt) {
-  if (c->nblk[kt] == 0) return;  // only 1D boxes: their EMFs are formed in the update
-  hipLaunchKernelGGL(kb32_emf, dim3(grid_for(c->nblk[kt])), dim3(BLK), 0, c->stream, c->boxes,
-                     wlist(c, kt), c->S, w, c->Q.fx, c->Q.fy, c->Q.fz, c->Q.ee);
+  for (int d = 2; d <= 3; ++d) {
+    const WList wl = wlist_dims(c, kt, d);
+    if (wl.hi == wl.lo) continue;
+    const dim3 grid(grid_for(wl.hi - wl.lo));
+    if (d == 2)
+      hipLaunchKernelGGL(kb32_emf<2>, grid, dim3(BLK), 0, c->stream, c->boxes, wl, c->S, w,
+                         c->Q.fx, c->Q.fy, c->Q.fz, c->Q.ee);
+    else
+      hipLaunchKernelGGL(kb32_emf<3>, grid, dim3(BLK), 0, c->stream, c->boxes, wl, c->S, w,
+                         c->Q.fx, c->Q.fy, c->Q.fz, c->Q.ee);
+  }
+}
+
+template <bool PRED>
+void updates(bnz_batch *c) {
+  const int fr = c->p.dt_fms_ref;
+  for (int d = 1; d <= 3; ++d) {
+    const WList wl = wlist_dims(c, KUPD, d);
+    if (wl.hi == wl.lo) continue;
+    const dim3 grid(grid_for(wl.hi - wl.lo));
+    if (d == 1)
+      hipLaunchKernelGGL((kb32_update<PRED, 1>), grid, dim3(BLK), 0, c->stream, c->boxes, wl,
+                         c->S, c->P, c->Q, fr);
+    else if (d == 2)
+      hipLaunchKernelGGL((kb32_update<PRED, 2>), grid, dim3(BLK), 0, c->stream, c->boxes, wl,
+                         c->S, c->P, c->Q, fr);
+    else
+      hipLaunchKernelGGL((kb32_update<PRED, 3>), grid, dim3(BLK), 0, c->stream, c->boxes, wl,
+                         c->S, c->P, c->Q, fr);
+  }
 }
 
 Ghost32 ghost_arrays(bnz_batch *c) {
@@ -224,15 +244,12 @@ namespace bnzb {
 // and time kernels of the caller (the sequence of the single-grid
 // f32_step_body).
 void b32_step(bnz_batch *c) {
-  const int fr = c->p.dt_fms_ref;
   fluxes(c, 0, c->Q.w32, (const double *)c->P.b, KF0);
   emfs(c, c->Q.w32, KEM0);
-  hipLaunchKernelGGL(kb32_update<true>, dim3(grid_for(c->nblk[KUPD])), dim3(BLK), 0, c->stream,
-                     c->boxes, wlist(c, KUPD), c->S, c->P, c->Q, fr);
+  updates<true>(c);
   fluxes(c, c->order, c->Q.ws32, (const float *)c->Q.bs32, KF1);
   emfs(c, c->Q.ws32, KEM1);
-  hipLaunchKernelGGL(kb32_update<false>, dim3(grid_for(c->nblk[KUPD])), dim3(BLK), 0,
-                     c->stream, c->boxes, wlist(c, KUPD), c->S, c->P, c->Q, fr);
+  updates<false>(c);
   std::swap(c->P.b, c->P.bs);
   const Ghost32 p = ghost_arrays(c);
   if (c->nghost > 0)

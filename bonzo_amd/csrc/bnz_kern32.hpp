@@ -54,15 +54,16 @@ __device__ __forceinline__ void flux32_face(const Geo &g, const float *__restric
 
 
 // All face fluxes of one stage whose lower cell is (i, j, k) (as the FP64
-// flux_cell); ng counts guarded HLLD star denominators.
-template <int ORDER, int RS, typename BF>
+// flux_cell, DIMS included); ng counts guarded HLLD star denominators.
+template <int ORDER, int RS, typename BF, int DIMS = 0>
 __device__ __forceinline__ void flux32_cell(const Geo &g, const float *__restrict__ w,
                                             const BF *__restrict__ bf, float *__restrict__ fx,
                                             float *__restrict__ fy, float *__restrict__ fz,
                                             const Lims &L, int i, int j, int k, int &ng) {
+  const int dims = DIMS > 0 ? DIMS : g.dims;
   if (i >= L.lo[0]) flux32_face<0, ORDER, RS>(g, w, bf, fx, i, j, k, ng);
-  if (g.dims >= 2 && j >= L.lo[1]) flux32_face<1, ORDER, RS>(g, w, bf, fy, i, j, k, ng);
-  if (g.dims >= 3 && k >= L.lo[2]) flux32_face<2, ORDER, RS>(g, w, bf, fz, i, j, k, ng);
+  if (dims >= 2 && j >= L.lo[1]) flux32_face<1, ORDER, RS>(g, w, bf, fy, i, j, k, ng);
+  if (dims >= 3 && k >= L.lo[2]) flux32_face<2, ORDER, RS>(g, w, bf, fz, i, j, k, ng);
 }
 
 

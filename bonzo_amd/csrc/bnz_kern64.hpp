@@ -63,16 +63,18 @@ __device__ __forceinline__ void flux_face(const Geo &g, const double *__restrict
 // Every face flux of one stage whose lower cell is (i, j, k): the x, y and
 // z faces of godunov_fluxes' three sweeps (godunov.pyx:20-210), each only
 // where that sweep's range includes it.  Over flux3_box(L) this covers
-// exactly the faces of the three per-direction launches.
-template <int ORDER, int RS>
+// exactly the faces of the three per-direction launches.  DIMS > 0 fixes
+// the dimensionality at compile time; 0 reads it from g.
+template <int ORDER, int RS, int DIMS = 0>
 __device__ __forceinline__ void flux_cell(const Geo &g, const double *__restrict__ w,
                                           const double *__restrict__ bf,
                                           double *__restrict__ fx, double *__restrict__ fy,
                                           double *__restrict__ fz, const Lims &L, int i, int j,
                                           int k) {
+  const int dims = DIMS > 0 ? DIMS : g.dims;
   if (i >= L.lo[0]) flux_face<0, ORDER, RS>(g, w, bf, fx, i, j, k);
-  if (g.dims >= 2 && j >= L.lo[1]) flux_face<1, ORDER, RS>(g, w, bf, fy, i, j, k);
-  if (g.dims >= 3 && k >= L.lo[2]) flux_face<2, ORDER, RS>(g, w, bf, fz, i, j, k);
+  if (dims >= 2 && j >= L.lo[1]) flux_face<1, ORDER, RS>(g, w, bf, fy, i, j, k);
+  if (dims >= 3 && k >= L.lo[2]) flux_face<2, ORDER, RS>(g, w, bf, fz, i, j, k);
 }
 
 __device__ __forceinline__ double ec_comp(const double *w, long sv, long c, int comp) {

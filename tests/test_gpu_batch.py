@@ -133,6 +133,46 @@ def test_batch_max_steps_and_resume():
     b.close()
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32c"])
+def test_batch_finished_boxes_across_pieces(precision):
+    """Boxes that reach t_max long before the others sit out many steps (the
+    face buffers swap without them); advancing in pieces of odd and even
+    length, graph and eager, and reading a finished box in between, must
+    give each box its uninterrupted state."""
+    say(f"case: 2D + 1D boxes finishing at different steps, pieces, {precision}")
+    from bonzo_amd._lib import Batch
+    rng = np.random.Generator(np.random.PCG64(1234))
+    cases = [(P.riemann_tube(64, rng), (1,) * 6, 0.01),
+             (P.orszag_tang(32), (0,) * 6, 0.005),
+             (P.riemann_tube(300, rng), (1,) * 6, 0.06),
+             (P.orszag_tang(48), (0,) * 6, 0.03)]
+    ref, tref, sref, _ = run_batch(cases, use_graph=False, precision=precision)
+    say(f"  reference steps {list(sref)}")
+    b = Batch([box_desc(c[0][0], c[1], c[2]) for c in cases], precision=precision)
+    for q, ((s, prim, bfld), _, _) in enumerate(cases):
+        b.upload(q, prim, bfld)
+    b.init_state()
+    pieces = [(3, False), (5, True), (2, False), (7, False), (4, True)]
+    checked = set()
+    for it in range(400):
+        n, g = pieces[it % len(pieces)]
+        b.advance(max_steps=n, use_graph=g)
+        t, steps = b.times()
+        for q in range(len(cases)):
+            if q not in checked and t[q] == tref[q]:
+                p_, b_ = b.download(q)
+                assert np.array_equal(p_, ref[q][0]) and np.array_equal(b_, ref[q][1]), q
+                checked.add(q)
+        if all(t[q] == tref[q] for q in range(len(cases))):
+            break
+    assert np.array_equal(steps, sref)
+    for q in range(len(cases)):
+        p_, b_ = b.download(q)
+        assert np.array_equal(p_, ref[q][0]), q
+        assert np.array_equal(b_, ref[q][1]), q
+    b.close()
+
+
 def test_tube_sweep_conserves_and_reaches_tmax():
     """A slice of the 1D sweep workload (64 tubes of 64..4096 cells) runs to
     t=0.1 with the graph; every box ends exactly at t_max with finite state."""
