@@ -103,6 +103,72 @@ __global__ __launch_bounds__(BLK) void kb_update(const BoxDev *__restrict__ boxe
   }
 }
 
+// Fluxes of a run of 1D faces held in LDS, s[n * BLK + slot]: slot t + off
+// is the face whose lower cell is the updating thread's cell + off.
+struct FluxLds1D {
+  const double *s;
+  int t;
+  __device__ __forceinline__ double x(int n, long off) const { return s[n * BLK + t + off]; }
+  __device__ __forceinline__ double y(int, long) const { return 0.; }
+  __device__ __forceinline__ double z(int, long) const { return 0.; }
+};
+struct EmfLds1D {  // Emf1D on the LDS fluxes
+  const double *s;
+  int t;
+  __device__ __forceinline__ double operator()(int comp, long off) const {
+    return comp == 2 ? -s[BY * BLK + t + off] : comp == 1 ? s[BZ * BLK + t + off] : 0.;
+  }
+};
+
+// One stage of the 1D boxes in one pass: thread t computes the flux of face
+// i0 + t into LDS, then updates cell i0 + t from faces i0 + t and i0 + t + 1
+// (t < BLK - 1), so the fluxes never go through HBM.  The face and cell
+// arithmetic is that of kb_flux (flux_face_to) and kb_update
+// (update_cell_e), on the same faces and cells.  The corrector reads the
+// normal field of its faces from the step's faces rather than the
+// predictor's: in 1D the predictor copies x faces unchanged, so the values
+// are the same, and the corrector's face writes cannot race the reads.
+template <bool PRED, int ORDER, int RS>
+__global__ __launch_bounds__(BLK) void kb_stage1(const BoxDev *__restrict__ boxes, WList wl,
+                                                 BoxState st, Pools p, int fms_ref) {
+  __shared__ double sf[NMODE * BLK];
+  const int tid = threadIdx.x;
+  for (int wb = wl.lo + blockIdx.x; wb < wl.hi; wb += gridDim.x) {
+    const int b = wl.bmap[wb];
+    if (!st.run[b]) continue;
+    const BoxDev &d = boxes[b];
+    const Geo &g = d.g;
+    const Lims &L = d.L[PRED ? 0 : 1];
+    const long o = d.off, sv = g.sv;
+    const int i = (wb - d.blk0[wl.kt]) * (BLK - 1) + tid;
+    if (i >= L.lo[0] && i <= L.hi[0] + 1) {  // the stage's faces (flux3_box)
+      bool got = false;
+      flux_face_to<0, ORDER, RS>(g, PRED ? p.w + o : p.ws + o, p.b + o, i, 0, 0,
+                                 [&](const double *f, long) {
+#pragma unroll
+                                   for (int n = 0; n < NMODE; ++n)
+                                     if (n != BX) sf[n * BLK + tid] = f[n];
+                                   got = true;
+                                 });
+      if (!got)  // no region matched: the flux row keeps what it held (flux_face)
+        for (int n = 0; n < NMODE; ++n)
+          if (n != BX) sf[n * BLK + tid] = p.fx[n * sv + o + i];
+    }
+    __syncthreads();
+    double cds = 0.;
+    if (tid < BLK - 1 && i < g.nt[0]) {
+      const double r = update_cell_e<1, PRED, !PRED>(
+          g, GeoArr{g}, p.w + o, FluxLds1D{sf, tid}, EmfLds1D{sf, tid}, p.b + o, p.bs + o,
+          PRED ? p.ws + o : p.w + o, L, st.dt + b, d.act, fms_ref, i, 0, 0);
+      if (!PRED) cds = r;
+    }
+    if (!PRED)
+      block_max_atomic(cds, st.cmax + b);  // ends with a barrier: sf is free again
+    else
+      __syncthreads();
+  }
+}
+
 // CFL maximum of the current prim over the active cells (after init).
 __global__ __launch_bounds__(BLK) void kb_cfl(const BoxDev *__restrict__ boxes, WList wl,
                                               BoxState st, const double *__restrict__ w,
@@ -342,16 +408,14 @@ Box mkbox(int i0, int ni, int j0, int nj, int k0, int nk) {
 
 
 
+// fluxes of the 2D and 3D boxes (1D boxes: kb_stage1)
 template <int ORDER, int RS>
 void b_fluxes(bnz_batch *c, const double *w, const double *bf, int kt) {
-  for (int d = 1; d <= 3; ++d) {
+  for (int d = 2; d <= 3; ++d) {
     const WList wl = wlist_dims(c, kt, d);
     if (wl.hi == wl.lo) continue;
     const dim3 grid(grid_for(wl.hi - wl.lo));
-    if (d == 1)
-      hipLaunchKernelGGL((kb_flux<ORDER, RS, 1>), grid, dim3(BLK), 0, c->stream, c->boxes, wl,
-                         c->S, w, bf, c->P.fx, c->P.fy, c->P.fz);
-    else if (d == 2)
+    if (d == 2)
       hipLaunchKernelGGL((kb_flux<ORDER, RS, 2>), grid, dim3(BLK), 0, c->stream, c->boxes, wl,
                          c->S, w, bf, c->P.fx, c->P.fy, c->P.fz);
     else
@@ -375,22 +439,38 @@ void b_emf(bnz_batch *c, const double *w, int kt) {
   }
 }
 
+// stage updates of the 2D and 3D boxes, then the fused stage of the 1D boxes
+template <bool PRED, int ORDER, int RS>
+void b_stage1(bnz_batch *c) {
+  const WList wl = wlist(c, KS1);
+  if (wl.hi == wl.lo) return;
+  hipLaunchKernelGGL((kb_stage1<PRED, ORDER, RS>), dim3(grid_for(wl.hi - wl.lo)), dim3(BLK), 0,
+                     c->stream, c->boxes, wl, c->S, c->P, c->p.dt_fms_ref);
+}
+
 template <bool PRED>
 void b_update(bnz_batch *c) {
   const int fr = c->p.dt_fms_ref;
-  for (int d = 1; d <= 3; ++d) {
+  for (int d = 2; d <= 3; ++d) {
     const WList wl = wlist_dims(c, KUPD, d);
     if (wl.hi == wl.lo) continue;
     const dim3 grid(grid_for(wl.hi - wl.lo));
-    if (d == 1)
-      hipLaunchKernelGGL((kb_update<PRED, 1>), grid, dim3(BLK), 0, c->stream, c->boxes, wl, c->S,
-                         c->P, fr);
-    else if (d == 2)
+    if (d == 2)
       hipLaunchKernelGGL((kb_update<PRED, 2>), grid, dim3(BLK), 0, c->stream, c->boxes, wl, c->S,
                          c->P, fr);
     else
       hipLaunchKernelGGL((kb_update<PRED, 3>), grid, dim3(BLK), 0, c->stream, c->boxes, wl, c->S,
                          c->P, fr);
+  }
+  const int order = PRED ? 0 : c->order, rs = c->p.rsolver;
+  if (order == 0) {
+    if (rs == BNZ_RS_HLLD) b_stage1<PRED, 0, BNZ_RS_HLLD>(c);
+    else if (rs == BNZ_RS_HLLT) b_stage1<PRED, 0, BNZ_RS_HLLT>(c);
+    else b_stage1<PRED, 0, BNZ_RS_HLL>(c);
+  } else {
+    if (rs == BNZ_RS_HLLD) b_stage1<PRED, 1, BNZ_RS_HLLD>(c);
+    else if (rs == BNZ_RS_HLLT) b_stage1<PRED, 1, BNZ_RS_HLLT>(c);
+    else b_stage1<PRED, 1, BNZ_RS_HLL>(c);
   }
 }
 
@@ -548,6 +628,7 @@ int bnz_batch_create(const bnz_params *p, const bnz_box *boxes, int nbox, bnz_ba
                       dims >= 3 ? kl : 0, dims >= 3 ? ku + 2 - kl : 1);
     }
     d.items[KUPD] = mkbox(0, nt[0], 0, nt[1], 0, nt[2]);
+    d.items[KS1] = dims == 1 ? mkbox(0, nt[0], 0, 1, 0, 1) : mkbox(0, 0, 0, 0, 0, 0);
     d.act = mkbox(a1[0], s.nx, a1[1], dims >= 2 ? s.ny : 1, a1[2], dims >= 3 ? s.nz : 1);
     d.items[KACT] = d.act;
     for (int a = 0; a < 3; ++a) {  // reflecting wall planes (FP32C EMFs)
@@ -574,7 +655,7 @@ int bnz_batch_create(const bnz_params *p, const bnz_box *boxes, int nbox, bnz_ba
         BoxDev &d = c->hbox[b];
         if (d.g.dims != dims) continue;
         d.blk0[kt] = (int)nb;
-        const long blocks = (d.items[kt].total() + BLK - 1) / BLK;
+        const long blocks = (d.items[kt].total() + blk_items(kt) - 1) / blk_items(kt);
         for (long q = 0; q < blocks; ++q) hmap.push_back(b);
         nb += blocks;
       }

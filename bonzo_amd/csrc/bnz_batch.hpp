@@ -12,8 +12,12 @@ using namespace bnz;
 
 // work lists: fluxes (all directions, one item per cell of the union of the
 // face ranges) and EMFs of the predictor and of the corrector, all cells
-// (stage updates), active cells (init and the first CFL maximum)
-enum { KF0 = 0, KEM0, KF1, KEM1, KUPD, KACT, NK };
+// (stage updates), active cells (init and the first CFL maximum), and the
+// cells of the 1D boxes for the fused 1D stage, BLK - 1 per work block
+enum { KF0 = 0, KEM0, KF1, KEM1, KUPD, KACT, KS1, NK };
+
+// cells per work block of list kt
+__host__ __device__ constexpr int blk_items(int kt) { return kt == KS1 ? BLK - 1 : BLK; }
 
 struct BoxDev {
   Geo g;         // strides and geometry of the box; g.sv is the pool stride

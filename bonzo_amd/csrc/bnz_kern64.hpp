@@ -12,11 +12,13 @@ namespace bnz {
 // Godunov fluxes of one direction (godunov.pyx:20-210 with the double-buffered
 // left state of D1).  Face index c is the lower face of cell c, between cells
 // c-1 and c.  The flux is stored in global variable order, flux[n][c], except
-// the normal-field slot, which no consumer reads.
-template <int DIR, int ORDER, int RS>
-__device__ __forceinline__ void flux_face(const Geo &g, const double *__restrict__ w,
-                                          const double *__restrict__ bf,
-                                          double *__restrict__ fl, int i, int j, int k) {
+// the normal-field slot, which no consumer reads.  flux_face_to hands the
+// flux (local slots) of a face with a matched region to sink(f, c);
+// flux_face stores it.
+template <int DIR, int ORDER, int RS, class SINK>
+__device__ __forceinline__ void flux_face_to(const Geo &g, const double *__restrict__ w,
+                                             const double *__restrict__ bf, int i, int j, int k,
+                                             const SINK &sink) {
   const long c = ((long)k * g.nt[1] + j) * g.nt[0] + i;
   const long sd = (DIR == 0) ? 1 : (DIR == 1) ? g.sy : g.sz;
   const double bn = bf[(long)DIR * g.sv + c];
@@ -50,14 +52,24 @@ __device__ __forceinline__ void flux_face(const Geo &g, const double *__restrict
   else
     reg = flux_hll(f, wl, wr, g.gam, g.gamm1i);
   if (reg == 0) return;  // reference leaves the flux row untouched
-  // streaming stores: the fluxes are read back by the next kernels, long
-  // after any L2 could hold them, and bypassing it keeps the stencil rows
-  // of the prim there (-4 % on the flux launches, -3 % on the EMFs)
+  sink(f, c);
+}
+
+template <int DIR, int ORDER, int RS>
+__device__ __forceinline__ void flux_face(const Geo &g, const double *__restrict__ w,
+                                          const double *__restrict__ bf,
+                                          double *__restrict__ fl, int i, int j, int k) {
+  const long sv = g.sv;
+  flux_face_to<DIR, ORDER, RS>(g, w, bf, i, j, k, [fl, sv](const double *f, long c) {
+    // streaming stores: the fluxes are read back by the next kernels, long
+    // after any L2 could hold them, and bypassing it keeps the stencil rows
+    // of the prim there (-4 % on the flux launches, -3 % on the EMFs)
 #pragma unroll
-  for (int n = 0; n < NMODE; ++n) {
-    if (n == BX) continue;
-    __builtin_nontemporal_store(f[n], &fl[(long)perm<DIR>(n) * g.sv + c]);
-  }
+    for (int n = 0; n < NMODE; ++n) {
+      if (n == BX) continue;
+      __builtin_nontemporal_store(f[n], &fl[(long)perm<DIR>(n) * sv + c]);
+    }
+  });
 }
 
 // Every face flux of one stage whose lower cell is (i, j, k): the x, y and
@@ -170,11 +182,19 @@ struct Emf1D {
   }
 };
 
-template <int DIMS, bool PRED, bool DTRED, class EMF, class GM>
+// Face fluxes as the update reads them: variable n of the face whose lower
+// cell is c + off.  FluxGlobal reads the arrays of the flux kernels.
+struct FluxGlobal {
+  const double *fx, *fy, *fz;
+  long sv, c;
+  __device__ __forceinline__ double x(int n, long off) const { return fx[n * sv + c + off]; }
+  __device__ __forceinline__ double y(int n, long off) const { return fy[n * sv + c + off]; }
+  __device__ __forceinline__ double z(int n, long off) const { return fz[n * sv + c + off]; }
+};
+
+template <int DIMS, bool PRED, bool DTRED, class EMF, class GM, class FL>
 __device__ __forceinline__ double update_cell_e(const Geo &g, const GM &gm, const double *w0,
-                                                const double *__restrict__ fx,
-                                                const double *__restrict__ fy,
-                                                const double *__restrict__ fz, const EMF &em,
+                                                const FL &fl, const EMF &em,
                                                 const double *__restrict__ b0,
                                                 double *__restrict__ b1, double *w1,
                                                 const Lims &L, const double *__restrict__ dts,
@@ -209,15 +229,15 @@ __device__ __forceinline__ double update_cell_e(const Geo &g, const GM &gm, cons
       fzu[NMODE - 3];
 #pragma unroll
   for (int n = 0; n < NMODE - 3; ++n) {
-    fxl[n] = fx[n * sv + c];
-    fxu[n] = fx[n * sv + c + ox];
+    fxl[n] = fl.x(n, 0);
+    fxu[n] = fl.x(n, ox);
     if (DIMS >= 2) {
-      fyl[n] = fy[n * sv + c];
-      fyu[n] = fy[n * sv + c + oy];
+      fyl[n] = fl.y(n, 0);
+      fyu[n] = fl.y(n, oy);
     }
     if (DIMS >= 3) {
-      fzl[n] = fz[n * sv + c];
-      fzu[n] = fz[n * sv + c + oz];
+      fzl[n] = fl.z(n, 0);
+      fzu[n] = fl.z(n, oz);
     }
   }
   const double dlx = gm.dlf(0, i), dly = gm.dlf(1, j), dlz = gm.dlf(2, k);
@@ -322,11 +342,12 @@ __device__ __forceinline__ double update_cell_d(const Geo &g, const double *w0,
   const long c = ((long)k * g.nt[1] + j) * g.nt[0] + i;
   using GM = typename std::conditional<UG, GeoUni, GeoArr>::type;
   const GM gm{g};
+  const FluxGlobal fl{fx, fy, fz, g.sv, c};
   if (DIMS == 1)
-    return update_cell_e<1, PRED, DTRED>(g, gm, w0, fx, fy, fz, Emf1D{fx, g.sv, c}, b0, b1, w1,
-                                         L, dts, act, fms_ref, i, j, k);
-  return update_cell_e<DIMS, PRED, DTRED>(g, gm, w0, fx, fy, fz, EmfGlobal{ee, g.sv, c}, b0, b1,
-                                          w1, L, dts, act, fms_ref, i, j, k);
+    return update_cell_e<1, PRED, DTRED>(g, gm, w0, fl, Emf1D{fx, g.sv, c}, b0, b1, w1, L, dts,
+                                         act, fms_ref, i, j, k);
+  return update_cell_e<DIMS, PRED, DTRED>(g, gm, w0, fl, EmfGlobal{ee, g.sv, c}, b0, b1, w1, L,
+                                          dts, act, fms_ref, i, j, k);
 }
 
 // Same with the dimensionality read from g (batched boxes of mixed dims).
