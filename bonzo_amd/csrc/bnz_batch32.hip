@@ -101,6 +101,86 @@ __global__ __launch_bounds__(BLK) void kb32_update(const BoxDev *__restrict__ bo
   }
 }
 
+// Fluxes of a run of 1D faces in LDS (as the FP64 FluxLds1D) and the 1D
+// EMFs formed from them with the wall planes zeroed (as Emf32x1D).
+struct FluxLds32 {
+  const float *s;
+  int t;
+  __device__ __forceinline__ float x(int n, long off) const { return s[n * BLK + t + off]; }
+  __device__ __forceinline__ float y(int, long) const { return 0.f; }
+  __device__ __forceinline__ float z(int, long) const { return 0.f; }
+};
+struct EmfLds32x1D {
+  const float *s;
+  int t, i, wlo, whi;
+  __device__ __forceinline__ float operator()(int comp, long off) const {
+    const int e = i + (int)off;
+    if (comp == 0 || e == wlo || e == whi) return 0.f;
+    return comp == 2 ? -s[BY * BLK + t + off] : s[BZ * BLK + t + off];
+  }
+};
+
+// One FP32C stage of the 1D boxes in one pass, as the FP64 kb_stage1: the
+// fluxes of BLK faces go to LDS, the BLK - 1 cells between them are updated
+// from there with the kb32_flux / kb32_update arithmetic.  The predictor
+// writes the FP32 predictor faces and prim (bs32, ws32), which the
+// corrector's fluxes read; the corrector writes FP64 faces to the other
+// face buffer and its own cell's state, so no read of the pass races a
+// write.
+template <bool PRED, int ORDER, int RS>
+__global__ __launch_bounds__(BLK) void kb32_stage1(const BoxDev *__restrict__ boxes, WList wl,
+                                                   BoxState st, Pools P, Pools32 Q, int fms_ref,
+                                                   int *nguard) {
+  __shared__ float sf[NMODE * BLK];
+  const int tid = threadIdx.x;
+  int ng = 0;
+  for (int wb = wl.lo + blockIdx.x; wb < wl.hi; wb += gridDim.x) {
+    const int b = wl.bmap[wb];
+    if (!st.run[b]) continue;
+    const BoxDev &d = boxes[b];
+    const Geo &g = d.g;
+    const Lims &L = d.L[PRED ? 0 : 1];
+    const long o = d.off, sv = g.sv;
+    const int i = (wb - d.blk0[wl.kt]) * (BLK - 1) + tid;
+    if (i >= L.lo[0] && i <= L.hi[0] + 1) {  // the stage's faces (flux3_box)
+      bool got = false;
+      const auto sink = [&](const float *f, long) {
+#pragma unroll
+        for (int n = 0; n < NMODE; ++n)
+          if (n != BX) sf[n * BLK + tid] = f[n];
+        got = true;
+      };
+      if (PRED)
+        flux32_face_to<0, ORDER, RS>(g, Q.w32 + o, P.b + o, i, 0, 0, ng, sink);
+      else
+        flux32_face_to<0, ORDER, RS>(g, Q.ws32 + o, (const float *)Q.bs32 + o, i, 0, 0, ng, sink);
+      if (!got)  // no region matched: the flux row keeps what it held
+        for (int n = 0; n < NMODE; ++n)
+          if (n != BX) sf[n * BLK + tid] = Q.fx[n * sv + o + i];
+    }
+    __syncthreads();
+    double cds = 0.;
+    if (tid < BLK - 1 && i < g.nt[0]) {
+      const FluxLds32 fl{sf, tid};
+      const EmfLds32x1D em{sf, tid, i, d.walls[0], d.walls[1]};
+      if (PRED) {
+        const Pred32 a{Q.w32 + o, Q.uh + o, P.b + o, Q.fx + o, Q.fy + o,
+                       Q.fz + o,  Q.ee + o, Q.bs32 + o, Q.ws32 + o};
+        update32_pred_e<1, GeoArr>(g, a, L, st.dt + b, fl, em, i, 0, 0);
+      } else {
+        const Corr32 a{Q.uh + o, Q.ul + o, P.b + o, Q.fx + o, Q.fy + o,
+                       Q.fz + o, Q.ee + o, P.bs + o, Q.w32 + o};
+        cds = update32_corr_e<1, GeoArr>(g, a, L, st.dt + b, d.act, fms_ref, fl, em, i, 0, 0);
+      }
+    }
+    if (!PRED)
+      block_max_atomic(cds, st.cmax + b);  // ends with a barrier: sf is free again
+    else
+      __syncthreads();
+  }
+  if (ng) atomicAdd(nguard, ng);
+}
+
 // FP64 prim -> FP32C state over every cell of every box (init).
 __global__ __launch_bounds__(BLK) void kb32_from64(const BoxDev *__restrict__ boxes, WList wl,
                                                    Pools P, Pools32 Q) {
@@ -161,16 +241,14 @@ __global__ __launch_bounds__(BLK) void kb32_ghost_ops(Ghost32 p, const GhostOp *
     ghost32_op(p, ops[t]);
 }
 
+// fluxes of the 2D and 3D boxes (1D boxes: kb32_stage1)
 template <int ORDER, int RS, typename BF>
 void flux_launch(bnz_batch *c, const float *w, const BF *bf, int kt) {
-  for (int d = 1; d <= 3; ++d) {
+  for (int d = 2; d <= 3; ++d) {
     const WList wl = wlist_dims(c, kt, d);
     if (wl.hi == wl.lo) continue;
     const dim3 grid(grid_for(wl.hi - wl.lo));
-    if (d == 1)
-      hipLaunchKernelGGL((kb32_flux<ORDER, RS, BF, 1>), grid, dim3(BLK), 0, c->stream, c->boxes,
-                         wl, c->S, w, bf, c->Q.fx, c->Q.fy, c->Q.fz, c->nguard);
-    else if (d == 2)
+    if (d == 2)
       hipLaunchKernelGGL((kb32_flux<ORDER, RS, BF, 2>), grid, dim3(BLK), 0, c->stream, c->boxes,
                          wl, c->S, w, bf, c->Q.fx, c->Q.fy, c->Q.fz, c->nguard);
     else
@@ -206,22 +284,36 @@ void emfs(bnz_batch *c, const float *w, int kt) {
   }
 }
 
+template <bool PRED, int ORDER, int RS>
+void stage1(bnz_batch *c) {
+  const WList wl = wlist(c, KS1);
+  if (wl.hi == wl.lo) return;
+  hipLaunchKernelGGL((kb32_stage1<PRED, ORDER, RS>), dim3(grid_for(wl.hi - wl.lo)), dim3(BLK), 0,
+                     c->stream, c->boxes, wl, c->S, c->P, c->Q, c->p.dt_fms_ref, c->nguard);
+}
+
+// stage updates of the 2D and 3D boxes, then the fused stage of the 1D boxes
 template <bool PRED>
 void updates(bnz_batch *c) {
   const int fr = c->p.dt_fms_ref;
-  for (int d = 1; d <= 3; ++d) {
+  for (int d = 2; d <= 3; ++d) {
     const WList wl = wlist_dims(c, KUPD, d);
     if (wl.hi == wl.lo) continue;
     const dim3 grid(grid_for(wl.hi - wl.lo));
-    if (d == 1)
-      hipLaunchKernelGGL((kb32_update<PRED, 1>), grid, dim3(BLK), 0, c->stream, c->boxes, wl,
-                         c->S, c->P, c->Q, fr);
-    else if (d == 2)
+    if (d == 2)
       hipLaunchKernelGGL((kb32_update<PRED, 2>), grid, dim3(BLK), 0, c->stream, c->boxes, wl,
                          c->S, c->P, c->Q, fr);
     else
       hipLaunchKernelGGL((kb32_update<PRED, 3>), grid, dim3(BLK), 0, c->stream, c->boxes, wl,
                          c->S, c->P, c->Q, fr);
+  }
+  const int order = PRED ? 0 : c->order;
+  if (order == 0) {
+    if (c->p.rsolver == BNZ_RS_HLLD) stage1<PRED, 0, BNZ_RS_HLLD>(c);
+    else stage1<PRED, 0, BNZ_RS_HLL>(c);
+  } else {
+    if (c->p.rsolver == BNZ_RS_HLLD) stage1<PRED, 1, BNZ_RS_HLLD>(c);
+    else stage1<PRED, 1, BNZ_RS_HLL>(c);
   }
 }
 

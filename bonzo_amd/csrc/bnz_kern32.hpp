@@ -17,10 +17,12 @@ __device__ __forceinline__ double join(float hi, float lo) { return (double)hi +
 
 // Godunov flux of one face, FP32 (same stencil as the FP64 flux_face); ng
 // counts guarded HLLD star denominators.
-template <int DIR, int ORDER, int RS, typename BF>
-__device__ __forceinline__ void flux32_face(const Geo &g, const float *__restrict__ w,
-                                            const BF *__restrict__ bf, float *__restrict__ fl,
-                                            int i, int j, int k, int &ng) {
+// flux32_face_to hands the flux (local slots) of a face with a matched region
+// to sink(f, c); flux32_face stores it.
+template <int DIR, int ORDER, int RS, typename BF, class SINK>
+__device__ __forceinline__ void flux32_face_to(const Geo &g, const float *__restrict__ w,
+                                               const BF *__restrict__ bf, int i, int j, int k,
+                                               int &ng, const SINK &sink) {
   const long c = ((long)k * g.nt[1] + j) * g.nt[0] + i;
   const long sd = (DIR == 0) ? 1 : (DIR == 1) ? g.sy : g.sz;
   const float bn = (float)bf[(long)DIR * g.sv + c];
@@ -45,11 +47,21 @@ __device__ __forceinline__ void flux32_face(const Geo &g, const float *__restric
   const int reg = (RS == BNZ_RS_HLLD) ? flux_hlld(f, wl, wr, gam, gm1i, &ng)
                                       : flux_hll(f, wl, wr, gam, gm1i);
   if (reg == 0) return;
+  sink(f, c);
+}
+
+template <int DIR, int ORDER, int RS, typename BF>
+__device__ __forceinline__ void flux32_face(const Geo &g, const float *__restrict__ w,
+                                            const BF *__restrict__ bf, float *__restrict__ fl,
+                                            int i, int j, int k, int &ng) {
+  const long sv = g.sv;
+  flux32_face_to<DIR, ORDER, RS>(g, w, bf, i, j, k, ng, [fl, sv](const float *f, long c) {
 #pragma unroll
-  for (int n = 0; n < NMODE; ++n) {
-    if (n == BX) continue;
-    fl[(long)perm<DIR>(n) * g.sv + c] = f[n];
-  }
+    for (int n = 0; n < NMODE; ++n) {
+      if (n == BX) continue;
+      fl[(long)perm<DIR>(n) * sv + c] = f[n];
+    }
+  });
 }
 
 
@@ -258,6 +270,16 @@ struct EmfSel32<1> {
   }
 };
 
+// FP32 face fluxes as the updates read them: variable n of the face whose
+// lower cell is c + off (the arrays of the flux kernels).
+struct FluxArr32 {
+  const float *fx, *fy, *fz;
+  long sv, c;
+  __device__ __forceinline__ float x(int n, long off) const { return fx[n * sv + c + off]; }
+  __device__ __forceinline__ float y(int n, long off) const { return fy[n * sv + c + off]; }
+  __device__ __forceinline__ float z(int n, long off) const { return fz[n * sv + c + off]; }
+};
+
 struct Pred32 {
   const float *w32, *uh;
   const double *b;
@@ -269,10 +291,11 @@ struct Pred32 {
 // Predictor (dt/2): FP32 update of the hi words, FP64 pressure recovery.
 // All operands are loaded up front; only the stores (and the copy of w into
 // ws outside the limits) depend on the roles.
-template <int DIMS, class GM = GeoArr>
-__device__ __forceinline__ void update32_pred(const Geo &g, const Pred32 &a, const Lims &L,
-                                              const double *__restrict__ dts, const Walls &wl,
-                                              int i, int j, int k) {
+// Fluxes come from fl (x, y, z accessors), EMFs from em.
+template <int DIMS, class GM, class FL, class EM>
+__device__ __forceinline__ void update32_pred_e(const Geo &g, const Pred32 &a, const Lims &L,
+                                                const double *__restrict__ dts, const FL &fl,
+                                                const EM &em, int i, int j, int k) {
   const long c = ((long)k * g.nt[1] + j) * g.nt[0] + i, sv = g.sv;
   const double dt = 0.5 * dts[0];
   const Roles<DIMS> ro(g, L, i, j, k);
@@ -281,21 +304,20 @@ __device__ __forceinline__ void update32_pred(const Geo &g, const Pred32 &a, con
 #pragma unroll
   for (int n = 0; n < NMODE - 3; ++n) {
     uh[n] = a.uh[n * sv + c];
-    fxl[n] = a.fx[n * sv + c];
-    fxu[n] = a.fx[n * sv + c + ro.ox];
+    fxl[n] = fl.x(n, 0);
+    fxu[n] = fl.x(n, ro.ox);
     if (DIMS >= 2) {
-      fyl[n] = a.fy[n * sv + c];
-      fyu[n] = a.fy[n * sv + c + ro.oy];
+      fyl[n] = fl.y(n, 0);
+      fyu[n] = fl.y(n, ro.oy);
     }
     if (DIMS >= 3) {
-      fzl[n] = a.fz[n * sv + c];
-      fzu[n] = a.fz[n * sv + c + ro.oz];
+      fzl[n] = fl.z(n, 0);
+      fzu[n] = fl.z(n, ro.oz);
     }
   }
   float lo[3], up[3];
   const GM gm{g};
-  faces_d<DIMS, float>(gm, Faces32{a.b, sv}, EmfSel32<DIMS>::get(a.ee, a.fx, sv, c, i, wl), dt, i,
-                       j, k, c, ro.ox, ro.oy, ro.oz, lo, up);
+  faces_d<DIMS, float>(gm, Faces32{a.b, sv}, em, dt, i, j, k, c, ro.ox, ro.oy, ro.oz, lo, up);
   const float dlx = (float)gm.dlf(0, i), dly = (float)gm.dlf(1, j), dlz = (float)gm.dlf(2, k);
   const float dtdv = (float)(dt / (gm.dlf(0, i) * gm.dlf(1, j) * gm.dlf(2, k)));
   const float ax = dly * dlz, ay = dlx * dlz, az = dlx * dly;
@@ -329,6 +351,15 @@ __device__ __forceinline__ void update32_pred(const Geo &g, const Pred32 &a, con
   }
 }
 
+template <int DIMS, class GM = GeoArr>
+__device__ __forceinline__ void update32_pred(const Geo &g, const Pred32 &a, const Lims &L,
+                                              const double *__restrict__ dts, const Walls &wl,
+                                              int i, int j, int k) {
+  const long c = ((long)k * g.nt[1] + j) * g.nt[0] + i, sv = g.sv;
+  update32_pred_e<DIMS, GM>(g, a, L, dts, FluxArr32{a.fx, a.fy, a.fz, sv, c},
+                            EmfSel32<DIMS>::get(a.ee, a.fx, sv, c, i, wl), i, j, k);
+}
+
 
 struct Corr32 {
   float *uh, *ul;
@@ -342,11 +373,11 @@ struct Corr32 {
 // Corrector (dt): FP64 update of the hi + lo state, FP64 pressure recovery,
 // returns the CFL speed of the new state (0 outside the active cells).  Operands loaded up front as in the
 // predictor.
-template <int DIMS, class GM = GeoArr>
-__device__ __forceinline__ double update32_corr(const Geo &g, const Corr32 &a, const Lims &L,
-                                                const double *__restrict__ dts, const Box &act,
-                                                int fms_ref, const Walls &wl, int i, int j,
-                                                int k) {
+template <int DIMS, class GM, class FL, class EM>
+__device__ __forceinline__ double update32_corr_e(const Geo &g, const Corr32 &a, const Lims &L,
+                                                  const double *__restrict__ dts, const Box &act,
+                                                  int fms_ref, const FL &fl, const EM &em, int i,
+                                                  int j, int k) {
   const long c = ((long)k * g.nt[1] + j) * g.nt[0] + i, sv = g.sv;
   double cdsmax = 0.;
   const double dt = dts[0];
@@ -357,21 +388,20 @@ __device__ __forceinline__ double update32_corr(const Geo &g, const Corr32 &a, c
   for (int n = 0; n < NMODE - 3; ++n) {
     uh[n] = a.uh[n * sv + c];
     ul[n] = a.ul[n * sv + c];
-    fxl[n] = a.fx[n * sv + c];
-    fxu[n] = a.fx[n * sv + c + ro.ox];
+    fxl[n] = fl.x(n, 0);
+    fxu[n] = fl.x(n, ro.ox);
     if (DIMS >= 2) {
-      fyl[n] = a.fy[n * sv + c];
-      fyu[n] = a.fy[n * sv + c + ro.oy];
+      fyl[n] = fl.y(n, 0);
+      fyu[n] = fl.y(n, ro.oy);
     }
     if (DIMS >= 3) {
-      fzl[n] = a.fz[n * sv + c];
-      fzu[n] = a.fz[n * sv + c + ro.oz];
+      fzl[n] = fl.z(n, 0);
+      fzu[n] = fl.z(n, ro.oz);
     }
   }
   double lo[3], up[3];
   const GM gm{g};
-  faces_d<DIMS, double>(gm, Faces64{a.b, sv}, EmfSel32<DIMS>::get(a.ee, a.fx, sv, c, i, wl), dt,
-                        i, j, k, c, ro.ox, ro.oy, ro.oz, lo, up);
+  faces_d<DIMS, double>(gm, Faces64{a.b, sv}, em, dt, i, j, k, c, ro.ox, ro.oy, ro.oz, lo, up);
   const double dlx = gm.dlf(0, i), dly = gm.dlf(1, j), dlz = gm.dlf(2, k);
   const double dtdv = dt / (dlx * dly * dlz);
   const double ax = dly * dlz, ay = dlx * dlz, az = dlx * dly;
@@ -411,6 +441,17 @@ __device__ __forceinline__ double update32_corr(const Geo &g, const Corr32 &a, c
       cdsmax = cds_cell_g(gm, g, wv, fms_ref, i, j, k);
   }
   return cdsmax;
+}
+
+template <int DIMS, class GM = GeoArr>
+__device__ __forceinline__ double update32_corr(const Geo &g, const Corr32 &a, const Lims &L,
+                                                const double *__restrict__ dts, const Box &act,
+                                                int fms_ref, const Walls &wl, int i, int j,
+                                                int k) {
+  const long c = ((long)k * g.nt[1] + j) * g.nt[0] + i, sv = g.sv;
+  return update32_corr_e<DIMS, GM>(g, a, L, dts, act, fms_ref,
+                                   FluxArr32{a.fx, a.fy, a.fz, sv, c},
+                                   EmfSel32<DIMS>::get(a.ee, a.fx, sv, c, i, wl), i, j, k);
 }
 
 
