@@ -212,11 +212,27 @@ def test_batch_fp32c_bitwise_vs_single_fp32c(use_graph):
     rng = np.random.Generator(np.random.PCG64(4005))
     cases = [(P.riemann_tube(n, rng), (1,) * 6, tm)
              for n, tm in ((96, 0.05), (300, 0.1), (64, 0.02))]
+    walled = P.riemann_tube(150, rng)
+    walled[2][0][:] = 0.0  # no normal field through the wall
+    cases += [(walled, (2, 1, 0, 0, 0, 0), 0.05),
+              (P.riemann_tube(128, rng), (0,) * 6, 0.03)]
     cases += [(P.blast2d(40, seed=14), (0,) * 6, 0.02),
               (_blast_guide_field(32), (2,) * 6, 0.03),
               (_wall_state(24, 20, 0), (2, 2, 0, 0, 0, 0), 0.03),
               (P.turbulence(12, seed_b=15, seed_v=16), (0,) * 6, 0.02)]
     check_against_single(cases, use_graph=use_graph, precision="fp32c")
+
+
+def test_batch_hllt_bitwise_vs_single():
+    say("case: HLLT batch, PLM, tubes (outflow, periodic, walled) + a 2D box")
+    rng = np.random.Generator(np.random.PCG64(4007))
+    walled = P.riemann_tube(140, rng)
+    walled[2][0][:] = 0.0
+    cases = [(P.riemann_tube(200, rng), (1,) * 6, 0.05),
+             (P.riemann_tube(96, rng), (0,) * 6, 0.03),
+             (walled, (1, 2, 0, 0, 0, 0), 0.04),
+             (P.orszag_tang(32), (0,) * 6, 0.02)]
+    check_against_single(cases, rsolver="hllt")
 
 
 def test_batch_fp32c_hll_const_bitwise_vs_single_fp32c():
