@@ -223,7 +223,9 @@ def batch_rel_l1(cases, a, b):
     return {names[v]: num[v] / den[v] for v in range(12) if den[v] > 0}
 
 
-def batch_benches(reps, fp32c=True):
+def batch_benches(reps, fp32c=True, use_graph=True):
+    """The batched sweeps; ``use_graph=False`` launches every step eagerly
+    (the same kernels, for kernel-trace profiling)."""
     from bonzo_amd import problems
     out = {}
     sweeps = [("tubes", [(c, (1,) * 6, 0.1) for c in problems.tube_sweep(512, seed=4004)], reps,
@@ -234,13 +236,14 @@ def batch_benches(reps, fp32c=True):
                "64 periodic 2D MHD blast boxes, sides 64-512 on [0,1]^2, VL + PLM + HLLD + CT, "
                "CFL 0.4, each to t=0.2")]
     for name, cases, nrep, desc in sweeps:
-        r64, s64 = run_batch_workload(cases, nrep, keep_state=fp32c)
+        r64, s64 = run_batch_workload(cases, nrep, use_graph=use_graph, keep_state=fp32c)
         r64["workload"] = desc + ", FP64"
         out[name] = r64
-        if name == "tubes":
+        if name == "tubes" and use_graph:
             out["tubes_no_graph"], _ = run_batch_workload(cases, 1, use_graph=False)
         if fp32c:
-            r32, s32 = run_batch_workload(cases, nrep, precision="fp32c", keep_state=True)
+            r32, s32 = run_batch_workload(cases, nrep, use_graph=use_graph, precision="fp32c",
+                                          keep_state=True)
             r32["workload"] = desc + ", FP32-compensated"
             r32["speedup_vs_fp64"] = r32["value"] / r64["value"]
             l1 = batch_rel_l1(cases, s32, s64)
@@ -270,6 +273,8 @@ def main():
                     help="timed repetitions of the batched tube sweep (blast boxes: half)")
     ap.add_argument("--batch-only", action="store_true",
                     help="only run the batched-box workloads")
+    ap.add_argument("--batch-eager", action="store_true",
+                    help="batched workloads without the captured graph (for kernel traces)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB mebibytes (rocprof counter scale)")
     args = ap.parse_args()
@@ -286,7 +291,8 @@ def main():
     from bonzo_amd import problems
 
     if args.batch_only:
-        print(json.dumps({"batched": batch_benches(args.batch_reps, not args.no_fp32c)}),
+        print(json.dumps({"batched": batch_benches(args.batch_reps, not args.no_fp32c,
+                                                   not args.batch_eager)}),
               flush=True)
         return
     n = args.n
