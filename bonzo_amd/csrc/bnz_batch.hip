@@ -235,52 +235,26 @@ __global__ __launch_bounds__(BLK) void kb_dt(BoxState st, int nbox, double cour)
   }
 }
 
-// The end of one step and the dt of the next in one launch: t += dt of the
-// running boxes and the count of boxes still short of t_max, then the dt and
-// run flag of every box from the CFL maximum the corrector just folded (what
-// kb_dt at the start of the next step would compute from the same maximum
-// and time).
+// The end of one step and the dt of the next (time_dt_block); alone when a
+// batch has no ghost cells, else in the last workgroup of the step's last
+// ghost launch.
 __global__ __launch_bounds__(BLK) void kb_time_dt(BoxState st, int nbox, double cour) {
-  __shared__ int cnt[BLK / 64];
-  int n = 0;
-  for (int b = threadIdx.x; b < nbox; b += BLK) {
-    double t = st.t[b];
-    if (st.run[b]) {
-      t += st.dt[b];
-      st.t[b] = t;
-      st.steps[b] += 1;
-    } else {
-      st.fpar[b] ^= 1;  // the face buffers swapped without this box
-    }
-    const double tmax = st.tmax[b];
-    n += (t < tmax) ? 1 : 0;
-    const double m = __longlong_as_double((long long)st.cmax[b]);
-    st.cmax[b] = 0ull;
-    int run = 0;
-    if (t < tmax) {
-      double dt = cour / m;
-      if (t + dt > tmax) dt = tmax - t;
-      st.dt[b] = dt;
-      run = 1;
-    }
-    st.run[b] = run;
-  }
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off, 64);
-  if ((threadIdx.x & 63) == 0) cnt[threadIdx.x >> 6] = n;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int tot = 0;
-#pragma unroll
-    for (int q = 0; q < BLK / 64; ++q) tot += cnt[q];
-    *st.nrun = tot;
-  }
+  time_dt_block(st, nbox, cour);
 }
 
+// Ghost pairs (ghost cell, source cell), pool indices.  With with_time the
+// last workgroup runs time_dt_block instead (it touches no cell).
 __global__ __launch_bounds__(BLK) void kb_ghost(long sv, double *w, double *b,
-                                                const int2 *__restrict__ pairs, long npairs) {
-  for (long t = (long)blockIdx.x * BLK + threadIdx.x; t < npairs; t += (long)gridDim.x * BLK) {
-    const int2 ds = pairs[t];  // (ghost cell, source cell), pool indices
+                                                const int2 *__restrict__ pairs, long npairs,
+                                                BoxState st, int nbox, double cour,
+                                                int with_time) {
+  if (with_time && blockIdx.x == gridDim.x - 1) {
+    time_dt_block(st, nbox, cour);
+    return;
+  }
+  const long nb = gridDim.x - with_time;
+  for (long t = (long)blockIdx.x * BLK + threadIdx.x; t < npairs; t += nb * BLK) {
+    const int2 ds = pairs[t];
     double v[NMODE + 3];
 #pragma unroll
     for (int n = 0; n < NMODE; ++n) v[n] = w[n * sv + ds.y];
@@ -295,8 +269,15 @@ __global__ __launch_bounds__(BLK) void kb_ghost(long sv, double *w, double *b,
 
 // Ghost cells a reflecting side touches (GhostOp), pool indices.
 __global__ __launch_bounds__(BLK) void kb_ghost_ops(long sv, double *w, double *b,
-                                                    const GhostOp *__restrict__ ops, long nops) {
-  for (long t = (long)blockIdx.x * BLK + threadIdx.x; t < nops; t += (long)gridDim.x * BLK) {
+                                                    const GhostOp *__restrict__ ops, long nops,
+                                                    BoxState st, int nbox, double cour,
+                                                    int with_time) {
+  if (with_time && blockIdx.x == gridDim.x - 1) {
+    time_dt_block(st, nbox, cour);
+    return;
+  }
+  const long nb = gridDim.x - with_time;
+  for (long t = (long)blockIdx.x * BLK + threadIdx.x; t < nops; t += nb * BLK) {
     const GhostOp o = ops[t];
     if (!(o.bits & (1u << 15))) {
 #pragma unroll
@@ -342,6 +323,12 @@ __global__ void kb_zero(unsigned long long *a, int n) {
 }
 
 }  // namespace
+
+namespace bnzb {
+void b_time_dt(bnz_batch *c) {
+  hipLaunchKernelGGL(kb_time_dt, dim3(1), dim3(BLK), 0, c->stream, c->S, c->nbox, c->p.cour);
+}
+}  // namespace bnzb
 
 // ==========================================================================
 // Host side
@@ -487,13 +474,20 @@ void b_flux_dispatch(bnz_batch *c, int order, const double *w, const double *bf,
   }
 }
 
-void b_ghost(bnz_batch *c) {
+// Ghost fill of every box; with_time: the end-of-step time and dt kernel
+// rides in the last ghost launch (or runs alone if there is none).
+void b_ghost(bnz_batch *c, bool with_time) {
+  const int tp = with_time && c->ngops == 0 ? 1 : 0, to = with_time ? 1 : 0;
   if (c->nghost > 0)
-    hipLaunchKernelGGL(kb_ghost, dim3(grid_for((c->nghost + BLK - 1) / BLK)), dim3(BLK), 0,
-                       c->stream, c->ncell, c->P.w, c->P.b, c->ghost, c->nghost);
+    hipLaunchKernelGGL(kb_ghost, dim3(grid_for((c->nghost + BLK - 1) / BLK) + tp), dim3(BLK), 0,
+                       c->stream, c->ncell, c->P.w, c->P.b, c->ghost, c->nghost, c->S, c->nbox,
+                       c->p.cour, tp);
   if (c->ngops > 0)
-    hipLaunchKernelGGL(kb_ghost_ops, dim3(grid_for((c->ngops + BLK - 1) / BLK)), dim3(BLK), 0,
-                       c->stream, c->ncell, c->P.w, c->P.b, c->gops, c->ngops);
+    hipLaunchKernelGGL(kb_ghost_ops, dim3(grid_for((c->ngops + BLK - 1) / BLK) + to), dim3(BLK),
+                       0, c->stream, c->ncell, c->P.w, c->P.b, c->gops, c->ngops, c->S, c->nbox,
+                       c->p.cour, to);
+  else if (with_time && c->nghost == 0)
+    b_time_dt(c);
 }
 
 // One VL step of every running box (vanleer.pyx:94-167): the same launch
@@ -502,8 +496,7 @@ void b_ghost(bnz_batch *c) {
 // step's kb_time_dt (or by init_state's kb_dt).
 void b_step(bnz_batch *c) {
   if (c->f32) {
-    b32_step(c);
-    hipLaunchKernelGGL(kb_time_dt, dim3(1), dim3(BLK), 0, c->stream, c->S, c->nbox, c->p.cour);
+    b32_step(c);  // ends with the ghost fill and the time kernel
     return;
   }
   // predictor: donor cell, dt/2, limits of order 0
@@ -516,8 +509,7 @@ void b_step(bnz_batch *c) {
   b_emf(c, c->P.ws, KEM1);
   b_update<false>(c);
   std::swap(c->P.b, c->P.bs);
-  b_ghost(c);
-  hipLaunchKernelGGL(kb_time_dt, dim3(1), dim3(BLK), 0, c->stream, c->S, c->nbox, c->p.cour);
+  b_ghost(c, true);
 }
 
 int blaunch_check(bnz_batch *c) {
@@ -795,7 +787,7 @@ int bnz_batch_download(bnz_batch *c, int box, double *prim, double *bfld) {
   if (!c->stream) return bfail(c, BNZ_ESTATE, "batch was not created");
   if (c->f32 && c->w_stale) {  // FP32C state -> FP64 prim of every box, then its ghosts
     b32_to64(c);
-    b_ghost(c);
+    b_ghost(c, false);
     int rc = blaunch_check(c);
     if (rc) return rc;
     c->w_stale = false;
@@ -816,10 +808,10 @@ int bnz_batch_init_state(bnz_batch *c) {
   if (!c) return BNZ_EINVAL;
   if (!c->stream) return bfail(c, BNZ_ESTATE, "batch was not created");
   const int nb = c->nbox;
-  b_ghost(c);
+  b_ghost(c, false);
   hipLaunchKernelGGL(kb_interp, dim3(grid_for(c->nblk[KACT])), dim3(BLK), 0, c->stream, c->boxes,
                      wlist(c, KACT), c->P.w, c->P.b);
-  b_ghost(c);
+  b_ghost(c, false);
   std::vector<double> z(nb, 0.);
   std::vector<int> zi(2 * nb + 1, 0);  // steps, fpar, nrun
   BCHK(c, hipMemcpyAsync(c->S.t, z.data(), nb * sizeof(double), hipMemcpyHostToDevice, c->stream));

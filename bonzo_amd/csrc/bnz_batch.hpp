@@ -66,6 +66,48 @@ __device__ __forceinline__ void block_max_atomic(double v, unsigned long long *a
   __syncthreads();
 }
 
+// The end of one step and the dt of the next, by one workgroup of BLK
+// threads: t += dt of the running boxes and the count of boxes still short
+// of t_max, then the dt and run flag of every box from the CFL maximum the
+// corrector just folded (what kb_dt at the start of the next step would
+// compute from the same maximum and time).
+__device__ __forceinline__ void time_dt_block(const BoxState &st, int nbox, double cour) {
+  __shared__ int cnt[BLK / 64];
+  int n = 0;
+  for (int b = threadIdx.x; b < nbox; b += BLK) {
+    double t = st.t[b];
+    if (st.run[b]) {
+      t += st.dt[b];
+      st.t[b] = t;
+      st.steps[b] += 1;
+    } else {
+      st.fpar[b] ^= 1;  // the face buffers swapped without this box
+    }
+    const double tmax = st.tmax[b];
+    n += (t < tmax) ? 1 : 0;
+    const double m = __longlong_as_double((long long)st.cmax[b]);
+    st.cmax[b] = 0ull;
+    int run = 0;
+    if (t < tmax) {
+      double dt = cour / m;
+      if (t + dt > tmax) dt = tmax - t;
+      st.dt[b] = dt;
+      run = 1;
+    }
+    st.run[b] = run;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off, 64);
+  if ((threadIdx.x & 63) == 0) cnt[threadIdx.x >> 6] = n;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int tot = 0;
+#pragma unroll
+    for (int q = 0; q < BLK / 64; ++q) tot += cnt[q];
+    *st.nrun = tot;
+  }
+}
+
 // Work list kt as seen by a kernel: work blocks [lo, hi).  Each list holds
 // the blocks of the 1D boxes, then those of the 2D and of the 3D boxes, so
 // the hot kernels run one launch per dimensionality present, compiled for it.
@@ -124,6 +166,8 @@ inline WList wlist_dims(const bnz_batch *c, int kt, int dims) {
 
 // FP32-compensated step (bnz_batch32.hip)
 void b32_step(bnz_batch *c);
+// the end-of-step time and dt kernel alone (bnz_batch.hip)
+void b_time_dt(bnz_batch *c);
 void b32_init(bnz_batch *c);   // FP64 prim pool -> FP32C state; first CFL maxima
 void b32_to64(bnz_batch *c);   // FP32C state -> FP64 prim pool (active cells)
 

@@ -227,17 +227,31 @@ __global__ __launch_bounds__(BLK) void kb32_to64(const BoxDev *__restrict__ boxe
   }
 }
 
+// Ghost fill; with with_time the last workgroup ends the step instead
+// (time_dt_block), as in the FP64 kb_ghost.
 __global__ __launch_bounds__(BLK) void kb32_ghost(Ghost32 p, const int2 *__restrict__ pairs,
-                                                  long npairs) {
-  for (long t = (long)blockIdx.x * BLK + threadIdx.x; t < npairs; t += (long)gridDim.x * BLK) {
+                                                  long npairs, BoxState st, int nbox,
+                                                  double cour, int with_time) {
+  if (with_time && blockIdx.x == gridDim.x - 1) {
+    time_dt_block(st, nbox, cour);
+    return;
+  }
+  const long nb = gridDim.x - with_time;
+  for (long t = (long)blockIdx.x * BLK + threadIdx.x; t < npairs; t += nb * BLK) {
     const int2 ds = pairs[t];
     ghost32_pair(p, ds.x, ds.y);
   }
 }
 
 __global__ __launch_bounds__(BLK) void kb32_ghost_ops(Ghost32 p, const GhostOp *__restrict__ ops,
-                                                      long nops) {
-  for (long t = (long)blockIdx.x * BLK + threadIdx.x; t < nops; t += (long)gridDim.x * BLK)
+                                                      long nops, BoxState st, int nbox,
+                                                      double cour, int with_time) {
+  if (with_time && blockIdx.x == gridDim.x - 1) {
+    time_dt_block(st, nbox, cour);
+    return;
+  }
+  const long nb = gridDim.x - with_time;
+  for (long t = (long)blockIdx.x * BLK + threadIdx.x; t < nops; t += nb * BLK)
     ghost32_op(p, ops[t]);
 }
 
@@ -344,12 +358,16 @@ void b32_step(bnz_batch *c) {
   updates<false>(c);
   std::swap(c->P.b, c->P.bs);
   const Ghost32 p = ghost_arrays(c);
+  // the end-of-step time and dt kernel rides in the last ghost launch
+  const int tp = c->ngops == 0 ? 1 : 0;
   if (c->nghost > 0)
-    hipLaunchKernelGGL(kb32_ghost, dim3(grid_for((c->nghost + BLK - 1) / BLK)), dim3(BLK), 0,
-                       c->stream, p, c->ghost, c->nghost);
+    hipLaunchKernelGGL(kb32_ghost, dim3(grid_for((c->nghost + BLK - 1) / BLK) + tp), dim3(BLK),
+                       0, c->stream, p, c->ghost, c->nghost, c->S, c->nbox, c->p.cour, tp);
   if (c->ngops > 0)
-    hipLaunchKernelGGL(kb32_ghost_ops, dim3(grid_for((c->ngops + BLK - 1) / BLK)), dim3(BLK), 0,
-                       c->stream, p, c->gops, c->ngops);
+    hipLaunchKernelGGL(kb32_ghost_ops, dim3(grid_for((c->ngops + BLK - 1) / BLK) + 1), dim3(BLK),
+                       0, c->stream, p, c->gops, c->ngops, c->S, c->nbox, c->p.cour, 1);
+  else if (c->nghost == 0)
+    b_time_dt(c);
 }
 
 void b32_init(bnz_batch *c) {
