@@ -169,11 +169,15 @@ def run_path(prec, n, prim, bfld, args):
     }
 
 
-def run_batch_workload(cases, reps, use_graph=True, precision="fp64", keep_state=False):
+def run_batch_workload(cases, reps, use_graph=True, precision="fp64", keep_state=False,
+                       tune=True, blocks=None):
     """Time a batch of boxes from their initial states to their t_max:
     ``reps`` timed repetitions after one warm-up (which also captures the
-    graph); cell-updates = sum over boxes of active cells x steps taken."""
+    graph); cell-updates = sum over boxes of active cells x steps taken.
+    ``blocks`` (flux, emf, update workgroup caps) are applied as given, else
+    ``tune`` picks them with the batch autotuner."""
     from bonzo_amd._lib import Batch
+    from bonzo_amd.autotune import autotune_batch
     b = Batch([dict(nx=s.nx, ny=s.ny, nz=s.nz, bc=bc, lmin=s.lmin, lmax=s.lmax, tmax=tm)
                for (s, _, _), bc, tm in cases], rsolver="hlld", reconstr="linear", cour=0.4,
               precision=precision)
@@ -183,6 +187,14 @@ def run_batch_workload(cases, reps, use_graph=True, precision="fp64", keep_state
             b.upload(q, prim, bfld)
         b.init_state()
 
+    table = {}
+    if blocks is not None:
+        tuned = tuple(blocks)
+        b.set_tuning(*tuned)
+    elif tune:
+        tuned, table = autotune_batch(b, reset)
+    else:
+        tuned = (0, 0, 0)
     reset()
     b.advance(max_steps=2, use_graph=use_graph)
     walls = []
@@ -202,7 +214,10 @@ def run_batch_workload(cases, reps, use_graph=True, precision="fp64", keep_state
            "wall_s": walls, "cell_updates": cu, "boxes": len(cases), "cells": sum(cells),
            "steps_min": int(steps.min()), "steps_max": int(steps.max()),
            "all_at_tmax": bool(all(t[q] == cases[q][2] for q in range(len(cases)))),
-           "graph": use_graph, "precision": precision}
+           "graph": use_graph, "precision": precision,
+           "tuned_blocks": dict(zip(("flux", "emf", "update"), tuned)),
+           "autotune_ms": {k: {str(c): round(v * 1e3, 3) for c, v in t.items()}
+                           for k, t in table.items()}}
     return out, state
 
 
@@ -240,7 +255,8 @@ def batch_benches(reps, fp32c=True, use_graph=True):
         r64["workload"] = desc + ", FP64"
         out[name] = r64
         if name == "tubes" and use_graph:
-            out["tubes_no_graph"], _ = run_batch_workload(cases, 1, use_graph=False)
+            tb = tuple(r64["tuned_blocks"].values())
+            out["tubes_no_graph"], _ = run_batch_workload(cases, 1, use_graph=False, blocks=tb)
         if fp32c:
             r32, s32 = run_batch_workload(cases, nrep, use_graph=use_graph, precision="fp32c",
                                           keep_state=True)

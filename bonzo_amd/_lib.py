@@ -26,7 +26,7 @@ ABI_SYMBOLS = (
     "bnz_gpu_flux_pencils",
     "bnz_batch_create", "bnz_batch_destroy", "bnz_batch_last_error", "bnz_batch_box_shape",
     "bnz_batch_upload", "bnz_batch_download", "bnz_batch_init_state", "bnz_batch_advance",
-    "bnz_batch_get_times", "bnz_batch_synchronize",
+    "bnz_batch_get_times", "bnz_batch_synchronize", "bnz_batch_set_tuning",
 )
 
 
@@ -117,6 +117,7 @@ def load():
     L.bnz_batch_synchronize.argtypes = [vp]
     L.bnz_batch_advance.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     L.bnz_batch_get_times.argtypes = [vp, _dp, ctypes.POINTER(ctypes.c_int)]
+    L.bnz_batch_set_tuning.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     for name in ABI_SYMBOLS:
         if name not in ("bnz_last_error", "bnz_batch_last_error"):
             getattr(L, name).restype = ctypes.c_int
@@ -359,6 +360,12 @@ class Batch:
 
     def synchronize(self):
         self._check(load().bnz_batch_synchronize(self._h))
+
+    def set_tuning(self, flux_blocks=0, emf_blocks=0, update_blocks=0):
+        """Workgroup caps of the flux, EMF and update (and fused 1D) launches;
+        0 is the default.  Results do not depend on them."""
+        self._check(load().bnz_batch_set_tuning(self._h, int(flux_blocks), int(emf_blocks),
+                                                int(update_blocks)))
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:

@@ -1,4 +1,5 @@
-"""Autotuner for the work shapes of the 3D kernels.
+"""Autotuners: work shapes of a single grid's 3D kernels (autotune) and
+workgroup caps of a batch's stage launches (autotune_batch).
 
 Each candidate shape (0 = one thread per face/edge/cell in x-fastest order,
 1 = 64x4 tiles ordered along z, n > 1 = slab columns of n rows walked plane
@@ -12,7 +13,10 @@ keeps the candidate with the smallest summed time:
 The runs advance the context's state: callers tune first and then upload
 their initial state again, or tune a scratch context of the same shape.
 """
+import time
+
 CANDIDATES = (0, 1, 4, 8, 16, 32)
+BATCH_CANDIDATES = (2048, 4096, 8192, 16384, 65536)
 
 
 def _class_times(kt):
@@ -45,3 +49,35 @@ def autotune(ctx, steps=2, candidates=CANDIDATES):
     best = tuple(min(candidates, key=lambda c: table[c][k]) for k in ("flux", "emf", "update"))
     ctx.set_tuning(*best)
     return best, table
+
+
+def autotune_batch(batch, reset, steps=16, candidates=BATCH_CANDIDATES):
+    """Pick the workgroup caps of ``batch``'s flux, EMF and update launches
+    (bnz_batch_set_tuning) and apply them.
+
+    The launches walk their work blocks with at most that many workgroups;
+    fewer, longer-lived workgroups pay less dispatch, more of them balance
+    the end of a launch better.  The classes are tuned one after the other,
+    the others held at their best so far; each candidate runs ``steps`` eager
+    steps from the state ``reset()`` restores (upload and init_state), timed
+    around the synchronising advance.  Returns (best, table) with table[class]
+    = {candidate: seconds}; the batch is left reset."""
+    best = [0, 0, 0]
+    table = {}
+    for k, name in enumerate(("flux", "emf", "update")):
+        times = {}
+        for c in candidates:
+            trial = list(best)
+            trial[k] = c
+            batch.set_tuning(*trial)
+            reset()
+            batch.advance(max_steps=1, use_graph=False)  # first launch of a size pays setup
+            batch.synchronize()
+            t0 = time.perf_counter()
+            batch.advance(max_steps=steps, use_graph=False)
+            times[c] = time.perf_counter() - t0
+        best[k] = min(times, key=times.get)
+        table[name] = times
+    batch.set_tuning(*best)
+    reset()
+    return tuple(best), table

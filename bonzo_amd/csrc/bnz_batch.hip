@@ -401,7 +401,7 @@ void b_fluxes(bnz_batch *c, const double *w, const double *bf, int kt) {
   for (int d = 2; d <= 3; ++d) {
     const WList wl = wlist_dims(c, kt, d);
     if (wl.hi == wl.lo) continue;
-    const dim3 grid(grid_for(wl.hi - wl.lo));
+    const dim3 grid(grid_for(wl.hi - wl.lo, c->cap[TF]));
     if (d == 2)
       hipLaunchKernelGGL((kb_flux<ORDER, RS, 2>), grid, dim3(BLK), 0, c->stream, c->boxes, wl,
                          c->S, w, bf, c->P.fx, c->P.fy, c->P.fz);
@@ -416,7 +416,7 @@ void b_emf(bnz_batch *c, const double *w, int kt) {
   for (int d = 2; d <= 3; ++d) {
     const WList wl = wlist_dims(c, kt, d);
     if (wl.hi == wl.lo) continue;
-    const dim3 grid(grid_for(wl.hi - wl.lo));
+    const dim3 grid(grid_for(wl.hi - wl.lo, c->cap[TE]));
     if (d == 2)
       hipLaunchKernelGGL(kb_emf<2>, grid, dim3(BLK), 0, c->stream, c->boxes, wl, c->S, w,
                          c->P.fx, c->P.fy, c->P.fz, c->P.ee);
@@ -431,7 +431,8 @@ template <bool PRED, int ORDER, int RS>
 void b_stage1(bnz_batch *c) {
   const WList wl = wlist(c, KS1);
   if (wl.hi == wl.lo) return;
-  hipLaunchKernelGGL((kb_stage1<PRED, ORDER, RS>), dim3(grid_for(wl.hi - wl.lo)), dim3(BLK), 0,
+  const dim3 grid(grid_for(wl.hi - wl.lo, c->cap[TU]));
+  hipLaunchKernelGGL((kb_stage1<PRED, ORDER, RS>), grid, dim3(BLK), 0,
                      c->stream, c->boxes, wl, c->S, c->P, c->p.dt_fms_ref);
 }
 
@@ -441,7 +442,7 @@ void b_update(bnz_batch *c) {
   for (int d = 2; d <= 3; ++d) {
     const WList wl = wlist_dims(c, KUPD, d);
     if (wl.hi == wl.lo) continue;
-    const dim3 grid(grid_for(wl.hi - wl.lo));
+    const dim3 grid(grid_for(wl.hi - wl.lo, c->cap[TU]));
     if (d == 2)
       hipLaunchKernelGGL((kb_update<PRED, 2>), grid, dim3(BLK), 0, c->stream, c->boxes, wl, c->S,
                          c->P, fr);
@@ -894,6 +895,17 @@ int bnz_batch_advance(bnz_batch *c, int max_steps, int use_graph, int *nsteps) {
   BCHK(c, hipStreamSynchronize(c->stream));
   if (nsteps) *nsteps = n;
   c->w_stale = c->f32;
+  return BNZ_OK;
+}
+
+int bnz_batch_set_tuning(bnz_batch *c, int flux_blocks, int emf_blocks, int update_blocks) {
+  if (!c) return BNZ_EINVAL;
+  const int v[NTUNE] = {flux_blocks, emf_blocks, update_blocks};
+  for (int q = 0; q < NTUNE; ++q)
+    if (v[q] < 0 || v[q] > (1 << 24)) return bfail(c, BNZ_EINVAL, "workgroup cap out of range");
+  if (c->stream) BCHK(c, hipStreamSynchronize(c->stream));
+  drop_graph(c);  // the captured launches carry the old grid sizes
+  for (int q = 0; q < NTUNE; ++q) c->cap[q] = v[q] ? v[q] : PERSIST;
   return BNZ_OK;
 }
 

@@ -117,9 +117,16 @@ struct WList {
   int kt;
 };
 
-constexpr int PERSIST = 256 * 8;  // workgroups of a persistent launch
+// Workgroups of a launch over nblk work blocks: at most cap (each walks the
+// blocks cap apart), at least one.  PERSIST is the cap of the setup kernels
+// and the default of the stage kernels, whose caps bnz_batch_set_tuning sets
+// per kernel class.
+constexpr int PERSIST = 16384;
+enum { TF = 0, TE, TU, NTUNE };  // tuning classes: fluxes, EMFs, updates (+ 1D stages)
 
-inline int grid_for(long nblk) { return (int)std::max<long>(1, std::min<long>(nblk, PERSIST)); }
+inline int grid_for(long nblk, long cap = PERSIST) {
+  return (int)std::max<long>(1, std::min<long>(nblk, cap));
+}
 
 }  // namespace bnzb
 
@@ -130,6 +137,7 @@ struct bnz_batch {
   int nblk[bnzb::NK] = {};
   int bm_off[bnzb::NK] = {};
   int cls[bnzb::NK][4] = {};  // first work block of the 1D, 2D, 3D boxes; nblk
+  int cap[bnzb::NTUNE] = {bnzb::PERSIST, bnzb::PERSIST, bnzb::PERSIST};
   long ncell = 0, nghost = 0, ngops = 0;
   hipStream_t stream = nullptr;
   bnzb::BoxDev *boxes = nullptr;

@@ -261,7 +261,7 @@ void flux_launch(bnz_batch *c, const float *w, const BF *bf, int kt) {
   for (int d = 2; d <= 3; ++d) {
     const WList wl = wlist_dims(c, kt, d);
     if (wl.hi == wl.lo) continue;
-    const dim3 grid(grid_for(wl.hi - wl.lo));
+    const dim3 grid(grid_for(wl.hi - wl.lo, c->cap[TF]));
     if (d == 2)
       hipLaunchKernelGGL((kb32_flux<ORDER, RS, BF, 2>), grid, dim3(BLK), 0, c->stream, c->boxes,
                          wl, c->S, w, bf, c->Q.fx, c->Q.fy, c->Q.fz, c->nguard);
@@ -288,7 +288,7 @@ void emfs(bnz_batch *c, const float *w, int kt) {
   for (int d = 2; d <= 3; ++d) {
     const WList wl = wlist_dims(c, kt, d);
     if (wl.hi == wl.lo) continue;
-    const dim3 grid(grid_for(wl.hi - wl.lo));
+    const dim3 grid(grid_for(wl.hi - wl.lo, c->cap[TE]));
     if (d == 2)
       hipLaunchKernelGGL(kb32_emf<2>, grid, dim3(BLK), 0, c->stream, c->boxes, wl, c->S, w,
                          c->Q.fx, c->Q.fy, c->Q.fz, c->Q.ee);
@@ -302,7 +302,8 @@ template <bool PRED, int ORDER, int RS>
 void stage1(bnz_batch *c) {
   const WList wl = wlist(c, KS1);
   if (wl.hi == wl.lo) return;
-  hipLaunchKernelGGL((kb32_stage1<PRED, ORDER, RS>), dim3(grid_for(wl.hi - wl.lo)), dim3(BLK), 0,
+  const dim3 grid(grid_for(wl.hi - wl.lo, c->cap[TU]));
+  hipLaunchKernelGGL((kb32_stage1<PRED, ORDER, RS>), grid, dim3(BLK), 0,
                      c->stream, c->boxes, wl, c->S, c->P, c->Q, c->p.dt_fms_ref, c->nguard);
 }
 
@@ -313,7 +314,7 @@ void updates(bnz_batch *c) {
   for (int d = 2; d <= 3; ++d) {
     const WList wl = wlist_dims(c, KUPD, d);
     if (wl.hi == wl.lo) continue;
-    const dim3 grid(grid_for(wl.hi - wl.lo));
+    const dim3 grid(grid_for(wl.hi - wl.lo, c->cap[TU]));
     if (d == 2)
       hipLaunchKernelGGL((kb32_update<PRED, 2>), grid, dim3(BLK), 0, c->stream, c->boxes, wl,
                          c->S, c->P, c->Q, fr);

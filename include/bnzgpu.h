@@ -186,6 +186,13 @@ int bnz_batch_init_state(bnz_batch *b);
    finished (a box at its t_max idles).  *nsteps gets the number of step
    launches; bnz_batch_get_times gives the steps each box took. */
 int bnz_batch_advance(bnz_batch *b, int max_steps, int use_graph, int *nsteps);
+/* Workgroups of the stage launches: at most flux_blocks (2D/3D fluxes),
+   emf_blocks (EMFs) and update_blocks (2D/3D updates and the fused 1D
+   stages); each workgroup walks the work blocks of its launch that many
+   apart.  0 restores the default (16384).  Results do not depend on it;
+   bonzo_amd.autotune.autotune_batch picks it on the device.  Drops the
+   captured graph.  No reference counterpart (the reference has no batch). */
+int bnz_batch_set_tuning(bnz_batch *b, int flux_blocks, int emf_blocks, int update_blocks);
 /* Per-box time and step count (arrays of nbox) */
 int bnz_batch_get_times(bnz_batch *b, double *t, int *steps);
 int bnz_batch_synchronize(bnz_batch *b);

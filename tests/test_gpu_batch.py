@@ -173,6 +173,35 @@ def test_batch_finished_boxes_across_pieces(precision):
     b.close()
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32c"])
+def test_batch_launch_sizes_do_not_change_results(precision):
+    """The workgroup caps of the stage launches (bnz_batch_set_tuning) only
+    change how the work blocks are dealt out: one workgroup for everything,
+    odd caps and the defaults give the same states bit for bit."""
+    say(f"case: launch-size caps (1,1,1), (7,3,5), default; 1D + 2D + 3D boxes, {precision}")
+    from bonzo_amd._lib import Batch, BnzError
+    rng = np.random.Generator(np.random.PCG64(4242))
+    cases = [(P.riemann_tube(300, rng), (1,) * 6, 0.03),
+             (P.orszag_tang(40), (0,) * 6, 0.02),
+             (P.turbulence(12, seed_b=21, seed_v=22), (0,) * 6, 0.01)]
+    ref, tref, sref, _ = run_batch(cases, precision=precision)
+    for caps in ((1, 1, 1), (7, 3, 5)):
+        b = Batch([box_desc(c[0][0], c[1], c[2]) for c in cases], precision=precision)
+        b.set_tuning(*caps)
+        for q, ((s, prim, bfld), _, _) in enumerate(cases):
+            b.upload(q, prim, bfld)
+        b.init_state()
+        b.advance()
+        t, steps = b.times()
+        assert np.array_equal(t, tref) and np.array_equal(steps, sref), caps
+        for q in range(len(cases)):
+            p_, b_ = b.download(q)
+            assert np.array_equal(p_, ref[q][0]) and np.array_equal(b_, ref[q][1]), (caps, q)
+        with pytest.raises(BnzError):
+            b.set_tuning(-1, 0, 0)
+        b.close()
+
+
 def test_tube_sweep_conserves_and_reaches_tmax():
     """A slice of the 1D sweep workload (64 tubes of 64..4096 cells) runs to
     t=0.1 with the graph; every box ends exactly at t_max with finite state."""
