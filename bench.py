@@ -45,8 +45,8 @@ KERNEL_BYTES = {
 # WRITE_SIZE x 1.0, scaled by a plain device copy), from the rocprofv3 --pmc
 # runs summarised in the named profile; the bench itself runs no counters.
 PROFILED_TRAFFIC = {
-    ("fp64", "flux3_plm"): (6.019e9, "profiles/vl_step_kernels_turb256.md"),
-    ("fp32c", "update_corr"): (4.777e9, "profiles/vl_step_kernels_turb256.md"),
+    ("fp64", "flux3_plm"): (6.009e9, "profiles/vl_step_kernels_turb256.md"),
+    ("fp32c", "update_corr"): (4.773e9, "profiles/vl_step_kernels_turb256.md"),
 }
 # Algorithmic bytes per cell-update of a whole VL step: 5 state volumes of 9
 # words (FP64 360 B; FP32C 252 B with hi + lo on the stage-2 read of S_n and
