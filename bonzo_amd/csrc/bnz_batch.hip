@@ -103,65 +103,22 @@ __global__ __launch_bounds__(BLK) void kb_update(const BoxDev *__restrict__ boxe
   }
 }
 
-// Fluxes of a run of 1D faces held in LDS, s[n * BLK + slot]: slot t + off
-// is the face whose lower cell is the updating thread's cell + off.
-struct FluxLds1D {
-  const double *s;
-  int t;
-  __device__ __forceinline__ double x(int n, long off) const { return s[n * BLK + t + off]; }
-  __device__ __forceinline__ double y(int, long) const { return 0.; }
-  __device__ __forceinline__ double z(int, long) const { return 0.; }
-};
-struct EmfLds1D {  // Emf1D on the LDS fluxes
-  const double *s;
-  int t;
-  __device__ __forceinline__ double operator()(int comp, long off) const {
-    return comp == 2 ? -s[BY * BLK + t + off] : comp == 1 ? s[BZ * BLK + t + off] : 0.;
-  }
-};
-
-// One stage of the 1D boxes in one pass: thread t computes the flux of face
-// i0 + t into LDS, then updates cell i0 + t from faces i0 + t and i0 + t + 1
-// (t < BLK - 1), so the fluxes never go through HBM.  The face and cell
-// arithmetic is that of kb_flux (flux_face_to) and kb_update
-// (update_cell_e), on the same faces and cells.  The corrector reads the
-// normal field of its faces from the step's faces rather than the
-// predictor's: in 1D the predictor copies x faces unchanged, so the values
-// are the same, and the corrector's face writes cannot race the reads.
+// One stage of the 1D boxes in one pass (stage1_cell): the fluxes of BLK
+// faces go to LDS, the BLK - 1 cells between them are updated from there,
+// so the fluxes never go through HBM.
 template <bool PRED, int ORDER, int RS>
 __global__ __launch_bounds__(BLK) void kb_stage1(const BoxDev *__restrict__ boxes, WList wl,
                                                  BoxState st, Pools p, int fms_ref) {
   __shared__ double sf[NMODE * BLK];
-  const int tid = threadIdx.x;
   for (int wb = wl.lo + blockIdx.x; wb < wl.hi; wb += gridDim.x) {
     const int b = wl.bmap[wb];
     if (!st.run[b]) continue;
     const BoxDev &d = boxes[b];
-    const Geo &g = d.g;
-    const Lims &L = d.L[PRED ? 0 : 1];
-    const long o = d.off, sv = g.sv;
-    const int i = (wb - d.blk0[wl.kt]) * (BLK - 1) + tid;
-    if (i >= L.lo[0] && i <= L.hi[0] + 1) {  // the stage's faces (flux3_box)
-      bool got = false;
-      flux_face_to<0, ORDER, RS>(g, PRED ? p.w + o : p.ws + o, p.b + o, i, 0, 0,
-                                 [&](const double *f, long) {
-#pragma unroll
-                                   for (int n = 0; n < NMODE; ++n)
-                                     if (n != BX) sf[n * BLK + tid] = f[n];
-                                   got = true;
-                                 });
-      if (!got)  // no region matched: the flux row keeps what it held (flux_face)
-        for (int n = 0; n < NMODE; ++n)
-          if (n != BX) sf[n * BLK + tid] = p.fx[n * sv + o + i];
-    }
-    __syncthreads();
-    double cds = 0.;
-    if (tid < BLK - 1 && i < g.nt[0]) {
-      const double r = update_cell_e<1, PRED, !PRED>(
-          g, GeoArr{g}, p.w + o, FluxLds1D{sf, tid}, EmfLds1D{sf, tid}, p.b + o, p.bs + o,
-          PRED ? p.ws + o : p.w + o, L, st.dt + b, d.act, fms_ref, i, 0, 0);
-      if (!PRED) cds = r;
-    }
+    const long o = d.off;
+    const int i = (wb - d.blk0[wl.kt]) * (BLK - 1) + threadIdx.x;
+    const double cds = stage1_cell<PRED, ORDER, RS>(
+        d.g, PRED ? p.w + o : p.ws + o, p.b + o, p.bs + o, p.w + o, PRED ? p.ws + o : p.w + o,
+        p.fx + o, d.L[PRED ? 0 : 1], st.dt + b, d.act, fms_ref, i, sf);
     if (!PRED)
       block_max_atomic(cds, st.cmax + b);  // ends with a barrier: sf is free again
     else

@@ -86,93 +86,41 @@ __global__ __launch_bounds__(BLK) void kb32_update(const BoxDev *__restrict__ bo
     if (t < it.total()) {
       int i, j, k;
       box_idx(it, t, i, j, k);
-      const Walls wa = box_walls(d);
       if (PRED) {
         const Pred32 a{Q.w32 + o, Q.uh + o, P.b + o, Q.fx + o, Q.fy + o,
                        Q.fz + o,  Q.ee + o, Q.bs32 + o, Q.ws32 + o};
-        update32_pred<DIMS>(d.g, a, d.L[0], st.dt + b, wa, i, j, k);
+        update32_pred<DIMS>(d.g, a, d.L[0], st.dt + b, i, j, k);
       } else {
         const Corr32 a{Q.uh + o, Q.ul + o, P.b + o, Q.fx + o, Q.fy + o,
                        Q.fz + o, Q.ee + o, P.bs + o, Q.w32 + o};
-        cds = update32_corr<DIMS>(d.g, a, d.L[1], st.dt + b, d.act, fms_ref, wa, i, j, k);
+        cds = update32_corr<DIMS>(d.g, a, d.L[1], st.dt + b, d.act, fms_ref, i, j, k);
       }
     }
     if (!PRED) block_max_atomic(cds, st.cmax + b);
   }
 }
 
-// Fluxes of a run of 1D faces in LDS (as the FP64 FluxLds1D) and the 1D
-// EMFs formed from them with the wall planes zeroed (as Emf32x1D).
-struct FluxLds32 {
-  const float *s;
-  int t;
-  __device__ __forceinline__ float x(int n, long off) const { return s[n * BLK + t + off]; }
-  __device__ __forceinline__ float y(int, long) const { return 0.f; }
-  __device__ __forceinline__ float z(int, long) const { return 0.f; }
-};
-struct EmfLds32x1D {
-  const float *s;
-  int t, i, wlo, whi;
-  __device__ __forceinline__ float operator()(int comp, long off) const {
-    const int e = i + (int)off;
-    if (comp == 0 || e == wlo || e == whi) return 0.f;
-    return comp == 2 ? -s[BY * BLK + t + off] : s[BZ * BLK + t + off];
-  }
-};
-
-// One FP32C stage of the 1D boxes in one pass, as the FP64 kb_stage1: the
-// fluxes of BLK faces go to LDS, the BLK - 1 cells between them are updated
-// from there with the kb32_flux / kb32_update arithmetic.  The predictor
-// writes the FP32 predictor faces and prim (bs32, ws32), which the
-// corrector's fluxes read; the corrector writes FP64 faces to the other
-// face buffer and its own cell's state, so no read of the pass races a
-// write.
+// One FP32C stage of the 1D boxes in one pass (stage32_cell), as the FP64
+// kb_stage1.
 template <bool PRED, int ORDER, int RS>
 __global__ __launch_bounds__(BLK) void kb32_stage1(const BoxDev *__restrict__ boxes, WList wl,
                                                    BoxState st, Pools P, Pools32 Q, int fms_ref,
                                                    int *nguard) {
   __shared__ float sf[NMODE * BLK];
-  const int tid = threadIdx.x;
   int ng = 0;
   for (int wb = wl.lo + blockIdx.x; wb < wl.hi; wb += gridDim.x) {
     const int b = wl.bmap[wb];
     if (!st.run[b]) continue;
     const BoxDev &d = boxes[b];
-    const Geo &g = d.g;
-    const Lims &L = d.L[PRED ? 0 : 1];
-    const long o = d.off, sv = g.sv;
-    const int i = (wb - d.blk0[wl.kt]) * (BLK - 1) + tid;
-    if (i >= L.lo[0] && i <= L.hi[0] + 1) {  // the stage's faces (flux3_box)
-      bool got = false;
-      const auto sink = [&](const float *f, long) {
-#pragma unroll
-        for (int n = 0; n < NMODE; ++n)
-          if (n != BX) sf[n * BLK + tid] = f[n];
-        got = true;
-      };
-      if (PRED)
-        flux32_face_to<0, ORDER, RS>(g, Q.w32 + o, P.b + o, i, 0, 0, ng, sink);
-      else
-        flux32_face_to<0, ORDER, RS>(g, Q.ws32 + o, (const float *)Q.bs32 + o, i, 0, 0, ng, sink);
-      if (!got)  // no region matched: the flux row keeps what it held
-        for (int n = 0; n < NMODE; ++n)
-          if (n != BX) sf[n * BLK + tid] = Q.fx[n * sv + o + i];
-    }
-    __syncthreads();
-    double cds = 0.;
-    if (tid < BLK - 1 && i < g.nt[0]) {
-      const FluxLds32 fl{sf, tid};
-      const EmfLds32x1D em{sf, tid, i, d.walls[0], d.walls[1]};
-      if (PRED) {
-        const Pred32 a{Q.w32 + o, Q.uh + o, P.b + o, Q.fx + o, Q.fy + o,
-                       Q.fz + o,  Q.ee + o, Q.bs32 + o, Q.ws32 + o};
-        update32_pred_e<1, GeoArr>(g, a, L, st.dt + b, fl, em, i, 0, 0);
-      } else {
-        const Corr32 a{Q.uh + o, Q.ul + o, P.b + o, Q.fx + o, Q.fy + o,
-                       Q.fz + o, Q.ee + o, P.bs + o, Q.w32 + o};
-        cds = update32_corr_e<1, GeoArr>(g, a, L, st.dt + b, d.act, fms_ref, fl, em, i, 0, 0);
-      }
-    }
+    const long o = d.off;
+    const int i = (wb - d.blk0[wl.kt]) * (BLK - 1) + threadIdx.x;
+    const Pred32 ap{Q.w32 + o, Q.uh + o, P.b + o, Q.fx + o, Q.fy + o,
+                    Q.fz + o,  Q.ee + o, Q.bs32 + o, Q.ws32 + o};
+    const Corr32 ac{Q.uh + o, Q.ul + o, P.b + o, Q.fx + o, Q.fy + o,
+                    Q.fz + o, Q.ee + o, P.bs + o, Q.w32 + o};
+    const double cds = stage32_cell<PRED, ORDER, RS>(
+        d.g, Q.ws32 + o, Q.bs32 + o, ap, ac, d.L[PRED ? 0 : 1], st.dt + b, d.act, fms_ref,
+        d.walls[0], d.walls[1], i, sf, ng);
     if (!PRED)
       block_max_atomic(cds, st.cmax + b);  // ends with a barrier: sf is free again
     else

@@ -46,6 +46,27 @@ __global__ __launch_bounds__(BLK) void k_flux3(Geo g, const double *__restrict__
   flux_cell<ORDER, RS>(g, w, bf, fx, fy, fz, L, i, j, k);
 }
 
+// One stage of a 1D grid in one pass (stage1_cell): workgroup b owns faces
+// b * (BLK - 1) .. + BLK - 1 and the BLK - 1 cells between them; the
+// corrector leaves its workgroup's CFL maximum in part.
+template <bool PRED, int ORDER, int RS>
+__global__ __launch_bounds__(BLK) void k_stage1(Geo g, const double *__restrict__ wf,
+                                                const double *b0, double *b1, const double *w0,
+                                                double *w1, const double *__restrict__ fx_old,
+                                                Lims L, const double *__restrict__ dts, Box act,
+                                                int fms_ref, double *__restrict__ part) {
+  __shared__ double sf[NMODE * BLK];
+  const int i = blockIdx.x * (BLK - 1) + threadIdx.x;
+  if (dts[0] == 0.) {  // a step past t_max: state unchanged, faces carried over
+    if (!PRED && threadIdx.x < BLK - 1 && i < g.nt[0])
+      for (int a = 0; a < 3; ++a) b1[a * g.sv + i] = b0[a * g.sv + i];
+    return;
+  }
+  const double cds =
+      stage1_cell<PRED, ORDER, RS>(g, wf, b0, b1, w0, w1, fx_old, L, dts, act, fms_ref, i, sf);
+  if (!PRED) block_max_to(cds, part);
+}
+
 // --------------------------------------------------------------------------
 // Edge EMFs, interp_ee1 (src/integration/ct.pyx:153-215) with the cell-centred
 // E = -v x B of ec_from_prim (ct.pyx:128-148) computed on the fly.
@@ -476,8 +497,7 @@ void launch_emf(bnz_ctx *c, const double *w, const Lims &L, const char *name) {
   dim3(sh.blocks), dim3(BLK), 0, c->stream, c->g, w, c->fx, c->fy, c->fz, c->ee, e, sh.ty
   if (sh.tiled) hipLaunchKernelGGL((k_emf<3, true>), EMF_ARGS);
   else if (c->dims == 3) hipLaunchKernelGGL((k_emf<3, false>), EMF_ARGS);
-  else if (c->dims == 2) hipLaunchKernelGGL((k_emf<2, false>), EMF_ARGS);
-  else hipLaunchKernelGGL((k_emf<1, false>), EMF_ARGS);
+  else hipLaunchKernelGGL((k_emf<2, false>), EMF_ARGS);  // 1D grids run k_stage1
 #undef EMF_ARGS
 }
 
@@ -532,32 +552,60 @@ void launch_update(bnz_ctx *c, bool pred, const Lims &L) {
   } else if (c->dims == 3) {
     if (pred) hipLaunchKernelGGL((k_update<3, true, false, false>), UPD_ARGS);
     else hipLaunchKernelGGL((k_update<3, false, true, false>), UPD_ARGS);
-  } else if (c->dims == 2) {
+  } else {  // 2D (1D grids run k_stage1)
     if (pred) hipLaunchKernelGGL((k_update<2, true, false, false>), UPD_ARGS);
     else hipLaunchKernelGGL((k_update<2, false, true, false>), UPD_ARGS);
-  } else {
-    if (pred) hipLaunchKernelGGL((k_update<1, true, false, false>), UPD_ARGS);
-    else hipLaunchKernelGGL((k_update<1, false, true, false>), UPD_ARGS);
   }
 #undef UPD_ARGS
   // the corrector leaves the next step's CFL partials in part[0..blocks)
   c->dt_parts = pred ? 0 : sh.blocks;
 }
 
+// A 1D stage as one fused launch (k_stage1): predictor w -> ws, bs;
+// corrector ws -> w (in place), bs.
+template <bool PRED, int ORDER, int RS>
+void stage1_launch(bnz_ctx *c, const Lims &L) {
+  const int nb = (c->nt[0] + BLK - 2) / (BLK - 1);
+  Launch t(c, PRED ? "stage1_pred" : "stage1_corr");
+  hipLaunchKernelGGL((k_stage1<PRED, ORDER, RS>), dim3(nb), dim3(BLK), 0, c->stream, c->g,
+                     PRED ? c->w : c->ws, c->b, c->bs, c->w, PRED ? c->ws : c->w, c->fx, L,
+                     c->dev, active_box(c), c->p.dt_fms_ref, c->part);
+  c->dt_parts = PRED ? 0 : nb;
+}
+
+template <bool PRED>
+void launch_stage1(bnz_ctx *c, int order, const Lims &L) {
+  const int rs = c->p.rsolver;
+  if (order == 0) {
+    if (rs == BNZ_RS_HLLD) stage1_launch<PRED, 0, BNZ_RS_HLLD>(c, L);
+    else if (rs == BNZ_RS_HLLT) stage1_launch<PRED, 0, BNZ_RS_HLLT>(c, L);
+    else stage1_launch<PRED, 0, BNZ_RS_HLL>(c, L);
+  } else {
+    if (rs == BNZ_RS_HLLD) stage1_launch<PRED, 1, BNZ_RS_HLLD>(c, L);
+    else if (rs == BNZ_RS_HLLT) stage1_launch<PRED, 1, BNZ_RS_HLLT>(c, L);
+    else stage1_launch<PRED, 1, BNZ_RS_HLL>(c, L);
+  }
+}
+
 // One VL step after the dt kernels (vanleer.pyx:100-163).
 void launch_step_body(bnz_ctx *c) {
-  // predictor: donor cell, dt/2, lims of order 0; ws and bs are new buffers
   Lims L0 = integr_lims(c, 0);
-  launch_flux_dispatch(c, 0, c->w, c->b, L0);
-  if (c->dims > 1) launch_emf(c, c->w, L0, "emf_pred");  // 1D: EMFs formed in the update
-  launch_update(c, true, L0);
-  // corrector: reconstruction order, dt; prim is updated in place (each cell
-  // reads only its own w_n), b_{n+1} goes to the predictor's face buffer,
-  // which is dead once the corrector fluxes and EMFs exist
   Lims L1 = integr_lims(c, c->order);
-  launch_flux_dispatch(c, c->order, c->ws, c->bs, L1);
-  if (c->dims > 1) launch_emf(c, c->ws, L1, "emf_corr");
-  launch_update(c, false, L1);
+  if (c->dims == 1) {  // fluxes through LDS
+    launch_stage1<true>(c, 0, L0);
+    launch_stage1<false>(c, c->order, L1);
+  } else {
+    // predictor: donor cell, dt/2, lims of order 0; ws and bs are new buffers
+    launch_flux_dispatch(c, 0, c->w, c->b, L0);
+    launch_emf(c, c->w, L0, "emf_pred");
+    launch_update(c, true, L0);
+    // corrector: reconstruction order, dt; prim is updated in place (each
+    // cell reads only its own w_n), b_{n+1} goes to the predictor's face
+    // buffer, which is dead once the corrector fluxes and EMFs exist
+    launch_flux_dispatch(c, c->order, c->ws, c->bs, L1);
+    launch_emf(c, c->ws, L1, "emf_corr");
+    launch_update(c, false, L1);
+  }
   std::swap(c->b, c->bs);
   launch_ghost(c);
   hipLaunchKernelGGL(k_time_advance, dim3(1), dim3(64), 0, c->stream, c->dev);
@@ -746,8 +794,8 @@ int bnz_create(const bnz_params *p, bnz_ctx **out) {
       grid.n[q] = c->nt[q];
     }
     c->npart_cap = std::max({(long)blocks_for(a.total()) * std::max(NDIAG, NHIST) + 64,
-                             (long)blocks_for(c->ncell),
-                             (long)tiled_blocks(grid, 1, 2)});
+                             (long)blocks_for(c->ncell), (long)tiled_blocks(grid, 1, 2),
+                             (long)(c->nt[0] + BLK - 2) / (BLK - 1)});
   }
   HIPCHK(c, hipMalloc(&c->part, c->npart_cap * sizeof(double)));
 

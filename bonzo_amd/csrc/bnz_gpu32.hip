@@ -58,11 +58,11 @@ __global__ __launch_bounds__(BLK) void k32_emf(Geo g, const float *__restrict__ 
 template <int DIMS, bool UG = false>
 __global__ __launch_bounds__(BLK) void k32_update_pred(Geo g, Pred32 a, Lims L,
                                                        const double *__restrict__ dts,
-                                                       Box grid, int ty, Walls wl) {
+                                                       Box grid, int ty) {
   int i, j, k;
   if (dts[0] == 0.) return;  // a step past t_max (see k_update)
   if (!cell_index<false, 1, 2>(grid, ty, i, j, k)) return;
-  update32_pred<DIMS, typename std::conditional<UG, GeoUni, GeoArr>::type>(g, a, L, dts, wl, i, j,
+  update32_pred<DIMS, typename std::conditional<UG, GeoUni, GeoArr>::type>(g, a, L, dts, i, j,
                                                                           k);
 }
 
@@ -71,7 +71,7 @@ template <int DIMS, bool UG = false>
 __global__ __launch_bounds__(BLK) void k32_update_corr(Geo g, Corr32 a, Lims L,
                                                        const double *__restrict__ dts, Box act,
                                                        int fms_ref, double *__restrict__ part,
-                                                       Box grid, int ty, Walls wl) {
+                                                       Box grid, int ty) {
   int i, j, k;
   double cdsmax = 0.;
   const bool have = cell_index<false, 1, 2>(grid, ty, i, j, k);
@@ -84,8 +84,30 @@ __global__ __launch_bounds__(BLK) void k32_update_corr(Geo g, Corr32 a, Lims L,
   }
   if (have)
     cdsmax = update32_corr<DIMS, typename std::conditional<UG, GeoUni, GeoArr>::type>(
-        g, a, L, dts, act, fms_ref, wl, i, j, k);
+        g, a, L, dts, act, fms_ref, i, j, k);
   block_max_to(cdsmax, part);
+}
+
+// One FP32C stage of a 1D grid in one pass (stage32_cell): workgroup b owns
+// faces b * (BLK - 1) .. + BLK - 1 and the BLK - 1 cells between them; the
+// corrector leaves its workgroup's CFL maximum in part.
+template <bool PRED, int ORDER, int RS>
+__global__ __launch_bounds__(BLK) void k32_stage1(Geo g, Pred32 ap, Corr32 ac, Lims L,
+                                                  const double *__restrict__ dts, Box act,
+                                                  int fms_ref, double *__restrict__ part,
+                                                  Walls wl, int *nguard) {
+  __shared__ float sf[NMODE * BLK];
+  const int i = blockIdx.x * (BLK - 1) + threadIdx.x;
+  if (dts[0] == 0.) {  // a step past t_max: state unchanged, faces carried over
+    if (!PRED && threadIdx.x < BLK - 1 && i < g.nt[0])
+      for (int q = 0; q < 3; ++q) ac.b1[q * g.sv + i] = ac.b[q * g.sv + i];
+    return;
+  }
+  int ng = 0;
+  const double cds = stage32_cell<PRED, ORDER, RS>(g, ap.ws, ap.bs, ap, ac, L, dts, act,
+                                                   fms_ref, wl.at[0], wl.at[1], i, sf, ng);
+  if (!PRED) block_max_to(cds, part);
+  if (ng) atomicAdd(nguard, ng);
 }
 
 // CFL maximum from the FP32 primitives (first step after an upload).
@@ -174,8 +196,7 @@ void emf32(bnz_ctx *c, const float *w, const Lims &L, const char *name) {
 #define E32_ARGS \
   dim3(sh.blocks), dim3(BLK), 0, c->stream, c->g, w, c->fx32, c->fy32, c->fz32, c->ee32, e, wl, sh.ty
   if (c->dims == 3) hipLaunchKernelGGL(k32_emf<3>, E32_ARGS);
-  else if (c->dims == 2) hipLaunchKernelGGL(k32_emf<2>, E32_ARGS);
-  else hipLaunchKernelGGL(k32_emf<1>, E32_ARGS);
+  else hipLaunchKernelGGL(k32_emf<2>, E32_ARGS);  // 1D grids run k32_stage1
 #undef E32_ARGS
 }
 
@@ -249,7 +270,39 @@ void f32_dt(bnz_ctx *c, int use_fixed, double fixed_dt) {
   launch_dt_final(c, nb, use_fixed, fixed_dt);
 }
 
+template <bool PRED, int ORDER, int RS>
+void stage32_launch(bnz_ctx *c, const Lims &L) {
+  const int nb = (c->nt[0] + BLK - 2) / (BLK - 1);
+  const Pred32 ap{c->w32, c->uh, c->b, c->fx32, c->fy32, c->fz32, c->ee32, c->bs32, c->ws32};
+  const Corr32 ac{c->uh, c->ul, c->b, c->fx32, c->fy32, c->fz32, c->ee32, c->bs, c->w32};
+  Launch t(c, PRED ? "stage1_pred" : "stage1_corr");
+  hipLaunchKernelGGL((k32_stage1<PRED, ORDER, RS>), dim3(nb), dim3(BLK), 0, c->stream, c->g, ap,
+                     ac, L, c->dev, active_box(c), c->p.dt_fms_ref, c->part, walls32(c),
+                     (int *)(c->dev + 4));  // guarded-face counter, as launch_fluxes32
+  c->dt_parts = PRED ? 0 : nb;
+}
+
+template <bool PRED>
+void stage32(bnz_ctx *c, int order, const Lims &L) {
+  const bool d = c->p.rsolver == BNZ_RS_HLLD;
+  if (order == 0) {
+    if (d) stage32_launch<PRED, 0, BNZ_RS_HLLD>(c, L);
+    else stage32_launch<PRED, 0, BNZ_RS_HLL>(c, L);
+  } else {
+    if (d) stage32_launch<PRED, 1, BNZ_RS_HLLD>(c, L);
+    else stage32_launch<PRED, 1, BNZ_RS_HLL>(c, L);
+  }
+}
+
 void f32_step_body(bnz_ctx *c) {
+  if (c->dims == 1) {  // fluxes through LDS
+    stage32<true>(c, 0, integr_lims(c, 0));
+    stage32<false>(c, c->order, integr_lims(c, c->order));
+    std::swap(c->b, c->bs);
+    ghost32(c);
+    launch_time_advance(c);
+    return;
+  }
   Box grid;
   for (int q = 0; q < 3; ++q) {
     grid.lo[q] = 0;
@@ -258,32 +311,29 @@ void f32_step_body(bnz_ctx *c) {
   const Shape us = work_shape32(grid, c->dims, c->update_chunk);
   const int nb = us.blocks;
   Lims L0 = integr_lims(c, 0);
-  const Walls wl = walls32(c);
   fluxes32(c, 0, c->w32, c->b, L0);
-  if (c->dims > 1) emf32(c, c->w32, L0, "emf_pred");  // 1D: EMFs formed in the update
+  emf32(c, c->w32, L0, "emf_pred");
   {
     Pred32 a{c->w32, c->uh, c->b, c->fx32, c->fy32, c->fz32, c->ee32, c->bs32, c->ws32};
     Launch t(c, "update_pred");
-#define P32_ARGS dim3(nb), dim3(BLK), 0, c->stream, c->g, a, L0, c->dev, grid, us.ty, wl
+#define P32_ARGS dim3(nb), dim3(BLK), 0, c->stream, c->g, a, L0, c->dev, grid, us.ty
     if (c->dims == 3 && c->g.unif) hipLaunchKernelGGL((k32_update_pred<3, true>), P32_ARGS);
     else if (c->dims == 3) hipLaunchKernelGGL(k32_update_pred<3>, P32_ARGS);
-    else if (c->dims == 2) hipLaunchKernelGGL(k32_update_pred<2>, P32_ARGS);
-    else hipLaunchKernelGGL(k32_update_pred<1>, P32_ARGS);
+    else hipLaunchKernelGGL(k32_update_pred<2>, P32_ARGS);
 #undef P32_ARGS
   }
   Lims L1 = integr_lims(c, c->order);
   fluxes32(c, c->order, c->ws32, c->bs32, L1);
-  if (c->dims > 1) emf32(c, c->ws32, L1, "emf_corr");
+  emf32(c, c->ws32, L1, "emf_corr");
   {
     Corr32 a{c->uh, c->ul, c->b, c->fx32, c->fy32, c->fz32, c->ee32, c->bs, c->w32};
     Launch t(c, "update_corr");
 #define C32_ARGS \
   dim3(nb), dim3(BLK), 0, c->stream, c->g, a, L1, c->dev, active_box(c), c->p.dt_fms_ref, c->part, \
-      grid, us.ty, wl
+      grid, us.ty
     if (c->dims == 3 && c->g.unif) hipLaunchKernelGGL((k32_update_corr<3, true>), C32_ARGS);
     else if (c->dims == 3) hipLaunchKernelGGL(k32_update_corr<3>, C32_ARGS);
-    else if (c->dims == 2) hipLaunchKernelGGL(k32_update_corr<2>, C32_ARGS);
-    else hipLaunchKernelGGL(k32_update_corr<1>, C32_ARGS);
+    else hipLaunchKernelGGL(k32_update_corr<2>, C32_ARGS);
 #undef C32_ARGS
   }
   c->dt_parts = nb;

@@ -146,25 +146,15 @@ __device__ __forceinline__ void emf32_edge(const Geo &g, const float *__restrict
 
 // FP32 edge EMFs of the stage update: em(comp, off) is component comp at
 // the edge whose lower corner is cell c + off.  Emf32Global reads the ee
-// array of the EMF kernel.  On 1D grids Emf32x1D forms them from the x
-// fluxes as emf32_edge<1> does (ey = F_x[Bz], ez = -F_x[By], both zero on a
-// reflecting wall plane; a 1D cell reads no Ex), so 1D steps need no EMF
-// kernel.
+// array of the EMF kernel; on 1D grids EmfLds32x1D (below) forms them from
+// the x fluxes as emf32_edge<1> does (ey = F_x[Bz], ez = -F_x[By], both
+// zero on a reflecting wall plane; a 1D cell reads no Ex), so 1D steps need
+// no EMF kernel.
 struct Emf32Global {
   const float *ee;
   long sv, c;
   __device__ __forceinline__ float operator()(int comp, long off) const {
     return ee[comp * sv + c + off];
-  }
-};
-struct Emf32x1D {
-  const float *fx;
-  long sv, c;
-  int i, wlo, whi;  // the cell's x index and the x wall planes (-1: none)
-  __device__ __forceinline__ float operator()(int comp, long off) const {
-    const int e = i + (int)off;  // 1D: off is 0 or 1
-    if (comp == 0 || e == wlo || e == whi) return 0.f;
-    return comp == 2 ? -fx[BY * sv + c + off] : fx[BZ * sv + c + off];
   }
 };
 
@@ -254,22 +244,6 @@ struct Faces64 {
   __device__ double operator()(int a, long c) const { return b[a * sv + c]; }
 };
 
-// EMF source of a DIMS-dimensional update of cell (i, c): flux-derived in 1D.
-template <int DIMS>
-struct EmfSel32 {
-  static __device__ __forceinline__ Emf32Global get(const float *ee, const float *, long sv,
-                                                   long c, int, const Walls &) {
-    return Emf32Global{ee, sv, c};
-  }
-};
-template <>
-struct EmfSel32<1> {
-  static __device__ __forceinline__ Emf32x1D get(const float *, const float *fx, long sv, long c,
-                                                int i, const Walls &wl) {
-    return Emf32x1D{fx, sv, c, i, wl.at[0], wl.at[1]};
-  }
-};
-
 // FP32 face fluxes as the updates read them: variable n of the face whose
 // lower cell is c + off (the arrays of the flux kernels).
 struct FluxArr32 {
@@ -351,13 +325,15 @@ __device__ __forceinline__ void update32_pred_e(const Geo &g, const Pred32 &a, c
   }
 }
 
+// 2D and 3D updates read the flux and EMF arrays (1D grids run stage32_cell).
 template <int DIMS, class GM = GeoArr>
 __device__ __forceinline__ void update32_pred(const Geo &g, const Pred32 &a, const Lims &L,
-                                              const double *__restrict__ dts, const Walls &wl,
-                                              int i, int j, int k) {
+                                              const double *__restrict__ dts, int i, int j,
+                                              int k) {
+  static_assert(DIMS >= 2, "1D grids run stage32_cell");
   const long c = ((long)k * g.nt[1] + j) * g.nt[0] + i, sv = g.sv;
   update32_pred_e<DIMS, GM>(g, a, L, dts, FluxArr32{a.fx, a.fy, a.fz, sv, c},
-                            EmfSel32<DIMS>::get(a.ee, a.fx, sv, c, i, wl), i, j, k);
+                            Emf32Global{a.ee, sv, c}, i, j, k);
 }
 
 
@@ -446,14 +422,78 @@ __device__ __forceinline__ double update32_corr_e(const Geo &g, const Corr32 &a,
 template <int DIMS, class GM = GeoArr>
 __device__ __forceinline__ double update32_corr(const Geo &g, const Corr32 &a, const Lims &L,
                                                 const double *__restrict__ dts, const Box &act,
-                                                int fms_ref, const Walls &wl, int i, int j,
-                                                int k) {
+                                                int fms_ref, int i, int j, int k) {
+  static_assert(DIMS >= 2, "1D grids run stage32_cell");
   const long c = ((long)k * g.nt[1] + j) * g.nt[0] + i, sv = g.sv;
   return update32_corr_e<DIMS, GM>(g, a, L, dts, act, fms_ref,
-                                   FluxArr32{a.fx, a.fy, a.fz, sv, c},
-                                   EmfSel32<DIMS>::get(a.ee, a.fx, sv, c, i, wl), i, j, k);
+                                   FluxArr32{a.fx, a.fy, a.fz, sv, c}, Emf32Global{a.ee, sv, c},
+                                   i, j, k);
 }
 
+
+// Fluxes of a run of 1D faces in LDS (as the FP64 FluxLds1D) and the 1D
+// EMFs formed from them, zero on the reflecting wall planes wlo / whi.
+struct FluxLds32 {
+  const float *s;
+  int t;
+  __device__ __forceinline__ float x(int n, long off) const { return s[n * BLK + t + off]; }
+  __device__ __forceinline__ float y(int, long) const { return 0.f; }
+  __device__ __forceinline__ float z(int, long) const { return 0.f; }
+};
+struct EmfLds32x1D {
+  const float *s;
+  int t, i, wlo, whi;
+  __device__ __forceinline__ float operator()(int comp, long off) const {
+    const int e = i + (int)off;
+    if (comp == 0 || e == wlo || e == whi) return 0.f;
+    return comp == 2 ? -s[BY * BLK + t + off] : s[BZ * BLK + t + off];
+  }
+};
+
+// One FP32C stage of a 1D grid for thread threadIdx.x of a workgroup of BLK,
+// as the FP64 stage1_cell: the flux of face i into sf (NMODE x BLK floats),
+// a barrier, then the update of cell i (update32_pred_e / update32_corr_e on
+// the structs ap / ac) from faces i and i + 1.  The predictor's fluxes read
+// the FP32 prim and the FP64 faces of the step (ap.w32, ap.b); the
+// corrector's read the predictor's FP32 prim and faces (ws32, bs32), which
+// the corrector does not write.  wlo / whi are the x wall planes (-1: none).
+// Returns the CFL speed of the updated cell (corrector).
+template <bool PRED, int ORDER, int RS>
+__device__ __forceinline__ double stage32_cell(const Geo &g, const float *__restrict__ ws32,
+                                               const float *__restrict__ bs32, const Pred32 &ap,
+                                               const Corr32 &ac, const Lims &L,
+                                               const double *__restrict__ dts, const Box &act,
+                                               int fms_ref, int wlo, int whi, int i, float *sf,
+                                               int &ng) {
+  const int tid = threadIdx.x;
+  if (i >= L.lo[0] && i <= L.hi[0] + 1) {  // the stage's faces (flux3_box)
+    bool got = false;
+    const auto sink = [&](const float *f, long) {
+#pragma unroll
+      for (int n = 0; n < NMODE; ++n)
+        if (n != BX) sf[n * BLK + tid] = f[n];
+      got = true;
+    };
+    if (PRED)
+      flux32_face_to<0, ORDER, RS>(g, ap.w32, ap.b, i, 0, 0, ng, sink);
+    else
+      flux32_face_to<0, ORDER, RS>(g, ws32, bs32, i, 0, 0, ng, sink);
+    if (!got)  // no region matched: the flux row keeps what it held
+      for (int n = 0; n < NMODE; ++n)
+        if (n != BX) sf[n * BLK + tid] = ap.fx[n * g.sv + i];
+  }
+  __syncthreads();
+  double cds = 0.;
+  if (tid < BLK - 1 && i < g.nt[0]) {
+    const FluxLds32 fl{sf, tid};
+    const EmfLds32x1D em{sf, tid, i, wlo, whi};
+    if (PRED)
+      update32_pred_e<1, GeoArr>(g, ap, L, dts, fl, em, i, 0, 0);
+    else
+      cds = update32_corr_e<1, GeoArr>(g, ac, L, dts, act, fms_ref, fl, em, i, 0, 0);
+  }
+  return cds;
+}
 
 // CFL speed of one active cell from the FP32 primitives.
 __device__ __forceinline__ double cfl32_cell(const Geo &g, const float *__restrict__ w,

@@ -164,21 +164,14 @@ __device__ __forceinline__ void emf_edge(const Geo &g, const double *__restrict_
 //
 // The edge EMFs come from em(comp, off): component comp at the edge whose
 // lower corner is cell c + off.  EmfGlobal reads the ee array of the EMF
-// kernel; on 1D grids Emf1D forms them from the x fluxes as that kernel
-// does (ey = F_x[Bz], ez = -F_x[By]; a 1D cell reads no Ex), so 1D steps
-// need no EMF kernel.
+// kernel; on 1D grids EmfLds1D (below) forms them from the x fluxes as that
+// kernel does (ey = F_x[Bz], ez = -F_x[By]; a 1D cell reads no Ex), so 1D
+// steps need no EMF kernel.
 struct EmfGlobal {
   const double *ee;
   long sv, c;
   __device__ __forceinline__ double operator()(int comp, long off) const {
     return ee[comp * sv + c + off];
-  }
-};
-struct Emf1D {
-  const double *fx;
-  long sv, c;
-  __device__ __forceinline__ double operator()(int comp, long off) const {
-    return comp == 2 ? -fx[BY * sv + c + off] : comp == 1 ? fx[BZ * sv + c + off] : 0.;
   }
 };
 
@@ -190,6 +183,23 @@ struct FluxGlobal {
   __device__ __forceinline__ double x(int n, long off) const { return fx[n * sv + c + off]; }
   __device__ __forceinline__ double y(int n, long off) const { return fy[n * sv + c + off]; }
   __device__ __forceinline__ double z(int n, long off) const { return fz[n * sv + c + off]; }
+};
+
+// Fluxes of a run of 1D faces held in LDS, s[n * BLK + slot]: slot t + off
+// is the face whose lower cell is the updating thread's cell + off.
+struct FluxLds1D {
+  const double *s;
+  int t;
+  __device__ __forceinline__ double x(int n, long off) const { return s[n * BLK + t + off]; }
+  __device__ __forceinline__ double y(int, long) const { return 0.; }
+  __device__ __forceinline__ double z(int, long) const { return 0.; }
+};
+struct EmfLds1D {  // the 1D EMFs from the LDS fluxes
+  const double *s;
+  int t;
+  __device__ __forceinline__ double operator()(int comp, long off) const {
+    return comp == 2 ? -s[BY * BLK + t + off] : comp == 1 ? s[BZ * BLK + t + off] : 0.;
+  }
 };
 
 template <int DIMS, bool PRED, bool DTRED, class EMF, class GM, class FL>
@@ -342,33 +352,51 @@ __device__ __forceinline__ double update_cell_d(const Geo &g, const double *w0,
   const long c = ((long)k * g.nt[1] + j) * g.nt[0] + i;
   using GM = typename std::conditional<UG, GeoUni, GeoArr>::type;
   const GM gm{g};
+  static_assert(DIMS >= 2, "1D grids run stage1_cell");
   const FluxGlobal fl{fx, fy, fz, g.sv, c};
-  if (DIMS == 1)
-    return update_cell_e<1, PRED, DTRED>(g, gm, w0, fl, Emf1D{fx, g.sv, c}, b0, b1, w1, L, dts,
-                                         act, fms_ref, i, j, k);
   return update_cell_e<DIMS, PRED, DTRED>(g, gm, w0, fl, EmfGlobal{ee, g.sv, c}, b0, b1, w1, L,
                                           dts, act, fms_ref, i, j, k);
 }
 
-// Same with the dimensionality read from g (batched boxes of mixed dims).
-template <bool PRED, bool DTRED>
-__device__ __forceinline__ double update_cell(const Geo &g, const double *w0,
-                                              const double *__restrict__ fx,
-                                              const double *__restrict__ fy,
-                                              const double *__restrict__ fz,
-                                              const double *__restrict__ ee,
-                                              const double *__restrict__ b0,
-                                              double *__restrict__ b1, double *w1, const Lims &L,
-                                              const double *__restrict__ dts, const Box &act,
-                                              int fms_ref, int i, int j, int k) {
-  if (g.dims == 3)
-    return update_cell_d<3, PRED, DTRED>(g, w0, fx, fy, fz, ee, b0, b1, w1, L, dts, act, fms_ref,
-                                         i, j, k);
-  if (g.dims == 2)
-    return update_cell_d<2, PRED, DTRED>(g, w0, fx, fy, fz, ee, b0, b1, w1, L, dts, act, fms_ref,
-                                         i, j, k);
-  return update_cell_d<1, PRED, DTRED>(g, w0, fx, fy, fz, ee, b0, b1, w1, L, dts, act, fms_ref,
-                                       i, j, k);
+// One stage of a 1D grid for thread threadIdx.x of a workgroup of BLK that
+// owns faces i0 .. i0 + BLK - 1 and cells i0 .. i0 + BLK - 2 (i = i0 +
+// threadIdx.x): the flux of face i (flux_face_to on the stage's prim wf)
+// goes to sf (NMODE x BLK doubles of LDS), then after a barrier cell i is
+// updated from faces i and i + 1 (update_cell_e), so the fluxes never go
+// through HBM.  Every thread must call it.  The normal field of the faces
+// is read from the step's faces b0: in 1D the predictor copies x faces
+// unchanged, so they equal the predictor's, and the corrector's writes to
+// b1 cannot race these reads.  A face whose solver matches no region
+// (non-finite speeds) takes the old row of fx_old, which a fused pass never
+// writes.  Returns the CFL speed of the updated cell (corrector).
+template <bool PRED, int ORDER, int RS>
+__device__ __forceinline__ double stage1_cell(const Geo &g, const double *__restrict__ wf,
+                                              const double *b0, double *b1, const double *w0,
+                                              double *w1, const double *__restrict__ fx_old,
+                                              const Lims &L, const double *__restrict__ dts,
+                                              const Box &act, int fms_ref, int i, double *sf) {
+  const int tid = threadIdx.x;
+  if (i >= L.lo[0] && i <= L.hi[0] + 1) {  // the stage's faces (flux3_box)
+    bool got = false;
+    flux_face_to<0, ORDER, RS>(g, wf, b0, i, 0, 0, [&](const double *f, long) {
+#pragma unroll
+      for (int n = 0; n < NMODE; ++n)
+        if (n != BX) sf[n * BLK + tid] = f[n];
+      got = true;
+    });
+    if (!got)
+      for (int n = 0; n < NMODE; ++n)
+        if (n != BX) sf[n * BLK + tid] = fx_old[n * g.sv + i];
+  }
+  __syncthreads();
+  double cds = 0.;
+  if (tid < BLK - 1 && i < g.nt[0]) {
+    const double r = update_cell_e<1, PRED, !PRED>(g, GeoArr{g}, w0, FluxLds1D{sf, tid},
+                                                   EmfLds1D{sf, tid}, b0, b1, w1, L, dts, act,
+                                                   fms_ref, i, 0, 0);
+    if (!PRED) cds = r;
+  }
+  return cds;
 }
 
 }  // namespace bnz

@@ -402,9 +402,10 @@ def test_advance_chunks_and_idle_steps_change_nothing():
     calls at t_max give the uninterrupted run bit for bit, in both
     precisions."""
     from bonzo_amd._lib import Context
-    say("case: OT 32^2 and turbulence 16^3 to t=0.03: max_steps pieces, repeat at t_max, resume")
+    say("case: Brio-Wu 64, OT 32^2, turbulence 16^3 to t=0.03: max_steps pieces, repeat at "
+        "t_max, resume")
     for prec in ("fp64", "fp32c"):
-        for s, prim, bfld in (P.orszag_tang(32), P.turbulence(16)):
+        for s, prim, bfld in (P.brio_wu(64, transverse=0), P.orszag_tang(32), P.turbulence(16)):
             ref = Context(s.nx, s.ny, s.nz, precision=prec)
             ref.upload(prim, bfld)
             ref.init_state()
