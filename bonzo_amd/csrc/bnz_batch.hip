@@ -126,6 +126,82 @@ __global__ __launch_bounds__(BLK) void kb_stage1(const BoxDev *__restrict__ boxe
   }
 }
 
+// A whole VL step of a 1D box in one pass (all-1D FP64 batches): a work
+// block owns the C = BLK - 5 cells i0 .. i0 + C - 1.  Thread t computes the
+// predictor face and cell i0 - 2 + t (fluxes and prims in LDS; the 2 cells
+// below and 3 above i0's range are computed by the neighbouring blocks too),
+// then the corrector face i0 + t from the LDS prims and, for t < C, the
+// corrector cell i0 + t.  The arithmetic is kb_stage1's.  The pass reads the
+// step's prim w and faces b only and writes the new ones to ws and bs,
+// which the caller swaps with w and b, so no read races a write; the
+// predictor's faces are not kept (in 1D the corrector needs only the x
+// faces, which the predictor copies unchanged).
+template <int ORDER, int RS>
+__global__ __launch_bounds__(BLK) void kb_step1(const BoxDev *__restrict__ boxes, WList wl,
+                                                BoxState st, Pools p, int fms_ref) {
+  constexpr int C = BLK - 5;
+  __shared__ double sf[NMODE * BLK];  // face fluxes: predictor, then corrector
+  __shared__ double sw[NMODE * BLK];  // predictor prim of cell i0 - 2 + t
+  const int t = threadIdx.x;
+  for (int wb = wl.lo + blockIdx.x; wb < wl.hi; wb += gridDim.x) {
+    const int b = wl.bmap[wb];
+    if (!st.run[b]) continue;
+    const BoxDev &d = boxes[b];
+    const Geo &g = d.g;
+    const long o = d.off, sv = g.sv;
+    const Lims &L0 = d.L[0], &L1 = d.L[1];
+    const int i0 = (wb - d.blk0[wl.kt]) * C;
+    const double *w = p.w + o, *bf = p.b + o;
+    const auto sink = [&](const double *f, long) {
+#pragma unroll
+      for (int n = 0; n < NMODE; ++n)
+        if (n != BX) sf[n * BLK + t] = f[n];
+    };
+    // predictor (donor cell, dt/2)
+    const int ip = i0 - 2 + t;
+    if (ip >= L0.lo[0] && ip <= L0.hi[0] + 1) {
+      bool got = false;
+      flux_face_to<0, 0, RS>(g, w, bf, ip, 0, 0, [&](const double *f, long q) {
+        sink(f, q);
+        got = true;
+      });
+      if (!got)  // no region matched: the flux row keeps what it held (flux_face)
+        for (int n = 0; n < NMODE; ++n)
+          if (n != BX) sf[n * BLK + t] = p.fx[n * sv + o + ip];
+    }
+    __syncthreads();
+    if (t < BLK - 1 && ip >= 0 && ip < g.nt[0])
+      update_cell_e<1, true, false, EmfLds1D, GeoArr, FluxLds1D, true>(
+          g, GeoArr{g}, w, FluxLds1D{sf, t}, EmfLds1D{sf, t}, bf, nullptr, sw, L0, st.dt + b,
+          d.act, fms_ref, ip, 0, 0, t);
+    __syncthreads();
+    // corrector (reconstruction order, dt): fluxes from the LDS prims, read
+    // through a geometry whose variable stride is the LDS row (BLK) and
+    // whose cell ip sits at slot ip - (i0 - 2)
+    const int ic = i0 + t;
+    if (t <= C && ic >= L1.lo[0] && ic <= L1.hi[0] + 1) {
+      Geo gl = g;
+      gl.sv = BLK;
+      bool got = false;
+      flux_face_to<0, ORDER, RS>(gl, sw - (i0 - 2), bf, ic, 0, 0,
+                                 [&](const double *f, long q) {
+                                   sink(f, q);
+                                   got = true;
+                                 });
+      if (!got)
+        for (int n = 0; n < NMODE; ++n)
+          if (n != BX) sf[n * BLK + t] = p.fx[n * sv + o + ic];
+    }
+    __syncthreads();
+    double cds = 0.;
+    if (t < C && ic < g.nt[0])
+      cds = update_cell_e<1, false, true>(g, GeoArr{g}, w, FluxLds1D{sf, t}, EmfLds1D{sf, t}, bf,
+                                          p.bs + o, p.ws + o, L1, st.dt + b, d.act, fms_ref, ic,
+                                          0, 0);
+    block_max_atomic(cds, st.cmax + b);  // ends with a barrier: sf, sw are free again
+  }
+}
+
 // CFL maximum of the current prim over the active cells (after init).
 __global__ __launch_bounds__(BLK) void kb_cfl(const BoxDev *__restrict__ boxes, WList wl,
                                               BoxState st, const double *__restrict__ w,
@@ -251,12 +327,15 @@ __global__ __launch_bounds__(BLK) void kb_ghost_ops(long sv, double *w, double *
   }
 }
 
-// Faces of the boxes that sat out an odd number of steps, from the other
-// buffer into the current one (all cells of the box), at the end of an
-// advance; kb_fpar_clear then resets the parities.
+// Faces (and, with fused 1D steps, prims) of the boxes that sat out an odd
+// number of steps, from the other buffer into the current one (all cells of
+// the box), at the end of an advance; kb_fpar_clear then resets the
+// parities.
 __global__ __launch_bounds__(BLK) void kb_face_fix(const BoxDev *__restrict__ boxes, WList wl,
                                                    BoxState st, double *__restrict__ b,
-                                                   const double *__restrict__ other) {
+                                                   const double *__restrict__ other,
+                                                   double *__restrict__ w,
+                                                   const double *__restrict__ wother) {
   for (int wb = wl.lo + blockIdx.x; wb < wl.hi; wb += gridDim.x) {
     const int bx = wl.bmap[wb];
     if (!st.fpar[bx]) continue;
@@ -266,6 +345,8 @@ __global__ __launch_bounds__(BLK) void kb_face_fix(const BoxDev *__restrict__ bo
     const long sv = d.g.sv, c = d.off + t;
 #pragma unroll
     for (int n = 0; n < 3; ++n) b[n * sv + c] = other[n * sv + c];
+    if (w)  // fused 1D steps: the prim buffers swap as well
+      for (int n = 0; n < NMODE; ++n) w[n * sv + c] = wother[n * sv + c];
   }
 }
 
@@ -452,9 +533,32 @@ void b_ghost(bnz_batch *c, bool with_time) {
 // sequence as the single grid's step.
 // One VL step of every running box; the box dt's were set by the previous
 // step's kb_time_dt (or by init_state's kb_dt).
+template <int ORDER, int RS>
+void b_step1(bnz_batch *c) {
+  const WList wl = wlist(c, KT1);
+  hipLaunchKernelGGL((kb_step1<ORDER, RS>), dim3(grid_for(wl.hi - wl.lo, c->cap[TU])), dim3(BLK),
+                     0, c->stream, c->boxes, wl, c->S, c->P, c->p.dt_fms_ref);
+}
+
 void b_step(bnz_batch *c) {
   if (c->f32) {
     b32_step(c);  // ends with the ghost fill and the time kernel
+    return;
+  }
+  if (c->step1) {  // every box 1D: one fused pass per step
+    const int rs = c->p.rsolver;
+    if (c->order == 0) {
+      if (rs == BNZ_RS_HLLD) b_step1<0, BNZ_RS_HLLD>(c);
+      else if (rs == BNZ_RS_HLLT) b_step1<0, BNZ_RS_HLLT>(c);
+      else b_step1<0, BNZ_RS_HLL>(c);
+    } else {
+      if (rs == BNZ_RS_HLLD) b_step1<1, BNZ_RS_HLLD>(c);
+      else if (rs == BNZ_RS_HLLT) b_step1<1, BNZ_RS_HLLT>(c);
+      else b_step1<1, BNZ_RS_HLL>(c);
+    }
+    std::swap(c->P.b, c->P.bs);
+    std::swap(c->P.w, c->P.ws);
+    b_ghost(c, true);
     return;
   }
   // predictor: donor cell, dt/2, limits of order 0
@@ -579,6 +683,7 @@ int bnz_batch_create(const bnz_params *p, const bnz_box *boxes, int nbox, bnz_ba
     }
     d.items[KUPD] = mkbox(0, nt[0], 0, nt[1], 0, nt[2]);
     d.items[KS1] = dims == 1 ? mkbox(0, nt[0], 0, 1, 0, 1) : mkbox(0, 0, 0, 0, 0, 0);
+    d.items[KT1] = d.items[KS1];
     d.act = mkbox(a1[0], s.nx, a1[1], dims >= 2 ? s.ny : 1, a1[2], dims >= 3 ? s.nz : 1);
     d.items[KACT] = d.act;
     for (int a = 0; a < 3; ++a) {  // reflecting wall planes (FP32C EMFs)
@@ -591,6 +696,8 @@ int bnz_batch_create(const bnz_params *p, const bnz_box *boxes, int nbox, bnz_ba
       return bfail(c, BNZ_EINVAL, "batch too large: pool cell indices are 32-bit");
   }
   c->ncell = off;
+  c->step1 = !c->f32;
+  for (int b = 0; b < nbox; ++b) c->step1 = c->step1 && c->hbox[b].g.dims == 1;
   c->nghost = (long)pairs.size();
   c->ngops = (long)gops.size();
   // work blocks: box b owns ceil(items / BLK) consecutive blocks of each
@@ -845,7 +952,8 @@ int bnz_batch_advance(bnz_batch *c, int max_steps, int use_graph, int *nsteps) {
   }
   // faces of the boxes that finished an odd number of steps ago
   hipLaunchKernelGGL(kb_face_fix, dim3(grid_for(c->nblk[KUPD])), dim3(BLK), 0, c->stream,
-                     c->boxes, wlist(c, KUPD), c->S, c->P.b, c->P.bs);
+                     c->boxes, wlist(c, KUPD), c->S, c->P.b, c->P.bs,
+                     c->step1 ? c->P.w : nullptr, c->P.ws);
   hipLaunchKernelGGL(kb_fpar_clear, dim3(1), dim3(BLK), 0, c->stream, c->S.fpar, c->nbox);
   rc = blaunch_check(c);
   if (rc) return rc;

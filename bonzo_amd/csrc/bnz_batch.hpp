@@ -12,12 +12,15 @@ using namespace bnz;
 
 // work lists: fluxes (all directions, one item per cell of the union of the
 // face ranges) and EMFs of the predictor and of the corrector, all cells
-// (stage updates), active cells (init and the first CFL maximum), and the
-// cells of the 1D boxes for the fused 1D stage, BLK - 1 per work block
-enum { KF0 = 0, KEM0, KF1, KEM1, KUPD, KACT, KS1, NK };
+// (stage updates), active cells (init and the first CFL maximum), the cells
+// of the 1D boxes for the fused 1D stage (BLK - 1 per work block) and for
+// the fused 1D step of all-1D FP64 batches (BLK - 5 per work block)
+enum { KF0 = 0, KEM0, KF1, KEM1, KUPD, KACT, KS1, KT1, NK };
 
 // cells per work block of list kt
-__host__ __device__ constexpr int blk_items(int kt) { return kt == KS1 ? BLK - 1 : BLK; }
+__host__ __device__ constexpr int blk_items(int kt) {
+  return kt == KS1 ? BLK - 1 : kt == KT1 ? BLK - 5 : BLK;
+}
 
 struct BoxDev {
   Geo g;         // strides and geometry of the box; g.sv is the pool stride
@@ -147,6 +150,7 @@ struct bnz_batch {
   bnz::GhostOp *gops = nullptr;
   bnzb::Pools P{};
   bool f32 = false;
+  bool step1 = false;  // all boxes 1D, FP64: fused steps, prim buffers swap like the faces
   bnzb::Pools32 Q{};
   int *nguard = nullptr;  // guarded FP32 HLLD faces
   bool w_stale = false;   // FP32C: the FP64 prim pool lags the state

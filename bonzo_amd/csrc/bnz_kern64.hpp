@@ -202,13 +202,16 @@ struct EmfLds1D {  // the 1D EMFs from the LDS fluxes
   }
 };
 
-template <int DIMS, bool PRED, bool DTRED, class EMF, class GM, class FL>
+// LDS_OUT (predictor of a fused 1D step): the new prim goes to the LDS
+// array w1 as w1[n * BLK + lslot] and no face is stored (b1 unused).
+template <int DIMS, bool PRED, bool DTRED, class EMF, class GM, class FL, bool LDS_OUT = false>
 __device__ __forceinline__ double update_cell_e(const Geo &g, const GM &gm, const double *w0,
                                                 const FL &fl, const EMF &em,
                                                 const double *__restrict__ b0,
                                                 double *__restrict__ b1, double *w1,
                                                 const Lims &L, const double *__restrict__ dts,
-                                                const Box &act, int fms_ref, int i, int j, int k) {
+                                                const Box &act, int fms_ref, int i, int j, int k,
+                                                int lslot = 0) {
   const long c = ((long)k * g.nt[1] + j) * g.nt[0] + i;
   const long sv = g.sv;
   const double dt = PRED ? 0.5 * dts[0] : dts[0];
@@ -323,6 +326,16 @@ __device__ __forceinline__ double update_cell_e(const Geo &g, const GM &gm, cons
   cons2prim(wn, u, g.gam);
 
   // ---- stores
+  if (LDS_OUT) {
+    if (cell) {
+#pragma unroll
+      for (int n = 0; n < NMODE; ++n) w1[n * BLK + lslot] = wn[n];
+    } else if (PRED) {
+#pragma unroll
+      for (int n = 0; n < NMODE; ++n) w1[n * BLK + lslot] = wv[n];
+    }
+    return 0.;
+  }
   if (fX) b1[c] = bxl;
   if (fY) b1[sv + c] = byl;
   if (fZ) b1[2 * sv + c] = bzl;

@@ -133,19 +133,26 @@ def test_batch_max_steps_and_resume():
     b.close()
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32c"])
-def test_batch_finished_boxes_across_pieces(precision):
+@pytest.mark.parametrize("precision,mix", [("fp64", "2d"), ("fp32c", "2d"), ("fp64", "1d")])
+def test_batch_finished_boxes_across_pieces(precision, mix):
     """Boxes that reach t_max long before the others sit out many steps (the
-    face buffers swap without them); advancing in pieces of odd and even
-    length, graph and eager, and reading a finished box in between, must
-    give each box its uninterrupted state."""
-    say(f"case: 2D + 1D boxes finishing at different steps, pieces, {precision}")
+    face buffers, and in an all-1D FP64 batch the prim buffers, swap without
+    them); advancing in pieces of odd and even length, graph and eager, and
+    reading a finished box in between, must give each box its uninterrupted
+    state."""
+    say(f"case: boxes finishing at different steps, pieces, {precision}, {mix}")
     from bonzo_amd._lib import Batch
     rng = np.random.Generator(np.random.PCG64(1234))
-    cases = [(P.riemann_tube(64, rng), (1,) * 6, 0.01),
-             (P.orszag_tang(32), (0,) * 6, 0.005),
-             (P.riemann_tube(300, rng), (1,) * 6, 0.06),
-             (P.orszag_tang(48), (0,) * 6, 0.03)]
+    if mix == "2d":
+        cases = [(P.riemann_tube(64, rng), (1,) * 6, 0.01),
+                 (P.orszag_tang(32), (0,) * 6, 0.005),
+                 (P.riemann_tube(300, rng), (1,) * 6, 0.06),
+                 (P.orszag_tang(48), (0,) * 6, 0.03)]
+    else:
+        cases = [(P.riemann_tube(64, rng), (1,) * 6, 0.01),
+                 (P.riemann_tube(700, rng), (0,) * 6, 0.03),
+                 (P.riemann_tube(300, rng), (1,) * 6, 0.06),
+                 (P.riemann_tube(96, rng), (1,) * 6, 0.002)]
     ref, tref, sref, _ = run_batch(cases, use_graph=False, precision=precision)
     say(f"  reference steps {list(sref)}")
     b = Batch([box_desc(c[0][0], c[1], c[2]) for c in cases], precision=precision)
@@ -250,6 +257,26 @@ def test_batch_fp32c_bitwise_vs_single_fp32c(use_graph):
               (_wall_state(24, 20, 0), (2, 2, 0, 0, 0, 0), 0.03),
               (P.turbulence(12, seed_b=15, seed_v=16), (0,) * 6, 0.02)]
     check_against_single(cases, use_graph=use_graph, precision="fp32c")
+
+
+@pytest.mark.parametrize("rsolver,reconstr", [("hlld", "linear"), ("hllt", "linear"),
+                                               ("hll", "const"), ("hlld", "const")])
+def test_batch_all_1d_fused_steps_bitwise_vs_single(rsolver, reconstr):
+    """An FP64 batch of 1D boxes only runs each step as one fused pass
+    (predictor and corrector in LDS, prim buffers swapped like the faces):
+    every box equals its single-grid run, with outflow, periodic and walled
+    tubes of lengths that do and do not fill the last work block."""
+    say(f"case: all-1D FP64 batch, fused steps, {rsolver} + {reconstr}")
+    ng = 2 if reconstr == "const" else 3
+    rng = np.random.Generator(np.random.PCG64(4008))
+    walled = P.riemann_tube(251, rng, ng=ng)
+    walled[2][0][:] = 0.0
+    cases = [(P.riemann_tube(n, rng, ng=ng), bc, tm)
+             for n, bc, tm in ((64, (1,) * 6, 0.05), (502, (0,) * 6, 0.02),
+                               (245, (1,) * 6, 0.04), (1000, (1,) * 6, 0.01))]
+    cases.append((walled, (2, 2, 0, 0, 0, 0), 0.03))
+    check_against_single(cases, rsolver=rsolver, reconstr=reconstr)
+    check_against_single(cases, rsolver=rsolver, reconstr=reconstr, use_graph=False)
 
 
 def test_batch_hllt_bitwise_vs_single():
