@@ -696,7 +696,7 @@ int bnz_batch_create(const bnz_params *p, const bnz_box *boxes, int nbox, bnz_ba
       return bfail(c, BNZ_EINVAL, "batch too large: pool cell indices are 32-bit");
   }
   c->ncell = off;
-  c->step1 = !c->f32;
+  c->step1 = true;
   for (int b = 0; b < nbox; ++b) c->step1 = c->step1 && c->hbox[b].g.dims == 1;
   c->nghost = (long)pairs.size();
   c->ngops = (long)gops.size();
@@ -753,6 +753,12 @@ int bnz_batch_create(const bnz_params *p, const bnz_box *boxes, int nbox, bnz_ba
         return bfail(c, BNZ_ENOMEM, "device allocation of the FP32 box pools failed");
       BCHK(c, hipMemsetAsync(*q.q, 0, q.rows * cells * sizeof(float), c->stream));
     }
+    if (c->step1)  // fused 1D steps write the new hi and lo words to a second buffer
+      for (float **q : {&c->Q.uh2, &c->Q.ul2}) {
+        if (hipMalloc(q, 6 * cells * sizeof(float)) != hipSuccess)
+          return bfail(c, BNZ_ENOMEM, "device allocation of the FP32 box pools failed");
+        BCHK(c, hipMemsetAsync(*q, 0, 6 * cells * sizeof(float), c->stream));
+      }
     BCHK(c, hipMalloc(&c->nguard, sizeof(int)));
     BCHK(c, hipMemsetAsync(c->nguard, 0, sizeof(int), c->stream));
   }
@@ -806,8 +812,8 @@ int bnz_batch_destroy(bnz_batch *c) {
                     c->P.fy, c->P.fz, c->P.ee, c->geo,  c->sbuf};
   for (double *q : bufs)
     if (q) (void)hipFree(q);
-  float *b32[] = {c->Q.uh, c->Q.ul, c->Q.w32, c->Q.ws32, c->Q.bs32,
-                  c->Q.fx, c->Q.fy, c->Q.fz,  c->Q.ee};
+  float *b32[] = {c->Q.uh, c->Q.ul, c->Q.w32, c->Q.ws32, c->Q.bs32, c->Q.fx,
+                  c->Q.fy, c->Q.fz, c->Q.ee,  c->Q.uh2,  c->Q.ul2};
   for (float *q : b32)
     if (q) (void)hipFree(q);
   if (c->nguard) (void)hipFree(c->nguard);
@@ -953,7 +959,8 @@ int bnz_batch_advance(bnz_batch *c, int max_steps, int use_graph, int *nsteps) {
   // faces of the boxes that finished an odd number of steps ago
   hipLaunchKernelGGL(kb_face_fix, dim3(grid_for(c->nblk[KUPD])), dim3(BLK), 0, c->stream,
                      c->boxes, wlist(c, KUPD), c->S, c->P.b, c->P.bs,
-                     c->step1 ? c->P.w : nullptr, c->P.ws);
+                     c->step1 && !c->f32 ? c->P.w : nullptr, c->P.ws);
+  if (c->step1 && c->f32) b32_fix(c);
   hipLaunchKernelGGL(kb_fpar_clear, dim3(1), dim3(BLK), 0, c->stream, c->S.fpar, c->nbox);
   rc = blaunch_check(c);
   if (rc) return rc;

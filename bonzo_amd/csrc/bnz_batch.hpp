@@ -39,6 +39,7 @@ struct Pools {
 // FP32-compensated pools (face fields stay in Pools::b / bs, FP64)
 struct Pools32 {
   float *uh, *ul, *w32, *ws32, *bs32, *fx, *fy, *fz, *ee;
+  float *uh2, *ul2;  // the other hi / lo words of fused 1D steps (all-1D batches only)
 };
 
 struct BoxState {
@@ -150,7 +151,7 @@ struct bnz_batch {
   bnz::GhostOp *gops = nullptr;
   bnzb::Pools P{};
   bool f32 = false;
-  bool step1 = false;  // all boxes 1D, FP64: fused steps, prim buffers swap like the faces
+  bool step1 = false;  // all boxes 1D: fused steps, state buffers swap like the faces
   bnzb::Pools32 Q{};
   int *nguard = nullptr;  // guarded FP32 HLLD faces
   bool w_stale = false;   // FP32C: the FP64 prim pool lags the state
@@ -181,6 +182,7 @@ void b32_step(bnz_batch *c);
 // the end-of-step time and dt kernel alone (bnz_batch.hip)
 void b_time_dt(bnz_batch *c);
 void b32_init(bnz_batch *c);   // FP64 prim pool -> FP32C state; first CFL maxima
+void b32_fix(bnz_batch *c);    // FP32C state of finished boxes into the current buffers
 void b32_to64(bnz_batch *c);   // FP32C state -> FP64 prim pool (active cells)
 
 }  // namespace bnzb

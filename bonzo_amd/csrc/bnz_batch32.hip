@@ -129,6 +129,103 @@ __global__ __launch_bounds__(BLK) void kb32_stage1(const BoxDev *__restrict__ bo
   if (ng) atomicAdd(nguard, ng);
 }
 
+// A whole FP32C VL step of a 1D box in one pass (all-1D batches), as the
+// FP64 kb_step1: predictor fluxes and prims, then corrector fluxes, in LDS;
+// the corrector reads the step's hi / lo words, FP32 prim and faces and
+// writes the new ones to the other buffers (uh2, ul2, ws32, P.bs), which
+// the caller swaps in.  The corrector's fluxes take the normal field from
+// the FP64 faces rounded to FP32, the value the predictor's FP32 faces hold.
+template <int ORDER, int RS>
+__global__ __launch_bounds__(BLK) void kb32_step1(const BoxDev *__restrict__ boxes, WList wl,
+                                                  BoxState st, Pools P, Pools32 Q, int fms_ref,
+                                                  int *nguard) {
+  constexpr int C = BLK - 5;
+  __shared__ float sf[NMODE * BLK];  // face fluxes: predictor, then corrector
+  __shared__ float sw[NMODE * BLK];  // predictor FP32 prim of cell i0 - 2 + t
+  const int t = threadIdx.x;
+  int ng = 0;
+  for (int wb = wl.lo + blockIdx.x; wb < wl.hi; wb += gridDim.x) {
+    const int b = wl.bmap[wb];
+    if (!st.run[b]) continue;
+    const BoxDev &d = boxes[b];
+    const Geo &g = d.g;
+    const long o = d.off, sv = g.sv;
+    const Lims &L0 = d.L[0], &L1 = d.L[1];
+    const int i0 = (wb - d.blk0[wl.kt]) * C;
+    const double *bf = P.b + o;
+    const auto sink = [&](const float *f, long) {
+#pragma unroll
+      for (int n = 0; n < NMODE; ++n)
+        if (n != BX) sf[n * BLK + t] = f[n];
+    };
+    const int ip = i0 - 2 + t;
+    if (ip >= L0.lo[0] && ip <= L0.hi[0] + 1) {
+      bool got = false;
+      flux32_face_to<0, 0, RS>(g, Q.w32 + o, bf, ip, 0, 0, ng, [&](const float *f, long q) {
+        sink(f, q);
+        got = true;
+      });
+      if (!got)  // no region matched: the flux row keeps what it held
+        for (int n = 0; n < NMODE; ++n)
+          if (n != BX) sf[n * BLK + t] = Q.fx[n * sv + o + ip];
+    }
+    __syncthreads();
+    if (t < BLK - 1 && ip >= 0 && ip < g.nt[0]) {
+      const Pred32 ap{Q.w32 + o, Q.uh + o, bf, Q.fx + o, Q.fy + o, Q.fz + o, Q.ee + o,
+                      nullptr, sw};
+      update32_pred_e<1, GeoArr, FluxLds32, EmfLds32x1D, true>(
+          g, ap, L0, st.dt + b, FluxLds32{sf, t}, EmfLds32x1D{sf, t, ip, d.walls[0], d.walls[1]},
+          ip, 0, 0, t);
+    }
+    __syncthreads();
+    const int ic = i0 + t;
+    if (t <= C && ic >= L1.lo[0] && ic <= L1.hi[0] + 1) {
+      Geo gl = g;  // the LDS prims: variable stride BLK, cell ip at slot ip - (i0 - 2)
+      gl.sv = BLK;
+      bool got = false;
+      flux32_face_to<0, ORDER, RS>(gl, sw - (i0 - 2), bf, ic, 0, 0, ng,
+                                   [&](const float *f, long q) {
+                                     sink(f, q);
+                                     got = true;
+                                   });
+      if (!got)
+        for (int n = 0; n < NMODE; ++n)
+          if (n != BX) sf[n * BLK + t] = Q.fx[n * sv + o + ic];
+    }
+    __syncthreads();
+    double cds = 0.;
+    if (t < C && ic < g.nt[0]) {
+      const Corr32 ac{Q.uh + o, Q.ul + o, bf, Q.fx + o, Q.fy + o, Q.fz + o, Q.ee + o, P.bs + o,
+                      Q.ws32 + o};
+      cds = update32_corr_e<1, GeoArr, FluxLds32, EmfLds32x1D, true>(
+          g, ac, L1, st.dt + b, d.act, fms_ref, FluxLds32{sf, t},
+          EmfLds32x1D{sf, t, ic, d.walls[0], d.walls[1]}, ic, 0, 0, Q.uh2 + o, Q.ul2 + o);
+    }
+    block_max_atomic(cds, st.cmax + b);  // ends with a barrier: sf, sw are free again
+  }
+  if (ng) atomicAdd(nguard, ng);
+}
+
+// FP32C state (hi, lo, FP32 prim) of the boxes that sat out an odd number
+// of fused steps, from the other buffers into the current ones (the faces:
+// kb_face_fix).
+__global__ __launch_bounds__(BLK) void kb32_fix(const BoxDev *__restrict__ boxes, WList wl,
+                                                BoxState st, Pools32 Q) {
+  for (int wb = wl.lo + blockIdx.x; wb < wl.hi; wb += gridDim.x) {
+    const int bx = wl.bmap[wb];
+    if (!st.fpar[bx]) continue;
+    const BoxDev &d = boxes[bx];
+    const long t = (long)(wb - d.blk0[wl.kt]) * BLK + threadIdx.x;
+    if (t >= d.items[wl.kt].total()) continue;
+    const long sv = d.g.sv, c = d.off + t;
+    for (int n = 0; n < NMODE - 3; ++n) {
+      Q.uh[n * sv + c] = Q.uh2[n * sv + c];
+      Q.ul[n * sv + c] = Q.ul2[n * sv + c];
+    }
+    for (int n = 0; n < NMODE; ++n) Q.w32[n * sv + c] = Q.ws32[n * sv + c];
+  }
+}
+
 // FP64 prim -> FP32C state over every cell of every box (init).
 __global__ __launch_bounds__(BLK) void kb32_from64(const BoxDev *__restrict__ boxes, WList wl,
                                                    Pools P, Pools32 Q) {
@@ -298,13 +395,35 @@ namespace bnzb {
 // The FP32C stages of one VL step of every running box, between the dt
 // and time kernels of the caller (the sequence of the single-grid
 // f32_step_body).
+template <int ORDER, int RS>
+void step1_launch(bnz_batch *c) {
+  const WList wl = wlist(c, KT1);
+  hipLaunchKernelGGL((kb32_step1<ORDER, RS>), dim3(grid_for(wl.hi - wl.lo, c->cap[TU])),
+                     dim3(BLK), 0, c->stream, c->boxes, wl, c->S, c->P, c->Q, c->p.dt_fms_ref,
+                     c->nguard);
+}
+
 void b32_step(bnz_batch *c) {
-  fluxes(c, 0, c->Q.w32, (const double *)c->P.b, KF0);
-  emfs(c, c->Q.w32, KEM0);
-  updates<true>(c);
-  fluxes(c, c->order, c->Q.ws32, (const float *)c->Q.bs32, KF1);
-  emfs(c, c->Q.ws32, KEM1);
-  updates<false>(c);
+  if (c->step1) {  // every box 1D: one fused pass per step
+    const bool d = c->p.rsolver == BNZ_RS_HLLD;
+    if (c->order == 0) {
+      if (d) step1_launch<0, BNZ_RS_HLLD>(c);
+      else step1_launch<0, BNZ_RS_HLL>(c);
+    } else {
+      if (d) step1_launch<1, BNZ_RS_HLLD>(c);
+      else step1_launch<1, BNZ_RS_HLL>(c);
+    }
+    std::swap(c->Q.uh, c->Q.uh2);
+    std::swap(c->Q.ul, c->Q.ul2);
+    std::swap(c->Q.w32, c->Q.ws32);
+  } else {
+    fluxes(c, 0, c->Q.w32, (const double *)c->P.b, KF0);
+    emfs(c, c->Q.w32, KEM0);
+    updates<true>(c);
+    fluxes(c, c->order, c->Q.ws32, (const float *)c->Q.bs32, KF1);
+    emfs(c, c->Q.ws32, KEM1);
+    updates<false>(c);
+  }
   std::swap(c->P.b, c->P.bs);
   const Ghost32 p = ghost_arrays(c);
   // the end-of-step time and dt kernel rides in the last ghost launch
@@ -324,6 +443,11 @@ void b32_init(bnz_batch *c) {
                      c->boxes, wlist(c, KUPD), c->P, c->Q);
   hipLaunchKernelGGL(kb32_cfl, dim3(grid_for(c->nblk[KACT])), dim3(BLK), 0, c->stream, c->boxes,
                      wlist(c, KACT), c->S, c->Q.w32, c->p.dt_fms_ref);
+}
+
+void b32_fix(bnz_batch *c) {
+  hipLaunchKernelGGL(kb32_fix, dim3(grid_for(c->nblk[KUPD])), dim3(BLK), 0, c->stream, c->boxes,
+                     wlist(c, KUPD), c->S, c->Q);
 }
 
 void b32_to64(bnz_batch *c) {

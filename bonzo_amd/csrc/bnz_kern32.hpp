@@ -265,11 +265,14 @@ struct Pred32 {
 // Predictor (dt/2): FP32 update of the hi words, FP64 pressure recovery.
 // All operands are loaded up front; only the stores (and the copy of w into
 // ws outside the limits) depend on the roles.
-// Fluxes come from fl (x, y, z accessors), EMFs from em.
-template <int DIMS, class GM, class FL, class EM>
+// Fluxes come from fl (x, y, z accessors), EMFs from em.  LDS_OUT (the
+// predictor of a fused 1D step): the new prim goes to the LDS array a.ws as
+// a.ws[n * BLK + lslot] and no face is stored.
+template <int DIMS, class GM, class FL, class EM, bool LDS_OUT = false>
 __device__ __forceinline__ void update32_pred_e(const Geo &g, const Pred32 &a, const Lims &L,
                                                 const double *__restrict__ dts, const FL &fl,
-                                                const EM &em, int i, int j, int k) {
+                                                const EM &em, int i, int j, int k,
+                                                int lslot = 0) {
   const long c = ((long)k * g.nt[1] + j) * g.nt[0] + i, sv = g.sv;
   const double dt = 0.5 * dts[0];
   const Roles<DIMS> ro(g, L, i, j, k);
@@ -313,6 +316,16 @@ __device__ __forceinline__ void update32_pred_e(const Geo &g, const Pred32 &a, c
   else
     u[BZ] = lo[2];
   cons2prim(wv, u, g.gam);
+  if (LDS_OUT) {
+    if (ro.cell) {
+#pragma unroll
+      for (int n = 0; n < NMODE; ++n) a.ws[n * BLK + lslot] = (float)wv[n];
+    } else {
+#pragma unroll
+      for (int n = 0; n < NMODE; ++n) a.ws[n * BLK + lslot] = a.w32[n * sv + c];
+    }
+    return;
+  }
   if (ro.fX) a.bs[c] = lo[0];
   if (ro.fY) a.bs[sv + c] = lo[1];
   if (ro.fZ) a.bs[2 * sv + c] = lo[2];
@@ -349,11 +362,14 @@ struct Corr32 {
 // Corrector (dt): FP64 update of the hi + lo state, FP64 pressure recovery,
 // returns the CFL speed of the new state (0 outside the active cells).  Operands loaded up front as in the
 // predictor.
-template <int DIMS, class GM, class FL, class EM>
+// SPLIT (the corrector of a fused 1D step): the new hi and lo words go to
+// uh1 and ul1 instead of back to a.uh and a.ul.
+template <int DIMS, class GM, class FL, class EM, bool SPLIT = false>
 __device__ __forceinline__ double update32_corr_e(const Geo &g, const Corr32 &a, const Lims &L,
                                                   const double *__restrict__ dts, const Box &act,
                                                   int fms_ref, const FL &fl, const EM &em, int i,
-                                                  int j, int k) {
+                                                  int j, int k, float *uh1 = nullptr,
+                                                  float *ul1 = nullptr) {
   const long c = ((long)k * g.nt[1] + j) * g.nt[0] + i, sv = g.sv;
   double cdsmax = 0.;
   const double dt = dts[0];
@@ -407,8 +423,8 @@ __device__ __forceinline__ double update32_corr_e(const Geo &g, const Corr32 &a,
   if (ro.cell) {
 #pragma unroll
     for (int n = 0; n < NMODE - 3; ++n) {
-      a.uh[n * sv + c] = nh[n];
-      a.ul[n * sv + c] = nl[n];
+      (SPLIT ? uh1 : a.uh)[n * sv + c] = nh[n];
+      (SPLIT ? ul1 : a.ul)[n * sv + c] = nl[n];
     }
 #pragma unroll
     for (int n = 0; n < NMODE; ++n) a.w32[n * sv + c] = (float)wv[n];

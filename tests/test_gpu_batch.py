@@ -133,7 +133,8 @@ def test_batch_max_steps_and_resume():
     b.close()
 
 
-@pytest.mark.parametrize("precision,mix", [("fp64", "2d"), ("fp32c", "2d"), ("fp64", "1d")])
+@pytest.mark.parametrize("precision,mix", [("fp64", "2d"), ("fp32c", "2d"), ("fp64", "1d"),
+                                           ("fp32c", "1d")])
 def test_batch_finished_boxes_across_pieces(precision, mix):
     """Boxes that reach t_max long before the others sit out many steps (the
     face buffers, and in an all-1D FP64 batch the prim buffers, swap without
@@ -259,14 +260,16 @@ def test_batch_fp32c_bitwise_vs_single_fp32c(use_graph):
     check_against_single(cases, use_graph=use_graph, precision="fp32c")
 
 
-@pytest.mark.parametrize("rsolver,reconstr", [("hlld", "linear"), ("hllt", "linear"),
-                                               ("hll", "const"), ("hlld", "const")])
-def test_batch_all_1d_fused_steps_bitwise_vs_single(rsolver, reconstr):
-    """An FP64 batch of 1D boxes only runs each step as one fused pass
-    (predictor and corrector in LDS, prim buffers swapped like the faces):
-    every box equals its single-grid run, with outflow, periodic and walled
-    tubes of lengths that do and do not fill the last work block."""
-    say(f"case: all-1D FP64 batch, fused steps, {rsolver} + {reconstr}")
+@pytest.mark.parametrize("rsolver,reconstr,precision",
+                         [("hlld", "linear", "fp64"), ("hllt", "linear", "fp64"),
+                          ("hll", "const", "fp64"), ("hlld", "const", "fp64"),
+                          ("hlld", "linear", "fp32c"), ("hll", "const", "fp32c")])
+def test_batch_all_1d_fused_steps_bitwise_vs_single(rsolver, reconstr, precision):
+    """A batch of 1D boxes only runs each step as one fused pass (predictor
+    and corrector in LDS, state buffers swapped like the faces): every box
+    equals its single-grid run, with outflow, periodic and walled tubes of
+    lengths that do and do not fill the last work block."""
+    say(f"case: all-1D batch, fused steps, {rsolver} + {reconstr}, {precision}")
     ng = 2 if reconstr == "const" else 3
     rng = np.random.Generator(np.random.PCG64(4008))
     walled = P.riemann_tube(251, rng, ng=ng)
@@ -275,8 +278,9 @@ def test_batch_all_1d_fused_steps_bitwise_vs_single(rsolver, reconstr):
              for n, bc, tm in ((64, (1,) * 6, 0.05), (502, (0,) * 6, 0.02),
                                (245, (1,) * 6, 0.04), (1000, (1,) * 6, 0.01))]
     cases.append((walled, (2, 2, 0, 0, 0, 0), 0.03))
-    check_against_single(cases, rsolver=rsolver, reconstr=reconstr)
-    check_against_single(cases, rsolver=rsolver, reconstr=reconstr, use_graph=False)
+    check_against_single(cases, rsolver=rsolver, reconstr=reconstr, precision=precision)
+    check_against_single(cases, rsolver=rsolver, reconstr=reconstr, use_graph=False,
+                         precision=precision)
 
 
 def test_batch_hllt_bitwise_vs_single():
