@@ -151,7 +151,8 @@ int bnz_gpu_flux_pencils(int rsolver, const double *wl, const double *wr, const 
  * is the grid that one bnz_create with the same bnz_params would build (its
  * own sizes, extent, boundary flags and t_max); all boxes advance together,
  * one launch per stage over every box of a dimensionality (1D boxes run each
- * stage as one fused flux-and-update pass), with a per-box CFL dt, time and
+ * stage as one fused flux-and-update pass, and a batch of 1D boxes only each
+ * whole step as one pass), with a per-box CFL dt, time and
  * "running" flag on the device, so that each box reproduces the single-grid
  * run of vanleer.advance (src/integration/vanleer.pyx:87-167) to its own
  * t_max bit for bit.  The reference has no batched form: a sweep there is one
