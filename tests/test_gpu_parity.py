@@ -124,6 +124,17 @@ def test_orszag_tang_2d_bitwise():
     assert compare(res, "OT64")
 
 
+def test_orszag_tang_128_to_half_bitwise():
+    """The reference's own CPU case: Orszag-Tang 128^2 run to t = 0.5 through
+    advance(t_max) (about 400 steps, the last one clipped to t_max): state,
+    time and step count equal the oracle's bit for bit."""
+    say("case: Orszag-Tang 128^2 to t=0.5, hlld+plm, periodic")
+    res = run_pair(P.orszag_tang(128), tmax=0.5)
+    assert compare(res, "OT128 t=0.5")
+    assert res["t"] == res["o"].time == 0.5
+    assert res["n"] == res["o"].step
+
+
 def test_turbulence_3d():
     say("case: turbulence 24^3, 6 steps, hlld+plm, periodic")
     res = run_pair(P.turbulence(24, seed_b=31, seed_v=32), steps=6)
