@@ -674,7 +674,7 @@ int bnz_batch_create(const bnz_params *p, const bnz_box *boxes, int nbox, bnz_ba
       if (dims < 3) L.lo[2] = L.hi[2] = 0;
       const int il = L.lo[0], iu = L.hi[0], jl = L.lo[1], ju = L.hi[1], kl = L.lo[2], ku = L.hi[2];
       d.items[o == 0 ? KF0 : KF1] = flux3_box(dims, L);
-      // 1D boxes form their EMFs in the update (Emf1D, Emf32x1D): no EMF items
+      // 1D boxes form their EMFs from the LDS fluxes (EmfLds1D, EmfLds32x1D): no EMF items
       d.items[o == 0 ? KEM0 : KEM1] =
           dims == 1
               ? mkbox(0, 0, 0, 0, 0, 0)

@@ -135,16 +135,6 @@ __device__ __forceinline__ void emf_edge_d(const Geo &g, const double *__restric
   }
 }
 
-__device__ __forceinline__ void emf_edge(const Geo &g, const double *__restrict__ w,
-                                         const double *__restrict__ fx,
-                                         const double *__restrict__ fy,
-                                         const double *__restrict__ fz,
-                                         double *__restrict__ ee, int i, int j, int k) {
-  if (g.dims == 3) emf_edge_d<3>(g, w, fx, fy, fz, ee, i, j, k);
-  else if (g.dims == 2) emf_edge_d<2>(g, w, fx, fy, fz, ee, i, j, k);
-  else emf_edge_d<1>(g, w, fx, fy, fz, ee, i, j, k);
-}
-
 // --------------------------------------------------------------------------
 // Stage update: advance_hydro (godunov.pyx:215-252), advance_b (ct.pyx:15-91),
 // interp_bc (ct.pyx:96-123) and cons2prim (eos.pyx:13-41) fused, one thread
