@@ -126,78 +126,24 @@ __global__ __launch_bounds__(BLK) void kb_stage1(const BoxDev *__restrict__ boxe
   }
 }
 
-// A whole VL step of a 1D box in one pass (all-1D FP64 batches): a work
-// block owns the C = BLK - 5 cells i0 .. i0 + C - 1.  Thread t computes the
-// predictor face and cell i0 - 2 + t (fluxes and prims in LDS; the 2 cells
-// below and 3 above i0's range are computed by the neighbouring blocks too),
-// then the corrector face i0 + t from the LDS prims and, for t < C, the
-// corrector cell i0 + t.  The arithmetic is kb_stage1's.  The pass reads the
-// step's prim w and faces b only and writes the new ones to ws and bs,
-// which the caller swaps with w and b, so no read races a write; the
-// predictor's faces are not kept (in 1D the corrector needs only the x
-// faces, which the predictor copies unchanged).
+// A whole VL step of a 1D box in one pass (all-1D FP64 batches), work
+// block by work block (step1_block): the pass reads the step's prim w and
+// faces b only and writes the new ones to ws and bs, which the caller swaps
+// with w and b.
 template <int ORDER, int RS>
 __global__ __launch_bounds__(BLK) void kb_step1(const BoxDev *__restrict__ boxes, WList wl,
                                                 BoxState st, Pools p, int fms_ref) {
-  constexpr int C = BLK - 5;
-  __shared__ double sf[NMODE * BLK];  // face fluxes: predictor, then corrector
-  __shared__ double sw[NMODE * BLK];  // predictor prim of cell i0 - 2 + t
-  const int t = threadIdx.x;
+  __shared__ double sf[NMODE * BLK];
+  __shared__ double sw[NMODE * BLK];
   for (int wb = wl.lo + blockIdx.x; wb < wl.hi; wb += gridDim.x) {
     const int b = wl.bmap[wb];
     if (!st.run[b]) continue;
     const BoxDev &d = boxes[b];
-    const Geo &g = d.g;
-    const long o = d.off, sv = g.sv;
-    const Lims &L0 = d.L[0], &L1 = d.L[1];
-    const int i0 = (wb - d.blk0[wl.kt]) * C;
-    const double *w = p.w + o, *bf = p.b + o;
-    const auto sink = [&](const double *f, long) {
-#pragma unroll
-      for (int n = 0; n < NMODE; ++n)
-        if (n != BX) sf[n * BLK + t] = f[n];
-    };
-    // predictor (donor cell, dt/2)
-    const int ip = i0 - 2 + t;
-    if (ip >= L0.lo[0] && ip <= L0.hi[0] + 1) {
-      bool got = false;
-      flux_face_to<0, 0, RS>(g, w, bf, ip, 0, 0, [&](const double *f, long q) {
-        sink(f, q);
-        got = true;
-      });
-      if (!got)  // no region matched: the flux row keeps what it held (flux_face)
-        for (int n = 0; n < NMODE; ++n)
-          if (n != BX) sf[n * BLK + t] = p.fx[n * sv + o + ip];
-    }
-    __syncthreads();
-    if (t < BLK - 1 && ip >= 0 && ip < g.nt[0])
-      update_cell_e<1, true, false, EmfLds1D, GeoArr, FluxLds1D, true>(
-          g, GeoArr{g}, w, FluxLds1D{sf, t}, EmfLds1D{sf, t}, bf, nullptr, sw, L0, st.dt + b,
-          d.act, fms_ref, ip, 0, 0, t);
-    __syncthreads();
-    // corrector (reconstruction order, dt): fluxes from the LDS prims, read
-    // through a geometry whose variable stride is the LDS row (BLK) and
-    // whose cell ip sits at slot ip - (i0 - 2)
-    const int ic = i0 + t;
-    if (t <= C && ic >= L1.lo[0] && ic <= L1.hi[0] + 1) {
-      Geo gl = g;
-      gl.sv = BLK;
-      bool got = false;
-      flux_face_to<0, ORDER, RS>(gl, sw - (i0 - 2), bf, ic, 0, 0,
-                                 [&](const double *f, long q) {
-                                   sink(f, q);
-                                   got = true;
-                                 });
-      if (!got)
-        for (int n = 0; n < NMODE; ++n)
-          if (n != BX) sf[n * BLK + t] = p.fx[n * sv + o + ic];
-    }
-    __syncthreads();
-    double cds = 0.;
-    if (t < C && ic < g.nt[0])
-      cds = update_cell_e<1, false, true>(g, GeoArr{g}, w, FluxLds1D{sf, t}, EmfLds1D{sf, t}, bf,
-                                          p.bs + o, p.ws + o, L1, st.dt + b, d.act, fms_ref, ic,
-                                          0, 0);
+    const long o = d.off;
+    const double cds = step1_block<ORDER, RS>(d.g, p.w + o, p.b + o, p.ws + o, p.bs + o,
+                                              p.fx + o, d.L[0], d.L[1], st.dt + b, d.act,
+                                              fms_ref, (wb - d.blk0[wl.kt]) * STEP1_CELLS, sf,
+                                              sw);
     block_max_atomic(cds, st.cmax + b);  // ends with a barrier: sf, sw are free again
   }
 }

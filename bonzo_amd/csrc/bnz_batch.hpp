@@ -19,7 +19,7 @@ enum { KF0 = 0, KEM0, KF1, KEM1, KUPD, KACT, KS1, KT1, NK };
 
 // cells per work block of list kt
 __host__ __device__ constexpr int blk_items(int kt) {
-  return kt == KS1 ? BLK - 1 : kt == KT1 ? BLK - 5 : BLK;
+  return kt == KS1 ? BLK - 1 : kt == KT1 ? STEP1_CELLS : BLK;
 }
 
 struct BoxDev {

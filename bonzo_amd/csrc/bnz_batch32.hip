@@ -139,7 +139,7 @@ template <int ORDER, int RS>
 __global__ __launch_bounds__(BLK) void kb32_step1(const BoxDev *__restrict__ boxes, WList wl,
                                                   BoxState st, Pools P, Pools32 Q, int fms_ref,
                                                   int *nguard) {
-  constexpr int C = BLK - 5;
+  constexpr int C = STEP1_CELLS;
   __shared__ float sf[NMODE * BLK];  // face fluxes: predictor, then corrector
   __shared__ float sw[NMODE * BLK];  // predictor FP32 prim of cell i0 - 2 + t
   const int t = threadIdx.x;

@@ -16,6 +16,9 @@ namespace bnz {
 
 constexpr int NVAR_STATE = 9;
 constexpr int BLK = 256;
+// cells a workgroup owns in a fused 1D step (step1_block): the predictor
+// also computes 2 cells below and 3 above them
+constexpr int STEP1_CELLS = BLK - 5;
 
 // --------------------------------------------------------------------------
 // Geometry passed by value to every kernel.

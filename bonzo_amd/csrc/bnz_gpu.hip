@@ -46,25 +46,32 @@ __global__ __launch_bounds__(BLK) void k_flux3(Geo g, const double *__restrict__
   flux_cell<ORDER, RS>(g, w, bf, fx, fy, fz, L, i, j, k);
 }
 
-// One stage of a 1D grid in one pass (stage1_cell): workgroup b owns faces
-// b * (BLK - 1) .. + BLK - 1 and the BLK - 1 cells between them; the
-// corrector leaves its workgroup's CFL maximum in part.
-template <bool PRED, int ORDER, int RS>
-__global__ __launch_bounds__(BLK) void k_stage1(Geo g, const double *__restrict__ wf,
-                                                const double *b0, double *b1, const double *w0,
-                                                double *w1, const double *__restrict__ fx_old,
-                                                Lims L, const double *__restrict__ dts, Box act,
-                                                int fms_ref, double *__restrict__ part) {
+// A whole VL step of a 1D grid in one pass (step1_block): workgroup b owns
+// the cells from b * STEP1_CELLS; the new prim and faces go to wout and
+// bout, which the host then swaps with w and b; the workgroup's CFL maximum
+// goes to part.
+template <int ORDER, int RS>
+__global__ __launch_bounds__(BLK) void k_step1(Geo g, const double *__restrict__ w,
+                                               const double *__restrict__ bf,
+                                               double *__restrict__ wout,
+                                               double *__restrict__ bout,
+                                               const double *__restrict__ fx_old, Lims L0,
+                                               Lims L1, const double *__restrict__ dts, Box act,
+                                               int fms_ref, double *__restrict__ part) {
   __shared__ double sf[NMODE * BLK];
-  const int i = blockIdx.x * (BLK - 1) + threadIdx.x;
-  if (dts[0] == 0.) {  // a step past t_max: state unchanged, faces carried over
-    if (!PRED && threadIdx.x < BLK - 1 && i < g.nt[0])
-      for (int a = 0; a < 3; ++a) b1[a * g.sv + i] = b0[a * g.sv + i];
+  __shared__ double sw[NMODE * BLK];
+  const int i0 = blockIdx.x * STEP1_CELLS;
+  if (dts[0] == 0.) {  // a step past t_max: the state is carried into the new buffers
+    const int i = i0 + threadIdx.x;
+    if (threadIdx.x < STEP1_CELLS && i < g.nt[0]) {
+      for (int n = 0; n < NMODE; ++n) wout[n * g.sv + i] = w[n * g.sv + i];
+      for (int a = 0; a < 3; ++a) bout[a * g.sv + i] = bf[a * g.sv + i];
+    }
     return;
   }
-  const double cds =
-      stage1_cell<PRED, ORDER, RS>(g, wf, b0, b1, w0, w1, fx_old, L, dts, act, fms_ref, i, sf);
-  if (!PRED) block_max_to(cds, part);
+  const double cds = step1_block<ORDER, RS>(g, w, bf, wout, bout, fx_old, L0, L1, dts, act,
+                                            fms_ref, i0, sf, sw);
+  block_max_to(cds, part);
 }
 
 // --------------------------------------------------------------------------
@@ -497,7 +504,7 @@ void launch_emf(bnz_ctx *c, const double *w, const Lims &L, const char *name) {
   dim3(sh.blocks), dim3(BLK), 0, c->stream, c->g, w, c->fx, c->fy, c->fz, c->ee, e, sh.ty
   if (sh.tiled) hipLaunchKernelGGL((k_emf<3, true>), EMF_ARGS);
   else if (c->dims == 3) hipLaunchKernelGGL((k_emf<3, false>), EMF_ARGS);
-  else hipLaunchKernelGGL((k_emf<2, false>), EMF_ARGS);  // 1D grids run k_stage1
+  else hipLaunchKernelGGL((k_emf<2, false>), EMF_ARGS);  // 1D grids run k_step1
 #undef EMF_ARGS
 }
 
@@ -552,7 +559,7 @@ void launch_update(bnz_ctx *c, bool pred, const Lims &L) {
   } else if (c->dims == 3) {
     if (pred) hipLaunchKernelGGL((k_update<3, true, false, false>), UPD_ARGS);
     else hipLaunchKernelGGL((k_update<3, false, true, false>), UPD_ARGS);
-  } else {  // 2D (1D grids run k_stage1)
+  } else {  // 2D (1D grids run k_step1)
     if (pred) hipLaunchKernelGGL((k_update<2, true, false, false>), UPD_ARGS);
     else hipLaunchKernelGGL((k_update<2, false, true, false>), UPD_ARGS);
   }
@@ -561,29 +568,28 @@ void launch_update(bnz_ctx *c, bool pred, const Lims &L) {
   c->dt_parts = pred ? 0 : sh.blocks;
 }
 
-// A 1D stage as one fused launch (k_stage1): predictor w -> ws, bs;
-// corrector ws -> w (in place), bs.
-template <bool PRED, int ORDER, int RS>
-void stage1_launch(bnz_ctx *c, const Lims &L) {
-  const int nb = (c->nt[0] + BLK - 2) / (BLK - 1);
-  Launch t(c, PRED ? "stage1_pred" : "stage1_corr");
-  hipLaunchKernelGGL((k_stage1<PRED, ORDER, RS>), dim3(nb), dim3(BLK), 0, c->stream, c->g,
-                     PRED ? c->w : c->ws, c->b, c->bs, c->w, PRED ? c->ws : c->w, c->fx, L,
-                     c->dev, active_box(c), c->p.dt_fms_ref, c->part);
-  c->dt_parts = PRED ? 0 : nb;
+// A 1D step as one fused launch (k_step1): w, b -> ws, bs, then swapped.
+template <int ORDER, int RS>
+void step1_launch(bnz_ctx *c, const Lims &L0, const Lims &L1) {
+  const int nb = (c->nt[0] + STEP1_CELLS - 1) / STEP1_CELLS;
+  Launch t(c, "step1");
+  hipLaunchKernelGGL((k_step1<ORDER, RS>), dim3(nb), dim3(BLK), 0, c->stream, c->g, c->w, c->b,
+                     c->ws, c->bs, c->fx, L0, L1, c->dev, active_box(c), c->p.dt_fms_ref,
+                     c->part);
+  c->dt_parts = nb;
+  std::swap(c->w, c->ws);  // the faces swap with the other stages' below
 }
 
-template <bool PRED>
-void launch_stage1(bnz_ctx *c, int order, const Lims &L) {
+void launch_step1(bnz_ctx *c, const Lims &L0, const Lims &L1) {
   const int rs = c->p.rsolver;
-  if (order == 0) {
-    if (rs == BNZ_RS_HLLD) stage1_launch<PRED, 0, BNZ_RS_HLLD>(c, L);
-    else if (rs == BNZ_RS_HLLT) stage1_launch<PRED, 0, BNZ_RS_HLLT>(c, L);
-    else stage1_launch<PRED, 0, BNZ_RS_HLL>(c, L);
+  if (c->order == 0) {
+    if (rs == BNZ_RS_HLLD) step1_launch<0, BNZ_RS_HLLD>(c, L0, L1);
+    else if (rs == BNZ_RS_HLLT) step1_launch<0, BNZ_RS_HLLT>(c, L0, L1);
+    else step1_launch<0, BNZ_RS_HLL>(c, L0, L1);
   } else {
-    if (rs == BNZ_RS_HLLD) stage1_launch<PRED, 1, BNZ_RS_HLLD>(c, L);
-    else if (rs == BNZ_RS_HLLT) stage1_launch<PRED, 1, BNZ_RS_HLLT>(c, L);
-    else stage1_launch<PRED, 1, BNZ_RS_HLL>(c, L);
+    if (rs == BNZ_RS_HLLD) step1_launch<1, BNZ_RS_HLLD>(c, L0, L1);
+    else if (rs == BNZ_RS_HLLT) step1_launch<1, BNZ_RS_HLLT>(c, L0, L1);
+    else step1_launch<1, BNZ_RS_HLL>(c, L0, L1);
   }
 }
 
@@ -591,9 +597,8 @@ void launch_stage1(bnz_ctx *c, int order, const Lims &L) {
 void launch_step_body(bnz_ctx *c) {
   Lims L0 = integr_lims(c, 0);
   Lims L1 = integr_lims(c, c->order);
-  if (c->dims == 1) {  // fluxes through LDS
-    launch_stage1<true>(c, 0, L0);
-    launch_stage1<false>(c, c->order, L1);
+  if (c->dims == 1) {  // the whole step through LDS
+    launch_step1(c, L0, L1);
   } else {
     // predictor: donor cell, dt/2, lims of order 0; ws and bs are new buffers
     launch_flux_dispatch(c, 0, c->w, c->b, L0);
@@ -795,7 +800,7 @@ int bnz_create(const bnz_params *p, bnz_ctx **out) {
     }
     c->npart_cap = std::max({(long)blocks_for(a.total()) * std::max(NDIAG, NHIST) + 64,
                              (long)blocks_for(c->ncell), (long)tiled_blocks(grid, 1, 2),
-                             (long)(c->nt[0] + BLK - 2) / (BLK - 1)});
+                             (long)(c->nt[0] + STEP1_CELLS - 1) / STEP1_CELLS});
   }
   HIPCHK(c, hipMalloc(&c->part, c->npart_cap * sizeof(double)));
 

@@ -402,4 +402,72 @@ __device__ __forceinline__ double stage1_cell(const Geo &g, const double *__rest
   return cds;
 }
 
+// A whole VL step of a 1D grid for a workgroup of BLK threads that owns the
+// STEP1_CELLS cells i0 .. i0 + STEP1_CELLS - 1.  Thread t computes the
+// predictor face and cell i0 - 2 + t (donor cell, dt/2; fluxes in sf, the
+// new prims in sw; the 2 cells below and 3 above the owned range are
+// computed by the neighbouring workgroups too), then the corrector face
+// i0 + t from the prims in sw and, for t < STEP1_CELLS, the corrector cell
+// i0 + t.  The arithmetic is stage1_cell's.  It reads the step's prim w and
+// faces b only and writes the new prim and faces to wout and bout, so no
+// read races a write; the predictor's faces are not kept (in 1D the
+// corrector needs only the x faces, which the predictor copies).  Every
+// thread must call it; returns the CFL speed of the thread's corrector cell.
+template <int ORDER, int RS>
+__device__ __forceinline__ double step1_block(const Geo &g, const double *w,
+                                              const double *bf, double *wout, double *bout,
+                                              const double *__restrict__ fx_old,
+                                              const Lims &L0, const Lims &L1,
+                                              const double *__restrict__ dts, const Box &act,
+                                              int fms_ref, int i0, double *sf, double *sw) {
+  constexpr int C = STEP1_CELLS;
+  const int t = threadIdx.x;
+  const long sv = g.sv;
+  const auto sink = [&](const double *f, long) {
+#pragma unroll
+    for (int n = 0; n < NMODE; ++n)
+      if (n != BX) sf[n * BLK + t] = f[n];
+  };
+  // predictor (donor cell, dt/2)
+  const int ip = i0 - 2 + t;
+  if (ip >= L0.lo[0] && ip <= L0.hi[0] + 1) {
+    bool got = false;
+    flux_face_to<0, 0, RS>(g, w, bf, ip, 0, 0, [&](const double *f, long q) {
+      sink(f, q);
+      got = true;
+    });
+    if (!got)  // no region matched: the flux row keeps what it held (flux_face)
+      for (int n = 0; n < NMODE; ++n)
+        if (n != BX) sf[n * BLK + t] = fx_old[n * sv + ip];
+  }
+  __syncthreads();
+  if (t < BLK - 1 && ip >= 0 && ip < g.nt[0])
+    update_cell_e<1, true, false, EmfLds1D, GeoArr, FluxLds1D, true>(
+        g, GeoArr{g}, w, FluxLds1D{sf, t}, EmfLds1D{sf, t}, bf, nullptr, sw, L0, dts, act,
+        fms_ref, ip, 0, 0, t);
+  __syncthreads();
+  // corrector (reconstruction order, dt): fluxes from the prims in sw, read
+  // through a geometry whose variable stride is the LDS row (BLK) and whose
+  // cell ip sits at slot ip - (i0 - 2)
+  const int ic = i0 + t;
+  if (t <= C && ic >= L1.lo[0] && ic <= L1.hi[0] + 1) {
+    Geo gl = g;
+    gl.sv = BLK;
+    bool got = false;
+    flux_face_to<0, ORDER, RS>(gl, sw - (i0 - 2), bf, ic, 0, 0, [&](const double *f, long q) {
+      sink(f, q);
+      got = true;
+    });
+    if (!got)
+      for (int n = 0; n < NMODE; ++n)
+        if (n != BX) sf[n * BLK + t] = fx_old[n * sv + ic];
+  }
+  __syncthreads();
+  double cds = 0.;
+  if (t < C && ic < g.nt[0])
+    cds = update_cell_e<1, false, true>(g, GeoArr{g}, w, FluxLds1D{sf, t}, EmfLds1D{sf, t}, bf,
+                                        bout, wout, L1, dts, act, fms_ref, ic, 0, 0);
+  return cds;
+}
+
 }  // namespace bnz
