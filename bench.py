@@ -48,6 +48,10 @@ PROFILED_TRAFFIC = {
     ("fp64", "flux3_plm"): (6.009e9, "profiles/vl_step_kernels_turb256.md"),
     ("fp32c", "update_corr"): (4.773e9, "profiles/vl_step_kernels_turb256.md"),
 }
+# Fraction of the sampled SIMDs' cycles spent issuing VALU work in the same
+# kernels (SQ_ACTIVE_INST_VALU x 4 / (GRBM_GUI_ACTIVE x SIMDs), same profile):
+# near 1 the launch is bound by arithmetic issue rather than by HBM.
+PROFILED_VALU_BUSY = {("fp64", "flux3_plm"): 0.77, ("fp32c", "update_corr"): 0.34}
 # Algorithmic bytes per cell-update of a whole VL step: 5 state volumes of 9
 # words (FP64 360 B; FP32C 252 B with hi + lo on the stage-2 read of S_n and
 # the write of S_{n+1}).
@@ -156,12 +160,13 @@ def run_path(prec, n, prim, bfld, args):
     b = kernel_bytes(prec, dname, n)
     ach = b / (davg * 1e-3) / 1e9
     prof = PROFILED_TRAFFIC.get((prec, dname), (None, None)) if n == 256 else (None, None)
+    busy = PROFILED_VALU_BUSY.get((prec, dname)) if n == 256 else None
     return {
         "value": value, "ms_per_step": 1e3 * el / args.steps, "tuned": tuned, "table": table,
         "kernels": {k: v[0] / v[1] for k, v in sorted(kt.items())},
         "roofline": {"bound": "hbm", "kernel": dname, "achieved": ach, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                     "traffic": prof[0], "traffic_source": prof[1],
+                     "traffic": prof[0], "traffic_source": prof[1], "valu_busy": busy,
                      "avg_launch_ms": davg, "algorithmic_bytes_per_launch": b},
         "step_GBs": STEP_BYTES[prec] * value / 1e9,
         "divb_max": d["divb_max"], "hlld_guards": d["hlld_guards"],
