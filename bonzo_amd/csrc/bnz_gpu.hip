@@ -861,8 +861,17 @@ int bnz_get_shape(const bnz_ctx *c, int *ntot3, int *ng) {
 int bnz_upload(bnz_ctx *c, const double *prim, const double *bfld) {
   if (!c || !c->stream) return BNZ_EINVAL;
   c->dt_parts = 0;
-  if (prim) HIPCHK(c, hipMemcpy(c->w, prim, NMODE * c->ncell * sizeof(double), hipMemcpyHostToDevice));
-  if (bfld) HIPCHK(c, hipMemcpy(c->b, bfld, 3 * c->ncell * sizeof(double), hipMemcpyHostToDevice));
+  // ordered after the steps still running on the context's stream (a plain
+  // hipMemcpy would not wait for a non-blocking stream)
+  if (prim)
+    HIPCHK(c, hipMemcpyAsync(c->w, prim, NMODE * c->ncell * sizeof(double),
+                             hipMemcpyHostToDevice, c->stream));
+  if (bfld)
+    HIPCHK(c, hipMemcpyAsync(c->b, bfld, 3 * c->ncell * sizeof(double), hipMemcpyHostToDevice,
+                             c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  // the uploaded state has no ghosts, cell-centred field or FP32C words yet
+  c->ready = false;
   return BNZ_OK;
 }
 

@@ -77,7 +77,9 @@ const char *bnz_last_error(const bnz_ctx *ctx);
 /* Ntot per axis and ng (src/coordinates/grid.pyx:129-161) */
 int bnz_get_shape(const bnz_ctx *ctx, int *ntot3, int *ng);
 
-/* Host <-> device copies of the grid arrays (GridData, grid.pyx:28-52) */
+/* Host <-> device copies of the grid arrays (GridData, grid.pyx:28-52).
+   upload waits for the context's queued work; the state then needs
+   bnz_init_state (or bnz_load_state) before the next step. */
 int bnz_upload(bnz_ctx *ctx, const double *prim, const double *bfld);
 int bnz_download(bnz_ctx *ctx, double *prim, double *bfld, double *cons);
 

@@ -479,3 +479,33 @@ def test_advance_chunks_and_idle_steps_change_nothing():
             assert n2 > 0 and g.time()[0] == 0.04
             ref.close()
             g.close()
+
+
+def test_upload_waits_for_queued_steps_and_needs_init():
+    """upload() is ordered after the steps still queued on the context's
+    stream and leaves a state that must be initialised before the next step;
+    after init_state the context runs exactly like a fresh one, in both
+    precisions."""
+    from bonzo_amd._lib import BNZ_ESTATE, BnzError, Context
+    say("case: re-upload after 12 queued steps, turbulence 20^3 (two seeds), fp64 and fp32c")
+    s, prim0, bfld0 = P.turbulence(20, seed_b=31, seed_v=32)
+    _, prim1, bfld1 = P.turbulence(20, seed_b=41, seed_v=42)
+    for prec in ("fp64", "fp32c"):
+        a = Context(20, 20, 20, precision=prec)
+        a.upload(prim0, bfld0)
+        a.init_state()
+        a.run_steps(12)                  # stream-ordered, no host synchronisation
+        a.upload(prim1, bfld1)
+        with pytest.raises(BnzError) as e:
+            a.run_steps(1)
+        assert e.value.code == BNZ_ESTATE
+        a.init_state()
+        a.run_steps(4)
+        b = Context(20, 20, 20, precision=prec)
+        b.upload(prim1, bfld1)
+        b.init_state()
+        b.run_steps(4)
+        assert a.time() == b.time()
+        assert np.array_equal(a.save_state(), b.save_state())
+        a.close()
+        b.close()
