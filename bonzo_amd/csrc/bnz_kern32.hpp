@@ -36,8 +36,8 @@ __device__ __forceinline__ void flux32_face_to(const Geo &g, const float *__rest
       wr[n] = p[0];
     } else {
       const float wm2 = p[-2 * sd], wm1 = p[-sd], w0 = p[0], wp1 = p[sd];
-      wl[n] = wm1 + 0.5f * vl_lim(wm1 - wm2, w0 - wm1);
-      wr[n] = w0 - 0.5f * vl_lim(w0 - wm1, wp1 - w0);
+      wl[n] = wm1 + vl_half(wm1 - wm2, w0 - wm1);
+      wr[n] = w0 - vl_half(w0 - wm1, wp1 - w0);
     }
   }
   wl[BX] = bn;

@@ -34,8 +34,8 @@ __device__ __forceinline__ void flux_face_to(const Geo &g, const double *__restr
     } else {
       const double wm2 = p[-2 * sd], wm1 = p[-sd], w0 = p[0], wp1 = p[sd];
       // cell c-1, upper state; cell c, lower state (reconstr_c.c:54-89)
-      const double dl = 0.5 * vl_lim(wm1 - wm2, w0 - wm1);
-      const double dr = 0.5 * vl_lim(w0 - wm1, wp1 - w0);
+      const double dl = vl_half(wm1 - wm2, w0 - wm1);
+      const double dr = vl_half(w0 - wm1, wp1 - w0);
       wl[n] = wm1 + dl;
       wr[n] = w0 - dr;
     }

@@ -147,14 +147,18 @@ __device__ __forceinline__ T fms(const T *w, T gam) {
   return Ar<T>::root(Ar<T>::div(T(0.5), w[RHO]) * (c1 + c2));
 }
 
-// van Leer limiter, src/coordinates/reconstr_c.h:39-44
+// Half the van Leer limited slope, 0.5 * vl_lim(a, b) as PLM adds it
+// (src/coordinates/reconstr_c.h:39-44, reconstr_c.c:77): vl_lim is
+// 2ab / (a+b) where ab > 0, else 0.  Scaling by 2 commutes with rounding, so
+// 0.5 * (2ab / (a+b)) is ab / (a+b) bit for bit while 2ab is finite and the
+// quotient is a normal number.  Branch-free: the division runs in every lane
+// (so the stencil loads around it stay in one basic block) and its result is
+// dropped where ab <= 0, NaN included.
 template <typename T>
-__device__ __forceinline__ T vl_lim(T a, T b) {
-  T ab = a * b;
-  // branch-free: the division always runs, on (0, 1) where the limiter is 0,
-  // so the stencil loads around it stay in one basic block
-  const bool pos = ab > T(0);
-  return Ar<T>::div(pos ? T(2) * ab : T(0), pos ? a + b : T(1));
+__device__ __forceinline__ T vl_half(T a, T b) {
+  const T ab = a * b;
+  const T q = Ar<T>::div(ab, a + b);
+  return ab > T(0) ? q : T(0);
 }
 
 // sLsR_simple, src/mhd/fluxes_c.c:25-42 (FMAX/FMIN as fmax/fmin)
