@@ -572,6 +572,15 @@ int bnz_batch_create(const bnz_params *p, const bnz_box *boxes, int nbox, bnz_ba
     const bnz_box &s = boxes[b];
     if (s.nx < 1 || s.ny < 1 || s.nz < 1 || (s.nz > 1 && s.ny == 1))
       return bfail(c, BNZ_EINVAL, "box " + std::to_string(b) + ": bad size");
+    {
+      // before any per-box host array is sized from it
+      const long mx = 1L << 30, ext = 2 * ng;
+      const long cells = (long)std::min<long>(s.nx, mx) + ext;
+      const long cy = s.ny > 1 ? (long)std::min<long>(s.ny, mx) + ext : 1;
+      const long cz = s.nz > 1 ? (long)std::min<long>(s.nz, mx) + ext : 1;
+      if (s.nx > mx || s.ny > mx || s.nz > mx || off + cells * cy * cz >= (1L << 31))
+        return bfail(c, BNZ_EINVAL, "batch too large: pool cell indices are 32-bit");
+    }
     for (int q = 0; q < 6; ++q)
       if (s.bc[q] != BNZ_BC_PERIODIC && s.bc[q] != BNZ_BC_OUTFLOW && s.bc[q] != BNZ_BC_REFLECT)
         return bfail(c, BNZ_EUNSUPPORTED,

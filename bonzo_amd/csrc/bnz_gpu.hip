@@ -691,6 +691,8 @@ int bnz_create(const bnz_params *p, bnz_ctx **out) {
     return code;
   };
   if (p->nx < 1 || p->ny < 1 || p->nz < 1) return bad(BNZ_EINVAL, "nx, ny, nz must be >= 1");
+  if (p->nx > (1 << 30) || p->ny > (1 << 30) || p->nz > (1 << 30))
+    return bad(BNZ_EINVAL, "grid too large: cell indices are 32-bit");
   if (p->nz > 1 && p->ny == 1) return bad(BNZ_EINVAL, "nz > 1 needs ny > 1 (grid.pyx:114-122)");
   if (!(p->gam > 1.0)) return bad(BNZ_EINVAL, "gam must be > 1");
   if (!(p->cour > 0.0)) return bad(BNZ_EINVAL, "cour must be > 0");

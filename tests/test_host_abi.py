@@ -112,6 +112,23 @@ def test_batch_create_rejects_bad_selection_without_touching_gpu():
     assert b"box 1" in L.bnz_batch_last_error(h)
     L.bnz_batch_destroy(h)
     assert L.bnz_batch_create(ctypes.byref(p), boxes, 0, ctypes.byref(h)) == _lib.BNZ_EINVAL
+    # oversized boxes are rejected before any host or device array is sized
+    boxes[1].bc[0] = 0
+    for size in ((2 ** 30 + 1, 1, 1), (2 ** 31 - 1, 1, 1), (1300, 1300, 1300)):
+        boxes[1].nx, boxes[1].ny, boxes[1].nz = size
+        assert L.bnz_batch_create(ctypes.byref(p), boxes, 2, ctypes.byref(h)) == _lib.BNZ_EINVAL
+        assert b"too large" in L.bnz_batch_last_error(h)
+        L.bnz_batch_destroy(h)
+
+
+def test_create_rejects_oversized_grids_without_touching_gpu():
+    L = _lib.load()
+    h = ctypes.c_void_p()
+    for size in ((2 ** 30 + 1, 1, 1), (2 ** 31 - 1, 1, 1), (1300, 1300, 1300)):
+        p = _lib.make_params(nx=size[0], ny=size[1], nz=size[2])
+        assert L.bnz_create(ctypes.byref(p), ctypes.byref(h)) == _lib.BNZ_EINVAL
+        assert b"too large" in L.bnz_last_error(h)
+        L.bnz_destroy(h)
 
 
 def test_selection_semantics_follow_reference(tmp_path):
