@@ -245,8 +245,9 @@ struct bnz_ctx {
   bnz::GhostOp *gops = nullptr;  // ghost cells touched by a reflecting side
   int ngops = 0;
   double *dev = nullptr;      // dt, t, tmax, step
-  double *part = nullptr;     // reduction partials
+  double *part = nullptr;     // CFL partials (the last corrector's, kept for the next dt)
   long npart_cap = 0;
+  double *rpart = nullptr;    // partials of the diag and history reductions
   std::vector<double> h_dlf[3], h_dlfi[3], h_icm[3], h_icp[3];
   bnz::Geo g{};
   // FP32-compensated state (BNZ_PREC_FP32C): conserved hydro variables as

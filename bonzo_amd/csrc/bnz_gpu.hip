@@ -805,6 +805,10 @@ int bnz_create(const bnz_params *p, bnz_ctx **out) {
                              (long)(c->nt[0] + STEP1_CELLS - 1) / STEP1_CELLS});
   }
   HIPCHK(c, hipMalloc(&c->part, c->npart_cap * sizeof(double)));
+  // diag and history reduce into their own buffer, so the CFL partials the
+  // last corrector left stay valid and the next dt does not depend on them
+  HIPCHK(c, hipMalloc(&c->rpart, (size_t)blocks_for(a.total()) * std::max(NDIAG, NHIST) *
+                                     sizeof(double)));
 
   Geo &g = c->g;
   for (int q = 0; q < 3; ++q) g.nt[q] = c->nt[q];
@@ -838,7 +842,8 @@ int bnz_destroy(bnz_ctx *c) {
   if (!c) return BNZ_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   collect_timing(c);
-  double *bufs[] = {c->w, c->ws, c->b, c->bs, c->fx, c->fy, c->fz, c->ee, c->geo, c->dev, c->part};
+  double *bufs[] = {c->w,   c->ws,  c->b,   c->bs,   c->fx,  c->fy,
+                    c->fz,  c->ee,  c->geo, c->dev,  c->part, c->rpart};
   for (double *q : bufs)
     if (q) (void)hipFree(q);
   if (c->ghost) (void)hipFree(c->ghost);
@@ -1104,18 +1109,18 @@ int bnz_load_state(bnz_ctx *c, const void *host, long bytes) {
 
 int bnz_diag(bnz_ctx *c, bnz_diag_out *out) {
   if (!c || !out) return BNZ_EINVAL;
-  c->dt_parts = 0;  // the partial buffer is reused below
   if (c->f32 && c->ready) {
     int rc0 = f32_to64(c);
     if (rc0) return rc0;
   }
   Box a = active_box(c);
   const int nb = blocks_for(a.total());
-  hipLaunchKernelGGL(k_diag, dim3(nb), dim3(BLK), 0, c->stream, c->g, c->w, c->b, a, c->part);
+  hipLaunchKernelGGL(k_diag, dim3(nb), dim3(BLK), 0, c->stream, c->g, c->w, c->b, a, c->rpart);
   int rc = check_launch(c);
   if (rc) return rc;
   std::vector<double> h((size_t)nb * NDIAG);
-  HIPCHK(c, hipMemcpyAsync(h.data(), c->part, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(h.data(), c->rpart, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   double s[NDIAG] = {0};
   s[6] = s[7] = 0.;
@@ -1146,19 +1151,18 @@ int bnz_diag(bnz_ctx *c, bnz_diag_out *out) {
 int bnz_history(bnz_ctx *c, double *out16) {
   if (!c || !out16) return BNZ_EINVAL;
   if (!c->ready) return fail(c, BNZ_ESTATE, "bnz_init_state has not run");
-  c->dt_parts = 0;  // the partial buffer is reused below
   if (c->f32) {
     int rc0 = f32_to64(c);
     if (rc0) return rc0;
   }
   Box a = active_box(c);
   const int nb = blocks_for(a.total());
-  hipLaunchKernelGGL(k_hist, dim3(nb), dim3(BLK), 0, c->stream, c->g, c->w, a, c->part);
+  hipLaunchKernelGGL(k_hist, dim3(nb), dim3(BLK), 0, c->stream, c->g, c->w, a, c->rpart);
   int rc = check_launch(c);
   if (rc) return rc;
   std::vector<double> h((size_t)nb * NHIST);
   double d[4];
-  HIPCHK(c, hipMemcpyAsync(h.data(), c->part, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+  HIPCHK(c, hipMemcpyAsync(h.data(), c->rpart, h.size() * sizeof(double), hipMemcpyDeviceToHost,
                            c->stream));
   HIPCHK(c, hipMemcpyAsync(d, c->dev, sizeof(d), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1184,8 +1188,6 @@ int bnz_set_tuning(bnz_ctx *c, int flux_chunk, int emf_chunk, int update_chunk) 
   c->flux_chunk = flux_chunk;
   c->emf_chunk = emf_chunk;
   c->update_chunk = update_chunk;
-  // partial count of the fused CFL reduction depends on the update shape
-  c->dt_parts = 0;
   Box grid;
   for (int q = 0; q < 3; ++q) {
     grid.lo[q] = 0;
@@ -1197,6 +1199,9 @@ int bnz_set_tuning(bnz_ctx *c, int flux_chunk, int emf_chunk, int update_chunk) 
     double *np = nullptr;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipMalloc(&np, need * sizeof(double)));
+    // the last corrector's CFL partials move along: the next dt is the same
+    if (c->dt_parts > 0)
+      HIPCHK(c, hipMemcpy(np, c->part, c->dt_parts * sizeof(double), hipMemcpyDeviceToDevice));
     (void)hipFree(c->part);
     c->part = np;
     c->npart_cap = need;

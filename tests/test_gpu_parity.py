@@ -509,3 +509,32 @@ def test_upload_waits_for_queued_steps_and_needs_init():
         assert np.array_equal(a.save_state(), b.save_state())
         a.close()
         b.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32c"])
+def test_diagnostics_and_retuning_do_not_change_the_run(prec):
+    """diag(), history() and set_tuning() between steps leave the run bit for
+    bit as it is: the next dt still comes from the CFL maximum the last
+    corrector reduced (in FP32C from the FP64 state, which a fresh reduction
+    over the FP32 prim would not reproduce)."""
+    from bonzo_amd._lib import Context
+    say(f"case: diag/history/set_tuning between 8 steps, turbulence 20^3, {prec}")
+    s, prim, bfld = P.turbulence(20, seed_b=51, seed_v=52)
+    a = Context(20, 20, 20, precision=prec)
+    a.upload(prim, bfld)
+    a.init_state()
+    a.run_steps(8)
+    b = Context(20, 20, 20, precision=prec)
+    b.upload(prim, bfld)
+    b.init_state()
+    shapes = ((0, 0, 0), (4, 0, 8), (1, 1, 1), (64, 64, 64))
+    for q in range(8):
+        b.run_steps(1)
+        d = b.diag()
+        assert d["nonfinite"] == 0 and d["divb_max"] < 1e-12
+        b.history()
+        b.set_tuning(*shapes[q % 4])
+    assert a.time() == b.time()
+    assert np.array_equal(a.save_state(), b.save_state())
+    a.close()
+    b.close()
