@@ -382,7 +382,6 @@ Ghost32 ghost_arrays(bnz_batch *c) {
   int q = 0;
   const long n = c->ncell;
   for (int v = 0; v < 6; ++v) p.a[q++] = c->Q.uh + v * n;
-  for (int v = 0; v < 6; ++v) p.a[q++] = c->Q.ul + v * n;
   for (int v = 0; v < 9; ++v) p.a[q++] = c->Q.w32 + v * n;
   for (int v = 0; v < 3; ++v) p.d[v] = c->P.b + v * n;
   return p;

@@ -204,7 +204,6 @@ void ghost32(bnz_ctx *c) {
   Ghost32 p;
   int q = 0;
   for (int n = 0; n < 6; ++n) p.a[q++] = c->uh + n * c->ncell;
-  for (int n = 0; n < 6; ++n) p.a[q++] = c->ul + n * c->ncell;
   for (int n = 0; n < 9; ++n) p.a[q++] = c->w32 + n * c->ncell;
   for (int n = 0; n < 3; ++n) p.d[n] = c->b + n * c->ncell;
   Launch t(c, "ghost");

@@ -521,41 +521,47 @@ __device__ __forceinline__ double cfl32_cell(const Geo &g, const float *__restri
   return cds_cell(g, wv, fms_ref, i, j, k);
 }
 
+// The ghost fill of the FP32C state: conserved hi words (uh) and FP32 prim
+// (w32) of every variable, and the face fields.  The low words (ul) get no
+// ghosts: only the corrector reads them, and of the cells it updates only the
+// active ones keep their result (the fill overwrites the others' hi words and
+// prim after the step).
+constexpr int NGHOST32 = 6 + NMODE;
 struct Ghost32 {
-  float *a[21];   // uh, ul, w32
-  double *d[3];   // face fields
+  float *a[NGHOST32];  // uh (6), w32 (9)
+  double *d[3];        // face fields
 };
 
 // One (ghost, source) pair: sources are active cells the fill never writes,
 // so every value is loaded before any is stored.
 __device__ __forceinline__ void ghost32_pair(const Ghost32 &p, int dst, int src) {
-  float v[21];
+  float v[NGHOST32];
   double f[3];
 #pragma unroll
-  for (int n = 0; n < 21; ++n) v[n] = p.a[n][src];
+  for (int n = 0; n < NGHOST32; ++n) v[n] = p.a[n][src];
 #pragma unroll
   for (int n = 0; n < 3; ++n) f[n] = p.d[n][src];
 #pragma unroll
-  for (int n = 0; n < 21; ++n) p.a[n][dst] = v[n];
+  for (int n = 0; n < NGHOST32; ++n) p.a[n][dst] = v[n];
 #pragma unroll
   for (int n = 0; n < 3; ++n) p.d[n][dst] = f[n];
 }
 
 // A ghost cell a reflecting side touches (GhostOp); the sign bit of slot v
-// applies to conserved slot v of uh/ul (momentum flips with velocity) and to
+// applies to conserved slot v of uh (momentum flips with velocity) and to
 // prim slot v of w32.
 __device__ __forceinline__ void ghost32_op(const Ghost32 &p, const GhostOp &o) {
   const bool cells = !(o.bits & (1u << 15));
-  float v[21];
+  float v[NGHOST32];
   double f[3];
 #pragma unroll
-  for (int n = 0; n < 21; ++n) v[n] = cells ? p.a[n][o.src] : 0.f;
+  for (int n = 0; n < NGHOST32; ++n) v[n] = cells ? p.a[n][o.src] : 0.f;
 #pragma unroll
   for (int a = 0; a < 3; ++a) f[a] = ((o.bits >> (12 + a)) & 1u) ? 0. : p.d[a][o.srcf[a]];
   if (cells) {
 #pragma unroll
-    for (int n = 0; n < 21; ++n) {
-      const int slot = n < 12 ? n % 6 : n - 12;
+    for (int n = 0; n < NGHOST32; ++n) {
+      const int slot = n < 6 ? n : n - 6;
       p.a[n][o.dst] = ((o.bits >> slot) & 1u) ? -v[n] : v[n];
     }
   }
