@@ -45,13 +45,13 @@ KERNEL_BYTES = {
 # WRITE_SIZE x 1.0, scaled by a plain device copy), from the rocprofv3 --pmc
 # runs summarised in the named profile; the bench itself runs no counters.
 PROFILED_TRAFFIC = {
-    ("fp64", "flux3_plm"): (6.019e9, "profiles/vl_step_kernels_turb256.md"),
+    ("fp64", "flux3_plm"): (6.008e9, "profiles/vl_step_kernels_turb256.md"),
     ("fp32c", "update_corr"): (4.773e9, "profiles/vl_step_kernels_turb256.md"),
 }
 # Fraction of the sampled SIMDs' cycles spent issuing VALU work in the same
 # kernels (SQ_ACTIVE_INST_VALU x 4 / (GRBM_GUI_ACTIVE x SIMDs), same profile):
 # near 1 the launch is bound by arithmetic issue rather than by HBM.
-PROFILED_VALU_BUSY = {("fp64", "flux3_plm"): 0.71, ("fp32c", "update_corr"): 0.33}
+PROFILED_VALU_BUSY = {("fp64", "flux3_plm"): 0.72, ("fp32c", "update_corr"): 0.33}
 # Algorithmic bytes per cell-update of a whole VL step: 5 state volumes of 9
 # words (FP64 360 B; FP32C 252 B with hi + lo on the stage-2 read of S_n and
 # the write of S_{n+1}).
