@@ -255,6 +255,105 @@ def linear_wave(n, wave="fast", amp=1e-6, ng=3):
     return s, prim, bfld
 
 
+# Eigenvectors of the linear waves of the state rho=1, p=0.6, B=(1, sqrt 2,
+# 1/2), gamma=5/3 in the frame (normal, first transverse, second transverse),
+# per unit amplitude; every mode travels towards -normal at its speed.
+_R5 = math.sqrt(5.0)
+WAVES = {
+    "fast": (2.0, dict(rho=1 / _R5, vn=-2 / _R5, v1=2 * math.sqrt(2) / 3 / _R5, v2=1.0 / 3 / _R5,
+                       b1=4 * math.sqrt(2) / 3 / _R5, b2=2.0 / 3 / _R5, p=1.0 / _R5)),
+    "alfven": (1.0, dict(rho=0.0, vn=0.0, v1=-1.0 / 3, v2=2 * math.sqrt(2) / 3, b1=-1.0 / 3,
+                         b2=2 * math.sqrt(2) / 3, p=0.0)),
+    "slow": (0.5, dict(rho=2 / _R5, vn=-1 / _R5, v1=-4 * math.sqrt(2) / 3 / _R5,
+                       v2=-2.0 / 3 / _R5, b1=-2 * math.sqrt(2) / 3 / _R5, b2=-1.0 / 3 / _R5,
+                       p=2.0 / _R5)),
+}
+
+
+def oblique_frame(dims):
+    """Unit vectors (normal, e1, e2) of a right-handed frame whose normal is
+    the diagonal of the unit box in ``dims`` dimensions: (1,0,0) in 1D,
+    (1,1,0)/sqrt 2 in 2D, (1,1,1)/sqrt 3 in 3D, with e2 = normal x e1."""
+    if dims == 1:
+        kn = np.array([1.0, 0.0, 0.0])
+        e1 = np.array([0.0, 1.0, 0.0])
+    elif dims == 2:
+        kn = np.array([1.0, 1.0, 0.0]) / math.sqrt(2.0)
+        e1 = np.array([-1.0, 1.0, 0.0]) / math.sqrt(2.0)
+    else:
+        kn = np.array([1.0, 1.0, 1.0]) / math.sqrt(3.0)
+        e1 = np.array([-1.0, 1.0, 0.0]) / math.sqrt(2.0)
+    return kn, e1, np.cross(kn, e1)
+
+
+def linear_wave_oblique(n, wave="fast", dims=2, amp=1e-6, t=0.0, ng=3):
+    """Linear MHD eigenmode (the waves of linear_wave) propagating along the
+    diagonal of the periodic unit box in ``dims`` dimensions (n cells per
+    axis), with wave vector k = 2 pi (1, 1, 1)[:dims], so one wavelength is
+    1/sqrt(dims) along the diagonal; the state at time ``t`` of the exact
+    linear solution, amp * cos(k.x + omega t) times the eigenvector rotated
+    onto oblique_frame(dims).  Cell variables are point values at the cell
+    centres; the background field sits on the faces as it is, and the
+    perturbation of the field is the discrete curl of an edge-centred
+    vector potential, so the face field is solenoidal to round-off.
+    Returns (shape, prim, bfld, period)."""
+    speed, ev = WAVES[wave]
+    kn, e1, e2 = oblique_frame(dims)
+    kvec = 2 * math.pi * np.array([1.0 if a < dims else 0.0 for a in range(3)])
+    kmag = float(np.linalg.norm(kvec))
+    omega = speed * kmag
+    s = GridShape(n, n if dims >= 2 else 1, n if dims >= 3 else 1, ng)
+    prim, bfld = s.empty()
+    h = 1.0 / n
+    c = (np.arange(n) + 0.5) * h  # centres
+    f = np.arange(n) * h          # lower faces
+    one = np.zeros(1)
+
+    def axes(xs, ys, zs):  # arrays [k][j][i]
+        Z, Y, X = np.meshgrid(zs if dims >= 3 else one, ys if dims >= 2 else one, xs,
+                              indexing="ij")
+        return X, Y, Z
+
+    def phase(X, Y, Z):
+        return kvec[0] * X + kvec[1] * Y + kvec[2] * Z + omega * t
+
+    X, Y, Z = axes(c, c, c)
+    a = amp * np.cos(phase(X, Y, Z))
+    b0 = 1.0 * kn + math.sqrt(2.0) * e1 + 0.5 * e2
+    dv = ev["vn"] * kn + ev["v1"] * e1 + ev["v2"] * e2
+    db = ev["b1"] * e1 + ev["b2"] * e2
+    _put(prim[RHO], s, 1.0 + ev["rho"] * a)
+    _put(prim[PR], s, 0.6 + ev["p"] * a)
+    for q, var in enumerate((VX, VY, VZ)):
+        _put(prim[var], s, dv[q] * a)
+    # perturbation of B: curl of A = Ahat sin(k.x + omega t) is
+    # cos(.) k x Ahat, which equals db cos(.) for Ahat = -(k x db) / |k|^2
+    ahat = -amp * np.cross(kvec, db) / kmag ** 2
+    # edges: A_x at (centre x, face y, face z), A_y at (face x, centre y, face z),
+    # A_z at (face x, face y, centre z); absent axes sit at 0
+    ax = ahat[0] * np.sin(phase(*axes(c, f, f)))
+    ay = ahat[1] * np.sin(phase(*axes(f, c, f)))
+    az = ahat[2] * np.sin(phase(*axes(f, f, c)))
+
+    def up(arr, axis):  # value at index + 1 (periodic) along grid axis 0=x
+        if axis >= dims:
+            return arr
+        return np.roll(arr, -1, axis=2 - axis)
+
+    bx = (up(az, 1) - az) / h - (up(ay, 2) - ay) / h
+    by = (up(ax, 2) - ax) / h - (up(az, 0) - az) / h
+    bz = (up(ay, 0) - ay) / h - (up(ax, 1) - ax) / h
+    # components along absent axes: the face value is the point value there
+    if dims < 2:
+        by = db[1] * a
+    if dims < 3:
+        bz = db[2] * a
+    _put(bfld[0], s, b0[0] + bx)
+    _put(bfld[1], s, b0[1] + by)
+    _put(bfld[2], s, b0[2] + bz)
+    return s, prim, bfld, 1.0 / math.sqrt(dims) / speed
+
+
 def blast2d(n, ng=3, seed=5005, p_in=10.0, p_out=0.1, radius=0.1, theta=None, bmag=None):
     """2D MHD blast on the periodic unit square: rho=1, p=p_out, uniform B at
     a random angle with |B| in [0.5, 2], p=p_in inside r<radius of the centre

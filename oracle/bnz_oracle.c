@@ -820,6 +820,16 @@ static void ec_from_prim(or_sim *s, const double *w, const int *lims)
       }
 }
 
+/* Weight of the cell-centred EMFs in interp_ee1's corner average
+   (ct.pyx:181-205 subtracts 0.25 * the sum of four).  Only a mutant build
+   for the oracle's own sensitivity test (oracle/Makefile `mutants`, built
+   with -DOR_MUTANT_EC_SIGN) flips its sign; every other build uses 0.25. */
+#ifdef OR_MUTANT_EC_SIGN
+#define OR_EC_W (-0.25)
+#else
+#define OR_EC_W 0.25
+#endif
+
 /* interp_ee1: src/integration/ct.pyx:153-215 */
 static void interp_ee1(or_sim *s, const int *lims)
 {
@@ -835,15 +845,15 @@ static void interp_ee1(or_sim *s, const int *lims)
         for (int i = il; i <= iu; ++i) {
           A(ee, 0, k, j, i) = (0.5 * (-A(fy, BZ, k - 1, j, i) - A(fy, BZ, k, j, i) +
                                       A(fz, BY, k, j - 1, i) + A(fz, BY, k, j, i)) -
-                               0.25 * (A(ec, 0, k, j - 1, i) + A(ec, 0, k, j, i) +
+                               OR_EC_W * (A(ec, 0, k, j - 1, i) + A(ec, 0, k, j, i) +
                                        A(ec, 0, k - 1, j - 1, i) + A(ec, 0, k - 1, j, i)));
           A(ee, 1, k, j, i) = (0.5 * (-A(fz, BX, k, j, i - 1) - A(fz, BX, k, j, i) +
                                       A(fx, BZ, k - 1, j, i) + A(fx, BZ, k, j, i)) -
-                               0.25 * (A(ec, 1, k, j, i - 1) + A(ec, 1, k, j, i) +
+                               OR_EC_W * (A(ec, 1, k, j, i - 1) + A(ec, 1, k, j, i) +
                                        A(ec, 1, k - 1, j, i - 1) + A(ec, 1, k - 1, j, i)));
           A(ee, 2, k, j, i) = (0.5 * (-A(fx, BY, k, j - 1, i) - A(fx, BY, k, j, i) +
                                       A(fy, BX, k, j, i - 1) + A(fy, BX, k, j, i)) -
-                               0.25 * (A(ec, 2, k, j, i - 1) + A(ec, 2, k, j, i) +
+                               OR_EC_W * (A(ec, 2, k, j, i - 1) + A(ec, 2, k, j, i) +
                                        A(ec, 2, k, j - 1, i - 1) + A(ec, 2, k, j - 1, i)));
         }
   } else if (s->dims == 2) {
@@ -853,7 +863,7 @@ static void interp_ee1(or_sim *s, const int *lims)
         A(ee, 1, 0, j, i) = A(fx, BZ, 0, j, i);
         A(ee, 2, 0, j, i) = (0.5 * (-A(fx, BY, 0, j - 1, i) - A(fx, BY, 0, j, i) +
                                     A(fy, BX, 0, j, i - 1) + A(fy, BX, 0, j, i)) -
-                             0.25 * (A(ec, 2, 0, j, i - 1) + A(ec, 2, 0, j, i) +
+                             OR_EC_W * (A(ec, 2, 0, j, i - 1) + A(ec, 2, 0, j, i) +
                                      A(ec, 2, 0, j - 1, i - 1) + A(ec, 2, 0, j - 1, i)));
       }
   } else {

@@ -16,7 +16,7 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
-_lib = None
+_libs = {}
 
 RS_IDS = {"hll": 0, "hllc": 1, "hlld": 2, "hlla": 3, "hllt": 4}
 
@@ -28,12 +28,21 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(_LIB_PATH):
+def mutant_path(name):
+    """Path of a mutant oracle (``make mutants``: one planted mistake, for the
+    oracle's own sensitivity tests), built on first use."""
+    path = os.path.join(_HERE, "_mut", f"liboracle_{name}.so")
+    subprocess.run(["make", "-s", "-C", _HERE, os.path.join("_mut", f"liboracle_{name}.so")],
+                   check=True)
+    return path
+
+
+def lib(path=None):
+    path = path or _LIB_PATH
+    if path not in _libs:
+        if path == _LIB_PATH and not os.path.exists(_LIB_PATH):
             build()
-        L = ctypes.CDLL(_LIB_PATH)
+        L = ctypes.CDLL(path)
         L.or_create.restype = ctypes.c_void_p
         L.or_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                 _dp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_int,
@@ -80,8 +89,8 @@ def lib():
             getattr(L, name).argtypes = [_dpp, _dpp, ctypes.POINTER(_dpp),
                                          ctypes.POINTER(_dpp), ctypes.c_int, ctypes.c_int,
                                          ctypes.c_int]
-        _lib = L
-    return _lib
+        _libs[path] = L
+    return _libs[path]
 
 
 def _ptr(a):
@@ -173,8 +182,8 @@ class OracleSim:
 
     def __init__(self, nx, ny=1, nz=1, reconstr_order=1, lmin=(0., 0., 0.),
                  lmax=(1., 1., 1.), gam=5. / 3., cour=0.4, rsolver="hlld",
-                 bc=(0, 0, 0, 0, 0, 0), nthreads=None, dt_fms_ref=True):
-        L = lib()
+                 bc=(0, 0, 0, 0, 0, 0), nthreads=None, dt_fms_ref=True, lib_path=None):
+        L = self._L = lib(lib_path)
         lmin = np.asarray(lmin, dtype=np.float64)
         lmax = np.asarray(lmax, dtype=np.float64)
         bcs = (ctypes.c_int * 6)(*bc)
@@ -192,46 +201,46 @@ class OracleSim:
 
     def coord(self, which, axis):
         n = self.ntot[axis] + (1 if which == 0 else 0)
-        return np.ctypeslib.as_array(lib().or_coord(self._h, which, axis), shape=(n,)).copy()
+        return np.ctypeslib.as_array(self._L.or_coord(self._h, which, axis), shape=(n,)).copy()
 
     def init(self):
-        lib().or_init(self._h)
+        self._L.or_init(self._h)
 
     def apply_bc(self):
-        lib().or_apply_grid_bc(self._h)
+        self._L.or_apply_grid_bc(self._h)
 
     def new_dt(self):
-        return lib().or_new_dt(self._h)
+        return self._L.or_new_dt(self._h)
 
     def vl_step(self, dt):
-        lib().or_vl_step(self._h, dt)
+        self._L.or_vl_step(self._h, dt)
 
     def advance(self, tmax, max_steps=0):
-        return lib().or_advance(self._h, tmax, max_steps)
+        return self._L.or_advance(self._h, tmax, max_steps)
 
     def advance_steps(self, n):
-        return lib().or_advance_steps(self._h, n)
+        return self._L.or_advance_steps(self._h, n)
 
     @property
     def time(self):
-        return lib().or_time(self._h)
+        return self._L.or_time(self._h)
 
     @property
     def step(self):
-        return lib().or_step(self._h)
+        return self._L.or_step(self._h)
 
     @property
     def last_dt(self):
-        return lib().or_last_dt(self._h)
+        return self._L.or_last_dt(self._h)
 
     def history(self):
         out = np.zeros(16)
-        lib().or_history(self._h, _ptr(out))
+        self._L.or_history(self._h, _ptr(out))
         return out
 
     def close(self):
         if getattr(self, "_h", None):
-            lib().or_destroy(self._h)
+            self._L.or_destroy(self._h)
             self._h = None
             self.prim = self.cons = self.bfld = None
 

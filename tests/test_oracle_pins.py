@@ -429,6 +429,155 @@ def test_linear_fast_wave_second_order():
     assert errs[0] / errs[1] > 3.2 and errs[1] / errs[2] > 3.2
 
 
+# ------------------------------------------------- multi-D constrained transport
+def wave_error(dims, n, wave, t, lib_path=None):
+    """L1 error norm sqrt(sum_v mean|q_v - exact_v|^2) over rho, v, p and the
+    three face fields of an oblique linear wave run to time t on the oracle,
+    against the exact linear solution at t (problems.linear_wave_oblique)."""
+    s, prim, bfld, _ = P.linear_wave_oblique(n, wave, dims, amp=1e-6)
+    _, pe, be, _ = P.linear_wave_oblique(n, wave, dims, amp=1e-6, t=t)
+    o = orc.OracleSim(s.nx, s.ny, s.nz, cour=0.4, lib_path=lib_path)
+    o.prim[:] = prim
+    o.bfld[:] = bfld
+    o.init()
+    o.advance(t)
+    assert o.time == t
+    sk, sj, si = s.active()
+    e = 0.0
+    for v in (0, 1, 2, 3, 4):
+        e += np.mean(np.abs(o.prim[v][sk, sj, si] - pe[v][sk, sj, si])) ** 2
+    for c in range(3):
+        e += np.mean(np.abs(o.bfld[c][sk, sj, si] - be[c][sk, sj, si])) ** 2
+    o.close()
+    return math.sqrt(e)
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("wave", ["alfven", "slow"])
+def test_linear_wave_1d_second_order(wave):
+    """Alfven and slow eigenmodes in 1D after one period, against the exact
+    solution: the error falls by ~4x per doubling (the fast one is above)."""
+    _, _, _, period = P.linear_wave_oblique(8, wave, 1)
+    errs = [wave_error(1, n, wave, period) for n in (32, 64, 128)]
+    assert errs[0] / errs[1] > 3.2 and errs[1] / errs[2] > 3.2, errs
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("dims,sizes", [(2, (32, 64, 128)), (3, (16, 32, 64))])
+@pytest.mark.parametrize("wave", ["fast", "alfven", "slow"])
+def test_oblique_linear_wave_second_order(dims, sizes, wave):
+    """Fast, Alfven and slow eigenmodes travelling along the diagonal of a
+    periodic 2D square or 3D cube (wave vector 2 pi (1, 1[, 1])), compared with
+    the exact linear solution: every corner EMF of interp_ee1 (ct.pyx:170-205),
+    both the face-flux and the cell-EMF terms, moves the field, so a wrong sign,
+    weight or index in any of them breaks the second-order convergence.  The
+    time is one fast period in 2D and half of one in 3D (slower waves cover a
+    fraction of their period), to keep the 64^3 runs short."""
+    _, _, _, tfast = P.linear_wave_oblique(8, "fast", dims)
+    t = tfast if dims == 2 else 0.5 * tfast
+    errs = [wave_error(dims, n, wave, t) for n in sizes]
+    assert errs[0] / errs[1] > 3.2 and errs[1] / errs[2] > 3.2, errs
+    assert errs[-1] < 1e-8      # and small against the amplitude 1e-6
+
+
+def place_tube(prob1, dims, axis, transverse=4):
+    """The state of a 1D tube (x, outflow) laid along ``axis`` of a grid of
+    ``dims`` dimensions with ``transverse`` cells on the other axes: the
+    tube's (normal, first, second transverse) components of v and B go to
+    axes (axis, axis+1, axis+2) mod 3, a cyclic rotation."""
+    s1, p1, b1 = prob1
+    n = s1.nx
+    sizes = [transverse if a < dims else 1 for a in range(3)]
+    sizes[axis] = n
+    s = P.GridShape(*sizes, ng=s1.ng)
+    prim, bfld = s.empty()
+    sk, sj, si = s.active()
+    prof = p1[:, 0, 0, s1.ng:s1.ng + n]
+    bprof = b1[:, 0, 0, s1.ng:s1.ng + n]
+    shape = [1, 1, 1]
+    shape[2 - axis] = n
+    ax3 = [axis, (axis + 1) % 3, (axis + 2) % 3]
+    for v in (0, 4, 5):
+        prim[v][sk, sj, si] = prof[v].reshape(shape)
+    for q in range(3):
+        prim[1 + ax3[q]][sk, sj, si] = prof[1 + q].reshape(shape)
+        bfld[ax3[q]][sk, sj, si] = bprof[q].reshape(shape)
+    bc = [0] * 6
+    bc[2 * axis] = bc[2 * axis + 1] = 1
+    return s, prim, bfld, tuple(bc), ax3
+
+
+def planar_deviation(dims, axis, steps=20, lib_path=None):
+    """Largest deviation, relative to the largest magnitude of each variable,
+    between a seeded 1D MHD Riemann tube stepped on a 1D grid and the same
+    tube laid along ``axis`` of a 2D or 3D grid, stepped with the same fixed
+    dt.  Planar data make every transverse flux difference vanish, and the
+    multi-D corner EMF (1/2 of the four face EMFs minus 1/4 of the four cell
+    EMFs, ct.pyx:170-205) then reduces to the 1D edge EMF (ct.pyx:211-215),
+    so the two runs agree to round-off in every active cell."""
+    rng = np.random.Generator(np.random.PCG64(77))
+    prob1 = P.riemann_tube(48, rng)
+    s1 = prob1[0]
+    o1 = orc.OracleSim(s1.nx, 1, 1, bc=(1,) * 6)
+    o1.prim[:] = prob1[1]
+    o1.bfld[:] = prob1[2]
+    o1.init()
+    dt = 0.8 * o1.new_dt()
+    for _ in range(steps):
+        o1.vl_step(dt)
+    s, prim, bfld, bc, ax3 = place_tube(prob1, dims, axis)
+    o = orc.OracleSim(s.nx, s.ny, s.nz, bc=bc, lib_path=lib_path)
+    o.prim[:] = prim
+    o.bfld[:] = bfld
+    o.init()
+    for _ in range(steps):
+        o.vl_step(dt)
+    g = s1.ng
+    ref_p = o1.prim[:, 0, 0, g:g + s1.nx]
+    ref_b = o1.bfld[:, 0, 0, g:g + s1.nx]
+    sk, sj, si = s.active()
+    worst = 0.0
+    perm = {0: 0, 4: 4, 5: 5}
+    for q in range(3):
+        perm[1 + q] = 1 + ax3[q]
+        perm[6 + q] = 6 + ax3[q]
+    for v in range(9):
+        got = np.moveaxis(o.prim[perm[v]][sk, sj, si], 2 - axis, -1)
+        scale = max(float(np.max(np.abs(ref_p[v]))), 1e-300)
+        worst = max(worst, float(np.max(np.abs(got - ref_p[v]))) / scale)
+    for q in range(3):
+        got = np.moveaxis(o.bfld[ax3[q]][sk, sj, si], 2 - axis, -1)
+        scale = max(float(np.max(np.abs(ref_b[q]))), 1e-300)
+        worst = max(worst, float(np.max(np.abs(got - ref_b[q]))) / scale)
+    # the tube evolved (so the comparison is not of two initial states)
+    assert np.max(np.abs(ref_p[0] - prob1[1][0, 0, 0, g:g + s1.nx])) > 1e-2
+    o.close()
+    o1.close()
+    return worst
+
+
+@pytest.mark.parametrize("dims,axis", [(2, 0), (2, 1), (3, 0), (3, 1), (3, 2)])
+def test_planar_tube_in_2d_3d_equals_1d(dims, axis):
+    assert planar_deviation(dims, axis) <= 1e-13
+
+
+def test_pins_fail_on_flipped_cell_emf_sign():
+    """Sensitivity of the two pins above: an oracle built with the sign of
+    the -1/4 sum of cell EMFs flipped (oracle/Makefile `mutants`) fails both:
+    the planar identity by orders of magnitude, and the oblique waves, whose
+    error no longer falls with resolution."""
+    mut = orc.mutant_path("ecsign")
+    assert planar_deviation(3, 1, steps=5) <= 1e-13
+    assert planar_deviation(3, 1, steps=5, lib_path=mut) > 1e-3
+    _, _, _, tfast = P.linear_wave_oblique(8, "fast", 2)
+    for wave in ("fast", "alfven"):
+        good = [wave_error(2, n, wave, 0.25 * tfast) for n in (16, 32)]
+        bad = [wave_error(2, n, wave, 0.25 * tfast, lib_path=mut) for n in (16, 32)]
+        assert good[0] / good[1] > 3.2
+        # the mutant does not converge: its error stays near the amplitude
+        assert bad[0] / bad[1] < 1.5 and bad[1] > 10 * good[1], (good, bad)
+
+
 def _wall_state(n, m, axis, g=3):
     """2D state on [0,1]^2, n x m cells, periodic along the other axis, whose
     face-normal field vanishes on the two walls of ``axis`` (B from a corner
