@@ -106,26 +106,42 @@ def orszag_tang(n, ng=3, b_eps=1e-20):
     return s, prim, bfld
 
 
-def _kgrid(n):
-    k = np.fft.fftfreq(n, d=1.0 / n)
-    return k
-
-
-def _band_field(rng, n, slope, kmin, kmax, ncomp):
-    """Real Gaussian fields with |f_k| proportional to k**slope on the shell
-    band kmin <= |k| <= kmax (integer wavenumbers of a unit box), from white
-    noise filtered in Fourier space.  Returns complex spectra (ncomp, n, n, n)."""
-    k = _kgrid(n)
-    KZ, KY, KX = np.meshgrid(k, k, k, indexing="ij")
+def _band_modes(rng, slope, kmin, kmax, ncomp):
+    """Seeded complex Gaussian coefficients c_k (unit variance of the real
+    and imaginary parts) times |k|**slope on the shell band kmin <= |k| <=
+    kmax of integer wavenumbers, zero elsewhere.  Returns (coef, kvec):
+    coef has shape (ncomp, 2K+1, 2K+1, 2K+1) over (kz, ky, kx) in [-K, K]
+    with K = floor(kmax), kvec the three wavenumber grids of that shape."""
+    K = int(math.floor(kmax))
+    k1 = np.arange(-K, K + 1, dtype=np.float64)
+    KZ, KY, KX = np.meshgrid(k1, k1, k1, indexing="ij")
     kk = np.sqrt(KX ** 2 + KY ** 2 + KZ ** 2)
     band = (kk >= kmin) & (kk <= kmax)
     amp = np.zeros_like(kk)
     amp[band] = kk[band] ** slope
-    out = []
-    for _ in range(ncomp):
-        noise = rng.standard_normal((n, n, n))
-        out.append(np.fft.fftn(noise) * amp)
-    return np.array(out), (KX, KY, KZ)
+    shp = (ncomp,) + kk.shape
+    coef = (rng.standard_normal(shp) + 1j * rng.standard_normal(shp)) * amp
+    return coef, (KX, KY, KZ)
+
+
+def _synth(coef, n, stagger):
+    """Real field f(x) = Re sum_k coef_k exp(2 pi i k.x) of one coefficient
+    cube (from _band_modes) on the n^3 grid of the unit box, [k][j][i]
+    layout.  stagger[a] = 0 samples axis a at the cell centres (i+1/2)/n,
+    1 at the lower faces i/n.  Evaluated as three separable sums, one axis
+    at a time (the band has few modes), instead of an n^3 inverse FFT."""
+    K = (coef.shape[0] - 1) // 2
+    k1 = np.arange(-K, K + 1, dtype=np.float64)
+    ex = [np.exp(2j * math.pi * np.outer((np.arange(n) + (0.0 if stagger[a] else 0.5)) / n, k1))
+          for a in range(3)]                                  # [position, k] per axis
+    t = np.einsum("zyk,xk->zyx", coef, ex[0])                 # (kz, ky, x)
+    t = np.einsum("zkx,yk->zyx", t, ex[1])                    # (kz, y, x)
+    out = np.empty((n, n, n))
+    flat = t.reshape(t.shape[0], n * n)
+    for z0 in range(0, n, 32):                                # slabs of z: bounded memory
+        z1 = min(n, z0 + 32)
+        out[z0:z1] = (ex[2][z0:z1] @ flat).real.reshape(z1 - z0, n, n)
+    return out
 
 
 def turbulence(n, ng=3, seed_b=1001, seed_v=1002, b0=0.2, db_rms=0.05, mach=0.5,
@@ -134,34 +150,33 @@ def turbulence(n, ng=3, seed_b=1001, seed_v=1002, b0=0.2, db_rms=0.05, mach=0.5,
 
     rho=1, p=0.6 (c_s=1 at gamma=5/3), uniform Bx=b0.  The velocity is a
     solenoidal Gaussian field with a k^-2 shell spectrum on kmin<=|k|<=kmax
-    and rms |v| = mach.  The field perturbation is the discrete curl of an
-    edge-centred Gaussian vector potential with |A_k| ~ k^-3 (k^-2 shell
-    spectrum of B) scaled to rms |dB| = db_rms; it is solenoidal to round-off
-    on the staggered grid."""
+    (|v_k| ~ k^-2, coefficients projected perpendicular to k) and rms |v| =
+    mach, sampled at the cell centres.  The field perturbation is the
+    discrete curl of an edge-centred Gaussian vector potential with |A_k| ~
+    k^-3 (k^-2 shell spectrum of B) scaled to rms |dB| = db_rms; it is
+    solenoidal to round-off on the staggered grid.  The random coefficients
+    are drawn per mode of the band from PCG64(seed), so the realisation is
+    the same function of position at every n."""
     s = GridShape(n, n, n, ng)
     prim, bfld = s.empty()
     d = 1.0 / n
-    # velocity: |v_k| ~ k^-2 (shell spectrum 4 pi k^2 |v_k|^2 ~ k^-2)
     rng = np.random.Generator(np.random.PCG64(seed_v))
-    vk, (KX, KY, KZ) = _band_field(rng, n, -2.0, kmin, kmax, 3)
+    vk, (KX, KY, KZ) = _band_modes(rng, -2.0, kmin, kmax, 3)
     k2 = KX ** 2 + KY ** 2 + KZ ** 2
-    k2[0, 0, 0] = 1.0
+    k2[k2 == 0] = 1.0
     kdotv = KX * vk[0] + KY * vk[1] + KZ * vk[2]
     vk[0] -= KX * kdotv / k2
     vk[1] -= KY * kdotv / k2
     vk[2] -= KZ * kdotv / k2
-    v = np.real(np.fft.ifftn(vk, axes=(1, 2, 3)))
-    v *= mach / math.sqrt(np.mean(np.sum(v ** 2, axis=0)))
-    del vk, kdotv
+    v = [_synth(vk[c], n, (0, 0, 0)) for c in range(3)]
+    vscale = mach / math.sqrt(np.mean(v[0] ** 2 + v[1] ** 2 + v[2] ** 2))
     # vector potential on cell edges: A_x at (i, j-1/2, k-1/2),
     # A_y at (i-1/2, j, k-1/2), A_z at (i-1/2, j-1/2, k)
     rng = np.random.Generator(np.random.PCG64(seed_b))
-    ak, _ = _band_field(rng, n, -3.0, kmin, kmax, 3)
-    ph = lambda kv: np.exp(-1j * math.pi * kv / n)  # shift by -dx/2 of a unit box grid
-    ax = np.real(np.fft.ifftn(ak[0] * ph(KY) * ph(KZ)))
-    ay = np.real(np.fft.ifftn(ak[1] * ph(KX) * ph(KZ)))
-    az = np.real(np.fft.ifftn(ak[2] * ph(KX) * ph(KY)))
-    del ak, KX, KY, KZ, k2
+    ak, _ = _band_modes(rng, -3.0, kmin, kmax, 3)
+    ax = _synth(ak[0], n, (0, 1, 1))
+    ay = _synth(ak[1], n, (1, 0, 1))
+    az = _synth(ak[2], n, (1, 1, 0))
 
     def curl(ax, ay, az):
         # arrays are [k][j][i]; roll(-1) along an axis reads index+1
@@ -172,17 +187,19 @@ def turbulence(n, ng=3, seed_b=1001, seed_v=1002, b0=0.2, db_rms=0.05, mach=0.5,
 
     bx, by, bz = curl(ax, ay, az)
     # face-to-centre average for the rms of the perturbation
-    cx = 0.5 * (bx + np.roll(bx, -1, axis=2))
-    cy = 0.5 * (by + np.roll(by, -1, axis=1))
-    cz = 0.5 * (bz + np.roll(bz, -1, axis=0))
-    scale = db_rms / math.sqrt(np.mean(cx ** 2 + cy ** 2 + cz ** 2))
-    del cx, cy, cz
+    msq = np.mean((0.5 * (bx + np.roll(bx, -1, axis=2))) ** 2)
+    msq += np.mean((0.5 * (by + np.roll(by, -1, axis=1))) ** 2)
+    msq += np.mean((0.5 * (bz + np.roll(bz, -1, axis=0))) ** 2)
+    scale = db_rms / math.sqrt(msq)
+    del bx, by, bz
     bx, by, bz = curl(ax * scale, ay * scale, az * scale)
+    del ax, ay, az
     _put(prim[RHO], s, rho0)
     _put(prim[PR], s, p0)
-    _put(prim[VX], s, v[0])
-    _put(prim[VY], s, v[1])
-    _put(prim[VZ], s, v[2])
+    _put(prim[VX], s, v[0] * vscale)
+    _put(prim[VY], s, v[1] * vscale)
+    _put(prim[VZ], s, v[2] * vscale)
+    del v
     _put(bfld[0], s, bx + b0)
     _put(bfld[1], s, by)
     _put(bfld[2], s, bz)

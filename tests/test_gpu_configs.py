@@ -1,0 +1,258 @@
+"""The measured configurations at their full size, through the C ABI.
+
+- 3D turbulence 64^3 for 100 steps: FP64 against the oracle;
+- 3D turbulence 256^3 for 100 steps: FP32-compensated against FP64;
+- 3D turbulence 512^3 (the large size of the precision configuration): a few
+  FP32C steps against the FP64 path, both with div B at round-off;
+- the batched tube sweep (512 seeded 1D Riemann tubes of 64-4096 cells) and
+  the batched 2D blast sweep (64 periodic boxes of side 64-512), every box in
+  FP64 and FP32C: each reaches its t_max, stays physical, FP32C stays within
+  1e-5 relative L1 of FP64; boxes sampled from each sweep equal the oracle;
+  the blast boxes conserve mass and energy and keep div B at round-off.
+
+The FP64 single-grid path is tied to the oracle bit for bit by
+tests/test_gpu_parity.py; here the batch boxes are tied to it directly.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from bonzo_amd import problems as P
+
+pytestmark = pytest.mark.gpu
+
+GAM = 5.0 / 3.0
+NAMES = ["rho", "vx", "vy", "vz", "p", "psc", "bxc", "byc", "bzc", "bxf", "byf", "bzf"]
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def rel_l1_vars(s, a, b):
+    """Per-variable relative L1 of state a = (prim, bfld) against b over the
+    active cells (variables whose reference is zero everywhere are skipped)."""
+    sk, sj, si = s.active()
+    out = {}
+    for v in range(12):
+        x = a[0][v][sk, sj, si] if v < 9 else a[1][v - 9][sk, sj, si]
+        y = b[0][v][sk, sj, si] if v < 9 else b[1][v - 9][sk, sj, si]
+        den = float(np.sum(np.abs(y)))
+        if den > 0:
+            out[NAMES[v]] = float(np.sum(np.abs(x - y))) / den
+    return out
+
+
+def divb_max(s, bfld):
+    g = s.ng
+    nx, ny, nz = s.nact
+    sk, sj, si = s.active()
+    div = (bfld[0][sk, sj, g + 1:g + nx + 1] - bfld[0][sk, sj, si]) / s.dx(0)
+    if s.dims >= 2:
+        div = div + (bfld[1][sk, g + 1:g + ny + 1, si] - bfld[1][sk, sj, si]) / s.dx(1)
+    if s.dims >= 3:
+        div = div + (bfld[2][g + 1:g + nz + 1, sj, si] - bfld[2][sk, sj, si]) / s.dx(2)
+    return float(np.abs(div).max())
+
+
+def totals(s, prim):
+    """Active-cell sums of mass and total energy (the cell-centred field is
+    prim's, as the conserved energy uses it)."""
+    sk, sj, si = s.active()
+    w = prim[:, sk, sj, si]
+    v2 = w[1] ** 2 + w[2] ** 2 + w[3] ** 2
+    b2 = w[6] ** 2 + w[7] ** 2 + w[8] ** 2
+    e = 0.5 * (w[0] * v2 + b2) + w[4] / (GAM - 1.0)
+    return float(np.sum(w[0])), float(np.sum(e))
+
+
+def oracle_run(s, prim, bfld, bc, tmax):
+    from oracle.oracle import OracleSim
+    o = OracleSim(s.nx, s.ny, s.nz, bc=bc, nthreads=1 if s.dims == 1 else None)
+    o.prim[:] = prim
+    o.bfld[:] = bfld
+    o.init()
+    n = o.advance(tmax)
+    out = (o.prim.copy(), o.bfld.copy(), o.time, n)
+    o.close()
+    return out
+
+
+def batch_sweep(cases, precision):
+    """Run a sweep as one batch (captured graph); returns the initial and
+    final states of every box and the per-box time and step count."""
+    from bonzo_amd._lib import Batch
+    b = Batch([dict(nx=s.nx, ny=s.ny, nz=s.nz, bc=bc, lmin=s.lmin, lmax=s.lmax, tmax=tm)
+               for (s, _, _), bc, tm in cases], rsolver="hlld", reconstr="linear", cour=0.4,
+              precision=precision)
+    for q, ((s, prim, bfld), _, _) in enumerate(cases):
+        b.upload(q, prim, bfld)
+    b.init_state()
+    init = [b.download(q) for q in range(len(cases))]
+    n = b.advance(use_graph=True)
+    t, steps = b.times()
+    out = [b.download(q) for q in range(len(cases))]
+    b.close()
+    say(f"  {precision}: {n} step launches, steps per box {steps.min()}..{steps.max()}")
+    return init, out, t, steps
+
+
+def sweep_rel_l1(cases, a, b):
+    num = np.zeros(12)
+    den = np.zeros(12)
+    for q, ((s, _, _), _, _) in enumerate(cases):
+        sk, sj, si = s.active()
+        for v in range(12):
+            x = a[q][0][v][sk, sj, si] if v < 9 else a[q][1][v - 9][sk, sj, si]
+            y = b[q][0][v][sk, sj, si] if v < 9 else b[q][1][v - 9][sk, sj, si]
+            num[v] += float(np.sum(np.abs(x - y)))
+            den[v] += float(np.sum(np.abs(y)))
+    return {NAMES[v]: num[v] / den[v] for v in range(12) if den[v] > 0}
+
+
+def check_physical(cases, out, t):
+    for q, ((s, _, _), _, tm) in enumerate(cases):
+        assert t[q] == tm, (q, t[q], tm)
+        sk, sj, si = s.active()
+        w = out[q][0][:, sk, sj, si]
+        assert np.all(np.isfinite(w)) and np.all(np.isfinite(out[q][1][:, sk, sj, si])), q
+        assert np.all(w[0] > 0) and np.all(w[4] > 0), q
+
+
+def check_vs_oracle(cases, out, t, steps, picks):
+    for q in picks:
+        (s, prim, bfld), bc, tm = cases[q]
+        op, ob, ot, on = oracle_run(s, prim, bfld, bc, tm)
+        assert steps[q] == on and t[q] == ot, (q, steps[q], on, t[q], ot)
+        l1 = rel_l1_vars(s, out[q], (op, ob))
+        sk, sj, si = s.active()
+        bitwise = (np.array_equal(out[q][0][:, sk, sj, si], op[:, sk, sj, si]) and
+                   np.array_equal(out[q][1][:, sk, sj, si], ob[:, sk, sj, si]))
+        say(f"  box {q} ({s.nx}x{s.ny}): {on} steps, worst rel L1 vs oracle "
+            f"{max(l1.values()):.3e}, bitwise={bitwise}")
+        assert max(l1.values()) <= 1e-12, (q, l1)
+
+
+def test_tube_sweep_512_full_size():
+    """The batched 1D sweep at full size: 512 tubes, FP64 and FP32C to t=0.1."""
+    say("case: tube sweep, 512 tubes of 64..4096 cells (seed 4004), HLLD+PLM, to t=0.1, "
+        "FP64 and FP32C, graph")
+    cases = [(c, (1,) * 6, 0.1) for c in P.tube_sweep(512, seed=4004)]
+    _, out64, t64, st64 = batch_sweep(cases, "fp64")
+    check_physical(cases, out64, t64)
+    _, out32, t32, _ = batch_sweep(cases, "fp32c")
+    check_physical(cases, out32, t32)
+    l1 = sweep_rel_l1(cases, out32, out64)
+    say(f"  FP32C vs FP64: worst rel L1 {max(l1.values()):.3e} ({max(l1, key=l1.get)})")
+    assert max(l1.values()) <= 1e-5, l1
+    # eight tubes against the oracle: the shortest, the longest, six seeded picks
+    lengths = np.array([c[0][0].nx for c in cases])
+    rng = np.random.Generator(np.random.PCG64(44))
+    picks = [int(np.argmin(lengths)), int(np.argmax(lengths))]
+    picks += [int(q) for q in rng.choice(len(cases), 6, replace=False) if q not in picks][:6]
+    check_vs_oracle(cases, out64, t64, st64, picks)
+
+
+def test_blast_sweep_64_full_size():
+    """The batched 2D sweep at full size: 64 periodic blast boxes of side
+    64..512, FP64 and FP32C to t=0.2: per box, mass and energy conserved and
+    div B at round-off; FP32C within 1e-5 of FP64; the two smallest boxes
+    equal the oracle."""
+    say("case: blast sweep, 64 periodic 2D boxes of side 64..512 (seed 5005), HLLD+PLM+CT, "
+        "to t=0.2, FP64 and FP32C, graph")
+    cases = [(c, (0,) * 6, 0.2) for c in P.blast_sweep(64, seed=5005)]
+    res = {}
+    for prec in ("fp64", "fp32c"):
+        init, out, t, steps = batch_sweep(cases, prec)
+        check_physical(cases, out, t)
+        worst_div = worst_m = worst_e = 0.0
+        for q, ((s, _, _), _, _) in enumerate(cases):
+            worst_div = max(worst_div, divb_max(s, out[q][1]))
+            m0, e0 = totals(s, init[q][0])
+            m1, e1 = totals(s, out[q][0])
+            worst_m = max(worst_m, abs(m1 - m0) / abs(m0))
+            worst_e = max(worst_e, abs(e1 - e0) / abs(e0))
+        say(f"  {prec}: max div B {worst_div:.3e}, mass drift {worst_m:.3e}, "
+            f"energy drift {worst_e:.3e}")
+        assert worst_div <= 1e-12
+        assert worst_m <= 1e-12 and worst_e <= 1e-12
+        res[prec] = (out, t, steps)
+    l1 = sweep_rel_l1(cases, res["fp32c"][0], res["fp64"][0])
+    say(f"  FP32C vs FP64: worst rel L1 {max(l1.values()):.3e} ({max(l1, key=l1.get)})")
+    assert max(l1.values()) <= 1e-5, l1
+    sides = np.array([c[0][0].nx for c in cases])
+    picks = [int(q) for q in np.argsort(sides, kind="stable")[:2]]
+    out, t, steps = res["fp64"]
+    check_vs_oracle(cases, out, t, steps, picks)
+
+
+def test_turbulence_64_100_steps_bitwise():
+    """The turbulence configuration's 100 steps, at 64^3 so that the oracle
+    runs them in seconds: the FP64 path equals the oracle after every one."""
+    from bonzo_amd._lib import Context
+    from oracle.oracle import OracleSim
+    say("case: turbulence 64^3, 100 steps, hlld+plm, periodic, FP64 vs oracle")
+    s, prim, bfld = P.turbulence(64)
+    g = Context(64, 64, 64)
+    g.upload(prim, bfld)
+    g.init_state()
+    g.run_steps(100)
+    gp, gb = g.download()
+    t, n, _ = g.time()
+    g.close()
+    o = OracleSim(64, 64, 64, nthreads=16)
+    o.prim[:] = prim
+    o.bfld[:] = bfld
+    o.init()
+    o.advance_steps(100)
+    assert n == o.step == 100 and t == o.time
+    l1 = rel_l1_vars(s, (gp, gb), (o.prim, o.bfld))
+    sk, sj, si = s.active()
+    bitwise = (np.array_equal(gp[:, sk, sj, si], o.prim[:, sk, sj, si]) and
+               np.array_equal(gb[:, sk, sj, si], o.bfld[:, sk, sj, si]))
+    say(f"  worst rel L1 {max(l1.values()):.3e}, bitwise={bitwise}")
+    assert max(l1.values()) <= 1e-12
+    o.close()
+
+
+def _turb_pair(n, steps):
+    from bonzo_amd._lib import Context
+    s, prim, bfld = P.turbulence(n)
+    say(f"  initial state {n}^3 generated")
+    res = {}
+    for prec in ("fp64", "fp32c"):
+        g = Context(n, n, n, precision=prec)
+        g.upload(prim, bfld)
+        g.init_state()
+        g.run_steps(steps)
+        divb = g.diag()["divb_max"]
+        res[prec] = (g.download(), divb)
+        g.close()
+        say(f"  {prec}: {steps} steps, max div B {divb:.3e}")
+    return s, res
+
+
+def test_turbulence_256_100_steps_fp32c_vs_fp64():
+    """The precision configuration at 256^3 for its 100 steps: FP32C within
+    1e-5 relative L1 per variable of the FP64 path, div B <= 1e-12 on both."""
+    say("case: turbulence 256^3, 100 steps, FP32C vs FP64")
+    s, res = _turb_pair(256, 100)
+    l1 = rel_l1_vars(s, res["fp32c"][0], res["fp64"][0])
+    say(f"  FP32C vs FP64: worst rel L1 {max(l1.values()):.3e} ({max(l1, key=l1.get)})")
+    assert max(l1.values()) <= 1e-5, l1
+    assert res["fp64"][1] <= 1e-12 and res["fp32c"][1] <= 1e-12
+
+
+@pytest.mark.timeout(900)
+def test_turbulence_512_fp32c_vs_fp64():
+    """The precision configuration at 512^3 (62 GB FP64 and 51 GB FP32C of
+    device state, run one after the other): three steps, FP32C within 1e-5
+    relative L1 of the FP64 path, div B <= 1e-12 on both."""
+    say("case: turbulence 512^3, 3 steps, FP32C vs FP64")
+    s, res = _turb_pair(512, 3)
+    l1 = rel_l1_vars(s, res["fp32c"][0], res["fp64"][0])
+    say(f"  FP32C vs FP64: worst rel L1 {max(l1.values()):.3e} ({max(l1, key=l1.get)})")
+    assert max(l1.values()) <= 1e-5, l1
+    assert res["fp64"][1] <= 1e-12 and res["fp32c"][1] <= 1e-12
