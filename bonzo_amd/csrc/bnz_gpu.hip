@@ -646,6 +646,97 @@ void any_step(bnz_ctx *c) {
   else launch_step_body(c);
 }
 
+// Validation, shape, index ranges and coordinates of a grid (the host part
+// of BnzGrid/BnzIntegr construction); no device work.
+int configure_grid(bnz_ctx *c, const bnz_params *p) {
+  c->p = *p;
+  if (p->nx < 1 || p->ny < 1 || p->nz < 1) return fail(c, BNZ_EINVAL, "nx, ny, nz must be >= 1");
+  if (p->nx > (1 << 30) || p->ny > (1 << 30) || p->nz > (1 << 30))
+    return fail(c, BNZ_EINVAL, "grid too large: cell indices are 32-bit");
+  if (p->nz > 1 && p->ny == 1) return fail(c, BNZ_EINVAL, "nz > 1 needs ny > 1 (grid.pyx:114-122)");
+  if (!(p->gam > 1.0)) return fail(c, BNZ_EINVAL, "gam must be > 1");
+  if (!(p->cour > 0.0)) return fail(c, BNZ_EINVAL, "cour must be > 0");
+  if (p->precision != BNZ_PREC_FP64 && p->precision != BNZ_PREC_FP32C)
+    return fail(c, BNZ_EINVAL, "precision must be BNZ_PREC_FP64 or BNZ_PREC_FP32C");
+  if (p->rsolver == BNZ_RS_HLLC)
+    return fail(c, BNZ_EUNSUPPORTED, "hllc is a hydro-only solver (fluxes_c.c:230); MHD needs hll or hlld");
+  if (p->rsolver != BNZ_RS_HLLD && p->rsolver != BNZ_RS_HLL && p->rsolver != BNZ_RS_HLLT)
+    return fail(c, BNZ_EUNSUPPORTED, "rsolver not implemented on the device (hll, hlld, hllt are)");
+  if (p->rsolver == BNZ_RS_HLLT && p->precision != BNZ_PREC_FP64)
+    return fail(c, BNZ_EUNSUPPORTED, "hllt runs in FP64 only");
+  if (p->reconstr != BNZ_RCN_CONST && p->reconstr != BNZ_RCN_LINEAR)
+    return fail(c, BNZ_EUNSUPPORTED, "reconstr must be const or linear (weno/parab are commented out in reconstr_c.c)");
+  for (int q = 0; q < 6; ++q)
+    if (p->bc[q] != BNZ_BC_PERIODIC && p->bc[q] != BNZ_BC_OUTFLOW && p->bc[q] != BNZ_BC_REFLECT)
+      return fail(c, BNZ_EUNSUPPORTED,
+                 "boundary flags must be 0 (periodic), 1 (outflow) or 2 (reflect)");
+  c->dims = (p->nz > 1) ? 3 : (p->ny > 1) ? 2 : 1;
+  c->order = (p->reconstr == BNZ_RCN_CONST) ? 0 : 1;
+  c->ng = (c->order == 0) ? 2 : 3;
+  c->nt[0] = p->nx + 2 * c->ng;
+  c->nt[1] = (c->dims >= 2) ? p->ny + 2 * c->ng : 1;
+  c->nt[2] = (c->dims >= 3) ? p->nz + 2 * c->ng : 1;
+  c->i1 = c->ng; c->i2 = p->nx + c->ng - 1;
+  c->j1 = (c->dims >= 2) ? c->ng : 0; c->j2 = (c->dims >= 2) ? p->ny + c->ng - 1 : 0;
+  c->k1 = (c->dims >= 3) ? c->ng : 0; c->k2 = (c->dims >= 3) ? p->nz + c->ng - 1 : 0;
+  c->ncell = (long)c->nt[0] * c->nt[1] * c->nt[2];
+  if (c->ncell >= (1L << 31)) return fail(c, BNZ_EINVAL, "grid too large: cell indices are 32-bit");
+  {
+    const int nact[3] = {p->nx, p->ny, p->nz};
+    for (int a = 0; a < c->dims; ++a)
+      if ((p->bc[2 * a] != BNZ_BC_OUTFLOW || p->bc[2 * a + 1] != BNZ_BC_OUTFLOW) &&
+          nact[a] < c->ng)
+        // with fewer active cells than ghost layers the reference's periodic
+        // and reflect copies read ghost cells they have not filled yet
+        return fail(c, BNZ_EINVAL,
+                   "a periodic or reflecting axis needs at least ng active cells");
+  }
+  set_coordinates(c);
+
+  return BNZ_OK;
+}
+
+// Per-axis geometry arrays (dlf, dlfi, icm, icp) as one host table, with the
+// offset of each array in it.
+size_t geo_table(const bnz_ctx *c, std::vector<double> &hg, size_t off[3][4]) {
+  hg.clear();
+  for (int a = 0; a < 3; ++a) {
+    off[a][0] = hg.size(); hg.insert(hg.end(), c->h_dlf[a].begin(), c->h_dlf[a].end());
+    off[a][1] = hg.size(); hg.insert(hg.end(), c->h_dlfi[a].begin(), c->h_dlfi[a].end());
+    off[a][2] = hg.size(); hg.insert(hg.end(), c->h_icm[a].begin(), c->h_icm[a].end());
+    off[a][3] = hg.size(); hg.insert(hg.end(), c->h_icp[a].begin(), c->h_icp[a].end());
+  }
+  return hg.size();
+}
+
+// The kernels' Geo of a configured grid whose geometry table sits at c->geo.
+void set_geo(bnz_ctx *c, const size_t off[3][4]) {
+  Geo &g = c->g;
+  for (int q = 0; q < 3; ++q) g.nt[q] = c->nt[q];
+  g.dims = c->dims;
+  g.sv = c->ncell;
+  g.sy = c->nt[0];
+  g.sz = c->nt[0] * c->nt[1];
+  for (int q = 0; q < 3; ++q) {
+    g.dlf[q] = c->geo + off[q][0];
+    g.dlfi[q] = c->geo + off[q][1];
+    g.icm[q] = c->geo + off[q][2];
+    g.icp[q] = c->geo + off[q][3];
+  }
+  g.gam = c->p.gam;
+  g.gamm1i = 1. / (c->p.gam - 1.);
+  g.unif = 1;
+  for (int q = 0; q < 3; ++q) {
+    const std::vector<double> *arr[4] = {&c->h_dlf[q], &c->h_dlfi[q], &c->h_icm[q], &c->h_icp[q]};
+    double *u[4] = {&g.udlf[q], &g.udlfi[q], &g.uicm[q], &g.uicp[q]};
+    for (int m = 0; m < 4; ++m) {
+      *u[m] = (*arr[m])[0];
+      for (double v : *arr[m])
+        if (v != *u[m]) g.unif = 0;
+    }
+  }
+}
+
 }  // namespace
 
 // ==========================================================================
@@ -683,57 +774,13 @@ int bnz_create(const bnz_params *p, bnz_ctx **out) {
   if (!p || !out) return BNZ_EINVAL;
   *out = nullptr;
   bnz_ctx *c = new bnz_ctx();
-  c->p = *p;
-  auto bad = [&](int code, const std::string &m) {
-    // keep the context so the caller can read the message, then destroy it
-    c->err = m;
-    *out = c;
-    return code;
-  };
-  if (p->nx < 1 || p->ny < 1 || p->nz < 1) return bad(BNZ_EINVAL, "nx, ny, nz must be >= 1");
-  if (p->nx > (1 << 30) || p->ny > (1 << 30) || p->nz > (1 << 30))
-    return bad(BNZ_EINVAL, "grid too large: cell indices are 32-bit");
-  if (p->nz > 1 && p->ny == 1) return bad(BNZ_EINVAL, "nz > 1 needs ny > 1 (grid.pyx:114-122)");
-  if (!(p->gam > 1.0)) return bad(BNZ_EINVAL, "gam must be > 1");
-  if (!(p->cour > 0.0)) return bad(BNZ_EINVAL, "cour must be > 0");
-  if (p->precision != BNZ_PREC_FP64 && p->precision != BNZ_PREC_FP32C)
-    return bad(BNZ_EINVAL, "precision must be BNZ_PREC_FP64 or BNZ_PREC_FP32C");
-  if (p->rsolver == BNZ_RS_HLLC)
-    return bad(BNZ_EUNSUPPORTED, "hllc is a hydro-only solver (fluxes_c.c:230); MHD needs hll or hlld");
-  if (p->rsolver != BNZ_RS_HLLD && p->rsolver != BNZ_RS_HLL && p->rsolver != BNZ_RS_HLLT)
-    return bad(BNZ_EUNSUPPORTED, "rsolver not implemented on the device (hll, hlld, hllt are)");
-  if (p->rsolver == BNZ_RS_HLLT && p->precision != BNZ_PREC_FP64)
-    return bad(BNZ_EUNSUPPORTED, "hllt runs in FP64 only");
-  if (p->reconstr != BNZ_RCN_CONST && p->reconstr != BNZ_RCN_LINEAR)
-    return bad(BNZ_EUNSUPPORTED, "reconstr must be const or linear (weno/parab are commented out in reconstr_c.c)");
-  for (int q = 0; q < 6; ++q)
-    if (p->bc[q] != BNZ_BC_PERIODIC && p->bc[q] != BNZ_BC_OUTFLOW && p->bc[q] != BNZ_BC_REFLECT)
-      return bad(BNZ_EUNSUPPORTED,
-                 "boundary flags must be 0 (periodic), 1 (outflow) or 2 (reflect)");
-  c->dims = (p->nz > 1) ? 3 : (p->ny > 1) ? 2 : 1;
-  c->order = (p->reconstr == BNZ_RCN_CONST) ? 0 : 1;
-  c->ng = (c->order == 0) ? 2 : 3;
-  c->nt[0] = p->nx + 2 * c->ng;
-  c->nt[1] = (c->dims >= 2) ? p->ny + 2 * c->ng : 1;
-  c->nt[2] = (c->dims >= 3) ? p->nz + 2 * c->ng : 1;
-  c->i1 = c->ng; c->i2 = p->nx + c->ng - 1;
-  c->j1 = (c->dims >= 2) ? c->ng : 0; c->j2 = (c->dims >= 2) ? p->ny + c->ng - 1 : 0;
-  c->k1 = (c->dims >= 3) ? c->ng : 0; c->k2 = (c->dims >= 3) ? p->nz + c->ng - 1 : 0;
-  c->ncell = (long)c->nt[0] * c->nt[1] * c->nt[2];
-  if (c->ncell >= (1L << 31)) return bad(BNZ_EINVAL, "grid too large: cell indices are 32-bit");
-  {
-    const int nact[3] = {p->nx, p->ny, p->nz};
-    for (int a = 0; a < c->dims; ++a)
-      if ((p->bc[2 * a] != BNZ_BC_OUTFLOW || p->bc[2 * a + 1] != BNZ_BC_OUTFLOW) &&
-          nact[a] < c->ng)
-        // with fewer active cells than ghost layers the reference's periodic
-        // and reflect copies read ghost cells they have not filled yet
-        return bad(BNZ_EINVAL,
-                   "a periodic or reflecting axis needs at least ng active cells");
-  }
-  set_coordinates(c);
-
+  // on a configuration error the context is kept so that the caller can
+  // read the message, then destroy it
   *out = c;
+  {
+    const int rc = configure_grid(c, p);
+    if (rc) return rc;
+  }
   if (hipSetDevice(p->device) != hipSuccess) return fail(c, BNZ_EHIP, "hipSetDevice failed");
   HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   HIPCHK(c, hipHostMalloc(&c->hbuf, 8 * sizeof(double), hipHostMallocDefault));
@@ -761,18 +808,10 @@ int bnz_create(const bnz_params *p, bnz_ctx **out) {
     if (rc) return rc;
   }
 
-  size_t geo_n = 0;
-  for (int a = 0; a < 3; ++a) geo_n += 4 * (size_t)c->nt[a];
-  HIPCHK(c, hipMalloc(&c->geo, geo_n * sizeof(double)));
   std::vector<double> hg;
-  hg.reserve(geo_n);
   size_t off[3][4];
-  for (int a = 0; a < 3; ++a) {
-    off[a][0] = hg.size(); hg.insert(hg.end(), c->h_dlf[a].begin(), c->h_dlf[a].end());
-    off[a][1] = hg.size(); hg.insert(hg.end(), c->h_dlfi[a].begin(), c->h_dlfi[a].end());
-    off[a][2] = hg.size(); hg.insert(hg.end(), c->h_icm[a].begin(), c->h_icm[a].end());
-    off[a][3] = hg.size(); hg.insert(hg.end(), c->h_icp[a].begin(), c->h_icp[a].end());
-  }
+  const size_t geo_n = geo_table(c, hg, off);
+  HIPCHK(c, hipMalloc(&c->geo, geo_n * sizeof(double)));
   HIPCHK(c, hipMemcpy(c->geo, hg.data(), geo_n * sizeof(double), hipMemcpyHostToDevice));
 
   {
@@ -810,30 +849,7 @@ int bnz_create(const bnz_params *p, bnz_ctx **out) {
   HIPCHK(c, hipMalloc(&c->rpart, (size_t)blocks_for(a.total()) * std::max(NDIAG, NHIST) *
                                      sizeof(double)));
 
-  Geo &g = c->g;
-  for (int q = 0; q < 3; ++q) g.nt[q] = c->nt[q];
-  g.dims = c->dims;
-  g.sv = c->ncell;
-  g.sy = c->nt[0];
-  g.sz = c->nt[0] * c->nt[1];
-  for (int q = 0; q < 3; ++q) {
-    g.dlf[q] = c->geo + off[q][0];
-    g.dlfi[q] = c->geo + off[q][1];
-    g.icm[q] = c->geo + off[q][2];
-    g.icp[q] = c->geo + off[q][3];
-  }
-  g.gam = p->gam;
-  g.gamm1i = 1. / (p->gam - 1.);
-  g.unif = 1;
-  for (int q = 0; q < 3; ++q) {
-    const std::vector<double> *arr[4] = {&c->h_dlf[q], &c->h_dlfi[q], &c->h_icm[q], &c->h_icp[q]};
-    double *u[4] = {&g.udlf[q], &g.udlfi[q], &g.uicm[q], &g.uicp[q]};
-    for (int m = 0; m < 4; ++m) {
-      *u[m] = (*arr[m])[0];
-      for (double v : *arr[m])
-        if (v != *u[m]) g.unif = 0;
-    }
-  }
+  set_geo(c, off);
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return BNZ_OK;
 }
