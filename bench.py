@@ -83,27 +83,59 @@ def kernel_bytes(prec, name, n):
     return table[key] * kernel_units(name, n)
 
 
-def cpu_baseline(budget_s=15.0):
-    from bonzo_amd import problems
-    from oracle.oracle import OracleSim
-    n = 64
-    threads = min(16, os.cpu_count() or 1)
-    s, prim, bfld = problems.turbulence(n)
-    o = OracleSim(n, n, n, nthreads=threads)
-    o.prim[:] = prim
-    o.bfld[:] = bfld
-    o.init()
-    t0 = time.perf_counter()
-    o.advance_steps(1)
-    one = time.perf_counter() - t0
-    k = max(1, min(1000, int(budget_s / max(one, 1e-3))))
-    t0 = time.perf_counter()
-    o.advance_steps(k)
-    el = time.perf_counter() - t0
-    o.close()
-    return {"value": n ** 3 * k / el, "unit": "cell-updates/s", "cores": threads, "kind": "port",
-            "sample": f"oracle (C restatement, OpenMP sweeps as in the reference) on {n}^3 "
-                      f"turbulence, {k} steps after 1 warm-up step, {el:.1f} s"}
+def lscpu():
+    """Model, sockets, cores per socket and threads per core of the host."""
+    import subprocess
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, check=True).stdout
+    except (OSError, subprocess.CalledProcessError):
+        return info
+    keys = {"Model name": "model", "Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+            "Thread(s) per core": "threads_per_core", "CPU(s)": "cpus"}
+    for line in out.splitlines():
+        k, _, v = line.partition(":")
+        if k.strip() in keys:
+            v = v.strip()
+            info[keys[k.strip()]] = int(v) if v.isdigit() else v
+    return info
+
+
+def cpu_run(n, steps, build, threads):
+    """One child process timing the oracle (oracle/cpu_bench.py) with
+    ``threads`` OpenMP threads pinned one per core."""
+    import subprocess
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close",
+               OMP_PLACES="cores")
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "oracle", "cpu_bench.py"), "--n",
+                        str(n), "--steps", str(steps), "--build", build], env=env,
+                       capture_output=True, text=True, timeout=900)
+    if r.returncode != 0:
+        return {"build": build, "n": n, "threads": threads, "error": r.stderr[-400:]}
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def cpu_baseline(n):
+    """The oracle (a C restatement of the reference's algorithm with its
+    OpenMP sweeps) on the host: the benchmark's own workload (n^3, a few
+    steps) with one pinned thread per physical core of this process's CPU
+    share, in the parity build and in the reference's fast-math compiler
+    preset, plus one core on a 64^3 sample."""
+    cpu = lscpu()
+    physical = int(cpu.get("sockets", 1)) * int(cpu.get("cores_per_socket", os.cpu_count() or 1))
+    share = int(os.environ.get("OMP_NUM_THREADS") or physical)
+    threads = max(1, min(physical, share))
+    runs = [cpu_run(n, 2, "parity", threads), cpu_run(n, 2, "reference-flags", threads),
+            cpu_run(64, 3, "parity", 1)]
+    main_run = runs[0]
+    return {"value": main_run.get("value"), "unit": "cell-updates/s", "cores": threads,
+            "kind": "port",
+            "sample": f"oracle (C restatement, OpenMP sweeps as in the reference) on the {n}^3 "
+                      f"turbulence workload, 2 timed steps after 1 untimed, {threads} threads "
+                      "pinned one per core (OMP_PROC_BIND=close, OMP_PLACES=cores; the host's "
+                      f"{physical} physical cores, capped at this process's CPU share "
+                      f"OMP_NUM_THREADS={share}), parity build",
+            "cpu": cpu, "runs": runs}
 
 
 def copy_calibration(mib):
@@ -381,7 +413,7 @@ def main():
     if not args.no_batch:
         out["batched"] = batch_benches(args.batch_reps, not args.no_fp32c)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline()
+        out["cpu_baseline"] = cpu_baseline(n)
     print(json.dumps(out), flush=True)
 
 

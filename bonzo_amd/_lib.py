@@ -46,7 +46,9 @@ class BnzDiag(ctypes.Structure):
         ("mass", ctypes.c_double), ("mom", ctypes.c_double * 3), ("energy", ctypes.c_double),
         ("scalar", ctypes.c_double), ("divb_max", ctypes.c_double), ("bmax", ctypes.c_double),
         ("pmin", ctypes.c_double), ("rhomin", ctypes.c_double), ("nonfinite", ctypes.c_long),
-        ("hlld_guards", ctypes.c_long),
+        ("hlld_guards", ctypes.c_long), ("npneg", ctypes.c_long), ("nrhoneg", ctypes.c_long),
+        ("bad_cell", ctypes.c_long), ("bad_ijk", ctypes.c_int * 3), ("bad_kind", ctypes.c_int),
+        ("step", ctypes.c_int), ("time", ctypes.c_double),
     ]
 
 
@@ -235,7 +237,9 @@ class Context:
         self._check(load().bnz_diag(self._h, ctypes.byref(d)))
         return dict(mass=d.mass, mom=tuple(d.mom), energy=d.energy, scalar=d.scalar,
                     divb_max=d.divb_max, bmax=d.bmax, pmin=d.pmin, rhomin=d.rhomin,
-                    nonfinite=d.nonfinite, hlld_guards=d.hlld_guards)
+                    nonfinite=d.nonfinite, hlld_guards=d.hlld_guards, npneg=d.npneg,
+                    nrhoneg=d.nrhoneg, bad_cell=d.bad_cell, bad_ijk=tuple(d.bad_ijk),
+                    bad_kind=d.bad_kind, step=d.step, time=d.time)
 
     def history(self):
         """One history row (step, time, 14 cell means; see bnz_history)."""

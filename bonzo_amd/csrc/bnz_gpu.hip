@@ -335,7 +335,12 @@ __global__ __launch_bounds__(BLK) void k_hist(Geo g, const double *__restrict__ 
   }
 }
 
-constexpr int NDIAG = 11;  // mass, mx, my, mz, E, psc, divb, bmax, -pmin, -rhomin, nonfinite
+// Per-block partials of the diagnostics: sums of mass, mx, my, mz, E, psc
+// (0-5), maxima of |div B|, |B|, -p, -rho (6-9), counts of non-finite cells,
+// cells with p <= 0 and with rho <= 0 (10-12), and the largest -index of a
+// failing cell (13: -1e300 if none), so the host finds the lowest index.
+constexpr int NDIAG = 14;
+__device__ __forceinline__ bool diag_is_max(int q) { return (q >= 6 && q <= 9) || q == 13; }
 __global__ __launch_bounds__(BLK) void k_diag(Geo g, const double *__restrict__ w,
                                               const double *__restrict__ b, Box box,
                                               double *__restrict__ part) {
@@ -346,6 +351,7 @@ __global__ __launch_bounds__(BLK) void k_diag(Geo g, const double *__restrict__ 
   for (int q = 0; q < NDIAG; ++q) v[q] = 0.;
   v[8] = -1e300;
   v[9] = -1e300;
+  v[13] = -1e300;
   if (t < box.total()) {
     int i, j, k;
     box_idx(box, t, i, j, k);
@@ -368,12 +374,16 @@ __global__ __launch_bounds__(BLK) void k_diag(Geo g, const double *__restrict__ 
     v[7] = sqrt(sqr(wv[BX]) + sqr(wv[BY]) + sqr(wv[BZ]));
     v[8] = -wv[PR];
     v[9] = -wv[RHO];
+    const bool pneg = wv[PR] <= 0., rneg = wv[RHO] <= 0.;
     v[10] = finite ? 0. : 1.;
+    v[11] = pneg ? 1. : 0.;
+    v[12] = rneg ? 1. : 0.;
+    if (!finite || pneg || rneg) v[13] = -(double)c;
   }
 #pragma unroll
   for (int q = 0; q < NDIAG; ++q) {
     double x = v[q];
-    const bool is_max = (q >= 6 && q <= 9);
+    const bool is_max = diag_is_max(q);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
       double y = __shfl_xor(x, off, 64);
@@ -384,7 +394,7 @@ __global__ __launch_bounds__(BLK) void k_diag(Geo g, const double *__restrict__ 
   __syncthreads();
   if (threadIdx.x < NDIAG) {
     const int q = threadIdx.x;
-    const bool is_max = (q >= 6 && q <= 9);
+    const bool is_max = diag_is_max(q);
     double x = red[q][0];
     for (int p = 1; p < BLK / 64; ++p) x = is_max ? fmax(x, red[q][p]) : x + red[q][p];
     part[(long)blockIdx.x * NDIAG + q] = x;
@@ -1140,11 +1150,11 @@ int bnz_diag(bnz_ctx *c, bnz_diag_out *out) {
   HIPCHK(c, hipStreamSynchronize(c->stream));
   double s[NDIAG] = {0};
   s[6] = s[7] = 0.;
-  s[8] = s[9] = -1e300;
+  s[8] = s[9] = s[13] = -1e300;
   for (int q = 0; q < nb; ++q)
     for (int v = 0; v < NDIAG; ++v) {
       const double x = h[(size_t)q * NDIAG + v];
-      if (v >= 6 && v <= 9) s[v] = std::fmax(s[v], x);
+      if ((v >= 6 && v <= 9) || v == 13) s[v] = std::fmax(s[v], x);
       else s[v] += x;
     }
   out->mass = s[0];
@@ -1158,9 +1168,33 @@ int bnz_diag(bnz_ctx *c, bnz_diag_out *out) {
   out->pmin = -s[8];
   out->rhomin = -s[9];
   out->nonfinite = (long)s[10];
+  out->npneg = (long)s[11];
+  out->nrhoneg = (long)s[12];
+  out->bad_cell = -1;
+  out->bad_kind = 0;
+  for (int a = 0; a < 3; ++a) out->bad_ijk[a] = -1;
+  if (s[13] > -1e299) {  // the failing cell of lowest index, and what fails there
+    const long cell = (long)(-s[13]);
+    out->bad_cell = cell;
+    out->bad_ijk[0] = (int)(cell % c->nt[0]);
+    out->bad_ijk[1] = (int)((cell / c->nt[0]) % c->nt[1]);
+    out->bad_ijk[2] = (int)(cell / ((long)c->nt[0] * c->nt[1]));
+    double wv[NMODE];
+    for (int n = 0; n < NMODE; ++n)
+      HIPCHK(c, hipMemcpy(&wv[n], c->w + n * c->ncell + cell, sizeof(double),
+                          hipMemcpyDeviceToHost));
+    for (int n = 0; n < NMODE; ++n)
+      if (!std::isfinite(wv[n])) out->bad_kind |= 1;
+    if (wv[PR] <= 0.) out->bad_kind |= 2;
+    if (wv[RHO] <= 0.) out->bad_kind |= 4;
+  }
   int ng = 0;
   HIPCHK(c, hipMemcpy(&ng, (int *)(c->dev + 4), sizeof(int), hipMemcpyDeviceToHost));
   out->hlld_guards = ng;
+  double d[4];
+  HIPCHK(c, hipMemcpy(d, c->dev, sizeof(d), hipMemcpyDeviceToHost));
+  out->time = d[1];
+  out->step = (int)d[3];
   return BNZ_OK;
 }
 

@@ -125,6 +125,20 @@ class BnzIntegr:
             raise NotImplementedError("parab/weno are commented out in src/coordinates/reconstr_c.c")
         if self.char_proj:
             raise NotImplementedError("charact_proj=1 is not supported (src/mhd/eos_c.c:61-63)")
+        # Physics the reference adds around the step whenever its coefficients
+        # are non-zero: constant gravity (src/srcterms/gravity.pyx:15-17,
+        # applied in integrator.pyx:146-157), turbulence driving
+        # (src/srcterms/turb_driv.pyx:26-28, integrator.pyx:161-190) and the
+        # super-time-stepped diffusion (src/dissipation/diffusion.pyx:37-67,
+        # 158-164).  None is built, so a configuration that asks for any of
+        # them is refused rather than run without it.
+        for key in ("gx", "gy", "gz", "f0", "kappa0", "nuiic0", "kL", "mu", "eta", "mu4",
+                    "eta4"):
+            if params.get("physics", key, "f", 0.0) != 0.0:
+                raise NotImplementedError(
+                    f"[physics] {key} is non-zero: gravity, turbulence driving and diffusion "
+                    "are not built (src/integration/integrator.pyx:146-190, "
+                    "src/dissipation/diffusion.pyx:158-164)")
 
 
 # ---------------------------------------------------------------- grid
@@ -147,6 +161,11 @@ class BnzGrid:
         self.k1, self.k2 = (g, self.Nact[2] + g - 1) if self.dims >= 3 else (0, 0)
         self.lmin = [params.get("computation", k, "f", 0.0) for k in ("xmin", "ymin", "zmin")]
         self.lmax = [params.get("computation", k, "f", 1.0) for k in ("xmax", "ymax", "zmax")]
+        geom = params.get("physics", "geometry", "s", "car")
+        if geom != "car":
+            # cylindrical and spherical metrics and their geometric source
+            # terms (src/coordinates/grid.pyx:171-190, coord.add_geom_src_terms)
+            raise NotImplementedError(f"geometry {geom}: only Cartesian ('car') grids are built")
         self.bc_flags = [params.get("physics", k, "i", 0)
                          for k in ("bc_x1", "bc_x2", "bc_y1", "bc_y2", "bc_z1", "bc_z2")]
         for f in self.bc_flags:

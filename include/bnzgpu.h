@@ -67,6 +67,17 @@ typedef struct {
   long nonfinite;                      /* active cells with a non-finite prim value  */
   long hlld_guards;                    /* FP32 HLLD faces whose star denominators
                                           were guarded (cumulative; 0 in FP64)      */
+  long npneg, nrhoneg;                 /* active cells with p <= 0, with rho <= 0     */
+  long bad_cell;                       /* failure location: the lowest linear index
+                                          (k*Ntot_y + j)*Ntot_x + i of an active cell
+                                          that is non-finite or has p <= 0 or rho <= 0,
+                                          -1 if there is none                         */
+  int bad_ijk[3];                      /* its i, j, k over the full grid (ghosts
+                                          included); -1 if none                      */
+  int bad_kind;                        /* what is wrong there: bit 0 a non-finite
+                                          prim value, bit 1 p <= 0, bit 2 rho <= 0  */
+  int step;                            /* steps taken when the diagnostics ran       */
+  double time;                         /* and the time                               */
 } bnz_diag_out;
 
 /* Replaces BnzGrid/BnzIntegr construction (src/coordinates/grid.pyx:59-82,
@@ -104,7 +115,8 @@ int bnz_run_steps(bnz_ctx *ctx, int nsteps);
 int bnz_get_time(bnz_ctx *ctx, double *time, int *step, double *last_dt);
 int bnz_set_time(bnz_ctx *ctx, double time, int step);
 /* Conservation, div B and failure diagnostics (src/diagnostics.pyx:18-125
-   prints the energies; the div B and positivity checks have no reference
+   prints the energies; the div B and positivity checks, and the location of
+   the first failing cell with the step it was found at, have no reference
    counterpart). */
 int bnz_diag(bnz_ctx *ctx, bnz_diag_out *out);
 /* One history row of get_hst_vars_grid (src/io/output.pyx:20-138, the

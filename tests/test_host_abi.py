@@ -41,10 +41,13 @@ def test_params_struct_layout_matches_header(tmp_path):
     src = tmp_path / "layout.c"
     src.write_text(
         '#include <stdio.h>\n#include <stddef.h>\n#include "bnzgpu.h"\n'
-        'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(bnz_params),'
+        'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n",'
+        ' sizeof(bnz_params),'
         ' offsetof(bnz_params, lmin), offsetof(bnz_params, gam), offsetof(bnz_params, device),'
         ' sizeof(bnz_diag_out), offsetof(bnz_diag_out, divb_max),'
-        ' offsetof(bnz_diag_out, nonfinite), sizeof(bnz_box), offsetof(bnz_box, lmin),'
+        ' offsetof(bnz_diag_out, nonfinite), offsetof(bnz_diag_out, bad_ijk),'
+        ' offsetof(bnz_diag_out, step), offsetof(bnz_diag_out, time),'
+        ' sizeof(bnz_box), offsetof(bnz_box, lmin),'
         ' offsetof(bnz_box, tmax)); return 0;}\n')
     exe = tmp_path / "layout"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)],
@@ -54,6 +57,7 @@ def test_params_struct_layout_matches_header(tmp_path):
     mine = [ctypes.sizeof(_lib.BnzParams), _lib.BnzParams.lmin.offset, _lib.BnzParams.gam.offset,
             _lib.BnzParams.device.offset, ctypes.sizeof(_lib.BnzDiag),
             _lib.BnzDiag.divb_max.offset, _lib.BnzDiag.nonfinite.offset,
+            _lib.BnzDiag.bad_ijk.offset, _lib.BnzDiag.step.offset, _lib.BnzDiag.time.offset,
             ctypes.sizeof(_lib.BnzBox), _lib.BnzBox.lmin.offset, _lib.BnzBox.tmax.offset]
     assert vals == mine
 
@@ -149,6 +153,35 @@ def test_selection_semantics_follow_reference(tmp_path):
         it = BnzIntegr(_Params(config={"computation": {"cour": 0.4, "tintegr": "rk3"},
                                        "physics": {"gam": 1.4}}))
     assert it.tintegr == TIntegr.TINT_RK3
+
+
+@pytest.mark.parametrize("key", ["gx", "gy", "gz", "f0", "kappa0", "nuiic0", "kL", "mu", "eta",
+                                 "mu4", "eta4"])
+def test_unbuilt_physics_is_refused_not_dropped(key):
+    """Gravity, turbulence driving and diffusion act in the reference as soon
+    as their coefficient is non-zero (integrator.pyx:146-190,
+    diffusion.pyx:158-164); they are not built here, so such a configuration
+    raises instead of silently running without them.  Zero values (the
+    reference's sample config) are accepted."""
+    base = {"computation": {"cour": 0.4, "rsolver": "hlld"}, "physics": {"gam": 1.4}}
+    zero = {k: dict(v) for k, v in base.items()}
+    zero["physics"][key] = 0.0
+    BnzIntegr(_Params(config=zero))
+    bad = {k: dict(v) for k, v in base.items()}
+    bad["physics"][key] = 0.5
+    with pytest.raises(NotImplementedError, match=key):
+        BnzIntegr(_Params(config=bad))
+
+
+def test_non_cartesian_geometry_is_refused():
+    from bonzo_amd.simulation import BnzGrid
+    cfg = {"computation": {"cour": 0.4, "Nx": 16, "Ny": 1, "Nz": 1},
+           "physics": {"gam": 1.4, "geometry": "cyl"}}
+    integr = BnzIntegr(_Params(config=cfg))
+    with pytest.raises(NotImplementedError, match="geometry"):
+        BnzGrid(_Params(config=cfg), integr)
+    cfg["physics"]["geometry"] = "car"
+    BnzGrid(_Params(config=cfg), integr)
 
 
 def _divb(s, bfld):
