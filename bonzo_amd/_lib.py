@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(_HERE, "libbnzgpu.so")
 BNZ_OK, BNZ_EINVAL, BNZ_ENOMEM, BNZ_EHIP, BNZ_EUNSUPPORTED, BNZ_ESTATE = 0, -1, -2, -3, -4, -5
 RS = {"hll": 0, "hllc": 1, "hlld": 2, "hlla": 3, "hllt": 4}
 RCN = {"const": 0, "linear": 1, "weno": 2, "parab": 3}
-PREC = {"fp64": 0, "fp32c": 1}
+PREC = {"fp64": 0, "fp32c": 1, "fp64fma": 2}
 
 ABI_SYMBOLS = (
     "bnz_create", "bnz_destroy", "bnz_last_error", "bnz_get_shape", "bnz_upload",

@@ -549,6 +549,8 @@ int bnz_batch_create(const bnz_params *p, const bnz_box *boxes, int nbox, bnz_ba
   *out = c;
   c->p = *p;
   c->nbox = nbox;
+  if (p->precision == BNZ_PREC_FP64_FMA)
+    return bfail(c, BNZ_EUNSUPPORTED, "batches run BNZ_PREC_FP64 or BNZ_PREC_FP32C");
   if (p->precision != BNZ_PREC_FP64 && p->precision != BNZ_PREC_FP32C)
     return bfail(c, BNZ_EINVAL, "precision must be BNZ_PREC_FP64 or BNZ_PREC_FP32C");
   c->f32 = p->precision == BNZ_PREC_FP32C;

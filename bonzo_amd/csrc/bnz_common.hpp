@@ -1,6 +1,7 @@
 // bnz_common.hpp -- types and helpers shared by every translation unit of
-// libbnzgpu.so: single grids in FP64 (bnz_gpu.hip) and FP32C (bnz_gpu32.hip),
-// batched boxes (bnz_batch.hip, bnz_batch32.hip).
+// libbnzgpu.so: single grids in FP64 (bnz_gpu.hip; with FMA contraction
+// bnz_gpu64f.hip) and FP32C (bnz_gpu32.hip), batched boxes (bnz_batch.hip,
+// bnz_batch32.hip).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -508,5 +509,7 @@ void f32_step_body(bnz_ctx *c);
 void launch_dt_final(bnz_ctx *c, int npart, int use_fixed, double fixed_dt);
 void launch_time_advance(bnz_ctx *c);
 void launch_ghost64(bnz_ctx *c);
+// the FP64 step compiled with contraction (bnz_gpu64f.hip)
+void fma_step_body(bnz_ctx *c);
 
 }  // namespace bnz

@@ -1,0 +1,243 @@
+// bnz_step64.hpp -- the FP64 step's kernels and their launches, included by
+// two translation units that compile the same code with different
+// arithmetic: bnz_gpu.hip (no contraction: bit for bit the CPU oracle) and
+// bnz_gpu64f.hip (a*b+c contracted into FMAs: the fast FP64 mode).  Every
+// definition is in an anonymous namespace, so each unit gets its own copy of
+// the kernels, compiled with its own flags.
+//
+// One VL step (src/integration/vanleer.pyx:87-167), after the dt kernels:
+//   predictor: flux3, emf, update<pred>        (donor cell, dt/2)
+//   corrector: flux3, emf, update<corr>        (PLM, dt; CFL maximum fused)
+//   ghost (+ ghost_ops for reflecting sides), time_advance
+// 1D grids run the whole step as one fused pass (k_step1).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <utility>
+
+#include "bnz_kern64.hpp"
+
+namespace {
+
+using namespace bnz;
+
+// --------------------------------------------------------------------------
+// Work shapes (3D): ty == 0 one thread per item in x-fastest order; TILED
+// 64 x 4 tiles ordered along z; ty > 1 slab columns of ty rows walked plane
+// by plane (see cell_index).
+//
+// All three face directions of a stage in one launch: one thread per cell of
+// the union of the face ranges computes the x, y and z faces whose lower
+// cell it is, so the prim stencil of a cell is fetched once per stage.
+template <int ORDER, int RS, bool TILED = false>
+__global__ __launch_bounds__(BLK) void k_flux3(Geo g, const double *__restrict__ w,
+                                               const double *__restrict__ bf,
+                                               double *__restrict__ fx, double *__restrict__ fy,
+                                               double *__restrict__ fz, Box box, Lims L,
+                                               int ty, const double *__restrict__ dts) {
+  if (dts[0] == 0.) return;  // a step past t_max (see k_update)
+  int i, j, k;
+  if (!cell_index<TILED, 1, 2>(box, ty, i, j, k)) return;
+  flux_cell<ORDER, RS>(g, w, bf, fx, fy, fz, L, i, j, k);
+}
+
+// A whole VL step of a 1D grid in one pass (step1_block): workgroup b owns
+// the cells from b * STEP1_CELLS; the new prim and faces go to wout and
+// bout, which the host then swaps with w and b; the workgroup's CFL maximum
+// goes to part.
+template <int ORDER, int RS>
+__global__ __launch_bounds__(BLK) void k_step1(Geo g, const double *__restrict__ w,
+                                               const double *__restrict__ bf,
+                                               double *__restrict__ wout,
+                                               double *__restrict__ bout,
+                                               const double *__restrict__ fx_old, Lims L0,
+                                               Lims L1, const double *__restrict__ dts, Box act,
+                                               int fms_ref, double *__restrict__ part) {
+  __shared__ double sf[NMODE * BLK];
+  __shared__ double sw[NMODE * BLK];
+  const int i0 = blockIdx.x * STEP1_CELLS;
+  if (dts[0] == 0.) {  // a step past t_max: the state is carried into the new buffers
+    const int i = i0 + threadIdx.x;
+    if (threadIdx.x < STEP1_CELLS && i < g.nt[0]) {
+      for (int n = 0; n < NMODE; ++n) wout[n * g.sv + i] = w[n * g.sv + i];
+      for (int a = 0; a < 3; ++a) bout[a * g.sv + i] = bf[a * g.sv + i];
+    }
+    return;
+  }
+  const double cds = step1_block<ORDER, RS>(g, w, bf, wout, bout, fx_old, L0, L1, dts, act,
+                                            fms_ref, i0, sf, sw);
+  block_max_to(cds, part);
+}
+
+// --------------------------------------------------------------------------
+// Edge EMFs, interp_ee1 (src/integration/ct.pyx:153-215) with the cell-centred
+// E = -v x B of ec_from_prim (ct.pyx:128-148) computed on the fly.
+template <int DIMS, bool TILED = false>
+__global__ __launch_bounds__(BLK) void k_emf(Geo g, const double *__restrict__ w,
+                                             const double *__restrict__ fx,
+                                             const double *__restrict__ fy,
+                                             const double *__restrict__ fz,
+                                             double *__restrict__ ee, Box box, int ty) {
+  int i, j, k;
+  if (!cell_index<TILED, 1, 2>(box, ty, i, j, k)) return;
+  emf_edge_d<DIMS>(g, w, fx, fy, fz, ee, i, j, k);
+}
+
+template <int DIMS, bool PRED, bool DTRED = false, bool TILED = false, bool UG = false>
+__global__ __launch_bounds__(BLK) void k_update(Geo g, const double *w0, const double *__restrict__ fx,
+                                                const double *__restrict__ fy,
+                                                const double *__restrict__ fz,
+                                                const double *__restrict__ ee,
+                                                const double *__restrict__ b0,
+                                                double *__restrict__ b1, double *w1, Lims L,
+                                                const double *__restrict__ dts, Box grid, Box act,
+                                                int fms_ref, double *__restrict__ part,
+                                                int ty) {
+  int i, j, k;
+  double cdsmax = 0.;
+  const bool have = cell_index<TILED, 1, 2>(grid, ty, i, j, k);
+  if (dts[0] == 0.) {  // a step past t_max: state unchanged, faces carried over
+    if (!PRED && have) {
+      const long c = ((long)k * g.nt[1] + j) * g.nt[0] + i;
+      for (int a = 0; a < 3; ++a) b1[a * g.sv + c] = b0[a * g.sv + c];
+    }
+    return;  // uniform over the workgroup; the CFL partials stay those of the state
+  }
+  if (have)
+    cdsmax = update_cell_d<DIMS, PRED, DTRED, UG>(g, w0, fx, fy, fz, ee, b0, b1, w1, L, dts, act,
+                                                  fms_ref, i, j, k);
+  if (DTRED) block_max_to(cdsmax, part);
+}
+
+// ==========================================================================
+// Launches
+// ==========================================================================
+
+template <int ORDER, int RS>
+void launch_fluxes(bnz_ctx *c, const double *w, const double *bf, const Lims &L) {
+  // all face directions in one launch over the union of the face ranges
+  const Box u = flux3_box(c->dims, L);
+  const Shape sh = work_shape(u, c->dims, c->flux_chunk);
+  Launch t(c, ORDER ? "flux3_plm" : "flux3_dc");
+  if (sh.tiled)
+    hipLaunchKernelGGL((k_flux3<ORDER, RS, true>), dim3(sh.blocks), dim3(BLK), 0, c->stream, c->g,
+                       w, bf, c->fx, c->fy, c->fz, u, L, 0, c->dev);
+  else
+    hipLaunchKernelGGL((k_flux3<ORDER, RS, false>), dim3(sh.blocks), dim3(BLK), 0, c->stream,
+                       c->g, w, bf, c->fx, c->fy, c->fz, u, L, sh.ty, c->dev);
+}
+
+void launch_flux_dispatch(bnz_ctx *c, int order, const double *w, const double *bf,
+                          const Lims &L) {
+  const int rs = c->p.rsolver;
+  if (order == 0) {
+    if (rs == BNZ_RS_HLLD) launch_fluxes<0, BNZ_RS_HLLD>(c, w, bf, L);
+    else if (rs == BNZ_RS_HLLT) launch_fluxes<0, BNZ_RS_HLLT>(c, w, bf, L);
+    else launch_fluxes<0, BNZ_RS_HLL>(c, w, bf, L);
+  } else {
+    if (rs == BNZ_RS_HLLD) launch_fluxes<1, BNZ_RS_HLLD>(c, w, bf, L);
+    else if (rs == BNZ_RS_HLLT) launch_fluxes<1, BNZ_RS_HLLT>(c, w, bf, L);
+    else launch_fluxes<1, BNZ_RS_HLL>(c, w, bf, L);
+  }
+}
+
+void launch_emf(bnz_ctx *c, const double *w, const Lims &L, const char *name) {
+  Box e;
+  e.lo[0] = L.lo[0]; e.n[0] = L.hi[0] + 1 - L.lo[0] + 1;
+  e.lo[1] = (c->dims >= 2) ? L.lo[1] : 0; e.n[1] = (c->dims >= 2) ? L.hi[1] + 1 - L.lo[1] + 1 : 1;
+  e.lo[2] = (c->dims >= 3) ? L.lo[2] : 0; e.n[2] = (c->dims >= 3) ? L.hi[2] + 1 - L.lo[2] + 1 : 1;
+  const Shape sh = work_shape(e, c->dims, c->emf_chunk);
+  Launch t(c, name);
+#define EMF_ARGS                                                                              \
+  dim3(sh.blocks), dim3(BLK), 0, c->stream, c->g, w, c->fx, c->fy, c->fz, c->ee, e, sh.ty
+  if (sh.tiled) hipLaunchKernelGGL((k_emf<3, true>), EMF_ARGS);
+  else if (c->dims == 3) hipLaunchKernelGGL((k_emf<3, false>), EMF_ARGS);
+  else hipLaunchKernelGGL((k_emf<2, false>), EMF_ARGS);  // 1D grids run k_step1
+#undef EMF_ARGS
+}
+
+// One VL step after the dt kernels (vanleer.pyx:100-163).
+void launch_update(bnz_ctx *c, bool pred, const Lims &L) {
+  double *w1 = pred ? c->ws : c->w;
+  Box grid;
+  for (int q = 0; q < 3; ++q) {
+    grid.lo[q] = 0;
+    grid.n[q] = c->nt[q];
+  }
+  const Box act = active_box(c);
+  const int fr = c->p.dt_fms_ref;
+  const Shape sh = work_shape(grid, c->dims, c->update_chunk);
+  Launch t(c, pred ? "update_pred" : "update_corr");
+#define UPD_ARGS                                                                              \
+  dim3(sh.blocks), dim3(BLK), 0, c->stream, c->g, c->w, c->fx, c->fy, c->fz, c->ee, c->b, c->bs, \
+      w1, L, c->dev, grid, act, fr, c->part, sh.ty
+  // a uniform grid (every cell width bit for bit the same) takes its
+  // geometry from kernel arguments instead of per-cell loads
+  const bool ug = c->g.unif != 0;
+  if (sh.tiled) {  // 3D only
+    if (pred) hipLaunchKernelGGL((k_update<3, true, false, true>), UPD_ARGS);
+    else hipLaunchKernelGGL((k_update<3, false, true, true>), UPD_ARGS);
+  } else if (c->dims == 3 && ug) {
+    if (pred) hipLaunchKernelGGL((k_update<3, true, false, false, true>), UPD_ARGS);
+    else hipLaunchKernelGGL((k_update<3, false, true, false, true>), UPD_ARGS);
+  } else if (c->dims == 3) {
+    if (pred) hipLaunchKernelGGL((k_update<3, true, false, false>), UPD_ARGS);
+    else hipLaunchKernelGGL((k_update<3, false, true, false>), UPD_ARGS);
+  } else {  // 2D (1D grids run k_step1)
+    if (pred) hipLaunchKernelGGL((k_update<2, true, false, false>), UPD_ARGS);
+    else hipLaunchKernelGGL((k_update<2, false, true, false>), UPD_ARGS);
+  }
+#undef UPD_ARGS
+  // the corrector leaves the next step's CFL partials in part[0..blocks)
+  c->dt_parts = pred ? 0 : sh.blocks;
+}
+
+// A 1D step as one fused launch (k_step1): w, b -> ws, bs, then swapped.
+template <int ORDER, int RS>
+void step1_launch(bnz_ctx *c, const Lims &L0, const Lims &L1) {
+  const int nb = (c->nt[0] + STEP1_CELLS - 1) / STEP1_CELLS;
+  Launch t(c, "step1");
+  hipLaunchKernelGGL((k_step1<ORDER, RS>), dim3(nb), dim3(BLK), 0, c->stream, c->g, c->w, c->b,
+                     c->ws, c->bs, c->fx, L0, L1, c->dev, active_box(c), c->p.dt_fms_ref,
+                     c->part);
+  c->dt_parts = nb;
+  std::swap(c->w, c->ws);  // the faces swap with the other stages' below
+}
+
+void launch_step1(bnz_ctx *c, const Lims &L0, const Lims &L1) {
+  const int rs = c->p.rsolver;
+  if (c->order == 0) {
+    if (rs == BNZ_RS_HLLD) step1_launch<0, BNZ_RS_HLLD>(c, L0, L1);
+    else if (rs == BNZ_RS_HLLT) step1_launch<0, BNZ_RS_HLLT>(c, L0, L1);
+    else step1_launch<0, BNZ_RS_HLL>(c, L0, L1);
+  } else {
+    if (rs == BNZ_RS_HLLD) step1_launch<1, BNZ_RS_HLLD>(c, L0, L1);
+    else if (rs == BNZ_RS_HLLT) step1_launch<1, BNZ_RS_HLLT>(c, L0, L1);
+    else step1_launch<1, BNZ_RS_HLL>(c, L0, L1);
+  }
+}
+
+// One VL step after the dt kernels (vanleer.pyx:100-163).
+void launch_step_body(bnz_ctx *c) {
+  Lims L0 = integr_lims(c, 0);
+  Lims L1 = integr_lims(c, c->order);
+  if (c->dims == 1) {  // the whole step through LDS
+    launch_step1(c, L0, L1);
+  } else {
+    // predictor: donor cell, dt/2, lims of order 0; ws and bs are new buffers
+    launch_flux_dispatch(c, 0, c->w, c->b, L0);
+    launch_emf(c, c->w, L0, "emf_pred");
+    launch_update(c, true, L0);
+    // corrector: reconstruction order, dt; prim is updated in place (each
+    // cell reads only its own w_n), b_{n+1} goes to the predictor's face
+    // buffer, which is dead once the corrector fluxes and EMFs exist
+    launch_flux_dispatch(c, c->order, c->ws, c->bs, L1);
+    launch_emf(c, c->ws, L1, "emf_corr");
+    launch_update(c, false, L1);
+  }
+  std::swap(c->b, c->bs);
+  launch_ghost64(c);
+  launch_time_advance(c);
+}
+
+}  // namespace

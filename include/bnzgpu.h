@@ -39,10 +39,14 @@ enum { BNZ_RS_HLL = 0, BNZ_RS_HLLC = 1, BNZ_RS_HLLD = 2, BNZ_RS_HLLA = 3, BNZ_RS
 enum { BNZ_RCN_CONST = 0, BNZ_RCN_LINEAR = 1, BNZ_RCN_WENO = 2, BNZ_RCN_PARAB = 3 };
 /* Boundary flags: src/bc/grid_bc.pyx:45-50 (0 periodic, 1 outflow, 2 reflect, 3 user) */
 enum { BNZ_BC_PERIODIC = 0, BNZ_BC_OUTFLOW = 1, BNZ_BC_REFLECT = 2, BNZ_BC_USER = 3 };
-/* Arithmetic of the step: FP64 throughout (parity path), or FP32 fluxes,
-   reconstruction and EMFs with an FP32 hi + lo conserved state updated in
-   FP64 (pressure recovery, CT increments and dt in FP64). */
-enum { BNZ_PREC_FP64 = 0, BNZ_PREC_FP32C = 1 };
+/* Arithmetic of the step: FP64 throughout (parity path: IEEE, no
+   contraction, bit for bit the CPU restatement of the reference); FP32
+   fluxes, reconstruction and EMFs with an FP32 hi + lo conserved state
+   updated in FP64 (pressure recovery, CT increments and dt in FP64); or the
+   FP64 step with a*b+c fused into FMAs (single grids only: within 1e-12
+   relative L1 of the parity path, as the reference's own fast-math build
+   is, setup.py:436-451). */
+enum { BNZ_PREC_FP64 = 0, BNZ_PREC_FP32C = 1, BNZ_PREC_FP64_FMA = 2 };
 
 typedef struct {
   int nx, ny, nz;       /* active cells (ny=nz=1 in 1D, nz=1 in 2D), [computation] Nx,Ny,Nz */

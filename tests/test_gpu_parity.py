@@ -29,7 +29,7 @@ def rel_l1(a, b):
 
 
 def run_pair(prob, steps=None, tmax=None, rsolver="hlld", reconstr="linear", bc=(0,) * 6,
-             dt_fms_ref=True, cour=0.4):
+             dt_fms_ref=True, cour=0.4, precision="fp64"):
     from bonzo_amd._lib import Context
     from oracle.oracle import OracleSim
     s, prim, bfld = prob
@@ -43,7 +43,7 @@ def run_pair(prob, steps=None, tmax=None, rsolver="hlld", reconstr="linear", bc=
     o.bfld[:] = bfld
     o.init()
     g = Context(s.nx, s.ny, s.nz, rsolver=rsolver, reconstr=reconstr, bc=bc, cour=cour,
-                dt_fms_ref=dt_fms_ref)
+                dt_fms_ref=dt_fms_ref, precision=precision)
     g.upload(prim, bfld)
     g.init_state()
     if steps is not None:
@@ -538,3 +538,32 @@ def test_diagnostics_and_retuning_do_not_change_the_run(prec):
     assert np.array_equal(a.save_state(), b.save_state())
     a.close()
     b.close()
+
+
+FMA_CASES = {
+    "ot128_to_half": dict(prob=lambda: P.orszag_tang(128), tmax=0.5),
+    "turb64_100": dict(prob=lambda: P.turbulence(64), steps=100),
+    "turb24_hll_const": dict(prob=lambda: P.turbulence(24, seed_b=3, seed_v=4), steps=20,
+                             rsolver="hll", reconstr="const"),
+    "brio_wu_1d": dict(prob=lambda: P.brio_wu(400, axis=0, transverse=0), tmax=0.08,
+                       bc=(1,) * 6),
+    "hllt_ot32": dict(prob=lambda: P.orszag_tang(32), steps=40, rsolver="hllt"),
+    "blast2d_walls": dict(prob=lambda: P.blast2d(48, seed=21), tmax=0.05, bc=(2,) * 6),
+}
+
+
+@pytest.mark.parametrize("case", sorted(FMA_CASES))
+def test_fp64_fma_mode_within_1e12_of_oracle(case):
+    """The FP64 step with contracted multiply-adds (BNZ_PREC_FP64_FMA): the
+    same algorithm with each a*b+c rounded once, so it is not bit for bit the
+    oracle; every variable stays within the FP64 bar of 1e-12 relative L1,
+    with the same step count and time on the runs to t_max."""
+    kw = dict(FMA_CASES[case])
+    prob = kw.pop("prob")()
+    say(f"case: FP64-FMA mode vs oracle, {case}")
+    res = run_pair(prob, precision="fp64fma", **kw)
+    if "tmax" in kw:
+        assert res["n"] == res["o"].step and res["t"] == res["o"].time
+    compare(res, f"fp64fma {case}")
+    res["o"].close()
+    res["g"].close()
