@@ -463,6 +463,9 @@ int configure_grid(bnz_ctx *c, const bnz_params *p) {
     return fail(c, BNZ_EUNSUPPORTED, "hllt runs in FP64 only");
   if (p->reconstr != BNZ_RCN_CONST && p->reconstr != BNZ_RCN_LINEAR)
     return fail(c, BNZ_EUNSUPPORTED, "reconstr must be const or linear (weno/parab are commented out in reconstr_c.c)");
+  if (p->char_proj != 0 && p->char_proj != 1) return fail(c, BNZ_EINVAL, "char_proj must be 0 or 1");
+  if (p->char_proj && p->precision == BNZ_PREC_FP32C)
+    return fail(c, BNZ_EUNSUPPORTED, "characteristic limiting runs in FP64 only");
   for (int q = 0; q < 6; ++q)
     if (p->bc[q] != BNZ_BC_PERIODIC && p->bc[q] != BNZ_BC_OUTFLOW && p->bc[q] != BNZ_BC_REFLECT)
       return fail(c, BNZ_EUNSUPPORTED,

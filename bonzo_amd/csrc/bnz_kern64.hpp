@@ -15,7 +15,10 @@ namespace bnz {
 // the normal-field slot, which no consumer reads.  flux_face_to hands the
 // flux (local slots) of a face with a matched region to sink(f, c);
 // flux_face stores it.
-template <int DIR, int ORDER, int RS, class SINK>
+// CP (PLM only): the slopes are limited in characteristic variables of each
+// cell's own state (charact_proj, reconstr_c.c:64-80), which reads the
+// cell-centred normal field too.
+template <int DIR, int ORDER, int RS, class SINK, bool CP = false>
 __device__ __forceinline__ void flux_face_to(const Geo &g, const double *__restrict__ w,
                                              const double *__restrict__ bf, int i, int j, int k,
                                              const SINK &sink) {
@@ -24,20 +27,39 @@ __device__ __forceinline__ void flux_face_to(const Geo &g, const double *__restr
   const double bn = bf[(long)DIR * g.sv + c];
 
   double wl[NMODE], wr[NMODE];
+  if (ORDER == 1 && CP) {
+    double wm2[NMODE], wm1[NMODE], w0[NMODE], wp1[NMODE];
 #pragma unroll
-  for (int n = 0; n < NMODE; ++n) {
-    if (n == BX) continue;
-    const double *p = w + (long)perm<DIR>(n) * g.sv + c;
-    if (ORDER == 0) {
-      wl[n] = p[-sd];
-      wr[n] = p[0];
-    } else {
-      const double wm2 = p[-2 * sd], wm1 = p[-sd], w0 = p[0], wp1 = p[sd];
-      // cell c-1, upper state; cell c, lower state (reconstr_c.c:54-89)
-      const double dl = vl_half(wm1 - wm2, w0 - wm1);
-      const double dr = vl_half(w0 - wm1, wp1 - w0);
-      wl[n] = wm1 + dl;
-      wr[n] = w0 - dr;
+    for (int n = 0; n < NMODE; ++n) {
+      const double *p = w + (long)perm<DIR>(n) * g.sv + c;
+      wm2[n] = p[-2 * sd];
+      wm1[n] = p[-sd];
+      w0[n] = p[0];
+      wp1[n] = p[sd];
+    }
+    double d[NMODE];
+    plm_half_slope<true>(d, wm2, wm1, w0, g.gam);  // cell c-1, upper state
+#pragma unroll
+    for (int n = 0; n < NMODE; ++n) wl[n] = wm1[n] + d[n];
+    plm_half_slope<true>(d, wm1, w0, wp1, g.gam);  // cell c, lower state
+#pragma unroll
+    for (int n = 0; n < NMODE; ++n) wr[n] = w0[n] - d[n];
+  } else {
+#pragma unroll
+    for (int n = 0; n < NMODE; ++n) {
+      if (n == BX) continue;
+      const double *p = w + (long)perm<DIR>(n) * g.sv + c;
+      if (ORDER == 0) {
+        wl[n] = p[-sd];
+        wr[n] = p[0];
+      } else {
+        const double wm2 = p[-2 * sd], wm1 = p[-sd], w0 = p[0], wp1 = p[sd];
+        // cell c-1, upper state; cell c, lower state (reconstr_c.c:54-89)
+        const double dl = vl_half(wm1 - wm2, w0 - wm1);
+        const double dr = vl_half(w0 - wm1, wp1 - w0);
+        wl[n] = wm1 + dl;
+        wr[n] = w0 - dr;
+      }
     }
   }
   wl[BX] = bn;
@@ -55,12 +77,12 @@ __device__ __forceinline__ void flux_face_to(const Geo &g, const double *__restr
   sink(f, c);
 }
 
-template <int DIR, int ORDER, int RS>
+template <int DIR, int ORDER, int RS, bool CP = false>
 __device__ __forceinline__ void flux_face(const Geo &g, const double *__restrict__ w,
                                           const double *__restrict__ bf,
                                           double *__restrict__ fl, int i, int j, int k) {
   const long sv = g.sv;
-  flux_face_to<DIR, ORDER, RS>(g, w, bf, i, j, k, [fl, sv](const double *f, long c) {
+  const auto store = [fl, sv](const double *f, long c) {
     // streaming stores: the fluxes are read back by the next kernels, long
     // after any L2 could hold them, and bypassing it keeps the stencil rows
     // of the prim there (-4 % on the flux launches, -3 % on the EMFs)
@@ -69,7 +91,8 @@ __device__ __forceinline__ void flux_face(const Geo &g, const double *__restrict
       if (n == BX) continue;
       __builtin_nontemporal_store(f[n], &fl[(long)perm<DIR>(n) * sv + c]);
     }
-  });
+  };
+  flux_face_to<DIR, ORDER, RS, decltype(store), CP>(g, w, bf, i, j, k, store);
 }
 
 // Every face flux of one stage whose lower cell is (i, j, k): the x, y and
@@ -77,16 +100,16 @@ __device__ __forceinline__ void flux_face(const Geo &g, const double *__restrict
 // where that sweep's range includes it.  Over flux3_box(L) this covers
 // exactly the faces of the three per-direction launches.  DIMS > 0 fixes
 // the dimensionality at compile time; 0 reads it from g.
-template <int ORDER, int RS, int DIMS = 0>
+template <int ORDER, int RS, int DIMS = 0, bool CP = false>
 __device__ __forceinline__ void flux_cell(const Geo &g, const double *__restrict__ w,
                                           const double *__restrict__ bf,
                                           double *__restrict__ fx, double *__restrict__ fy,
                                           double *__restrict__ fz, const Lims &L, int i, int j,
                                           int k) {
   const int dims = DIMS > 0 ? DIMS : g.dims;
-  if (i >= L.lo[0]) flux_face<0, ORDER, RS>(g, w, bf, fx, i, j, k);
-  if (dims >= 2 && j >= L.lo[1]) flux_face<1, ORDER, RS>(g, w, bf, fy, i, j, k);
-  if (dims >= 3 && k >= L.lo[2]) flux_face<2, ORDER, RS>(g, w, bf, fz, i, j, k);
+  if (i >= L.lo[0]) flux_face<0, ORDER, RS, CP>(g, w, bf, fx, i, j, k);
+  if (dims >= 2 && j >= L.lo[1]) flux_face<1, ORDER, RS, CP>(g, w, bf, fy, i, j, k);
+  if (dims >= 3 && k >= L.lo[2]) flux_face<2, ORDER, RS, CP>(g, w, bf, fz, i, j, k);
 }
 
 __device__ __forceinline__ double ec_comp(const double *w, long sv, long c, int comp) {
@@ -413,7 +436,7 @@ __device__ __forceinline__ double stage1_cell(const Geo &g, const double *__rest
 // read races a write; the predictor's faces are not kept (in 1D the
 // corrector needs only the x faces, which the predictor copies).  Every
 // thread must call it; returns the CFL speed of the thread's corrector cell.
-template <int ORDER, int RS>
+template <int ORDER, int RS, bool CP = false>
 __device__ __forceinline__ double step1_block(const Geo &g, const double *w,
                                               const double *bf, double *wout, double *bout,
                                               const double *__restrict__ fx_old,
@@ -454,10 +477,11 @@ __device__ __forceinline__ double step1_block(const Geo &g, const double *w,
     Geo gl = g;
     gl.sv = BLK;
     bool got = false;
-    flux_face_to<0, ORDER, RS>(gl, sw - (i0 - 2), bf, ic, 0, 0, [&](const double *f, long q) {
+    const auto sink_c = [&](const double *f, long q) {
       sink(f, q);
       got = true;
-    });
+    };
+    flux_face_to<0, ORDER, RS, decltype(sink_c), CP>(gl, sw - (i0 - 2), bf, ic, 0, 0, sink_c);
     if (!got)
       for (int n = 0; n < NMODE; ++n)
         if (n != BX) sf[n * BLK + t] = fx_old[n * sv + ic];

@@ -161,6 +161,130 @@ __device__ __forceinline__ T vl_half(T a, T b) {
   return ab > T(0) ? q : T(0);
 }
 
+// Characteristic projection of PLM (src/mhd/eos_c.h:100-280, MFIELD branch),
+// used when charact_proj is set (src/coordinates/reconstr_c.c:64-80): the
+// primitive differences of a cell are limited as amplitudes of the
+// eigenvectors of the cell's own state w (local frame, w[BX] the cell-centred
+// normal field).  Slots after prim2char: RHO fast-, VX Alfven-, VY slow-,
+// VZ entropy, PR slow+, BY Alfven+, BZ fast+; PSC and BX pass through.
+template <typename T>
+struct CharWeights {
+  T rhoi, s, a2rho, a, cf, cs, alphaf, alphas, betay, betaz;
+  __device__ __forceinline__ CharWeights(const T *w, T gam) {
+    const T w_rho = w[RHO], w_p = w[PR], w_bx = w[BX], w_by = w[BY], w_bz = w[BZ];
+    rhoi = Ar<T>::div(T(1), w_rho);
+    s = sign0(w_bx);
+    a2rho = gam * w_p;
+    a = Ar<T>::root(a2rho * rhoi);
+    const T bx2 = sqr(w_bx);
+    const T bperp2 = sqr(w_by) + sqr(w_bz);
+    const T b2 = bx2 + bperp2;
+    const T x = a2rho + b2;
+    const T y = Ar<T>::root(fabs(x * x - T(4) * a2rho * bx2));
+    const T cf2rho = T(0.5) * (x + y);
+    const T cs2rho = T(0.5) * (x - y);
+    cf = Ar<T>::root(rhoi * cf2rho);
+    cs = Ar<T>::root(rhoi * fabs(cs2rho));
+    if (cf <= cs) {
+      alphaf = T(1);
+      alphas = T(0);
+    } else if (a <= cs) {
+      alphaf = T(0);
+      alphas = T(1);
+    } else if (cf <= a) {
+      alphaf = T(1);
+      alphas = T(0);
+    } else {
+      // z = 1 / (cf2rho - cs2rho) is needed only here, where cf > cs
+      const T z = Ar<T>::div(T(1), cf2rho - cs2rho);
+      alphaf = Ar<T>::root(z * fabs(a2rho - cs2rho));
+      alphas = Ar<T>::root(z * fabs(cf2rho - a2rho));
+    }
+    betay = T(0);
+    betaz = T(0);
+    if (bperp2 != T(0)) {
+      const T bperpi = Ar<T>::div(T(1), Ar<T>::root(bperp2));
+      betay = bperpi * w_by;
+      betaz = bperpi * w_bz;
+    }
+  }
+};
+
+template <typename T>
+__device__ __forceinline__ void prim2char(T *vc, const CharWeights<T> &c) {
+  const T vc_rho = vc[RHO], vc_vx = vc[VX], vc_vy = vc[VY], vc_vz = vc[VZ], vc_p = vc[PR];
+  const T vc_by = vc[BY], vc_bz = vc[BZ];
+  const T rhoi = c.rhoi, cf = c.cf, cs = c.cs, betay = c.betay, betaz = c.betaz;
+  const T nf = Ar<T>::div(T(0.5), c.a2rho * rhoi);
+  const T qf_ = nf * cf * c.alphaf * c.s;
+  const T qs_ = nf * cs * c.alphas * c.s;
+  const T alphaf_ = nf * c.alphaf;
+  const T alphas_ = nf * c.alphas;
+  const T rhoisr = Ar<T>::root(rhoi);
+  const T arhoisr = c.a * rhoisr;
+  const T Af_ = alphaf_ * arhoisr;
+  const T As_ = alphas_ * arhoisr;
+  const T x = c.s * rhoisr;
+  vc[RHO] = (alphaf_ * (rhoi * vc_p - cf * vc_vx) + qs_ * (betay * vc_vy + betaz * vc_vz) +
+             As_ * (betay * vc_by + betaz * vc_bz));
+  vc[VX] = T(0.5) * ((-betaz * vc_vy + betay * vc_vz) + x * (-betaz * vc_by + betay * vc_bz));
+  vc[VY] = (alphas_ * (rhoi * vc_p - cs * vc_vx) - qf_ * (betay * vc_vy + betaz * vc_vz) -
+            Af_ * (betay * vc_by + betaz * vc_bz));
+  vc[VZ] = vc_rho - T(2) * nf * vc_p;
+  vc[PR] = (alphas_ * (rhoi * vc_p + cs * vc_vx) + qf_ * (betay * vc_vy + betaz * vc_vz) -
+            Af_ * (betay * vc_by + betaz * vc_bz));
+  vc[BY] = T(0.5) * ((betaz * vc_vy - betay * vc_vz) + x * (-betaz * vc_by + betay * vc_bz));
+  vc[BZ] = (alphaf_ * (rhoi * vc_p + cf * vc_vx) - qs_ * (betay * vc_vy + betaz * vc_vz) +
+            As_ * (betay * vc_by + betaz * vc_bz));
+}
+
+template <typename T>
+__device__ __forceinline__ void char2prim(T *vc, const T *w, const CharWeights<T> &c) {
+  const T vc_rho = vc[RHO], vc_vx = vc[VX], vc_vy = vc[VY], vc_vz = vc[VZ], vc_p = vc[PR];
+  const T vc_by = vc[BY], vc_bz = vc[BZ];
+  const T w_rho = w[RHO];
+  const T cf = c.cf, cs = c.cs, alphaf = c.alphaf, alphas = c.alphas, s = c.s;
+  const T betay = c.betay, betaz = c.betaz;
+  const T qf = cf * alphaf * s;
+  const T qs = cs * alphas * s;
+  const T rhosr = Ar<T>::root(w_rho);
+  const T arhosr = c.a * rhosr;
+  const T Af = alphaf * arhosr;
+  const T As = alphas * arhosr;
+  vc[RHO] = w_rho * (alphaf * (vc_rho + vc_bz) + alphas * (vc_vy + vc_p)) + vc_vz;
+  vc[VX] = cf * alphaf * (vc_bz - vc_rho) + cs * alphas * (vc_p - vc_vy);
+  vc[VY] = betay * (qs * (vc_rho - vc_bz) - qf * (vc_vy - vc_p)) - betaz * (vc_vx - vc_by);
+  vc[VZ] = betaz * (qs * (vc_rho - vc_bz) - qf * (vc_vy - vc_p)) + betay * (vc_vx - vc_by);
+  vc[PR] = c.a2rho * (alphaf * (vc_rho + vc_bz) + alphas * (vc_vy + vc_p));
+  vc[BY] = betay * (As * (vc_rho + vc_bz) - Af * (vc_vy + vc_p)) - betaz * s * rhosr * (vc_vx + vc_by);
+  vc[BZ] = betaz * (As * (vc_rho + vc_bz) - Af * (vc_vy + vc_p)) + betay * s * rhosr * (vc_vx + vc_by);
+}
+
+// Half the limited slope of a cell (PLM), d = 0.5 vl_lim(dm, dp) for every
+// slot, in characteristic variables of the cell state wc when CP is set
+// (reconstr_c.c:54-91).
+template <bool CP, typename T>
+__device__ __forceinline__ void plm_half_slope(T *d, const T *wm, const T *wc, const T *wp,
+                                               T gam) {
+  T dm[NMODE], dp[NMODE];
+#pragma unroll
+  for (int n = 0; n < NMODE; ++n) {
+    dm[n] = wc[n] - wm[n];
+    dp[n] = wp[n] - wc[n];
+  }
+  if (CP) {
+    const CharWeights<T> cw(wc, gam);
+    prim2char(dm, cw);
+    prim2char(dp, cw);
+#pragma unroll
+    for (int n = 0; n < NMODE; ++n) d[n] = vl_half(dm[n], dp[n]);
+    char2prim(d, wc, cw);
+  } else {
+#pragma unroll
+    for (int n = 0; n < NMODE; ++n) d[n] = vl_half(dm[n], dp[n]);
+  }
+}
+
 // sLsR_simple, src/mhd/fluxes_c.c:25-42 (FMAX/FMIN as fmax/fmin)
 template <typename T>
 __device__ __forceinline__ void speeds_simple(T &sL, T &sR, const T *wL, const T *wR, T gam) {

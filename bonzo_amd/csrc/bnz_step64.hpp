@@ -29,7 +29,7 @@ using namespace bnz;
 // All three face directions of a stage in one launch: one thread per cell of
 // the union of the face ranges computes the x, y and z faces whose lower
 // cell it is, so the prim stencil of a cell is fetched once per stage.
-template <int ORDER, int RS, bool TILED = false>
+template <int ORDER, int RS, bool TILED = false, bool CP = false>
 __global__ __launch_bounds__(BLK) void k_flux3(Geo g, const double *__restrict__ w,
                                                const double *__restrict__ bf,
                                                double *__restrict__ fx, double *__restrict__ fy,
@@ -38,14 +38,14 @@ __global__ __launch_bounds__(BLK) void k_flux3(Geo g, const double *__restrict__
   if (dts[0] == 0.) return;  // a step past t_max (see k_update)
   int i, j, k;
   if (!cell_index<TILED, 1, 2>(box, ty, i, j, k)) return;
-  flux_cell<ORDER, RS>(g, w, bf, fx, fy, fz, L, i, j, k);
+  flux_cell<ORDER, RS, 0, CP>(g, w, bf, fx, fy, fz, L, i, j, k);
 }
 
 // A whole VL step of a 1D grid in one pass (step1_block): workgroup b owns
 // the cells from b * STEP1_CELLS; the new prim and faces go to wout and
 // bout, which the host then swaps with w and b; the workgroup's CFL maximum
 // goes to part.
-template <int ORDER, int RS>
+template <int ORDER, int RS, bool CP = false>
 __global__ __launch_bounds__(BLK) void k_step1(Geo g, const double *__restrict__ w,
                                                const double *__restrict__ bf,
                                                double *__restrict__ wout,
@@ -64,8 +64,8 @@ __global__ __launch_bounds__(BLK) void k_step1(Geo g, const double *__restrict__
     }
     return;
   }
-  const double cds = step1_block<ORDER, RS>(g, w, bf, wout, bout, fx_old, L0, L1, dts, act,
-                                            fms_ref, i0, sf, sw);
+  const double cds = step1_block<ORDER, RS, CP>(g, w, bf, wout, bout, fx_old, L0, L1, dts, act,
+                                                fms_ref, i0, sf, sw);
   block_max_to(cds, part);
 }
 
@@ -113,18 +113,25 @@ __global__ __launch_bounds__(BLK) void k_update(Geo g, const double *w0, const d
 // Launches
 // ==========================================================================
 
-template <int ORDER, int RS>
-void launch_fluxes(bnz_ctx *c, const double *w, const double *bf, const Lims &L) {
+template <int ORDER, int RS, bool CP>
+void launch_fluxes_cp(bnz_ctx *c, const double *w, const double *bf, const Lims &L) {
   // all face directions in one launch over the union of the face ranges
   const Box u = flux3_box(c->dims, L);
   const Shape sh = work_shape(u, c->dims, c->flux_chunk);
-  Launch t(c, ORDER ? "flux3_plm" : "flux3_dc");
+  Launch t(c, ORDER ? (CP ? "flux3_plm_char" : "flux3_plm") : "flux3_dc");
   if (sh.tiled)
-    hipLaunchKernelGGL((k_flux3<ORDER, RS, true>), dim3(sh.blocks), dim3(BLK), 0, c->stream, c->g,
-                       w, bf, c->fx, c->fy, c->fz, u, L, 0, c->dev);
+    hipLaunchKernelGGL((k_flux3<ORDER, RS, true, CP>), dim3(sh.blocks), dim3(BLK), 0, c->stream,
+                       c->g, w, bf, c->fx, c->fy, c->fz, u, L, 0, c->dev);
   else
-    hipLaunchKernelGGL((k_flux3<ORDER, RS, false>), dim3(sh.blocks), dim3(BLK), 0, c->stream,
+    hipLaunchKernelGGL((k_flux3<ORDER, RS, false, CP>), dim3(sh.blocks), dim3(BLK), 0, c->stream,
                        c->g, w, bf, c->fx, c->fy, c->fz, u, L, sh.ty, c->dev);
+}
+
+// PLM limits in characteristic variables when the context asks for it
+template <int ORDER, int RS>
+void launch_fluxes(bnz_ctx *c, const double *w, const double *bf, const Lims &L) {
+  if (ORDER == 1 && c->p.char_proj) launch_fluxes_cp<ORDER, RS, true>(c, w, bf, L);
+  else launch_fluxes_cp<ORDER, RS, false>(c, w, bf, L);
 }
 
 void launch_flux_dispatch(bnz_ctx *c, int order, const double *w, const double *bf,
@@ -197,9 +204,14 @@ template <int ORDER, int RS>
 void step1_launch(bnz_ctx *c, const Lims &L0, const Lims &L1) {
   const int nb = (c->nt[0] + STEP1_CELLS - 1) / STEP1_CELLS;
   Launch t(c, "step1");
-  hipLaunchKernelGGL((k_step1<ORDER, RS>), dim3(nb), dim3(BLK), 0, c->stream, c->g, c->w, c->b,
-                     c->ws, c->bs, c->fx, L0, L1, c->dev, active_box(c), c->p.dt_fms_ref,
-                     c->part);
+  if (ORDER == 1 && c->p.char_proj)
+    hipLaunchKernelGGL((k_step1<ORDER, RS, true>), dim3(nb), dim3(BLK), 0, c->stream, c->g, c->w,
+                       c->b, c->ws, c->bs, c->fx, L0, L1, c->dev, active_box(c),
+                       c->p.dt_fms_ref, c->part);
+  else
+    hipLaunchKernelGGL((k_step1<ORDER, RS, false>), dim3(nb), dim3(BLK), 0, c->stream, c->g, c->w,
+                       c->b, c->ws, c->bs, c->fx, L0, L1, c->dev, active_box(c),
+                       c->p.dt_fms_ref, c->part);
   c->dt_parts = nb;
   std::swap(c->w, c->ws);  // the faces swap with the other stages' below
 }

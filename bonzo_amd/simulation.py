@@ -123,8 +123,11 @@ class BnzIntegr:
                           "(src/simulation.pyx:67-75); so does this one", stacklevel=2)
         if self.reconstr in (Reconstr.RCN_PARAB, Reconstr.RCN_WENO):
             raise NotImplementedError("parab/weno are commented out in src/coordinates/reconstr_c.c")
-        if self.char_proj:
-            raise NotImplementedError("charact_proj=1 is not supported (src/mhd/eos_c.c:61-63)")
+        # charact_proj (integrator.pyx:245; read into the attribute the
+        # reconstruction uses, char_proj, integrator.pxd:167, as intended):
+        # PLM slopes limited in characteristic variables
+        if self.char_proj not in (0, 1):
+            raise ValueError("charact_proj must be 0 or 1")
         # Physics the reference adds around the step whenever its coefficients
         # are non-zero: constant gravity (src/srcterms/gravity.pyx:15-17,
         # applied in integrator.pyx:146-157), turbulence driving
@@ -209,7 +212,7 @@ class Simulation:
                                 gam=self.integr.gam, cour=self.integr.cour,
                                 dt_fms_ref=self.integr.dt_fms_ref,
                                 precision=p.get("computation", "precision", "s", "fp64"),
-                                device=device)
+                                device=device, char_proj=bool(self.integr.char_proj))
         if set_problem is None:
             name = p.get("problem", "name", "s", "otvortex")
             prim, bfld = PROBLEMS[name](self.grid, p)

@@ -61,6 +61,10 @@ typedef struct {
                            src/integration/new_dt.pyx:76-91 does; 0: per axis */
   int precision;        /* BNZ_PREC_*                                      */
   int device;           /* HIP device ordinal                              */
+  int char_proj;        /* [computation] charact_proj: 1 limits the PLM
+                           slopes in characteristic variables of each cell's
+                           state (src/coordinates/reconstr_c.c:64-80,
+                           src/mhd/eos_c.h:100-280); FP64 single grids only */
 } bnz_params;
 
 typedef struct {

@@ -396,6 +396,131 @@ void or_hlld_flux(double **_flux, double **_wL, double **_wR, double *_bx, int i
 }
 
 /* ------------------------------------------------------------------------ */
+/* Characteristic projection (MFIELD branch of src/mhd/eos_c.h:100-280)      */
+/* ------------------------------------------------------------------------ */
+
+/* Wave speeds and eigenvector weights of the state w, shared by prim2char
+   (eos_c.h:100-161) and char2prim (eos_c.h:222-262). */
+typedef struct {
+  double rhoi, s, a2rho, a, cf, cs, alphaf, alphas, betay, betaz;
+} or_charw;
+
+static void char_weights(or_charw *c, const double *w, double gam)
+{
+  double w_rho = w[RHO], w_p = w[PR], w_bx = w[BX], w_by = w[BY], w_bz = w[BZ];
+  c->rhoi = 1. / w_rho;
+  c->s = sign0(w_bx);
+  c->a2rho = gam * w_p;
+  c->a = sqrt(c->a2rho * c->rhoi);
+  double bx2 = sqr(w_bx);
+  double bperp2 = sqr(w_by) + sqr(w_bz);
+  double b2 = bx2 + bperp2;
+  double x = c->a2rho + b2;
+  double y = sqrt(fabs(x * x - 4. * c->a2rho * bx2));
+  double cf2rho = 0.5 * (x + y);
+  double cs2rho = 0.5 * (x - y);
+  c->cf = sqrt(c->rhoi * cf2rho);
+  c->cs = sqrt(c->rhoi * fabs(cs2rho));
+  double z = 1. / (cf2rho - cs2rho);
+  if (c->cf <= c->cs) {
+    c->alphaf = 1.;
+    c->alphas = 0.;
+  } else if (c->a <= c->cs) {
+    c->alphaf = 0.;
+    c->alphas = 1.;
+  } else if (c->cf <= c->a) {
+    c->alphaf = 1.;
+    c->alphas = 0.;
+  } else {
+    c->alphaf = sqrt(z * fabs(c->a2rho - cs2rho));
+    c->alphas = sqrt(z * fabs(cf2rho - c->a2rho));
+  }
+  c->betay = 0.;
+  c->betaz = 0.;
+  if (bperp2 != 0.) {
+    double bperpi = 1. / sqrt(bperp2);
+    c->betay = bperpi * w_by;
+    c->betaz = bperpi * w_bz;
+  }
+}
+
+/* prim2char: primitive vector vc -> characteristic amplitudes in the
+   eigensystem of state w (slots RHO fast-, VX Alfven-, VY slow-, VZ entropy,
+   PR slow+, BY Alfven+, BZ fast+; PSC and BX pass through). */
+void or_prim2char(double *vc, const double *w, double gam)
+{
+  double vc_rho = vc[RHO], vc_vx = vc[VX], vc_vy = vc[VY], vc_vz = vc[VZ], vc_p = vc[PR];
+  double vc_by = vc[BY], vc_bz = vc[BZ];
+  or_charw c;
+  char_weights(&c, w, gam);
+  double rhoi = c.rhoi, cf = c.cf, cs = c.cs, betay = c.betay, betaz = c.betaz;
+  double nf = 0.5 / (c.a2rho * rhoi);
+  double qf_ = nf * cf * c.alphaf * c.s;
+  double qs_ = nf * cs * c.alphas * c.s;
+  double alphaf_ = nf * c.alphaf;
+  double alphas_ = nf * c.alphas;
+  double rhoisr = sqrt(rhoi);
+  double arhoisr = c.a * rhoisr;
+  double Af_ = alphaf_ * arhoisr;
+  double As_ = alphas_ * arhoisr;
+  double x = c.s * rhoisr;
+  vc[RHO] = (alphaf_ * (rhoi * vc_p - cf * vc_vx) + qs_ * (betay * vc_vy + betaz * vc_vz) +
+             As_ * (betay * vc_by + betaz * vc_bz));
+  vc[VX] = 0.5 * ((-betaz * vc_vy + betay * vc_vz) + x * (-betaz * vc_by + betay * vc_bz));
+  vc[VY] = (alphas_ * (rhoi * vc_p - cs * vc_vx) - qf_ * (betay * vc_vy + betaz * vc_vz) -
+            Af_ * (betay * vc_by + betaz * vc_bz));
+  vc[VZ] = vc_rho - 2. * nf * vc_p;
+  vc[PR] = (alphas_ * (rhoi * vc_p + cs * vc_vx) + qf_ * (betay * vc_vy + betaz * vc_vz) -
+            Af_ * (betay * vc_by + betaz * vc_bz));
+  vc[BY] = 0.5 * ((betaz * vc_vy - betay * vc_vz) + x * (-betaz * vc_by + betay * vc_bz));
+  vc[BZ] = (alphaf_ * (rhoi * vc_p + cf * vc_vx) - qs_ * (betay * vc_vy + betaz * vc_vz) +
+            As_ * (betay * vc_by + betaz * vc_bz));
+}
+
+/* char2prim: the inverse map (eos_c.h:191-280). */
+void or_char2prim(double *vc, const double *w, double gam)
+{
+  double vc_rho = vc[RHO], vc_vx = vc[VX], vc_vy = vc[VY], vc_vz = vc[VZ], vc_p = vc[PR];
+  double vc_by = vc[BY], vc_bz = vc[BZ];
+  double w_rho = w[RHO];
+  or_charw c;
+  char_weights(&c, w, gam);
+  double cf = c.cf, cs = c.cs, alphaf = c.alphaf, alphas = c.alphas, s = c.s;
+  double betay = c.betay, betaz = c.betaz;
+  double qf = cf * alphaf * s;
+  double qs = cs * alphas * s;
+  double rhosr = sqrt(w_rho);
+  double arhosr = c.a * rhosr;
+  double Af = alphaf * arhosr;
+  double As = alphas * arhosr;
+  vc[RHO] = w_rho * (alphaf * (vc_rho + vc_bz) + alphas * (vc_vy + vc_p)) + vc_vz;
+  vc[VX] = cf * alphaf * (vc_bz - vc_rho) + cs * alphas * (vc_p - vc_vy);
+  vc[VY] = betay * (qs * (vc_rho - vc_bz) - qf * (vc_vy - vc_p)) - betaz * (vc_vx - vc_by);
+  vc[VZ] = betaz * (qs * (vc_rho - vc_bz) - qf * (vc_vy - vc_p)) + betay * (vc_vx - vc_by);
+  vc[PR] = c.a2rho * (alphaf * (vc_rho + vc_bz) + alphas * (vc_vy + vc_p));
+  vc[BY] = betay * (As * (vc_rho + vc_bz) - Af * (vc_vy + vc_p)) - betaz * s * rhosr * (vc_vx + vc_by);
+  vc[BZ] = betaz * (As * (vc_rho + vc_bz) - Af * (vc_vy + vc_p)) + betay * s * rhosr * (vc_vx + vc_by);
+}
+
+/* prim2char_1 / char2prim_1 (src/mhd/eos_c.c:53-94) as their loops evidently
+   intend (D4: the shipped bodies put only the w[n] copy inside the n loop,
+   leaving vc[n] with an undeclared n): every slot of rows vc1 and w1 at
+   each i in [i1, i2] is gathered, projected, and written back. */
+static void prim2char_rows(double **vc1, double **w1, int i1, int i2, double gam, int inverse)
+{
+  double w[NMODE], vc[NMODE];
+  for (int i = i1; i <= i2; ++i) {
+    for (int n = 0; n < NMODE; ++n) {
+      w[n] = w1[n][i];
+      vc[n] = vc1[n][i];
+    }
+    if (inverse) or_char2prim(vc, w, gam);
+    else or_prim2char(vc, w, gam);
+    for (int n = 0; n < NMODE; ++n) vc1[n][i] = vc[n];
+  }
+}
+
+/* ------------------------------------------------------------------------ */
 /* Reconstruction                                                            */
 /* ------------------------------------------------------------------------ */
 
@@ -412,9 +537,11 @@ void or_reconstr_const(double **wl, double **wr, double ***w, double ***scr, int
     }
 }
 
-/* src/coordinates/reconstr_c.c:29-91, uniform flat branch, char_proj=0 */
-void or_reconstr_linear(double **wl, double **wr, double ***w, double ***scr, int ax, int i1,
-                        int i2)
+/* src/coordinates/reconstr_c.c:29-91, uniform flat branch; char_proj
+   limits the slopes in characteristic variables of the cell's own state
+   (reconstr_c.c:64-67,79-80). */
+void or_reconstr_linear_cp(double **wl, double **wr, double ***w, double ***scr, int ax, int i1,
+                           int i2, int char_proj, double gam)
 {
   double **dm = scr[0], **dp = scr[1], **d = scr[0], **wc = scr[2];
   int ip = (ax == 0) ? 1 : 0;
@@ -424,13 +551,24 @@ void or_reconstr_linear(double **wl, double **wr, double ***w, double ***scr, in
       dm[n][i] = w[1][n][i] - w[0][n][i];
       dp[n][i] = w[2][n][i] - w[1][n][i];
     }
+  if (char_proj) {
+    prim2char_rows(dm, wc, i1, i2, gam, 0);
+    prim2char_rows(dp, wc, i1, i2, gam, 0);
+  }
   for (int n = 0; n < NMODE; n++)
     for (int i = i1; i <= i2; i++) d[n][i] = 0.5 * or_vl_lim(dm[n][i], dp[n][i]);
+  if (char_proj) prim2char_rows(d, wc, i1, i2, gam, 1);
   for (int n = 0; n < NMODE; n++)
     for (int i = i1; i <= i2; i++) {
       wr[n][i] = wc[n][i] - d[n][i];
       wl[n][i + ip] = wc[n][i] + d[n][i];
     }
+}
+
+void or_reconstr_linear(double **wl, double **wr, double ***w, double ***scr, int ax, int i1,
+                        int i2)
+{
+  or_reconstr_linear_cp(wl, wr, w, scr, ax, i1, i2, 0, 0.);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -441,7 +579,7 @@ typedef void (*rsolver_fn)(double **, double **, double **, double *, int, int, 
 typedef void (*reconstr_fn)(double **, double **, double ***, double ***, int, int, int);
 
 struct or_sim {
-  int dims, ng, reconstr_order, nthreads, dt_fms_ref;
+  int dims, ng, reconstr_order, nthreads, dt_fms_ref, char_proj;
   int Nact[3], Ntot[3];
   int i1, i2, j1, j2, k1, k2;
   double lmin[3], lmax[3];
@@ -496,6 +634,7 @@ void or_history(or_sim *s, double *out)
   for (int q = 13; q < 16; ++q) out[q] = sqrt(fabs(out[q] - sqr(out[q - 3])));
 }
 void or_set_dt_fms_ref(or_sim *s, int on) { s->dt_fms_ref = on; }
+void or_set_char_proj(or_sim *s, int on) { s->char_proj = on; }
 
 double *or_coord(or_sim *s, int which, int axis)
 {
@@ -672,6 +811,14 @@ static void free_rows(double **r)
 }
 
 /* godunov_fluxes: src/integration/godunov.pyx:20-210 (D1 applied) */
+/* Reconstruction of the stage (reconstr_func of godunov.pyx:103-197): donor
+   cell for order 0, PLM otherwise, in characteristic variables when
+   char_proj is set (integrator.pyx:245, D4). */
+#define RECONSTR(wl_, wr_, w_, scr_, ax_, a_, b_)                                    \
+  ((order) == 0 ? or_reconstr_const(wl_, wr_, w_, scr_, ax_, a_, b_)                \
+                : or_reconstr_linear_cp(wl_, wr_, w_, scr_, ax_, a_, b_, s->char_proj, \
+                                        s->gam))
+
 static void godunov_fluxes(or_sim *s, const double *prim, const double *bfld, const int *lims,
                            int order)
 {
@@ -681,7 +828,6 @@ static void godunov_fluxes(or_sim *s, const double *prim, const double *bfld, co
   swap_mhd_vec(1, varsy);
   swap_mhd_vec(2, varsz);
   rsolver_fn rs = pick_rsolver(s->rsolver);
-  reconstr_fn rc = (order == 0) ? or_reconstr_const : or_reconstr_linear;
   int nx = s->Ntot[0];
   int len = nx + 4;
   int nst = 2 * order + 1;
@@ -708,7 +854,7 @@ static void godunov_fluxes(or_sim *s, const double *prim, const double *bfld, co
         for (int d = 0; d < nst; ++d)
           for (int n = 0; n < NMODE; ++n)
             prim1[d][n] = pbase + IDX(s, varsx[n], k, j, 0) + d - order;
-        rc(wl, wr, prim1, scr, 0, il - 1, iu + 1);
+        RECONSTR(wl, wr, prim1, scr, 0, il - 1, iu + 1);
         for (int n = 0; n < NMODE; ++n) flx1[n] = s->flx[0] + IDX(s, varsx[n], k, j, 0);
         rs(flx1, wl, wr, (double *)bfld + IDX(s, 0, k, j, 0), il, iu + 1, s->gam);
       }
@@ -725,12 +871,12 @@ static void godunov_fluxes(or_sim *s, const double *prim, const double *bfld, co
         for (int d = 0; d < nst; ++d)
           for (int n = 0; n < NMODE; ++n)
             prim1[d][n] = pbase + IDX(s, varsy[n], k, jl - 1 + d - order, 0);
-        rc(wlc, wr, prim1, scr, 1, ib, ie);
+        RECONSTR(wlc, wr, prim1, scr, 1, ib, ie);
         for (int j = jl; j <= ju + 1; ++j) {
           for (int d = 0; d < nst; ++d)
             for (int n = 0; n < NMODE; ++n)
               prim1[d][n] = pbase + IDX(s, varsy[n], k, j + d - order, 0);
-          rc(wln, wr, prim1, scr, 1, ib, ie);
+          RECONSTR(wln, wr, prim1, scr, 1, ib, ie);
           for (int n = 0; n < NMODE; ++n) flx1[n] = s->flx[1] + IDX(s, varsy[n], k, j, 0);
           rs(flx1, wlc, wr, (double *)bfld + IDX(s, 1, k, j, 0), ib, ie, s->gam);
           double **t = wlc; /* D1: the intended swap */
@@ -751,12 +897,12 @@ static void godunov_fluxes(or_sim *s, const double *prim, const double *bfld, co
         for (int d = 0; d < nst; ++d)
           for (int n = 0; n < NMODE; ++n)
             prim1[d][n] = pbase + IDX(s, varsz[n], kl - 1 + d - order, j, 0);
-        rc(wlc, wr, prim1, scr, 2, ib, ie);
+        RECONSTR(wlc, wr, prim1, scr, 2, ib, ie);
         for (int k = kl; k <= ku + 1; ++k) {
           for (int d = 0; d < nst; ++d)
             for (int n = 0; n < NMODE; ++n)
               prim1[d][n] = pbase + IDX(s, varsz[n], k + d - order, j, 0);
-          rc(wln, wr, prim1, scr, 2, ib, ie);
+          RECONSTR(wln, wr, prim1, scr, 2, ib, ie);
           for (int n = 0; n < NMODE; ++n) flx1[n] = s->flx[2] + IDX(s, varsz[n], k, j, 0);
           rs(flx1, wlc, wr, (double *)bfld + IDX(s, 2, k, j, 0), ib, ie, s->gam);
           double **t = wlc;

@@ -33,6 +33,10 @@ void   or_prim2cons(double *u, const double *w, double gam);
 void   or_primcons2flux(double *f, const double *w, const double *u, double gam);
 double or_fms(const double *w, double bx, double gam);
 double or_vl_lim(double a, double b);
+/* characteristic projection (src/mhd/eos_c.h:100-280): in place on vc, in the
+   eigensystem of state w */
+void   or_prim2char(double *vc, const double *w, double gam);
+void   or_char2prim(double *vc, const double *w, double gam);
 
 /* ---- pencil ABI (src/mhd/fluxes_c.h:3-21; integrator.pxd:76-79) ---- */
 void or_hll_flux (double **flux, double **wl, double **wr, double *bx, int i1, int i2, double gam);
@@ -42,6 +46,8 @@ void or_hlld_flux(double **flux, double **wl, double **wr, double *bx, int i1, i
 /* ---- reconstruction (src/coordinates/reconstr_c.h:4-12), uniform grids ---- */
 void or_reconstr_const (double **wl, double **wr, double ***w, double ***scr, int ax, int i1, int i2);
 void or_reconstr_linear(double **wl, double **wr, double ***w, double ***scr, int ax, int i1, int i2);
+void or_reconstr_linear_cp(double **wl, double **wr, double ***w, double ***scr, int ax, int i1,
+                           int i2, int char_proj, double gam);
 
 /* ---- whole-grid simulation object ---- */
 typedef struct or_sim or_sim;
@@ -59,6 +65,8 @@ double *or_cons(or_sim *s);
 double *or_bfld(or_sim *s);
 double *or_coord(or_sim *s, int which, int axis); /* 0 lf,1 lv,2 dlf,3 dlf_inv */
 void    or_set_dt_fms_ref(or_sim *s, int on);
+/* PLM in characteristic variables (charact_proj=1, src/coordinates/reconstr_c.c:64-80) */
+void    or_set_char_proj(or_sim *s, int on);
 void    or_apply_grid_bc(or_sim *s);
 void    or_init(or_sim *s);
 double  or_new_dt(or_sim *s);

@@ -58,6 +58,9 @@ def lib(path=None):
         L.or_ng.restype = ctypes.c_int
         L.or_ng.argtypes = [ctypes.c_void_p]
         L.or_set_dt_fms_ref.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.or_set_char_proj.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        for name in ("or_prim2char", "or_char2prim"):
+            getattr(L, name).argtypes = [_dp, _dp, ctypes.c_double]
         L.or_apply_grid_bc.argtypes = [ctypes.c_void_p]
         L.or_init.argtypes = [ctypes.c_void_p]
         L.or_new_dt.restype = ctypes.c_double
@@ -136,6 +139,23 @@ def vl_lim(a, b):
     return lib().or_vl_lim(float(a), float(b))
 
 
+def prim2char(vc, w, gam):
+    """Characteristic amplitudes of the primitive vector vc in the eigensystem
+    of state w (src/mhd/eos_c.h:100-186)."""
+    vc = np.array(vc, dtype=np.float64)
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    lib().or_prim2char(_ptr(vc), _ptr(w), gam)
+    return vc
+
+
+def char2prim(vc, w, gam):
+    """Inverse of prim2char (src/mhd/eos_c.h:191-280)."""
+    vc = np.array(vc, dtype=np.float64)
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    lib().or_char2prim(_ptr(vc), _ptr(w), gam)
+    return vc
+
+
 def riemann(rsolver, wl, wr, bx, gam):
     """Pencil call of a solver: wl, wr are [9, n] primitive states, bx [n].
 
@@ -182,7 +202,8 @@ class OracleSim:
 
     def __init__(self, nx, ny=1, nz=1, reconstr_order=1, lmin=(0., 0., 0.),
                  lmax=(1., 1., 1.), gam=5. / 3., cour=0.4, rsolver="hlld",
-                 bc=(0, 0, 0, 0, 0, 0), nthreads=None, dt_fms_ref=True, lib_path=None):
+                 bc=(0, 0, 0, 0, 0, 0), nthreads=None, dt_fms_ref=True, lib_path=None,
+                 char_proj=False):
         L = self._L = lib(lib_path)
         lmin = np.asarray(lmin, dtype=np.float64)
         lmax = np.asarray(lmax, dtype=np.float64)
@@ -191,6 +212,7 @@ class OracleSim:
         self._h = L.or_create(nx, ny, nz, reconstr_order, _ptr(lmin), _ptr(lmax), gam, cour,
                               RS_IDS[rsolver], bcs, nthreads)
         L.or_set_dt_fms_ref(self._h, int(bool(dt_fms_ref)))
+        L.or_set_char_proj(self._h, int(bool(char_proj)))
         self.ntot = tuple(L.or_ntot(self._h, a) for a in range(3))
         self.ng = L.or_ng(self._h)
         nc = self.ntot[0] * self.ntot[1] * self.ntot[2]

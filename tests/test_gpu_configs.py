@@ -8,7 +8,9 @@
   the batched 2D blast sweep (64 periodic boxes of side 64-512), every box in
   FP64 and FP32C: each reaches its t_max, stays physical, FP32C stays within
   1e-5 relative L1 of FP64; boxes sampled from each sweep equal the oracle;
-  the blast boxes conserve mass and energy and keep div B at round-off.
+  the blast boxes conserve mass and energy and keep div B at round-off
+  (measured against max|B|/dx, since their cells go down to 1/512 and
+  their field up to |B| = 2).
 
 The FP64 single-grid path is tied to the oracle bit for bit by
 tests/test_gpu_parity.py; here the batch boxes are tied to it directly.
@@ -158,8 +160,8 @@ def test_tube_sweep_512_full_size():
 def test_blast_sweep_64_full_size():
     """The batched 2D sweep at full size: 64 periodic blast boxes of side
     64..512, FP64 and FP32C to t=0.2: per box, mass and energy conserved and
-    div B at round-off; FP32C within 1e-5 of FP64; the two smallest boxes
-    equal the oracle."""
+    div B at round-off (<= 1e-12 of max|B|/dx); FP32C within 1e-5 of FP64;
+    the two smallest boxes equal the oracle."""
     say("case: blast sweep, 64 periodic 2D boxes of side 64..512 (seed 5005), HLLD+PLM+CT, "
         "to t=0.2, FP64 and FP32C, graph")
     cases = [(c, (0,) * 6, 0.2) for c in P.blast_sweep(64, seed=5005)]
@@ -167,16 +169,22 @@ def test_blast_sweep_64_full_size():
     for prec in ("fp64", "fp32c"):
         init, out, t, steps = batch_sweep(cases, prec)
         check_physical(cases, out, t)
-        worst_div = worst_m = worst_e = 0.0
+        worst_div = worst_rel = worst_m = worst_e = 0.0
         for q, ((s, _, _), _, _) in enumerate(cases):
-            worst_div = max(worst_div, divb_max(s, out[q][1]))
+            d = divb_max(s, out[q][1])
+            sk, sj, si = s.active()
+            bmax = float(np.sqrt(np.max(np.sum(out[q][0][6:9, sk, sj, si] ** 2, axis=0))))
+            worst_div = max(worst_div, d)
+            worst_rel = max(worst_rel, d * s.dx(0) / bmax)
             m0, e0 = totals(s, init[q][0])
             m1, e1 = totals(s, out[q][0])
             worst_m = max(worst_m, abs(m1 - m0) / abs(m0))
             worst_e = max(worst_e, abs(e1 - e0) / abs(e0))
-        say(f"  {prec}: max div B {worst_div:.3e}, mass drift {worst_m:.3e}, "
-            f"energy drift {worst_e:.3e}")
-        assert worst_div <= 1e-12
+        say(f"  {prec}: max div B {worst_div:.3e} (code units), {worst_rel:.3e} of max|B|/dx; "
+            f"mass drift {worst_m:.3e}, energy drift {worst_e:.3e}")
+        # round-off: the boxes hold |B| up to 2 on cells down to 1/512, so
+        # div B is measured against the field's own scale max|B|/dx
+        assert worst_rel <= 1e-12
         assert worst_m <= 1e-12 and worst_e <= 1e-12
         res[prec] = (out, t, steps)
     l1 = sweep_rel_l1(cases, res["fp32c"][0], res["fp64"][0])

@@ -29,7 +29,7 @@ def rel_l1(a, b):
 
 
 def run_pair(prob, steps=None, tmax=None, rsolver="hlld", reconstr="linear", bc=(0,) * 6,
-             dt_fms_ref=True, cour=0.4, precision="fp64"):
+             dt_fms_ref=True, cour=0.4, precision="fp64", char_proj=False):
     from bonzo_amd._lib import Context
     from oracle.oracle import OracleSim
     s, prim, bfld = prob
@@ -38,12 +38,12 @@ def run_pair(prob, steps=None, tmax=None, rsolver="hlld", reconstr="linear", bc=
         prim, bfld = _regrid(s, prim, bfld, 2)
         s = P.GridShape(s.nx, s.ny, s.nz, 2, s.lmin, s.lmax)
     o = OracleSim(s.nx, s.ny, s.nz, reconstr_order=order, rsolver=rsolver, bc=bc, cour=cour,
-                  dt_fms_ref=dt_fms_ref)
+                  dt_fms_ref=dt_fms_ref, char_proj=char_proj)
     o.prim[:] = prim
     o.bfld[:] = bfld
     o.init()
     g = Context(s.nx, s.ny, s.nz, rsolver=rsolver, reconstr=reconstr, bc=bc, cour=cour,
-                dt_fms_ref=dt_fms_ref, precision=precision)
+                dt_fms_ref=dt_fms_ref, precision=precision, char_proj=char_proj)
     g.upload(prim, bfld)
     g.init_state()
     if steps is not None:
@@ -565,5 +565,36 @@ def test_fp64_fma_mode_within_1e12_of_oracle(case):
     if "tmax" in kw:
         assert res["n"] == res["o"].step and res["t"] == res["o"].time
     compare(res, f"fp64fma {case}")
+    res["o"].close()
+    res["g"].close()
+
+
+CHAR_CASES = {
+    "ot64_2d": dict(prob=lambda: P.orszag_tang(64), steps=25),
+    "turb24_3d": dict(prob=lambda: P.turbulence(24, seed_b=3, seed_v=4), steps=6),
+    "brio_wu_1d": dict(prob=lambda: P.brio_wu(400, axis=0, transverse=0), tmax=0.08,
+                       bc=(1,) * 6),
+    "brio_wu_y_3d": dict(prob=lambda: P.brio_wu(64, axis=1, transverse=4), steps=20,
+                         bc=(0, 0, 1, 1, 0, 0)),
+    "hll_blast_walls": dict(prob=lambda: P.blast2d(40, seed=23), steps=15, bc=(2,) * 6,
+                            rsolver="hll"),
+}
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp64fma"])
+@pytest.mark.parametrize("case", sorted(CHAR_CASES))
+def test_characteristic_limiting_matches_oracle(case, precision):
+    """charact_proj=1: PLM slopes limited in characteristic variables
+    (src/coordinates/reconstr_c.c:64-80 with the projections of
+    src/mhd/eos_c.h:100-280), on the 3D, 2D and fused 1D kernels: the FP64
+    path equals the oracle (1e-12 relative L1, expected bitwise), and the FMA
+    mode stays within 1e-12."""
+    kw = dict(CHAR_CASES[case])
+    prob = kw.pop("prob")()
+    say(f"case: charact_proj=1, {case}, {precision}")
+    res = run_pair(prob, precision=precision, char_proj=True, **kw)
+    bitwise = compare(res, f"char {case} {precision}")
+    if precision == "fp64":
+        assert bitwise
     res["o"].close()
     res["g"].close()

@@ -41,9 +41,10 @@ def test_params_struct_layout_matches_header(tmp_path):
     src = tmp_path / "layout.c"
     src.write_text(
         '#include <stdio.h>\n#include <stddef.h>\n#include "bnzgpu.h"\n'
-        'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n",'
+        'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n",'
         ' sizeof(bnz_params),'
         ' offsetof(bnz_params, lmin), offsetof(bnz_params, gam), offsetof(bnz_params, device),'
+        ' offsetof(bnz_params, char_proj),'
         ' sizeof(bnz_diag_out), offsetof(bnz_diag_out, divb_max),'
         ' offsetof(bnz_diag_out, nonfinite), offsetof(bnz_diag_out, bad_ijk),'
         ' offsetof(bnz_diag_out, step), offsetof(bnz_diag_out, time),'
@@ -55,7 +56,8 @@ def test_params_struct_layout_matches_header(tmp_path):
     vals = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True,
                                            check=True).stdout.split()]
     mine = [ctypes.sizeof(_lib.BnzParams), _lib.BnzParams.lmin.offset, _lib.BnzParams.gam.offset,
-            _lib.BnzParams.device.offset, ctypes.sizeof(_lib.BnzDiag),
+            _lib.BnzParams.device.offset, _lib.BnzParams.char_proj.offset,
+            ctypes.sizeof(_lib.BnzDiag),
             _lib.BnzDiag.divb_max.offset, _lib.BnzDiag.nonfinite.offset,
             _lib.BnzDiag.bad_ijk.offset, _lib.BnzDiag.step.offset, _lib.BnzDiag.time.offset,
             ctypes.sizeof(_lib.BnzBox), _lib.BnzBox.lmin.offset, _lib.BnzBox.tmax.offset]

@@ -723,3 +723,149 @@ def test_history_rms_of_a_known_field():
     h = o.history()
     assert abs(h[12]) < 1e-15 and abs(h[15] - 0.3) < 1e-15
     assert abs(h[5] - 0.5 * 0.09) < 1e-15
+
+
+# ------------------------------------------------- characteristic projection
+def _random_state(rng):
+    w = np.zeros(9)
+    w[0] = rng.uniform(0.1, 3)
+    w[1:4] = rng.uniform(-1, 1, 3)
+    w[4] = rng.uniform(0.05, 3)
+    w[5] = rng.uniform(0, 1)
+    w[6:9] = rng.uniform(-2, 2, 3)
+    return w
+
+
+def test_char_projection_round_trip():
+    """char2prim inverts prim2char (src/mhd/eos_c.h:100-280) for states with a
+    non-zero normal and transverse field; the passive scalar and normal-field
+    slots pass through both unchanged."""
+    rng = np.random.default_rng(21)
+    for _ in range(500):
+        w = _random_state(rng)
+        x = rng.standard_normal(9)
+        c = orc.prim2char(x, w, GAM)
+        assert c[5] == x[5] and c[6] == x[6]
+        y = orc.char2prim(c, w, GAM)
+        assert np.max(np.abs(y - x)) <= 1e-12 * np.max(np.abs(x)), (w, x, y)
+
+
+@pytest.mark.parametrize("wave,slot", [("fast", 0), ("alfven", 1), ("slow", 2)])
+def test_char_projection_isolates_each_eigenmode(wave, slot):
+    """A linear eigenmode of the state rho=1, p=0.6, B=(1, sqrt 2, 1/2)
+    (problems.WAVES, the waves whose convergence the linear-wave tests pin)
+    projects onto one characteristic only, with amplitude 1 (the waves
+    travel towards -x: the fast-, Alfven- and slow- slots 0, 1, 2); the
+    entropy mode (a density jump alone) lands in slot 3.  A sign or weight
+    slip in any eigenvector row leaks the mode into other slots."""
+    w = np.array([1.0, 0, 0, 0, 0.6, 0, 1.0, math.sqrt(2.0), 0.5])
+    _, ev = P.WAVES[wave]
+    d = np.zeros(9)
+    d[0], d[1], d[2], d[3], d[4], d[7], d[8] = (ev["rho"], ev["vn"], ev["v1"], ev["v2"], ev["p"],
+                                                ev["b1"], ev["b2"])
+    c = orc.prim2char(d, w, GAM)
+    want = np.zeros(9)
+    want[slot] = 1.0
+    assert np.max(np.abs(c - want)) < 1e-14, c
+    e = np.zeros(9)
+    e[0] = 1.0
+    want = np.zeros(9)
+    want[3] = 1.0
+    assert np.max(np.abs(orc.prim2char(e, w, GAM) - want)) < 1e-15
+
+
+def test_char_plm_exact_on_linear_data():
+    """PLM in characteristic variables reconstructs linear data exactly:
+    equal left and right differences project to equal amplitudes, which the
+    van Leer limiter keeps, and char2prim maps the half slope back."""
+    from oracle.oracle import _rows, _dp, _dpp, lib
+    import ctypes
+    x = np.arange(12, dtype=float)
+    w = np.zeros((9, 12))
+    base = [1.0, 0.2, -0.1, 0.05, 0.8, 0.3, 0.9, 0.6, -0.4]
+    slope = [0.01, 0.002, -0.003, 0.001, 0.02, 0.01, 0.0, 0.005, -0.004]
+    for n in range(9):
+        w[n] = base[n] + slope[n] * x
+    n_ = w.shape[1]
+    wl = np.zeros((9, n_ + 4))
+    wr = np.zeros((9, n_ + 4))
+    scr = [np.zeros((9, n_ + 4)) for _ in range(3)]
+    keep, st = [], []
+    for d in range(3):
+        rows = (_dp * 9)()
+        for v in range(9):
+            rows[v] = ctypes.cast(w[v].ctypes.data + 8 * (d - 1), _dp)
+        keep.append(rows)
+        st.append(ctypes.cast(rows, _dpp))
+    stc = (_dpp * 3)(*st)
+    scr_rows = [_rows(a) for a in scr]
+    sc = (_dpp * 3)(*[ctypes.cast(r, _dpp) for r in scr_rows])
+    L = lib()
+    L.or_reconstr_linear_cp.argtypes = [_dpp, _dpp, ctypes.POINTER(_dpp), ctypes.POINTER(_dpp),
+                                        ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_double]
+    L.or_reconstr_linear_cp(_rows(wl), _rows(wr), stc, sc, 0, 1, n_ - 2, 1, GAM)
+    for i in range(1, n_ - 1):
+        for v in range(9):
+            assert abs(wr[v, i] - (w[v, i] - 0.5 * slope[v])) < 1e-14
+            assert abs(wl[v, i + 1] - (w[v, i] + 0.5 * slope[v])) < 1e-14
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("wave", ["fast", "alfven", "slow"])
+def test_char_plm_linear_waves_second_order(wave):
+    """With charact_proj=1 the step still converges at second order on the
+    linear waves, in 1D and along the 2D diagonal."""
+    for dims, sizes in ((1, (32, 64, 128)), (2, (32, 64))):
+        s0, _, _, period = P.linear_wave_oblique(8, wave, dims)
+        t = period if dims == 1 else 0.25 * P.linear_wave_oblique(8, "fast", 2)[3]
+        errs = []
+        for n in sizes:
+            s, prim, bfld, _ = P.linear_wave_oblique(n, wave, dims, amp=1e-6)
+            _, pe, be, _ = P.linear_wave_oblique(n, wave, dims, amp=1e-6, t=t)
+            o = orc.OracleSim(s.nx, s.ny, s.nz, cour=0.4, char_proj=True)
+            o.prim[:] = prim
+            o.bfld[:] = bfld
+            o.init()
+            o.advance(t)
+            sk, sj, si = s.active()
+            e = sum(np.mean(np.abs(o.prim[v][sk, sj, si] - pe[v][sk, sj, si])) ** 2
+                    for v in range(5))
+            e += sum(np.mean(np.abs(o.bfld[c][sk, sj, si] - be[c][sk, sj, si])) ** 2
+                     for c in range(3))
+            errs.append(math.sqrt(e))
+            o.close()
+        for q in range(len(errs) - 1):
+            assert errs[q] / errs[q + 1] > 3.2, (dims, errs)
+
+
+def test_char_plm_changes_a_shock_tube_and_keeps_invariants():
+    """On a Brio-Wu tube the characteristic limiting gives a different,
+    still conservative and non-oscillatory-in-density state than primitive
+    limiting (so the flag is wired), and div B / conservation hold in 3D."""
+    prob = P.brio_wu(64, axis=0, transverse=0)
+    s, prim, bfld = prob
+    outs = []
+    for cp in (False, True):
+        o = orc.OracleSim(64, 1, 1, bc=(1,) * 6, char_proj=cp)
+        o.prim[:] = prim
+        o.bfld[:] = bfld
+        o.init()
+        o.advance_steps(30)
+        sk, sj, si = s.active()
+        outs.append(o.prim[0][sk, sj, si].copy())
+        o.close()
+    assert np.max(np.abs(outs[0] - outs[1])) > 1e-4
+    assert np.all(outs[1] > 0.1) and np.all(outs[1] < 1.01)
+    t = P.turbulence(16, seed_b=11, seed_v=12)
+    o0 = orc.OracleSim(16, 16, 16, char_proj=True)
+    o0.prim[:] = t[1]
+    o0.bfld[:] = t[2]
+    o0.init()
+    sk, sj, si = t[0].active()
+    tot0 = [o0.cons[v][sk, sj, si].sum() for v in range(6)]
+    o0.advance_steps(8)
+    assert divb_max(t[0], o0.bfld) < 1e-12
+    tot1 = [o0.cons[v][sk, sj, si].sum() for v in range(6)]
+    for a, b in zip(tot0, tot1):
+        assert abs(a - b) < 1e-10 * max(1.0, abs(a))
