@@ -271,12 +271,12 @@ class Simulation:
     FACE_NAMES = ("bxf", "byf", "bzf")
 
     def save_restart(self, path):
-        """Restart file.  The reference writes the grid datasets and
-        attributes to rst/rst_grid.h5 (src/io/restart.pyx:23-86, dataset
-        names of src/io/output.pyx:248-385); HDF5 is not available here, so
-        an .npz holds the same names (rho ... bzc, bxf, byf, bzf, time, step,
-        dt, xf, yf, zf) and, under 'state', the exact device state
-        (bnz_save_state) that load_restart resumes from bit for bit."""
+        """Exact restart file: an .npz with the reference's dataset names
+        (rho ... bzc, bxf, byf, bzf, time, step, dt, xf, yf, zf; src/io/
+        output.pyx:248-385) and, under 'state', the exact device state
+        (bnz_save_state, FP32C low words included) that load_restart resumes
+        from bit for bit.  write_restart_h5 writes the reference's own HDF5
+        restart file."""
         prim, bfld = self.ctx.download()
         t, step, dt = self.ctx.time()
         data = {n: prim[q] for q, n in enumerate(self.GRID_NAMES)}
@@ -298,6 +298,60 @@ class Simulation:
         precision)."""
         with np.load(path, allow_pickle=False) as f:
             self.ctx.load_state(f["state"])
+        self.integr.time, self.integr.step, self.integr.dt = self.ctx.time()
+
+    def write_grid(self, grid_dt, var_type="prim", write_ghost=False, path=None):
+        """Grid output as write_grid writes it (src/io/output.pyx:248-385):
+        ``usr_dir/out/grid_NNN.h5`` with NNN = int(time / grid_dt), the cell
+        variables (primitive, or conserved with var_type="cons") and the face
+        fields over the active cells (or the whole grid with write_ghost),
+        and the attributes time, step, dt, xf, yf, zf.  Returns the path."""
+        from . import h5io
+        t, step, dt = self.ctx.time()
+        if path is None:
+            if self.usr_dir is None:
+                raise ValueError("no usr_dir: pass the output path")
+            path = os.path.join(self.usr_dir, "out", "grid_{:03d}.h5".format(int(t / grid_dt)))
+        prim, bfld, cons = self.ctx.download(cons=True)
+        ds = h5io.grid_datasets(self.grid.shape, prim, bfld,
+                                cons=cons if var_type == "cons" else None,
+                                write_ghost=write_ghost)
+        xf, yf, zf = h5io.face_coords(self.grid.shape)
+        h5io.write_h5(path, ds, dict(time=t, step=int(step), dt=dt, xf=xf, yf=yf, zf=zf),
+                      chunked=not write_ghost)
+        return path
+
+    def write_restart_h5(self, path=None):
+        """Restart file as the reference writes it (write_grid with rst=1,
+        src/io/output.pyx:270-281): ``usr_dir/rst/rst_grid.h5`` with the
+        primitive variables and face fields of the active cells."""
+        if path is None:
+            if self.usr_dir is None:
+                raise ValueError("no usr_dir: pass the restart path")
+            path = os.path.join(self.usr_dir, "rst", "rst_grid.h5")
+        return self.write_grid(1.0, "prim", False, path)
+
+    def read_restart_h5(self, path=None):
+        """set_restart_grid (src/io/restart.pyx:23-86): the active cells of
+        prim and bfld and the time and step from an HDF5 restart file, then
+        init_sim's ghost fill and cell-centred field (vanleer.pyx:38-48)."""
+        from . import h5io
+        if path is None:
+            path = os.path.join(self.usr_dir, "rst", "rst_grid.h5")
+        ds, attrs = h5io.read_h5(path)
+        shape = self.grid.shape
+        prim, bfld = shape.empty()
+        act = shape.active()
+        for n, name in enumerate(h5io.PRIM_NAMES):
+            if ds[name].shape != prim[n][act].shape:
+                raise ValueError(f"{path}: {name} has shape {ds[name].shape}, the grid "
+                                 f"{prim[n][act].shape}")
+            prim[n][act] = ds[name]
+        for n, name in enumerate(h5io.FACE_NAMES):
+            bfld[n][act] = ds[name]
+        self.ctx.upload(prim, bfld)
+        self.ctx.init_state()
+        self.ctx.set_time(attrs["time"], int(attrs["step"]))
         self.integr.time, self.integr.step, self.integr.dt = self.ctx.time()
 
     def close(self):

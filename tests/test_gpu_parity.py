@@ -6,6 +6,7 @@ prim and every active face of bfld is compared.  Tolerance: relative L1
 order without contraction, so the differences are expected to be zero and
 the tests report whether they are.
 """
+import os
 import sys
 
 import numpy as np
@@ -447,6 +448,43 @@ def test_simulation_restart_file(tmp_path):
         assert f["rho"].shape == a.prim[0].shape
         assert abs(float(f["time"]) - 0.02) < 1e-15 and int(f["step"]) > 0
         assert set(Simulation.GRID_NAMES + Simulation.FACE_NAMES) <= set(f.files)
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_simulation_hdf5_restart_resumes_bitwise(tmp_path, dims):
+    """The reference's own restart file (rst/rst_grid.h5: primitive
+    variables and face fields of the active cells, time and step;
+    src/io/restart.pyx:23-86): a run restarted from it continues bit for
+    bit like the uninterrupted run, and the grid output file holds the
+    state of the run (src/io/output.pyx:248-385)."""
+    from bonzo_amd import h5io
+    from bonzo_amd.simulation import Simulation
+    say(f"case: Simulation HDF5 restart, {dims}D")
+    n = 32 if dims == 2 else 16
+    cfg = {"computation": {"Nx": n, "Ny": n, "Nz": n if dims == 3 else 1, "tmax": 0.1,
+                           "cour": 0.4, "rsolver": "hlld", "reconstr": "linear"},
+           "physics": {"gam": 5.0 / 3.0},
+           "problem": {"name": "otvortex" if dims == 2 else "turbulence"}}
+    a = Simulation(usr_dir=str(tmp_path), config=cfg)
+    a.advance(0.02)
+    rst = a.write_restart_h5()
+    assert rst == os.path.join(str(tmp_path), "rst", "rst_grid.h5")
+    out = a.write_grid(0.01)
+    assert os.path.basename(out) == "grid_002.h5"
+    ds, attrs = h5io.read_h5(out)
+    act = a.grid.shape.active()
+    assert np.array_equal(ds["p"], a.prim[4][act]) and np.array_equal(ds["bzf"], a.bfld[2][act])
+    assert attrs["time"] == a.integr.time and attrs["step"] == a.integr.step
+    a.advance(0.05)
+    b = Simulation(usr_dir=str(tmp_path), config=cfg)
+    b.read_restart_h5()
+    assert b.integr.time == attrs["time"] and b.integr.step == attrs["step"]
+    b.advance(0.05)
+    assert a.integr.step == b.integr.step
+    assert np.array_equal(a.prim[:, act[0], act[1], act[2]], b.prim[:, act[0], act[1], act[2]])
+    assert np.array_equal(a.bfld[:, act[0], act[1], act[2]], b.bfld[:, act[0], act[1], act[2]])
     a.close()
     b.close()
 
