@@ -23,7 +23,7 @@ ABI_SYMBOLS = (
     "bnz_advance", "bnz_run_steps", "bnz_get_time", "bnz_set_time", "bnz_diag", "bnz_history",
     "bnz_synchronize", "bnz_set_tuning", "bnz_set_kernel_timing", "bnz_kernel_timing",
     "bnz_state_size", "bnz_save_state", "bnz_load_state",
-    "bnz_gpu_flux_pencils",
+    "bnz_gpu_flux_pencils", "bnz_hll_flux", "bnz_hlld_flux", "bnz_hllt_flux",
     "bnz_batch_create", "bnz_batch_destroy", "bnz_batch_last_error", "bnz_batch_box_shape",
     "bnz_batch_upload", "bnz_batch_download", "bnz_batch_init_state", "bnz_batch_advance",
     "bnz_batch_get_times", "bnz_batch_synchronize", "bnz_batch_set_tuning",
@@ -108,6 +108,10 @@ def load():
                                     ctypes.POINTER(ctypes.c_long)]
     L.bnz_gpu_flux_pencils.argtypes = [ctypes.c_int, vp, vp, vp, vp, ctypes.c_long,
                                        ctypes.c_double]
+    for name in ("bnz_hll_flux", "bnz_hlld_flux", "bnz_hllt_flux"):
+        getattr(L, name).argtypes = [ctypes.POINTER(_dp)] * 3 + [_dp, ctypes.c_int, ctypes.c_int,
+                                                                  ctypes.c_double]
+        getattr(L, name).restype = None
     L.bnz_batch_create.argtypes = [ctypes.POINTER(BnzParams), ctypes.POINTER(BnzBox),
                                    ctypes.c_int, ctypes.POINTER(vp)]
     L.bnz_batch_destroy.argtypes = [vp]
@@ -122,7 +126,8 @@ def load():
     L.bnz_batch_get_times.argtypes = [vp, _dp, ctypes.POINTER(ctypes.c_int)]
     L.bnz_batch_set_tuning.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     for name in ABI_SYMBOLS:
-        if name not in ("bnz_last_error", "bnz_batch_last_error"):
+        if name not in ("bnz_last_error", "bnz_batch_last_error", "bnz_hll_flux",
+                        "bnz_hlld_flux", "bnz_hllt_flux"):
             getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L

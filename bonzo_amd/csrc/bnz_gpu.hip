@@ -1092,4 +1092,79 @@ int bnz_gpu_flux_pencils(int rsolver, const double *wl, const double *wr, const 
   return BNZ_OK;
 }
 
+
+// The reference's Riemann-solver seam, RSolverFunc: host row pointers flux[n],
+// wl[n], wr[n] (n < 9) and bx, faces i1..i2.  Each call packs the rows into
+// one host block, runs k_pencil on the device and unpacks the flux rows.
+// The flux rows travel to the device too, so a face whose solver matches no
+// region keeps its old row, as in the reference.  Staging memory is kept per
+// host thread (the reference calls the solver from OpenMP threads).
+static int pencil_rows(int rs, double **flux, double **wl, double **wr, double *bx, int i1,
+                       int i2, double gam) {
+  if (!flux || !wl || !wr || !bx || i2 < i1 || !(gam > 1.)) return BNZ_EINVAL;
+  for (int q = 0; q < NMODE; ++q)
+    if (!flux[q] || !wl[q] || !wr[q]) return BNZ_EINVAL;
+  const long n = (long)i2 - i1 + 1;
+  thread_local std::vector<double> host;
+  thread_local double *dev = nullptr;
+  thread_local size_t cap = 0;
+  const size_t need = (size_t)28 * n;  // wl 9n | wr 9n | bx n | flux 9n
+  if (need > cap) {
+    if (dev) (void)hipFree(dev);
+    dev = nullptr;
+    cap = 0;
+    if (hipMalloc(&dev, need * sizeof(double)) != hipSuccess) return BNZ_ENOMEM;
+    cap = need;
+  }
+  host.resize(need);
+  double *h = host.data();
+  for (int q = 0; q < NMODE; ++q)
+    for (long i = 0; i < n; ++i) {
+      h[q * n + i] = wl[q][i1 + i];
+      h[(9 + q) * n + i] = wr[q][i1 + i];
+      h[(19 + q) * n + i] = flux[q][i1 + i];
+    }
+  for (long i = 0; i < n; ++i) h[18 * n + i] = bx[i1 + i];
+  if (hipMemcpy(dev, h, need * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+    return BNZ_EHIP;
+  const double gm1i = 1. / (gam - 1.);
+  const double *dwl = dev, *dwr = dev + 9 * n, *dbx = dev + 18 * n;
+  double *dfl = dev + 19 * n;
+  if (rs == BNZ_RS_HLLD)
+    hipLaunchKernelGGL(k_pencil<BNZ_RS_HLLD>, dim3(blocks_for(n)), dim3(BLK), 0, 0, dwl, dwr, dbx,
+                       dfl, n, gam, gm1i);
+  else if (rs == BNZ_RS_HLLT)
+    hipLaunchKernelGGL(k_pencil<BNZ_RS_HLLT>, dim3(blocks_for(n)), dim3(BLK), 0, 0, dwl, dwr, dbx,
+                       dfl, n, gam, gm1i);
+  else
+    hipLaunchKernelGGL(k_pencil<BNZ_RS_HLL>, dim3(blocks_for(n)), dim3(BLK), 0, 0, dwl, dwr, dbx,
+                       dfl, n, gam, gm1i);
+  if (hipGetLastError() != hipSuccess) return BNZ_EHIP;
+  if (hipMemcpy(h + 19 * n, dfl, 9 * n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+    return BNZ_EHIP;
+  for (int q = 0; q < NMODE; ++q)
+    for (long i = 0; i < n; ++i) flux[q][i1 + i] = h[(19 + q) * n + i];
+  return BNZ_OK;
+}
+
+static void pencil_seam(const char *name, int rs, double **flux, double **wl, double **wr,
+                        double *bx, int i1, int i2, double gam) {
+  const int rc = pencil_rows(rs, flux, wl, wr, bx, i1, i2, gam);
+  if (rc != BNZ_OK)  // the seam returns nothing: report, leave the flux rows as they were
+    std::fprintf(stderr, "%s: faces %d..%d not computed (error %d)\n", name, i1, i2, rc);
+}
+
+void bnz_hll_flux(double **flux, double **wl, double **wr, double *bx, int i1, int i2,
+                  double gam) {
+  pencil_seam("bnz_hll_flux", BNZ_RS_HLL, flux, wl, wr, bx, i1, i2, gam);
+}
+void bnz_hlld_flux(double **flux, double **wl, double **wr, double *bx, int i1, int i2,
+                   double gam) {
+  pencil_seam("bnz_hlld_flux", BNZ_RS_HLLD, flux, wl, wr, bx, i1, i2, gam);
+}
+void bnz_hllt_flux(double **flux, double **wl, double **wr, double *bx, int i1, int i2,
+                   double gam) {
+  pencil_seam("bnz_hllt_flux", BNZ_RS_HLLT, flux, wl, wr, bx, i1, i2, gam);
+}
+
 }  // extern "C"

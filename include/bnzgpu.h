@@ -168,6 +168,25 @@ int bnz_kernel_timing(bnz_ctx *ctx, int idx, char *name, int name_len, double *t
 int bnz_gpu_flux_pencils(int rsolver, const double *wl, const double *wr, const double *bx,
                          double *flux, long n, double gam);
 
+/* The reference's Riemann-solver seam itself: the signature of RSolverFunc
+   (src/integration/integrator.pxd:76-79) and of hll_flux / hlld_flux /
+   hllt_flux (src/mhd/fluxes_c.h:3-21), so that a maintainer can assign them
+   to integr.rsolver_func in the reference's CPU driver.  Host row pointers:
+   flux[n], wl[n], wr[n] for the 9 slots of the sweep's local frame and bx,
+   indexed by face, faces i1..i2 inclusive.  Each call stages the rows
+   through device memory and runs the solver on the GPU (synchronous; the
+   PCIe round trip per pencil makes this a compatibility seam, not the fast
+   path, which is bnz_vl_step).  A face whose solver matches no region keeps
+   its flux row, as in the reference; the seam has no status, so a failure
+   prints to stderr and leaves every flux row as it was.  Thread-safe: the
+   staging memory is per host thread. */
+void bnz_hll_flux(double **flux, double **wl, double **wr, double *bx, int i1, int i2,
+                  double gam);
+void bnz_hlld_flux(double **flux, double **wl, double **wr, double *bx, int i1, int i2,
+                   double gam);
+void bnz_hllt_flux(double **flux, double **wl, double **wr, double *bx, int i1, int i2,
+                   double gam);
+
 /* ------------------------------------------------------------------------
  * Batches of independent boxes of uneven size (parameter sweeps).  Each box
  * is the grid that one bnz_create with the same bnz_params would build (its

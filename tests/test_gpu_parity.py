@@ -119,6 +119,41 @@ def test_pencil_solvers_match_oracle():
         assert np.array_equal(got, ref), f"{rs}: max diff {np.max(np.abs(got - ref))}"
 
 
+def test_rsolver_seam_matches_oracle_on_host_rows():
+    """bnz_hll_flux / bnz_hlld_flux / bnz_hllt_flux have RSolverFunc's
+    signature (host row pointers, faces i1..i2, integrator.pxd:76-79): on
+    the same rows they write what the oracle's pencil solvers write, bit for
+    bit, only at faces i1..i2, and a face with NaN states keeps its old
+    flux row (the reference leaves an unmatched face untouched)."""
+    from bonzo_amd._lib import load
+    from oracle import oracle as orc
+    from oracle.oracle import _ptr, _rows
+    from tests import mathref as M
+    say("case: RSolverFunc seam hll/hlld/hllt on host rows, 2048 faces, i1=5, i2=2040")
+    rng = np.random.default_rng(22)
+    n, i1, i2 = 2048, 5, 2040
+    wl, wr, bx = M.random_states(rng, n, bx_min=0.0)
+    wl[1] += rng.uniform(-3, 3, n)
+    wr[1] += rng.uniform(-3, 3, n)
+    wl[:, 100] = np.nan  # no region matches: the row stays as it was
+    L = load()
+    for rs in ("hll", "hlld", "hllt"):
+        sentinel = 7.25
+        want = np.full((9, n), sentinel)
+        fn_o = {"hll": orc.lib().or_hll_flux, "hlld": orc.lib().or_hlld_flux,
+                "hllt": orc.lib().or_hllt_flux}[rs]
+        fn_o(_rows(want), _rows(wl), _rows(wr), _ptr(bx), i1, i2, 5.0 / 3.0)
+        got = np.full((9, n), sentinel)
+        getattr(L, f"bnz_{rs}_flux")(_rows(got), _rows(wl), _rows(wr), _ptr(bx), i1, i2,
+                                     5.0 / 3.0)
+        want[6, i1:i2 + 1] = got[6, i1:i2 + 1]  # normal-field slot: no consumer reads it
+        assert np.all(got[:, :i1] == sentinel) and np.all(got[:, i2 + 1:] == sentinel)
+        if rs != "hll":  # hll writes every face (NaN here), as the reference's does
+            assert np.all(got[:, 100] == sentinel), rs
+        assert np.array_equal(got, want, equal_nan=True), \
+            f"{rs}: max diff {np.nanmax(np.abs(got - want))}"
+
+
 def test_orszag_tang_2d_bitwise():
     say("case: Orszag-Tang 64^2, 20 steps, hlld+plm, periodic")
     res = run_pair(P.orszag_tang(64), steps=20)
