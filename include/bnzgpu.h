@@ -105,7 +105,10 @@ int bnz_download(bnz_ctx *ctx, double *prim, double *bfld, double *cons);
 /* init_sim after set_problem: ghost fill, interp_bc, ghost fill of cell B,
    prim2cons (src/integration/vanleer.pyx:38-48) */
 int bnz_init_state(bnz_ctx *ctx);
-/* GridBC.apply_grid_bc over all NVAR (src/bc/grid_bc.pyx:323-339) */
+/* GridBC.apply_grid_bc over all NVAR (src/bc/grid_bc.pyx:323-339).  An
+   FP32C context fills the hi words of the conserved state, the FP32 prim and
+   the faces; the lo words of ghost cells are never read by the step and stay
+   undefined (also in bnz_save_state blobs). */
 int bnz_fill_ghosts(bnz_ctx *ctx);
 /* new_dt (src/integration/new_dt.pyx:18-135): cour / max (|v|+c_f)/dx */
 int bnz_new_dt(bnz_ctx *ctx, double *dt);

@@ -106,8 +106,9 @@ def test_fp32c_init_and_download_round_trip():
 
 
 def test_reflecting_walls_fp32c():
-    """FP32C ghost fill with reflecting walls (sign flips on hi, lo and FP32
-    prim, zeroed wall faces) against the FP64 path on a smooth 2D state."""
+    """FP32C ghost fill with reflecting walls (sign flips on the hi words and
+    the FP32 prim, zeroed wall faces; the lo words are not filled, since only
+    active cells read them) against the FP64 path on a smooth 2D state."""
     from tests.test_oracle_pins import _wall_state
     say("case: smooth 2D state 64x48, reflecting x walls, 40 steps, fp32c vs fp64")
     s, out = pair(_wall_state(64, 48, 0), 40, bc=(2, 2, 0, 0, 0, 0))
