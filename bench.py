@@ -38,6 +38,7 @@ HBM_PEAK_GBS = 8000.0
 #             + 3 faces (8 bytes), write 6 hi + 6 lo + 9 prim (4) + 3 faces (8)
 KERNEL_BYTES = {
     "fp64": {"flux": 18 * 8, "emf": 15 * 8, "update_pred": 45 * 8, "update_corr": 45 * 8},
+    "fp64fma": {"flux": 18 * 8, "emf": 15 * 8, "update_pred": 45 * 8, "update_corr": 45 * 8},
     "fp32c": {"flux": 18 * 4, "emf": 15 * 4, "update_pred": 48 * 4 + 3 * 8,
               "update_corr": 54 * 4 + 6 * 8},
 }
@@ -55,7 +56,7 @@ PROFILED_VALU_BUSY = {("fp64", "flux3_plm"): 0.72, ("fp32c", "update_corr"): 0.3
 # Algorithmic bytes per cell-update of a whole VL step: 5 state volumes of 9
 # words (FP64 360 B; FP32C 252 B with hi + lo on the stage-2 read of S_n and
 # the write of S_{n+1}).
-STEP_BYTES = {"fp64": 360.0, "fp32c": 252.0}
+STEP_BYTES = {"fp64": 360.0, "fp64fma": 360.0, "fp32c": 252.0}
 
 
 def kernel_units(name, n, ng=3):
@@ -74,7 +75,7 @@ def kernel_units(name, n, ng=3):
 
 
 def kernel_bytes(prec, name, n):
-    word = 8 if prec == "fp64" else 4
+    word = 4 if prec == "fp32c" else 8
     if name.startswith("flux3"):
         faces = 3 * kernel_units(name.replace("flux3", "flux"), n)
         return kernel_units(name, n) * 12 * word + faces * 8 * word
@@ -319,6 +320,8 @@ def main():
     ap.add_argument("--shapes", default="", metavar="F,E,U",
                     help="fixed work shapes of the flux, EMF and update kernels (no autotuning)")
     ap.add_argument("--no-fp32c", action="store_true", help="time the FP64 path only")
+    ap.add_argument("--fp64-ieee-only", action="store_true",
+                    help="time only the IEEE FP64 path (no FMA mode); it is then the value")
     ap.add_argument("--accuracy-steps", type=int, default=100,
                     help="steps after which FP32C is compared with FP64 (0: skip)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-box workloads")
@@ -350,9 +353,30 @@ def main():
         return
     n = args.n
     s, prim, bfld = problems.turbulence(n)
-    r64 = run_path("fp64", n, prim, bfld, args)
+    # FP64, bit for bit the oracle (IEEE, no contraction)
+    rie = run_path("fp64", n, prim, bfld, args)
+    # FP64 with multiply-adds fused (within 1e-12 of the oracle): the FP64 value
+    rfm = None if args.fp64_ieee_only else run_path("fp64fma", n, prim, bfld, args)
     r32 = None if args.no_fp32c else run_path("fp32c", n, prim, bfld, args)
     del prim, bfld
+    r64 = rfm if rfm is not None else rie
+    prec64 = "fp64fma" if rfm is not None else "fp64"
+
+    def rel_l1(a, b):
+        sk, sj, si = s.active()
+        names = ["rho", "vx", "vy", "vz", "p", "psc", "bxc", "byc", "bzc"]
+        l1 = {}
+        for v in range(9):
+            x = a[0][v][sk, sj, si]
+            y = b[0][v][sk, sj, si]
+            den = float(np.sum(np.abs(y)))
+            if den > 0:
+                l1[names[v]] = float(np.sum(np.abs(x - y)) / den)
+        for c, nm in enumerate(("bxf", "byf", "bzf")):
+            x = a[1][c][sk, sj, si]
+            y = b[1][c][sk, sj, si]
+            l1[nm] = float(np.sum(np.abs(x - y)) / np.sum(np.abs(y)))
+        return l1
 
     out = {
         "metric": METRIC,
@@ -370,7 +394,10 @@ def main():
         "config": {"workload": f"3D decaying MHD turbulence {n}^3 periodic, VL + PLM + HLLD + CT, "
                                "FP64 plain path, CFL 0.4, seeded k^-2 fields "
                                "(rho=1, p=0.6, Bx=0.2, dB_rms=0.05, Mach 0.5)",
-                   "n": n, "precision": "fp64", "rsolver": "hlld", "reconstr": "linear",
+                   "n": n, "precision": prec64, "rsolver": "hlld", "reconstr": "linear",
+                   "arithmetic": ("FP64, a*b+c fused into FMA (within 1e-12 relative L1 of the "
+                                  "IEEE path and the oracle)" if rfm is not None else
+                                  "FP64, IEEE without contraction (bit for bit the oracle)"),
                    "shapes": dict(zip(("flux", "emf", "update"), r64["tuned"]))},
         "roofline": r64["roofline"],
         "step_roofline": {"bytes_per_cell_update": STEP_BYTES["fp64"],
@@ -381,9 +408,22 @@ def main():
         "autotune_ms": {str(c): {k: round(v, 3) for k, v in t.items()}
                         for c, t in r64["table"].items()},
     }
+    if rfm is not None:
+        ieee = {"value": rie["value"], "ms_per_step": rie["ms_per_step"],
+                "dtype": "f64 (IEEE, no contraction: bit for bit the oracle)",
+                "roofline": rie["roofline"], "kernels_ms_per_launch": rie["kernels"],
+                "shapes": dict(zip(("flux", "emf", "update"), rie["tuned"])),
+                "divb_max": rie["divb_max"]}
+        if rie["state"] is not None and rfm["state"] is not None:
+            l1 = rel_l1(rfm["state"], rie["state"])
+            out["fma_rel_l1_vs_ieee"] = l1
+            out["fma_rel_l1_max"] = max(l1.values())
+            out["accuracy_steps"] = rie["steps_total"]
+        out["fp64_ieee"] = ieee
     if r32 is not None:
         fp32c = {"value": r32["value"], "ms_per_step": r32["ms_per_step"],
                  "speedup_vs_fp64": r32["value"] / r64["value"],
+                 "speedup_vs_fp64_ieee": r32["value"] / rie["value"],
                  "dtype": "f32 (hi+lo state; FP64 update, pressure recovery, CT increments, dt)",
                  "roofline": r32["roofline"],
                  "step_roofline": {"bytes_per_cell_update": STEP_BYTES["fp32c"],
@@ -392,23 +432,11 @@ def main():
                  "shapes": dict(zip(("flux", "emf", "update"), r32["tuned"])),
                  "kernels_ms_per_launch": r32["kernels"], "divb_max": r32["divb_max"],
                  "hlld_guards": r32["hlld_guards"]}
-        if r64["state"] is not None and r32["state"] is not None:
-            sk, sj, si = s.active()
-            names = ["rho", "vx", "vy", "vz", "p", "psc", "bxc", "byc", "bzc"]
-            l1 = {}
-            for v in range(9):
-                a = r32["state"][0][v][sk, sj, si]
-                b = r64["state"][0][v][sk, sj, si]
-                den = float(np.sum(np.abs(b)))
-                if den > 0:
-                    l1[names[v]] = float(np.sum(np.abs(a - b)) / den)
-            for c, nm in enumerate(("bxf", "byf", "bzf")):
-                a = r32["state"][1][c][sk, sj, si]
-                b = r64["state"][1][c][sk, sj, si]
-                l1[nm] = float(np.sum(np.abs(a - b)) / np.sum(np.abs(b)))
+        if rie["state"] is not None and r32["state"] is not None:
+            l1 = rel_l1(r32["state"], rie["state"])
             fp32c["rel_l1_vs_fp64"] = l1
             fp32c["rel_l1_max"] = max(l1.values())
-            fp32c["accuracy_steps"] = r64["steps_total"]
+            fp32c["accuracy_steps"] = rie["steps_total"]
         out["fp32c"] = fp32c
     if not args.no_batch:
         out["batched"] = batch_benches(args.batch_reps, not args.no_fp32c)
