@@ -39,16 +39,22 @@ def read_param(section, key, dtype, usr_dir):
 
 
 class _Params:
-    """Parameter source: a usr_dir with config.cfg, or a dict of sections."""
+    """Parameter source: a usr_dir with config.cfg, or a dict of sections.
+
+    With a dict, usr_dir may have no config.cfg (it is then only where the
+    output and restart files go) and keys the dict lacks take their
+    defaults; without a dict a missing config.cfg raises."""
 
     def __init__(self, usr_dir=None, config=None):
         self.usr_dir = usr_dir
         self.config = config or {}
+        self.has_file = usr_dir is not None and (
+            not self.config or os.path.exists(os.path.join(usr_dir, "config.cfg")))
 
     def get(self, section, key, dtype, default=None):
         if key in self.config.get(section, {}):
             v = self.config[section][key]
-        elif self.usr_dir is not None:
+        elif self.has_file:
             try:
                 v = read_param(section, key, dtype, self.usr_dir)
             except (configparser.NoOptionError, configparser.NoSectionError):
@@ -351,7 +357,8 @@ class Simulation:
             bfld[n][act] = ds[name]
         self.ctx.upload(prim, bfld)
         self.ctx.init_state()
-        self.ctx.set_time(attrs["time"], int(attrs["step"]))
+        # attributes come back as one-element arrays
+        self.ctx.set_time(float(np.ravel(attrs["time"])[0]), int(np.ravel(attrs["step"])[0]))
         self.integr.time, self.integr.step, self.integr.dt = self.ctx.time()
 
     def close(self):

@@ -24,9 +24,19 @@ def say(msg):
     sys.stdout.flush()
 
 
-def rel_l1(a, b):
-    den = np.sum(np.abs(b))
+def rel_l1(a, b, scale=0.0):
+    """sum|a - b| / sum|b|.  When scale is given and sum|b| < NOISE * scale,
+    b is round-off noise of a quantity that vanishes in exact arithmetic (a
+    transverse velocity or field component of a planar problem), and the
+    difference is measured against scale instead of against that noise."""
+    den = float(np.sum(np.abs(b)))
+    if den < NOISE * scale:
+        den = scale
     return float(np.sum(np.abs(a - b)) / den) if den > 0 else float(np.sum(np.abs(a - b)))
+
+
+# Below NOISE times the density's L1 norm a variable is round-off noise.
+NOISE = 1e-9
 
 
 def run_pair(prob, steps=None, tmax=None, rsolver="hlld", reconstr="linear", bc=(0,) * 6,
@@ -73,26 +83,27 @@ def _regrid(s, prim, bfld, ng):
 def compare(res, label):
     s, o = res["s"], res["o"]
     sk, sj, si = s.active()
+    scale = float(np.sum(np.abs(o.prim[0][sk, sj, si])))
     worst = 0.0
     bitwise = True
     for v in range(9):
         a = res["gp"][v][sk, sj, si]
         b = o.prim[v][sk, sj, si]
-        r = rel_l1(a, b)
+        r = rel_l1(a, b, scale)
         bitwise &= bool(np.array_equal(a, b))
         worst = max(worst, r)
         assert r <= TOL, f"{label}: prim[{v}] rel L1 {r:.3e}"
     for c in range(3):
         a = res["gb"][c][sk, sj, si]
         b = o.bfld[c][sk, sj, si]
-        r = rel_l1(a, b)
+        r = rel_l1(a, b, scale)
         bitwise &= bool(np.array_equal(a, b))
         worst = max(worst, r)
         assert r <= TOL, f"{label}: bfld[{c}] rel L1 {r:.3e}"
     for v in range(9):
         a = res["gc"][v][sk, sj, si]
         b = o.cons[v][sk, sj, si]
-        assert rel_l1(a, b) <= TOL, f"{label}: cons[{v}]"
+        assert rel_l1(a, b, scale) <= TOL, f"{label}: cons[{v}]"
     say(f"{label}: worst rel L1 {worst:.3e}, bitwise={bitwise}")
     return bitwise
 
@@ -123,8 +134,11 @@ def test_rsolver_seam_matches_oracle_on_host_rows():
     """bnz_hll_flux / bnz_hlld_flux / bnz_hllt_flux have RSolverFunc's
     signature (host row pointers, faces i1..i2, integrator.pxd:76-79): on
     the same rows they write what the oracle's pencil solvers write, bit for
-    bit, only at faces i1..i2, and a face with NaN states keeps its old
-    flux row (the reference leaves an unmatched face untouched)."""
+    bit, only at faces i1..i2, and a face with NaN states on both sides
+    keeps its old flux row (the reference leaves an unmatched face
+    untouched).  A NaN on one side only does not stop the solver: fmin/fmax
+    of the wave-speed estimates drop the NaN, and the face takes the other
+    side's region (fluxes_c.c:38-40)."""
     from bonzo_amd._lib import load
     from oracle import oracle as orc
     from oracle.oracle import _ptr, _rows
@@ -135,7 +149,9 @@ def test_rsolver_seam_matches_oracle_on_host_rows():
     wl, wr, bx = M.random_states(rng, n, bx_min=0.0)
     wl[1] += rng.uniform(-3, 3, n)
     wr[1] += rng.uniform(-3, 3, n)
-    wl[:, 100] = np.nan  # no region matches: the row stays as it was
+    wl[:, 100] = np.nan  # NaN on both sides: no region matches, the row stays
+    wr[:, 100] = np.nan
+    wl[:, 101] = np.nan  # NaN on the left only: the speeds come from the right
     L = load()
     for rs in ("hll", "hlld", "hllt"):
         sentinel = 7.25
@@ -150,6 +166,7 @@ def test_rsolver_seam_matches_oracle_on_host_rows():
         assert np.all(got[:, :i1] == sentinel) and np.all(got[:, i2 + 1:] == sentinel)
         if rs != "hll":  # hll writes every face (NaN here), as the reference's does
             assert np.all(got[:, 100] == sentinel), rs
+            assert np.all(np.isfinite(got[:, 101])), rs
         assert np.array_equal(got, want, equal_nan=True), \
             f"{rs}: max diff {np.nanmax(np.abs(got - want))}"
 
@@ -620,8 +637,11 @@ FMA_CASES = {
                              rsolver="hll", reconstr="const"),
     "brio_wu_1d": dict(prob=lambda: P.brio_wu(400, axis=0, transverse=0), tmax=0.08,
                        bc=(1,) * 6),
-    "hllt_ot32": dict(prob=lambda: P.orszag_tang(32), steps=40, rsolver="hllt"),
-    "blast2d_walls": dict(prob=lambda: P.blast2d(48, seed=21), tmax=0.05, bc=(2,) * 6),
+    # HLLT's zero fan-region energy flux loses positivity after 6 steps here
+    "hllt_ot32": dict(prob=lambda: P.orszag_tang(32), steps=5, rsolver="hllt"),
+    # the guide-field blast: a seeded blast whose field crosses the walls
+    # loses positivity by t = 0.02 in the reference's scheme (no floor)
+    "blast2d_walls": dict(prob=lambda: _blast_guide_field(48), tmax=0.05, bc=(2,) * 6),
 }
 
 
@@ -635,6 +655,10 @@ def test_fp64_fma_mode_within_1e12_of_oracle(case):
     prob = kw.pop("prob")()
     say(f"case: FP64-FMA mode vs oracle, {case}")
     res = run_pair(prob, precision="fp64fma", **kw)
+    o, (sk, sj, si) = res["o"], res["s"].active()
+    # a case is only meaningful while the oracle's own run stays physical
+    assert np.all(np.isfinite(o.prim[:, sk, sj, si]))
+    assert o.prim[0][sk, sj, si].min() > 0 and o.prim[4][sk, sj, si].min() > 0
     if "tmax" in kw:
         assert res["n"] == res["o"].step and res["t"] == res["o"].time
     compare(res, f"fp64fma {case}")
@@ -649,7 +673,7 @@ CHAR_CASES = {
                        bc=(1,) * 6),
     "brio_wu_y_3d": dict(prob=lambda: P.brio_wu(64, axis=1, transverse=4), steps=20,
                          bc=(0, 0, 1, 1, 0, 0)),
-    "hll_blast_walls": dict(prob=lambda: P.blast2d(40, seed=23), steps=15, bc=(2,) * 6,
+    "hll_blast_walls": dict(prob=lambda: _blast_guide_field(40), steps=40, bc=(2,) * 6,
                             rsolver="hll"),
 }
 
@@ -666,6 +690,9 @@ def test_characteristic_limiting_matches_oracle(case, precision):
     prob = kw.pop("prob")()
     say(f"case: charact_proj=1, {case}, {precision}")
     res = run_pair(prob, precision=precision, char_proj=True, **kw)
+    o, (sk, sj, si) = res["o"], res["s"].active()
+    assert np.all(np.isfinite(o.prim[:, sk, sj, si]))
+    assert o.prim[0][sk, sj, si].min() > 0 and o.prim[4][sk, sj, si].min() > 0
     bitwise = compare(res, f"char {case} {precision}")
     if precision == "fp64":
         assert bitwise

@@ -175,6 +175,28 @@ def test_unbuilt_physics_is_refused_not_dropped(key):
         BnzIntegr(_Params(config=bad))
 
 
+def test_config_dict_with_usr_dir_that_has_no_config_file(tmp_path):
+    """A dict of sections with a usr_dir that holds no config.cfg (the usr_dir
+    only receives out/ and rst/): keys the dict lacks take their defaults.
+    Without a dict the missing file still raises."""
+    from bonzo_amd.simulation import BnzGrid
+    cfg = {"computation": {"Nx": 16, "Ny": 16, "Nz": 1, "tmax": 0.1, "cour": 0.4,
+                           "rsolver": "hlld"},
+           "physics": {"gam": 5.0 / 3.0}}
+    p = _Params(str(tmp_path), cfg)
+    integr = BnzIntegr(p)
+    assert integr.tintegr == TIntegr.TINT_VL and integr.rsolver == RSolver.RS_HLLD
+    assert tuple(BnzGrid(p, integr).Nact) == (16, 16, 1)
+    with pytest.raises(FileNotFoundError):
+        BnzIntegr(_Params(str(tmp_path)))
+    # with the file present, the dict still overrides it key by key
+    (tmp_path / "config.cfg").write_text("[computation]\ncour: 0.2\ntintegr: rk3\n")
+    p = _Params(str(tmp_path), cfg)
+    with pytest.warns(UserWarning, match="rk3"):
+        integr = BnzIntegr(p)
+    assert integr.cour == 0.4
+
+
 def test_non_cartesian_geometry_is_refused():
     from bonzo_amd.simulation import BnzGrid
     cfg = {"computation": {"cour": 0.4, "Nx": 16, "Ny": 1, "Nz": 1},
