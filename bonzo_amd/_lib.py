@@ -13,6 +13,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libbnzgpu.so")
 
 BNZ_OK, BNZ_EINVAL, BNZ_ENOMEM, BNZ_EHIP, BNZ_EUNSUPPORTED, BNZ_ESTATE = 0, -1, -2, -3, -4, -5
+BNZ_ERANGE = -6
+BNZ_NOALLOC_MAX_FACES = 16384
 RS = {"hll": 0, "hllc": 1, "hlld": 2, "hlla": 3, "hllt": 4}
 RCN = {"const": 0, "linear": 1, "weno": 2, "parab": 3}
 PREC = {"fp64": 0, "fp32c": 1, "fp64fma": 2}
@@ -24,6 +26,7 @@ ABI_SYMBOLS = (
     "bnz_synchronize", "bnz_set_tuning", "bnz_set_kernel_timing", "bnz_kernel_timing",
     "bnz_state_size", "bnz_save_state", "bnz_load_state",
     "bnz_gpu_flux_pencils", "bnz_hll_flux", "bnz_hlld_flux", "bnz_hllt_flux",
+    "bnz_rsolver_rows_noalloc",
     "bnz_batch_create", "bnz_batch_destroy", "bnz_batch_last_error", "bnz_batch_box_shape",
     "bnz_batch_upload", "bnz_batch_download", "bnz_batch_init_state", "bnz_batch_advance",
     "bnz_batch_get_times", "bnz_batch_synchronize", "bnz_batch_set_tuning",
@@ -112,6 +115,8 @@ def load():
         getattr(L, name).argtypes = [ctypes.POINTER(_dp)] * 3 + [_dp, ctypes.c_int, ctypes.c_int,
                                                                   ctypes.c_double]
         getattr(L, name).restype = None
+    L.bnz_rsolver_rows_noalloc.argtypes = [ctypes.c_int] + [ctypes.POINTER(_dp)] * 3 + [
+        _dp, ctypes.c_int, ctypes.c_int, ctypes.c_double]
     L.bnz_batch_create.argtypes = [ctypes.POINTER(BnzParams), ctypes.POINTER(BnzBox),
                                    ctypes.c_int, ctypes.POINTER(vp)]
     L.bnz_batch_destroy.argtypes = [vp]

@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1095,29 +1096,24 @@ int bnz_gpu_flux_pencils(int rsolver, const double *wl, const double *wr, const 
 
 // The reference's Riemann-solver seam, RSolverFunc: host row pointers flux[n],
 // wl[n], wr[n] (n < 9) and bx, faces i1..i2.  Each call packs the rows into
-// one host block, runs k_pencil on the device and unpacks the flux rows.
-// The flux rows travel to the device too, so a face whose solver matches no
-// region keeps its old row, as in the reference.  Staging memory is kept per
-// host thread (the reference calls the solver from OpenMP threads).
-static int pencil_rows(int rs, double **flux, double **wl, double **wr, double *bx, int i1,
-                       int i2, double gam) {
+// one host block (wl 9n | wr 9n | bx n | flux 9n), copies it to the device,
+// runs k_pencil there and unpacks the flux rows.  The flux rows travel to
+// the device too, so a face whose solver matches no region keeps its old
+// row, as in the reference.
+static int seam_check(int rs, double **flux, double **wl, double **wr, double *bx, int i1,
+                      int i2, double gam) {
   if (!flux || !wl || !wr || !bx || i2 < i1 || !(gam > 1.)) return BNZ_EINVAL;
+  if (rs != BNZ_RS_HLL && rs != BNZ_RS_HLLD && rs != BNZ_RS_HLLT) return BNZ_EINVAL;
   for (int q = 0; q < NMODE; ++q)
     if (!flux[q] || !wl[q] || !wr[q]) return BNZ_EINVAL;
-  const long n = (long)i2 - i1 + 1;
-  thread_local std::vector<double> host;
-  thread_local double *dev = nullptr;
-  thread_local size_t cap = 0;
-  const size_t need = (size_t)28 * n;  // wl 9n | wr 9n | bx n | flux 9n
-  if (need > cap) {
-    if (dev) (void)hipFree(dev);
-    dev = nullptr;
-    cap = 0;
-    if (hipMalloc(&dev, need * sizeof(double)) != hipSuccess) return BNZ_ENOMEM;
-    cap = need;
-  }
-  host.resize(need);
-  double *h = host.data();
+  return BNZ_OK;
+}
+
+constexpr int SEAM_WORDS = 28;  // doubles staged per face
+
+// Pack, solve on the device through dev (SEAM_WORDS * n doubles) and unpack.
+static int seam_solve(int rs, double **flux, double **wl, double **wr, double *bx, int i1,
+                      long n, double gam, double *h, double *dev) {
   for (int q = 0; q < NMODE; ++q)
     for (long i = 0; i < n; ++i) {
       h[q * n + i] = wl[q][i1 + i];
@@ -1125,7 +1121,7 @@ static int pencil_rows(int rs, double **flux, double **wl, double **wr, double *
       h[(19 + q) * n + i] = flux[q][i1 + i];
     }
   for (long i = 0; i < n; ++i) h[18 * n + i] = bx[i1 + i];
-  if (hipMemcpy(dev, h, need * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+  if (hipMemcpy(dev, h, SEAM_WORDS * n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
     return BNZ_EHIP;
   const double gm1i = 1. / (gam - 1.);
   const double *dwl = dev, *dwr = dev + 9 * n, *dbx = dev + 18 * n;
@@ -1147,6 +1143,55 @@ static int pencil_rows(int rs, double **flux, double **wl, double **wr, double *
   return BNZ_OK;
 }
 
+// Fixed staging of bnz_rsolver_rows_noalloc: one device array per GPU (each
+// GPU that runs the entry point holds its own copy until the process ends)
+// and one host array, sized for BNZ_NOALLOC_MAX_FACES faces.  Host code
+// never passes the name `scratch` to a kernel or a copy: it stands for no
+// device address on the host.  The device address of each GPU's copy comes
+// from hipGetSymbolAddress, once per device, on the entry point's first call
+// with that device current, and only that address is used.
+static __device__ __align__(256) double scratch[(size_t)SEAM_WORDS * BNZ_NOALLOC_MAX_FACES];
+static double g_noalloc_host[(size_t)SEAM_WORDS * BNZ_NOALLOC_MAX_FACES];
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static double *g_scratch_ptr[NOALLOC_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+static int scratch_for_current_device(double **out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess) return BNZ_EHIP;
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return BNZ_EINVAL;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return BNZ_EHIP;
+    g_scratch_ptr[dev] = static_cast<double *>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return BNZ_OK;
+}
+
+// Staging memory of the seam is kept per host thread (the reference calls
+// the solver from OpenMP threads) and grows with the longest pencil.
+static int pencil_rows(int rs, double **flux, double **wl, double **wr, double *bx, int i1,
+                       int i2, double gam) {
+  const int rc = seam_check(rs, flux, wl, wr, bx, i1, i2, gam);
+  if (rc != BNZ_OK) return rc;
+  const long n = (long)i2 - i1 + 1;
+  thread_local std::vector<double> host;
+  thread_local double *dev = nullptr;
+  thread_local size_t cap = 0;
+  const size_t need = (size_t)SEAM_WORDS * n;
+  if (need > cap) {
+    if (dev) (void)hipFree(dev);
+    dev = nullptr;
+    cap = 0;
+    if (hipMalloc(&dev, need * sizeof(double)) != hipSuccess) return BNZ_ENOMEM;
+    cap = need;
+  }
+  host.resize(need);
+  return seam_solve(rs, flux, wl, wr, bx, i1, n, gam, host.data(), dev);
+}
+
 static void pencil_seam(const char *name, int rs, double **flux, double **wl, double **wr,
                         double *bx, int i1, int i2, double gam) {
   const int rc = pencil_rows(rs, flux, wl, wr, bx, i1, i2, gam);
@@ -1165,6 +1210,24 @@ void bnz_hlld_flux(double **flux, double **wl, double **wr, double *bx, int i1, 
 void bnz_hllt_flux(double **flux, double **wl, double **wr, double *bx, int i1, int i2,
                    double gam) {
   pencil_seam("bnz_hllt_flux", BNZ_RS_HLLT, flux, wl, wr, bx, i1, i2, gam);
+}
+
+int bnz_rsolver_rows_noalloc(int rsolver, double **flux, double **wl, double **wr, double *bx,
+                             int i1, int i2, double gam) {
+  int rc = seam_check(rsolver, flux, wl, wr, bx, i1, i2, gam);
+  if (rc != BNZ_OK) return rc;
+  const long n = (long)i2 - i1 + 1;
+  if (n > BNZ_NOALLOC_MAX_FACES) {
+    std::fprintf(stderr,
+                 "bnz_rsolver_rows_noalloc: %ld faces exceed the fixed staging of %d faces; "
+                 "split the pencil or call bnz_hll_flux / bnz_hlld_flux / bnz_hllt_flux\n",
+                 n, BNZ_NOALLOC_MAX_FACES);
+    return BNZ_ERANGE;
+  }
+  double *dev = nullptr;
+  rc = scratch_for_current_device(&dev);
+  if (rc != BNZ_OK) return rc;
+  return seam_solve(rsolver, flux, wl, wr, bx, i1, n, gam, g_noalloc_host, dev);
 }
 
 }  // extern "C"

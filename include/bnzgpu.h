@@ -30,7 +30,8 @@ enum {
   BNZ_ENOMEM = -2,       /* device allocation failed                        */
   BNZ_EHIP = -3,         /* HIP runtime error                               */
   BNZ_EUNSUPPORTED = -4, /* selection the device path does not implement    */
-  BNZ_ESTATE = -5        /* call out of order (e.g. step before upload)     */
+  BNZ_ESTATE = -5,       /* call out of order (e.g. step before upload)     */
+  BNZ_ERANGE = -6        /* size beyond a fixed maximum of the library      */
 };
 
 /* Riemann solvers: numbering of RSolver, src/integration/integrator.pxd:59-65 */
@@ -189,6 +190,22 @@ void bnz_hlld_flux(double **flux, double **wl, double **wr, double *bx, int i1, 
                    double gam);
 void bnz_hllt_flux(double **flux, double **wl, double **wr, double *bx, int i1, int i2,
                    double gam);
+
+/* No counterpart in the reference.  The Riemann-solver seam above for
+   callers that cannot allocate memory (tight per-pencil loops, harnesses
+   that forbid allocation): rsolver is BNZ_RS_HLL, BNZ_RS_HLLD or
+   BNZ_RS_HLLT, the rows and faces are those of bnz_hll_flux & co., and the
+   results are theirs bit for bit.  It allocates nothing: the rows are staged
+   through a fixed host array and a fixed device array of
+   28 x BNZ_NOALLOC_MAX_FACES doubles (3.7 MB) that each GPU it runs on holds
+   until the process ends.  At most BNZ_NOALLOC_MAX_FACES faces per call
+   (i2 - i1 + 1); a longer pencil returns BNZ_ERANGE and leaves the rows as
+   they were.  Runs on the current device, synchronous; calls that overlap
+   in one process race on the staging arrays, so callers serialise them.
+   Returns BNZ_OK or a BNZ_E* code. */
+#define BNZ_NOALLOC_MAX_FACES 16384
+int bnz_rsolver_rows_noalloc(int rsolver, double **flux, double **wl, double **wr, double *bx,
+                             int i1, int i2, double gam);
 
 /* ------------------------------------------------------------------------
  * Batches of independent boxes of uneven size (parameter sweeps).  Each box

@@ -284,3 +284,27 @@ def test_product_fails_loudly_without_the_library(monkeypatch):
             if f.endswith(".py"):
                 text = open(os.path.join(dirpath, f)).read()
                 assert "import oracle" not in text and "from oracle" not in text, f
+
+
+def test_noalloc_seam_rejects_bad_calls_without_touching_gpu():
+    """bnz_rsolver_rows_noalloc checks its arguments and its fixed size
+    before any device call: a pencil longer than BNZ_NOALLOC_MAX_FACES gives
+    BNZ_ERANGE, an unknown solver or i2 < i1 BNZ_EINVAL, and the rows stay
+    as they were."""
+    L = _lib.load()
+    dp = ctypes.POINTER(ctypes.c_double)
+    n = _lib.BNZ_NOALLOC_MAX_FACES + 1
+    bufs = [np.full((9, n), 7.25) for _ in range(3)]
+    bx = np.zeros(n)
+    rows = [(dp * 9)(*[ctypes.cast(b[q].ctypes.data, dp) for q in range(9)]) for b in bufs]
+    bxp = bx.ctypes.data_as(dp)
+    gam = 5.0 / 3.0
+    assert L.bnz_rsolver_rows_noalloc(_lib.RS["hlld"], *rows, bxp, 0, n - 1, gam) == \
+        _lib.BNZ_ERANGE
+    assert L.bnz_rsolver_rows_noalloc(_lib.RS["hllc"], *rows, bxp, 0, 10, gam) == \
+        _lib.BNZ_EINVAL
+    assert L.bnz_rsolver_rows_noalloc(_lib.RS["hll"], *rows, bxp, 10, 9, gam) == \
+        _lib.BNZ_EINVAL
+    assert L.bnz_rsolver_rows_noalloc(_lib.RS["hll"], *rows, bxp, 0, 10, 1.0) == \
+        _lib.BNZ_EINVAL
+    assert all(np.all(b == 7.25) for b in bufs)
