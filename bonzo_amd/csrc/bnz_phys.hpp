@@ -19,10 +19,12 @@ enum { RHO = 0, MX = 1, MY = 2, MZ = 3, EN = 4, PSC = 5, BX = 6, BY = 7, BZ = 8 
 enum { VX = 1, VY = 2, VZ = 3, PR = 4 };
 constexpr int NMODE = 9;
 
-// FP64 division and square root by the Newton-Raphson steps of the
-// compiler's IEEE expansion for gfx950, without its range scaling and
-// special-value fix-up.  Division: hardware reciprocal, two refinements
-// r += r(1 - b r), quotient q = a r and Markstein's correction
+// FP64 division and square root.
+//
+// Default (the IEEE unit, bit for bit the CPU oracle): the Newton-Raphson
+// steps of the compiler's IEEE expansion for gfx950, without its range
+// scaling and special-value fix-up.  Division: hardware reciprocal, two
+// refinements r += r(1 - b r), quotient q = a r and Markstein's correction
 // q + r(a - b q) (P. Markstein, IA-64 and Elementary Functions, 2000).
 // Square root: reciprocal square root y, g = x y, h = y/2, one coupled
 // refinement of g and h and two corrections g += h(x - g g).  The expansion
@@ -32,17 +34,19 @@ constexpr int NMODE = 9;
 // the expansion's steps and give the IEEE result bit for bit.  Differences
 // are confined to: a numerator of -0 (gives +0), a zero denominator (NaN
 // instead of an infinity).  A zero root argument is passed through.
-__device__ __forceinline__ double nr_div(double a, double b) {
-  double r = __builtin_amdgcn_rcp(b);
-  double e = fma(-b, r, 1.0);
-  r = fma(r, e, r);
-  e = fma(-b, r, 1.0);
-  r = fma(r, e, r);
-  const double q = a * r;
-  const double rem = fma(-b, q, a);
-  return fma(rem, r, q);
-}
-__device__ __forceinline__ double nr_sqrt(double x) {
+//
+// BNZ_FAST_F64 (defined only by the FMA-contracted unit, whose bar is 1e-12
+// relative L1 against the oracle, not bit equality): the hardware
+// reciprocal and reciprocal square root are good to 2^-24.4 and 2^-24.2
+// (measured on gfx950), so one correction step squares their error.
+// Division: q = a r, then q + r(a - b q), relative error about 2^-48;
+// square root: the coupled refinement of g = x y and h = y/2 alone.  Each
+// saves four of the default's eight or ten operations.  Device code is
+// compiled per translation unit (no relocatable device code), so the two
+// definitions never meet.
+// The IEEE square-root expansion, in both units: the characteristic
+// weights need sqrt(x*x) == |x| exactly (see CharWeights).
+__device__ __forceinline__ double nr_sqrt_ieee(double x) {
   const double y = __builtin_amdgcn_rsq(x);
   double g = x * y;
   double h = y * 0.5;
@@ -56,12 +60,41 @@ __device__ __forceinline__ double nr_sqrt(double x) {
   return x == 0.0 ? x : g;
 }
 
+#if defined(BNZ_FAST_F64)
+__device__ __forceinline__ double nr_div(double a, double b) {
+  const double r = __builtin_amdgcn_rcp(b);
+  const double q = a * r;
+  return fma(fma(-b, q, a), r, q);
+}
+__device__ __forceinline__ double nr_sqrt(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  const double g = x * y;
+  const double h = y * 0.5;
+  const double r = fma(-h, g, 0.5);
+  return x == 0.0 ? x : fma(g, r, g);
+}
+#else
+__device__ __forceinline__ double nr_div(double a, double b) {
+  double r = __builtin_amdgcn_rcp(b);
+  double e = fma(-b, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-b, r, 1.0);
+  r = fma(r, e, r);
+  const double q = a * r;
+  const double rem = fma(-b, q, a);
+  return fma(rem, r, q);
+}
+__device__ __forceinline__ double nr_sqrt(double x) { return nr_sqrt_ieee(x); }
+#endif
+
 template <typename T> struct Ar;
 template <> struct Ar<double> {
   static __device__ __forceinline__ double div(double a, double b) { return nr_div(a, b); }
   // quotient whose numerator may be -0: the full IEEE expansion
   static __device__ __forceinline__ double divz(double a, double b) { return a / b; }
   static __device__ __forceinline__ double root(double a) { return nr_sqrt(a); }
+  // square root rounded as IEEE sqrt in every unit
+  static __device__ __forceinline__ double root_ieee(double a) { return nr_sqrt_ieee(a); }
   static constexpr bool guarded = false;
 };
 template <> struct Ar<float> {
@@ -70,6 +103,7 @@ template <> struct Ar<float> {
   }
   static __device__ __forceinline__ float divz(float a, float b) { return div(a, b); }
   static __device__ __forceinline__ float root(float a) { return __builtin_amdgcn_sqrtf(a); }
+  static __device__ __forceinline__ float root_ieee(float a) { return __builtin_sqrtf(a); }
   static constexpr bool guarded = true;
 };
 
@@ -180,7 +214,9 @@ struct CharWeights {
     const T bperp2 = sqr(w_by) + sqr(w_bz);
     const T b2 = bx2 + bperp2;
     const T x = a2rho + b2;
-    const T y = Ar<T>::root(fabs(x * x - T(4) * a2rho * bx2));
+    // cs2rho = (x - y)/2 cancels where bx is small; with bx = 0 it is 0
+    // only if y = sqrt(x * x) is |x| exactly, which needs IEEE rounding
+    const T y = Ar<T>::root_ieee(fabs(x * x - T(4) * a2rho * bx2));
     const T cf2rho = T(0.5) * (x + y);
     const T cs2rho = T(0.5) * (x - y);
     cf = Ar<T>::root(rhoi * cf2rho);
