@@ -560,6 +560,8 @@ int bnz_batch_create(const bnz_params *p, const bnz_box *boxes, int nbox, bnz_ba
     return bfail(c, BNZ_EUNSUPPORTED, "hllt runs in FP64 only");
   if (p->reconstr != BNZ_RCN_CONST && p->reconstr != BNZ_RCN_LINEAR)
     return bfail(c, BNZ_EUNSUPPORTED, "reconstr must be const or linear");
+  if (p->hydro)
+    return bfail(c, BNZ_EUNSUPPORTED, "the hydro build runs on single grids only");
   if (p->char_proj)
     return bfail(c, BNZ_EUNSUPPORTED, "characteristic limiting runs on single grids only");
   if (!(p->gam > 1.) || !(p->cour > 0.)) return bfail(c, BNZ_EINVAL, "need gam > 1 and cour > 0");

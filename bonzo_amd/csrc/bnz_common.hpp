@@ -23,6 +23,12 @@ constexpr int STEP1_CELLS = BLK - 5;
 
 // --------------------------------------------------------------------------
 // Geometry passed by value to every kernel.
+// Kernel-side solver id of the hydro build's HLL (sound speed in the wave
+// estimate); the public ids are BNZ_RS_*.  CFL_HYDRO: the fms_ref value of
+// the hydro build's dt.
+constexpr int RS_HLL_HYDRO = 5;
+constexpr int CFL_HYDRO = 2;
+
 struct Geo {
   int nt[3];       // Ntot per axis
   int dims;
@@ -190,10 +196,20 @@ __device__ __forceinline__ void block_max_to(double v, double *part) {
   }
 }
 
-// CFL speed of one active cell (new_dt.pyx:74-91).
+// CFL speed of one active cell (new_dt.pyx:74-91).  fms_ref is 1 (the
+// reference's x fast speed on every axis, D2), 0 (per-axis fast speeds) or
+// CFL_HYDRO (the hydro build: the sound speed on every axis).
 template <class GM>
 __device__ __forceinline__ double cds_cell_g(const GM &gm, const Geo &g, const double *wv,
                                              int fms_ref, int i, int j, int k) {
+  if (fms_ref == CFL_HYDRO) {
+    const double a = fms_hydro(wv, g.gam);
+    const double cx = fabs(wv[VX]) + a;
+    const double cy = g.dims >= 2 ? fabs(wv[VY]) + a : 0.;
+    const double cz = g.dims >= 3 ? fabs(wv[VZ]) + a : 0.;
+    double cds = fmax(fmax(cx / gm.dlf(0, i), cy / gm.dlf(1, j)), cz / gm.dlf(2, k));
+    return (cds > 0.) ? cds : 0.;
+  }
   const double cx = fabs(wv[VX]) + fms(wv, g.gam);
   double cy = 0., cz = 0.;
   if (g.dims >= 2) {
@@ -292,6 +308,13 @@ inline int fail(bnz_ctx *c, int code, const std::string &msg) {
   } while (0)
 
 inline int blocks_for(long n) { return (int)((n + BLK - 1) / BLK); }
+
+// Kernel-side solver id and CFL mode of a context: the hydro build's HLL
+// takes the sound speed in its estimate and the hydro dt the sound speed.
+inline int rs_kern(const bnz_ctx *c) {
+  return (c->p.hydro && c->p.rsolver == BNZ_RS_HLL) ? RS_HLL_HYDRO : c->p.rsolver;
+}
+inline int cfl_mode(const bnz_ctx *c) { return c->p.hydro ? CFL_HYDRO : c->p.dt_fms_ref; }
 
 // Workgroups of a tiled per-cell kernel over box b (see cell_index).
 inline int tiled_blocks(const Box &b, int ta, int ma) {

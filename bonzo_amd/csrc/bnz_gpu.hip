@@ -331,6 +331,7 @@ __global__ __launch_bounds__(BLK) void k_pencil(const double *wl_, const double 
   wr[BX] = bx[t];
   int reg = (RS == BNZ_RS_HLLD)   ? flux_hlld(f, wl, wr, gam, gamm1i)
             : (RS == BNZ_RS_HLLT) ? flux_hllt(f, wl, wr, gam, gamm1i)
+            : (RS == BNZ_RS_HLLC) ? flux_hllc(f, wl, wr, gam, gamm1i)
                                   : flux_hll(f, wl, wr, gam, gamm1i);
   if (reg == 0) return;
 #pragma unroll
@@ -405,7 +406,7 @@ void launch_dt(bnz_ctx *c, int use_fixed, double fixed_dt) {
     nb = blocks_for(a.total());
     Launch t(c, "dt_partial");
     hipLaunchKernelGGL(k_dt_partial, dim3(nb), dim3(BLK), 0, c->stream, c->g, c->w, a,
-                       c->p.dt_fms_ref, c->part);
+                       cfl_mode(c), c->part);
   }
   Launch t(c, "dt_final");
   hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(BLK), 0, c->stream, c->part, nb, c->p.cour,
@@ -456,10 +457,22 @@ int configure_grid(bnz_ctx *c, const bnz_params *p) {
   if (p->precision != BNZ_PREC_FP64 && p->precision != BNZ_PREC_FP32C &&
       p->precision != BNZ_PREC_FP64_FMA)
     return fail(c, BNZ_EINVAL, "precision must be BNZ_PREC_FP64, BNZ_PREC_FP32C or BNZ_PREC_FP64_FMA");
-  if (p->rsolver == BNZ_RS_HLLC)
-    return fail(c, BNZ_EUNSUPPORTED, "hllc is a hydro-only solver (fluxes_c.c:230); MHD needs hll or hlld");
-  if (p->rsolver != BNZ_RS_HLLD && p->rsolver != BNZ_RS_HLL && p->rsolver != BNZ_RS_HLLT)
-    return fail(c, BNZ_EUNSUPPORTED, "rsolver not implemented on the device (hll, hlld, hllt are)");
+  if (p->hydro != 0 && p->hydro != 1) return fail(c, BNZ_EINVAL, "hydro must be 0 or 1");
+  if (p->hydro) {
+    // the hydro build binds hll and hllc here (integrator.pyx:253-262)
+    if (p->rsolver != BNZ_RS_HLL && p->rsolver != BNZ_RS_HLLC)
+      return fail(c, BNZ_EUNSUPPORTED, "the hydro build runs hll or hllc on the device");
+    if (p->precision == BNZ_PREC_FP32C)
+      return fail(c, BNZ_EUNSUPPORTED, "the hydro build runs in FP64 (BNZ_PREC_FP64 or _FMA)");
+    if (p->char_proj)
+      return fail(c, BNZ_EUNSUPPORTED, "characteristic limiting of the hydro build is not built");
+  } else if (p->rsolver == BNZ_RS_HLLC) {
+    return fail(c, BNZ_EUNSUPPORTED,
+                "hllc is a hydro-only solver (fluxes_c.c:230); set hydro = 1, or use hll or hlld");
+  }
+  if (p->rsolver != BNZ_RS_HLLD && p->rsolver != BNZ_RS_HLL && p->rsolver != BNZ_RS_HLLT &&
+      p->rsolver != BNZ_RS_HLLC)
+    return fail(c, BNZ_EUNSUPPORTED, "rsolver not implemented on the device (hll, hlld, hllt, hllc are)");
   if (p->rsolver == BNZ_RS_HLLT && p->precision == BNZ_PREC_FP32C)
     return fail(c, BNZ_EUNSUPPORTED, "hllt runs in FP64 only");
   if (p->reconstr != BNZ_RCN_CONST && p->reconstr != BNZ_RCN_LINEAR)
@@ -687,12 +700,18 @@ int bnz_upload(bnz_ctx *c, const double *prim, const double *bfld) {
   c->dt_parts = 0;
   // ordered after the steps still running on the context's stream (a plain
   // hipMemcpy would not wait for a non-blocking stream)
+  if (c->p.hydro && bfld)  // the hydro build has no field
+    for (long q = 0; q < 3 * c->ncell; ++q)
+      if (bfld[q] != 0.)
+        return fail(c, BNZ_EINVAL, "hydro build: every face field value must be zero");
   if (prim)
     HIPCHK(c, hipMemcpyAsync(c->w, prim, NMODE * c->ncell * sizeof(double),
                              hipMemcpyHostToDevice, c->stream));
   if (bfld)
     HIPCHK(c, hipMemcpyAsync(c->b, bfld, 3 * c->ncell * sizeof(double), hipMemcpyHostToDevice,
                              c->stream));
+  else if (c->p.hydro)
+    HIPCHK(c, hipMemsetAsync(c->b, 0, 3 * c->ncell * sizeof(double), c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   // the uploaded state has no ghosts, cell-centred field or FP32C words yet
   c->ready = false;
@@ -1079,6 +1098,9 @@ int bnz_gpu_flux_pencils(int rsolver, const double *wl, const double *wr, const 
   const double gm1i = 1. / (gam - 1.);
   if (rsolver == BNZ_RS_HLLD)
     hipLaunchKernelGGL(k_pencil<BNZ_RS_HLLD>, dim3(blocks_for(n)), dim3(BLK), 0, 0, wl, wr, bx,
+                       flux, n, gam, gm1i);
+  else if (rsolver == BNZ_RS_HLLC)
+    hipLaunchKernelGGL(k_pencil<BNZ_RS_HLLC>, dim3(blocks_for(n)), dim3(BLK), 0, 0, wl, wr, bx,
                        flux, n, gam, gm1i);
   else if (rsolver == BNZ_RS_HLL)
     hipLaunchKernelGGL(k_pencil<BNZ_RS_HLL>, dim3(blocks_for(n)), dim3(BLK), 0, 0, wl, wr, bx,

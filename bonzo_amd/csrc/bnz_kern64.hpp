@@ -71,6 +71,10 @@ __device__ __forceinline__ void flux_face_to(const Geo &g, const double *__restr
     reg = flux_hlld(f, wl, wr, g.gam, g.gamm1i);
   else if (RS == BNZ_RS_HLLT)
     reg = flux_hllt(f, wl, wr, g.gam, g.gamm1i);
+  else if (RS == BNZ_RS_HLLC)
+    reg = flux_hllc(f, wl, wr, g.gam, g.gamm1i);
+  else if (RS == RS_HLL_HYDRO)
+    reg = flux_hll<double, true>(f, wl, wr, g.gam, g.gamm1i);
   else
     reg = flux_hll(f, wl, wr, g.gam, g.gamm1i);
   if (reg == 0) return;  // reference leaves the flux row untouched

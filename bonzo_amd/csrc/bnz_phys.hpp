@@ -181,6 +181,14 @@ __device__ __forceinline__ T fms(const T *w, T gam) {
   return Ar<T>::root(Ar<T>::div(T(0.5), w[RHO]) * (c1 + c2));
 }
 
+// Sound speed of the hydro build (MFIELD=0), the branch of fms without
+// field (src/mhd/eos_c.h:523-527): sqrt(gam p / rho).
+template <typename T>
+__device__ __forceinline__ T fms_hydro(const T *w, T gam) {
+  const T gamp = gam * w[PR];
+  return Ar<T>::root(Ar<T>::div(gamp, w[RHO]));
+}
+
 // Half the van Leer limited slope, 0.5 * vl_lim(a, b) as PLM adds it
 // (src/coordinates/reconstr_c.h:39-44, reconstr_c.c:77): vl_lim is
 // 2ab / (a+b) where ab > 0, else 0.  Scaling by 2 commutes with rounding, so
@@ -321,18 +329,21 @@ __device__ __forceinline__ void plm_half_slope(T *d, const T *wm, const T *wc, c
   }
 }
 
-// sLsR_simple, src/mhd/fluxes_c.c:25-42 (FMAX/FMIN as fmax/fmin)
-template <typename T>
+// sLsR_simple, src/mhd/fluxes_c.c:25-42 (FMAX/FMIN as fmax/fmin); HYDRO:
+// the hydro build, whose fms is the sound speed
+template <typename T, bool HYDRO = false>
 __device__ __forceinline__ void speeds_simple(T &sL, T &sR, const T *wL, const T *wR, T gam) {
-  T cfL = fms(wL, gam);
-  T cfR = fms(wR, gam);
+  T cfL = HYDRO ? fms_hydro(wL, gam) : fms(wL, gam);
+  T cfR = HYDRO ? fms_hydro(wR, gam) : fms(wR, gam);
   T cf_max = fmax(cfL, cfR);
   sL = fmin(wL[VX], wR[VX]) - cf_max;
   sR = fmax(wL[VX], wR[VX]) + cf_max;
 }
 
-// HLL flux, src/mhd/fluxes_c.c:175-223.  Always writes (returns 1).
-template <typename T>
+// HLL flux, src/mhd/fluxes_c.c:175-223.  Always writes (returns 1).  HYDRO:
+// the hydro build's solver (sound speed in the estimate); with zero field
+// the MHD prim2cons and primcons2flux are its hydro forms bit for bit.
+template <typename T, bool HYDRO = false>
 __device__ __forceinline__ int flux_hll(T *flux, const T *wL, const T *wR, T gam, T gamm1i) {
   T uL[NMODE], uR[NMODE], fL[NMODE], fR[NMODE];
   prim2cons(uL, wL, gamm1i);
@@ -340,7 +351,7 @@ __device__ __forceinline__ int flux_hll(T *flux, const T *wL, const T *wR, T gam
   primcons2flux(fL, wL, uL);
   primcons2flux(fR, wR, uR);
   T sL, sR;
-  speeds_simple(sL, sR, wL, wR, gam);
+  speeds_simple<T, HYDRO>(sL, sR, wL, wR, gam);
   T aL = fmin(sL, T(0));
   T aR = fmax(sR, T(0));
   T aLaR = aL * aR;
@@ -348,6 +359,78 @@ __device__ __forceinline__ int flux_hll(T *flux, const T *wL, const T *wR, T gam
 #pragma unroll
   for (int n = 0; n < NMODE; ++n)
     flux[n] = (aR * fL[n] - aL * fR[n] + aLaR * (uR[n] - uL[n])) * aRmaLi;
+  flux[BX] = T(0);
+  return 1;
+}
+
+// Toro's pressure-based wave speeds (PVRS), sLsR_pressure of
+// src/mhd/fluxes_c.c:138-168 in the hydro build (no TWOTEMP).
+template <typename T>
+__device__ __forceinline__ void speeds_pressure(T &sL, T &sR, const T *wL, const T *wR, T gam) {
+  const T g1gh = T(0.5) * (gam + T(1)) / gam;
+  const T rhoL = wL[RHO], rhoR = wR[RHO], vxL = wL[VX], vxR = wR[VX];
+  const T ptL = wL[PR], ptR = wR[PR];
+  const T aL = fms_hydro(wL, gam);
+  const T aR = fms_hydro(wR, gam);
+  const T pvrs = fmax(T(0), T(0.5) * (ptL + ptR) - T(0.125) * (vxR - vxL) * (rhoL + rhoR) * (aL + aR));
+  const T qL = (pvrs < ptL) ? T(1) : Ar<T>::root(T(1) + g1gh * (Ar<T>::div(pvrs, ptL) - T(1)));
+  const T qR = (pvrs < ptR) ? T(1) : Ar<T>::root(T(1) + g1gh * (Ar<T>::div(pvrs, ptR) - T(1)));
+  sL = vxL - aL * qL;
+  sR = vxR + aR * qR;
+}
+
+// HLLC flux of the hydro build, src/mhd/fluxes_c.c:230-327: Toro's HLLC in
+// the form z (F - s U) + s/(s - sM) p* (0, 1, 0, 0, sM), with the outer
+// speeds clipped at 0 so the supersonic cases take F_L or F_R.  9-slot
+// states with zero field; the field slots of the flux come out zero.
+// Always writes (returns 1).
+template <typename T>
+__device__ __forceinline__ int flux_hllc(T *flux, const T *wL, const T *wR, T gam, T gamm1i) {
+  T uL[NMODE], uR[NMODE];
+  prim2cons(uL, wL, gamm1i);
+  prim2cons(uR, wR, gamm1i);
+  T sL, sR;
+  speeds_pressure(sL, sR, wL, wR, gam);
+  const T ptL = wL[PR], ptR = wR[PR];
+  T vLsL = wL[VX] - sL;
+  T vRsR = wR[VX] - sR;
+  const T xL = uL[RHO] * vLsL;
+  const T xR = uR[RHO] * vRsR;
+  const T xRxLi = Ar<T>::div(T(1), xR - xL);
+  const T yL = ptL + uL[MX] * vLsL;
+  const T yR = ptR + uR[MX] * vRsR;
+  const T sM = xRxLi * (yR - yL);
+  const T pts = xRxLi * (xR * yL - xL * yR);
+  sL = fmin(sL, T(0));
+  sR = fmax(sR, T(0));
+  vLsL = wL[VX] - sL;
+  vRsR = wR[VX] - sR;
+  T fL[NMODE], fR[NMODE];
+#pragma unroll
+  for (int n = 0; n < NMODE; ++n) {
+    fL[n] = uL[n] * vLsL;
+    fR[n] = uR[n] * vRsR;
+  }
+  fL[MX] += ptL;
+  fR[MX] += ptR;
+  fL[EN] += ptL * wL[VX];
+  fR[EN] += ptR * wR[VX];
+  const T sLsMi = Ar<T>::div(T(1), sL - sM);
+  const T sRsMi = Ar<T>::div(T(1), sR - sM);
+  T zL, zR, sLR;
+  if (sM > T(0)) {
+    zL = -sM * sLsMi;
+    zR = T(0);
+    sLR = sL * sLsMi;
+  } else {
+    zL = T(0);
+    zR = -sM * sRsMi;
+    sLR = sR * sRsMi;
+  }
+#pragma unroll
+  for (int n = 0; n < NMODE; ++n) flux[n] = zL * fL[n] + zR * fR[n];
+  flux[MX] += sLR * pts;
+  flux[EN] += sLR * pts * sM;
   flux[BX] = T(0);
   return 1;
 }

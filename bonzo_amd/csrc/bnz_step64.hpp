@@ -136,14 +136,18 @@ void launch_fluxes(bnz_ctx *c, const double *w, const double *bf, const Lims &L)
 
 void launch_flux_dispatch(bnz_ctx *c, int order, const double *w, const double *bf,
                           const Lims &L) {
-  const int rs = c->p.rsolver;
+  const int rs = rs_kern(c);
   if (order == 0) {
     if (rs == BNZ_RS_HLLD) launch_fluxes<0, BNZ_RS_HLLD>(c, w, bf, L);
     else if (rs == BNZ_RS_HLLT) launch_fluxes<0, BNZ_RS_HLLT>(c, w, bf, L);
+    else if (rs == BNZ_RS_HLLC) launch_fluxes<0, BNZ_RS_HLLC>(c, w, bf, L);
+    else if (rs == RS_HLL_HYDRO) launch_fluxes<0, RS_HLL_HYDRO>(c, w, bf, L);
     else launch_fluxes<0, BNZ_RS_HLL>(c, w, bf, L);
   } else {
     if (rs == BNZ_RS_HLLD) launch_fluxes<1, BNZ_RS_HLLD>(c, w, bf, L);
     else if (rs == BNZ_RS_HLLT) launch_fluxes<1, BNZ_RS_HLLT>(c, w, bf, L);
+    else if (rs == BNZ_RS_HLLC) launch_fluxes<1, BNZ_RS_HLLC>(c, w, bf, L);
+    else if (rs == RS_HLL_HYDRO) launch_fluxes<1, RS_HLL_HYDRO>(c, w, bf, L);
     else launch_fluxes<1, BNZ_RS_HLL>(c, w, bf, L);
   }
 }
@@ -172,7 +176,7 @@ void launch_update(bnz_ctx *c, bool pred, const Lims &L) {
     grid.n[q] = c->nt[q];
   }
   const Box act = active_box(c);
-  const int fr = c->p.dt_fms_ref;
+  const int fr = cfl_mode(c);
   const Shape sh = work_shape(grid, c->dims, c->update_chunk);
   Launch t(c, pred ? "update_pred" : "update_corr");
 #define UPD_ARGS                                                                              \
@@ -207,24 +211,28 @@ void step1_launch(bnz_ctx *c, const Lims &L0, const Lims &L1) {
   if (ORDER == 1 && c->p.char_proj)
     hipLaunchKernelGGL((k_step1<ORDER, RS, true>), dim3(nb), dim3(BLK), 0, c->stream, c->g, c->w,
                        c->b, c->ws, c->bs, c->fx, L0, L1, c->dev, active_box(c),
-                       c->p.dt_fms_ref, c->part);
+                       cfl_mode(c), c->part);
   else
     hipLaunchKernelGGL((k_step1<ORDER, RS, false>), dim3(nb), dim3(BLK), 0, c->stream, c->g, c->w,
                        c->b, c->ws, c->bs, c->fx, L0, L1, c->dev, active_box(c),
-                       c->p.dt_fms_ref, c->part);
+                       cfl_mode(c), c->part);
   c->dt_parts = nb;
   std::swap(c->w, c->ws);  // the faces swap with the other stages' below
 }
 
 void launch_step1(bnz_ctx *c, const Lims &L0, const Lims &L1) {
-  const int rs = c->p.rsolver;
+  const int rs = rs_kern(c);
   if (c->order == 0) {
     if (rs == BNZ_RS_HLLD) step1_launch<0, BNZ_RS_HLLD>(c, L0, L1);
     else if (rs == BNZ_RS_HLLT) step1_launch<0, BNZ_RS_HLLT>(c, L0, L1);
+    else if (rs == BNZ_RS_HLLC) step1_launch<0, BNZ_RS_HLLC>(c, L0, L1);
+    else if (rs == RS_HLL_HYDRO) step1_launch<0, RS_HLL_HYDRO>(c, L0, L1);
     else step1_launch<0, BNZ_RS_HLL>(c, L0, L1);
   } else {
     if (rs == BNZ_RS_HLLD) step1_launch<1, BNZ_RS_HLLD>(c, L0, L1);
     else if (rs == BNZ_RS_HLLT) step1_launch<1, BNZ_RS_HLLT>(c, L0, L1);
+    else if (rs == BNZ_RS_HLLC) step1_launch<1, BNZ_RS_HLLC>(c, L0, L1);
+    else if (rs == RS_HLL_HYDRO) step1_launch<1, RS_HLL_HYDRO>(c, L0, L1);
     else step1_launch<1, BNZ_RS_HLL>(c, L0, L1);
   }
 }

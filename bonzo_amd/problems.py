@@ -237,6 +237,44 @@ def brio_wu(n, axis=0, ng=3, transverse=4):
     return s, prim, bfld
 
 
+def sod(n, axis=0, ng=3, transverse=0, vel=0.0):
+    """Sod's shock tube for the hydro build (no field): left rho=1, p=1,
+    right rho=0.125, p=0.1, both moving with ``vel`` along the tube, laid
+    along ``axis`` with ``transverse`` cells in the other active axes (1D
+    if 0); interface at the box centre."""
+    sizes = [transverse] * 3 if transverse else [1, 1, 1]
+    sizes[axis] = n
+    if not transverse and axis != 0:
+        raise ValueError("1D tubes lie along x")
+    s = GridShape(sizes[0], sizes[1], sizes[2], ng)
+    prim, bfld = s.empty()
+    shape = [1, 1, 1]
+    shape[2 - axis] = n
+    L = (s.centres(axis) <= 0.5).reshape(shape)
+    full = np.ones((s.nz if s.dims >= 3 else 1, s.ny if s.dims >= 2 else 1, s.nx))
+    act = s.active()
+    prim[RHO][act] = np.where(L, 1.0, 0.125) * full
+    prim[PR][act] = np.where(L, 1.0, 0.1) * full
+    prim[VX + axis][act] = vel * full
+    return s, prim, bfld
+
+
+def hydro_blast(n, ng=3, dims=2, p_in=10.0, p_out=0.1, radius=0.1, seed=7):
+    """Blast without field for the hydro build: rho=1 (seeded +-5 %
+    perturbation), p=p_out, p=p_in inside r<radius of the box centre, on a
+    periodic square (dims=2) or cube (dims=3)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    s = GridShape(n, n, n if dims == 3 else 1, ng)
+    prim, bfld = s.empty()
+    X = np.meshgrid(*[s.centres(a) for a in range(dims)][::-1], indexing="ij")
+    r2 = sum((x - 0.5) ** 2 for x in X)
+    shp = prim[RHO][s.active()].shape
+    _put(prim[RHO], s, (1.0 + 0.05 * rng.uniform(-1, 1, shp)))
+    _put(prim[PR], s, np.where(r2.reshape(shp) < radius ** 2, p_in, p_out))
+    _put(prim[PSC], s, np.where(r2.reshape(shp) < radius ** 2, 1.0, 0.0))
+    return s, prim, bfld
+
+
 def linear_wave(n, wave="fast", amp=1e-6, ng=3):
     """1D linear MHD eigenmode on [0,1] (state of src/problem/linwaves1d.pyx:
     48-104): rho=1, p=0.6, B=(1, sqrt 2, 1/2), gamma=5/3 (c_s=1, c_A,x=1,

@@ -95,10 +95,13 @@ class BnzIntegr:
 
     Unknown ``tintegr``/``rsolver``/``reconstr`` names fall back to VL/HLL/
     linear as in the reference.  Combinations the reference leaves with a
-    NULL function pointer (hllc or hlla with a magnetic field) raise here
-    instead of crashing later."""
+    NULL function pointer (hllc or hlla with a magnetic field, hlld without)
+    raise here instead of crashing later.  ``mfield`` is the reference's
+    build switch (setup.py -mf, MFIELD): False is its hydro build, which
+    binds hll and hllc (integrator.pyx:253-262)."""
 
-    def __init__(self, params):
+    def __init__(self, params, mfield=True):
+        self.mfield = bool(mfield)
         self.cour = params.get("computation", "cour", "f")
         t = params.get("computation", "tintegr", "s", "vl")
         self.tintegr = TIntegr.TINT_RK3 if t == "rk3" else TIntegr.TINT_VL
@@ -118,9 +121,15 @@ class BnzIntegr:
         self.time = 0.0
         self.step = 0
         self.dt = 0.0
-        if self.rsolver in (RSolver.RS_HLLC, RSolver.RS_HLLA):
+        if self.mfield and self.rsolver in (RSolver.RS_HLLC, RSolver.RS_HLLA):
             raise ValueError(f"rsolver {r} has no magnetic-field solver bound "
                              "(src/integration/integrator.pyx:253-268)")
+        if not self.mfield:
+            if self.rsolver in (RSolver.RS_HLLD, RSolver.RS_HLLA):
+                raise ValueError(f"rsolver {r} needs the magnetic field "
+                                 "(src/integration/integrator.pyx:258-268)")
+            if self.rsolver == RSolver.RS_HLLT:
+                raise NotImplementedError("hllt of the hydro build is not built (hll, hllc are)")
         if self.tintegr == TIntegr.TINT_RK3:
             # The reference parses rk3 and allocates its extra stage arrays
             # (integrator.pyx:339-361), but Simulation always drives
@@ -134,6 +143,8 @@ class BnzIntegr:
         # PLM slopes limited in characteristic variables
         if self.char_proj not in (0, 1):
             raise ValueError("charact_proj must be 0 or 1")
+        if self.char_proj and not self.mfield:
+            raise NotImplementedError("characteristic limiting of the hydro build is not built")
         # Physics the reference adds around the step whenever its coefficients
         # are non-zero: constant gravity (src/srcterms/gravity.pyx:15-17,
         # applied in integrator.pyx:146-157), turbulence driving
@@ -194,6 +205,10 @@ PROBLEMS = {
     "brio_wu": lambda grid, p: problems.brio_wu(grid.Nact[0], 0, grid.ng, transverse=0)[1:],
     "blast": lambda grid, p: problems.blast2d(grid.Nact[0], grid.ng,
                                               seed=p.get("problem", "seed", "i", 5005))[1:],
+    # hydro build (mfield=False)
+    "sod": lambda grid, p: problems.sod(grid.Nact[0], 0, grid.ng)[1:],
+    "hydro_blast": lambda grid, p: problems.hydro_blast(
+        grid.Nact[0], grid.ng, dims=grid.dims, seed=p.get("problem", "seed", "i", 7))[1:],
 }
 
 
@@ -205,20 +220,21 @@ class Simulation:
     conditions over the full grid (active cells set); by default the
     ``[problem] name`` entry picks one of ``PROBLEMS``."""
 
-    def __init__(self, usr_dir=None, config=None, set_problem=None, device=0):
+    def __init__(self, usr_dir=None, config=None, set_problem=None, device=0, mfield=True):
         p = _Params(usr_dir, config)
         self.usr_dir = usr_dir
-        self.integr = BnzIntegr(p)
+        self.integr = BnzIntegr(p, mfield=mfield)
         self.grid = BnzGrid(p, self.integr)
-        rs = {RSolver.RS_HLL: "hll", RSolver.RS_HLLD: "hlld", RSolver.RS_HLLT: "hllt"}[
-            self.integr.rsolver]
+        rs = {RSolver.RS_HLL: "hll", RSolver.RS_HLLD: "hlld", RSolver.RS_HLLT: "hllt",
+              RSolver.RS_HLLC: "hllc"}[self.integr.rsolver]
         self.ctx = _lib.Context(*self.grid.Nact, rsolver=rs,
                                 reconstr="const" if self.integr.reconstr_order == 0 else "linear",
                                 bc=self.grid.bc_flags, lmin=self.grid.lmin, lmax=self.grid.lmax,
                                 gam=self.integr.gam, cour=self.integr.cour,
                                 dt_fms_ref=self.integr.dt_fms_ref,
                                 precision=p.get("computation", "precision", "s", "fp64"),
-                                device=device, char_proj=bool(self.integr.char_proj))
+                                device=device, char_proj=bool(self.integr.char_proj),
+                                hydro=not self.integr.mfield)
         if set_problem is None:
             name = p.get("problem", "name", "s", "otvortex")
             prim, bfld = PROBLEMS[name](self.grid, p)

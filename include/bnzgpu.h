@@ -66,6 +66,13 @@ typedef struct {
                            slopes in characteristic variables of each cell's
                            state (src/coordinates/reconstr_c.c:64-80,
                            src/mhd/eos_c.h:100-280); FP64 single grids only */
+  int hydro;            /* 1: the reference's hydro build (setup.py without
+                           -mf, MFIELD=0): rsolver BNZ_RS_HLL or BNZ_RS_HLLC
+                           (fluxes_c.c:230-327) with the sound speed in their
+                           wave estimates and in the dt (eos_c.h:523-527).
+                           The state keeps the field slots, which must be
+                           zero (bnz_upload checks) and stay exactly zero.
+                           FP64 single grids only; 0 is the MHD build */
 } bnz_params;
 
 typedef struct {
@@ -168,7 +175,8 @@ int bnz_kernel_timing(bnz_ctx *ctx, int idx, char *name, int name_len, double *t
 /* Riemann solver on n faces, device pointers, layout [9][n] in the local
    frame of the face (slot 6 = normal field, overwritten by bx), on the
    default stream; synchronous.  Same seam as the pencil solvers of
-   src/mhd/fluxes_c.h:3-21 (hll_flux, hlld_flux). */
+   src/mhd/fluxes_c.h:3-21 (hll_flux, hlld_flux).  BNZ_RS_HLLC is the hydro
+   build's HLLC (fluxes_c.c:230-327) on states with zero field. */
 int bnz_gpu_flux_pencils(int rsolver, const double *wl, const double *wr, const double *bx,
                          double *flux, long n, double gam);
 

@@ -122,6 +122,15 @@ double or_fms(const double *w, double bx, double gam)
   return sqrt(0.5 / w[RHO] * (c1 + c2));
 }
 
+/* Sound speed of the hydro build (MFIELD=0): src/mhd/eos_c.h:468-530, the
+   branch without field, SQRT(gamp / w[RHO]). */
+double or_fms_hydro(const double *w, double gam)
+{
+  double p = w[PR];
+  double gamp = gam * p;
+  return sqrt(gamp / w[RHO]);
+}
+
 /* src/coordinates/reconstr_c.h:39-44 */
 double or_vl_lim(double a, double b)
 {
@@ -143,6 +152,43 @@ static inline void sLsR_simple(double *sL, double *sR, const double *wL, const d
   double cf_max = fmax(cfL, cfR);
   *sL = fmin(wL[VX], wR[VX]) - cf_max;
   *sR = fmax(wL[VX], wR[VX]) + cf_max;
+}
+
+/* sLsR_simple of the hydro build: the same estimate with the sound speed
+   (src/mhd/fluxes_c.c:25-42, bx = 0 without MFIELD) */
+static inline void sLsR_simple_hydro(double *sL, double *sR, const double *wL, const double *wR,
+                                     double gam)
+{
+  double cfL = or_fms_hydro(wL, gam);
+  double cfR = or_fms_hydro(wR, gam);
+  double cf_max = fmax(cfL, cfR);
+  *sL = fmin(wL[VX], wR[VX]) - cf_max;
+  *sR = fmax(wL[VX], wR[VX]) + cf_max;
+}
+
+/* Toro's pressure-based wave speeds (PVRS), src/mhd/fluxes_c.c:138-168,
+   hydro build (fms is the sound speed; no TWOTEMP) */
+static inline void sLsR_pressure(double *sL, double *sR, const double *wL, const double *wR,
+                                 double gam)
+{
+  double rhoL, rhoR, vxL, vxR;
+  double aL, aR, ptL, ptR, qL, qR, pvrs;
+  double g1gh = 0.5 * (gam + 1.) / gam;
+  rhoL = wL[RHO];
+  rhoR = wR[RHO];
+  vxL = wL[VX];
+  vxR = wR[VX];
+  ptL = wL[PR];
+  ptR = wR[PR];
+  aL = or_fms_hydro(wL, gam);
+  aR = or_fms_hydro(wR, gam);
+  pvrs = fmax(0., 0.5 * (ptL + ptR) - 0.125 * (vxR - vxL) * (rhoL + rhoR) * (aL + aR));
+  if (pvrs < ptL) qL = 1.;
+  else qL = sqrt(1. + g1gh * (pvrs / ptL - 1.));
+  if (pvrs < ptR) qR = 1.;
+  else qR = sqrt(1. + g1gh * (pvrs / ptR - 1.));
+  *sL = vxL - aL * qL;
+  *sR = vxR + aR * qR;
 }
 
 static inline void load_face(double *wL, double *wR, double **_wL, double **_wR,
@@ -175,6 +221,89 @@ void or_hll_flux(double **_flux, double **_wL, double **_wR, double *_bx, int i1
     double aRmaLi = 1. / (aR - aL);
     for (int n = 0; n < NMODE; ++n)
       _flux[n][i] = (aR * fL[n] - aL * fR[n] + aLaR * (uR[n] - uL[n])) * aRmaLi;
+    _flux[BX][i] = 0.;
+  }
+}
+
+/* hll_flux of the hydro build (src/mhd/fluxes_c.c:175-223 without MFIELD):
+   the sound speed in the wave estimate.  Rows are the 9-slot MHD layout
+   with zero field (prim2cons, primcons2flux and their B terms then reduce
+   to the hydro forms exactly); the B slots of the flux come out zero. */
+void or_hll_flux_hydro(double **_flux, double **_wL, double **_wR, double *_bx, int i1,
+                       int i2, double gam)
+{
+  double uL[NMODE], uR[NMODE], wL[NMODE], wR[NMODE], fL[NMODE], fR[NMODE];
+  for (int i = i1; i <= i2; ++i) {
+    load_face(wL, wR, _wL, _wR, _bx, i);
+    or_prim2cons(uL, wL, gam);
+    or_prim2cons(uR, wR, gam);
+    or_primcons2flux(fL, wL, uL, gam);
+    or_primcons2flux(fR, wR, uR, gam);
+    double sL = 0., sR = 0.;
+    sLsR_simple_hydro(&sL, &sR, wL, wR, gam);
+    double aL = fmin(sL, 0.);
+    double aR = fmax(sR, 0.);
+    double aLaR = aL * aR;
+    double aRmaLi = 1. / (aR - aL);
+    for (int n = 0; n < NMODE; ++n)
+      _flux[n][i] = (aR * fL[n] - aL * fR[n] + aLaR * (uR[n] - uL[n])) * aRmaLi;
+    _flux[BX][i] = 0.;
+  }
+}
+
+/* hllc_flux, src/mhd/fluxes_c.c:230-327 (hydro build, no TWOTEMP): Toro's
+   HLLC in the form F* = z (F - s U) + s/(s - sM) p* (0, 1, 0, 0, sM), with
+   z = -sM/(s - sM) on the side sM points away from and the outer speeds
+   clipped at 0 so the supersonic cases take F_L or F_R.  9-slot rows with
+   zero field, as for or_hll_flux_hydro. */
+void or_hllc_flux(double **_flux, double **_wL, double **_wR, double *_bx, int i1, int i2,
+                  double gam)
+{
+  double wL[NMODE], wR[NMODE], uL[NMODE], uR[NMODE], fL[NMODE], fR[NMODE];
+  for (int i = i1; i <= i2; ++i) {
+    load_face(wL, wR, _wL, _wR, _bx, i);
+    double sL = 0., sR = 0.;
+    or_prim2cons(uL, wL, gam);
+    or_prim2cons(uR, wR, gam);
+    sLsR_pressure(&sL, &sR, wL, wR, gam);
+    double ptL = wL[PR];
+    double ptR = wR[PR];
+    double vLsL = wL[VX] - sL;
+    double vRsR = wR[VX] - sR;
+    double xL = uL[RHO] * vLsL;
+    double xR = uR[RHO] * vRsR;
+    double xRxLi = 1. / (xR - xL);
+    double yL = ptL + uL[MX] * vLsL;
+    double yR = ptR + uR[MX] * vRsR;
+    double sM = xRxLi * (yR - yL);
+    double pts = xRxLi * (xR * yL - xL * yR);
+    sL = fmin(sL, 0.);
+    sR = fmax(sR, 0.);
+    vLsL = wL[VX] - sL;
+    vRsR = wR[VX] - sR;
+    for (int n = 0; n < NMODE; ++n) {
+      fL[n] = uL[n] * vLsL;
+      fR[n] = uR[n] * vRsR;
+    }
+    fL[MX] += ptL;
+    fR[MX] += ptR;
+    fL[EN] += ptL * wL[VX];
+    fR[EN] += ptR * wR[VX];
+    double sLsMi = 1. / (sL - sM);
+    double sRsMi = 1. / (sR - sM);
+    double zL, zR, sLR;
+    if (sM > 0.) {
+      zL = -sM * sLsMi;
+      zR = 0.;
+      sLR = sL * sLsMi;
+    } else {
+      zL = 0.;
+      zR = -sM * sRsMi;
+      sLR = sR * sRsMi;
+    }
+    for (int n = 0; n < NMODE; n++) _flux[n][i] = zL * fL[n] + zR * fR[n];
+    _flux[MX][i] += sLR * pts;
+    _flux[EN][i] += sLR * pts * sM;
     _flux[BX][i] = 0.;
   }
 }
@@ -579,7 +708,7 @@ typedef void (*rsolver_fn)(double **, double **, double **, double *, int, int, 
 typedef void (*reconstr_fn)(double **, double **, double ***, double ***, int, int, int);
 
 struct or_sim {
-  int dims, ng, reconstr_order, nthreads, dt_fms_ref, char_proj;
+  int dims, ng, reconstr_order, nthreads, dt_fms_ref, char_proj, hydro, ct_upwind;
   int Nact[3], Ntot[3];
   int i1, i2, j1, j2, k1, k2;
   double lmin[3], lmax[3];
@@ -635,6 +764,30 @@ void or_history(or_sim *s, double *out)
 }
 void or_set_dt_fms_ref(or_sim *s, int on) { s->dt_fms_ref = on; }
 void or_set_char_proj(or_sim *s, int on) { s->char_proj = on; }
+/* hydro build (MFIELD=0): the sound speed in the wave estimates and the CFL
+   dt, HLL or HLLC; the field arrays stay in place and must hold zeros (CT
+   then keeps them exactly zero) */
+void or_set_hydro(or_sim *s, int on) { s->hydro = on; }
+/* corner EMF: 0 interp_ee1 (the reference's), 1 the upwind interp_ee2 */
+void or_set_ct_upwind(or_sim *s, int on) { s->ct_upwind = on; }
+/* For the oracle's own tests: the face-flux arrays [9][cells] and the edge
+   EMFs [3][cells], and one corner-EMF evaluation (ec_from_prim, then
+   interp_ee1 or interp_ee2) over the corrector's limits from the current
+   prim and the flux arrays as they stand. */
+static void ec_from_prim(or_sim *s, const double *w, const int *lims);
+static void interp_ee1(or_sim *s, const int *lims);
+static void interp_ee2(or_sim *s, const int *lims, const double *w, double dt);
+static void integr_lims(const or_sim *s, int order, int *l);
+double *or_flux(or_sim *s, int axis) { return s->flx[axis]; }
+double *or_edge_emf(or_sim *s) { return s->ee; }
+void or_corner_emf(or_sim *s, int upwind, double dt)
+{
+  int lims[6];
+  integr_lims(s, s->reconstr_order, lims);
+  ec_from_prim(s, s->prim, lims);
+  if (upwind) interp_ee2(s, lims, s->prim, dt);
+  else interp_ee1(s, lims);
+}
 
 double *or_coord(or_sim *s, int which, int axis)
 {
@@ -788,8 +941,10 @@ static void swap_mhd_vec(int ax, int *vars)
   }
 }
 
-static rsolver_fn pick_rsolver(int id)
+/* integrator.pyx:253-268: the solver the build binds to the selection */
+static rsolver_fn pick_rsolver(int id, int hydro)
 {
+  if (hydro) return id == OR_RS_HLLC ? or_hllc_flux : or_hll_flux_hydro;
   switch (id) {
   case OR_RS_HLLD: return or_hlld_flux;
   case OR_RS_HLLT: return or_hllt_flux;
@@ -827,7 +982,7 @@ static void godunov_fluxes(or_sim *s, const double *prim, const double *bfld, co
   swap_mhd_vec(0, varsx);
   swap_mhd_vec(1, varsy);
   swap_mhd_vec(2, varsz);
-  rsolver_fn rs = pick_rsolver(s->rsolver);
+  rsolver_fn rs = pick_rsolver(s->rsolver, s->hydro);
   int nx = s->Ntot[0];
   int len = nx + 4;
   int nst = 2 * order + 1;
@@ -1019,6 +1174,111 @@ static void interp_ee1(or_sim *s, const int *lims)
       A(ee, 2, 0, 0, i) = -A(fx, BY, 0, 0, i);
     }
   }
+}
+
+/* get_ct_weight: src/integration/ct.pyx:221-226.  Weight of the lower
+   (left) cell's EMF slope: 1/2 + clip(1e3 frho dt / (min(rho) ds), +-1/2),
+   a smoothed sign of the mass flux through the face. */
+static inline double ct_weight(double frho, double rho_l, double rho_r, double ds, double dt)
+{
+  double v_c = 1e3 * frho * dt / (fmin(rho_l, rho_r) * ds);
+  return 0.5 + fmax(-0.5, fmin(0.5, v_c));
+}
+
+/* interp_ee2, the upwind corner EMF (Gardiner & Stone 2005 with a smoothed
+   upwind weight): src/integration/ct.pyx:229-358, which the reference never
+   calls (integrator.pyx:106-108 comments it out).  Followed in 3D with two
+   cell-EMF indices corrected: in the Ex slopes dedz_34 and dedy_34 the
+   reference reads ec[0,k,j-1,i] where the cyclic image of its own Ez and Ey
+   formulas reads ec[0,k-1,j-1,i]; with the correction the three components
+   are cyclic images of each other and the scheme reduces to interp_ee1 when
+   every weight is 1/2 (no mass flux), which fails with the indices as
+   written.  In 2D and 1D the reference's loops are empty (their k range is
+   range(0, 0)); the 2D Ez formula and the Ex/Ey branches it intends are
+   used.  rho is the stage's density, dt the full step (self.dt). */
+static void interp_ee2(or_sim *s, const int *lims, const double *w, double dt)
+{
+  int il = lims[0], iu = lims[1] + 1, jl = lims[2], ju = lims[3] + 1;
+  int kl = lims[4], ku = lims[5] + 1;
+  if (s->dims < 2) jl = ju = 0;
+  if (s->dims < 3) kl = ku = 0;
+  double *ee = s->ee, *ec = s->ec, *fx = s->flx[0], *fy = s->flx[1], *fz = s->flx[2];
+  double **dlf = s->dlf, **dlv = s->dlv;
+  for (int k = kl; k <= ku; ++k)
+    for (int j = jl; j <= ju; ++j)
+      for (int i = il; i <= iu; ++i) {
+        if (s->dims == 1) {
+          A(ee, 0, 0, 0, i) = A(ec, 0, 0, 0, i);
+          A(ee, 1, 0, 0, i) = A(fx, BZ, 0, 0, i);
+          A(ee, 2, 0, 0, i) = -A(fx, BY, 0, 0, i);
+          continue;
+        }
+        /* ---- Ez at (i-1/2, j-1/2) */
+        {
+          double dsx = dlf[0][i];
+          double wgtx = ct_weight(A(fx, RHO, k, j, i), A(w, RHO, k, j, i - 1),
+                                  A(w, RHO, k, j, i), dsx, dt);
+          double dedy_14 = (wgtx * (A(ec, 2, k, j, i - 1) - A(fy, BX, k, j, i - 1)) +
+                            (1. - wgtx) * (A(ec, 2, k, j, i) - A(fy, BX, k, j, i)));
+          double dedy_34 = (wgtx * (A(fy, BX, k, j, i - 1) - A(ec, 2, k, j - 1, i - 1)) +
+                            (1. - wgtx) * (A(fy, BX, k, j, i) - A(ec, 2, k, j - 1, i)));
+          double dsy = dlv[1][j];
+          double wgty = ct_weight(A(fy, RHO, k, j, i), A(w, RHO, k, j - 1, i),
+                                  A(w, RHO, k, j, i), dsy, dt);
+          double dedx_14 = (wgty * (A(ec, 2, k, j - 1, i) + A(fx, BY, k, j - 1, i)) +
+                            (1. - wgty) * (A(ec, 2, k, j, i) + A(fx, BY, k, j, i)));
+          double dedx_34 = (wgty * (-A(fx, BY, k, j - 1, i) - A(ec, 2, k, j - 1, i - 1)) +
+                            (1. - wgty) * (-A(fx, BY, k, j, i) - A(ec, 2, k, j, i - 1)));
+          A(ee, 2, k, j, i) = 0.25 * (-A(fx, BY, k, j - 1, i) - A(fx, BY, k, j, i) +
+                                      A(fy, BX, k, j, i - 1) + A(fy, BX, k, j, i) + dedy_34 -
+                                      dedy_14 + dedx_34 - dedx_14);
+        }
+        if (s->dims == 2) {
+          A(ee, 0, k, j, i) = -A(fy, BZ, k, j, i);
+          A(ee, 1, k, j, i) = A(fx, BZ, k, j, i);
+          continue;
+        }
+        /* ---- Ex at (j-1/2, k-1/2) */
+        {
+          double dsy = dlf[1][j];
+          double wgty = ct_weight(A(fy, RHO, k, j, i), A(w, RHO, k, j - 1, i),
+                                  A(w, RHO, k, j, i), dsy, dt);
+          double dedz_14 = (wgty * (A(ec, 0, k, j - 1, i) - A(fz, BY, k, j - 1, i)) +
+                            (1. - wgty) * (A(ec, 0, k, j, i) - A(fz, BY, k, j, i)));
+          double dedz_34 = (wgty * (A(fz, BY, k, j - 1, i) - A(ec, 0, k - 1, j - 1, i)) +
+                            (1. - wgty) * (A(fz, BY, k, j, i) - A(ec, 0, k - 1, j, i)));
+          double dsz = dlv[2][k];
+          double wgtz = ct_weight(A(fz, RHO, k, j, i), A(w, RHO, k - 1, j, i),
+                                  A(w, RHO, k, j, i), dsz, dt);
+          double dedy_14 = (wgtz * (A(ec, 0, k - 1, j, i) + A(fy, BZ, k - 1, j, i)) +
+                            (1. - wgtz) * (A(ec, 0, k, j, i) + A(fy, BZ, k, j, i)));
+          double dedy_34 = (wgtz * (-A(fy, BZ, k - 1, j, i) - A(ec, 0, k - 1, j - 1, i)) +
+                            (1. - wgtz) * (-A(fy, BZ, k, j, i) - A(ec, 0, k, j - 1, i)));
+          A(ee, 0, k, j, i) = 0.25 * (-A(fy, BZ, k - 1, j, i) - A(fy, BZ, k, j, i) +
+                                      A(fz, BY, k, j - 1, i) + A(fz, BY, k, j, i) + dedz_34 -
+                                      dedz_14 + dedy_34 - dedy_14);
+        }
+        /* ---- Ey at (i-1/2, k-1/2) */
+        {
+          double dsz = dlf[2][k];
+          double wgtz = ct_weight(A(fz, RHO, k, j, i), A(w, RHO, k - 1, j, i),
+                                  A(w, RHO, k, j, i), dsz, dt);
+          double dedx_14 = (wgtz * (A(ec, 1, k - 1, j, i) - A(fx, BZ, k - 1, j, i)) +
+                            (1. - wgtz) * (A(ec, 1, k, j, i) - A(fx, BZ, k, j, i)));
+          double dedx_34 = (wgtz * (A(fx, BZ, k - 1, j, i) - A(ec, 1, k - 1, j, i - 1)) +
+                            (1. - wgtz) * (A(fx, BZ, k, j, i) - A(ec, 1, k, j, i - 1)));
+          double dsx = dlv[0][i];
+          double wgtx = ct_weight(A(fx, RHO, k, j, i), A(w, RHO, k, j, i - 1),
+                                  A(w, RHO, k, j, i), dsx, dt);
+          double dedz_14 = (wgtx * (A(ec, 1, k, j, i - 1) + A(fz, BX, k, j, i - 1)) +
+                            (1. - wgtx) * (A(ec, 1, k, j, i) + A(fz, BX, k, j, i)));
+          double dedz_34 = (wgtx * (-A(fz, BX, k, j, i - 1) - A(ec, 1, k - 1, j, i - 1)) +
+                            (1. - wgtx) * (-A(fz, BX, k, j, i) - A(ec, 1, k - 1, j, i)));
+          A(ee, 1, k, j, i) = 0.25 * (-A(fz, BX, k, j, i - 1) - A(fz, BX, k, j, i) +
+                                      A(fx, BZ, k - 1, j, i) + A(fx, BZ, k, j, i) + dedx_34 -
+                                      dedx_14 + dedz_34 - dedz_14);
+        }
+      }
 }
 
 /* advance_b: src/integration/ct.pyx:15-91 (D11: By over j in [jl,ju+1];
@@ -1260,14 +1520,15 @@ double or_new_dt(or_sim *s)
           py = wy;
           pz = wz;
         }
-        double vfx = or_fms(w, bx, s->gam);
+        /* the hydro build's fms is the sound speed (eos_c.h:523-527) */
+        double vfx = s->hydro ? or_fms_hydro(w, s->gam) : or_fms(w, bx, s->gam);
         cx = fabs(w[VX]) + vfx;
         if (s->dims >= 2) {
-          double vfy = or_fms(py, by, s->gam);
+          double vfy = s->hydro ? vfx : or_fms(py, by, s->gam);
           cy = fabs(w[VY]) + vfy;
         }
         if (s->dims >= 3) {
-          double vfz = or_fms(pz, bz, s->gam);
+          double vfz = s->hydro ? vfx : or_fms(pz, bz, s->gam);
           cz = fabs(w[VZ]) + vfz;
         }
         double dsx = s->dlf[0][i], dsy = s->dlf[1][j], dsz = s->dlf[2][k];
@@ -1293,7 +1554,8 @@ void or_vl_step(or_sim *s, double dt_full)
   godunov_fluxes(s, w, b, lims, order);
   advance_hydro(s, us, u, lims, dt);
   ec_from_prim(s, w, lims);
-  interp_ee1(s, lims);
+  if (s->ct_upwind) interp_ee2(s, lims, w, dt_full);
+  else interp_ee1(s, lims);
   advance_b(s, bs, b, lims, dt);
   interp_bc(s, us, bs, lims);
   cons2prim_lims(s, ws, us, lims);
@@ -1305,7 +1567,8 @@ void or_vl_step(or_sim *s, double dt_full)
   godunov_fluxes(s, ws, bs, lims, order);
   advance_hydro(s, u, u, lims, dt);
   ec_from_prim(s, ws, lims);
-  interp_ee1(s, lims);
+  if (s->ct_upwind) interp_ee2(s, lims, ws, dt_full);
+  else interp_ee1(s, lims);
   advance_b(s, b, b, lims, dt);
   interp_bc(s, u, b, lims);
   cons2prim_lims(s, w, u, lims);

@@ -32,6 +32,7 @@ void   or_cons2prim(double *w, const double *u, double gam);
 void   or_prim2cons(double *u, const double *w, double gam);
 void   or_primcons2flux(double *f, const double *w, const double *u, double gam);
 double or_fms(const double *w, double bx, double gam);
+double or_fms_hydro(const double *w, double gam);
 double or_vl_lim(double a, double b);
 /* characteristic projection (src/mhd/eos_c.h:100-280): in place on vc, in the
    eigensystem of state w */
@@ -42,6 +43,10 @@ void   or_char2prim(double *vc, const double *w, double gam);
 void or_hll_flux (double **flux, double **wl, double **wr, double *bx, int i1, int i2, double gam);
 void or_hllt_flux(double **flux, double **wl, double **wr, double *bx, int i1, int i2, double gam);
 void or_hlld_flux(double **flux, double **wl, double **wr, double *bx, int i1, int i2, double gam);
+/* hydro build (MFIELD=0), 9-slot rows with zero field */
+void or_hll_flux_hydro(double **flux, double **wl, double **wr, double *bx, int i1, int i2,
+                       double gam);
+void or_hllc_flux(double **flux, double **wl, double **wr, double *bx, int i1, int i2, double gam);
 
 /* ---- reconstruction (src/coordinates/reconstr_c.h:4-12), uniform grids ---- */
 void or_reconstr_const (double **wl, double **wr, double ***w, double ***scr, int ax, int i1, int i2);
@@ -67,6 +72,11 @@ double *or_coord(or_sim *s, int which, int axis); /* 0 lf,1 lv,2 dlf,3 dlf_inv *
 void    or_set_dt_fms_ref(or_sim *s, int on);
 /* PLM in characteristic variables (charact_proj=1, src/coordinates/reconstr_c.c:64-80) */
 void    or_set_char_proj(or_sim *s, int on);
+void    or_set_hydro(or_sim *s, int on);
+void    or_set_ct_upwind(or_sim *s, int on);
+double *or_flux(or_sim *s, int axis);   /* [9][cells] face fluxes (tests)  */
+double *or_edge_emf(or_sim *s);         /* [3][cells] edge EMFs (tests)    */
+void    or_corner_emf(or_sim *s, int upwind, double dt);
 void    or_apply_grid_bc(or_sim *s);
 void    or_init(or_sim *s);
 double  or_new_dt(or_sim *s);

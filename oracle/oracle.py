@@ -85,7 +85,18 @@ def lib(path=None):
         L.or_fms.argtypes = [_dp, ctypes.c_double, ctypes.c_double]
         L.or_vl_lim.restype = ctypes.c_double
         L.or_vl_lim.argtypes = [ctypes.c_double, ctypes.c_double]
-        for name in ("or_hll_flux", "or_hllt_flux", "or_hlld_flux"):
+        L.or_fms_hydro.restype = ctypes.c_double
+        L.or_fms_hydro.argtypes = [_dp, ctypes.c_double]
+        L.or_set_hydro.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.or_set_ct_upwind.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        for name in ("or_flux",):
+            getattr(L, name).restype = _dp
+            getattr(L, name).argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.or_edge_emf.restype = _dp
+        L.or_edge_emf.argtypes = [ctypes.c_void_p]
+        L.or_corner_emf.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
+        for name in ("or_hll_flux", "or_hllt_flux", "or_hlld_flux", "or_hll_flux_hydro",
+                     "or_hllc_flux"):
             getattr(L, name).argtypes = [_dpp, _dpp, _dpp, _dp, ctypes.c_int, ctypes.c_int,
                                          ctypes.c_double]
         for name in ("or_reconstr_const", "or_reconstr_linear"):
@@ -165,7 +176,8 @@ def riemann(rsolver, wl, wr, bx, gam):
     bx = np.ascontiguousarray(bx, dtype=np.float64)
     n = wl.shape[1]
     flux = np.full((9, n), np.nan)
-    fn = {"hll": lib().or_hll_flux, "hlld": lib().or_hlld_flux, "hllt": lib().or_hllt_flux}[rsolver]
+    fn = {"hll": lib().or_hll_flux, "hlld": lib().or_hlld_flux, "hllt": lib().or_hllt_flux,
+          "hll_hydro": lib().or_hll_flux_hydro, "hllc": lib().or_hllc_flux}[rsolver]
     fn(_rows(flux), _rows(wl), _rows(wr), _ptr(bx), 0, n - 1, gam)
     return flux
 
@@ -203,7 +215,7 @@ class OracleSim:
     def __init__(self, nx, ny=1, nz=1, reconstr_order=1, lmin=(0., 0., 0.),
                  lmax=(1., 1., 1.), gam=5. / 3., cour=0.4, rsolver="hlld",
                  bc=(0, 0, 0, 0, 0, 0), nthreads=None, dt_fms_ref=True, lib_path=None,
-                 char_proj=False):
+                 char_proj=False, hydro=False, ct_upwind=False):
         L = self._L = lib(lib_path)
         lmin = np.asarray(lmin, dtype=np.float64)
         lmax = np.asarray(lmax, dtype=np.float64)
@@ -213,6 +225,8 @@ class OracleSim:
                               RS_IDS[rsolver], bcs, nthreads)
         L.or_set_dt_fms_ref(self._h, int(bool(dt_fms_ref)))
         L.or_set_char_proj(self._h, int(bool(char_proj)))
+        L.or_set_hydro(self._h, int(bool(hydro)))
+        L.or_set_ct_upwind(self._h, int(bool(ct_upwind)))
         self.ntot = tuple(L.or_ntot(self._h, a) for a in range(3))
         self.ng = L.or_ng(self._h)
         nc = self.ntot[0] * self.ntot[1] * self.ntot[2]
@@ -224,6 +238,21 @@ class OracleSim:
     def coord(self, which, axis):
         n = self.ntot[axis] + (1 if which == 0 else 0)
         return np.ctypeslib.as_array(self._L.or_coord(self._h, which, axis), shape=(n,)).copy()
+
+    def flux(self, axis):
+        """The face-flux array of one axis, [9][k][j][i] (a view)."""
+        nc = self.ntot[0] * self.ntot[1] * self.ntot[2]
+        shp = (9, self.ntot[2], self.ntot[1], self.ntot[0])
+        return np.ctypeslib.as_array(self._L.or_flux(self._h, axis), shape=(9 * nc,)).reshape(shp)
+
+    def corner_emf(self, upwind, dt):
+        """One corner-EMF evaluation from the current prim and flux arrays;
+        returns a copy of the edge EMFs [3][k][j][i]."""
+        self._L.or_corner_emf(self._h, int(bool(upwind)), dt)
+        nc = self.ntot[0] * self.ntot[1] * self.ntot[2]
+        shp = (3, self.ntot[2], self.ntot[1], self.ntot[0])
+        return np.ctypeslib.as_array(self._L.or_edge_emf(self._h), shape=(3 * nc,)).reshape(
+            shp).copy()
 
     def init(self):
         self._L.or_init(self._h)

@@ -308,3 +308,49 @@ def test_noalloc_seam_rejects_bad_calls_without_touching_gpu():
     assert L.bnz_rsolver_rows_noalloc(_lib.RS["hll"], *rows, bxp, 0, 10, 1.0) == \
         _lib.BNZ_EINVAL
     assert all(np.all(b == 7.25) for b in bufs)
+
+
+def test_hydro_build_selection_follows_reference():
+    """mfield=False is the reference's hydro build: hll and hllc are bound,
+    hlld and hlla are not (integrator.pyx:253-268); hllt and characteristic
+    limiting of the hydro build are not built here and raise."""
+    base = {"computation": {"cour": 0.4, "rsolver": "hllc"}, "physics": {"gam": 1.4}}
+    it = BnzIntegr(_Params(config=base), mfield=False)
+    assert it.rsolver == RSolver.RS_HLLC and not it.mfield
+    with pytest.raises(ValueError):
+        BnzIntegr(_Params(config=base))            # hllc with the field: NULL in the reference
+    for r, exc in (("hlld", ValueError), ("hlla", ValueError), ("hllt", NotImplementedError)):
+        cfg = {"computation": {"cour": 0.4, "rsolver": r}, "physics": {"gam": 1.4}}
+        with pytest.raises(exc):
+            BnzIntegr(_Params(config=cfg), mfield=False)
+    cfg = {"computation": {"cour": 0.4, "rsolver": "hll", "charact_proj": 1},
+           "physics": {"gam": 1.4}}
+    with pytest.raises(NotImplementedError):
+        BnzIntegr(_Params(config=cfg), mfield=False)
+
+
+def test_create_validates_the_hydro_build_without_touching_gpu():
+    L = _lib.load()
+    h = ctypes.c_void_p()
+    bad = [dict(rsolver="hlld", hydro=True), dict(rsolver="hllt", hydro=True),
+           dict(rsolver="hllc", hydro=True, precision="fp32c"),
+           dict(rsolver="hllc", hydro=True, char_proj=True)]
+    for kw in bad:
+        p = _lib.make_params(nx=16, ny=16, **kw)
+        assert L.bnz_create(ctypes.byref(p), ctypes.byref(h)) == _lib.BNZ_EUNSUPPORTED, kw
+        L.bnz_destroy(h)
+    p = _lib.make_params(nx=16, ny=16, rsolver="hllc")
+    assert L.bnz_create(ctypes.byref(p), ctypes.byref(h)) == _lib.BNZ_EUNSUPPORTED
+    assert b"hydro" in L.bnz_last_error(h)
+    L.bnz_destroy(h)
+    p = _lib.make_params(nx=16, ny=16, rsolver="hll")
+    p.hydro = 2
+    assert L.bnz_create(ctypes.byref(p), ctypes.byref(h)) == _lib.BNZ_EINVAL
+    L.bnz_destroy(h)
+    boxes = (_lib.BnzBox * 1)()
+    boxes[0].nx, boxes[0].ny, boxes[0].nz, boxes[0].tmax = 16, 1, 1, 0.1
+    for a in range(3):
+        boxes[0].lmax[a] = 1.0
+    p = _lib.make_params(rsolver="hll", hydro=True)
+    assert L.bnz_batch_create(ctypes.byref(p), boxes, 1, ctypes.byref(h)) == _lib.BNZ_EUNSUPPORTED
+    L.bnz_batch_destroy(h)

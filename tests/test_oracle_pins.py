@@ -430,13 +430,13 @@ def test_linear_fast_wave_second_order():
 
 
 # ------------------------------------------------- multi-D constrained transport
-def wave_error(dims, n, wave, t, lib_path=None):
+def wave_error(dims, n, wave, t, lib_path=None, ct_upwind=False):
     """L1 error norm sqrt(sum_v mean|q_v - exact_v|^2) over rho, v, p and the
     three face fields of an oblique linear wave run to time t on the oracle,
     against the exact linear solution at t (problems.linear_wave_oblique)."""
     s, prim, bfld, _ = P.linear_wave_oblique(n, wave, dims, amp=1e-6)
     _, pe, be, _ = P.linear_wave_oblique(n, wave, dims, amp=1e-6, t=t)
-    o = orc.OracleSim(s.nx, s.ny, s.nz, cour=0.4, lib_path=lib_path)
+    o = orc.OracleSim(s.nx, s.ny, s.nz, cour=0.4, lib_path=lib_path, ct_upwind=ct_upwind)
     o.prim[:] = prim
     o.bfld[:] = bfld
     o.init()
@@ -507,7 +507,7 @@ def place_tube(prob1, dims, axis, transverse=4):
     return s, prim, bfld, tuple(bc), ax3
 
 
-def planar_deviation(dims, axis, steps=20, lib_path=None):
+def planar_deviation(dims, axis, steps=20, lib_path=None, ct_upwind=False):
     """Largest deviation, relative to the largest magnitude of each variable,
     between a seeded 1D MHD Riemann tube stepped on a 1D grid and the same
     tube laid along ``axis`` of a 2D or 3D grid, stepped with the same fixed
@@ -526,7 +526,7 @@ def planar_deviation(dims, axis, steps=20, lib_path=None):
     for _ in range(steps):
         o1.vl_step(dt)
     s, prim, bfld, bc, ax3 = place_tube(prob1, dims, axis)
-    o = orc.OracleSim(s.nx, s.ny, s.nz, bc=bc, lib_path=lib_path)
+    o = orc.OracleSim(s.nx, s.ny, s.nz, bc=bc, lib_path=lib_path, ct_upwind=ct_upwind)
     o.prim[:] = prim
     o.bfld[:] = bfld
     o.init()
@@ -869,3 +869,243 @@ def test_char_plm_changes_a_shock_tube_and_keeps_invariants():
     tot1 = [o0.cons[v][sk, sj, si].sum() for v in range(6)]
     for a, b in zip(tot0, tot1):
         assert abs(a - b) < 1e-10 * max(1.0, abs(a))
+
+
+# ------------------------------------------------------------------ hydro build
+# The hydro build (MFIELD=0) in the 9-slot layout with zero field: the sound
+# speed in the wave estimates and the dt, HLL or HLLC.
+
+def test_exact_euler_solver_reproduces_toro_table():
+    """The exact solver used below against Toro's table 4.3 (gamma 1.4):
+    star pressure, velocity and left star density of tests 1-4."""
+    cases = [((1.0, 0.0, 1.0), (0.125, 0.0, 0.1), (0.30313, 0.92745, 0.42632)),
+             ((1.0, -2.0, 0.4), (1.0, 2.0, 0.4), (0.00189, 0.0, 0.02185)),
+             ((1.0, 0.0, 1000.0), (1.0, 0.0, 0.01), (460.894, 19.5975, 0.57506)),
+             ((1.0, 0.0, 0.01), (1.0, 0.0, 100.0), (46.0950, -6.19633, 5.99242))]
+    for left, right, (ps, us, rls) in cases:
+        # just left of the contact: the left star state
+        r, u, p = M.exact_riemann_euler(left, right, 1.4,
+                                        np.array([us - 1e-3 * max(1.0, abs(us))]))[:, 0]
+        assert abs(p - ps) <= 5e-5 * max(ps, 1e-3) + 5e-6
+        assert abs(u - us) <= 5e-5 * max(abs(us), 1.0)
+        assert abs(r - rls) <= 5e-5 * max(rls, 1e-2) + 5e-6
+
+
+def test_hydro_sound_speed():
+    rng = np.random.default_rng(40)
+    for _ in range(50):
+        w = np.zeros(9)
+        w[0], w[4] = rng.uniform(0.1, 3.0), rng.uniform(0.01, 3.0)
+        w[1:4] = rng.uniform(-1, 1, 3)
+        assert orc.lib().or_fms_hydro(orc._ptr(w), GAM) == pytest.approx(
+            M.sound_speed(w, GAM), rel=1e-15)
+        # the MHD fast speed at zero field is the same speed (to rounding)
+        assert orc.fms(w, GAM) == pytest.approx(M.sound_speed(w, GAM), rel=4e-16)
+
+
+def test_hllc_matches_toro():
+    """HLLC in the reference's form (F - sU with p*, fluxes_c.c:230-327)
+    against Toro's star-state form (sec. 10.4), with PVRS speeds."""
+    rng = np.random.default_rng(41)
+    wl, wr, bx = M.random_hydro_states(rng, 600)
+    f = orc.riemann("hllc", wl, wr, bx, GAM)
+    for i in range(600):
+        want = M.hllc_toro(wl[:, i], wr[:, i], GAM)
+        assert rel(f[:, i], want) < 1e-12, i
+
+
+def test_hll_hydro_matches_textbook():
+    rng = np.random.default_rng(42)
+    wl, wr, bx = M.random_hydro_states(rng, 400)
+    f = orc.riemann("hll_hydro", wl, wr, bx, GAM)
+    for i in range(400):
+        assert rel(f[:, i], M.hll_hydro(wl[:, i], wr[:, i], GAM)) < 1e-12
+
+
+@pytest.mark.parametrize("solver", ["hllc", "hll_hydro"])
+def test_hydro_solvers_consistency_and_upwinding(solver):
+    """F(U, U) = F(U); supersonic states give F_L / F_R to a few ulp (HLLC
+    multiplies by -sM / (0 - sM) and HLL by aR / aR, each a rounding away
+    from 1); field slots zero."""
+    rng = np.random.default_rng(43)
+    wl, wr, bx = M.random_hydro_states(rng, 300)
+    f = orc.riemann(solver, wl, wl, bx, GAM)
+    for i in range(300):
+        assert rel(f[:, i], M.phys_flux(wl[:, i], GAM)) < 1e-12
+    assert np.all(f[6:9] == 0.0)
+    wl[1] += 20.0
+    wr[1] += 20.0
+    f = orc.riemann(solver, wl, wr, bx, GAM)
+    for i in range(300):
+        assert rel(f[:, i], M.phys_flux(wl[:, i], GAM), scale=np.abs(f[:, i]).max()) < 2e-15
+    wl[1] -= 40.0
+    wr[1] -= 40.0
+    f = orc.riemann(solver, wl, wr, bx, GAM)
+    for i in range(300):
+        assert rel(f[:, i], M.phys_flux(wr[:, i], GAM), scale=np.abs(f[:, i]).max()) < 2e-15
+
+
+def test_hllc_resolves_isolated_contact_hll_does_not():
+    """A contact (equal p and normal velocity, jumps in rho, tangential
+    velocity and scalar) is HLLC's defining case: its flux is the upwind
+    physical flux (moving contact) or carries no mass (stationary contact).
+    HLL smears it, so the check separates the two solvers."""
+    rng = np.random.default_rng(44)
+    n = 200
+    wl, wr, bx = M.random_hydro_states(rng, n)
+    wr[1] = wl[1]
+    wr[4] = wl[4]
+    f = orc.riemann("hllc", wl, wr, bx, GAM)
+    fh = orc.riemann("hll_hydro", wl, wr, bx, GAM)
+    hll_off = 0.0
+    for i in range(n):
+        up = wl[:, i] if wl[1, i] > 0 else wr[:, i]
+        want = M.phys_flux(up, GAM)
+        assert rel(f[:, i], want, scale=np.abs(want).max()) < 1e-13
+        hll_off = max(hll_off, rel(fh[:, i], want, scale=np.abs(want).max()))
+    assert hll_off > 1e-3
+    wl[1] = 0.0
+    wr[1] = 0.0
+    f = orc.riemann("hllc", wl, wr, bx, GAM)
+    assert np.max(np.abs(f[0])) < 1e-15 and np.max(np.abs(f[5])) < 1e-15
+    assert np.allclose(f[1], wl[4], rtol=1e-14, atol=0)
+
+
+def _sod_l1(n, rsolver, reconstr_order=1, axis=0, transverse=0, vel=0.0):
+    s, prim, bfld = P.sod(n, axis=axis, transverse=transverse, vel=vel)
+    bc = [0] * 6
+    bc[2 * axis] = bc[2 * axis + 1] = 1
+    o = orc.OracleSim(s.nx, s.ny, s.nz, rsolver=rsolver, bc=tuple(bc), hydro=True, gam=1.4,
+                      reconstr_order=reconstr_order)
+    if reconstr_order == 0:
+        s = P.GridShape(s.nx, s.ny, s.nz, 2, s.lmin, s.lmax)
+        p2, b2 = s.empty()
+        p2[(slice(None),) + s.active()] = P.sod(n, axis=axis, transverse=transverse,
+                                                vel=vel)[1][(slice(None),) +
+                                                            P.GridShape(s.nx, s.ny, s.nz, 3).active()]
+        prim, bfld = p2, b2
+    o.prim[:] = prim
+    o.bfld[:] = bfld
+    o.init()
+    o.advance(0.15)
+    x = s.centres(axis)
+    ex = M.exact_riemann_euler((1.0, vel, 1.0), (0.125, vel, 0.1), 1.4, (x - 0.5) / o.time)
+    act = s.active()
+    got = [o.prim[v][act] for v in (0, 1 + axis, 4)]
+    shape = [1, 1, 1]
+    shape[2 - axis] = n
+    err = [float(np.mean(np.abs(g - e.reshape(shape)))) for g, e in zip(got, ex)]
+    return err, o
+
+
+@pytest.mark.parametrize("rsolver", ["hllc", "hll"])
+def test_sod_tube_converges_to_the_exact_solution(rsolver):
+    """Hydro build, Sod's tube with PLM: the L1 errors of rho, u and p
+    against the exact solution (Toro ch. 4) are small and fall with
+    resolution, the field stays exactly zero, and HLLC (which resolves the
+    contact) beats HLL."""
+    e1, o1 = _sod_l1(100, rsolver)
+    e2, o2 = _sod_l1(200, rsolver)
+    assert max(e1) < 0.02 and max(e2) < 0.012
+    assert all(b < 0.75 * a for a, b in zip(e1, e2))
+    assert np.all(o2.bfld == 0.0) and np.all(o2.prim[6:9] == 0.0)
+    if rsolver == "hllc":
+        eh, _ = _sod_l1(200, "hll")
+        assert e2[0] < eh[0]
+
+
+def test_hydro_galilean_shift_and_axis_symmetry():
+    """Sod in a frame moving at u=0.3 converges to the shifted exact solution
+    too, and the tube laid along y of a 2D grid gives the 1D profile."""
+    err, _ = _sod_l1(200, "hllc", vel=0.3)
+    assert max(err) < 0.012
+    e1d, o1 = _sod_l1(64, "hllc")
+    e2d, o2 = _sod_l1(64, "hllc", axis=1, transverse=4)
+    s1 = P.sod(64)[0]
+    s2 = P.sod(64, axis=1, transverse=4)[0]
+    a1 = o1.prim[0][s1.active()].ravel()
+    k, j, i = s2.active()
+    a2 = o2.prim[0][k, j, i][0, :, 0]
+    assert np.array_equal(a1, a2)
+
+
+
+# ------------------------------------------------------------------ upwind CT
+# interp_ee2 (ct.pyx:229-358), the upwind corner EMF the reference leaves
+# commented out (integrator.pyx:106-108); built with two Ex indices corrected.
+
+def _random_emf_inputs(dims, seed, frho):
+    """Random prim and face fluxes on a small grid, with the mass-flux slot of
+    every face set by ``frho`` (a number, or 'random')."""
+    rng = np.random.default_rng(seed)
+    n = 6
+    o = orc.OracleSim(n, n, n if dims == 3 else 1)
+    o.prim[:] = rng.uniform(-1, 1, o.prim.shape)
+    o.prim[0] = rng.uniform(0.5, 2.0, o.prim[0].shape)
+    for a in range(dims):
+        f = o.flux(a)
+        f[:] = rng.uniform(-1, 1, f.shape)
+        f[0] = rng.uniform(-1, 1, f[0].shape) if frho == "random" else frho
+    return o
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_upwind_emf_reduces_to_average_without_mass_flux(dims):
+    """With no mass flux through any face every upwind weight is 1/2, and the
+    upwind corner EMF equals the arithmetic one (interp_ee1) in exact
+    arithmetic: here to 1e-14 on random fluxes and cell EMFs.  With mass
+    fluxes it does not (the weights select a side)."""
+    o = _random_emf_inputs(dims, 70 + dims, 0.0)
+    e1 = o.corner_emf(False, 0.01)
+    e2 = o.corner_emf(True, 0.01)
+    sk = slice(2, -3) if dims == 3 else slice(0, 1)
+    sl = (slice(None), sk, slice(2, -3), slice(2, -3))
+    assert np.max(np.abs(e2[sl] - e1[sl])) < 1e-14 * max(1.0, np.max(np.abs(e1[sl])))
+    o.close()
+    o = _random_emf_inputs(dims, 80 + dims, "random")
+    e1 = o.corner_emf(False, 0.01)
+    e2 = o.corner_emf(True, 0.01)
+    assert np.max(np.abs(e2[sl] - e1[sl])) > 1e-3
+    o.close()
+
+
+def test_upwind_emf_reduction_fails_with_the_reference_indices():
+    """The two Ex indices corrected here matter: evaluating the reference's
+    Ex slopes as written (ec[0,k,j-1,i] in dedz_34 and dedy_34) on the same
+    zero-mass-flux inputs does not reproduce interp_ee1, so only the
+    corrected form has the reduction above."""
+    o = _random_emf_inputs(3, 73, 0.0)
+    e1 = o.corner_emf(False, 0.01)
+    ec = np.zeros((3,) + o.prim.shape[1:])
+    w = o.prim
+    ec[0] = -(w[2] * w[8] - w[3] * w[7])
+    fy, fz = o.flux(1), o.flux(2)
+    k, j, i = 3, 3, 3
+    BY, BZ = 7, 8
+    wgt = 0.5
+    dedz_14 = wgt * (ec[0, k, j - 1, i] - fz[BY, k, j - 1, i]) + wgt * (ec[0, k, j, i] - fz[BY, k, j, i])
+    dedz_34 = wgt * (fz[BY, k, j - 1, i] - ec[0, k, j - 1, i]) + wgt * (fz[BY, k, j, i] - ec[0, k - 1, j, i])
+    dedy_14 = wgt * (ec[0, k - 1, j, i] + fy[BZ, k - 1, j, i]) + wgt * (ec[0, k, j, i] + fy[BZ, k, j, i])
+    dedy_34 = wgt * (-fy[BZ, k - 1, j, i] - ec[0, k, j - 1, i]) + wgt * (-fy[BZ, k, j, i] - ec[0, k, j - 1, i])
+    as_written = 0.25 * (-fy[BZ, k - 1, j, i] - fy[BZ, k, j, i] + fz[BY, k, j - 1, i] + fz[BY, k, j, i]
+                         + dedz_34 - dedz_14 + dedy_34 - dedy_14)
+    assert abs(as_written - e1[0, k, j, i]) > 1e-3
+    o.close()
+
+
+@pytest.mark.parametrize("dims,axis", [(2, 0), (2, 1), (3, 0), (3, 1), (3, 2)])
+def test_upwind_emf_planar_tube_equals_1d(dims, axis):
+    """Planar data reduce the upwind corner EMF to the 1D edge EMF as well
+    (the design property of Gardiner & Stone's upwind CT)."""
+    assert planar_deviation(dims, axis, ct_upwind=True) <= 1e-13
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("dims,sizes", [(2, (32, 64, 128)), (3, (16, 32, 64))])
+@pytest.mark.parametrize("wave", ["fast", "alfven"])
+def test_upwind_emf_oblique_waves_second_order(dims, sizes, wave):
+    _, _, _, tfast = P.linear_wave_oblique(8, "fast", dims)
+    t = tfast if dims == 2 else 0.5 * tfast
+    errs = [wave_error(dims, n, wave, t, ct_upwind=True) for n in sizes]
+    assert errs[0] / errs[1] > 3.2 and errs[1] / errs[2] > 3.2, errs
+    assert errs[-1] < 1e-8
