@@ -41,7 +41,7 @@ class BnzParams(ctypes.Structure):
         ("lmin", ctypes.c_double * 3), ("lmax", ctypes.c_double * 3),
         ("gam", ctypes.c_double), ("cour", ctypes.c_double),
         ("dt_fms_ref", ctypes.c_int), ("precision", ctypes.c_int), ("device", ctypes.c_int),
-        ("char_proj", ctypes.c_int), ("hydro", ctypes.c_int),
+        ("char_proj", ctypes.c_int), ("hydro", ctypes.c_int), ("ct_upwind", ctypes.c_int),
     ]
 
 
@@ -147,7 +147,8 @@ def _ptr(a):
 
 def make_params(nx=1, ny=1, nz=1, rsolver="hlld", reconstr="linear", bc=(0,) * 6,
                 lmin=(0.0, 0.0, 0.0), lmax=(1.0, 1.0, 1.0), gam=5.0 / 3.0, cour=0.4,
-                dt_fms_ref=True, precision="fp64", device=0, char_proj=False, hydro=False):
+                dt_fms_ref=True, precision="fp64", device=0, char_proj=False, hydro=False,
+                ct_upwind=False):
     p = BnzParams()
     p.nx, p.ny, p.nz = nx, ny, nz
     p.rsolver = RS[rsolver] if isinstance(rsolver, str) else int(rsolver)
@@ -163,6 +164,7 @@ def make_params(nx=1, ny=1, nz=1, rsolver="hlld", reconstr="linear", bc=(0,) * 6
     p.device = device
     p.char_proj = int(bool(char_proj))
     p.hydro = int(bool(hydro))
+    p.ct_upwind = int(bool(ct_upwind))
     return p
 
 
@@ -171,10 +173,11 @@ class Context:
 
     def __init__(self, nx, ny=1, nz=1, rsolver="hlld", reconstr="linear", bc=(0,) * 6,
                  lmin=(0.0, 0.0, 0.0), lmax=(1.0, 1.0, 1.0), gam=5.0 / 3.0, cour=0.4,
-                 dt_fms_ref=True, precision="fp64", device=0, char_proj=False, hydro=False):
+                 dt_fms_ref=True, precision="fp64", device=0, char_proj=False, hydro=False,
+                 ct_upwind=False):
         L = load()
         p = make_params(nx, ny, nz, rsolver, reconstr, bc, lmin, lmax, gam, cour, dt_fms_ref,
-                        precision, device, char_proj, hydro)
+                        precision, device, char_proj, hydro, ct_upwind)
         h = ctypes.c_void_p()
         rc = L.bnz_create(ctypes.byref(p), ctypes.byref(h))
         if rc != BNZ_OK:

@@ -562,6 +562,8 @@ int bnz_batch_create(const bnz_params *p, const bnz_box *boxes, int nbox, bnz_ba
     return bfail(c, BNZ_EUNSUPPORTED, "reconstr must be const or linear");
   if (p->hydro)
     return bfail(c, BNZ_EUNSUPPORTED, "the hydro build runs on single grids only");
+  if (p->ct_upwind)
+    return bfail(c, BNZ_EUNSUPPORTED, "the upwind corner EMF runs on single grids only");
   if (p->char_proj)
     return bfail(c, BNZ_EUNSUPPORTED, "characteristic limiting runs on single grids only");
   if (!(p->gam > 1.) || !(p->cour > 0.)) return bfail(c, BNZ_EINVAL, "need gam > 1 and cour > 0");
@@ -733,6 +735,7 @@ int bnz_batch_create(const bnz_params *p, const bnz_box *boxes, int nbox, bnz_ba
       g.dlfi[a] = c->geo + geo_off[b * 12 + a * 4 + 1];
       g.icm[a] = c->geo + geo_off[b * 12 + a * 4 + 2];
       g.icp[a] = c->geo + geo_off[b * 12 + a * 4 + 3];
+      g.dlv[a] = nullptr;  // only the upwind corner EMF reads it (single grids)
     }
   }
   BCHK(c, hipMalloc(&c->boxes, nbox * sizeof(BoxDev)));

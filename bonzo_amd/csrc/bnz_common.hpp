@@ -38,6 +38,7 @@ struct Geo {
   const double *dlfi[3];  // 1/width
   const double *icm[3];   // lv - lf       (interp_bc weights, ct.pyx:110-123)
   const double *icp[3];   // lf[i+1] - lv
+  const double *dlv[3];   // lv[i] - lv[i-1] (0 at i = 0), the upwind EMF's ds
   double gam, gamm1i;
   // unif = 1 when each of the four arrays above is constant along every
   // axis (bit for bit, e.g. 2^-8 wide cells); the constants are then here
@@ -265,7 +266,7 @@ struct bnz_ctx {
   double *part = nullptr;     // CFL partials (the last corrector's, kept for the next dt)
   long npart_cap = 0;
   double *rpart = nullptr;    // partials of the diag and history reductions
-  std::vector<double> h_dlf[3], h_dlfi[3], h_icm[3], h_icp[3];
+  std::vector<double> h_dlf[3], h_dlfi[3], h_icm[3], h_icp[3], h_dlv[3];
   bnz::Geo g{};
   // FP32-compensated state (BNZ_PREC_FP32C): conserved hydro variables as
   // FP32 hi + lo pairs, face fields in b/bs (FP64), FP32 primitives of the

@@ -379,6 +379,7 @@ void set_coordinates(bnz_ctx *c) {
     c->h_dlfi[n].resize(N);
     c->h_icm[n].resize(N);
     c->h_icp[n].resize(N);
+    c->h_dlv[n].assign(N, 0.);
     for (int i = 0; i < N; ++i) {
       c->h_dlf[n][i] = lf[i + 1] - lf[i];
       c->h_dlfi[n][i] = 1. / c->h_dlf[n][i];
@@ -386,6 +387,7 @@ void set_coordinates(bnz_ctx *c) {
       c->h_icm[n][i] = lv[i] - lf[i];
       c->h_icp[n][i] = lf[i + 1] - lv[i];
     }
+    for (int i = 1; i < N; ++i) c->h_dlv[n][i] = lv[i] - lv[i - 1];
   }
 }
 
@@ -478,6 +480,9 @@ int configure_grid(bnz_ctx *c, const bnz_params *p) {
   if (p->reconstr != BNZ_RCN_CONST && p->reconstr != BNZ_RCN_LINEAR)
     return fail(c, BNZ_EUNSUPPORTED, "reconstr must be const or linear (weno/parab are commented out in reconstr_c.c)");
   if (p->char_proj != 0 && p->char_proj != 1) return fail(c, BNZ_EINVAL, "char_proj must be 0 or 1");
+  if (p->ct_upwind != 0 && p->ct_upwind != 1) return fail(c, BNZ_EINVAL, "ct_upwind must be 0 or 1");
+  if (p->ct_upwind && p->precision == BNZ_PREC_FP32C)
+    return fail(c, BNZ_EUNSUPPORTED, "the upwind corner EMF runs in FP64 only");
   if (p->char_proj && p->precision == BNZ_PREC_FP32C)
     return fail(c, BNZ_EUNSUPPORTED, "characteristic limiting runs in FP64 only");
   for (int q = 0; q < 6; ++q)
@@ -510,21 +515,22 @@ int configure_grid(bnz_ctx *c, const bnz_params *p) {
   return BNZ_OK;
 }
 
-// Per-axis geometry arrays (dlf, dlfi, icm, icp) as one host table, with the
-// offset of each array in it.
-size_t geo_table(const bnz_ctx *c, std::vector<double> &hg, size_t off[3][4]) {
+// Per-axis geometry arrays (dlf, dlfi, icm, icp, dlv) as one host table,
+// with the offset of each array in it.
+size_t geo_table(const bnz_ctx *c, std::vector<double> &hg, size_t off[3][5]) {
   hg.clear();
   for (int a = 0; a < 3; ++a) {
     off[a][0] = hg.size(); hg.insert(hg.end(), c->h_dlf[a].begin(), c->h_dlf[a].end());
     off[a][1] = hg.size(); hg.insert(hg.end(), c->h_dlfi[a].begin(), c->h_dlfi[a].end());
     off[a][2] = hg.size(); hg.insert(hg.end(), c->h_icm[a].begin(), c->h_icm[a].end());
     off[a][3] = hg.size(); hg.insert(hg.end(), c->h_icp[a].begin(), c->h_icp[a].end());
+    off[a][4] = hg.size(); hg.insert(hg.end(), c->h_dlv[a].begin(), c->h_dlv[a].end());
   }
   return hg.size();
 }
 
 // The kernels' Geo of a configured grid whose geometry table sits at c->geo.
-void set_geo(bnz_ctx *c, const size_t off[3][4]) {
+void set_geo(bnz_ctx *c, const size_t off[3][5]) {
   Geo &g = c->g;
   for (int q = 0; q < 3; ++q) g.nt[q] = c->nt[q];
   g.dims = c->dims;
@@ -536,6 +542,7 @@ void set_geo(bnz_ctx *c, const size_t off[3][4]) {
     g.dlfi[q] = c->geo + off[q][1];
     g.icm[q] = c->geo + off[q][2];
     g.icp[q] = c->geo + off[q][3];
+    g.dlv[q] = c->geo + off[q][4];
   }
   g.gam = c->p.gam;
   g.gamm1i = 1. / (c->p.gam - 1.);
@@ -623,7 +630,7 @@ int bnz_create(const bnz_params *p, bnz_ctx **out) {
   }
 
   std::vector<double> hg;
-  size_t off[3][4];
+  size_t off[3][5];
   const size_t geo_n = geo_table(c, hg, off);
   HIPCHK(c, hipMalloc(&c->geo, geo_n * sizeof(double)));
   HIPCHK(c, hipMemcpy(c->geo, hg.data(), geo_n * sizeof(double), hipMemcpyHostToDevice));

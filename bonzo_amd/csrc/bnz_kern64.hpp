@@ -162,6 +162,81 @@ __device__ __forceinline__ void emf_edge_d(const Geo &g, const double *__restric
   }
 }
 
+// Weight of the lower cell's EMF slope in the upwind corner EMF,
+// get_ct_weight of src/integration/ct.pyx:221-226.
+__device__ __forceinline__ double ct_weight(double frho, double rho_l, double rho_r, double ds,
+                                            double dt) {
+  const double v_c = 1e3 * frho * dt / (fmin(rho_l, rho_r) * ds);
+  return 0.5 + fmax(-0.5, fmin(0.5, v_c));
+}
+
+// Upwind corner EMFs of cell corner (i, j, k), interp_ee2 of
+// src/integration/ct.pyx:229-358: each EMF slope at an edge quarter is taken
+// from the side the mass flux through the neighbouring face comes from
+// (smoothed by ct_weight).  3D: the reference's three components with the
+// two cell-EMF indices of its Ex slopes corrected to the cyclic image of its
+// Ez and Ey (ec[0,k-1,j-1,i]); 2D: its Ez, with Ex = -Fy[Bz] and
+// Ey = Fx[Bz].  dt is the step's dt.  Same operation order as the oracle.
+template <int DIMS>
+__device__ __forceinline__ void emf_edge_upwind(const Geo &g, const double *__restrict__ w,
+                                                const double *__restrict__ fx,
+                                                const double *__restrict__ fy,
+                                                const double *__restrict__ fz,
+                                                double *__restrict__ ee, double dt, int i, int j,
+                                                int k) {
+  const long sv = g.sv;
+  const long c = ((long)k * g.nt[1] + j) * g.nt[0] + i;
+  const long sy = g.sy, sz = DIMS == 3 ? g.sz : 0;
+  const double *rho = w;  // slot RHO
+  {  // Ez at (i-1/2, j-1/2)
+    const double wgtx = ct_weight(fx[RHO * sv + c], rho[c - 1], rho[c], g.dlf[0][i], dt);
+    const double dedy_14 = (wgtx * (ec_comp(w, sv, c - 1, 2) - fy[BX * sv + c - 1]) +
+                            (1. - wgtx) * (ec_comp(w, sv, c, 2) - fy[BX * sv + c]));
+    const double dedy_34 = (wgtx * (fy[BX * sv + c - 1] - ec_comp(w, sv, c - sy - 1, 2)) +
+                            (1. - wgtx) * (fy[BX * sv + c] - ec_comp(w, sv, c - sy, 2)));
+    const double wgty = ct_weight(fy[RHO * sv + c], rho[c - sy], rho[c], g.dlv[1][j], dt);
+    const double dedx_14 = (wgty * (ec_comp(w, sv, c - sy, 2) + fx[BY * sv + c - sy]) +
+                            (1. - wgty) * (ec_comp(w, sv, c, 2) + fx[BY * sv + c]));
+    const double dedx_34 = (wgty * (-fx[BY * sv + c - sy] - ec_comp(w, sv, c - sy - 1, 2)) +
+                            (1. - wgty) * (-fx[BY * sv + c] - ec_comp(w, sv, c - 1, 2)));
+    ee[2 * sv + c] = 0.25 * (-fx[BY * sv + c - sy] - fx[BY * sv + c] + fy[BX * sv + c - 1] +
+                             fy[BX * sv + c] + dedy_34 - dedy_14 + dedx_34 - dedx_14);
+  }
+  if (DIMS == 2) {
+    ee[0 * sv + c] = -fy[BZ * sv + c];
+    ee[1 * sv + c] = fx[BZ * sv + c];
+    return;
+  }
+  {  // Ex at (j-1/2, k-1/2)
+    const double wgty = ct_weight(fy[RHO * sv + c], rho[c - sy], rho[c], g.dlf[1][j], dt);
+    const double dedz_14 = (wgty * (ec_comp(w, sv, c - sy, 0) - fz[BY * sv + c - sy]) +
+                            (1. - wgty) * (ec_comp(w, sv, c, 0) - fz[BY * sv + c]));
+    const double dedz_34 = (wgty * (fz[BY * sv + c - sy] - ec_comp(w, sv, c - sz - sy, 0)) +
+                            (1. - wgty) * (fz[BY * sv + c] - ec_comp(w, sv, c - sz, 0)));
+    const double wgtz = ct_weight(fz[RHO * sv + c], rho[c - sz], rho[c], g.dlv[2][k], dt);
+    const double dedy_14 = (wgtz * (ec_comp(w, sv, c - sz, 0) + fy[BZ * sv + c - sz]) +
+                            (1. - wgtz) * (ec_comp(w, sv, c, 0) + fy[BZ * sv + c]));
+    const double dedy_34 = (wgtz * (-fy[BZ * sv + c - sz] - ec_comp(w, sv, c - sz - sy, 0)) +
+                            (1. - wgtz) * (-fy[BZ * sv + c] - ec_comp(w, sv, c - sy, 0)));
+    ee[0 * sv + c] = 0.25 * (-fy[BZ * sv + c - sz] - fy[BZ * sv + c] + fz[BY * sv + c - sy] +
+                             fz[BY * sv + c] + dedz_34 - dedz_14 + dedy_34 - dedy_14);
+  }
+  {  // Ey at (i-1/2, k-1/2)
+    const double wgtz = ct_weight(fz[RHO * sv + c], rho[c - sz], rho[c], g.dlf[2][k], dt);
+    const double dedx_14 = (wgtz * (ec_comp(w, sv, c - sz, 1) - fx[BZ * sv + c - sz]) +
+                            (1. - wgtz) * (ec_comp(w, sv, c, 1) - fx[BZ * sv + c]));
+    const double dedx_34 = (wgtz * (fx[BZ * sv + c - sz] - ec_comp(w, sv, c - sz - 1, 1)) +
+                            (1. - wgtz) * (fx[BZ * sv + c] - ec_comp(w, sv, c - 1, 1)));
+    const double wgtx = ct_weight(fx[RHO * sv + c], rho[c - 1], rho[c], g.dlv[0][i], dt);
+    const double dedz_14 = (wgtx * (ec_comp(w, sv, c - 1, 1) + fz[BX * sv + c - 1]) +
+                            (1. - wgtx) * (ec_comp(w, sv, c, 1) + fz[BX * sv + c]));
+    const double dedz_34 = (wgtx * (-fz[BX * sv + c - 1] - ec_comp(w, sv, c - sz - 1, 1)) +
+                            (1. - wgtx) * (-fz[BX * sv + c] - ec_comp(w, sv, c - sz, 1)));
+    ee[1 * sv + c] = 0.25 * (-fz[BX * sv + c - 1] - fz[BX * sv + c] + fx[BZ * sv + c - sz] +
+                             fx[BZ * sv + c] + dedx_34 - dedx_14 + dedz_34 - dedz_14);
+  }
+}
+
 // --------------------------------------------------------------------------
 // Stage update: advance_hydro (godunov.pyx:215-252), advance_b (ct.pyx:15-91),
 // interp_bc (ct.pyx:96-123) and cons2prim (eos.pyx:13-41) fused, one thread

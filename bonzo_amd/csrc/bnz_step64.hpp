@@ -83,6 +83,20 @@ __global__ __launch_bounds__(BLK) void k_emf(Geo g, const double *__restrict__ w
   emf_edge_d<DIMS>(g, w, fx, fy, fz, ee, i, j, k);
 }
 
+// The upwind corner EMFs (interp_ee2, emf_edge_upwind), weights from the
+// step's dt.
+template <int DIMS, bool TILED = false>
+__global__ __launch_bounds__(BLK) void k_emf_upw(Geo g, const double *__restrict__ w,
+                                                 const double *__restrict__ fx,
+                                                 const double *__restrict__ fy,
+                                                 const double *__restrict__ fz,
+                                                 double *__restrict__ ee, Box box, int ty,
+                                                 const double *__restrict__ dts) {
+  int i, j, k;
+  if (!cell_index<TILED, 1, 2>(box, ty, i, j, k)) return;
+  emf_edge_upwind<DIMS>(g, w, fx, fy, fz, ee, dts[0], i, j, k);
+}
+
 template <int DIMS, bool PRED, bool DTRED = false, bool TILED = false, bool UG = false>
 __global__ __launch_bounds__(BLK) void k_update(Geo g, const double *w0, const double *__restrict__ fx,
                                                 const double *__restrict__ fy,
@@ -161,9 +175,17 @@ void launch_emf(bnz_ctx *c, const double *w, const Lims &L, const char *name) {
   Launch t(c, name);
 #define EMF_ARGS                                                                              \
   dim3(sh.blocks), dim3(BLK), 0, c->stream, c->g, w, c->fx, c->fy, c->fz, c->ee, e, sh.ty
-  if (sh.tiled) hipLaunchKernelGGL((k_emf<3, true>), EMF_ARGS);
-  else if (c->dims == 3) hipLaunchKernelGGL((k_emf<3, false>), EMF_ARGS);
-  else hipLaunchKernelGGL((k_emf<2, false>), EMF_ARGS);  // 1D grids run k_step1
+  if (c->p.ct_upwind) {  // 1D grids run k_step1, whose 1D EMFs both forms share
+    if (sh.tiled) hipLaunchKernelGGL((k_emf_upw<3, true>), EMF_ARGS, c->dev);
+    else if (c->dims == 3) hipLaunchKernelGGL((k_emf_upw<3, false>), EMF_ARGS, c->dev);
+    else hipLaunchKernelGGL((k_emf_upw<2, false>), EMF_ARGS, c->dev);
+  } else if (sh.tiled) {
+    hipLaunchKernelGGL((k_emf<3, true>), EMF_ARGS);
+  } else if (c->dims == 3) {
+    hipLaunchKernelGGL((k_emf<3, false>), EMF_ARGS);
+  } else {
+    hipLaunchKernelGGL((k_emf<2, false>), EMF_ARGS);  // 1D grids run k_step1
+  }
 #undef EMF_ARGS
 }
 

@@ -145,6 +145,12 @@ class BnzIntegr:
             raise ValueError("charact_proj must be 0 or 1")
         if self.char_proj and not self.mfield:
             raise NotImplementedError("characteristic limiting of the hydro build is not built")
+        # corner EMF of CT: 0 interp_ee1 (the reference's), 1 the upwind
+        # interp_ee2 (src/integration/ct.pyx:229-358), which the reference
+        # selects only by editing integrator.pyx:106-108 (no config key there)
+        self.ct_upwind = params.get("computation", "ct_upwind", "i", 0)
+        if self.ct_upwind not in (0, 1):
+            raise ValueError("ct_upwind must be 0 or 1")
         # Physics the reference adds around the step whenever its coefficients
         # are non-zero: constant gravity (src/srcterms/gravity.pyx:15-17,
         # applied in integrator.pyx:146-157), turbulence driving
@@ -234,7 +240,8 @@ class Simulation:
                                 dt_fms_ref=self.integr.dt_fms_ref,
                                 precision=p.get("computation", "precision", "s", "fp64"),
                                 device=device, char_proj=bool(self.integr.char_proj),
-                                hydro=not self.integr.mfield)
+                                hydro=not self.integr.mfield,
+                                ct_upwind=bool(self.integr.ct_upwind))
         if set_problem is None:
             name = p.get("problem", "name", "s", "otvortex")
             prim, bfld = PROBLEMS[name](self.grid, p)

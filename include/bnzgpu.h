@@ -73,6 +73,12 @@ typedef struct {
                            The state keeps the field slots, which must be
                            zero (bnz_upload checks) and stay exactly zero.
                            FP64 single grids only; 0 is the MHD build */
+  int ct_upwind;        /* corner EMF of the CT update: 0 the reference's
+                           average interp_ee1 (src/integration/ct.pyx:153-215),
+                           1 the upwind interp_ee2 (ct.pyx:229-358, which the
+                           reference leaves commented out at
+                           integrator.pyx:106-108), weights from the face mass
+                           fluxes and the step's dt; FP64 single grids only */
 } bnz_params;
 
 typedef struct {

@@ -354,3 +354,30 @@ def test_create_validates_the_hydro_build_without_touching_gpu():
     p = _lib.make_params(rsolver="hll", hydro=True)
     assert L.bnz_batch_create(ctypes.byref(p), boxes, 1, ctypes.byref(h)) == _lib.BNZ_EUNSUPPORTED
     L.bnz_batch_destroy(h)
+
+
+def test_upwind_emf_selection_and_validation_without_touching_gpu():
+    """[computation] ct_upwind picks the upwind corner EMF (interp_ee2); the
+    device takes it in FP64 on single grids and refuses FP32C and batches."""
+    cfg = {"computation": {"cour": 0.4, "rsolver": "hlld", "ct_upwind": 1},
+           "physics": {"gam": 1.4}}
+    assert BnzIntegr(_Params(config=cfg)).ct_upwind == 1
+    cfg["computation"]["ct_upwind"] = 2
+    with pytest.raises(ValueError):
+        BnzIntegr(_Params(config=cfg))
+    L = _lib.load()
+    h = ctypes.c_void_p()
+    p = _lib.make_params(nx=16, ny=16, ct_upwind=True, precision="fp32c")
+    assert L.bnz_create(ctypes.byref(p), ctypes.byref(h)) == _lib.BNZ_EUNSUPPORTED
+    L.bnz_destroy(h)
+    p = _lib.make_params(nx=16, ny=16)
+    p.ct_upwind = 3
+    assert L.bnz_create(ctypes.byref(p), ctypes.byref(h)) == _lib.BNZ_EINVAL
+    L.bnz_destroy(h)
+    boxes = (_lib.BnzBox * 1)()
+    boxes[0].nx, boxes[0].ny, boxes[0].nz, boxes[0].tmax = 16, 16, 1, 0.1
+    for a in range(3):
+        boxes[0].lmax[a] = 1.0
+    p = _lib.make_params(ct_upwind=True)
+    assert L.bnz_batch_create(ctypes.byref(p), boxes, 1, ctypes.byref(h)) == _lib.BNZ_EUNSUPPORTED
+    L.bnz_batch_destroy(h)
