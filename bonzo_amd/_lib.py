@@ -42,6 +42,7 @@ class BnzParams(ctypes.Structure):
         ("gam", ctypes.c_double), ("cour", ctypes.c_double),
         ("dt_fms_ref", ctypes.c_int), ("precision", ctypes.c_int), ("device", ctypes.c_int),
         ("char_proj", ctypes.c_int), ("hydro", ctypes.c_int), ("ct_upwind", ctypes.c_int),
+        ("tintegr", ctypes.c_int),
     ]
 
 
@@ -148,7 +149,7 @@ def _ptr(a):
 def make_params(nx=1, ny=1, nz=1, rsolver="hlld", reconstr="linear", bc=(0,) * 6,
                 lmin=(0.0, 0.0, 0.0), lmax=(1.0, 1.0, 1.0), gam=5.0 / 3.0, cour=0.4,
                 dt_fms_ref=True, precision="fp64", device=0, char_proj=False, hydro=False,
-                ct_upwind=False):
+                ct_upwind=False, tintegr="vl"):
     p = BnzParams()
     p.nx, p.ny, p.nz = nx, ny, nz
     p.rsolver = RS[rsolver] if isinstance(rsolver, str) else int(rsolver)
@@ -165,6 +166,7 @@ def make_params(nx=1, ny=1, nz=1, rsolver="hlld", reconstr="linear", bc=(0,) * 6
     p.char_proj = int(bool(char_proj))
     p.hydro = int(bool(hydro))
     p.ct_upwind = int(bool(ct_upwind))
+    p.tintegr = {"vl": 0, "rk3": 1}[tintegr] if isinstance(tintegr, str) else int(tintegr)
     return p
 
 
@@ -174,10 +176,10 @@ class Context:
     def __init__(self, nx, ny=1, nz=1, rsolver="hlld", reconstr="linear", bc=(0,) * 6,
                  lmin=(0.0, 0.0, 0.0), lmax=(1.0, 1.0, 1.0), gam=5.0 / 3.0, cour=0.4,
                  dt_fms_ref=True, precision="fp64", device=0, char_proj=False, hydro=False,
-                 ct_upwind=False):
+                 ct_upwind=False, tintegr="vl"):
         L = load()
         p = make_params(nx, ny, nz, rsolver, reconstr, bc, lmin, lmax, gam, cour, dt_fms_ref,
-                        precision, device, char_proj, hydro, ct_upwind)
+                        precision, device, char_proj, hydro, ct_upwind, tintegr)
         h = ctypes.c_void_p()
         rc = L.bnz_create(ctypes.byref(p), ctypes.byref(h))
         if rc != BNZ_OK:

@@ -564,6 +564,8 @@ int bnz_batch_create(const bnz_params *p, const bnz_box *boxes, int nbox, bnz_ba
     return bfail(c, BNZ_EUNSUPPORTED, "the hydro build runs on single grids only");
   if (p->ct_upwind)
     return bfail(c, BNZ_EUNSUPPORTED, "the upwind corner EMF runs on single grids only");
+  if (p->tintegr)
+    return bfail(c, BNZ_EUNSUPPORTED, "the RK3 integrator runs on single grids only");
   if (p->char_proj)
     return bfail(c, BNZ_EUNSUPPORTED, "characteristic limiting runs on single grids only");
   if (!(p->gam > 1.) || !(p->cour > 0.)) return bfail(c, BNZ_EINVAL, "need gam > 1 and cour > 0");

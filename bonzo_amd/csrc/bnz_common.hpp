@@ -257,6 +257,9 @@ struct bnz_ctx {
   hipStream_t stream = nullptr;
   double *w = nullptr, *ws = nullptr, *b = nullptr, *bs = nullptr;
   double *fx = nullptr, *fy = nullptr, *fz = nullptr, *ee = nullptr;
+  // RK3 (tintegr = 1): U, U** (conserved) and B** (faces); U* and B* use ws
+  // and bs
+  double *uc = nullptr, *uss = nullptr, *bss = nullptr;
   double *geo = nullptr;
   int2 *ghost = nullptr;      // (ghost cell, source cell) pairs, see build_ghosts
   int nghost = 0;

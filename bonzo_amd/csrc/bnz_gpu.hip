@@ -481,6 +481,9 @@ int configure_grid(bnz_ctx *c, const bnz_params *p) {
     return fail(c, BNZ_EUNSUPPORTED, "reconstr must be const or linear (weno/parab are commented out in reconstr_c.c)");
   if (p->char_proj != 0 && p->char_proj != 1) return fail(c, BNZ_EINVAL, "char_proj must be 0 or 1");
   if (p->ct_upwind != 0 && p->ct_upwind != 1) return fail(c, BNZ_EINVAL, "ct_upwind must be 0 or 1");
+  if (p->tintegr != 0 && p->tintegr != 1) return fail(c, BNZ_EINVAL, "tintegr must be 0 (vl) or 1 (rk3)");
+  if (p->tintegr && p->precision == BNZ_PREC_FP32C)
+    return fail(c, BNZ_EUNSUPPORTED, "the RK3 integrator runs in FP64 only");
   if (p->ct_upwind && p->precision == BNZ_PREC_FP32C)
     return fail(c, BNZ_EUNSUPPORTED, "the upwind corner EMF runs in FP64 only");
   if (p->char_proj && p->precision == BNZ_PREC_FP32C)
@@ -491,7 +494,8 @@ int configure_grid(bnz_ctx *c, const bnz_params *p) {
                  "boundary flags must be 0 (periodic), 1 (outflow) or 2 (reflect)");
   c->dims = (p->nz > 1) ? 3 : (p->ny > 1) ? 2 : 1;
   c->order = (p->reconstr == BNZ_RCN_CONST) ? 0 : 1;
-  c->ng = (c->order == 0) ? 2 : 3;
+  // grid.pyx:129-143: RK3's stages run without a ghost fill between them
+  c->ng = p->tintegr ? 9 : (c->order == 0) ? 2 : 3;
   c->nt[0] = p->nx + 2 * c->ng;
   c->nt[1] = (c->dims >= 2) ? p->ny + 2 * c->ng : 1;
   c->nt[2] = (c->dims >= 3) ? p->nz + 2 * c->ng : 1;
@@ -615,6 +619,10 @@ int bnz_create(const bnz_params *p, bnz_ctx **out) {
     bufs9.insert(bufs9.end(), {&c->ws, &c->fx, &c->fy, &c->fz});
     bufs3.insert(bufs3.end(), {&c->ee});
   }
+  if (p->tintegr) {
+    bufs9.insert(bufs9.end(), {&c->uc, &c->uss});
+    bufs3.insert(bufs3.end(), {&c->bss});
+  }
   for (double **q : bufs9)
     if (hipMalloc(q, NVAR_STATE * cells * sizeof(double)) != hipSuccess)
       return fail(c, BNZ_ENOMEM, "device allocation of grid arrays failed");
@@ -679,8 +687,8 @@ int bnz_destroy(bnz_ctx *c) {
   if (!c) return BNZ_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   collect_timing(c);
-  double *bufs[] = {c->w,   c->ws,  c->b,   c->bs,   c->fx,  c->fy,
-                    c->fz,  c->ee,  c->geo, c->dev,  c->part, c->rpart};
+  double *bufs[] = {c->w,   c->ws,  c->b,    c->bs,  c->fx,  c->fy, c->fz, c->ee,
+                    c->geo, c->dev, c->part, c->rpart, c->uc, c->uss, c->bss};
   for (double *q : bufs)
     if (q) (void)hipFree(q);
   if (c->ghost) (void)hipFree(c->ghost);

@@ -186,6 +186,12 @@ __device__ __forceinline__ void emf_edge_upwind(const Geo &g, const double *__re
                                                 int k) {
   const long sv = g.sv;
   const long c = ((long)k * g.nt[1] + j) * g.nt[0] + i;
+  if (DIMS == 1) {  // interp_ee2's 1D branch (ct.pyx:303-304, 341-342, 282-283)
+    ee[0 * sv + c] = ec_comp(w, sv, c, 0);
+    ee[1 * sv + c] = fx[BZ * sv + c];
+    ee[2 * sv + c] = -fx[BY * sv + c];
+    return;
+  }
   const long sy = g.sy, sz = DIMS == 3 ? g.sz : 0;
   const double *rho = w;  // slot RHO
   {  // Ez at (i-1/2, j-1/2)

@@ -16,6 +16,7 @@
 #include <utility>
 
 #include "bnz_kern64.hpp"
+#include "bnz_rk3.hpp"
 
 namespace {
 
@@ -97,6 +98,42 @@ __global__ __launch_bounds__(BLK) void k_emf_upw(Geo g, const double *__restrict
   emf_edge_upwind<DIMS>(g, w, fx, fy, fz, ee, dts[0], i, j, k);
 }
 
+// One RK3 stage update (update_cell_rk) over the whole grid; stage 3 leaves
+// the next step's CFL partial maxima in part.
+template <int DIMS, int STAGE>
+__global__ __launch_bounds__(BLK) void k_update_rk(
+    Geo g, const double *__restrict__ u0, const double *__restrict__ fx,
+    const double *__restrict__ fy, const double *__restrict__ fz, const double *__restrict__ ee,
+    const double *__restrict__ b0, const double *ua, const double *ba, double *u1, double *b1,
+    double *w1, Lims L, const double *__restrict__ dts, Box grid, Box act, int fms_ref,
+    double *__restrict__ part) {
+  int i, j, k;
+  double cdsmax = 0.;
+  const bool have = cell_index<false, 1, 2>(grid, 0, i, j, k);
+  if (dts[0] == 0.) return;  // a step past t_max changes nothing (uniform)
+  if (have) {
+    const long c = ((long)k * g.nt[1] + j) * g.nt[0] + i;
+    cdsmax = update_cell_rk<DIMS, STAGE, STAGE == 3>(
+        g, u0, FluxGlobal{fx, fy, fz, g.sv, c}, EmfGlobal{ee, g.sv, c}, b0, ua, ba, u1, b1, w1,
+        L, dts[0], act, fms_ref, i, j, k);
+  }
+  if (STAGE == 3) block_max_to(cdsmax, part);
+}
+
+// U = prim2cons(W) over every cell (the RK3 step's U, _rk3.pyx:167).
+__global__ __launch_bounds__(BLK) void k_cons_of_prim(Geo g, const double *__restrict__ w,
+                                                      double *__restrict__ u,
+                                                      const double *__restrict__ dts) {
+  const long t = xcd_block() * BLK + threadIdx.x;
+  if (dts[0] == 0. || t >= g.sv) return;
+  double wv[NMODE], uv[NMODE];
+#pragma unroll
+  for (int n = 0; n < NMODE; ++n) wv[n] = w[n * g.sv + t];
+  prim2cons(uv, wv, g.gamm1i);
+#pragma unroll
+  for (int n = 0; n < NMODE; ++n) u[n * g.sv + t] = uv[n];
+}
+
 template <int DIMS, bool PRED, bool DTRED = false, bool TILED = false, bool UG = false>
 __global__ __launch_bounds__(BLK) void k_update(Geo g, const double *w0, const double *__restrict__ fx,
                                                 const double *__restrict__ fy,
@@ -175,16 +212,20 @@ void launch_emf(bnz_ctx *c, const double *w, const Lims &L, const char *name) {
   Launch t(c, name);
 #define EMF_ARGS                                                                              \
   dim3(sh.blocks), dim3(BLK), 0, c->stream, c->g, w, c->fx, c->fy, c->fz, c->ee, e, sh.ty
-  if (c->p.ct_upwind) {  // 1D grids run k_step1, whose 1D EMFs both forms share
+  // (1D grids reach here only with RK3: the VL step of a 1D grid is k_step1)
+  if (c->p.ct_upwind) {
     if (sh.tiled) hipLaunchKernelGGL((k_emf_upw<3, true>), EMF_ARGS, c->dev);
     else if (c->dims == 3) hipLaunchKernelGGL((k_emf_upw<3, false>), EMF_ARGS, c->dev);
-    else hipLaunchKernelGGL((k_emf_upw<2, false>), EMF_ARGS, c->dev);
+    else if (c->dims == 2) hipLaunchKernelGGL((k_emf_upw<2, false>), EMF_ARGS, c->dev);
+    else hipLaunchKernelGGL((k_emf_upw<1, false>), EMF_ARGS, c->dev);
   } else if (sh.tiled) {
     hipLaunchKernelGGL((k_emf<3, true>), EMF_ARGS);
   } else if (c->dims == 3) {
     hipLaunchKernelGGL((k_emf<3, false>), EMF_ARGS);
+  } else if (c->dims == 2) {
+    hipLaunchKernelGGL((k_emf<2, false>), EMF_ARGS);
   } else {
-    hipLaunchKernelGGL((k_emf<2, false>), EMF_ARGS);  // 1D grids run k_step1
+    hipLaunchKernelGGL((k_emf<1, false>), EMF_ARGS);
   }
 #undef EMF_ARGS
 }
@@ -259,8 +300,60 @@ void launch_step1(bnz_ctx *c, const Lims &L0, const Lims &L1) {
   }
 }
 
+// One RK3 stage update launch: conserved input u0 and faces b0, results to
+// u1 and b1 (and prim w), combined with U (uc) and B (b) in stages 2, 3.
+template <int STAGE>
+void launch_update_rk(bnz_ctx *c, const double *u0, const double *b0, double *u1, double *b1,
+                      const Lims &L) {
+  Box grid;
+  for (int q = 0; q < 3; ++q) {
+    grid.lo[q] = 0;
+    grid.n[q] = c->nt[q];
+  }
+  const Box act = active_box(c);
+  const int nb = blocks_for(grid.total());
+  Launch t(c, STAGE == 1 ? "update_rk1" : STAGE == 2 ? "update_rk2" : "update_rk3");
+#define RK_ARGS                                                                                 \
+  dim3(nb), dim3(BLK), 0, c->stream, c->g, u0, c->fx, c->fy, c->fz, c->ee, b0, c->uc, c->b, u1, \
+      b1, c->w, L, c->dev, grid, act, cfl_mode(c), c->part
+  if (c->dims == 3) hipLaunchKernelGGL((k_update_rk<3, STAGE>), RK_ARGS);
+  else if (c->dims == 2) hipLaunchKernelGGL((k_update_rk<2, STAGE>), RK_ARGS);
+  else hipLaunchKernelGGL((k_update_rk<1, STAGE>), RK_ARGS);
+#undef RK_ARGS
+  if (STAGE == 3) c->dt_parts = nb;
+}
+
+// One RK3 step after the dt kernels (_rk3.pyx:56-167): U from the prim, the
+// three stages on the limits of the reconstruction order, the ghost fill.
+void launch_rk3_body(bnz_ctx *c) {
+  const Lims L = integr_lims(c, c->order);
+  {
+    Launch t(c, "cons_of_prim");
+    hipLaunchKernelGGL(k_cons_of_prim, dim3(blocks_for(c->ncell)), dim3(BLK), 0, c->stream,
+                       c->g, c->w, c->uc, c->dev);
+  }
+  // I: U* = U + dt L(W, B)
+  launch_flux_dispatch(c, c->order, c->w, c->b, L);
+  launch_emf(c, c->w, L, "emf_rk");
+  launch_update_rk<1>(c, c->uc, c->b, c->ws, c->bs, L);
+  // II: U** = 3/4 U + 1/4 (U* + dt L(W*, B*))
+  launch_flux_dispatch(c, c->order, c->w, c->bs, L);
+  launch_emf(c, c->w, L, "emf_rk");
+  launch_update_rk<2>(c, c->ws, c->bs, c->uss, c->bss, L);
+  // III: U = 1/3 U + 2/3 (U** + dt L(W**, B**)), in place on U and B
+  launch_flux_dispatch(c, c->order, c->w, c->bss, L);
+  launch_emf(c, c->w, L, "emf_rk");
+  launch_update_rk<3>(c, c->uss, c->bss, c->uc, c->b, L);
+  launch_ghost64(c);
+  launch_time_advance(c);
+}
+
 // One VL step after the dt kernels (vanleer.pyx:100-163).
 void launch_step_body(bnz_ctx *c) {
+  if (c->p.tintegr) {
+    launch_rk3_body(c);
+    return;
+  }
   Lims L0 = integr_lims(c, 0);
   Lims L1 = integr_lims(c, c->order);
   if (c->dims == 1) {  // the whole step through LDS

@@ -13,7 +13,6 @@ The step itself runs in libbnzgpu.so on the GPU; there is no CPU fallback.
 """
 import configparser
 import os
-import warnings
 from enum import IntEnum
 
 import numpy as np
@@ -130,12 +129,12 @@ class BnzIntegr:
                                  "(src/integration/integrator.pyx:258-268)")
             if self.rsolver == RSolver.RS_HLLT:
                 raise NotImplementedError("hllt of the hydro build is not built (hll, hllc are)")
-        if self.tintegr == TIntegr.TINT_RK3:
-            # The reference parses rk3 and allocates its extra stage arrays
-            # (integrator.pyx:339-361), but Simulation always drives
-            # vanleer.advance (simulation.pyx:67-75); _rk3.pyx is not wired.
-            warnings.warn("tintegr=rk3: the reference's driver runs the VL integrator "
-                          "(src/simulation.pyx:67-75); so does this one", stacklevel=2)
+        # tintegr=rk3 selects the RK3 integrator of src/integration/_rk3.pyx
+        # (parsed at integrator.pyx:220-223, stage arrays at :339-361); the
+        # reference's Simulation drives vanleer.advance whatever it says
+        # (simulation.pyx:67-75), and its _rk3.pyx is written against an
+        # older API, so here the selection is honoured with the RK3 built
+        # from _rk3.pyx's algorithm (ng = 9, grid.pyx:142-143).
         if self.reconstr in (Reconstr.RCN_PARAB, Reconstr.RCN_WENO):
             raise NotImplementedError("parab/weno are commented out in src/coordinates/reconstr_c.c")
         # charact_proj (integrator.pyx:245; read into the attribute the
@@ -177,7 +176,8 @@ class BnzGrid:
         self.dims = 3 if self.Nact[2] > 1 else (2 if self.Nact[1] > 1 else 1)
         if self.Nact[2] > 1 and self.Nact[1] == 1:
             raise ValueError("Nz > 1 needs Ny > 1")
-        self.ng = 2 if integr.reconstr == Reconstr.RCN_CONST else 3
+        self.ng = (9 if integr.tintegr == TIntegr.TINT_RK3 else
+                   2 if integr.reconstr == Reconstr.RCN_CONST else 3)
         self.Ntot = [self.Nact[0] + 2 * self.ng,
                      self.Nact[1] + 2 * self.ng if self.dims >= 2 else 1,
                      self.Nact[2] + 2 * self.ng if self.dims >= 3 else 1]
@@ -241,7 +241,9 @@ class Simulation:
                                 precision=p.get("computation", "precision", "s", "fp64"),
                                 device=device, char_proj=bool(self.integr.char_proj),
                                 hydro=not self.integr.mfield,
-                                ct_upwind=bool(self.integr.ct_upwind))
+                                ct_upwind=bool(self.integr.ct_upwind),
+                                tintegr="rk3" if self.integr.tintegr == TIntegr.TINT_RK3
+                                else "vl")
         if set_problem is None:
             name = p.get("problem", "name", "s", "otvortex")
             prim, bfld = PROBLEMS[name](self.grid, p)

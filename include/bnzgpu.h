@@ -79,6 +79,11 @@ typedef struct {
                            reference leaves commented out at
                            integrator.pyx:106-108), weights from the face mass
                            fluxes and the step's dt; FP64 single grids only */
+  int tintegr;          /* [computation] tintegr: 0 vl (vanleer.pyx), 1 rk3,
+                           the three-stage SSP Runge-Kutta of
+                           src/integration/_rk3.pyx:20-175 with ng = 9 and no
+                           ghost fill between its stages (grid.pyx:142-143);
+                           FP64 single grids only */
 } bnz_params;
 
 typedef struct {

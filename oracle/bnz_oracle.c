@@ -708,7 +708,7 @@ typedef void (*rsolver_fn)(double **, double **, double **, double *, int, int, 
 typedef void (*reconstr_fn)(double **, double **, double ***, double ***, int, int, int);
 
 struct or_sim {
-  int dims, ng, reconstr_order, nthreads, dt_fms_ref, char_proj, hydro, ct_upwind;
+  int dims, ng, reconstr_order, nthreads, dt_fms_ref, char_proj, hydro, ct_upwind, tintegr;
   int Nact[3], Ntot[3];
   int i1, i2, j1, j2, k1, k2;
   double lmin[3], lmax[3];
@@ -718,6 +718,7 @@ struct or_sim {
   int bc[3][2];
   long ncell;
   double *prim, *cons, *bfld, *cons_s, *bfld_s, *flx[3], *ec, *ee;
+  double *cons_ss, *bfld_ss;  /* RK3's second stage (Uss, Bss) */
   double time, dt;
   int step;
 };
@@ -848,11 +849,23 @@ or_sim *or_create(int nx, int ny, int nz, int reconstr_order, const double *lmin
                   const double *lmax, double gam, double cour, int rsolver, const int *bc_flags,
                   int nthreads)
 {
+  return or_create_ex(nx, ny, nz, reconstr_order, lmin, lmax, gam, cour, rsolver, bc_flags,
+                      nthreads, 0);
+}
+
+/* tintegr 1: the RK3 integrator (src/integration/_rk3.pyx), whose three
+   stages run without a ghost fill between them and so need ng = 9
+   (src/coordinates/grid.pyx:142-143) */
+or_sim *or_create_ex(int nx, int ny, int nz, int reconstr_order, const double *lmin,
+                     const double *lmax, double gam, double cour, int rsolver,
+                     const int *bc_flags, int nthreads, int tintegr)
+{
   or_sim *s = calloc(1, sizeof(or_sim));
   s->dims = (nz > 1) ? 3 : (ny > 1) ? 2 : 1;
-  /* ng: src/coordinates/grid.pyx:129-134 */
+  /* ng: src/coordinates/grid.pyx:129-143 */
   s->reconstr_order = reconstr_order;
-  s->ng = (reconstr_order == 0) ? 2 : 3;
+  s->tintegr = tintegr;
+  s->ng = tintegr ? 9 : (reconstr_order == 0) ? 2 : 3;
   s->Nact[0] = nx;
   s->Nact[1] = ny;
   s->Nact[2] = nz;
@@ -887,6 +900,10 @@ or_sim *or_create(int nx, int ny, int nz, int reconstr_order, const double *lmin
   for (int d = 0; d < 3; ++d) s->flx[d] = calloc(NMODE * s->ncell, sizeof(double));
   s->ec = calloc(3 * s->ncell, sizeof(double));
   s->ee = calloc(3 * s->ncell, sizeof(double));
+  if (tintegr) {
+    s->cons_ss = calloc(NMODE * s->ncell, sizeof(double));
+    s->bfld_ss = calloc(3 * s->ncell, sizeof(double));
+  }
   return s;
 }
 
@@ -904,6 +921,8 @@ void or_destroy(or_sim *s)
   free(s->prim);
   free(s->cons);
   free(s->cons_s);
+  free(s->cons_ss);
+  free(s->bfld_ss);
   free(s->bfld);
   free(s->bfld_s);
   free(s->ec);
@@ -1577,7 +1596,77 @@ void or_vl_step(or_sim *s, double dt_full)
   prim2cons_lims(s, u, w, lims_tot);
 }
 
-/* vanleer.advance loop: src/integration/vanleer.pyx:87-167 */
+/* combine of src/integration/_rk3.pyx:196-204: a3 = c1 a1 + c2 a2 over the
+   cells of lims, nvar variables (a3 may alias a1 or a2) */
+static void combine(or_sim *s, double *a3, const double *a1, const double *a2, double c1,
+                    double c2, const int *lims, int nvar)
+{
+  for (int n = 0; n < nvar; ++n)
+    for (int k = lims[4]; k <= lims[5]; ++k)
+      for (int j = lims[2]; j <= lims[3]; ++j)
+        for (int i = lims[0]; i <= lims[1]; ++i) {
+          long c = IDX(s, n, k, j, i);
+          a3[c] = c1 * a1[c] + c2 * a2[c];
+        }
+}
+
+/* One stage of _rk3.pyx: integrate_hydro (u1 = u0 + dt L) and
+   integrate_field (b1 = b0 + dt L, cell-centred field of u1) from the
+   primitive state w and the faces b0 */
+static void rk3_stage(or_sim *s, double *u1, const double *u0, double *b1, const double *b0,
+                      const int *lims, double dt)
+{
+  double *w = s->prim;
+  godunov_fluxes(s, w, b0, lims, s->reconstr_order);
+  advance_hydro(s, u1, u0, lims, dt);
+  ec_from_prim(s, w, lims);
+  if (s->ct_upwind) interp_ee2(s, lims, w, dt);
+  else interp_ee1(s, lims);
+  advance_b(s, b1, b0, lims, dt);
+  interp_bc(s, u1, b1, lims);
+}
+
+/* One RK3 step with a given dt: src/integration/_rk3.pyx:20-175, the
+   three-stage strong-stability-preserving Runge-Kutta of Shu & Osher
+   (U* = U + dt L(U); U** = 3/4 U + 1/4 (U* + dt L(U*)); U_{n+1} = 1/3 U +
+   2/3 (U** + dt L(U**))), every stage at the reconstruction order and dt on
+   the same limits.  The primitive array is updated in place on the limits
+   after each stage (Ws and Wss alias W), with no ghost fill between the
+   stages: the cells next to the limits read the step's old values, and ng
+   = 9 keeps that three-stage contamination out of the active cells.  No
+   sources; the user hook (do_user_work_cons) is not a part of the path. */
+void or_rk3_step(or_sim *s, double dt)
+{
+  int lims[6];
+  int lims_tot[6] = {0, s->Ntot[0] - 1, 0, s->Ntot[1] - 1, 0, s->Ntot[2] - 1};
+  double *w = s->prim, *u = s->cons, *b = s->bfld;
+  double *us = s->cons_s, *bs = s->bfld_s, *uss = s->cons_ss, *bss = s->bfld_ss;
+  integr_lims(s, s->reconstr_order, lims);
+  /* I: Us = U + dt L(W, B) */
+  rk3_stage(s, us, u, bs, b, lims, dt);
+  cons2prim_lims(s, w, us, lims);
+  /* II: Uss = 3/4 U + 1/4 (Us + dt L(Ws, Bs)) */
+  rk3_stage(s, uss, us, bss, bs, lims, dt);
+  combine(s, uss, u, uss, 0.75, 0.25, lims, NMODE);
+  combine(s, bss, b, bss, 0.75, 0.25, lims, 3);
+  cons2prim_lims(s, w, uss, lims);
+  /* III: U = 1/3 U + 2/3 (Uss + dt L(Wss, Bss)), Us holding the bracket */
+  rk3_stage(s, us, uss, bs, bss, lims, dt);
+  combine(s, u, u, us, 1. / 3, 1. - 1. / 3, lims, NMODE);
+  combine(s, b, b, bs, 1. / 3, 1. - 1. / 3, lims, 3);
+  cons2prim_lims(s, w, u, lims);
+  or_apply_grid_bc(s);
+  prim2cons_lims(s, u, w, lims_tot);
+}
+
+static void step_any(or_sim *s, double dt)
+{
+  if (s->tintegr) or_rk3_step(s, dt);
+  else or_vl_step(s, dt);
+}
+
+/* vanleer.advance loop: src/integration/vanleer.pyx:87-167 (the RK3 loop of
+   _rk3.pyx:56-70 takes its dt the same way) */
 int or_advance(or_sim *s, double tmax, int max_steps)
 {
   int n = 0;
@@ -1585,7 +1674,7 @@ int or_advance(or_sim *s, double tmax, int max_steps)
     double dt = or_new_dt(s);
     if (s->time + dt > tmax) dt = tmax - s->time;
     s->dt = dt;
-    or_vl_step(s, dt);
+    step_any(s, dt);
     s->time += dt;
     s->step += 1;
     ++n;
@@ -1598,7 +1687,7 @@ int or_advance_steps(or_sim *s, int nsteps)
   for (int n = 0; n < nsteps; ++n) {
     double dt = or_new_dt(s);
     s->dt = dt;
-    or_vl_step(s, dt);
+    step_any(s, dt);
     s->time += dt;
     s->step += 1;
   }

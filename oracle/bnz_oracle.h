@@ -62,6 +62,9 @@ or_sim *or_create(int nx, int ny, int nz, int reconstr_order,
                   const double *lmin, const double *lmax,
                   double gam, double cour, int rsolver, const int *bc_flags,
                   int nthreads);
+or_sim *or_create_ex(int nx, int ny, int nz, int reconstr_order, const double *lmin,
+                     const double *lmax, double gam, double cour, int rsolver,
+                     const int *bc_flags, int nthreads, int tintegr); /* 1: RK3, ng 9 */
 void    or_destroy(or_sim *s);
 int     or_ntot(const or_sim *s, int axis);
 int     or_ng(const or_sim *s);
@@ -81,6 +84,7 @@ void    or_apply_grid_bc(or_sim *s);
 void    or_init(or_sim *s);
 double  or_new_dt(or_sim *s);
 void    or_vl_step(or_sim *s, double dt);
+void    or_rk3_step(or_sim *s, double dt);
 int     or_advance(or_sim *s, double tmax, int max_steps);
 int     or_advance_steps(or_sim *s, int nsteps);
 double  or_time(const or_sim *s);

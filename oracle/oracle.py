@@ -43,6 +43,11 @@ def lib(path=None):
         if path == _LIB_PATH and not os.path.exists(_LIB_PATH):
             build()
         L = ctypes.CDLL(path)
+        L.or_create_ex.restype = ctypes.c_void_p
+        L.or_create_ex.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                   _dp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                   ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_int]
+        L.or_rk3_step.argtypes = [ctypes.c_void_p, ctypes.c_double]
         L.or_create.restype = ctypes.c_void_p
         L.or_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                 _dp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_int,
@@ -215,14 +220,14 @@ class OracleSim:
     def __init__(self, nx, ny=1, nz=1, reconstr_order=1, lmin=(0., 0., 0.),
                  lmax=(1., 1., 1.), gam=5. / 3., cour=0.4, rsolver="hlld",
                  bc=(0, 0, 0, 0, 0, 0), nthreads=None, dt_fms_ref=True, lib_path=None,
-                 char_proj=False, hydro=False, ct_upwind=False):
+                 char_proj=False, hydro=False, ct_upwind=False, tintegr="vl"):
         L = self._L = lib(lib_path)
         lmin = np.asarray(lmin, dtype=np.float64)
         lmax = np.asarray(lmax, dtype=np.float64)
         bcs = (ctypes.c_int * 6)(*bc)
         nthreads = nthreads or min(8, os.cpu_count() or 1)
-        self._h = L.or_create(nx, ny, nz, reconstr_order, _ptr(lmin), _ptr(lmax), gam, cour,
-                              RS_IDS[rsolver], bcs, nthreads)
+        self._h = L.or_create_ex(nx, ny, nz, reconstr_order, _ptr(lmin), _ptr(lmax), gam, cour,
+                                 RS_IDS[rsolver], bcs, nthreads, 1 if tintegr == "rk3" else 0)
         L.or_set_dt_fms_ref(self._h, int(bool(dt_fms_ref)))
         L.or_set_char_proj(self._h, int(bool(char_proj)))
         L.or_set_hydro(self._h, int(bool(hydro)))
@@ -265,6 +270,9 @@ class OracleSim:
 
     def vl_step(self, dt):
         self._L.or_vl_step(self._h, dt)
+
+    def rk3_step(self, dt):
+        self._L.or_rk3_step(self._h, dt)
 
     def advance(self, tmax, max_steps=0):
         return self._L.or_advance(self._h, tmax, max_steps)

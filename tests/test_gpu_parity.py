@@ -41,22 +41,24 @@ NOISE = 1e-9
 
 def run_pair(prob, steps=None, tmax=None, rsolver="hlld", reconstr="linear", bc=(0,) * 6,
              dt_fms_ref=True, cour=0.4, precision="fp64", char_proj=False, hydro=False,
-             ct_upwind=False):
+             ct_upwind=False, tintegr="vl"):
     from bonzo_amd._lib import Context
     from oracle.oracle import OracleSim
     s, prim, bfld = prob
     order = 0 if reconstr == "const" else 1
-    if order == 0:  # ng=2 grid: rebuild arrays without the outermost ghost layer
-        prim, bfld = _regrid(s, prim, bfld, 2)
-        s = P.GridShape(s.nx, s.ny, s.nz, 2, s.lmin, s.lmax)
+    ng = 9 if tintegr == "rk3" else 2 if order == 0 else 3
+    if s.ng != ng:  # rebuild the arrays with the grid's ghost layers
+        prim, bfld = _regrid(s, prim, bfld, ng)
+        s = P.GridShape(s.nx, s.ny, s.nz, ng, s.lmin, s.lmax)
     o = OracleSim(s.nx, s.ny, s.nz, reconstr_order=order, rsolver=rsolver, bc=bc, cour=cour,
-                  dt_fms_ref=dt_fms_ref, char_proj=char_proj, hydro=hydro, ct_upwind=ct_upwind)
+                  dt_fms_ref=dt_fms_ref, char_proj=char_proj, hydro=hydro, ct_upwind=ct_upwind,
+                  tintegr=tintegr)
     o.prim[:] = prim
     o.bfld[:] = bfld
     o.init()
     g = Context(s.nx, s.ny, s.nz, rsolver=rsolver, reconstr=reconstr, bc=bc, cour=cour,
                 dt_fms_ref=dt_fms_ref, precision=precision, char_proj=char_proj, hydro=hydro,
-                ct_upwind=ct_upwind)
+                ct_upwind=ct_upwind, tintegr=tintegr)
     g.upload(prim, bfld)
     g.init_state()
     if steps is not None:
@@ -834,5 +836,46 @@ def test_upwind_corner_emf_matches_oracle(case, precision):
         assert bitwise
     d = res["g"].diag()
     assert d["divb_max"] <= 1e-11
+    o.close()
+    res["g"].close()
+
+
+RK3_CASES = {
+    "turb24_3d": dict(prob=lambda: P.turbulence(24, seed_b=9, seed_v=10), steps=6),
+    "turb20_3d_const_hll": dict(prob=lambda: P.turbulence(20, seed_b=11, seed_v=12), steps=6,
+                                rsolver="hll", reconstr="const"),
+    "ot64_to_tmax": dict(prob=lambda: P.orszag_tang(64), tmax=0.05),
+    "brio_wu_1d": dict(prob=lambda: P.brio_wu(256, axis=0, transverse=0), tmax=0.05,
+                       bc=(1,) * 6),
+    "blast_walls_2d": dict(prob=lambda: _blast_guide_field(40), steps=15, bc=(2,) * 6),
+    "sod_hydro_hllc": dict(prob=lambda: P.sod(200), tmax=0.1, bc=(1,) * 6, rsolver="hllc",
+                           hydro=True),
+    "turb16_3d_upwind": dict(prob=lambda: P.turbulence(16, seed_b=13, seed_v=14), steps=5,
+                             ct_upwind=True),
+}
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp64fma"])
+@pytest.mark.parametrize("case", sorted(RK3_CASES))
+def test_rk3_matches_oracle(case, precision):
+    """tintegr = rk3: the three-stage SSP Runge-Kutta of _rk3.pyx (ng = 9, no
+    ghost fill between the stages) on 1D, 2D and 3D grids, with PLM and
+    donor cell, HLLD, HLL and the hydro build's HLLC, the upwind corner EMF,
+    periodic, outflow and reflecting sides: the IEEE unit equals the oracle
+    bit for bit (time and step count on runs to t_max), the FMA unit within
+    1e-12."""
+    kw = dict(RK3_CASES[case])
+    prob = kw.pop("prob")()
+    say(f"case: rk3, {case}, {precision}")
+    res = run_pair(prob, precision=precision, tintegr="rk3", **kw)
+    o, (sk, sj, si) = res["o"], res["s"].active()
+    assert res["s"].ng == 9
+    assert np.all(np.isfinite(o.prim[:, sk, sj, si]))
+    assert o.prim[0][sk, sj, si].min() > 0 and o.prim[4][sk, sj, si].min() > 0
+    if "tmax" in kw:
+        assert res["n"] == o.step and res["t"] == o.time
+    bitwise = compare(res, f"rk3 {case} {precision}")
+    if precision == "fp64":
+        assert bitwise
     o.close()
     res["g"].close()

@@ -150,11 +150,14 @@ def test_selection_semantics_follow_reference(tmp_path):
     with pytest.raises(ValueError):
         BnzIntegr(_Params(config={"computation": {"cour": 0.4, "rsolver": "hllc"},
                                   "physics": {"gam": 1.4}}))
-    # rk3 is parsed, but the reference's Simulation always runs vanleer.advance
-    with pytest.warns(UserWarning, match="rk3"):
-        it = BnzIntegr(_Params(config={"computation": {"cour": 0.4, "tintegr": "rk3"},
-                                       "physics": {"gam": 1.4}}))
+    # rk3 selects the RK3 integrator, whose grid has ng = 9 (grid.pyx:142-143)
+    it = BnzIntegr(_Params(config={"computation": {"cour": 0.4, "tintegr": "rk3"},
+                                   "physics": {"gam": 1.4}}))
     assert it.tintegr == TIntegr.TINT_RK3
+    from bonzo_amd.simulation import BnzGrid
+    cfg = {"computation": {"cour": 0.4, "tintegr": "rk3", "Nx": 16, "Ny": 16, "Nz": 1},
+           "physics": {"gam": 1.4}}
+    assert BnzGrid(_Params(config=cfg), BnzIntegr(_Params(config=cfg))).ng == 9
 
 
 @pytest.mark.parametrize("key", ["gx", "gy", "gz", "f0", "kappa0", "nuiic0", "kL", "mu", "eta",
@@ -192,9 +195,8 @@ def test_config_dict_with_usr_dir_that_has_no_config_file(tmp_path):
     # with the file present, the dict still overrides it key by key
     (tmp_path / "config.cfg").write_text("[computation]\ncour: 0.2\ntintegr: rk3\n")
     p = _Params(str(tmp_path), cfg)
-    with pytest.warns(UserWarning, match="rk3"):
-        integr = BnzIntegr(p)
-    assert integr.cour == 0.4
+    integr = BnzIntegr(p)
+    assert integr.cour == 0.4 and integr.tintegr == TIntegr.TINT_RK3
 
 
 def test_non_cartesian_geometry_is_refused():
@@ -379,5 +381,24 @@ def test_upwind_emf_selection_and_validation_without_touching_gpu():
     for a in range(3):
         boxes[0].lmax[a] = 1.0
     p = _lib.make_params(ct_upwind=True)
+    assert L.bnz_batch_create(ctypes.byref(p), boxes, 1, ctypes.byref(h)) == _lib.BNZ_EUNSUPPORTED
+    L.bnz_batch_destroy(h)
+
+
+def test_rk3_validation_without_touching_gpu():
+    L = _lib.load()
+    h = ctypes.c_void_p()
+    p = _lib.make_params(nx=16, ny=16, tintegr="rk3", precision="fp32c")
+    assert L.bnz_create(ctypes.byref(p), ctypes.byref(h)) == _lib.BNZ_EUNSUPPORTED
+    L.bnz_destroy(h)
+    p = _lib.make_params(nx=8, ny=8, tintegr="rk3")  # periodic axes need ng = 9 cells
+    assert L.bnz_create(ctypes.byref(p), ctypes.byref(h)) == _lib.BNZ_EINVAL
+    assert b"at least ng" in L.bnz_last_error(h)
+    L.bnz_destroy(h)
+    boxes = (_lib.BnzBox * 1)()
+    boxes[0].nx, boxes[0].ny, boxes[0].nz, boxes[0].tmax = 16, 1, 1, 0.1
+    for a in range(3):
+        boxes[0].lmax[a] = 1.0
+    p = _lib.make_params(tintegr="rk3")
     assert L.bnz_batch_create(ctypes.byref(p), boxes, 1, ctypes.byref(h)) == _lib.BNZ_EUNSUPPORTED
     L.bnz_batch_destroy(h)

@@ -1109,3 +1109,118 @@ def test_upwind_emf_oblique_waves_second_order(dims, sizes, wave):
     errs = [wave_error(dims, n, wave, t, ct_upwind=True) for n in sizes]
     assert errs[0] / errs[1] > 3.2 and errs[1] / errs[2] > 3.2, errs
     assert errs[-1] < 1e-8
+
+
+# ------------------------------------------------------------------ RK3
+# src/integration/_rk3.pyx: Shu & Osher's three-stage SSP Runge-Kutta with
+# ng = 9 and no ghost fill between the stages (oracle or_rk3_step).
+
+def _fixed_dt_run(n, dt, nsteps, tintegr, order=0, wave="fast", dims=2):
+    ng = 9 if tintegr == "rk3" else (2 if order == 0 else 3)
+    s, prim, bfld, _ = P.linear_wave_oblique(n, wave, dims, amp=1e-6, ng=ng)
+    o = orc.OracleSim(s.nx, s.ny, s.nz, reconstr_order=order, rsolver="hlld", tintegr=tintegr)
+    o.prim[:] = prim
+    o.bfld[:] = bfld
+    o.init()
+    for _ in range(nsteps):
+        (o.rk3_step if tintegr == "rk3" else o.vl_step)(dt)
+    sk, sj, si = s.active()
+    out = o.prim[:, sk, sj, si].copy(), o.bfld[:, sk, sj, si].copy()
+    o.close()
+    return out
+
+
+def _state_dist(a, b):
+    return math.sqrt(sum(np.mean(np.abs(a[0][v] - b[0][v])) ** 2 for v in range(5)) +
+                     sum(np.mean(np.abs(a[1][c] - b[1][c])) ** 2 for c in range(3)))
+
+
+@pytest.mark.parametrize("tintegr,order_in_time", [("rk3", 3), ("vl", 2)])
+def test_time_order_of_the_integrators(tintegr, order_in_time):
+    """On a fixed 32^2 grid (donor cell, smooth oblique fast wave) the error
+    against a run with 64x smaller dt falls by 2^p per halving of dt: p = 3
+    for RK3 (a wrong stage weight drops it) and p = 2 for VL."""
+    T = 0.1
+    ref = _fixed_dt_run(32, T / 640, 640, tintegr)
+    errs = [_state_dist(_fixed_dt_run(32, T / m, m, tintegr), ref) for m in (10, 20, 40)]
+    ratios = [errs[i] / errs[i + 1] for i in range(2)]
+    target = 2.0 ** order_in_time
+    assert all(0.85 * target < r < 1.15 * target for r in ratios), (errs, ratios)
+
+
+@pytest.mark.slow
+def test_rk3_oblique_wave_second_order_in_space():
+    """RK3 with PLM: the oblique Alfven wave converges at second order."""
+    errs = []
+    for n in (32, 64, 128):
+        s, prim, bfld, period = P.linear_wave_oblique(n, "alfven", 2, amp=1e-6, ng=9)
+        t = 0.25 * P.linear_wave_oblique(8, "fast", 2)[3]
+        _, pe, be, _ = P.linear_wave_oblique(n, "alfven", 2, amp=1e-6, t=t, ng=9)
+        o = orc.OracleSim(s.nx, s.ny, s.nz, tintegr="rk3")
+        o.prim[:] = prim
+        o.bfld[:] = bfld
+        o.init()
+        o.advance(t)
+        sk, sj, si = s.active()
+        errs.append(_state_dist((o.prim[:, sk, sj, si], o.bfld[:, sk, sj, si]),
+                                (pe[:, sk, sj, si], be[:, sk, sj, si])))
+        o.close()
+    assert errs[0] / errs[1] > 3.2 and errs[1] / errs[2] > 3.2, errs
+
+
+def test_rk3_periodic_shift_is_exact():
+    """With periodic sides the RK3 result of data shifted by 5 cells is the
+    shifted result, bit for bit: the stale ghost values that the three
+    stages read near the grid edges (no fill between stages) never reach
+    the active cells with ng = 9."""
+    s, prim, bfld = P.orszag_tang(32, ng=9)
+    a = orc.OracleSim(32, 32, 1, tintegr="rk3")
+    a.prim[:] = prim
+    a.bfld[:] = bfld
+    a.init()
+    a.advance_steps(6)
+    sk, sj, si = s.active()
+    b = orc.OracleSim(32, 32, 1, tintegr="rk3")
+    b.prim[:, sk, sj, si] = np.roll(prim[:, sk, sj, si], 5, axis=2)
+    b.bfld[:, sk, sj, si] = np.roll(bfld[:, sk, sj, si], 5, axis=2)
+    b.init()
+    b.advance_steps(6)
+    assert a.step == b.step == 6 and a.time == b.time
+    assert np.array_equal(np.roll(a.prim[:, sk, sj, si], 5, axis=2), b.prim[:, sk, sj, si])
+    assert np.array_equal(np.roll(a.bfld[:, sk, sj, si], 5, axis=2), b.bfld[:, sk, sj, si])
+    a.close()
+    b.close()
+
+
+def test_rk3_planar_tube_equals_1d():
+    """A Riemann tube along y of a 3D grid, stepped by RK3 with a fixed dt,
+    equals the 1D RK3 run in every active cell (to round-off)."""
+    rng = np.random.Generator(np.random.PCG64(78))
+    prob1 = P.riemann_tube(40, rng, ng=9)
+    s1 = prob1[0]
+    o1 = orc.OracleSim(s1.nx, 1, 1, bc=(1,) * 6, tintegr="rk3")
+    o1.prim[:] = prob1[1]
+    o1.bfld[:] = prob1[2]
+    o1.init()
+    dt = 0.5 * o1.new_dt()
+    for _ in range(10):
+        o1.rk3_step(dt)
+    s, prim, bfld, bc, ax3 = place_tube(prob1, 3, 1)
+    o = orc.OracleSim(s.nx, s.ny, s.nz, bc=bc, tintegr="rk3")
+    o.prim[:] = prim
+    o.bfld[:] = bfld
+    o.init()
+    for _ in range(10):
+        o.rk3_step(dt)
+    g = s1.ng
+    sk, sj, si = s.active()
+    for v in (0, 4):
+        got = np.moveaxis(o.prim[v][sk, sj, si], 1, -1)
+        ref = o1.prim[v, 0, 0, g:g + s1.nx]
+        assert np.max(np.abs(got - ref)) <= 1e-13 * np.max(np.abs(ref))
+    got = np.moveaxis(o.bfld[ax3[1]][sk, sj, si], 1, -1)
+    ref = o1.bfld[1, 0, 0, g:g + s1.nx]
+    assert np.max(np.abs(got - ref)) <= 1e-13 * np.max(np.abs(ref))
+    assert np.max(np.abs(o1.prim[0, 0, 0, g:g + s1.nx] - prob1[1][0, 0, 0, g:g + s1.nx])) > 1e-2
+    o.close()
+    o1.close()
