@@ -23,7 +23,8 @@ ABI_SYMBOLS = (
     "bnz_create", "bnz_destroy", "bnz_last_error", "bnz_get_shape", "bnz_upload",
     "bnz_download", "bnz_init_state", "bnz_fill_ghosts", "bnz_new_dt", "bnz_vl_step",
     "bnz_advance", "bnz_run_steps", "bnz_get_time", "bnz_set_time", "bnz_diag", "bnz_history",
-    "bnz_synchronize", "bnz_set_tuning", "bnz_set_kernel_timing", "bnz_kernel_timing",
+    "bnz_synchronize", "bnz_set_tuning", "bnz_set_step_graph", "bnz_set_kernel_timing",
+    "bnz_kernel_timing",
     "bnz_state_size", "bnz_save_state", "bnz_load_state",
     "bnz_gpu_flux_pencils", "bnz_hll_flux", "bnz_hlld_flux", "bnz_hllt_flux",
     "bnz_rsolver_rows_noalloc",
@@ -107,6 +108,7 @@ def load():
     L.bnz_save_state.argtypes = [vp, vp, ctypes.c_long]
     L.bnz_load_state.argtypes = [vp, vp, ctypes.c_long]
     L.bnz_set_kernel_timing.argtypes = [vp, ctypes.c_int]
+    L.bnz_set_step_graph.argtypes = [vp, ctypes.c_int]
     L.bnz_set_tuning.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     L.bnz_kernel_timing.argtypes = [vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_int, _dp,
                                     ctypes.POINTER(ctypes.c_long)]
@@ -279,6 +281,11 @@ class Context:
 
     def set_tuning(self, flux_chunk=0, emf_chunk=0, update_chunk=0):
         self._check(load().bnz_set_tuning(self._h, flux_chunk, emf_chunk, update_chunk))
+
+    def set_step_graph(self, on):
+        """True: advance() replays captured chunks of eight steps; False (the
+        default): it launches every kernel directly (bnz_set_step_graph)."""
+        self._check(load().bnz_set_step_graph(self._h, int(bool(on))))
 
     def set_kernel_timing(self, on):
         self._check(load().bnz_set_kernel_timing(self._h, int(bool(on))))

@@ -260,6 +260,25 @@ struct bnz_ctx {
   // RK3 (tintegr = 1): U, U** (conserved) and B** (faces); U* and B* use ws
   // and bs
   double *uc = nullptr, *uss = nullptr, *bss = nullptr;
+  // bnz_advance: a captured chunk of steps, replayed while the buffers,
+  // CFL partial count and work shapes it was captured with are current
+  struct ChunkKey {
+    const void *w, *ws, *b, *bs, *part;
+    int dt_parts, flux_chunk, emf_chunk, update_chunk;
+    bool operator==(const ChunkKey &o) const {
+      return w == o.w && ws == o.ws && b == o.b && bs == o.bs && part == o.part &&
+             dt_parts == o.dt_parts && flux_chunk == o.flux_chunk &&
+             emf_chunk == o.emf_chunk && update_chunk == o.update_chunk;
+    }
+  };
+  bool step_graph = false;  // bnz_set_step_graph
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  ChunkKey gkey{};
+  // a key that a full eager chunk started and ended on: capturing from it
+  // gives a graph that leaves the context as it found it
+  ChunkKey steady{};
+  bool has_steady = false;
   double *geo = nullptr;
   int2 *ghost = nullptr;      // (ghost cell, source cell) pairs, see build_ghosts
   int nghost = 0;

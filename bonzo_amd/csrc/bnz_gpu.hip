@@ -582,6 +582,19 @@ struct StatePart {
   void *dev;
   size_t bytes;
 };
+// bnz_advance's captured chunk: its key and its release
+bnz_ctx::ChunkKey chunk_key(const bnz_ctx *c) {
+  return bnz_ctx::ChunkKey{c->w,          c->ws,         c->b,
+                           c->bs,         c->part,       c->dt_parts,
+                           c->flux_chunk, c->emf_chunk,  c->update_chunk};
+}
+void drop_chunk_graph(bnz_ctx *c) {
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->gexec = nullptr;
+  c->graph = nullptr;
+}
+
 std::vector<StatePart> state_parts(bnz_ctx *c) {
   const size_t n = (size_t)c->ncell;
   if (c->f32)
@@ -691,6 +704,7 @@ int bnz_destroy(bnz_ctx *c) {
                     c->geo, c->dev, c->part, c->rpart, c->uc, c->uss, c->bss};
   for (double *q : bufs)
     if (q) (void)hipFree(q);
+  drop_chunk_graph(c);
   if (c->ghost) (void)hipFree(c->ghost);
   if (c->gops) (void)hipFree(c->gops);
   f32_free(c);
@@ -833,12 +847,55 @@ int bnz_advance(bnz_ctx *c, double tmax, int max_steps, int *nsteps) {
     // Steps go out in chunks; the host reads the time after a chunk while
     // the next one runs.  A step that starts at t >= t_max gets dt = 0 and
     // changes nothing (k_dt_final, k_update), so a chunk may overrun t_max.
+    // With step graphs on, a full chunk in the steady state (kernel timing
+    // off, and buffers, CFL partial count and work shapes the same before
+    // and after a full eager chunk) is captured once in a hipGraph and
+    // replayed: its launches carry only device pointers and sizes, and the
+    // buffers it swaps are back in place after its even number of steps.
+    // Right after an upload, a state load or a change of work shapes the
+    // first chunk runs eagerly.
     constexpr int CHUNK = 8;
     int launched = 0;
     auto chunk = [&](int slot) -> int {
-      for (int r = 0; r < CHUNK && (max_steps <= 0 || launched < max_steps); ++r, ++launched) {
-        any_dt(c, 0, 0.);
-        any_step(c);
+      const bool full = max_steps <= 0 || launched + CHUNK <= max_steps;
+      const bnz_ctx::ChunkKey key = chunk_key(c);
+      const bool replay = c->gexec && key == c->gkey;
+      if (c->step_graph && full && !c->timing &&
+          (replay || (c->has_steady && key == c->steady))) {
+        if (!replay) drop_chunk_graph(c);
+        if (!c->gexec) {
+          HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+          for (int r = 0; r < CHUNK; ++r) {
+            any_dt(c, 0, 0.);
+            any_step(c);
+          }
+          hipGraph_t g = nullptr;
+          const hipError_t e = hipStreamEndCapture(c->stream, &g);
+          if (e != hipSuccess)
+            return fail(c, BNZ_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+          c->graph = g;
+          HIPCHK(c, hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+          c->gkey = key;
+        }
+        HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+        launched += CHUNK;
+        if (!(chunk_key(c) == key)) {
+          // the steps moved the context off the key: this launch holds the
+          // state the host now points at, but the graph cannot be replayed
+          HIPCHK(c, hipStreamSynchronize(c->stream));
+          drop_chunk_graph(c);
+          c->has_steady = false;
+        }
+      } else {
+        int r = 0;
+        for (; r < CHUNK && (max_steps <= 0 || launched < max_steps); ++r, ++launched) {
+          any_dt(c, 0, 0.);
+          any_step(c);
+        }
+        if (r == CHUNK && chunk_key(c) == key) {
+          c->steady = key;
+          c->has_steady = true;
+        }
       }
       int rc = check_launch(c);
       if (rc) return rc;
@@ -1081,6 +1138,13 @@ int bnz_set_tuning(bnz_ctx *c, int flux_chunk, int emf_chunk, int update_chunk) 
     c->part = np;
     c->npart_cap = need;
   }
+  return BNZ_OK;
+}
+
+int bnz_set_step_graph(bnz_ctx *c, int on) {
+  if (!c || on < 0 || on > 1) return BNZ_EINVAL;
+  c->step_graph = on != 0;
+  if (!c->step_graph) drop_chunk_graph(c);
   return BNZ_OK;
 }
 

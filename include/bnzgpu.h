@@ -176,6 +176,15 @@ int bnz_load_state(bnz_ctx *ctx, const void *host, long bytes);
    in the reference. */
 int bnz_set_tuning(bnz_ctx *ctx, int flux_chunk, int emf_chunk, int update_chunk);
 
+/* Step graphs in bnz_advance: on = 1 replays full chunks of eight steps
+   from a captured hipGraph, on = 0 (the default) launches every kernel from
+   the host.  On MI355X a replay measured slower than direct launches on 2D
+   and 3D grids (a step there is bound by the device's per-kernel time, not
+   by host launch cost) and up to 14 % faster on FP32C grids of a few
+   thousand cells.  Results do not depend on the setting.  No counterpart in
+   the reference. */
+int bnz_set_step_graph(bnz_ctx *ctx, int on);
+
 /* Kernel timing on the context's stream: when on, every launch is bracketed
    by HIP events and its duration accumulated per kernel name; switching it
    on clears the table. */

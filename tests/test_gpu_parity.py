@@ -879,3 +879,41 @@ def test_rk3_matches_oracle(case, precision):
         assert bitwise
     o.close()
     res["g"].close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp64fma", "fp32c"])
+def test_advance_graph_equals_eager_steps(prec):
+    """advance() replays a captured chunk of eight steps once a full chunk
+    has started and ended on the same buffers, CFL partial count and work
+    shapes (step graph on); with the step graph off it issues every launch
+    itself.  Both give the same state, time and step count bit for bit, on
+    1D (fused step), 2D and 3D grids and with RK3, including a run that ends
+    inside a chunk (max_steps), one that reaches t_max, one resumed from a
+    reloaded state (fresh CFL partials) and one after a change of work
+    shapes."""
+    from bonzo_amd._lib import Context
+    say(f"case: advance graph vs eager launches, {prec}")
+    probs = [("tube", P.brio_wu(200, axis=0, transverse=0), (1,) * 6, "vl"),
+             ("ot", P.orszag_tang(48), (0,) * 6, "vl"),
+             ("turb", P.turbulence(16, seed_b=3, seed_v=4), (0,) * 6, "vl")]
+    if prec != "fp32c":
+        probs.append(("turb_rk3", P.turbulence(16, seed_b=5, seed_v=6, ng=9), (0,) * 6, "rk3"))
+    for name, (s, prim, bfld), bc, tint in probs:
+        out = []
+        for on in (True, False):
+            g = Context(s.nx, s.ny, s.nz, bc=bc, precision=prec, tintegr=tint)
+            g.set_step_graph(on)
+            g.upload(prim, bfld)
+            g.init_state()
+            g.advance(0.03, max_steps=21)
+            g.advance(0.045)
+            g.load_state(g.save_state())
+            g.advance(0.06)
+            if s.nz > 1:
+                g.set_tuning(8, 0, 16)
+            g.advance(0.075)
+            out.append(g.download() + (g.time(),))
+            g.close()
+        (pa, ba, ta), (pb, bb, tb) = out
+        assert ta == tb, (name, ta, tb)
+        assert np.array_equal(pa, pb) and np.array_equal(ba, bb), name
