@@ -23,7 +23,8 @@ ABI_SYMBOLS = (
     "bnz_create", "bnz_destroy", "bnz_last_error", "bnz_get_shape", "bnz_upload",
     "bnz_download", "bnz_init_state", "bnz_fill_ghosts", "bnz_new_dt", "bnz_vl_step",
     "bnz_advance", "bnz_run_steps", "bnz_get_time", "bnz_set_time", "bnz_diag", "bnz_history",
-    "bnz_synchronize", "bnz_set_tuning", "bnz_set_step_graph", "bnz_set_kernel_timing",
+    "bnz_synchronize", "bnz_set_tuning", "bnz_set_user_bc", "bnz_set_step_graph",
+    "bnz_set_kernel_timing",
     "bnz_kernel_timing",
     "bnz_state_size", "bnz_save_state", "bnz_load_state",
     "bnz_gpu_flux_pencils", "bnz_hll_flux", "bnz_hlld_flux", "bnz_hllt_flux",
@@ -109,6 +110,8 @@ def load():
     L.bnz_load_state.argtypes = [vp, vp, ctypes.c_long]
     L.bnz_set_kernel_timing.argtypes = [vp, ctypes.c_int]
     L.bnz_set_step_graph.argtypes = [vp, ctypes.c_int]
+    L.bnz_set_user_bc.argtypes = [vp, ctypes.c_int, ctypes.c_int,
+                                  ctypes.POINTER(ctypes.c_int), _dp]
     L.bnz_set_tuning.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     L.bnz_kernel_timing.argtypes = [vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_int, _dp,
                                     ctypes.POINTER(ctypes.c_long)]
@@ -281,6 +284,17 @@ class Context:
 
     def set_tuning(self, flux_chunk=0, emf_chunk=0, update_chunk=0):
         self._check(load().bnz_set_tuning(self._h, flux_chunk, emf_chunk, update_chunk))
+
+    def set_user_bc(self, axis, side, spec):
+        """User boundary (flag 3) of one side: a user_bc.Fixed or
+        user_bc.Mapped spec, or raw (src, fixed) arrays in the layout of
+        bnz_set_user_bc."""
+        from .user_bc import side_arrays
+        dims = 3 if self.ntot[2] > 1 else 2 if self.ntot[1] > 1 else 1
+        src, fixed = side_arrays(spec, self.ntot, self.ng, dims, axis, side)
+        fp = _ptr(np.ascontiguousarray(fixed)) if fixed is not None else None
+        self._check(load().bnz_set_user_bc(
+            self._h, axis, side, src.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), fp))
 
     def set_step_graph(self, on):
         """True: advance() replays captured chunks of eight steps; False (the

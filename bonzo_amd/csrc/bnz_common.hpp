@@ -65,11 +65,13 @@ struct GeoUni {
 
 struct GhostOp {
   int dst;       // cell written
-  int src;       // source of the cell variables
+  int src;       // source of the cell variables (with bit 16: fixed-value row)
   int srcf[3];   // source of each face component
   unsigned bits; // bit v < 9: negate slot v; 9 + a: negate face a;
-                 // 12 + a: face a = 0; 15: cell variables untouched
+                 // 12 + a: face a = 0; 15: cell variables untouched;
+                 // 16: every value from row src of the fixed-value table
 };
+constexpr unsigned GHOST_FIXED = 1u << 16;
 
 struct Lims {
   int lo[3], hi[3];  // inclusive cell limits of get_integr_lims (integrator.pyx:375-383)
@@ -284,6 +286,18 @@ struct bnz_ctx {
   int nghost = 0;
   bnz::GhostOp *gops = nullptr;  // ghost cells touched by a reflecting side
   int ngops = 0;
+  // User boundaries (flag 3, bnz_set_user_bc): with one on any side the fill
+  // runs as one gather pass per side in the order z1, z2, y1, y2, x1, x2
+  // (gops[pass_off[p] .. pass_off[p + 1])), each pass reading what the
+  // earlier ones wrote.  Per side: the caller's slab entries (source cell,
+  // -1 fixed values, -2 untouched) and NMODE + 3 fixed values per entry.
+  std::vector<int> usr_src[6];
+  std::vector<double> usr_fix[6];
+  bool usr_set[6] = {false, false, false, false, false, false};
+  int npass = 0;
+  int pass_off[7] = {0, 0, 0, 0, 0, 0, 0};
+  double *ufix = nullptr;  // fixed values in use: NMODE prim + 3 faces each
+  float *ufix32 = nullptr; // their FP32C state: 6 conserved hi words + NMODE prim
   double *dev = nullptr;      // dt, t, tmax, step
   double *part = nullptr;     // CFL partials (the last corrector's, kept for the next dt)
   long npart_cap = 0;
@@ -483,6 +497,7 @@ inline void build_ghosts(const int nt[3], int dims, int ng, const int bc[6], lon
     const int a1 = ng, a2 = N - ng - 1;
     for (int side = 0; side < 2; ++side) {
       const int flag = bc[2 * a + side];
+      if (flag < 0) continue;  // side left alone (one-side passes)
       for (int g = 0; g < ng; ++g) {
         const int idx = side == 0 ? a1 - ng + g : a2 + 1 + g;
         if (flag == BNZ_BC_PERIODIC) {
@@ -549,6 +564,7 @@ void f32_free(bnz_ctx *c);
 int f32_from64(bnz_ctx *c);   // state from the FP64 prim/face arrays (all cells)
 int f32_to64(bnz_ctx *c);     // FP64 prim/face arrays from the state
 void f32_ghost(bnz_ctx *c);
+int f32_fixed_rows(bnz_ctx *c, int rows);  // FP32C state of the user fixed values
 void f32_dt(bnz_ctx *c, int use_fixed, double fixed_dt);
 void f32_step_body(bnz_ctx *c);
 // shared launches (bnz_gpu.hip)

@@ -51,23 +51,36 @@ __global__ __launch_bounds__(BLK) void k_ghost(Geo g, double *w, double *b,
 }
 
 // Ghost cells a reflecting side touches (see GhostOp): per-class sources,
-// sign flips and zeroed wall faces.
+// sign flips and zeroed wall faces; with user boundaries, every entry of a
+// one-side pass, fixed values (rows of fix, NMODE + 3 each) included.
+// bonly stores only the cell-centred field (the second fill of the init).
 __global__ __launch_bounds__(BLK) void k_ghost_ops(Geo g, double *w, double *b,
-                                                   const GhostOp *__restrict__ ops, int nops) {
+                                                   const GhostOp *__restrict__ ops, int nops,
+                                                   const double *__restrict__ fix, int bonly) {
   const long t = xcd_block() * BLK + threadIdx.x;
   if (t >= nops) return;
   const GhostOp o = ops[t];
   const long sv = g.sv;
-  const bool cells = !(o.bits & (1u << 15));
   double v[NMODE], f[3];
+  bool cells = !(o.bits & (1u << 15));
+  if (o.bits & GHOST_FIXED) {
+    const double *r = fix + (long)o.src * (NMODE + 3);
 #pragma unroll
-  for (int n = 0; n < NMODE; ++n) v[n] = cells ? w[n * sv + o.src] : 0.;
+    for (int n = 0; n < NMODE; ++n) v[n] = r[n];
 #pragma unroll
-  for (int a = 0; a < 3; ++a) f[a] = ((o.bits >> (12 + a)) & 1u) ? 0. : b[a * sv + o.srcf[a]];
+    for (int a = 0; a < 3; ++a) f[a] = r[NMODE + a];
+  } else {
+#pragma unroll
+    for (int n = 0; n < NMODE; ++n) v[n] = cells ? w[n * sv + o.src] : 0.;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) f[a] = ((o.bits >> (12 + a)) & 1u) ? 0. : b[a * sv + o.srcf[a]];
+  }
   if (cells) {
 #pragma unroll
-    for (int n = 0; n < NMODE; ++n) w[n * sv + o.dst] = ((o.bits >> n) & 1u) ? -v[n] : v[n];
+    for (int n = 0; n < NMODE; ++n)
+      if (!bonly || n >= BX) w[n * sv + o.dst] = ((o.bits >> n) & 1u) ? -v[n] : v[n];
   }
+  if (bonly) return;
 #pragma unroll
   for (int a = 0; a < 3; ++a) b[a * sv + o.dst] = ((o.bits >> (9 + a)) & 1u) ? -f[a] : f[a];
 }
@@ -391,14 +404,23 @@ void set_coordinates(bnz_ctx *c) {
   }
 }
 
-void launch_ghost(bnz_ctx *c) {
+void launch_ghost(bnz_ctx *c, int bonly = 0) {
   Launch t(c, "ghost");
+  if (c->npass > 0) {  // user boundaries: one pass per side, in order
+    for (int q = 0; q < 6; ++q) {
+      const int n = c->pass_off[q + 1] - c->pass_off[q];
+      if (n > 0)
+        hipLaunchKernelGGL(k_ghost_ops, dim3(blocks_for(n)), dim3(BLK), 0, c->stream, c->g, c->w,
+                           c->b, c->gops + c->pass_off[q], n, c->ufix, bonly);
+    }
+    return;
+  }
   if (c->nghost > 0)
     hipLaunchKernelGGL(k_ghost, dim3(blocks_for(c->nghost)), dim3(BLK), 0, c->stream, c->g, c->w,
                        c->b, c->ghost, c->nghost);
   if (c->ngops > 0)
     hipLaunchKernelGGL(k_ghost_ops, dim3(blocks_for(c->ngops)), dim3(BLK), 0, c->stream, c->g,
-                       c->w, c->b, c->gops, c->ngops);
+                       c->w, c->b, c->gops, c->ngops, (const double *)nullptr, 0);
 }
 
 void launch_dt(bnz_ctx *c, int use_fixed, double fixed_dt) {
@@ -489,9 +511,10 @@ int configure_grid(bnz_ctx *c, const bnz_params *p) {
   if (p->char_proj && p->precision == BNZ_PREC_FP32C)
     return fail(c, BNZ_EUNSUPPORTED, "characteristic limiting runs in FP64 only");
   for (int q = 0; q < 6; ++q)
-    if (p->bc[q] != BNZ_BC_PERIODIC && p->bc[q] != BNZ_BC_OUTFLOW && p->bc[q] != BNZ_BC_REFLECT)
+    if (p->bc[q] != BNZ_BC_PERIODIC && p->bc[q] != BNZ_BC_OUTFLOW &&
+        p->bc[q] != BNZ_BC_REFLECT && p->bc[q] != BNZ_BC_USER)
       return fail(c, BNZ_EUNSUPPORTED,
-                 "boundary flags must be 0 (periodic), 1 (outflow) or 2 (reflect)");
+                 "boundary flags must be 0 (periodic), 1 (outflow), 2 (reflect) or 3 (user)");
   c->dims = (p->nz > 1) ? 3 : (p->ny > 1) ? 2 : 1;
   c->order = (p->reconstr == BNZ_RCN_CONST) ? 0 : 1;
   // grid.pyx:129-143: RK3's stages run without a ghost fill between them
@@ -506,9 +529,9 @@ int configure_grid(bnz_ctx *c, const bnz_params *p) {
   if (c->ncell >= (1L << 31)) return fail(c, BNZ_EINVAL, "grid too large: cell indices are 32-bit");
   {
     const int nact[3] = {p->nx, p->ny, p->nz};
+    auto copies = [](int f) { return f == BNZ_BC_PERIODIC || f == BNZ_BC_REFLECT; };
     for (int a = 0; a < c->dims; ++a)
-      if ((p->bc[2 * a] != BNZ_BC_OUTFLOW || p->bc[2 * a + 1] != BNZ_BC_OUTFLOW) &&
-          nact[a] < c->ng)
+      if ((copies(p->bc[2 * a]) || copies(p->bc[2 * a + 1])) && nact[a] < c->ng)
         // with fewer active cells than ghost layers the reference's periodic
         // and reflect copies read ghost cells they have not filled yet
         return fail(c, BNZ_EINVAL,
@@ -595,6 +618,156 @@ void drop_chunk_graph(bnz_ctx *c) {
   c->graph = nullptr;
 }
 
+// User boundaries.  Slab of side q = 2 * axis + side: the ng ghost layers
+// along the axis times the full extent of the other axes.
+Box user_slab(const bnz_ctx *c, int q) {
+  Box s;
+  for (int d = 0; d < 3; ++d) {
+    s.lo[d] = 0;
+    s.n[d] = c->nt[d];
+  }
+  const int a = q / 2;
+  s.lo[a] = (q & 1) ? c->nt[a] - c->ng : 0;
+  s.n[a] = c->ng;
+  return s;
+}
+
+// One gather pass per side, in the order of GridBC.apply_grid_bc (z, then
+// y, then x, left before right: src/bc/grid_bc.pyx:612-623): a built-in side
+// from build_ghosts with every other side left alone (its copies over the
+// full extent of the other axes, as the per-side functions of
+// grid_bc_funcs.pyx run), a user side from its slab entries.  No entry of a
+// pass reads a cell the pass writes (built-in sources are active along the
+// axis; bnz_set_user_bc checks user sources), so each pass runs its entries
+// in any order.
+//   A pass may read a ghost cell that a later pass writes: it then gets what
+// the cell held between fills, which depends on how a step's stages use the
+// ghost cells (the reference's predictor writes them in place), not on the
+// boundaries.  Such a value may be overwritten later in the fill, but it may
+// not end up in a ghost cell: every ghost cell's final value must trace back
+// to active cells or fixed values through earlier passes ("clean").
+int check_clean(bnz_ctx *c, const std::vector<GhostOp> &all, const int off[7]) {
+  std::vector<signed char> st((size_t)c->ncell, -1);  // -1 unwritten, 0 stale, 1 clean
+  const int lo[3] = {c->i1, c->j1, c->k1}, hi[3] = {c->i2, c->j2, c->k2};
+  auto active = [&](long q) {
+    const int i = (int)(q % c->nt[0]), j = (int)((q / c->nt[0]) % c->nt[1]),
+              k = (int)(q / ((long)c->nt[0] * c->nt[1]));
+    return i >= lo[0] && i <= hi[0] && j >= lo[1] && j <= hi[1] && k >= lo[2] && k <= hi[2];
+  };
+  auto clean_src = [&](int q) { return active(q) || st[q] == 1; };
+  std::vector<signed char> res;
+  for (int t = 0; t < 6; ++t) {
+    res.assign(off[t + 1] - off[t], 0);
+    for (int e = off[t]; e < off[t + 1]; ++e) {
+      const GhostOp &o = all[e];
+      bool ok = (o.bits & GHOST_FIXED) != 0;
+      if (!ok) {
+        ok = (o.bits & (1u << 15)) || clean_src(o.src);
+        for (int a = 0; a < 3; ++a) ok = ok && (((o.bits >> (12 + a)) & 1u) || clean_src(o.srcf[a]));
+      }
+      res[e - off[t]] = ok ? 1 : 0;
+    }
+    for (int e = off[t]; e < off[t + 1]; ++e) st[all[e].dst] = res[e - off[t]];
+  }
+  for (long q = 0; q < c->ncell; ++q)
+    if (!active(q) && st[q] != 1) {
+      const int i = (int)(q % c->nt[0]), j = (int)((q / c->nt[0]) % c->nt[1]),
+                k = (int)(q / ((long)c->nt[0] * c->nt[1]));
+      return fail(c, BNZ_EINVAL,
+                  std::string("user boundaries: ghost cell (") + std::to_string(i) + ", " +
+                      std::to_string(j) + ", " + std::to_string(k) + ") " +
+                      (st[q] < 0 ? "is written by no side"
+                                 : "would take a value read from a ghost cell that a later side "
+                                   "fills (sides fill z1, z2, y1, y2, x1, x2 in turn)"));
+    }
+  return BNZ_OK;
+}
+
+int build_passes(bnz_ctx *c) {
+  static const int order[6] = {4, 5, 2, 3, 0, 1};
+  std::vector<GhostOp> all;
+  std::vector<double> fix;
+  int off[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (int t = 0; t < 6; ++t) {
+    const int q = order[t];
+    off[t] = (int)all.size();
+    if (q / 2 >= c->dims) continue;
+    const int flag = c->p.bc[q];
+    if (flag != BNZ_BC_USER) {
+      int one[6] = {-1, -1, -1, -1, -1, -1};
+      one[q] = flag;
+      std::vector<int2> pairs;
+      std::vector<GhostOp> ops;
+      build_ghosts(c->nt, c->dims, c->ng, one, 0, pairs, ops);
+      for (const int2 &d : pairs) {
+        GhostOp o;
+        o.dst = d.x;
+        o.src = o.srcf[0] = o.srcf[1] = o.srcf[2] = d.y;
+        o.bits = 0;
+        all.push_back(o);
+      }
+      all.insert(all.end(), ops.begin(), ops.end());
+      continue;
+    }
+    const Box sl = user_slab(c, q);
+    const std::vector<int> &src = c->usr_src[q];
+    for (long e = 0; e < sl.total(); ++e) {
+      if (src[e] == -2) continue;
+      const int i = sl.lo[0] + (int)(e % sl.n[0]);
+      const int j = sl.lo[1] + (int)((e / sl.n[0]) % sl.n[1]);
+      const int k = sl.lo[2] + (int)(e / ((long)sl.n[0] * sl.n[1]));
+      GhostOp o;
+      o.dst = (int)(((long)k * c->nt[1] + j) * c->nt[0] + i);
+      o.bits = 0;
+      if (src[e] >= 0) {
+        o.src = o.srcf[0] = o.srcf[1] = o.srcf[2] = src[e];
+      } else {
+        o.src = (int)(fix.size() / (NMODE + 3));
+        o.srcf[0] = o.srcf[1] = o.srcf[2] = 0;
+        o.bits = GHOST_FIXED;
+        fix.insert(fix.end(), c->usr_fix[q].begin() + e * (NMODE + 3),
+                   c->usr_fix[q].begin() + (e + 1) * (NMODE + 3));
+      }
+      all.push_back(o);
+    }
+  }
+  off[6] = (int)all.size();
+  if (const int rc = check_clean(c, all, off)) return rc;
+  // the lists may be in use by queued fills
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  drop_chunk_graph(c);
+  c->has_steady = false;
+  if (c->gops) (void)hipFree(c->gops);
+  if (c->ufix) (void)hipFree(c->ufix);
+  c->gops = nullptr;
+  c->ufix = nullptr;
+  HIPCHK(c, hipMalloc(&c->gops, std::max<size_t>(1, all.size()) * sizeof(GhostOp)));
+  if (!all.empty())
+    HIPCHK(c, hipMemcpy(c->gops, all.data(), all.size() * sizeof(GhostOp), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMalloc(&c->ufix, std::max<size_t>(1, fix.size()) * sizeof(double)));
+  if (!fix.empty())
+    HIPCHK(c, hipMemcpy(c->ufix, fix.data(), fix.size() * sizeof(double), hipMemcpyHostToDevice));
+  if (c->f32) {
+    const int rc = f32_fixed_rows(c, (int)(fix.size() / (NMODE + 3)));
+    if (rc) return rc;
+  }
+  c->ngops = (int)all.size();
+  c->nghost = 0;
+  for (int t = 0; t < 7; ++t) c->pass_off[t] = off[t];
+  c->npass = 6;
+  return BNZ_OK;
+}
+
+int user_sides_ready(bnz_ctx *c) {
+  static const char *names[6] = {"x1", "x2", "y1", "y2", "z1", "z2"};
+  for (int q = 0; q < 2 * c->dims; ++q)
+    if (c->p.bc[q] == BNZ_BC_USER && !c->usr_set[q])
+      return fail(c, BNZ_ESTATE,
+                  std::string("user boundary of side ") + names[q] +
+                      " not set (bnz_set_user_bc)");
+  return BNZ_OK;
+}
+
 std::vector<StatePart> state_parts(bnz_ctx *c) {
   const size_t n = (size_t)c->ncell;
   if (c->f32)
@@ -658,9 +831,12 @@ int bnz_create(const bnz_params *p, bnz_ctx **out) {
 
   {
     // (ghost, source) pairs of every ghost cell, from the per-axis maps
+    // (user sides are left alone here: bnz_set_user_bc builds the passes)
     std::vector<int2> pairs;
     std::vector<GhostOp> ops;
-    build_ghosts(c->nt, c->dims, c->ng, p->bc, 0, pairs, ops);
+    int bc0[6];
+    for (int q = 0; q < 6; ++q) bc0[q] = p->bc[q] == BNZ_BC_USER ? -1 : p->bc[q];
+    build_ghosts(c->nt, c->dims, c->ng, bc0, 0, pairs, ops);
     c->nghost = (int)pairs.size();
     c->ngops = (int)ops.size();
     HIPCHK(c, hipMalloc(&c->ghost, std::max<size_t>(1, pairs.size()) * sizeof(int2)));
@@ -707,6 +883,7 @@ int bnz_destroy(bnz_ctx *c) {
   drop_chunk_graph(c);
   if (c->ghost) (void)hipFree(c->ghost);
   if (c->gops) (void)hipFree(c->gops);
+  if (c->ufix) (void)hipFree(c->ufix);
   f32_free(c);
   for (hipEvent_t e : c->hev)
     if (e) (void)hipEventDestroy(e);
@@ -777,6 +954,7 @@ int bnz_download(bnz_ctx *c, double *prim, double *bfld, double *cons) {
 
 int bnz_fill_ghosts(bnz_ctx *c) {
   if (!c || !c->stream) return BNZ_EINVAL;
+  if (const int rc = user_sides_ready(c)) return rc;
   if (c->f32 && c->ready) f32_ghost(c);
   else launch_ghost(c);
   return check_launch(c);
@@ -784,12 +962,15 @@ int bnz_fill_ghosts(bnz_ctx *c) {
 
 int bnz_init_state(bnz_ctx *c) {
   if (!c || !c->stream) return BNZ_EINVAL;
+  if (const int rc = user_sides_ready(c)) return rc;
   c->dt_parts = 0;
   launch_ghost(c);
   Box a = active_box(c);
   hipLaunchKernelGGL(k_interp_active, dim3(blocks_for(a.total())), dim3(BLK), 0, c->stream,
                      c->g, c->w, c->b, a);
-  launch_ghost(c);
+  // the cell-centred field again (vanleer.pyx:44-46); without user sides
+  // every source is active and refilling all variables gives the same
+  launch_ghost(c, 1);
   if (c->f32) {
     int rc = f32_from64(c);
     if (rc) return rc;
@@ -1139,6 +1320,55 @@ int bnz_set_tuning(bnz_ctx *c, int flux_chunk, int emf_chunk, int update_chunk) 
     c->npart_cap = need;
   }
   return BNZ_OK;
+}
+
+int bnz_set_user_bc(bnz_ctx *c, int axis, int side, const int *src, const double *fixed) {
+  if (!c || !c->stream) return BNZ_EINVAL;
+  if (axis < 0 || axis >= c->dims || side < 0 || side > 1 || !src)
+    return fail(c, BNZ_EINVAL, "user boundary: axis must be below the grid's dimension, side 0 or 1");
+  const int q = 2 * axis + side;
+  if (c->p.bc[q] != BNZ_BC_USER)
+    return fail(c, BNZ_EINVAL, "user boundary: the side's flag is not 3 (BNZ_BC_USER)");
+  const Box sl = user_slab(c, q);
+  const long n = sl.total();
+  // a source may not be a cell this side writes: the pass gathers in any order
+  std::vector<char> written(n, 0);
+  bool any_fixed = false;
+  for (long e = 0; e < n; ++e) {
+    if (src[e] < -2 || src[e] >= c->ncell)
+      return fail(c, BNZ_EINVAL, "user boundary: source index out of range");
+    written[e] = src[e] != -2;
+    any_fixed |= src[e] == -1;
+  }
+  if (any_fixed && !fixed)
+    return fail(c, BNZ_EINVAL, "user boundary: fixed values missing for entries marked -1");
+  for (long e = 0; e < n; ++e) {
+    if (src[e] < 0) continue;
+    const long s = src[e];
+    const int i = (int)(s % c->nt[0]), j = (int)((s / c->nt[0]) % c->nt[1]),
+              k = (int)(s / ((long)c->nt[0] * c->nt[1]));
+    const int id[3] = {i - sl.lo[0], j - sl.lo[1], k - sl.lo[2]};
+    if (id[0] >= 0 && id[0] < sl.n[0] && id[1] >= 0 && id[1] < sl.n[1] && id[2] >= 0 &&
+        id[2] < sl.n[2] && written[((long)id[2] * sl.n[1] + id[1]) * sl.n[0] + id[0]])
+      return fail(c, BNZ_EINVAL, "user boundary: a source is a cell the same side writes");
+  }
+  // a refused side leaves the boundaries as they were
+  std::vector<int> old_src = std::move(c->usr_src[q]);
+  std::vector<double> old_fix = std::move(c->usr_fix[q]);
+  const bool old_set = c->usr_set[q];
+  c->usr_src[q].assign(src, src + n);
+  if (fixed) c->usr_fix[q].assign(fixed, fixed + n * (NMODE + 3));
+  else c->usr_fix[q].assign(n * (NMODE + 3), 0.);
+  c->usr_set[q] = true;
+  for (int r = 0; r < 2 * c->dims; ++r)
+    if (c->p.bc[r] == BNZ_BC_USER && !c->usr_set[r]) return BNZ_OK;
+  const int rc = build_passes(c);
+  if (rc) {
+    c->usr_src[q] = std::move(old_src);
+    c->usr_fix[q] = std::move(old_fix);
+    c->usr_set[q] = old_set;
+  }
+  return rc;
 }
 
 int bnz_set_step_graph(bnz_ctx *c, int on) {

@@ -134,6 +134,22 @@ __global__ __launch_bounds__(BLK) void k32_ghost_ops(Geo g, Ghost32 p,
   ghost32_op(p, ops[t]);
 }
 
+// FP32C state of the user fixed-value rows (NMODE prim + 3 faces each): the
+// conserved hi words and FP32 prim a cell of that prim gets from k32_from64.
+__global__ __launch_bounds__(BLK) void k32_fixed_rows(const double *__restrict__ fix,
+                                                      float *fix32, int rows, double gamm1i) {
+  const long t = (long)blockIdx.x * BLK + threadIdx.x;
+  if (t >= rows) return;
+  double wv[NMODE], u[NMODE];
+#pragma unroll
+  for (int n = 0; n < NMODE; ++n) wv[n] = fix[t * (NMODE + 3) + n];
+  prim2cons(u, wv, gamm1i);
+#pragma unroll
+  for (int n = 0; n < 6; ++n) fix32[t * NGHOST32 + n] = (float)u[n];
+#pragma unroll
+  for (int n = 0; n < NMODE; ++n) fix32[t * NGHOST32 + 6 + n] = (float)wv[n];
+}
+
 // FP64 prim (all cells) -> FP32-compensated state.
 __global__ __launch_bounds__(BLK) void k32_from64(Geo g, const double *__restrict__ w, float *uh,
                                                   float *ul, float *w32) {
@@ -206,7 +222,18 @@ void ghost32(bnz_ctx *c) {
   for (int n = 0; n < 6; ++n) p.a[q++] = c->uh + n * c->ncell;
   for (int n = 0; n < 9; ++n) p.a[q++] = c->w32 + n * c->ncell;
   for (int n = 0; n < 3; ++n) p.d[n] = c->b + n * c->ncell;
+  p.fix32 = c->ufix32;
+  p.fix64 = c->ufix;
   Launch t(c, "ghost");
+  if (c->npass > 0) {  // user boundaries: one pass per side, in order
+    for (int s = 0; s < 6; ++s) {
+      const int n = c->pass_off[s + 1] - c->pass_off[s];
+      if (n > 0)
+        hipLaunchKernelGGL(k32_ghost_ops, dim3(blocks_for(n)), dim3(BLK), 0, c->stream, c->g, p,
+                           c->gops + c->pass_off[s], n);
+    }
+    return;
+  }
   if (c->nghost > 0)
     hipLaunchKernelGGL(k32_ghost, dim3(blocks_for(c->nghost)), dim3(BLK), 0, c->stream, c->g, p,
                        c->ghost, c->nghost);
@@ -235,7 +262,8 @@ int f32_alloc(bnz_ctx *c) {
 }
 
 void f32_free(bnz_ctx *c) {
-  float *bufs[] = {c->uh, c->ul, c->w32, c->ws32, c->bs32, c->fx32, c->fy32, c->fz32, c->ee32};
+  float *bufs[] = {c->uh,   c->ul,   c->w32,  c->ws32,  c->bs32,
+                   c->fx32, c->fy32, c->fz32, c->ee32, c->ufix32};
   for (float *q : bufs)
     if (q) (void)hipFree(q);
 }
@@ -256,6 +284,17 @@ int f32_to64(bnz_ctx *c) {
 }
 
 void f32_ghost(bnz_ctx *c) { ghost32(c); }
+
+int f32_fixed_rows(bnz_ctx *c, int rows) {
+  if (c->ufix32) (void)hipFree(c->ufix32);
+  c->ufix32 = nullptr;
+  HIPCHK(c, hipMalloc(&c->ufix32, (size_t)std::max(1, rows) * NGHOST32 * sizeof(float)));
+  if (rows > 0)
+    hipLaunchKernelGGL(k32_fixed_rows, dim3(blocks_for(rows)), dim3(BLK), 0, c->stream, c->ufix,
+                       c->ufix32, rows, c->g.gamm1i);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return check_launch(c);
+}
 
 void f32_dt(bnz_ctx *c, int use_fixed, double fixed_dt) {
   int nb = c->dt_parts;

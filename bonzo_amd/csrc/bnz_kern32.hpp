@@ -530,6 +530,8 @@ constexpr int NGHOST32 = 6 + NMODE;
 struct Ghost32 {
   float *a[NGHOST32];  // uh (6), w32 (9)
   double *d[3];        // face fields
+  const float *fix32;  // user fixed values: NGHOST32 state words per row
+  const double *fix64; // and NMODE + 3 FP64 values (faces at NMODE..)
 };
 
 // One (ghost, source) pair: sources are active cells the fill never writes,
@@ -554,10 +556,17 @@ __device__ __forceinline__ void ghost32_op(const Ghost32 &p, const GhostOp &o) {
   const bool cells = !(o.bits & (1u << 15));
   float v[NGHOST32];
   double f[3];
+  if (o.bits & GHOST_FIXED) {
 #pragma unroll
-  for (int n = 0; n < NGHOST32; ++n) v[n] = cells ? p.a[n][o.src] : 0.f;
+    for (int n = 0; n < NGHOST32; ++n) v[n] = p.fix32[(long)o.src * NGHOST32 + n];
 #pragma unroll
-  for (int a = 0; a < 3; ++a) f[a] = ((o.bits >> (12 + a)) & 1u) ? 0. : p.d[a][o.srcf[a]];
+    for (int a = 0; a < 3; ++a) f[a] = p.fix64[(long)o.src * (NMODE + 3) + NMODE + a];
+  } else {
+#pragma unroll
+    for (int n = 0; n < NGHOST32; ++n) v[n] = cells ? p.a[n][o.src] : 0.f;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) f[a] = ((o.bits >> (12 + a)) & 1u) ? 0. : p.d[a][o.srcf[a]];
+  }
   if (cells) {
 #pragma unroll
     for (int n = 0; n < NGHOST32; ++n) {

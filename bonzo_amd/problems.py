@@ -432,6 +432,82 @@ def blast2d(n, ng=3, seed=5005, p_in=10.0, p_out=0.1, radius=0.1, theta=None, bm
     return s, prim, bfld
 
 
+def _rot12(a):
+    """Tube frame -> grid frame of the oblique tube: the tube's x axis along
+    (1, 2)/sqrt 5 (st2d.pyx:23-30 is the same rotation)."""
+    r5 = math.sqrt(5.0)
+    return np.array([(a[0] - 2.0 * a[1]) / r5, (2.0 * a[0] + a[1]) / r5, a[2]])
+
+
+def oblique_tube(ny, ng=3, aspect=4):
+    """The reference's 2D oblique shock tube (src/problem/st2d.pyx): the
+    Ryu-Jones 2a Riemann problem (left rho=1.08, v=(1.2, 0.01, 0.5), p=0.95,
+    B=(2, 3.6, 2)/sqrt(4 pi); right rho=1, v=0, p=1, B=(2, 4, 2)/sqrt(4 pi))
+    with its normal along (1, 2)/sqrt 5 on an (aspect ny) x ny grid of
+    [0, aspect] x [0, 1] (square cells), the interface through the box
+    centre.  With aspect 4 the interface stays ny cells away from both x
+    sides, whose fixed states are the far-field ones.  Every cell, ghosts
+    included, is set from the integer test i' + 2 j' <= nx/2 + ny - 2
+    (i', j' counted from the first active cell), so the state is exactly
+    invariant under (i, j) -> (i - 2, j + 1).  Face fields are differences of
+    the corner potential A_z = J' dx (Bx + 2 By) + g(I' + 2 J'), with g
+    continuous and linear on each side (g' = -By dx per unit), which curls to
+    the two states (Bx + 2 By = sqrt 5 B . n is the same on both); each face
+    value is computed from I' + 2 J' alone, so the face fields are exactly
+    invariant too.
+
+    Returns (shape, prim, bfld, user): user maps (axis, side) to the user
+    boundaries of st2d.pyx:198-357: fixed left and right states on the x
+    sides over the active rows, and on the y sides the opposite active rows
+    shifted by -+2 ny cells along x.  st2d clamps the shifted column to the
+    grid (:318-320, 344-346), which reads the x sides' ghost cells before
+    the x sides fill them (the y sides come first), that is, whatever those
+    cells held between fills; here the clamp is to the first and last active
+    columns, which hold the same far-field states."""
+    nx = aspect * ny
+    if nx % 2:
+        raise ValueError("oblique_tube: aspect * ny must be even")
+    s = GridShape(nx, ny, 1, ng, (0.0, 0.0, 0.0), (float(aspect), 1.0, 1.0))
+    prim, bfld = s.empty()
+    nt = s.ntot
+    c4 = 1.0 / math.sqrt(4.0 * math.pi)
+    sides = []
+    for rho, v, p, b in ((1.08, (1.2, 0.01, 0.5), 0.95, (2.0, 3.6, 2.0)),
+                         (1.0, (0.0, 0.0, 0.0), 1.0, (2.0, 4.0, 2.0))):
+        w = np.zeros(9)
+        w[RHO], w[PR] = rho, p
+        w[VX:VZ + 1] = _rot12(v)
+        w[BXC:BZC + 1] = _rot12(np.array(b) * c4)
+        sides.append(w)
+    ip = np.arange(nt[0]) - ng
+    jp = np.arange(nt[1]) - ng
+    left = (ip[None, :] + 2 * jp[:, None]) <= nx // 2 + ny - 2
+    for n in (RHO, VX, VY, VZ, PR):
+        prim[n, 0] = np.where(left, sides[0][n], sides[1][n])
+    dx = 1.0 / ny
+    u0 = nx // 2 + ny  # corners with I' + 2 J' < u0 lie on the left
+    kb = sides[0][BXC] + 2.0 * sides[0][BYC]
+    byl, byr = sides[0][BYC], sides[1][BYC]
+
+    def g(u):
+        return np.where(u < u0, -byl * u * dx, -byr * u * dx + (byr - byl) * u0 * dx)
+
+    # face (I, j) of bx spans corners u and u + 2, face (i, J) of by u and u + 1
+    u = ip[None, :] + 2 * jp[:, None]
+    bfld[0, 0] = kb + (g(u + 2) - g(u)) / dx
+    bfld[1, 0] = -(g(u + 1) - g(u)) / dx
+    bfld[2, 0] = sides[0][BZC]
+    from .user_bc import Fixed, Mapped
+    i1, i2 = ng, ng + nx - 1
+    user = {
+        (0, 0): Fixed(sides[0], rows="active"),
+        (0, 1): Fixed(sides[1], rows="active"),
+        (1, 0): Mapped(lambda i, j, k: (np.clip(i - 2 * ny, i1, i2), j + ny, k)),
+        (1, 1): Mapped(lambda i, j, k: (np.clip(i + 2 * ny, i1, i2), j - ny, k)),
+    }
+    return s, prim, bfld, user
+
+
 def riemann_tube(n, rng, ng=3):
     """One seeded 1D MHD Riemann problem on [0,1]: rho in [0.1,1], p in
     [0.1,1], |v| <= 1 along a random direction, Bx in [0,1] shared by both

@@ -195,9 +195,8 @@ class BnzGrid:
         self.bc_flags = [params.get("physics", k, "i", 0)
                          for k in ("bc_x1", "bc_x2", "bc_y1", "bc_y2", "bc_z1", "bc_z2")]
         for f in self.bc_flags:
-            if f not in (0, 1, 2):
-                raise NotImplementedError("periodic (0), outflow (1) and reflect (2) boundaries "
-                                          "are built; user boundaries (3) are not")
+            if f not in (0, 1, 2, 3):
+                raise ValueError(f"boundary flag {f}: 0 periodic, 1 outflow, 2 reflect, 3 user")
         self.shape = problems.GridShape(self.Nact[0], self.Nact[1], self.Nact[2], self.ng,
                                         tuple(self.lmin), tuple(self.lmax))
 
@@ -215,6 +214,22 @@ PROBLEMS = {
     "sod": lambda grid, p: problems.sod(grid.Nact[0], 0, grid.ng)[1:],
     "hydro_blast": lambda grid, p: problems.hydro_blast(
         grid.Nact[0], grid.ng, dims=grid.dims, seed=p.get("problem", "seed", "i", 7))[1:],
+    "st2d": lambda grid, p: _st2d(grid)[1:3],
+}
+
+
+def _st2d(grid):
+    """The oblique tube on an Nx x Ny grid with Nx a multiple of Ny (square
+    cells need xmax / ymax = Nx / Ny)."""
+    if grid.dims != 2 or grid.Nact[0] % grid.Nact[1]:
+        raise ValueError("st2d: a 2D grid with Nx a multiple of Ny")
+    return problems.oblique_tube(grid.Nact[1], grid.ng, aspect=grid.Nact[0] // grid.Nact[1])
+
+
+# set_bc_grid_ptrs_user of the problems that have user boundaries
+# (src/problem/problem.pxd:13): {(axis, side): user_bc spec}
+PROBLEM_USER_BC = {
+    "st2d": lambda grid, p: _st2d(grid)[3],
 }
 
 
@@ -224,13 +239,23 @@ class Simulation:
     ``usr_dir`` holds config.cfg; ``config`` (dict of sections) overrides or
     replaces it.  ``set_problem(grid) -> (prim, bfld)`` supplies initial
     conditions over the full grid (active cells set); by default the
-    ``[problem] name`` entry picks one of ``PROBLEMS``."""
+    ``[problem] name`` entry picks one of ``PROBLEMS``.
+    ``set_bc_user(grid) -> {(axis, side): spec}`` supplies the sides with
+    boundary flag 3 (user_bc.Fixed or user_bc.Mapped), as the problem file's
+    set_bc_grid_ptrs_user does (src/problem/problem.pxd:13); by default the
+    problem's entry in ``PROBLEM_USER_BC``."""
 
-    def __init__(self, usr_dir=None, config=None, set_problem=None, device=0, mfield=True):
+    def __init__(self, usr_dir=None, config=None, set_problem=None, device=0, mfield=True,
+                 set_bc_user=None):
         p = _Params(usr_dir, config)
         self.usr_dir = usr_dir
         self.integr = BnzIntegr(p, mfield=mfield)
         self.grid = BnzGrid(p, self.integr)
+        name = p.get("problem", "name", "s", "otvortex")
+        user_sides = [q for q in range(2 * self.grid.dims) if self.grid.bc_flags[q] == 3]
+        if user_sides and set_bc_user is None and name not in PROBLEM_USER_BC:
+            raise ValueError(f"boundary flag 3 on side(s) {user_sides}: the problem "
+                             f"'{name}' defines no user boundaries (pass set_bc_user)")
         rs = {RSolver.RS_HLL: "hll", RSolver.RS_HLLD: "hlld", RSolver.RS_HLLT: "hllt",
               RSolver.RS_HLLC: "hllc"}[self.integr.rsolver]
         self.ctx = _lib.Context(*self.grid.Nact, rsolver=rs,
@@ -245,10 +270,15 @@ class Simulation:
                                 tintegr="rk3" if self.integr.tintegr == TIntegr.TINT_RK3
                                 else "vl")
         if set_problem is None:
-            name = p.get("problem", "name", "s", "otvortex")
             prim, bfld = PROBLEMS[name](self.grid, p)
         else:
             prim, bfld = set_problem(self.grid)
+        if user_sides:
+            specs = (set_bc_user or (lambda g: PROBLEM_USER_BC[name](g, p)))(self.grid)
+            for q in user_sides:
+                if (q // 2, q % 2) not in specs:
+                    raise ValueError(f"no user boundary given for side {q // 2},{q % 2}")
+                self.ctx.set_user_bc(q // 2, q % 2, specs[(q // 2, q % 2)])
         # init_sim: ghosts, interp_bc, ghosts of cell B, prim2cons
         # (src/integration/vanleer.pyx:38-48)
         self.ctx.upload(prim, bfld)

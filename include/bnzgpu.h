@@ -53,7 +53,8 @@ typedef struct {
   int nx, ny, nz;       /* active cells (ny=nz=1 in 1D, nz=1 in 2D), [computation] Nx,Ny,Nz */
   int rsolver;          /* BNZ_RS_*,  [computation] rsolver                */
   int reconstr;         /* BNZ_RCN_*, [computation] reconstr              */
-  int bc[6];            /* x1,x2,y1,y2,z1,z2: [physics] bc_x1 ... bc_z2     */
+  int bc[6];            /* x1,x2,y1,y2,z1,z2: [physics] bc_x1 ... bc_z2;
+                           BNZ_BC_USER sides need bnz_set_user_bc            */
   double lmin[3];       /* [computation] xmin,ymin,zmin                    */
   double lmax[3];       /* [computation] xmax,ymax,zmax                    */
   double gam;           /* [physics] gam                                   */
@@ -175,6 +176,33 @@ int bnz_load_state(bnz_ctx *ctx, const void *host, long bytes);
    shape.  Chosen by the autotuner (bonzo_amd/autotune.py); no counterpart
    in the reference. */
 int bnz_set_tuning(bnz_ctx *ctx, int flux_chunk, int emf_chunk, int update_chunk);
+
+/* User boundary (flag 3) of one side.  The reference leaves such a side's
+   ghost fill to code in the problem file (set_bc_grid_ptrs_user,
+   src/problem/problem.pxd:13; the hook grid_bc.pyx:266-268; the one use,
+   st2d.pyx:198-372: fixed inflow states on the x sides, rows copied with an
+   x shift on the y sides).  Here the caller describes the fill once, for
+   every cell of the side's slab: the ng ghost layers along the axis (side 0
+   the lowest indices, 1 the highest) times the full extent, ghosts
+   included, of the other axes, x fastest.
+     src[q] >= 0: cell q takes every variable (prim and face fields) of that
+                  cell of the full grid, (k * Ntot_y + j) * Ntot_x + i;
+     src[q] == -1: it takes fixed[12 q .. 12 q + 11] = rho, vx, vy, vz, p,
+                  scalar, bx, by, bz (cell-centred) and the x, y, z face
+                  fields (fixed may be NULL without such entries);
+     src[q] == -2: it is left alone.
+   No source may be a cell the same side writes.  With a user side on a grid
+   the fill runs one side after the other in the reference's order (z1, z2,
+   y1, y2, x1, x2), each side reading what the earlier ones wrote, as the
+   reference's per-side functions do.  Every ghost cell must end up with a
+   value that comes from active cells or fixed values through earlier sides:
+   a value read from a ghost cell that a later side fills (what that cell
+   held between fills) is refused, as is a ghost cell no side writes
+   (BNZ_EINVAL once the last flag-3 side is set; a refused side leaves the
+   boundaries as they were).  Every flag-3 side must be set before
+   bnz_init_state (BNZ_ESTATE otherwise); setting one again takes effect at
+   the next fill.  Single grids, FP64 and FP32C. */
+int bnz_set_user_bc(bnz_ctx *ctx, int axis, int side, const int *src, const double *fixed);
 
 /* Step graphs in bnz_advance: on = 1 replays full chunks of eight steps
    from a captured hipGraph, on = 0 (the default) launches every kernel from

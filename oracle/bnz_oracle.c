@@ -721,6 +721,10 @@ struct or_sim {
   double *cons_ss, *bfld_ss;  /* RK3's second stage (Uss, Bss) */
   double time, dt;
   int step;
+  /* user sides (flag 3): per slab cell a source cell, -1 (fixed values,
+     NMODE + 3 per cell) or -2 (left alone) */
+  int *usr_src[3][2];
+  double *usr_fix[3][2];
 };
 
 #define IDX(s, n, k, j, i) \
@@ -771,6 +775,32 @@ void or_set_char_proj(or_sim *s, int on) { s->char_proj = on; }
 void or_set_hydro(or_sim *s, int on) { s->hydro = on; }
 /* corner EMF: 0 interp_ee1 (the reference's), 1 the upwind interp_ee2 */
 void or_set_ct_upwind(or_sim *s, int on) { s->ct_upwind = on; }
+
+/* User side (flag 3): src and fixed over the side's slab (ng layers along
+   the axis, full extent of the others, x fastest); copied.  0 or -1. */
+int or_set_user_bc(or_sim *s, int axis, int side, const int *src, const double *fixed)
+{
+  if (axis < 0 || axis >= s->dims || side < 0 || side > 1 || s->bc[axis][side] != 3) return -1;
+  long n = (long)s->ng;
+  for (int a = 0; a < 3; ++a)
+    if (a != axis) n *= s->Ntot[a];
+  int *sc = malloc(n * sizeof(int));
+  double *fc = calloc(n * (NMODE + 3), sizeof(double));
+  for (long e = 0; e < n; ++e) {
+    if (src[e] < -2 || src[e] >= s->ncell) {
+      free(sc);
+      free(fc);
+      return -1;
+    }
+    sc[e] = src[e];
+  }
+  if (fixed) memcpy(fc, fixed, n * (NMODE + 3) * sizeof(double));
+  free(s->usr_src[axis][side]);
+  free(s->usr_fix[axis][side]);
+  s->usr_src[axis][side] = sc;
+  s->usr_fix[axis][side] = fc;
+  return 0;
+}
 /* For the oracle's own tests: the face-flux arrays [9][cells] and the edge
    EMFs [3][cells], and one corner-EMF evaluation (ec_from_prim, then
    interp_ee1 or interp_ee2) over the corrector's limits from the current
@@ -927,6 +957,11 @@ void or_destroy(or_sim *s)
   free(s->bfld_s);
   free(s->ec);
   free(s->ee);
+  for (int a = 0; a < 3; ++a)
+    for (int d = 0; d < 2; ++d) {
+      free(s->usr_src[a][d]);
+      free(s->usr_fix[a][d]);
+    }
   free(s);
 }
 
@@ -1441,6 +1476,25 @@ static void bc_axis(or_sim *s, double *A3, int ax, int side, int face, int n)
 #define PLUS(x) (x)
 #define MINUS(x) (-1 * (x))
 #define ZERO(x) (0.0)
+  if (flag == 3) {
+    /* user side: the fill the problem file's hook would do, given as data
+       (set_bc_grid_ptrs_user, src/problem/problem.pxd:13; the kinds of fill
+       of st2d.pyx:198-372: fixed values, copies from chosen cells) */
+    const int *src = s->usr_src[ax][side];
+    const double *fix = s->usr_fix[ax][side];
+    if (!src) return;
+    int lo[3] = {0, 0, 0}, hi[3] = {nx, ny, nz};
+    lo[ax] = side ? s->Ntot[ax] - ng : 0;
+    hi[ax] = lo[ax] + ng;
+    long e = 0;
+    for (int k = lo[2]; k < hi[2]; ++k)
+      for (int j = lo[1]; j < hi[1]; ++j)
+        for (int i = lo[0]; i < hi[0]; ++i, ++e) {
+          if (src[e] >= 0) AT(k, j, i) = A3[src[e]];
+          else if (src[e] == -1) AT(k, j, i) = fix[e * (NMODE + 3) + (face ? NMODE + n : n)];
+        }
+    return;
+  }
   if (flag == 2) {
     if (face && n == ax) {
       int src = 0;

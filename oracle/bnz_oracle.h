@@ -77,6 +77,8 @@ void    or_set_dt_fms_ref(or_sim *s, int on);
 void    or_set_char_proj(or_sim *s, int on);
 void    or_set_hydro(or_sim *s, int on);
 void    or_set_ct_upwind(or_sim *s, int on);
+/* user boundary (flag 3) of one side, the layout of bnz_set_user_bc */
+int     or_set_user_bc(or_sim *s, int axis, int side, const int *src, const double *fixed);
 double *or_flux(or_sim *s, int axis);   /* [9][cells] face fluxes (tests)  */
 double *or_edge_emf(or_sim *s);         /* [3][cells] edge EMFs (tests)    */
 void    or_corner_emf(or_sim *s, int upwind, double dt);

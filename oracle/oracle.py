@@ -94,6 +94,8 @@ def lib(path=None):
         L.or_fms_hydro.argtypes = [_dp, ctypes.c_double]
         L.or_set_hydro.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.or_set_ct_upwind.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.or_set_user_bc.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                     ctypes.POINTER(ctypes.c_int), _dp]
         for name in ("or_flux",):
             getattr(L, name).restype = _dp
             getattr(L, name).argtypes = [ctypes.c_void_p, ctypes.c_int]
@@ -239,6 +241,19 @@ class OracleSim:
         self.prim = np.ctypeslib.as_array(L.or_prim(self._h), shape=(9 * nc,)).reshape((9,) + shp)
         self.cons = np.ctypeslib.as_array(L.or_cons(self._h), shape=(9 * nc,)).reshape((9,) + shp)
         self.bfld = np.ctypeslib.as_array(L.or_bfld(self._h), shape=(3 * nc,)).reshape((3,) + shp)
+
+    def set_user_bc(self, axis, side, src, fixed=None):
+        """User boundary (flag 3) of one side, raw arrays in the layout of
+        bnz_set_user_bc (src per slab cell, fixed [n][12])."""
+        src = np.ascontiguousarray(src, dtype=np.int32)
+        fp = None
+        if fixed is not None:
+            fixed = np.ascontiguousarray(fixed, dtype=np.float64)
+            fp = _ptr(fixed)
+        rc = self._L.or_set_user_bc(self._h, axis, side,
+                                    src.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), fp)
+        if rc != 0:
+            raise ValueError("or_set_user_bc: bad side or source index")
 
     def coord(self, which, axis):
         n = self.ntot[axis] + (1 if which == 0 else 0)

@@ -113,7 +113,7 @@ def test_batch_create_rejects_bad_selection_without_touching_gpu():
     assert L.bnz_batch_create(ctypes.byref(p), boxes, 2, ctypes.byref(h)) == _lib.BNZ_EUNSUPPORTED
     L.bnz_batch_destroy(h)
     p = _lib.make_params()
-    boxes[1].bc[0] = 3  # user boundary: not built
+    boxes[1].bc[0] = 3  # user boundaries are single-grid only
     assert L.bnz_batch_create(ctypes.byref(p), boxes, 2, ctypes.byref(h)) == _lib.BNZ_EUNSUPPORTED
     assert b"box 1" in L.bnz_batch_last_error(h)
     L.bnz_batch_destroy(h)
@@ -402,3 +402,65 @@ def test_rk3_validation_without_touching_gpu():
     p = _lib.make_params(tintegr="rk3")
     assert L.bnz_batch_create(ctypes.byref(p), boxes, 1, ctypes.byref(h)) == _lib.BNZ_EUNSUPPORTED
     L.bnz_batch_destroy(h)
+
+
+def test_user_bc_specs_on_the_host():
+    """user_bc specs become the slab arrays of bnz_set_user_bc: slab order
+    x fastest over ng layers times the full other extents; Fixed over the
+    active rows leaves the others alone (-2); Mapped gives flat source
+    indices and refuses sources off the grid; flag 3 without a user
+    boundary for the problem fails before any device work."""
+    from bonzo_amd.simulation import Simulation
+    from bonzo_amd.user_bc import Fixed, Mapped, slab, side_arrays
+    ntot, ng = (10, 8, 1), 3
+    i, j, k = slab(ntot, ng, 0, 1)
+    assert i.size == ng * 8 and list(i[:3]) == [7, 8, 9] and list(j[:4]) == [0, 0, 0, 1]
+    i, j, k = slab(ntot, ng, 1, 0)
+    assert i.size == 10 * ng and list(i[:10]) == list(range(10)) and j.max() == ng - 1
+    w = np.arange(9.0)
+    src, fixed = side_arrays(Fixed(w, rows="active"), ntot, ng, 2, 0, 0)
+    jj = slab(ntot, ng, 0, 0)[1]
+    act = (jj >= ng) & (jj < 8 - ng)
+    assert np.all(src[act] == -1) and np.all(src[~act] == -2)
+    assert fixed.shape == (src.size, 12) and np.array_equal(fixed[0, :9], w)
+    assert np.array_equal(fixed[0, 9:], w[6:9])
+    src, fixed = side_arrays(Mapped(lambda i, j, k: (i, j + 2, k)), ntot, ng, 2, 1, 0)
+    i, j, k = slab(ntot, ng, 1, 0)
+    assert fixed is None and np.array_equal(src, (j + 2) * 10 + i)
+    with pytest.raises(ValueError, match="off the grid|out of the grid"):
+        side_arrays(Mapped(lambda i, j, k: (i - 5, j, k)), ntot, ng, 2, 1, 0)
+    with pytest.raises(ValueError, match="9 values"):
+        Fixed(np.zeros(8))
+    cfg = {"computation": {"Nx": 16, "Ny": 8, "Nz": 1, "tmax": 0.1, "cour": 0.4,
+                           "rsolver": "hlld", "reconstr": "linear"},
+           "physics": {"gam": 5.0 / 3.0, "bc_x1": 3}, "problem": {"name": "otvortex"}}
+    with pytest.raises(ValueError, match="defines no user boundaries"):
+        Simulation(config=cfg)
+    cfg["physics"]["bc_x1"] = 4
+    with pytest.raises(ValueError, match="boundary flag 4"):
+        Simulation(config=cfg)
+
+
+def test_oblique_tube_setup():
+    """The st2d setup: Ryu-Jones 2a states rotated onto (1, 2)/sqrt 5, the
+    state and face fields exactly invariant under (i, j) -> (i - 2, j + 1),
+    discrete div B zero to round-off, and the two far-field face fields the
+    rotated tube fields."""
+    from bonzo_amd import problems as P
+    s, prim, bfld, user = P.oblique_tube(8, aspect=4)
+    assert (s.nx, s.ny) == (32, 8) and s.dx(0) == s.dx(1)
+    for arr in (prim, bfld):
+        for a in arr:
+            assert np.array_equal(a[0, :-1, 2:], a[0, 1:, :-2])
+    g = s.ng
+    b = bfld[:, 0]
+    div = (b[0][g:-g, g + 1:-g + 1] - b[0][g:-g, g:-g]) + (b[1][g + 1:-g + 1, g:-g] - b[1][g:-g, g:-g])
+    assert np.abs(div).max() < 1e-14
+    c4 = 1 / np.sqrt(4 * np.pi)
+    r5 = np.sqrt(5.0)
+    bl = np.array([(2 - 2 * 3.6) / r5, (4 + 3.6) / r5]) * c4
+    br = np.array([(2 - 8) / r5, (4 + 4) / r5]) * c4
+    assert np.allclose([b[0][g, g], b[1][g, g]], bl, rtol=1e-13)
+    assert np.allclose([b[0][-g - 1, -g - 1], b[1][-g - 1, -g - 1]], br, rtol=1e-13)
+    assert np.isclose(prim[1, 0, g, g], (1.2 - 2 * 0.01) / r5) and prim[0, 0, -g - 1, -g - 1] == 1.0
+    assert set(user) == {(0, 0), (0, 1), (1, 0), (1, 1)}

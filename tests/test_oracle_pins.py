@@ -1224,3 +1224,150 @@ def test_rk3_planar_tube_equals_1d():
     assert np.max(np.abs(o1.prim[0, 0, 0, g:g + s1.nx] - prob1[1][0, 0, 0, g:g + s1.nx])) > 1e-2
     o.close()
     o1.close()
+
+
+# ------------------------------------------------------------ user boundaries
+def _oracle_user(prob, bc, specs, steps, **kw):
+    """Oracle run with flag-3 sides given by specs {(axis, side): spec}."""
+    from tests import userbc_util as U
+    s, prim, bfld = prob[:3]
+    o = orc.OracleSim(s.nx, s.ny, s.nz, bc=tuple(bc), lmax=s.lmax, **kw)
+    for (a, d), spec in specs.items():
+        src, fixed = U.arrays(spec, o.ntot, o.ng, s.dims, a, d)
+        o.set_user_bc(a, d, src, fixed)
+    o.prim[:] = prim
+    o.bfld[:] = bfld
+    o.init()
+    o.advance_steps(steps)
+    return o
+
+
+@pytest.mark.parametrize("dims,flags", [(2, (0, 0, 0, 0, 0, 0)), (2, (1, 1, 0, 0, 0, 0)),
+                                        (3, (0, 0, 0, 0, 0, 0)), (3, (1, 1, 0, 0, 1, 1))])
+def test_user_sides_restating_builtin_flags_equal_them(dims, flags):
+    """A flag-3 side given the copies of periodic or outflow (as data) gives
+    the run of the built-in flag, with every side user-given and with user
+    sides next to built-in ones: pins the slab layout (origin, order, full
+    extent of the other axes) and the side-by-side fill order of the user
+    path against the built-in fill (grid_bc.pyx:612-623).  Bit for bit where
+    the restated sides are outflow; to round-off where an axis was periodic,
+    because only periodic axes get their ghost faces re-spaced from the
+    opposite active faces (coord.pyx:201-257), so the ghost widths of a
+    flag-3 axis differ from the periodic ones in the last bits."""
+    from tests import userbc_util as U
+    if dims == 2:
+        prob = P.orszag_tang(24)
+    else:
+        prob = P.turbulence(12, seed_b=21, seed_v=22)
+    s = prob[0]
+    ref = _run(prob, 6, bc=flags)[1]
+    nt = ref.ntot
+    for which in ("all", "x"):
+        bc = list(flags)
+        specs = {}
+        for q in range(2 * dims):
+            if which == "x" and q >= 2:
+                continue
+            bc[q] = 3
+            specs[(q // 2, q % 2)] = U.builtin_as_user(nt, s.ng, flags[q], q // 2, q % 2)
+        o = _oracle_user(prob, bc, specs, 6)
+        replaced = [flags[q] for q in range(2 * dims) if bc[q] == 3]
+        if 0 not in replaced:
+            assert np.array_equal(o.prim, ref.prim) and np.array_equal(o.bfld, ref.bfld), which
+            assert o.time == ref.time
+        else:
+            sk, sj, si = s.active()
+            for n in range(9):
+                a, b = o.prim[n][sk, sj, si], ref.prim[n][sk, sj, si]
+                assert np.max(np.abs(a - b)) <= 1e-13 * max(np.max(np.abs(b)), 1e-300), (which, n)
+            assert abs(o.time - ref.time) <= 1e-14 * ref.time
+        o.close()
+    ref.close()
+
+
+def test_user_fixed_sides_keep_a_uniform_flow():
+    """Fixed inflow and outflow states equal to a uniform flow's state keep
+    it uniform to the last bits (the prim -> cons -> prim round trip of each
+    step moves a value by an ulp); the same sides with the face values or
+    the pressure slot of the fixed row changed do not (a misplaced slot of
+    the fixed-value row fails)."""
+    from bonzo_amd.user_bc import Fixed
+    s = P.GridShape(16, 8, 1, 3)
+    prim, bfld = s.empty()
+    w = np.array([1.3, 0.7, -0.2, 0.1, 0.9, 0.0, 0.4, 0.3, -0.5])
+    for n in range(9):
+        prim[n] = w[n]
+    for a in range(3):
+        bfld[a] = w[6 + a]
+    for bad in (None, "faces", "pressure"):
+        wf = w.copy()
+        faces = w[6:9].copy()
+        if bad == "faces":
+            faces[1] += 0.1
+        if bad == "pressure":
+            wf[4] *= 1.5
+        specs = {(0, 0): Fixed(wf, faces, rows="active"), (0, 1): Fixed(wf, faces, rows="active")}
+        o = _oracle_user((s, prim, bfld), (3, 3, 0, 0, 0, 0), specs, 8)
+        sk, sj, si = s.active()
+        dev = max(float(np.max(np.abs(o.prim[n][sk, sj, si] - w[n]))) for n in range(9))
+        if bad is None:
+            assert dev <= 1e-14
+        else:
+            assert dev > 1e-6, bad
+        o.close()
+
+
+def _shift_deviation(o, s):
+    """max |W(i, j) - W(i - 2, j + 1)| over pairs of active cells, scaled by
+    each variable's range, for prim and the x/y face fields."""
+    g = s.ng
+    worst = 0.0
+    for arr, nv in ((o.prim, 9), (o.bfld, 2)):
+        for n in range(nv):
+            a = arr[n, 0, g:g + s.ny, g:g + s.nx]
+            d = np.abs(a[:-1, 2:] - a[1:, :-2])
+            scale = max(float(np.ptp(a)), float(np.max(np.abs(a))), 1e-300)
+            worst = max(worst, float(d.max()) / scale)
+    return worst
+
+
+def test_oblique_tube_keeps_its_translation_symmetry():
+    """The oblique shock tube of st2d.pyx with its user boundaries (fixed
+    x sides, y sides copied from the opposite rows shifted by 2 ny along x):
+    the initial state and face fields are exactly invariant under
+    (i, j) -> (i - 2, j + 1), and so is every step if the user sides copy the
+    right cells; the only departure is the clamp of the shifted source
+    column to the active columns (st2d.pyx:318-320, 344-346 clamp to the
+    grid), which stands in far-field states for the true images.  On an 8:1
+    box those are 3 ny cells from the interface and the run stays symmetric
+    to round-off for 30 steps.  Copies shifted by one cell too many break
+    the symmetry at O(1e-3) or more."""
+    from bonzo_amd.user_bc import Mapped
+    prob = P.oblique_tube(16, aspect=8)
+    s, prim, bfld, user = prob
+    assert _shift_deviation_init(prim, bfld, s) == 0.0
+    o = _oracle_user(prob, (3, 3, 3, 3, 0, 0), user, 30)
+    assert o.step == 30
+    dev = _shift_deviation(o, s)
+    assert dev < 1e-12, dev
+    sk, sj, si = s.active()
+    assert np.max(np.abs(o.prim[0][sk, sj, si] - prim[0][sk, sj, si])) > 1e-1  # waves moved
+    i1, i2 = s.ng, s.ng + s.nx - 1
+    o.close()
+    bad = dict(user)
+    bad[(1, 0)] = Mapped(lambda i, j, k: (np.clip(i - 2 * 16 + 1, i1, i2), j + 16, k))
+    bad[(1, 1)] = Mapped(lambda i, j, k: (np.clip(i + 2 * 16 - 1, i1, i2), j - 16, k))
+    ob = _oracle_user(prob, (3, 3, 3, 3, 0, 0), bad, 30)
+    assert _shift_deviation(ob, s) > 1e-3
+    ob.close()
+
+
+def _shift_deviation_init(prim, bfld, s):
+    """max |W(i, j) - W(i - 2, j + 1)| of the initial prim and face fields,
+    ghosts included."""
+    worst = 0.0
+    for arr, nv in ((prim, 9), (bfld, 3)):
+        for n in range(nv):
+            a = arr[n, 0]
+            worst = max(worst, float(np.abs(a[:-1, 2:] - a[1:, :-2]).max()))
+    return worst
