@@ -199,6 +199,71 @@ __device__ __forceinline__ void block_max_to(double v, double *part) {
   }
 }
 
+// The dt of a step from its CFL partials (new_dt.pyx:18-135 after the
+// per-cell maxima): dt = cour / max over npart partials, or a fixed dt,
+// clipped to t_max.  dev[0]=dt, dev[1]=t, dev[2]=t_max (<0: no clip),
+// dev[3]=step counter, dev[7]=the dt of the last step taken.  k_dt_final
+// runs it as a kernel of its own; a step's first flux launch runs it in its
+// workgroup 0, where nothing reads dt before the predictor's update (DtFold).
+struct DtFold {
+  const double *part = nullptr;
+  int npart = 0;  // 0: dev[0] already holds the step's dt
+  int use_fixed = 0;
+  double cour = 0., fixed_dt = 0.;
+};
+
+// Called by every thread of one workgroup.
+__device__ __forceinline__ void dt_reduce_block(const DtFold &f, double *dev) {
+  __shared__ double red[BLK / 64];
+  double m[8] = {0., 0., 0., 0., 0., 0., 0., 0.};
+  int q = threadIdx.x;
+  for (; q + 7 * BLK < f.npart; q += 8 * BLK) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) m[u] = fmax(m[u], f.part[q + u * BLK]);
+  }
+  for (; q < f.npart; q += BLK) m[0] = fmax(m[0], f.part[q]);
+  double mm = m[0];
+#pragma unroll
+  for (int u = 1; u < 8; ++u) mm = fmax(mm, m[u]);
+  mm = wave_max(mm);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mm;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double x = red[0];
+    for (int r = 1; r < BLK / 64; ++r) x = fmax(x, red[r]);
+    double dt = f.use_fixed ? f.fixed_dt : f.cour / x;
+    const double tnow = dev[1], tmax = dev[2];
+    if (!f.use_fixed && tmax >= 0.) {
+      if (!(tnow < tmax))
+        dt = 0.;  // already at t_max: the step is a no-op (see k_update)
+      else if (tnow + dt > tmax)
+        dt = tmax - tnow;
+    }
+    dev[0] = dt;
+    if (dt != 0.) dev[7] = dt;  // the dt of the last step taken
+  }
+}
+
+// Whether a step is one past t_max, for the workgroups of a launch whose
+// workgroup 0 may still be computing dt: with a fold pending, from the
+// fixed dt or from t and t_max (dt = 0 exactly then, apart from an infinite
+// CFL speed, after which the update still sees dt = 0); else from dev[0].
+__device__ __forceinline__ bool step_past_tmax(const DtFold &f, const double *dev) {
+  if (f.npart == 0) return dev[0] == 0.;
+  if (f.use_fixed) return f.fixed_dt == 0.;
+  return dev[2] >= 0. && !(dev[1] < dev[2]);
+}
+
+// The time advance of a finished step, run by one thread of the step's
+// last launch (the ghost fill): t += dt, step += 1, unless the step was one
+// past t_max.
+__device__ __forceinline__ void time_advance_once(double *adv) {
+  if (adv && blockIdx.x == 0 && threadIdx.x == 0 && adv[0] != 0.) {
+    adv[1] += adv[0];
+    adv[3] += 1.0;
+  }
+}
+
 // CFL speed of one active cell (new_dt.pyx:74-91).  fms_ref is 1 (the
 // reference's x fast speed on every axis, D2), 0 (per-axis fast speeds) or
 // CFL_HYDRO (the hydro build: the sound speed on every axis).
@@ -321,6 +386,9 @@ struct bnz_ctx {
   int flux_chunk = 0, emf_chunk = 0, update_chunk = 0;
   // CFL partial maxima left by the last corrector for the next step
   int dt_parts = 0;
+  // the next step's dt reduction, left for its first flux launch (take_dt)
+  bnz::DtFold pdt{};
+  bool pdt_pending = false;
   // bnz_advance: pinned copies of (t, tmax, step) per chunk of steps, and the
   // events that mark them
   double *hbuf = nullptr;
@@ -570,7 +638,17 @@ void f32_step_body(bnz_ctx *c);
 // shared launches (bnz_gpu.hip)
 void launch_dt_final(bnz_ctx *c, int npart, int use_fixed, double fixed_dt);
 void launch_time_advance(bnz_ctx *c);
+// The pending dt reduction for a step's first flux launch (npart 0 if none),
+// or launched on its own (flush_dt) where dt is needed first.
+inline DtFold take_dt(bnz_ctx *c) {
+  DtFold f{};
+  if (c->pdt_pending) f = c->pdt;
+  c->pdt_pending = false;
+  return f;
+}
+void flush_dt(bnz_ctx *c);
 void launch_ghost64(bnz_ctx *c);
+void launch_ghost_step64(bnz_ctx *c);  // a step's fill: also t += dt, step += 1
 // the FP64 step compiled with contraction (bnz_gpu64f.hip)
 void fma_step_body(bnz_ctx *c);
 

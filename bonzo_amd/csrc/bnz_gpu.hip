@@ -32,7 +32,9 @@ namespace {
 // sequence composes per-axis index maps (periodic wrap, outflow clamp), so
 // each ghost cell gathers directly from the active cell its maps name.
 __global__ __launch_bounds__(BLK) void k_ghost(Geo g, double *w, double *b,
-                                               const int2 *__restrict__ pairs, int npairs) {
+                                               const int2 *__restrict__ pairs, int npairs,
+                                               double *adv) {
+  time_advance_once(adv);
   const long t = xcd_block() * BLK + threadIdx.x;
   if (t >= npairs) return;
   const int2 ds = pairs[t];  // (ghost cell, active source cell)
@@ -56,7 +58,9 @@ __global__ __launch_bounds__(BLK) void k_ghost(Geo g, double *w, double *b,
 // bonly stores only the cell-centred field (the second fill of the init).
 __global__ __launch_bounds__(BLK) void k_ghost_ops(Geo g, double *w, double *b,
                                                    const GhostOp *__restrict__ ops, int nops,
-                                                   const double *__restrict__ fix, int bonly) {
+                                                   const double *__restrict__ fix, int bonly,
+                                                   double *adv) {
+  time_advance_once(adv);
   const long t = xcd_block() * BLK + threadIdx.x;
   if (t >= nops) return;
   const GhostOp o = ops[t];
@@ -164,39 +168,8 @@ __global__ __launch_bounds__(BLK) void k_dt_partial(Geo g, const double *__restr
   }
 }
 
-// dev[0]=dt, dev[1]=t, dev[2]=tmax (<0: no clip), dev[3]=step counter
-__global__ __launch_bounds__(BLK) void k_dt_final(const double *__restrict__ part, int npart,
-                                                  double cour, double *dev, int use_fixed,
-                                                  double fixed_dt) {
-  __shared__ double red[BLK / 64];
-  double m[8] = {0., 0., 0., 0., 0., 0., 0., 0.};
-  int q = threadIdx.x;
-  for (; q + 7 * BLK < npart; q += 8 * BLK) {
-#pragma unroll
-    for (int u = 0; u < 8; ++u) m[u] = fmax(m[u], part[q + u * BLK]);
-  }
-  for (; q < npart; q += BLK) m[0] = fmax(m[0], part[q]);
-  double mm = m[0];
-#pragma unroll
-  for (int u = 1; u < 8; ++u) mm = fmax(mm, m[u]);
-  mm = wave_max(mm);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mm;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double x = red[0];
-    for (int r = 1; r < BLK / 64; ++r) x = fmax(x, red[r]);
-    double dt = use_fixed ? fixed_dt : cour / x;
-    const double tnow = dev[1], tmax = dev[2];
-    if (!use_fixed && tmax >= 0.) {
-      if (!(tnow < tmax))
-        dt = 0.;  // already at t_max: the step is a no-op (see k_update)
-      else if (tnow + dt > tmax)
-        dt = tmax - tnow;
-    }
-    dev[0] = dt;
-    if (dt != 0.) dev[7] = dt;  // the dt of the last step taken
-  }
-}
+// The dt reduction as a kernel of its own (dt_reduce_block)
+__global__ __launch_bounds__(BLK) void k_dt_final(DtFold f, double *dev) { dt_reduce_block(f, dev); }
 
 __global__ void k_time_advance(double *dev) {
   if (threadIdx.x == 0 && blockIdx.x == 0 && dev[0] != 0.) {
@@ -404,25 +377,36 @@ void set_coordinates(bnz_ctx *c) {
   }
 }
 
-void launch_ghost(bnz_ctx *c, int bonly = 0) {
+// The ghost fill; adv (a step's fill) is the time and step counter that its
+// first launch advances.  Returns whether a launch took adv.
+bool launch_ghost(bnz_ctx *c, int bonly = 0, double *adv = nullptr) {
   Launch t(c, "ghost");
   if (c->npass > 0) {  // user boundaries: one pass per side, in order
     for (int q = 0; q < 6; ++q) {
       const int n = c->pass_off[q + 1] - c->pass_off[q];
-      if (n > 0)
+      if (n > 0) {
         hipLaunchKernelGGL(k_ghost_ops, dim3(blocks_for(n)), dim3(BLK), 0, c->stream, c->g, c->w,
-                           c->b, c->gops + c->pass_off[q], n, c->ufix, bonly);
+                           c->b, c->gops + c->pass_off[q], n, c->ufix, bonly, adv);
+        adv = nullptr;
+      }
     }
-    return;
+    return adv == nullptr;
   }
-  if (c->nghost > 0)
+  if (c->nghost > 0) {
     hipLaunchKernelGGL(k_ghost, dim3(blocks_for(c->nghost)), dim3(BLK), 0, c->stream, c->g, c->w,
-                       c->b, c->ghost, c->nghost);
-  if (c->ngops > 0)
+                       c->b, c->ghost, c->nghost, adv);
+    adv = nullptr;
+  }
+  if (c->ngops > 0) {
     hipLaunchKernelGGL(k_ghost_ops, dim3(blocks_for(c->ngops)), dim3(BLK), 0, c->stream, c->g,
-                       c->w, c->b, c->gops, c->ngops, (const double *)nullptr, 0);
+                       c->w, c->b, c->gops, c->ngops, (const double *)nullptr, 0, adv);
+    adv = nullptr;
+  }
+  return adv == nullptr;
 }
 
+// The CFL partials of the step (the last corrector's, or k_dt_partial's),
+// and their reduction left to the step's first flux launch (take_dt).
 void launch_dt(bnz_ctx *c, int use_fixed, double fixed_dt) {
   int nb = c->dt_parts;
   if (nb <= 0) {
@@ -432,19 +416,29 @@ void launch_dt(bnz_ctx *c, int use_fixed, double fixed_dt) {
     hipLaunchKernelGGL(k_dt_partial, dim3(nb), dim3(BLK), 0, c->stream, c->g, c->w, a,
                        cfl_mode(c), c->part);
   }
-  Launch t(c, "dt_final");
-  hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(BLK), 0, c->stream, c->part, nb, c->p.cour,
-                     c->dev, use_fixed, fixed_dt);
+  launch_dt_final(c, nb, use_fixed, fixed_dt);
 }
 
 }  // namespace
 
 namespace bnz {
 
+// Leaves the reduction of npart partials pending: the step's first flux
+// launch runs it (take_dt), or flush_dt as a kernel of its own.
 void launch_dt_final(bnz_ctx *c, int npart, int use_fixed, double fixed_dt) {
+  c->pdt.part = c->part;
+  c->pdt.npart = npart;
+  c->pdt.use_fixed = use_fixed;
+  c->pdt.cour = c->p.cour;
+  c->pdt.fixed_dt = fixed_dt;
+  c->pdt_pending = true;
+}
+
+void flush_dt(bnz_ctx *c) {
+  const DtFold f = take_dt(c);
+  if (f.npart == 0) return;
   Launch t(c, "dt_final");
-  hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(BLK), 0, c->stream, c->part, npart, c->p.cour,
-                     c->dev, use_fixed, fixed_dt);
+  hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(BLK), 0, c->stream, f, c->dev);
 }
 
 void launch_time_advance(bnz_ctx *c) {
@@ -452,6 +446,11 @@ void launch_time_advance(bnz_ctx *c) {
 }
 
 void launch_ghost64(bnz_ctx *c) { launch_ghost(c); }
+
+// A step's ghost fill, which also advances its time and step counter.
+void launch_ghost_step64(bnz_ctx *c) {
+  if (!launch_ghost(c, 0, c->dev)) launch_time_advance(c);
+}
 
 }  // namespace bnz
 
@@ -463,6 +462,9 @@ void any_dt(bnz_ctx *c, int use_fixed, double fixed_dt) {
   else launch_dt(c, use_fixed, fixed_dt);
 }
 void any_step(bnz_ctx *c) {
+  // 1D VL steps and RK3 steps read dt in their first launch; 2D and 3D VL
+  // steps reduce it in their first flux launch
+  if (c->dims == 1 || c->p.tintegr) flush_dt(c);
   if (c->f32) f32_step_body(c);
   else if (c->p.precision == BNZ_PREC_FP64_FMA) fma_step_body(c);
   else launch_step_body(c);
@@ -904,6 +906,7 @@ int bnz_get_shape(const bnz_ctx *c, int *ntot3, int *ng) {
 int bnz_upload(bnz_ctx *c, const double *prim, const double *bfld) {
   if (!c || !c->stream) return BNZ_EINVAL;
   c->dt_parts = 0;
+  c->pdt_pending = false;
   // ordered after the steps still running on the context's stream (a plain
   // hipMemcpy would not wait for a non-blocking stream)
   if (c->p.hydro && bfld)  // the hydro build has no field
@@ -964,6 +967,7 @@ int bnz_init_state(bnz_ctx *c) {
   if (!c || !c->stream) return BNZ_EINVAL;
   if (const int rc = user_sides_ready(c)) return rc;
   c->dt_parts = 0;
+  c->pdt_pending = false;
   launch_ghost(c);
   Box a = active_box(c);
   hipLaunchKernelGGL(k_interp_active, dim3(blocks_for(a.total())), dim3(BLK), 0, c->stream,
@@ -988,6 +992,7 @@ int bnz_new_dt(bnz_ctx *c, double *dt) {
   const double nomax = -1.;
   HIPCHK(c, hipMemcpyAsync(c->dev + 2, &nomax, sizeof(double), hipMemcpyHostToDevice, c->stream));
   any_dt(c, 0, 0.);
+  flush_dt(c);
   int rc = check_launch(c);
   if (rc) return rc;
   HIPCHK(c, hipMemcpyAsync(dt, c->dev, sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -1188,8 +1193,10 @@ int bnz_load_state(bnz_ctx *c, const void *host, long bytes) {
     HIPCHK(c, hipMemcpyAsync(c->part, &h.cflmax, sizeof(double), hipMemcpyHostToDevice,
                              c->stream));
     c->dt_parts = 1;
+    c->pdt_pending = false;
   } else {
     c->dt_parts = 0;
+    c->pdt_pending = false;
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->ready = true;
@@ -1317,6 +1324,7 @@ int bnz_set_tuning(bnz_ctx *c, int flux_chunk, int emf_chunk, int update_chunk) 
       HIPCHK(c, hipMemcpy(np, c->part, c->dt_parts * sizeof(double), hipMemcpyDeviceToDevice));
     (void)hipFree(c->part);
     c->part = np;
+    c->pdt_pending = false;
     c->npart_cap = need;
   }
   return BNZ_OK;

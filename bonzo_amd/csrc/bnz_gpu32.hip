@@ -25,15 +25,16 @@ using namespace bnz;
 namespace {
 
 // All face directions of a stage in one launch, one thread per cell of the
-// union of the face ranges (as the FP64 k_flux3).
+// union of the face ranges (as the FP64 k_flux3); the step's first one also
+// reduces its dt in workgroup 0 (DtFold).
 template <int ORDER, int RS, typename BF>
 __global__ __launch_bounds__(BLK) void k32_flux3(Geo g, const float *__restrict__ w,
                                                  const BF *__restrict__ bf,
                                                  float *__restrict__ fx, float *__restrict__ fy,
                                                  float *__restrict__ fz, Box box, Lims L,
-                                                 int *nguard, int ty,
-                                                 const double *__restrict__ dts) {
-  if (dts[0] == 0.) return;  // a step past t_max (see k_update)
+                                                 int *nguard, int ty, double *dev, DtFold dtf) {
+  if (dtf.npart > 0 && blockIdx.x == 0) dt_reduce_block(dtf, dev);
+  if (step_past_tmax(dtf, dev)) return;  // a step past t_max (see k_update)
   int i, j, k;
   if (!cell_index<false, 1, 2>(box, ty, i, j, k)) return;
   int ng = 0;
@@ -119,8 +120,10 @@ __global__ __launch_bounds__(BLK) void k32_dt_partial(Geo g, const float *__rest
   block_max_to(cds, part);
 }
 
+// adv: the time and step counter a step's fill advances (one launch of it)
 __global__ __launch_bounds__(BLK) void k32_ghost(Geo g, Ghost32 p, const int2 *__restrict__ pairs,
-                                                 int npairs) {
+                                                 int npairs, double *adv) {
+  time_advance_once(adv);
   const long t = xcd_block() * BLK + threadIdx.x;
   if (t >= npairs) return;
   const int2 ds = pairs[t];
@@ -128,7 +131,9 @@ __global__ __launch_bounds__(BLK) void k32_ghost(Geo g, Ghost32 p, const int2 *_
 }
 
 __global__ __launch_bounds__(BLK) void k32_ghost_ops(Geo g, Ghost32 p,
-                                                     const GhostOp *__restrict__ ops, int nops) {
+                                                     const GhostOp *__restrict__ ops, int nops,
+                                                     double *adv) {
+  time_advance_once(adv);
   const long t = xcd_block() * BLK + threadIdx.x;
   if (t >= nops) return;
   ghost32_op(p, ops[t]);
@@ -173,9 +178,10 @@ void launch_fluxes32(bnz_ctx *c, const float *w, const BF *bf, const Lims &L) {
   int *ng = (int *)(c->dev + 4);
   const Box u = flux3_box(c->dims, L);
   const Shape sh = work_shape32(u, c->dims, c->flux_chunk);
+  const DtFold dtf = take_dt(c);
   Launch t(c, ORDER ? "flux3_plm" : "flux3_dc");
   hipLaunchKernelGGL((k32_flux3<ORDER, RS, BF>), dim3(sh.blocks), dim3(BLK), 0, c->stream, c->g,
-                     w, bf, c->fx32, c->fy32, c->fz32, u, L, ng, sh.ty, c->dev);
+                     w, bf, c->fx32, c->fy32, c->fz32, u, L, ng, sh.ty, c->dev, dtf);
 }
 
 template <typename BF>
@@ -216,7 +222,9 @@ void emf32(bnz_ctx *c, const float *w, const Lims &L, const char *name) {
 #undef E32_ARGS
 }
 
-void ghost32(bnz_ctx *c) {
+// The FP32C ghost fill; a step's fill (step) also advances the time and
+// step counter.
+void ghost32(bnz_ctx *c, bool step = false) {
   Ghost32 p;
   int q = 0;
   for (int n = 0; n < 6; ++n) p.a[q++] = c->uh + n * c->ncell;
@@ -224,22 +232,32 @@ void ghost32(bnz_ctx *c) {
   for (int n = 0; n < 3; ++n) p.d[n] = c->b + n * c->ncell;
   p.fix32 = c->ufix32;
   p.fix64 = c->ufix;
-  Launch t(c, "ghost");
-  if (c->npass > 0) {  // user boundaries: one pass per side, in order
-    for (int s = 0; s < 6; ++s) {
-      const int n = c->pass_off[s + 1] - c->pass_off[s];
-      if (n > 0)
-        hipLaunchKernelGGL(k32_ghost_ops, dim3(blocks_for(n)), dim3(BLK), 0, c->stream, c->g, p,
-                           c->gops + c->pass_off[s], n);
+  double *adv = step ? c->dev : nullptr;
+  {
+    Launch t(c, "ghost");
+    if (c->npass > 0) {  // user boundaries: one pass per side, in order
+      for (int s = 0; s < 6; ++s) {
+        const int n = c->pass_off[s + 1] - c->pass_off[s];
+        if (n > 0) {
+          hipLaunchKernelGGL(k32_ghost_ops, dim3(blocks_for(n)), dim3(BLK), 0, c->stream, c->g,
+                             p, c->gops + c->pass_off[s], n, adv);
+          adv = nullptr;
+        }
+      }
+    } else {
+      if (c->nghost > 0) {
+        hipLaunchKernelGGL(k32_ghost, dim3(blocks_for(c->nghost)), dim3(BLK), 0, c->stream, c->g,
+                           p, c->ghost, c->nghost, adv);
+        adv = nullptr;
+      }
+      if (c->ngops > 0) {
+        hipLaunchKernelGGL(k32_ghost_ops, dim3(blocks_for(c->ngops)), dim3(BLK), 0, c->stream,
+                           c->g, p, c->gops, c->ngops, adv);
+        adv = nullptr;
+      }
     }
-    return;
   }
-  if (c->nghost > 0)
-    hipLaunchKernelGGL(k32_ghost, dim3(blocks_for(c->nghost)), dim3(BLK), 0, c->stream, c->g, p,
-                       c->ghost, c->nghost);
-  if (c->ngops > 0)
-    hipLaunchKernelGGL(k32_ghost_ops, dim3(blocks_for(c->ngops)), dim3(BLK), 0, c->stream, c->g,
-                       p, c->gops, c->ngops);
+  if (adv) launch_time_advance(c);
 }
 
 }  // namespace
@@ -272,6 +290,7 @@ int f32_from64(bnz_ctx *c) {
   hipLaunchKernelGGL(k32_from64, dim3(blocks_for(c->ncell)), dim3(BLK), 0, c->stream, c->g, c->w,
                      c->uh, c->ul, c->w32);
   c->dt_parts = 0;
+  c->pdt_pending = false;
   return check_launch(c);
 }
 
@@ -337,8 +356,7 @@ void f32_step_body(bnz_ctx *c) {
     stage32<true>(c, 0, integr_lims(c, 0));
     stage32<false>(c, c->order, integr_lims(c, c->order));
     std::swap(c->b, c->bs);
-    ghost32(c);
-    launch_time_advance(c);
+    ghost32(c, true);
     return;
   }
   Box grid;
@@ -376,8 +394,7 @@ void f32_step_body(bnz_ctx *c) {
   }
   c->dt_parts = nb;
   std::swap(c->b, c->bs);
-  ghost32(c);
-  launch_time_advance(c);
+  ghost32(c, true);
 }
 
 }  // namespace bnz

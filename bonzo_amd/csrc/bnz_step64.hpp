@@ -30,13 +30,15 @@ using namespace bnz;
 // All three face directions of a stage in one launch: one thread per cell of
 // the union of the face ranges computes the x, y and z faces whose lower
 // cell it is, so the prim stencil of a cell is fetched once per stage.
+// A step's first flux launch also reduces its dt in workgroup 0 (DtFold).
 template <int ORDER, int RS, bool TILED = false, bool CP = false>
 __global__ __launch_bounds__(BLK) void k_flux3(Geo g, const double *__restrict__ w,
                                                const double *__restrict__ bf,
                                                double *__restrict__ fx, double *__restrict__ fy,
                                                double *__restrict__ fz, Box box, Lims L,
-                                               int ty, const double *__restrict__ dts) {
-  if (dts[0] == 0.) return;  // a step past t_max (see k_update)
+                                               int ty, double *dev, DtFold dtf) {
+  if (dtf.npart > 0 && blockIdx.x == 0) dt_reduce_block(dtf, dev);
+  if (step_past_tmax(dtf, dev)) return;  // a step past t_max (see k_update)
   int i, j, k;
   if (!cell_index<TILED, 1, 2>(box, ty, i, j, k)) return;
   flux_cell<ORDER, RS, 0, CP>(g, w, bf, fx, fy, fz, L, i, j, k);
@@ -169,13 +171,14 @@ void launch_fluxes_cp(bnz_ctx *c, const double *w, const double *bf, const Lims 
   // all face directions in one launch over the union of the face ranges
   const Box u = flux3_box(c->dims, L);
   const Shape sh = work_shape(u, c->dims, c->flux_chunk);
+  const DtFold dtf = take_dt(c);
   Launch t(c, ORDER ? (CP ? "flux3_plm_char" : "flux3_plm") : "flux3_dc");
   if (sh.tiled)
     hipLaunchKernelGGL((k_flux3<ORDER, RS, true, CP>), dim3(sh.blocks), dim3(BLK), 0, c->stream,
-                       c->g, w, bf, c->fx, c->fy, c->fz, u, L, 0, c->dev);
+                       c->g, w, bf, c->fx, c->fy, c->fz, u, L, 0, c->dev, dtf);
   else
     hipLaunchKernelGGL((k_flux3<ORDER, RS, false, CP>), dim3(sh.blocks), dim3(BLK), 0, c->stream,
-                       c->g, w, bf, c->fx, c->fy, c->fz, u, L, sh.ty, c->dev);
+                       c->g, w, bf, c->fx, c->fy, c->fz, u, L, sh.ty, c->dev, dtf);
 }
 
 // PLM limits in characteristic variables when the context asks for it
@@ -344,8 +347,7 @@ void launch_rk3_body(bnz_ctx *c) {
   launch_flux_dispatch(c, c->order, c->w, c->bss, L);
   launch_emf(c, c->w, L, "emf_rk");
   launch_update_rk<3>(c, c->uss, c->bss, c->uc, c->b, L);
-  launch_ghost64(c);
-  launch_time_advance(c);
+  launch_ghost_step64(c);  // and the time advance
 }
 
 // One VL step after the dt kernels (vanleer.pyx:100-163).
@@ -371,8 +373,7 @@ void launch_step_body(bnz_ctx *c) {
     launch_update(c, false, L1);
   }
   std::swap(c->b, c->bs);
-  launch_ghost64(c);
-  launch_time_advance(c);
+  launch_ghost_step64(c);  // and the time advance
 }
 
 }  // namespace
