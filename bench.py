@@ -46,13 +46,16 @@ KERNEL_BYTES = {
 # WRITE_SIZE x 1.0, scaled by a plain device copy), from the rocprofv3 --pmc
 # runs summarised in the named profile; the bench itself runs no counters.
 PROFILED_TRAFFIC = {
-    ("fp64", "flux3_plm"): (6.008e9, "profiles/vl_step_kernels_turb256.md"),
-    ("fp32c", "update_corr"): (4.773e9, "profiles/vl_step_kernels_turb256.md"),
+    ("fp64fma", "flux3_plm"): (5.995e9, "profiles/vl_step_kernels_turb256.md"),
+    ("fp64", "flux3_plm"): (5.992e9, "profiles/vl_step_kernels_turb256.md"),
+    ("fp32c", "update_corr"): (4.774e9, "profiles/vl_step_kernels_turb256.md"),
+    ("fp32c", "flux3_plm"): (2.944e9, "profiles/vl_step_kernels_turb256.md"),
 }
 # Fraction of the sampled SIMDs' cycles spent issuing VALU work in the same
 # kernels (SQ_ACTIVE_INST_VALU x 4 / (GRBM_GUI_ACTIVE x SIMDs), same profile):
 # near 1 the launch is bound by arithmetic issue rather than by HBM.
-PROFILED_VALU_BUSY = {("fp64", "flux3_plm"): 0.72, ("fp32c", "update_corr"): 0.33}
+PROFILED_VALU_BUSY = {("fp64fma", "flux3_plm"): 0.58, ("fp64", "flux3_plm"): 0.71,
+                      ("fp32c", "update_corr"): 0.32, ("fp32c", "flux3_plm"): 0.85}
 # Algorithmic bytes per cell-update of a whole VL step: 5 state volumes of 9
 # words (FP64 360 B; FP32C 252 B with hi + lo on the stage-2 read of S_n and
 # the write of S_{n+1}).
@@ -333,6 +336,9 @@ def main():
                     help="batched workloads without the captured graph (for kernel traces)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB mebibytes (rocprof counter scale)")
+    ap.add_argument("--only", choices=("fp64", "fp64fma", "fp32c"), default="",
+                    help="time one path only and print its figures (for profiler runs: the "
+                         "IEEE and FMA units hold kernels of the same names)")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
@@ -353,6 +359,14 @@ def main():
         return
     n = args.n
     s, prim, bfld = problems.turbulence(n)
+    if args.only:
+        r = run_path(args.only, n, prim, bfld, args)
+        print(json.dumps({"path": args.only, "n": n, "value": r["value"], "unit": "cell-updates/s",
+                          "ms_per_step": r["ms_per_step"], "roofline": r["roofline"],
+                          "kernels_ms_per_launch": r["kernels"], "divb_max": r["divb_max"],
+                          "shapes": dict(zip(("flux", "emf", "update"), r["tuned"]))}),
+              flush=True)
+        return
     # FP64, bit for bit the oracle (IEEE, no contraction)
     rie = run_path("fp64", n, prim, bfld, args)
     # FP64 with multiply-adds fused (within 1e-12 of the oracle): the FP64 value
