@@ -27,7 +27,7 @@ ABI_SYMBOLS = (
     "bnz_set_kernel_timing",
     "bnz_kernel_timing",
     "bnz_state_size", "bnz_save_state", "bnz_load_state",
-    "bnz_gpu_flux_pencils", "bnz_hll_flux", "bnz_hlld_flux", "bnz_hllt_flux",
+    "bnz_gpu_flux_pencils", "bnz_hll_flux", "bnz_hlld_flux", "bnz_hllt_flux", "bnz_hllc_flux",
     "bnz_rsolver_rows_noalloc",
     "bnz_batch_create", "bnz_batch_destroy", "bnz_batch_last_error", "bnz_batch_box_shape",
     "bnz_batch_upload", "bnz_batch_download", "bnz_batch_init_state", "bnz_batch_advance",
@@ -117,7 +117,7 @@ def load():
                                     ctypes.POINTER(ctypes.c_long)]
     L.bnz_gpu_flux_pencils.argtypes = [ctypes.c_int, vp, vp, vp, vp, ctypes.c_long,
                                        ctypes.c_double]
-    for name in ("bnz_hll_flux", "bnz_hlld_flux", "bnz_hllt_flux"):
+    for name in ("bnz_hll_flux", "bnz_hlld_flux", "bnz_hllt_flux", "bnz_hllc_flux"):
         getattr(L, name).argtypes = [ctypes.POINTER(_dp)] * 3 + [_dp, ctypes.c_int, ctypes.c_int,
                                                                   ctypes.c_double]
         getattr(L, name).restype = None
@@ -138,7 +138,7 @@ def load():
     L.bnz_batch_set_tuning.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     for name in ABI_SYMBOLS:
         if name not in ("bnz_last_error", "bnz_batch_last_error", "bnz_hll_flux",
-                        "bnz_hlld_flux", "bnz_hllt_flux"):
+                        "bnz_hlld_flux", "bnz_hllt_flux", "bnz_hllc_flux"):
             getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L

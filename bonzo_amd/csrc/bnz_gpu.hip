@@ -1434,16 +1434,25 @@ int bnz_gpu_flux_pencils(int rsolver, const double *wl, const double *wr, const 
 
 
 // The reference's Riemann-solver seam, RSolverFunc: host row pointers flux[n],
-// wl[n], wr[n] (n < 9) and bx, faces i1..i2.  Each call packs the rows into
+// wl[n], wr[n] (n < rows) and bx, faces i1..i2.  Each call packs the rows into
 // one host block (wl 9n | wr 9n | bx n | flux 9n), copies it to the device,
 // runs k_pencil there and unpacks the flux rows.  The flux rows travel to
 // the device too, so a face whose solver matches no region keeps its old
-// row, as in the reference.
+// row, as in the reference.  rows is 9 (the MHD build: hll, hlld, hllt) or
+// HYDRO_ROWS (the hydro build's hllc, fluxes_c.c:230-327: rho, v, p and the
+// scalar, no field rows and no bx, which it never reads); the hydro rows are
+// staged with a zero field, for which the device HLLC is the hydro one bit
+// for bit.
+constexpr int HYDRO_ROWS = 6;
 static int seam_check(int rs, double **flux, double **wl, double **wr, double *bx, int i1,
-                      int i2, double gam) {
-  if (!flux || !wl || !wr || !bx || i2 < i1 || !(gam > 1.)) return BNZ_EINVAL;
-  if (rs != BNZ_RS_HLL && rs != BNZ_RS_HLLD && rs != BNZ_RS_HLLT) return BNZ_EINVAL;
-  for (int q = 0; q < NMODE; ++q)
+                      int i2, double gam, int rows = NMODE) {
+  if (!flux || !wl || !wr || i2 < i1 || !(gam > 1.)) return BNZ_EINVAL;
+  if (rows == NMODE) {
+    if (!bx || (rs != BNZ_RS_HLL && rs != BNZ_RS_HLLD && rs != BNZ_RS_HLLT)) return BNZ_EINVAL;
+  } else if (rows != HYDRO_ROWS || rs != BNZ_RS_HLLC) {
+    return BNZ_EINVAL;
+  }
+  for (int q = 0; q < rows; ++q)
     if (!flux[q] || !wl[q] || !wr[q]) return BNZ_EINVAL;
   return BNZ_OK;
 }
@@ -1452,14 +1461,14 @@ constexpr int SEAM_WORDS = 28;  // doubles staged per face
 
 // Pack, solve on the device through dev (SEAM_WORDS * n doubles) and unpack.
 static int seam_solve(int rs, double **flux, double **wl, double **wr, double *bx, int i1,
-                      long n, double gam, double *h, double *dev) {
+                      long n, double gam, double *h, double *dev, int rows = NMODE) {
   for (int q = 0; q < NMODE; ++q)
     for (long i = 0; i < n; ++i) {
-      h[q * n + i] = wl[q][i1 + i];
-      h[(9 + q) * n + i] = wr[q][i1 + i];
-      h[(19 + q) * n + i] = flux[q][i1 + i];
+      h[q * n + i] = q < rows ? wl[q][i1 + i] : 0.;
+      h[(9 + q) * n + i] = q < rows ? wr[q][i1 + i] : 0.;
+      h[(19 + q) * n + i] = q < rows ? flux[q][i1 + i] : 0.;
     }
-  for (long i = 0; i < n; ++i) h[18 * n + i] = bx[i1 + i];
+  for (long i = 0; i < n; ++i) h[18 * n + i] = bx ? bx[i1 + i] : 0.;
   if (hipMemcpy(dev, h, SEAM_WORDS * n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
     return BNZ_EHIP;
   const double gm1i = 1. / (gam - 1.);
@@ -1471,13 +1480,16 @@ static int seam_solve(int rs, double **flux, double **wl, double **wr, double *b
   else if (rs == BNZ_RS_HLLT)
     hipLaunchKernelGGL(k_pencil<BNZ_RS_HLLT>, dim3(blocks_for(n)), dim3(BLK), 0, 0, dwl, dwr, dbx,
                        dfl, n, gam, gm1i);
+  else if (rs == BNZ_RS_HLLC)
+    hipLaunchKernelGGL(k_pencil<BNZ_RS_HLLC>, dim3(blocks_for(n)), dim3(BLK), 0, 0, dwl, dwr, dbx,
+                       dfl, n, gam, gm1i);
   else
     hipLaunchKernelGGL(k_pencil<BNZ_RS_HLL>, dim3(blocks_for(n)), dim3(BLK), 0, 0, dwl, dwr, dbx,
                        dfl, n, gam, gm1i);
   if (hipGetLastError() != hipSuccess) return BNZ_EHIP;
   if (hipMemcpy(h + 19 * n, dfl, 9 * n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
     return BNZ_EHIP;
-  for (int q = 0; q < NMODE; ++q)
+  for (int q = 0; q < rows; ++q)
     for (long i = 0; i < n; ++i) flux[q][i1 + i] = h[(19 + q) * n + i];
   return BNZ_OK;
 }
@@ -1512,8 +1524,8 @@ static int scratch_for_current_device(double **out) {
 // Staging memory of the seam is kept per host thread (the reference calls
 // the solver from OpenMP threads) and grows with the longest pencil.
 static int pencil_rows(int rs, double **flux, double **wl, double **wr, double *bx, int i1,
-                       int i2, double gam) {
-  const int rc = seam_check(rs, flux, wl, wr, bx, i1, i2, gam);
+                       int i2, double gam, int rows) {
+  const int rc = seam_check(rs, flux, wl, wr, bx, i1, i2, gam, rows);
   if (rc != BNZ_OK) return rc;
   const long n = (long)i2 - i1 + 1;
   thread_local std::vector<double> host;
@@ -1528,12 +1540,12 @@ static int pencil_rows(int rs, double **flux, double **wl, double **wr, double *
     cap = need;
   }
   host.resize(need);
-  return seam_solve(rs, flux, wl, wr, bx, i1, n, gam, host.data(), dev);
+  return seam_solve(rs, flux, wl, wr, bx, i1, n, gam, host.data(), dev, rows);
 }
 
 static void pencil_seam(const char *name, int rs, double **flux, double **wl, double **wr,
-                        double *bx, int i1, int i2, double gam) {
-  const int rc = pencil_rows(rs, flux, wl, wr, bx, i1, i2, gam);
+                        double *bx, int i1, int i2, double gam, int rows = NMODE) {
+  const int rc = pencil_rows(rs, flux, wl, wr, bx, i1, i2, gam, rows);
   if (rc != BNZ_OK)  // the seam returns nothing: report, leave the flux rows as they were
     std::fprintf(stderr, "%s: faces %d..%d not computed (error %d)\n", name, i1, i2, rc);
 }
@@ -1549,6 +1561,10 @@ void bnz_hlld_flux(double **flux, double **wl, double **wr, double *bx, int i1, 
 void bnz_hllt_flux(double **flux, double **wl, double **wr, double *bx, int i1, int i2,
                    double gam) {
   pencil_seam("bnz_hllt_flux", BNZ_RS_HLLT, flux, wl, wr, bx, i1, i2, gam);
+}
+void bnz_hllc_flux(double **flux, double **wl, double **wr, double *bx, int i1, int i2,
+                   double gam) {
+  pencil_seam("bnz_hllc_flux", BNZ_RS_HLLC, flux, wl, wr, bx, i1, i2, gam, HYDRO_ROWS);
 }
 
 int bnz_rsolver_rows_noalloc(int rsolver, double **flux, double **wl, double **wr, double *bx,

@@ -246,6 +246,12 @@ void bnz_hlld_flux(double **flux, double **wl, double **wr, double *bx, int i1, 
                    double gam);
 void bnz_hllt_flux(double **flux, double **wl, double **wr, double *bx, int i1, int i2,
                    double gam);
+/* The hydro build's hllc_flux (fluxes_c.c:230-327, bound in the build
+   without -mf, integrator.pyx:259-264): the same seam with that build's 6
+   rows (rho, vx, vy, vz, p, scalar; NMODE = 6) and bx unused (the hydro
+   driver passes no field row, godunov.pyx:111-112); may be NULL. */
+void bnz_hllc_flux(double **flux, double **wl, double **wr, double *bx, int i1, int i2,
+                   double gam);
 
 /* No counterpart in the reference.  The Riemann-solver seam above for
    callers that cannot allocate memory (tight per-pencil loops, harnesses

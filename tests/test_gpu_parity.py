@@ -175,6 +175,36 @@ def test_rsolver_seam_matches_oracle_on_host_rows():
             f"{rs}: max diff {np.nanmax(np.abs(got - want))}"
 
 
+def test_hydro_hllc_seam_on_six_rows():
+    """bnz_hllc_flux has the hydro build's RSolverFunc rows (rho, vx, vy, vz,
+    p, scalar; fluxes_c.c:230-327 with NMODE = 6) and no bx (NULL): its six
+    flux rows equal the oracle's HLLC on the same states with a zero field,
+    bit for bit, only at faces i1..i2; nothing beyond row 5 is touched."""
+    from bonzo_amd._lib import load
+    from oracle import oracle as orc
+    from oracle.oracle import _ptr, _rows
+    from tests import mathref as M
+    say("case: RSolverFunc seam hllc on 6 hydro rows, 1500 faces, i1=3, i2=1490")
+    rng = np.random.default_rng(23)
+    n, i1, i2 = 1500, 3, 1490
+    wl, wr, _ = M.random_states(rng, n, bx_min=0.0)
+    wl[1] += rng.uniform(-3, 3, n)
+    wr[1] += rng.uniform(-3, 3, n)
+    wl[6:] = 0.0
+    wr[6:] = 0.0
+    sentinel = 7.25
+    want = np.full((9, n), sentinel)
+    orc.lib().or_hllc_flux(_rows(want), _rows(wl), _rows(wr), _ptr(np.zeros(n)), i1, i2,
+                           5.0 / 3.0)
+    wl6 = np.ascontiguousarray(wl[:6])
+    wr6 = np.ascontiguousarray(wr[:6])
+    got = np.full((6, n), sentinel)
+    load().bnz_hllc_flux(_rows(got), _rows(wl6), _rows(wr6), None, i1, i2, 5.0 / 3.0)
+    assert np.all(got[:, :i1] == sentinel) and np.all(got[:, i2 + 1:] == sentinel)
+    assert np.array_equal(got, want[:6]), f"max diff {np.max(np.abs(got - want[:6]))}"
+    assert np.any(got[0, i1:i2 + 1] != 0.0)
+
+
 def test_orszag_tang_2d_bitwise():
     say("case: Orszag-Tang 64^2, 20 steps, hlld+plm, periodic")
     res = run_pair(P.orszag_tang(64), steps=20)
