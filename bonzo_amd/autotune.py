@@ -35,8 +35,12 @@ def autotune(ctx, steps=2, candidates=CANDIDATES):
     """Pick (flux_chunk, emf_chunk, update_chunk) for ``ctx`` and apply it.
 
     Returns (best, table) where table[c] holds the per-class milliseconds of
-    candidate c.  The context must hold an initialised state."""
+    candidate c.  The context must hold an initialised state.  Work shapes
+    exist on 3D grids only; other grids keep the defaults."""
     table = {}
+    if ctx.ntot[2] == 1:
+        ctx.set_tuning(0, 0, 0)
+        return (0, 0, 0), table
     for c in candidates:
         ctx.set_tuning(c, c, c)
         ctx.run_steps(1)                      # first launch of a shape pays setup

@@ -273,6 +273,31 @@ struct EmfGlobal {
   }
 };
 
+// The 2D edge EMFs formed where the update needs them (small 2D grids, no
+// EMF launch): emf_edge_d<2>'s expressions at the edge whose lower corner is
+// cell c + off, from the stage's prim we and flux arrays.  Each edge is
+// formed by the (up to) four cells around it, the same value each time.
+// Threads with no role in the update read at off = 0; an edge on the grid's
+// first row or column has no cells below it and gives 0 (no role reads it).
+struct EmfFly2D {
+  const double *we, *fx, *fy;
+  long sv, sy, c;
+  int i, j;
+  __device__ __forceinline__ double operator()(int comp, long off) const {
+    const long e = c + off;
+    if (comp == 0) return -fy[BZ * sv + e];
+    if (comp == 1) return fx[BZ * sv + e];
+    const int je = j + (off >= sy ? 1 : 0);
+    const int ie = i + (int)(off >= sy ? off - sy : off);
+    if (ie < 1 || je < 1) return 0.;
+    const double *w = we;
+    return (0.5 * (-fx[BY * sv + e - sy] - fx[BY * sv + e] + fy[BX * sv + e - 1] +
+                   fy[BX * sv + e]) -
+            0.25 * (ec_comp(w, sv, e - 1, 2) + ec_comp(w, sv, e, 2) +
+                    ec_comp(w, sv, e - sy - 1, 2) + ec_comp(w, sv, e - sy, 2)));
+  }
+};
+
 // Face fluxes as the update reads them: variable n of the face whose lower
 // cell is c + off.  FluxGlobal reads the arrays of the flux kernels.
 struct FluxGlobal {
@@ -450,7 +475,9 @@ __device__ __forceinline__ double update_cell_e(const Geo &g, const GM &gm, cons
   return 0.;
 }
 
-template <int DIMS, bool PRED, bool DTRED, bool UG = false>
+// FLY (2D): the edge EMFs formed from the stage's prim we (EmfFly2D) instead
+// of read from ee.
+template <int DIMS, bool PRED, bool DTRED, bool UG = false, bool FLY = false>
 __device__ __forceinline__ double update_cell_d(const Geo &g, const double *w0,
                                                 const double *__restrict__ fx,
                                                 const double *__restrict__ fy,
@@ -459,12 +486,18 @@ __device__ __forceinline__ double update_cell_d(const Geo &g, const double *w0,
                                                 const double *__restrict__ b0,
                                                 double *__restrict__ b1, double *w1,
                                                 const Lims &L, const double *__restrict__ dts,
-                                                const Box &act, int fms_ref, int i, int j, int k) {
+                                                const Box &act, int fms_ref, int i, int j, int k,
+                                                const double *we = nullptr) {
   const long c = ((long)k * g.nt[1] + j) * g.nt[0] + i;
   using GM = typename std::conditional<UG, GeoUni, GeoArr>::type;
   const GM gm{g};
   static_assert(DIMS >= 2, "1D grids run stage1_cell");
+  static_assert(!FLY || DIMS == 2, "EMFs in the update on 2D grids only");
   const FluxGlobal fl{fx, fy, fz, g.sv, c};
+  if (FLY)
+    return update_cell_e<DIMS, PRED, DTRED>(g, gm, w0, fl,
+                                            EmfFly2D{we, fx, fy, g.sv, (long)g.sy, c, i, j}, b0,
+                                            b1, w1, L, dts, act, fms_ref, i, j, k);
   return update_cell_e<DIMS, PRED, DTRED>(g, gm, w0, fl, EmfGlobal{ee, g.sv, c}, b0, b1, w1, L,
                                           dts, act, fms_ref, i, j, k);
 }

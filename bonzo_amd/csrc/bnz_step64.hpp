@@ -136,7 +136,8 @@ __global__ __launch_bounds__(BLK) void k_cons_of_prim(Geo g, const double *__res
   for (int n = 0; n < NMODE; ++n) u[n * g.sv + t] = uv[n];
 }
 
-template <int DIMS, bool PRED, bool DTRED = false, bool TILED = false, bool UG = false>
+template <int DIMS, bool PRED, bool DTRED = false, bool TILED = false, bool UG = false,
+          bool FLY = false>
 __global__ __launch_bounds__(BLK) void k_update(Geo g, const double *w0, const double *__restrict__ fx,
                                                 const double *__restrict__ fy,
                                                 const double *__restrict__ fz,
@@ -145,7 +146,7 @@ __global__ __launch_bounds__(BLK) void k_update(Geo g, const double *w0, const d
                                                 double *__restrict__ b1, double *w1, Lims L,
                                                 const double *__restrict__ dts, Box grid, Box act,
                                                 int fms_ref, double *__restrict__ part,
-                                                int ty) {
+                                                int ty, const double *we = nullptr) {
   int i, j, k;
   double cdsmax = 0.;
   const bool have = cell_index<TILED, 1, 2>(grid, ty, i, j, k);
@@ -157,8 +158,8 @@ __global__ __launch_bounds__(BLK) void k_update(Geo g, const double *w0, const d
     return;  // uniform over the workgroup; the CFL partials stay those of the state
   }
   if (have)
-    cdsmax = update_cell_d<DIMS, PRED, DTRED, UG>(g, w0, fx, fy, fz, ee, b0, b1, w1, L, dts, act,
-                                                  fms_ref, i, j, k);
+    cdsmax = update_cell_d<DIMS, PRED, DTRED, UG, FLY>(g, w0, fx, fy, fz, ee, b0, b1, w1, L, dts,
+                                                       act, fms_ref, i, j, k, we);
   if (DTRED) block_max_to(cdsmax, part);
 }
 
@@ -234,7 +235,19 @@ void launch_emf(bnz_ctx *c, const double *w, const Lims &L, const char *name) {
 }
 
 // One VL step after the dt kernels (vanleer.pyx:100-163).
-void launch_update(bnz_ctx *c, bool pred, const Lims &L) {
+// Small 2D grids form the edge EMFs in the update launches (EmfFly2D), so
+// their steps run no EMF launch: a launch there costs more than the four
+// cells' redundant EMF arithmetic.  On 2D grids emf_chunk (no work shape
+// there) picks it: 0 by size, 1 EMF launches, 2 or more in the updates.
+constexpr long EMF_FLY_2D_CELLS = 1L << 18;
+bool emf_fly(const bnz_ctx *c) {
+  if (c->dims != 2 || c->p.ct_upwind) return false;
+  if (c->emf_chunk == 0) return c->ncell <= EMF_FLY_2D_CELLS;
+  return c->emf_chunk >= 2;
+}
+
+// we: with emf_fly, the prim the stage's EMFs come from
+void launch_update(bnz_ctx *c, bool pred, const Lims &L, const double *we = nullptr) {
   double *w1 = pred ? c->ws : c->w;
   Box grid;
   for (int q = 0; q < 3; ++q) {
@@ -260,6 +273,9 @@ void launch_update(bnz_ctx *c, bool pred, const Lims &L) {
   } else if (c->dims == 3) {
     if (pred) hipLaunchKernelGGL((k_update<3, true, false, false>), UPD_ARGS);
     else hipLaunchKernelGGL((k_update<3, false, true, false>), UPD_ARGS);
+  } else if (we) {  // 2D, EMFs formed in the update
+    if (pred) hipLaunchKernelGGL((k_update<2, true, false, false, false, true>), UPD_ARGS, we);
+    else hipLaunchKernelGGL((k_update<2, false, true, false, false, true>), UPD_ARGS, we);
   } else {  // 2D (1D grids run k_step1)
     if (pred) hipLaunchKernelGGL((k_update<2, true, false, false>), UPD_ARGS);
     else hipLaunchKernelGGL((k_update<2, false, true, false>), UPD_ARGS);
@@ -363,14 +379,15 @@ void launch_step_body(bnz_ctx *c) {
   } else {
     // predictor: donor cell, dt/2, lims of order 0; ws and bs are new buffers
     launch_flux_dispatch(c, 0, c->w, c->b, L0);
-    launch_emf(c, c->w, L0, "emf_pred");
-    launch_update(c, true, L0);
+    const bool fly = emf_fly(c);
+    if (!fly) launch_emf(c, c->w, L0, "emf_pred");
+    launch_update(c, true, L0, fly ? c->w : nullptr);
     // corrector: reconstruction order, dt; prim is updated in place (each
     // cell reads only its own w_n), b_{n+1} goes to the predictor's face
     // buffer, which is dead once the corrector fluxes and EMFs exist
     launch_flux_dispatch(c, c->order, c->ws, c->bs, L1);
-    launch_emf(c, c->ws, L1, "emf_corr");
-    launch_update(c, false, L1);
+    if (!fly) launch_emf(c, c->ws, L1, "emf_corr");
+    launch_update(c, false, L1, fly ? c->ws : nullptr);
   }
   std::swap(c->b, c->bs);
   launch_ghost_step64(c);  // and the time advance

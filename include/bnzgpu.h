@@ -172,9 +172,12 @@ int bnz_load_state(bnz_ctx *ctx, const void *host, long bytes);
 /* Work shapes of the 3D Godunov-flux, edge-EMF and stage-update kernels:
    0 one thread per face/edge/cell in x-fastest order (the default), 1 tiles
    of 64 x 4 cells ordered along z (FP64; FP32C runs it flat), n > 1 slab
-   columns of n rows walked plane by plane.  Results do not depend on the
-   shape.  Chosen by the autotuner (bonzo_amd/autotune.py); no counterpart
-   in the reference. */
+   columns of n rows walked plane by plane.  On 2D FP64 grids emf_chunk
+   places the edge EMFs instead: 0 (the default) forms them inside the
+   update launches up to 2^18 cells with ghosts and in launches of their own
+   beyond, 1 always in launches of their own, 2 or more always inside the
+   updates.  Results do not depend on either choice.  Chosen by the
+   autotuner (bonzo_amd/autotune.py, 3D); no counterpart in the reference. */
 int bnz_set_tuning(bnz_ctx *ctx, int flux_chunk, int emf_chunk, int update_chunk);
 
 /* User boundary (flag 3) of one side.  The reference leaves such a side's

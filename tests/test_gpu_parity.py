@@ -205,6 +205,35 @@ def test_hydro_hllc_seam_on_six_rows():
     assert np.any(got[0, i1:i2 + 1] != 0.0)
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp64fma"])
+def test_2d_emf_placement_does_not_change_the_run(prec):
+    """On 2D grids the edge EMFs are formed either in launches of their own
+    or inside the update launches (emf_chunk 1 / 2; by default by grid
+    size): the same expressions, so the same state bit for bit, and the
+    IEEE unit equal to the oracle either way (periodic Orszag-Tang and a
+    blast between reflecting walls)."""
+    from bonzo_amd._lib import Context
+    say(f"case: 2D EMF placement (own launches / in the updates), {prec}")
+    for name, (s, prim, bfld), bc, steps in (
+            ("ot48", P.orszag_tang(48), (0,) * 6, 12),
+            ("blast40", P.blast2d(40, seed=11), (2, 2, 2, 2, 0, 0), 10)):
+        out = []
+        for mode in (1, 2):
+            g = Context(s.nx, s.ny, 1, bc=bc, precision=prec)
+            g.set_tuning(0, mode, 0)
+            g.upload(prim, bfld)
+            g.init_state()
+            g.run_steps(steps)
+            out.append(g.download() + (g.time(),))
+            g.close()
+        (pa, ba, ta), (pb, bb, tb) = out
+        assert np.array_equal(pa, pb) and np.array_equal(ba, bb) and ta == tb, name
+        if prec == "fp64":
+            res = run_pair((s, prim, bfld), steps=steps, bc=bc)
+            assert np.array_equal(res["gp"], pa) and np.array_equal(res["gb"], ba), name
+            assert compare(res, name)
+
+
 def test_orszag_tang_2d_bitwise():
     say("case: Orszag-Tang 64^2, 20 steps, hlld+plm, periodic")
     res = run_pair(P.orszag_tang(64), steps=20)
