@@ -513,6 +513,18 @@ inline Box flux3_box(int dims, const Lims &L) {
   return b;
 }
 
+// Small 2D grids form the edge EMFs in the update launches (EmfFly2D,
+// Emf32Fly2D), so their steps run no EMF launch: a launch there costs more
+// than the four cells' redundant EMF arithmetic.  On 2D grids emf_chunk (no
+// work shape there) picks it: 0 by size, 1 EMF launches, 2 or more in the
+// updates.
+constexpr long EMF_FLY_2D_CELLS = 1L << 18;
+inline bool emf_fly(const bnz_ctx *c) {
+  if (c->dims != 2 || c->p.ct_upwind) return false;
+  if (c->emf_chunk == 0) return c->ncell <= EMF_FLY_2D_CELLS;
+  return c->emf_chunk >= 2;
+}
+
 inline Box active_box(const bnz_ctx *c) {
   Box b;
   b.lo[0] = c->i1; b.n[0] = c->i2 - c->i1 + 1;

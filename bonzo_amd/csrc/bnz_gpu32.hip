@@ -56,23 +56,26 @@ __global__ __launch_bounds__(BLK) void k32_emf(Geo g, const float *__restrict__ 
 }
 
 // Predictor (update32_pred), one thread per cell of the grid.
-template <int DIMS, bool UG = false>
+// FLY (2D): the edge EMFs formed from the stage's prim we (Emf32Fly2D).
+template <int DIMS, bool UG = false, bool FLY = false>
 __global__ __launch_bounds__(BLK) void k32_update_pred(Geo g, Pred32 a, Lims L,
                                                        const double *__restrict__ dts,
-                                                       Box grid, int ty) {
+                                                       Box grid, int ty, const float *we = nullptr,
+                                                       Walls wl = Walls{}) {
   int i, j, k;
   if (dts[0] == 0.) return;  // a step past t_max (see k_update)
   if (!cell_index<false, 1, 2>(grid, ty, i, j, k)) return;
-  update32_pred<DIMS, typename std::conditional<UG, GeoUni, GeoArr>::type>(g, a, L, dts, i, j,
-                                                                          k);
+  update32_pred<DIMS, typename std::conditional<UG, GeoUni, GeoArr>::type, FLY>(g, a, L, dts, i,
+                                                                               j, k, we, wl);
 }
 
 // Corrector (update32_corr) with the CFL maximum reduced per block.
-template <int DIMS, bool UG = false>
+template <int DIMS, bool UG = false, bool FLY = false>
 __global__ __launch_bounds__(BLK) void k32_update_corr(Geo g, Corr32 a, Lims L,
                                                        const double *__restrict__ dts, Box act,
                                                        int fms_ref, double *__restrict__ part,
-                                                       Box grid, int ty) {
+                                                       Box grid, int ty, const float *we = nullptr,
+                                                       Walls wl = Walls{}) {
   int i, j, k;
   double cdsmax = 0.;
   const bool have = cell_index<false, 1, 2>(grid, ty, i, j, k);
@@ -84,8 +87,8 @@ __global__ __launch_bounds__(BLK) void k32_update_corr(Geo g, Corr32 a, Lims L,
     return;
   }
   if (have)
-    cdsmax = update32_corr<DIMS, typename std::conditional<UG, GeoUni, GeoArr>::type>(
-        g, a, L, dts, act, fms_ref, i, j, k);
+    cdsmax = update32_corr<DIMS, typename std::conditional<UG, GeoUni, GeoArr>::type, FLY>(
+        g, a, L, dts, act, fms_ref, i, j, k, we, wl);
   block_max_to(cdsmax, part);
 }
 
@@ -367,20 +370,24 @@ void f32_step_body(bnz_ctx *c) {
   const Shape us = work_shape32(grid, c->dims, c->update_chunk);
   const int nb = us.blocks;
   Lims L0 = integr_lims(c, 0);
+  const bool fly = emf_fly(c);
+  const Walls wl = walls32(c);
   fluxes32(c, 0, c->w32, c->b, L0);
-  emf32(c, c->w32, L0, "emf_pred");
+  if (!fly) emf32(c, c->w32, L0, "emf_pred");
   {
     Pred32 a{c->w32, c->uh, c->b, c->fx32, c->fy32, c->fz32, c->ee32, c->bs32, c->ws32};
     Launch t(c, "update_pred");
 #define P32_ARGS dim3(nb), dim3(BLK), 0, c->stream, c->g, a, L0, c->dev, grid, us.ty
     if (c->dims == 3 && c->g.unif) hipLaunchKernelGGL((k32_update_pred<3, true>), P32_ARGS);
     else if (c->dims == 3) hipLaunchKernelGGL(k32_update_pred<3>, P32_ARGS);
+    else if (fly)
+      hipLaunchKernelGGL((k32_update_pred<2, false, true>), P32_ARGS, (const float *)c->w32, wl);
     else hipLaunchKernelGGL(k32_update_pred<2>, P32_ARGS);
 #undef P32_ARGS
   }
   Lims L1 = integr_lims(c, c->order);
   fluxes32(c, c->order, c->ws32, c->bs32, L1);
-  emf32(c, c->ws32, L1, "emf_corr");
+  if (!fly) emf32(c, c->ws32, L1, "emf_corr");
   {
     Corr32 a{c->uh, c->ul, c->b, c->fx32, c->fy32, c->fz32, c->ee32, c->bs, c->w32};
     Launch t(c, "update_corr");
@@ -389,6 +396,8 @@ void f32_step_body(bnz_ctx *c) {
       grid, us.ty
     if (c->dims == 3 && c->g.unif) hipLaunchKernelGGL((k32_update_corr<3, true>), C32_ARGS);
     else if (c->dims == 3) hipLaunchKernelGGL(k32_update_corr<3>, C32_ARGS);
+    else if (fly)
+      hipLaunchKernelGGL((k32_update_corr<2, false, true>), C32_ARGS, (const float *)c->ws32, wl);
     else hipLaunchKernelGGL(k32_update_corr<2>, C32_ARGS);
 #undef C32_ARGS
   }

@@ -158,6 +158,32 @@ struct Emf32Global {
   }
 };
 
+// The FP32 2D edge EMFs formed where the update needs them (small 2D grids,
+// no EMF launch): emf32_edge<2>'s expressions and wall zeroing at the edge
+// whose lower corner is cell c + off, from the stage's FP32 prim we (as the
+// FP64 EmfFly2D; an edge on the grid's first row or column gives 0).
+struct Emf32Fly2D {
+  const float *we, *fx, *fy;
+  long sv, sy, c;
+  int i, j;
+  Walls wl;
+  __device__ __forceinline__ float operator()(int comp, long off) const {
+    const long e = c + off;
+    const int je = j + (off >= sy ? 1 : 0);
+    const int ie = i + (int)(off >= sy ? off - sy : off);
+    const bool wx = ie == wl.at[0] || ie == wl.at[1];
+    const bool wy = je == wl.at[2] || je == wl.at[3];
+    if (comp == 0) return wy ? 0.f : -fy[BZ * sv + e];
+    if (comp == 1) return wx ? 0.f : fx[BZ * sv + e];
+    if (wx || wy || ie < 1 || je < 1) return 0.f;
+    const float *w = we;
+    return 0.5f * (-fx[BY * sv + e - sy] - fx[BY * sv + e] + fy[BX * sv + e - 1] +
+                   fy[BX * sv + e]) -
+           0.25f * (ec32(w, sv, e - 1, 2) + ec32(w, sv, e, 2) + ec32(w, sv, e - sy - 1, 2) +
+                    ec32(w, sv, e - sy, 2));
+  }
+};
+
 // Face-field update by Stokes (ct.pyx:15-91) in working type R from face
 // values b and FP32 EMFs em: the lower faces of cell c (lo) and its upper
 // faces (up).  Neighbour offsets ox, oy, oz are zero where the cell needs no
@@ -338,15 +364,22 @@ __device__ __forceinline__ void update32_pred_e(const Geo &g, const Pred32 &a, c
   }
 }
 
-// 2D and 3D updates read the flux and EMF arrays (1D grids run stage32_cell).
-template <int DIMS, class GM = GeoArr>
+// 2D and 3D updates read the flux and EMF arrays (1D grids run stage32_cell);
+// FLY (2D) forms the EMFs from the stage's prim we instead (Emf32Fly2D).
+template <int DIMS, class GM = GeoArr, bool FLY = false>
 __device__ __forceinline__ void update32_pred(const Geo &g, const Pred32 &a, const Lims &L,
                                               const double *__restrict__ dts, int i, int j,
-                                              int k) {
+                                              int k, const float *we = nullptr,
+                                              const Walls &wl = Walls{}) {
   static_assert(DIMS >= 2, "1D grids run stage32_cell");
+  static_assert(!FLY || DIMS == 2, "EMFs in the update on 2D grids only");
   const long c = ((long)k * g.nt[1] + j) * g.nt[0] + i, sv = g.sv;
-  update32_pred_e<DIMS, GM>(g, a, L, dts, FluxArr32{a.fx, a.fy, a.fz, sv, c},
-                            Emf32Global{a.ee, sv, c}, i, j, k);
+  if (FLY)
+    update32_pred_e<DIMS, GM>(g, a, L, dts, FluxArr32{a.fx, a.fy, a.fz, sv, c},
+                              Emf32Fly2D{we, a.fx, a.fy, sv, (long)g.sy, c, i, j, wl}, i, j, k);
+  else
+    update32_pred_e<DIMS, GM>(g, a, L, dts, FluxArr32{a.fx, a.fy, a.fz, sv, c},
+                              Emf32Global{a.ee, sv, c}, i, j, k);
 }
 
 
@@ -435,12 +468,20 @@ __device__ __forceinline__ double update32_corr_e(const Geo &g, const Corr32 &a,
   return cdsmax;
 }
 
-template <int DIMS, class GM = GeoArr>
+template <int DIMS, class GM = GeoArr, bool FLY = false>
 __device__ __forceinline__ double update32_corr(const Geo &g, const Corr32 &a, const Lims &L,
                                                 const double *__restrict__ dts, const Box &act,
-                                                int fms_ref, int i, int j, int k) {
+                                                int fms_ref, int i, int j, int k,
+                                                const float *we = nullptr,
+                                                const Walls &wl = Walls{}) {
   static_assert(DIMS >= 2, "1D grids run stage32_cell");
+  static_assert(!FLY || DIMS == 2, "EMFs in the update on 2D grids only");
   const long c = ((long)k * g.nt[1] + j) * g.nt[0] + i, sv = g.sv;
+  if (FLY)
+    return update32_corr_e<DIMS, GM>(g, a, L, dts, act, fms_ref,
+                                     FluxArr32{a.fx, a.fy, a.fz, sv, c},
+                                     Emf32Fly2D{we, a.fx, a.fy, sv, (long)g.sy, c, i, j, wl}, i,
+                                     j, k);
   return update32_corr_e<DIMS, GM>(g, a, L, dts, act, fms_ref,
                                    FluxArr32{a.fx, a.fy, a.fz, sv, c}, Emf32Global{a.ee, sv, c},
                                    i, j, k);

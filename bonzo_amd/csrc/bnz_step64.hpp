@@ -235,17 +235,6 @@ void launch_emf(bnz_ctx *c, const double *w, const Lims &L, const char *name) {
 }
 
 // One VL step after the dt kernels (vanleer.pyx:100-163).
-// Small 2D grids form the edge EMFs in the update launches (EmfFly2D), so
-// their steps run no EMF launch: a launch there costs more than the four
-// cells' redundant EMF arithmetic.  On 2D grids emf_chunk (no work shape
-// there) picks it: 0 by size, 1 EMF launches, 2 or more in the updates.
-constexpr long EMF_FLY_2D_CELLS = 1L << 18;
-bool emf_fly(const bnz_ctx *c) {
-  if (c->dims != 2 || c->p.ct_upwind) return false;
-  if (c->emf_chunk == 0) return c->ncell <= EMF_FLY_2D_CELLS;
-  return c->emf_chunk >= 2;
-}
-
 // we: with emf_fly, the prim the stage's EMFs come from
 void launch_update(bnz_ctx *c, bool pred, const Lims &L, const double *we = nullptr) {
   double *w1 = pred ? c->ws : c->w;

@@ -205,13 +205,13 @@ def test_hydro_hllc_seam_on_six_rows():
     assert np.any(got[0, i1:i2 + 1] != 0.0)
 
 
-@pytest.mark.parametrize("prec", ["fp64", "fp64fma"])
+@pytest.mark.parametrize("prec", ["fp64", "fp64fma", "fp32c"])
 def test_2d_emf_placement_does_not_change_the_run(prec):
     """On 2D grids the edge EMFs are formed either in launches of their own
     or inside the update launches (emf_chunk 1 / 2; by default by grid
-    size): the same expressions, so the same state bit for bit, and the
-    IEEE unit equal to the oracle either way (periodic Orszag-Tang and a
-    blast between reflecting walls)."""
+    size): the same expressions (in FP32C with the same in-wall zeros), so
+    the same state bit for bit, and the IEEE unit equal to the oracle either
+    way (periodic Orszag-Tang and a blast between reflecting walls)."""
     from bonzo_amd._lib import Context
     say(f"case: 2D EMF placement (own launches / in the updates), {prec}")
     for name, (s, prim, bfld), bc, steps in (

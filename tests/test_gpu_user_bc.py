@@ -87,6 +87,41 @@ def test_oblique_tube_user_sides_match_oracle(prec):
     o.close()
 
 
+def test_oblique_tube_user_sides_rk3():
+    """The same user sides with the RK3 integrator (ng = 9, so the slabs are
+    nine layers deep and the y copies reach nine rows in): the device bit
+    for bit with the oracle after 12 steps."""
+    from bonzo_amd._lib import Context
+    from oracle.oracle import OracleSim
+    say("case: oblique tube 128x16 user sides, RK3, 12 steps")
+    s0, prim0, bfld0, _ = P.oblique_tube(16, aspect=8)
+    s, prim, bfld, user = P.oblique_tube(16, ng=9, aspect=8)
+    bc = (3, 3, 3, 3, 0, 0)
+    o = OracleSim(s.nx, s.ny, 1, bc=bc, lmax=s.lmax, tintegr="rk3")
+    g = Context(s.nx, s.ny, 1, bc=bc, lmax=s.lmax, tintegr="rk3")
+    assert o.ng == g.ng == 9
+    for (a, d), spec in user.items():
+        src, fixed = U.arrays(spec, o.ntot, o.ng, s.dims, a, d)
+        o.set_user_bc(a, d, src, fixed)
+        g.set_user_bc(a, d, spec)
+    o.prim[:] = prim
+    o.bfld[:] = bfld
+    o.init()
+    g.upload(prim, bfld)
+    g.init_state()
+    dt = 0.5 * o.new_dt()
+    for _ in range(12):
+        o.rk3_step(dt)
+    for _ in range(12):
+        g.vl_step(dt)
+    gp, gb = g.download()
+    assert np.array_equal(gp, o.prim) and np.array_equal(gb, o.bfld)
+    sk, sj, si = s.active()
+    assert np.max(np.abs(o.prim[0][sk, sj, si] - prim[0][sk, sj, si])) > 1e-2
+    g.close()
+    o.close()
+
+
 def test_oblique_tube_user_sides_fp32c():
     """FP32C with user sides (fixed rows converted to its conserved hi words
     and FP32 prim once): within the FP32C shock-tube bar of the FP64 path
