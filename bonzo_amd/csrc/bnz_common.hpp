@@ -525,6 +525,14 @@ inline bool emf_fly(const bnz_ctx *c) {
   return c->emf_chunk >= 2;
 }
 
+// Small 2D and 3D grids in the flat work order give each flux workgroup one
+// face direction (k_flux3 SPLIT): the launch there is bound by one thread's
+// chain of solves, not by the work.
+constexpr long FLUX_SPLIT_CELLS = 1L << 18;
+inline bool flux_split(const bnz_ctx *c, const Shape &sh) {
+  return c->dims >= 2 && !sh.tiled && sh.ty <= 0 && c->ncell <= FLUX_SPLIT_CELLS;
+}
+
 inline Box active_box(const bnz_ctx *c) {
   Box b;
   b.lo[0] = c->i1; b.n[0] = c->i2 - c->i1 + 1;

@@ -27,7 +27,9 @@ namespace {
 // All face directions of a stage in one launch, one thread per cell of the
 // union of the face ranges (as the FP64 k_flux3); the step's first one also
 // reduces its dt in workgroup 0 (DtFold).
-template <int ORDER, int RS, typename BF>
+// SPLIT > 0 (small grids, flat order): workgroup b computes the faces of
+// direction b % SPLIT only (as the FP64 k_flux3).
+template <int ORDER, int RS, typename BF, int SPLIT = 0>
 __global__ __launch_bounds__(BLK) void k32_flux3(Geo g, const float *__restrict__ w,
                                                  const BF *__restrict__ bf,
                                                  float *__restrict__ fx, float *__restrict__ fy,
@@ -36,8 +38,25 @@ __global__ __launch_bounds__(BLK) void k32_flux3(Geo g, const float *__restrict_
   if (dtf.npart > 0 && blockIdx.x == 0) dt_reduce_block(dtf, dev);
   if (step_past_tmax(dtf, dev)) return;  // a step past t_max (see k_update)
   int i, j, k;
-  if (!cell_index<false, 1, 2>(box, ty, i, j, k)) return;
   int ng = 0;
+  if (SPLIT > 0) {
+    const long lb = xcd_block();
+    const int dir = (int)(lb % SPLIT);
+    const long t = (lb / SPLIT) * BLK + threadIdx.x;
+    if (t >= box.total()) return;
+    box_idx(box, t, i, j, k);
+    // the faces flux32_cell computes for this cell, one direction
+    if (dir == 0) {
+      if (i >= L.lo[0]) flux32_face<0, ORDER, RS>(g, w, bf, fx, i, j, k, ng);
+    } else if (dir == 1) {
+      if (j >= L.lo[1]) flux32_face<1, ORDER, RS>(g, w, bf, fy, i, j, k, ng);
+    } else {
+      if (k >= L.lo[2]) flux32_face<2, ORDER, RS>(g, w, bf, fz, i, j, k, ng);
+    }
+    if (ng) atomicAdd(nguard, ng);
+    return;
+  }
+  if (!cell_index<false, 1, 2>(box, ty, i, j, k)) return;
   flux32_cell<ORDER, RS>(g, w, bf, fx, fy, fz, L, i, j, k, ng);
   if (ng) atomicAdd(nguard, ng);
 }
@@ -183,8 +202,18 @@ void launch_fluxes32(bnz_ctx *c, const float *w, const BF *bf, const Lims &L) {
   const Shape sh = work_shape32(u, c->dims, c->flux_chunk);
   const DtFold dtf = take_dt(c);
   Launch t(c, ORDER ? "flux3_plm" : "flux3_dc");
-  hipLaunchKernelGGL((k32_flux3<ORDER, RS, BF>), dim3(sh.blocks), dim3(BLK), 0, c->stream, c->g,
-                     w, bf, c->fx32, c->fy32, c->fz32, u, L, ng, sh.ty, c->dev, dtf);
+  if (flux_split(c, sh)) {
+    const int nb = blocks_for(u.total());
+    if (c->dims == 3)
+      hipLaunchKernelGGL((k32_flux3<ORDER, RS, BF, 3>), dim3(3 * nb), dim3(BLK), 0, c->stream,
+                         c->g, w, bf, c->fx32, c->fy32, c->fz32, u, L, ng, 0, c->dev, dtf);
+    else
+      hipLaunchKernelGGL((k32_flux3<ORDER, RS, BF, 2>), dim3(2 * nb), dim3(BLK), 0, c->stream,
+                         c->g, w, bf, c->fx32, c->fy32, c->fz32, u, L, ng, 0, c->dev, dtf);
+  } else {
+    hipLaunchKernelGGL((k32_flux3<ORDER, RS, BF>), dim3(sh.blocks), dim3(BLK), 0, c->stream,
+                       c->g, w, bf, c->fx32, c->fy32, c->fz32, u, L, ng, sh.ty, c->dev, dtf);
+  }
 }
 
 template <typename BF>

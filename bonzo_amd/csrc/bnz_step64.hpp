@@ -31,7 +31,10 @@ using namespace bnz;
 // the union of the face ranges computes the x, y and z faces whose lower
 // cell it is, so the prim stencil of a cell is fetched once per stage.
 // A step's first flux launch also reduces its dt in workgroup 0 (DtFold).
-template <int ORDER, int RS, bool TILED = false, bool CP = false>
+// SPLIT > 0 (small grids, flat order): workgroup b computes the faces of
+// direction b % SPLIT only, so a thread solves one face instead of SPLIT and
+// the launch, bound there by one thread's chain of solves, ends sooner.
+template <int ORDER, int RS, bool TILED = false, bool CP = false, int SPLIT = 0>
 __global__ __launch_bounds__(BLK) void k_flux3(Geo g, const double *__restrict__ w,
                                                const double *__restrict__ bf,
                                                double *__restrict__ fx, double *__restrict__ fy,
@@ -40,6 +43,22 @@ __global__ __launch_bounds__(BLK) void k_flux3(Geo g, const double *__restrict__
   if (dtf.npart > 0 && blockIdx.x == 0) dt_reduce_block(dtf, dev);
   if (step_past_tmax(dtf, dev)) return;  // a step past t_max (see k_update)
   int i, j, k;
+  if (SPLIT > 0) {
+    const long lb = xcd_block();
+    const int dir = (int)(lb % SPLIT);
+    const long t = (lb / SPLIT) * BLK + threadIdx.x;
+    if (t >= box.total()) return;
+    box_idx(box, t, i, j, k);
+    // the faces flux_cell computes for this cell, one direction
+    if (dir == 0) {
+      if (i >= L.lo[0]) flux_face<0, ORDER, RS, CP>(g, w, bf, fx, i, j, k);
+    } else if (dir == 1) {
+      if (j >= L.lo[1]) flux_face<1, ORDER, RS, CP>(g, w, bf, fy, i, j, k);
+    } else {
+      if (k >= L.lo[2]) flux_face<2, ORDER, RS, CP>(g, w, bf, fz, i, j, k);
+    }
+    return;
+  }
   if (!cell_index<TILED, 1, 2>(box, ty, i, j, k)) return;
   flux_cell<ORDER, RS, 0, CP>(g, w, bf, fx, fy, fz, L, i, j, k);
 }
@@ -174,7 +193,15 @@ void launch_fluxes_cp(bnz_ctx *c, const double *w, const double *bf, const Lims 
   const Shape sh = work_shape(u, c->dims, c->flux_chunk);
   const DtFold dtf = take_dt(c);
   Launch t(c, ORDER ? (CP ? "flux3_plm_char" : "flux3_plm") : "flux3_dc");
-  if (sh.tiled)
+  if (flux_split(c, sh)) {
+    const int nb = blocks_for(u.total());
+    if (c->dims == 3)
+      hipLaunchKernelGGL((k_flux3<ORDER, RS, false, CP, 3>), dim3(3 * nb), dim3(BLK), 0,
+                         c->stream, c->g, w, bf, c->fx, c->fy, c->fz, u, L, 0, c->dev, dtf);
+    else
+      hipLaunchKernelGGL((k_flux3<ORDER, RS, false, CP, 2>), dim3(2 * nb), dim3(BLK), 0,
+                         c->stream, c->g, w, bf, c->fx, c->fy, c->fz, u, L, 0, c->dev, dtf);
+  } else if (sh.tiled)
     hipLaunchKernelGGL((k_flux3<ORDER, RS, true, CP>), dim3(sh.blocks), dim3(BLK), 0, c->stream,
                        c->g, w, bf, c->fx, c->fy, c->fz, u, L, 0, c->dev, dtf);
   else
