@@ -27,6 +27,12 @@
  *       out-of-bounds write; only k=0 is updated here.
  *   D15 reflect boundaries follow the evident intent of grid_bc_funcs.pyx
  *       (see bc_axis).
+ *
+ * Parity unpinned: the energy flux of or_hllt_flux in the Alfven-fan regions
+ * 2-4.  hllt_flux never sets the star-state energy (and normal-field) slots
+ * (fluxes_c.c:504-573), so no closed form, invariant or golden vector fixes
+ * what they should be; this restatement uses 0 there, as the device does.
+ * Every other function is pinned by tests/test_oracle_pins.py.
  */
 #include "bnz_oracle.h"
 
