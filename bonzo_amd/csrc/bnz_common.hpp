@@ -656,7 +656,7 @@ int f32_fixed_rows(bnz_ctx *c, int rows);  // FP32C state of the user fixed valu
 void f32_dt(bnz_ctx *c, int use_fixed, double fixed_dt);
 void f32_step_body(bnz_ctx *c);
 // shared launches (bnz_gpu.hip)
-void launch_dt_final(bnz_ctx *c, int npart, int use_fixed, double fixed_dt);
+void defer_dt_final(bnz_ctx *c, int npart, int use_fixed, double fixed_dt);
 void launch_time_advance(bnz_ctx *c);
 // The pending dt reduction for a step's first flux launch (npart 0 if none),
 // or launched on its own (flush_dt) where dt is needed first.

@@ -416,7 +416,7 @@ void launch_dt(bnz_ctx *c, int use_fixed, double fixed_dt) {
     hipLaunchKernelGGL(k_dt_partial, dim3(nb), dim3(BLK), 0, c->stream, c->g, c->w, a,
                        cfl_mode(c), c->part);
   }
-  launch_dt_final(c, nb, use_fixed, fixed_dt);
+  defer_dt_final(c, nb, use_fixed, fixed_dt);
 }
 
 }  // namespace
@@ -425,7 +425,7 @@ namespace bnz {
 
 // Leaves the reduction of npart partials pending: the step's first flux
 // launch runs it (take_dt), or flush_dt as a kernel of its own.
-void launch_dt_final(bnz_ctx *c, int npart, int use_fixed, double fixed_dt) {
+void defer_dt_final(bnz_ctx *c, int npart, int use_fixed, double fixed_dt) {
   c->pdt.part = c->part;
   c->pdt.npart = npart;
   c->pdt.use_fixed = use_fixed;

@@ -356,7 +356,7 @@ void f32_dt(bnz_ctx *c, int use_fixed, double fixed_dt) {
     hipLaunchKernelGGL(k32_dt_partial, dim3(nb), dim3(BLK), 0, c->stream, c->g, c->w32, a,
                        c->p.dt_fms_ref, c->part);
   }
-  launch_dt_final(c, nb, use_fixed, fixed_dt);
+  defer_dt_final(c, nb, use_fixed, fixed_dt);
 }
 
 template <bool PRED, int ORDER, int RS>
