@@ -405,6 +405,73 @@ def test_work_shapes_do_not_change_results(chunks):
     assert np.array_equal(gb[:, sk, sj, si], o.bfld[:, sk, sj, si])
 
 
+@pytest.mark.parametrize("rs,order,cp", [("hlld", 1, False), ("hllt", 1, False),
+                                         ("hll", 1, False), ("hlld", 0, False),
+                                         ("hlld", 1, True)])
+def test_whole_tiles_match_the_oracle(rs, order, cp):
+    """Work shape 1 (64 x 4 tiles along z) on a box of whole and partial
+    tiles (a turbulent state cut to 70 x 12 x 8, periodic): the IEEE build
+    stays the oracle bit for bit for every solver, both orders, and with
+    characteristic limiting."""
+    from bonzo_amd._lib import Context
+    from oracle.oracle import OracleSim
+    nx, ny, nz = 70, 12, 8
+    ng = 3 if order else 2
+    say(f"case: turbulence cut to {nx}x{ny}x{nz}, shapes (1, 1, 1), {rs} order {order} "
+        f"char_proj {cp}, 3 steps")
+    s, prim, bfld = P.turbulence(nx, ng=ng, seed_b=63, seed_v=64)
+    sl = (slice(None), slice(0, nz + 2 * ng), slice(0, ny + 2 * ng), slice(0, nx + 2 * ng))
+    prim, bfld = np.ascontiguousarray(prim[sl]), np.ascontiguousarray(bfld[sl])
+    o = OracleSim(nx, ny, nz, reconstr_order=order, rsolver=rs, char_proj=cp)
+    o.prim[:] = prim
+    o.bfld[:] = bfld
+    o.init()
+    o.advance_steps(3)
+    g = Context(nx, ny, nz, rsolver=rs, reconstr="linear" if order else "const", char_proj=cp)
+    g.set_tuning(1, 1, 1)
+    g.upload(prim, bfld)
+    g.init_state()
+    g.run_steps(3)
+    gp, gb = g.download()
+    g.close()
+    act = (slice(None), slice(ng, -ng), slice(ng, -ng), slice(ng, -ng))
+    assert np.array_equal(gp[act], o.prim[act])
+    assert np.array_equal(gb[act], o.bfld[act])
+
+
+@pytest.mark.parametrize("dims", [(70, 20, 6), (33, 9, 4)])
+def test_tiles_fma(dims):
+    """Work shape 1 on boxes of several tile columns and rows with partial
+    ones: the IEEE build's tiled run is its flat run bit for bit, and the FMA
+    build stays within its bar (1e-12 relative L1 per variable) of the IEEE
+    run in both shapes (the compiler fuses each kernel's products on its own,
+    so the FMA shapes may differ in the last bits)."""
+    from bonzo_amd._lib import Context
+    nx, ny, nz = dims
+    say(f"case: turbulence cut to {nx}x{ny}x{nz}, fp64/fp64fma, shapes (0,0,0) vs (1,0,0), 6 steps")
+    s, prim, bfld = P.turbulence(max(dims), seed_b=65, seed_v=66)
+    sl = (slice(None), slice(0, nz + 6), slice(0, ny + 6), slice(0, nx + 6))
+    prim, bfld = np.ascontiguousarray(prim[sl]), np.ascontiguousarray(bfld[sl])
+    res = {}
+    for prec in ("fp64", "fp64fma"):
+        for shapes in ((0, 0, 0), (1, 0, 0)):
+            g = Context(nx, ny, nz, precision=prec)
+            g.set_tuning(*shapes)
+            g.upload(prim, bfld)
+            g.init_state()
+            g.run_steps(6)
+            res[prec, shapes] = g.download()[0][:, 3:-3, 3:-3, 3:-3]
+            g.close()
+    ref = res["fp64", (0, 0, 0)]
+    assert np.array_equal(res["fp64", (1, 0, 0)], ref)
+    for shapes in ((0, 0, 0), (1, 0, 0)):
+        got = res["fp64fma", shapes]
+        for v in range(9):
+            den = max(np.abs(ref[v]).sum(), 1e-300)
+            l1 = np.abs(got[v] - ref[v]).sum() / den
+            assert l1 <= 1e-12, (shapes, v, l1)
+
+
 def test_work_shapes_non_cubic():
     """Slab columns on a non-cubic grid with uneven column counts."""
     from bonzo_amd._lib import Context
