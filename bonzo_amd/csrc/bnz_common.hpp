@@ -652,7 +652,8 @@ void f32_free(bnz_ctx *c);
 int f32_from64(bnz_ctx *c);   // state from the FP64 prim/face arrays (all cells)
 int f32_to64(bnz_ctx *c);     // FP64 prim/face arrays from the state
 void f32_ghost(bnz_ctx *c);
-int f32_fixed_rows(bnz_ctx *c, int rows);  // FP32C state of the user fixed values
+// FP32C state of rows user fixed values (ufix) into a new buffer *out
+int f32_fixed_rows(bnz_ctx *c, const double *ufix, int rows, float **out);
 void f32_dt(bnz_ctx *c, int use_fixed, double fixed_dt);
 void f32_step_body(bnz_ctx *c);
 // shared launches (bnz_gpu.hip)

@@ -685,6 +685,16 @@ int check_clean(bnz_ctx *c, const std::vector<GhostOp> &all, const int off[7]) {
   return BNZ_OK;
 }
 
+// A new device buffer *dst holding v (at least one element's room); on a
+// failure *dst may hold the allocation, which the caller frees.
+template <typename T>
+int dev_upload(bnz_ctx *c, T **dst, const std::vector<T> &v) {
+  HIPCHK(c, hipMalloc(dst, std::max<size_t>(1, v.size()) * sizeof(T)));
+  if (!v.empty())
+    HIPCHK(c, hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return BNZ_OK;
+}
+
 int build_passes(bnz_ctx *c) {
   static const int order[6] = {4, 5, 2, 3, 0, 1};
   std::vector<GhostOp> all;
@@ -735,24 +745,31 @@ int build_passes(bnz_ctx *c) {
   }
   off[6] = (int)all.size();
   if (const int rc = check_clean(c, all, off)) return rc;
-  // the lists may be in use by queued fills
-  HIPCHK(c, hipStreamSynchronize(c->stream));
+  // the new device tables are complete before the old ones go, so that a
+  // failed allocation, copy or launch leaves the boundaries as they were
+  GhostOp *gops = nullptr;
+  double *ufix = nullptr;
+  float *ufix32 = nullptr;
+  int rc = dev_upload(c, &gops, all);
+  if (!rc) rc = dev_upload(c, &ufix, fix);
+  if (!rc && c->f32) rc = f32_fixed_rows(c, ufix, (int)(fix.size() / (NMODE + 3)), &ufix32);
+  // the old tables may be in use by queued fills
+  if (!rc && hipStreamSynchronize(c->stream) != hipSuccess)
+    rc = fail(c, BNZ_EHIP, "user boundary: hipStreamSynchronize failed");
+  if (rc) {
+    if (gops) (void)hipFree(gops);
+    if (ufix) (void)hipFree(ufix);
+    if (ufix32) (void)hipFree(ufix32);
+    return rc;
+  }
   drop_chunk_graph(c);
   c->has_steady = false;
   if (c->gops) (void)hipFree(c->gops);
   if (c->ufix) (void)hipFree(c->ufix);
-  c->gops = nullptr;
-  c->ufix = nullptr;
-  HIPCHK(c, hipMalloc(&c->gops, std::max<size_t>(1, all.size()) * sizeof(GhostOp)));
-  if (!all.empty())
-    HIPCHK(c, hipMemcpy(c->gops, all.data(), all.size() * sizeof(GhostOp), hipMemcpyHostToDevice));
-  HIPCHK(c, hipMalloc(&c->ufix, std::max<size_t>(1, fix.size()) * sizeof(double)));
-  if (!fix.empty())
-    HIPCHK(c, hipMemcpy(c->ufix, fix.data(), fix.size() * sizeof(double), hipMemcpyHostToDevice));
-  if (c->f32) {
-    const int rc = f32_fixed_rows(c, (int)(fix.size() / (NMODE + 3)));
-    if (rc) return rc;
-  }
+  if (c->ufix32) (void)hipFree(c->ufix32);
+  c->gops = gops;
+  c->ufix = ufix;
+  c->ufix32 = ufix32;
   c->ngops = (int)all.size();
   c->nghost = 0;
   for (int t = 0; t < 7; ++t) c->pass_off[t] = off[t];

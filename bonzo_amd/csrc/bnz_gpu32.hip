@@ -336,13 +336,11 @@ int f32_to64(bnz_ctx *c) {
 
 void f32_ghost(bnz_ctx *c) { ghost32(c); }
 
-int f32_fixed_rows(bnz_ctx *c, int rows) {
-  if (c->ufix32) (void)hipFree(c->ufix32);
-  c->ufix32 = nullptr;
-  HIPCHK(c, hipMalloc(&c->ufix32, (size_t)std::max(1, rows) * NGHOST32 * sizeof(float)));
+int f32_fixed_rows(bnz_ctx *c, const double *ufix, int rows, float **out) {
+  HIPCHK(c, hipMalloc(out, (size_t)std::max(1, rows) * NGHOST32 * sizeof(float)));
   if (rows > 0)
-    hipLaunchKernelGGL(k32_fixed_rows, dim3(blocks_for(rows)), dim3(BLK), 0, c->stream, c->ufix,
-                       c->ufix32, rows, c->g.gamm1i);
+    hipLaunchKernelGGL(k32_fixed_rows, dim3(blocks_for(rows)), dim3(BLK), 0, c->stream, ufix,
+                       *out, rows, c->g.gamm1i);
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return check_launch(c);
 }
