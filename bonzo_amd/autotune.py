@@ -31,25 +31,33 @@ def _class_times(kt):
     return out
 
 
-def autotune(ctx, steps=2, candidates=CANDIDATES):
+def autotune(ctx, steps=2, candidates=CANDIDATES, passes=2):
     """Pick (flux_chunk, emf_chunk, update_chunk) for ``ctx`` and apply it.
 
     Returns (best, table) where table[c] holds the per-class milliseconds of
-    candidate c.  The context must hold an initialised state.  Work shapes
-    exist on 3D grids only; other grids keep the defaults."""
+    candidate c, the smallest over ``passes`` measurements.  The candidates
+    are measured in order, then in reverse order, and so on, after a
+    warm-up of the device: a candidate measured only first, while the clocks
+    and caches settle, would read several per cent slow.  The context must
+    hold an initialised state.  Work shapes exist on 3D grids only; other
+    grids keep the defaults."""
     table = {}
     if ctx.ntot[2] == 1:
         ctx.set_tuning(0, 0, 0)
         return (0, 0, 0), table
-    for c in candidates:
-        ctx.set_tuning(c, c, c)
-        ctx.run_steps(1)                      # first launch of a shape pays setup
-        ctx.synchronize()
-        ctx.set_kernel_timing(True)
-        ctx.run_steps(steps)
-        ctx.synchronize()
-        ctx.set_kernel_timing(False)
-        table[c] = _class_times(ctx.kernel_timing())
+    ctx.set_tuning(candidates[0], candidates[0], candidates[0])
+    ctx.run_steps(steps)
+    for p in range(passes):
+        for c in (candidates if p % 2 == 0 else candidates[::-1]):
+            ctx.set_tuning(c, c, c)
+            ctx.run_steps(1)                      # first launch of a shape pays setup
+            ctx.synchronize()
+            ctx.set_kernel_timing(True)
+            ctx.run_steps(steps)
+            ctx.synchronize()
+            ctx.set_kernel_timing(False)
+            t = _class_times(ctx.kernel_timing())
+            table[c] = t if c not in table else {k: min(v, table[c][k]) for k, v in t.items()}
     best = tuple(min(candidates, key=lambda c: table[c][k]) for k in ("flux", "emf", "update"))
     ctx.set_tuning(*best)
     return best, table
