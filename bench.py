@@ -43,19 +43,35 @@ KERNEL_BYTES = {
               "update_corr": 54 * 4 + 6 * 8},
 }
 # HBM bytes per launch of the dominant kernels at 256^3 (FETCH_SIZE x 2.0 +
-# WRITE_SIZE x 1.0, scaled by a plain device copy), from the rocprofv3 --pmc
-# runs summarised in the named profile; the bench itself runs no counters.
-PROFILED_TRAFFIC = {
-    ("fp64fma", "flux3_plm"): (5.995e9, "profiles/vl_step_kernels_turb256.md"),
-    ("fp64", "flux3_plm"): (5.992e9, "profiles/vl_step_kernels_turb256.md"),
-    ("fp32c", "update_corr"): (4.774e9, "profiles/vl_step_kernels_turb256.md"),
-    ("fp32c", "flux3_plm"): (2.944e9, "profiles/vl_step_kernels_turb256.md"),
+# WRITE_SIZE x 1.0, scaled by a plain device copy) and the fraction of the
+# sampled SIMDs' cycles spent issuing VALU work (SQ_ACTIVE_INST_VALU x 4 /
+# (GRBM_GUI_ACTIVE x SIMDs); near 1 a launch is bound by arithmetic issue),
+# from the rocprofv3 --pmc runs summarised in the named profile, per work
+# shape of the kernel's class (0 flat, n > 1 slab columns of n rows; the
+# FP32C tile shape runs flat); the bench itself runs no counters.
+PROFILE = "profiles/vl_step_kernels_turb256.md"
+PROFILED = {
+    ("fp64fma", "flux3_plm", 0): (9.436e9, 0.59),
+    ("fp64fma", "flux3_plm", 8): (5.995e9, 0.58),
+    ("fp64", "flux3_plm", 0): (9.435e9, 0.72),
+    ("fp64", "flux3_plm", 8): (5.992e9, 0.71),
+    ("fp32c", "update_corr", 0): (5.402e9, 0.34),
+    ("fp32c", "update_corr", 16): (4.774e9, 0.32),
+    ("fp32c", "flux3_plm", 0): (4.652e9, 0.86),
+    ("fp32c", "flux3_plm", 8): (2.944e9, 0.85),
 }
-# Fraction of the sampled SIMDs' cycles spent issuing VALU work in the same
-# kernels (SQ_ACTIVE_INST_VALU x 4 / (GRBM_GUI_ACTIVE x SIMDs), same profile):
-# near 1 the launch is bound by arithmetic issue rather than by HBM.
-PROFILED_VALU_BUSY = {("fp64fma", "flux3_plm"): 0.58, ("fp64", "flux3_plm"): 0.71,
-                      ("fp32c", "update_corr"): 0.32, ("fp32c", "flux3_plm"): 0.85}
+
+
+def profiled(prec, kernel, tuned):
+    """(traffic, valu_busy) profiled for kernel at the work shape its class
+    ran with, or (None, None)."""
+    cls = 0 if kernel.startswith("flux") else 1 if kernel.startswith("emf") else 2
+    shape = tuned[cls]
+    if prec == "fp32c" and shape == 1:
+        shape = 0
+    return PROFILED.get((prec, kernel, shape), (None, None))
+
+
 # Algorithmic bytes per cell-update of a whole VL step: 5 state volumes of 9
 # words (FP64 360 B; FP32C 252 B with hi + lo on the stage-2 read of S_n and
 # the write of S_{n+1}).
@@ -195,14 +211,14 @@ def run_path(prec, n, prim, bfld, args):
     davg = dtot / dl
     b = kernel_bytes(prec, dname, n)
     ach = b / (davg * 1e-3) / 1e9
-    prof = PROFILED_TRAFFIC.get((prec, dname), (None, None)) if n == 256 else (None, None)
-    busy = PROFILED_VALU_BUSY.get((prec, dname)) if n == 256 else None
+    traffic, busy = profiled(prec, dname, tuned) if n == 256 else (None, None)
     return {
         "value": value, "ms_per_step": 1e3 * el / args.steps, "tuned": tuned, "table": table,
         "kernels": {k: v[0] / v[1] for k, v in sorted(kt.items())},
         "roofline": {"bound": "hbm", "kernel": dname, "achieved": ach, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                     "traffic": prof[0], "traffic_source": prof[1], "valu_busy": busy,
+                     "traffic": traffic, "traffic_source": PROFILE if traffic else None,
+                     "valu_busy": busy,
                      "avg_launch_ms": davg, "algorithmic_bytes_per_launch": b},
         "step_GBs": STEP_BYTES[prec] * value / 1e9,
         "divb_max": d["divb_max"], "hlld_guards": d["hlld_guards"],
