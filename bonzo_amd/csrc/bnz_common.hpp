@@ -103,7 +103,12 @@ __device__ __forceinline__ void box_idx(const Box &b, long t, int &i, int &j, in
 }
 
 // Thread -> cell of box b.
-//   flat (ty <= 0): x fastest over the whole box;
+//   flat (ty <= 0): x fastest over the whole box, in xcd_block order (XO)
+//     or in launch order, which the hardware deals round-robin over the
+//     XCDs, so that all of them work on the same few planes at once and
+//     share what those planes bring into the MALL (the flux and update
+//     launches run faster so: PLM flux -4 %, FP32C corrector -7 %; the EMF
+//     launches, which re-read the previous plane, slower);
 //   TILED (TA, MA): a workgroup is 64 x-lanes by 4 rows of axis TA at one
 //     index of axis MA, and consecutive workgroups step along MA, so stencil
 //     neighbours along MA are read by workgroups that run back to back on
@@ -112,7 +117,7 @@ __device__ __forceinline__ void box_idx(const Box &b, long t, int &i, int &j, in
 //     walked plane by plane (z), x fastest within a plane, so the rows of
 //     plane k+1 that plane k reads as neighbours are still in the XCD's L2
 //     when plane k+1 is processed.
-template <bool TILED, int TA, int MA>
+template <bool TILED, int TA, int MA, bool XO = true>
 __device__ __forceinline__ bool cell_index(const Box &b, int ty, int &i, int &j, int &k) {
   if (TILED) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -130,7 +135,7 @@ __device__ __forceinline__ bool cell_index(const Box &b, int ty, int &i, int &j,
     return true;
   }
   if (ty <= 0) {
-    const long t = xcd_block() * BLK + threadIdx.x;
+    const long t = (XO ? xcd_block() : (long)blockIdx.x) * BLK + threadIdx.x;
     if (t >= b.total()) return false;
     box_idx(b, t, i, j, k);
     return true;

@@ -59,7 +59,7 @@ __global__ __launch_bounds__(BLK) void k_flux3(Geo g, const double *__restrict__
     }
     return;
   }
-  if (!cell_index<TILED, 1, 2>(box, ty, i, j, k)) return;
+  if (!cell_index<TILED, 1, 2, false>(box, ty, i, j, k)) return;
   flux_cell<ORDER, RS, 0, CP>(g, w, bf, fx, fy, fz, L, i, j, k);
 }
 
@@ -168,7 +168,7 @@ __global__ __launch_bounds__(BLK) void k_update(Geo g, const double *w0, const d
                                                 int ty, const double *we = nullptr) {
   int i, j, k;
   double cdsmax = 0.;
-  const bool have = cell_index<TILED, 1, 2>(grid, ty, i, j, k);
+  const bool have = cell_index<TILED, 1, 2, false>(grid, ty, i, j, k);
   if (dts[0] == 0.) {  // a step past t_max: state unchanged, faces carried over
     if (!PRED && have) {
       const long c = ((long)k * g.nt[1] + j) * g.nt[0] + i;
