@@ -56,7 +56,7 @@ __global__ __launch_bounds__(BLK) void k32_flux3(Geo g, const float *__restrict_
     if (ng) atomicAdd(nguard, ng);
     return;
   }
-  if (!cell_index<false, 1, 2, false>(box, ty, i, j, k)) return;
+  if (!cell_index<false, 1, 2>(box, ty, i, j, k)) return;
   flux32_cell<ORDER, RS>(g, w, bf, fx, fy, fz, L, i, j, k, ng);
   if (ng) atomicAdd(nguard, ng);
 }
@@ -83,7 +83,7 @@ __global__ __launch_bounds__(BLK) void k32_update_pred(Geo g, Pred32 a, Lims L,
                                                        Walls wl = Walls{}) {
   int i, j, k;
   if (dts[0] == 0.) return;  // a step past t_max (see k_update)
-  if (!cell_index<false, 1, 2, false>(grid, ty, i, j, k)) return;
+  if (!cell_index<false, 1, 2>(grid, ty, i, j, k)) return;
   update32_pred<DIMS, typename std::conditional<UG, GeoUni, GeoArr>::type, FLY>(g, a, L, dts, i,
                                                                                j, k, we, wl);
 }
@@ -97,7 +97,7 @@ __global__ __launch_bounds__(BLK) void k32_update_corr(Geo g, Corr32 a, Lims L,
                                                        Walls wl = Walls{}) {
   int i, j, k;
   double cdsmax = 0.;
-  const bool have = cell_index<false, 1, 2, false>(grid, ty, i, j, k);
+  const bool have = cell_index<false, 1, 2>(grid, ty, i, j, k);
   if (dts[0] == 0.) {  // a step past t_max: state unchanged, faces carried over
     if (have) {
       const long c = ((long)k * g.nt[1] + j) * g.nt[0] + i;
@@ -199,7 +199,7 @@ template <int ORDER, int RS, typename BF>
 void launch_fluxes32(bnz_ctx *c, const float *w, const BF *bf, const Lims &L) {
   int *ng = (int *)(c->dev + 4);
   const Box u = flux3_box(c->dims, L);
-  const Shape sh = work_shape32(u, c->dims, c->flux_chunk);
+  const Shape sh = flat_order(c, work_shape32(u, c->dims, c->flux_chunk));
   const DtFold dtf = take_dt(c);
   Launch t(c, ORDER ? "flux3_plm" : "flux3_dc");
   if (flux_split(c, sh)) {
@@ -394,7 +394,7 @@ void f32_step_body(bnz_ctx *c) {
     grid.lo[q] = 0;
     grid.n[q] = c->nt[q];
   }
-  const Shape us = work_shape32(grid, c->dims, c->update_chunk);
+  const Shape us = flat_order(c, work_shape32(grid, c->dims, c->update_chunk));
   const int nb = us.blocks;
   Lims L0 = integr_lims(c, 0);
   const bool fly = emf_fly(c);

@@ -59,7 +59,7 @@ __global__ __launch_bounds__(BLK) void k_flux3(Geo g, const double *__restrict__
     }
     return;
   }
-  if (!cell_index<TILED, 1, 2, false>(box, ty, i, j, k)) return;
+  if (!cell_index<TILED, 1, 2>(box, ty, i, j, k)) return;
   flux_cell<ORDER, RS, 0, CP>(g, w, bf, fx, fy, fz, L, i, j, k);
 }
 
@@ -168,7 +168,7 @@ __global__ __launch_bounds__(BLK) void k_update(Geo g, const double *w0, const d
                                                 int ty, const double *we = nullptr) {
   int i, j, k;
   double cdsmax = 0.;
-  const bool have = cell_index<TILED, 1, 2, false>(grid, ty, i, j, k);
+  const bool have = cell_index<TILED, 1, 2>(grid, ty, i, j, k);
   if (dts[0] == 0.) {  // a step past t_max: state unchanged, faces carried over
     if (!PRED && have) {
       const long c = ((long)k * g.nt[1] + j) * g.nt[0] + i;
@@ -190,7 +190,7 @@ template <int ORDER, int RS, bool CP>
 void launch_fluxes_cp(bnz_ctx *c, const double *w, const double *bf, const Lims &L) {
   // all face directions in one launch over the union of the face ranges
   const Box u = flux3_box(c->dims, L);
-  const Shape sh = work_shape(u, c->dims, c->flux_chunk);
+  const Shape sh = flat_order(c, work_shape(u, c->dims, c->flux_chunk));
   const DtFold dtf = take_dt(c);
   Launch t(c, ORDER ? (CP ? "flux3_plm_char" : "flux3_plm") : "flux3_dc");
   if (flux_split(c, sh)) {
@@ -272,7 +272,7 @@ void launch_update(bnz_ctx *c, bool pred, const Lims &L, const double *we = null
   }
   const Box act = active_box(c);
   const int fr = cfl_mode(c);
-  const Shape sh = work_shape(grid, c->dims, c->update_chunk);
+  const Shape sh = flat_order(c, work_shape(grid, c->dims, c->update_chunk));
   Launch t(c, pred ? "update_pred" : "update_corr");
 #define UPD_ARGS                                                                              \
   dim3(sh.blocks), dim3(BLK), 0, c->stream, c->g, c->w, c->fx, c->fy, c->fz, c->ee, c->b, c->bs, \
