@@ -435,7 +435,9 @@ inline int slab_blocks(const Box &b, int ty) {
 }
 
 // Work shape of a kernel class from its tuning value (3D only): 0 flat,
-// 1 tiles of 64 x 4 along z, > 1 slab columns of that many rows.
+// 1 tiles of 64 x 4 along z, > 1 slab columns of that many rows.  ty is
+// what cell_index takes: 0 flat, -1 flat in launch order (flat_order),
+// > 1 the rows of a slab column.
 struct Shape {
   bool tiled;
   int ty;
