@@ -94,6 +94,21 @@ __device__ __forceinline__ long xcd_block() {
   return x * q + (x < r ? x : r) + idx;
 }
 
+// Workgroups in runs of XCD_RUN consecutive logical blocks, the runs dealt
+// round-robin over the XCDs (logical run r on XCD r % NXCD): all XCDs work
+// on the same few planes at once and share what those planes bring into the
+// MALL, while each XCD still walks whole rows.  Blocks past the last whole
+// round keep their launch order.  Runs of 4 to 32 measured alike; 1 (launch
+// order) and 64 or more were slower.
+constexpr long XCD_RUN = 16;
+__device__ __forceinline__ long run_block() {
+  const long b = blockIdx.x, nb = gridDim.x;
+  const long nfull = nb / (NXCD * XCD_RUN) * (NXCD * XCD_RUN);
+  if (b >= nfull) return b;
+  const long x = b % NXCD, idx = b / NXCD;
+  return ((idx / XCD_RUN) * NXCD + x) * XCD_RUN + idx % XCD_RUN;
+}
+
 __device__ __forceinline__ void box_idx(const Box &b, long t, int &i, int &j, int &k) {
   int nx = b.n[0], ny = b.n[1];
   i = b.lo[0] + (int)(t % nx);
@@ -104,9 +119,7 @@ __device__ __forceinline__ void box_idx(const Box &b, long t, int &i, int &j, in
 
 // Thread -> cell of box b.
 //   flat (ty <= 0): x fastest over the whole box; ty == 0 in xcd_block
-//     order, ty < 0 in launch order, which the hardware deals round-robin
-//     over the XCDs, so that all of them work on the same few planes at once
-//     and share what those planes bring into the MALL (see flat_order);
+//     order, ty < 0 in run_block order (see flat_order);
 //   TILED (TA, MA): a workgroup is 64 x-lanes by 4 rows of axis TA at one
 //     index of axis MA, and consecutive workgroups step along MA, so stencil
 //     neighbours along MA are read by workgroups that run back to back on
@@ -133,7 +146,7 @@ __device__ __forceinline__ bool cell_index(const Box &b, int ty, int &i, int &j,
     return true;
   }
   if (ty <= 0) {
-    const long t = (ty < 0 ? (long)blockIdx.x : xcd_block()) * BLK + threadIdx.x;
+    const long t = (ty < 0 ? run_block() : xcd_block()) * BLK + threadIdx.x;
     if (t >= b.total()) return false;
     box_idx(b, t, i, j, k);
     return true;
@@ -436,7 +449,7 @@ inline int slab_blocks(const Box &b, int ty) {
 
 // Work shape of a kernel class from its tuning value (3D only): 0 flat,
 // 1 tiles of 64 x 4 along z, > 1 slab columns of that many rows.  ty is
-// what cell_index takes: 0 flat, -1 flat in launch order (flat_order),
+// what cell_index takes: 0 flat, -1 flat in run_block order (flat_order),
 // > 1 the rows of a slab column.
 struct Shape {
   bool tiled;
@@ -455,12 +468,13 @@ inline Shape work_shape32(const Box &b, int dims, int chunk) {
 }
 
 // The flat shape of the flux and update launches of large 3D grids takes its
-// workgroups in launch order (ty -1, see cell_index): there all XCDs then
-// share the planes the MALL holds, and the launches run faster (256^3: PLM
-// flux -4 %, corrector update -7 %, step 7.42 -> 7.18 ms; 128^3 -4 %),
-// while 2D and small 3D grids run slower so (OT 128^2 +6 %, turbulence 32^3
-// +5 %).  EMF launches, which re-read the previous plane from their XCD's
-// L2, keep the XCD-contiguous order everywhere.
+// workgroups in runs dealt round-robin over the XCDs (ty -1, run_block):
+// there all XCDs then share the planes the MALL holds, and the launches run
+// faster than with one contiguous run per XCD (256^3: step 7.42 -> 7.05 ms;
+// 128^3 -4 % with single-block runs), while 2D and small 3D grids run slower
+// so (single-block runs: OT 128^2 +6 %, turbulence 32^3 +5 %).  EMF
+// launches, which re-read the previous plane from their XCD's L2, keep the
+// XCD-contiguous order everywhere.
 constexpr long LAUNCH_ORDER_CELLS = 1L << 21;
 inline Shape flat_order(const bnz_ctx *c, Shape sh) {
   if (c->dims == 3 && !sh.tiled && sh.ty == 0 && c->ncell >= LAUNCH_ORDER_CELLS) sh.ty = -1;
