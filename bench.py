@@ -50,16 +50,17 @@ KERNEL_BYTES = {
 # shape of the kernel's class (0 flat, n > 1 slab columns of n rows; the
 # FP32C tile shape runs flat); the bench itself runs no counters.  FETCH_SIZE
 # counts reads that the MALL serves too, which the flat launches (workgroups
-# in launch order) lean on, so for them it overstates the HBM bytes.
+# in runs dealt round-robin over the XCDs) lean on, so for them it bounds the
+# HBM bytes from above.
 PROFILE = "profiles/vl_step_kernels_turb256.md"
 PROFILED = {
-    ("fp64fma", "flux3_plm", 0): (12.158e9, 0.65),
+    ("fp64fma", "flux3_plm", 0): (8.304e9, 0.64),
     ("fp64fma", "flux3_plm", 8): (5.995e9, 0.58),
-    ("fp64", "flux3_plm", 0): (12.127e9, 0.77),
+    ("fp64", "flux3_plm", 0): (8.295e9, 0.75),
     ("fp64", "flux3_plm", 8): (5.992e9, 0.71),
-    ("fp32c", "update_corr", 0): (6.507e9, 0.39),
+    ("fp32c", "update_corr", 0): (5.325e9, 0.38),
     ("fp32c", "update_corr", 16): (4.774e9, 0.32),
-    ("fp32c", "flux3_plm", 0): (5.650e9, 0.95),
+    ("fp32c", "flux3_plm", 0): (3.929e9, 0.93),
     ("fp32c", "flux3_plm", 8): (2.944e9, 0.85),
 }
 
