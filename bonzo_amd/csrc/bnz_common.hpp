@@ -467,14 +467,14 @@ inline Shape work_shape32(const Box &b, int dims, int chunk) {
   return work_shape(b, dims, chunk == 1 ? 0 : chunk);
 }
 
-// The flat shape of the flux and update launches of large 3D grids takes its
-// workgroups in runs dealt round-robin over the XCDs (ty -1, run_block):
-// there all XCDs then share the planes the MALL holds, and the launches run
-// faster than with one contiguous run per XCD (256^3: step 7.42 -> 7.05 ms;
-// 128^3 -4 % with single-block runs), while 2D and small 3D grids run slower
-// so (single-block runs: OT 128^2 +6 %, turbulence 32^3 +5 %).  EMF
-// launches, which re-read the previous plane from their XCD's L2, keep the
-// XCD-contiguous order everywhere.
+// The flat shape of the flux, EMF and update launches of large 3D grids
+// takes its workgroups in runs dealt round-robin over the XCDs (ty -1,
+// run_block): there all XCDs then share the planes the MALL holds, and the
+// launches run faster than with one contiguous run per XCD (256^3: step
+// 7.42 -> 6.97 ms; 128^3 -4 %), while 2D and small 3D grids run slower so
+// (single-block runs: OT 128^2 +6 %, turbulence 32^3 +5 %).  The EMF, which
+// re-reads the previous plane, needs the runs: dealt one block at a time it
+// is 35 % slower than with one run per XCD, in runs of 16 6 % faster.
 constexpr long LAUNCH_ORDER_CELLS = 1L << 21;
 inline Shape flat_order(const bnz_ctx *c, Shape sh) {
   if (c->dims == 3 && !sh.tiled && sh.ty == 0 && c->ncell >= LAUNCH_ORDER_CELLS) sh.ty = -1;

@@ -246,7 +246,7 @@ void emf32(bnz_ctx *c, const float *w, const Lims &L, const char *name) {
   e.lo[2] = (c->dims >= 3) ? L.lo[2] : 0; e.n[2] = (c->dims >= 3) ? L.hi[2] + 1 - L.lo[2] + 1 : 1;
   Launch t(c, name);
   const Walls wl = walls32(c);
-  const Shape sh = work_shape32(e, c->dims, c->emf_chunk);
+  const Shape sh = flat_order(c, work_shape32(e, c->dims, c->emf_chunk));
 #define E32_ARGS \
   dim3(sh.blocks), dim3(BLK), 0, c->stream, c->g, w, c->fx32, c->fy32, c->fz32, c->ee32, e, wl, sh.ty
   if (c->dims == 3) hipLaunchKernelGGL(k32_emf<3>, E32_ARGS);

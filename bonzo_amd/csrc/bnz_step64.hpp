@@ -239,7 +239,7 @@ void launch_emf(bnz_ctx *c, const double *w, const Lims &L, const char *name) {
   e.lo[0] = L.lo[0]; e.n[0] = L.hi[0] + 1 - L.lo[0] + 1;
   e.lo[1] = (c->dims >= 2) ? L.lo[1] : 0; e.n[1] = (c->dims >= 2) ? L.hi[1] + 1 - L.lo[1] + 1 : 1;
   e.lo[2] = (c->dims >= 3) ? L.lo[2] : 0; e.n[2] = (c->dims >= 3) ? L.hi[2] + 1 - L.lo[2] + 1 : 1;
-  const Shape sh = work_shape(e, c->dims, c->emf_chunk);
+  const Shape sh = flat_order(c, work_shape(e, c->dims, c->emf_chunk));
   Launch t(c, name);
 #define EMF_ARGS                                                                              \
   dim3(sh.blocks), dim3(BLK), 0, c->stream, c->g, w, c->fx, c->fy, c->fz, c->ee, e, sh.ty
