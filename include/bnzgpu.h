@@ -171,8 +171,9 @@ int bnz_load_state(bnz_ctx *ctx, const void *host, long bytes);
 
 /* Work shapes of the 3D Godunov-flux, edge-EMF and stage-update kernels:
    0 one thread per face/edge/cell in x-fastest order (the default; the
-   flux and update launches of 3D grids of at least 2^21 cells with ghosts
-   then take their workgroups in runs of 16 dealt round-robin over the XCDs),
+   flux, EMF and update launches of 3D grids of at least 2^21 cells with
+   ghosts then take their workgroups in runs of 16 dealt round-robin over
+   the XCDs),
    1 tiles of 64 x 4 cells ordered along z (FP64; FP32C runs it flat),
    n > 1 slab columns of n rows walked plane by plane.  On 2D FP64 grids emf_chunk
    places the edge EMFs instead: 0 (the default) forms them inside the
