@@ -31,7 +31,7 @@ __global__ __launch_bounds__(BLK) void kb32_flux(const BoxDev *__restrict__ boxe
                                                  float *__restrict__ fx, float *__restrict__ fy,
                                                  float *__restrict__ fz, int *nguard) {
   int ng = 0;
-  for (int wb = wl.lo + blockIdx.x; wb < wl.hi; wb += gridDim.x) {
+  for (int wb = wl.lo + (int)run_block(); wb < wl.hi; wb += gridDim.x) {
     const int b = wl.bmap[wb];
     if (!st.run[b]) continue;
     const BoxDev &d = boxes[b];
@@ -54,7 +54,7 @@ __global__ __launch_bounds__(BLK) void kb32_emf(const BoxDev *__restrict__ boxes
                                                 const float *__restrict__ fy,
                                                 const float *__restrict__ fz,
                                                 float *__restrict__ ee) {
-  for (int wb = wl.lo + blockIdx.x; wb < wl.hi; wb += gridDim.x) {
+  for (int wb = wl.lo + (int)run_block(); wb < wl.hi; wb += gridDim.x) {
     const int b = wl.bmap[wb];
     if (!st.run[b]) continue;
     const BoxDev &d = boxes[b];
@@ -75,7 +75,7 @@ template <bool PRED, int DIMS>
 __global__ __launch_bounds__(BLK) void kb32_update(const BoxDev *__restrict__ boxes, WList wl,
                                                    BoxState st, Pools P, Pools32 Q,
                                                    int fms_ref) {
-  for (int wb = wl.lo + blockIdx.x; wb < wl.hi; wb += gridDim.x) {
+  for (int wb = wl.lo + (int)run_block(); wb < wl.hi; wb += gridDim.x) {
     const int b = wl.bmap[wb];
     if (!st.run[b]) continue;
     const BoxDev &d = boxes[b];
